@@ -1,0 +1,502 @@
+"""ctypes binding of ``libcfdgpu.so`` (include/cfd_gpu.h).
+
+``Solver`` mirrors the reference's ``Solver`` class (include/Solver.hpp): the
+method names follow the reference members they replace, arrays are
+``numpy`` float64 of shape (N, 6) in reference cell order, exactly the memory
+of ``std::vector<State>``.  Every call goes to the HIP library; there is no
+CPU fallback — if the library is missing or no GPU is present the call raises.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libcfdgpu.so")
+
+# enum values (Types.hpp:38-49 and include/cfd_gpu.h)
+LLF, ROE, HLLC, AUSM = 0, 1, 2, 3
+FIRST_ORDER, SECOND_ORDER_CENTRAL, SECOND_ORDER_LIMITED = 0, 1, 2
+BC_BUMP, BC_SUPERSONIC, BC_PLATE, BC_SWBLI = 0, 1, 2, 3
+
+
+class CfdError(RuntimeError):
+    pass
+
+
+class Params(C.Structure):
+    _fields_ = [
+        ("gamma", C.c_double), ("R_gas", C.c_double), ("M_inf", C.c_double),
+        ("P_total_inlet", C.c_double), ("T_total_inlet", C.c_double),
+        ("P_static_exit", C.c_double), ("nu", C.c_double), ("Pr", C.c_double),
+        ("flux", C.c_int), ("spatial", C.c_int), ("bcset", C.c_int), ("viscous", C.c_int),
+        ("nvar", C.c_int), ("turb", C.c_int), ("flux_alt", C.c_int), ("pad0", C.c_int),
+        ("shock_rho", C.c_double), ("shock_u", C.c_double), ("shock_v", C.c_double),
+        ("shock_p", C.c_double), ("reserved", C.c_double * 8),
+    ]
+
+    def copy(self) -> "Params":
+        p = Params()
+        C.memmove(C.byref(p), C.byref(self), C.sizeof(Params))
+        return p
+
+
+class MeshDesc(C.Structure):
+    _fields_ = [
+        ("n_nodes", C.c_int), ("x", C.POINTER(C.c_double)), ("y", C.POINTER(C.c_double)),
+        ("n_cells", C.c_int), ("cell_ptr", C.POINTER(C.c_int)), ("cell_nodes", C.POINTER(C.c_int)),
+        ("n_bedges", C.c_int), ("bedges", C.POINTER(C.c_int)),
+    ]
+
+
+class GmresOpts(C.Structure):
+    _fields_ = [("m", C.c_int), ("restarts", C.c_int), ("tol", C.c_double), ("jv", C.c_int),
+                ("precision", C.c_int), ("precond", C.c_int), ("pad", C.c_int)]
+
+
+class NewtonOpts(C.Structure):
+    _fields_ = [("tolerance", C.c_double), ("rel_drop", C.c_double), ("max_steps", C.c_int),
+                ("log", C.c_int), ("cfl0", C.c_double), ("gmres", GmresOpts)]
+
+
+class NewtonResult(C.Structure):
+    _fields_ = [("steps", C.c_int), ("converged", C.c_int), ("res_vm0", C.c_double),
+                ("res_vm", C.c_double), ("seconds", C.c_double), ("total_gmres_iters", C.c_int),
+                ("pad", C.c_int)]
+
+
+_DP = C.POINTER(C.c_double)
+_IP = C.POINTER(C.c_int)
+_VP = C.c_void_p
+
+# (name, restype, argtypes) of every exported function in include/cfd_gpu.h
+SIGNATURES = [
+    ("cfd_params_default", C.c_int, [C.POINTER(Params)]),
+    ("cfd_params_flow_conditions", C.c_int, [C.POINTER(Params), C.c_double, C.c_double, C.c_double]),
+    ("cfd_params_freestream_static", C.c_int, [C.POINTER(Params), C.c_double, C.c_double, C.c_double]),
+    ("cfd_params_kinematic_viscosity", C.c_int, [C.POINTER(Params), C.c_double, C.c_double]),
+    ("cfd_params_initial_state", C.c_int, [C.POINTER(Params), _DP]),
+    ("cfd_mesh_build", C.c_int, [C.POINTER(MeshDesc), C.POINTER(_VP)]),
+    ("cfd_mesh_read_gmsh", C.c_int, [C.c_char_p, C.POINTER(_VP)]),
+    ("cfd_mesh_generate", C.c_int, [C.c_int, C.c_double, C.POINTER(_VP)]),
+    ("cfd_mesh_destroy", C.c_int, [_VP]),
+    ("cfd_mesh_sizes", C.c_int, [_VP, _IP, _IP, _IP, _IP]),
+    ("cfd_mesh_get_desc", C.c_int, [_VP, C.POINTER(MeshDesc)]),
+    ("cfd_mesh_get_faces", C.c_int, [_VP, _IP, _IP, _IP, _IP, _IP, _DP, _DP, _DP, _DP, _DP]),
+    ("cfd_mesh_get_cells", C.c_int, [_VP, _DP, _DP, _DP]),
+    ("cfd_mesh_get_reorder", C.c_int, [_VP, _IP, _IP]),
+    ("cfd_mesh_hash", C.c_uint64, [_VP]),
+    ("cfd_mesh_write_gmsh", C.c_int, [_VP, C.c_char_p]),
+    ("cfd_ctx_create", C.c_int, [_VP, C.POINTER(Params), C.POINTER(_VP)]),
+    ("cfd_ctx_destroy", C.c_int, [_VP]),
+    ("cfd_last_error", C.c_char_p, [_VP]),
+    ("cfd_ctx_set_params", C.c_int, [_VP, C.POINTER(Params)]),
+    ("cfd_ctx_get_params", C.c_int, [_VP, C.POINTER(Params)]),
+    ("cfd_pitch", C.c_int, [_VP]),
+    ("cfd_ncells", C.c_int, [_VP]),
+    ("cfd_stream", C.c_void_p, [_VP]),
+    ("cfd_set_face_flux", C.c_int, [_VP, C.POINTER(C.c_ubyte)]),
+    ("cfd_set_assembly", C.c_int, [_VP, C.c_int]),
+    ("cfd_residual", C.c_int, [_VP, _DP, _DP, C.c_int]),
+    ("cfd_get_gradients", C.c_int, [_VP, _DP]),
+    ("cfd_get_limiters", C.c_int, [_VP, _DP]),
+    ("cfd_set_limiters", C.c_int, [_VP, _DP]),
+    ("cfd_local_timestep", C.c_int, [_VP, _DP, C.c_double]),
+    ("cfd_get_dt", C.c_int, [_VP, _DP]),
+    ("cfd_diag_preconditioner", C.c_int, [_VP, _DP, _DP]),
+    ("cfd_jv_fd", C.c_int, [_VP, _DP, _DP, _DP, _DP]),
+    ("cfd_jv_tangent", C.c_int, [_VP, _DP, _DP, _DP, C.c_int]),
+    ("cfd_vjp_adjoint", C.c_int, [_VP, _DP, _DP, _DP]),
+    ("cfd_norm_l2", C.c_int, [_VP, _DP, _DP]),
+    ("cfd_dot", C.c_int, [_VP, _DP, _DP, _DP]),
+    ("cfd_norm_vw", C.c_int, [_VP, _DP, _DP]),
+    ("cfd_gmres_opts_default", C.c_int, [C.POINTER(GmresOpts)]),
+    ("cfd_gmres", C.c_int, [_VP, _DP, _DP, _DP, _DP, C.POINTER(GmresOpts), _IP]),
+    ("cfd_newton_opts_default", C.c_int, [C.POINTER(NewtonOpts)]),
+    ("cfd_solve_implicit", C.c_int, [_VP, _DP, C.POINTER(NewtonOpts), C.POINTER(NewtonResult), _DP, _IP, _DP]),
+    ("cfd_explicit_step", C.c_int, [_VP, _DP, C.c_double]),
+    ("cfd_rk3_step", C.c_int, [_VP, _DP, C.c_double]),
+    ("cfd_wall_outputs", C.c_int, [_VP, _DP, _DP, _DP, _DP]),
+    ("cfd_dev_pack", C.c_int, [_VP, _VP, _VP]),
+    ("cfd_dev_unpack", C.c_int, [_VP, _VP, _VP]),
+    ("cfd_dev_residual", C.c_int, [_VP, _VP, _VP, C.c_int]),
+    ("cfd_dev_jv_fd", C.c_int, [_VP, _VP, _VP, _VP, _VP]),
+    ("cfd_dev_jv_tangent", C.c_int, [_VP, _VP, _VP, _VP, C.c_int]),
+    ("cfd_dev_local_timestep", C.c_int, [_VP, _VP, C.c_double]),
+    ("cfd_dev_synchronize", C.c_int, [_VP]),
+    ("cfd_dev_time_residual", C.c_int, [_VP, _VP, _VP, C.c_int, C.c_int, _DP, _IP]),
+]
+
+_lib = None
+
+
+def load_library(path: str = LIB_PATH):
+    """Load the HIP library; raises if it has not been built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(path):
+        raise CfdError(f"{path} is missing: run __graft_entry__.build() (hipcc, gfx950) first")
+    lib = C.CDLL(path)
+    for name, res, args in SIGNATURES:
+        fn = getattr(lib, name, None)
+        if fn is None:
+            continue
+        fn.restype = res
+        fn.argtypes = args
+    _lib = lib
+    return lib
+
+
+def _dp(a):
+    return a.ctypes.data_as(_DP)
+
+
+def _ip(a):
+    return a.ctypes.data_as(_IP)
+
+
+def _check(rc, ctx=None, what=""):
+    if rc != 0:
+        msg = ""
+        if ctx is not None:
+            m = load_library().cfd_last_error(ctx)
+            msg = m.decode() if m else ""
+        raise CfdError(f"{what} failed with status {rc}: {msg}")
+
+
+# ---- parameters ----------------------------------------------------------------
+def default_params() -> Params:
+    p = Params()
+    _check(load_library().cfd_params_default(C.byref(p)), what="cfd_params_default")
+    return p
+
+
+def flow_conditions(p: Params, M, P, T):
+    _check(load_library().cfd_params_flow_conditions(C.byref(p), M, P, T))
+
+
+def kinematic_viscosity(p: Params, Re, L):
+    _check(load_library().cfd_params_kinematic_viscosity(C.byref(p), Re, L))
+
+
+def initial_state(p: Params) -> np.ndarray:
+    u = np.zeros(6)
+    _check(load_library().cfd_params_initial_state(C.byref(p), _dp(u)))
+    return u
+
+
+# ---- mesh --------------------------------------------------------------------------
+class Mesh:
+    """Mesh::load equivalent: connectivity + geometry + RCM + edge colors (host)."""
+
+    def __init__(self, handle):
+        self._h = C.c_void_p(handle) if not isinstance(handle, C.c_void_p) else handle
+        self.lib = load_library()
+        nn, nc, nf, ncol = C.c_int(), C.c_int(), C.c_int(), C.c_int()
+        _check(self.lib.cfd_mesh_sizes(self._h, C.byref(nn), C.byref(nc), C.byref(nf), C.byref(ncol)))
+        self.n_nodes, self.n_cells, self.n_faces, self.n_colors = nn.value, nc.value, nf.value, ncol.value
+
+    @classmethod
+    def generate(cls, config: int, scale: float = 1.0) -> "Mesh":
+        h = C.c_void_p()
+        _check(load_library().cfd_mesh_generate(config, scale, C.byref(h)), what="cfd_mesh_generate")
+        m = cls(h)
+        m.config = config
+        return m
+
+    @classmethod
+    def read_gmsh(cls, path: str) -> "Mesh":
+        h = C.c_void_p()
+        _check(load_library().cfd_mesh_read_gmsh(path.encode(), C.byref(h)), what="cfd_mesh_read_gmsh")
+        return cls(h)
+
+    @classmethod
+    def from_arrays(cls, x, y, cell_ptr, cell_nodes, bedges) -> "Mesh":
+        x = np.ascontiguousarray(x, np.float64)
+        y = np.ascontiguousarray(y, np.float64)
+        cp = np.ascontiguousarray(cell_ptr, np.int32)
+        cn = np.ascontiguousarray(cell_nodes, np.int32)
+        be = np.ascontiguousarray(bedges, np.int32).reshape(-1)
+        d = MeshDesc(len(x) - 1, _dp(x), _dp(y), len(cp) - 1, _ip(cp), _ip(cn), len(be) // 3, _ip(be))
+        h = C.c_void_p()
+        _check(load_library().cfd_mesh_build(C.byref(d), C.byref(h)), what="cfd_mesh_build")
+        return cls(h)
+
+    def __del__(self):
+        try:
+            if self._h:
+                self.lib.cfd_mesh_destroy(self._h)
+                self._h = None
+        except Exception:
+            pass
+
+    @property
+    def handle(self):
+        return self._h
+
+    def desc(self):
+        """(x, y, cell_ptr, cell_nodes, bedges) copies of the description."""
+        d = MeshDesc()
+        _check(self.lib.cfd_mesh_get_desc(self._h, C.byref(d)))
+        x = np.ctypeslib.as_array(d.x, shape=(d.n_nodes + 1,)).copy()
+        y = np.ctypeslib.as_array(d.y, shape=(d.n_nodes + 1,)).copy()
+        cp = np.ctypeslib.as_array(d.cell_ptr, shape=(d.n_cells + 1,)).copy()
+        cn = np.ctypeslib.as_array(d.cell_nodes, shape=(int(cp[-1]),)).copy()
+        be = (np.ctypeslib.as_array(d.bedges, shape=(3 * d.n_bedges,)).copy().reshape(-1, 3)
+              if d.n_bedges else np.zeros((0, 3), np.int32))
+        return x, y, cp, cn, be
+
+    def faces(self) -> dict:
+        F = self.n_faces
+        ints = {k: np.zeros(F, np.int32) for k in ("node1", "node2", "owner", "neighbor", "bc_tag")}
+        dbl = {k: np.zeros(F) for k in ("nx", "ny", "area", "x_mid", "y_mid")}
+        _check(self.lib.cfd_mesh_get_faces(self._h, *[_ip(ints[k]) for k in ints], *[_dp(dbl[k]) for k in dbl]))
+        return {**ints, **dbl}
+
+    def cells(self) -> dict:
+        N = self.n_cells
+        out = {k: np.zeros(N) for k in ("volume", "x_c", "y_c")}
+        _check(self.lib.cfd_mesh_get_cells(self._h, _dp(out["volume"]), _dp(out["x_c"]), _dp(out["y_c"])))
+        return out
+
+    def reorder(self):
+        perm = np.zeros(self.n_cells, np.int32)
+        color = np.zeros(self.n_faces, np.int32)
+        _check(self.lib.cfd_mesh_get_reorder(self._h, _ip(perm), _ip(color)))
+        return perm, color
+
+    def hash(self) -> int:
+        return int(self.lib.cfd_mesh_hash(self._h))
+
+    def write_gmsh(self, path: str):
+        _check(self.lib.cfd_mesh_write_gmsh(self._h, path.encode()))
+
+
+# ---- solver ------------------------------------------------------------------------
+def _state(a, N):
+    a = np.ascontiguousarray(a, np.float64)
+    if a.shape != (N, 6):
+        raise ValueError(f"state arrays must have shape ({N}, 6), got {a.shape}")
+    return a
+
+
+class Solver:
+    """The reference Solver's residual / Jacobian / Krylov members on the GPU."""
+
+    def __init__(self, mesh: Mesh, params: Params, assembly: int = 0):
+        self.lib = load_library()
+        self.mesh = mesh
+        self.N = mesh.n_cells
+        self._h = C.c_void_p()
+        rc = self.lib.cfd_ctx_create(mesh.handle, C.byref(params), C.byref(self._h))
+        if rc != 0:
+            msg = self.lib.cfd_last_error(self._h) if self._h else b""
+            if self._h:
+                self.lib.cfd_ctx_destroy(self._h)
+            self._h = None
+            raise CfdError(f"cfd_ctx_create failed ({rc}): {msg.decode() if msg else ''}")
+        self.set_assembly(assembly)
+
+    def __del__(self):
+        try:
+            if self._h:
+                self.lib.cfd_ctx_destroy(self._h)
+                self._h = None
+        except Exception:
+            pass
+
+    @property
+    def handle(self):
+        return self._h
+
+    def _c(self, rc, what):
+        _check(rc, self._h, what)
+
+    def set_params(self, p: Params):
+        self._c(self.lib.cfd_ctx_set_params(self._h, C.byref(p)), "cfd_ctx_set_params")
+
+    def params(self) -> Params:
+        p = Params()
+        self._c(self.lib.cfd_ctx_get_params(self._h, C.byref(p)), "cfd_ctx_get_params")
+        return p
+
+    def set_assembly(self, mode: int):
+        self._c(self.lib.cfd_set_assembly(self._h, mode), "cfd_set_assembly")
+
+    def set_face_flux(self, use_alt):
+        a = np.ascontiguousarray(use_alt, np.uint8)
+        self._c(self.lib.cfd_set_face_flux(self._h, a.ctypes.data_as(C.POINTER(C.c_ubyte))), "cfd_set_face_flux")
+
+    # Solver::computeResiduals
+    def compute_residuals(self, U, update_limiters: bool = True) -> np.ndarray:
+        U = _state(U, self.N)
+        R = np.zeros((self.N, 6))
+        self._c(self.lib.cfd_residual(self._h, _dp(U), _dp(R), int(update_limiters)), "cfd_residual")
+        return R
+
+    def gradients(self) -> np.ndarray:
+        g = np.zeros((self.N, 10))
+        self._c(self.lib.cfd_get_gradients(self._h, _dp(g)), "cfd_get_gradients")
+        return g
+
+    def limiters(self) -> np.ndarray:
+        p = np.zeros(self.N)
+        self._c(self.lib.cfd_get_limiters(self._h, _dp(p)), "cfd_get_limiters")
+        return p
+
+    def set_limiters(self, phi):
+        phi = np.ascontiguousarray(phi, np.float64)
+        self._c(self.lib.cfd_set_limiters(self._h, _dp(phi)), "cfd_set_limiters")
+
+    # Solver::computeLocalTimeStep (with U_current = U)
+    def compute_local_time_step(self, U, cfl: float):
+        U = _state(U, self.N)
+        self._c(self.lib.cfd_local_timestep(self._h, _dp(U), cfl), "cfd_local_timestep")
+
+    def dt_local(self) -> np.ndarray:
+        d = np.zeros(self.N)
+        self._c(self.lib.cfd_get_dt(self._h, _dp(d)), "cfd_get_dt")
+        return d
+
+    # Solver::computeDiagonalPreconditioner
+    def diagonal_preconditioner(self, U) -> np.ndarray:
+        U = _state(U, self.N)
+        D = np.zeros(self.N)
+        self._c(self.lib.cfd_diag_preconditioner(self._h, _dp(U), _dp(D)), "cfd_diag_preconditioner")
+        return D
+
+    # Solver::applyJacobian
+    def apply_jacobian(self, U, R, v) -> np.ndarray:
+        U, R, v = (_state(a, self.N) for a in (U, R, v))
+        J = np.zeros((self.N, 6))
+        self._c(self.lib.cfd_jv_fd(self._h, _dp(U), _dp(R), _dp(v), _dp(J)), "cfd_jv_fd")
+        return J
+
+    def jv_tangent(self, U, v, precision: int = 64) -> np.ndarray:
+        U, v = (_state(a, self.N) for a in (U, v))
+        J = np.zeros((self.N, 6))
+        self._c(self.lib.cfd_jv_tangent(self._h, _dp(U), _dp(v), _dp(J), precision), "cfd_jv_tangent")
+        return J
+
+    def vjp_adjoint(self, U, w) -> np.ndarray:
+        U, w = (_state(a, self.N) for a in (U, w))
+        J = np.zeros((self.N, 6))
+        self._c(self.lib.cfd_vjp_adjoint(self._h, _dp(U), _dp(w), _dp(J)), "cfd_vjp_adjoint")
+        return J
+
+    def norm_L2(self, v) -> float:
+        v = _state(v, self.N)
+        out = C.c_double()
+        self._c(self.lib.cfd_norm_l2(self._h, _dp(v), C.byref(out)), "cfd_norm_l2")
+        return out.value
+
+    def dot_product(self, a, b) -> float:
+        a, b = _state(a, self.N), _state(b, self.N)
+        out = C.c_double()
+        self._c(self.lib.cfd_dot(self._h, _dp(a), _dp(b), C.byref(out)), "cfd_dot")
+        return out.value
+
+    def norm_L2_volume_weighted(self, R) -> float:
+        R = _state(R, self.N)
+        out = C.c_double()
+        self._c(self.lib.cfd_norm_vw(self._h, _dp(R), C.byref(out)), "cfd_norm_vw")
+        return out.value
+
+    @staticmethod
+    def gmres_opts(**kw) -> GmresOpts:
+        o = GmresOpts()
+        load_library().cfd_gmres_opts_default(C.byref(o))
+        for k, v in kw.items():
+            setattr(o, k, v)
+        return o
+
+    @staticmethod
+    def newton_opts(gmres: GmresOpts | None = None, **kw) -> NewtonOpts:
+        o = NewtonOpts()
+        load_library().cfd_newton_opts_default(C.byref(o))
+        for k, v in kw.items():
+            setattr(o, k, v)
+        if gmres is not None:
+            o.gmres = gmres
+        return o
+
+    # Solver::solveGMRES
+    def solve_gmres(self, U, R, RHS, opts: GmresOpts | None = None):
+        U, R, RHS = (_state(a, self.N) for a in (U, R, RHS))
+        x = np.zeros((self.N, 6))
+        it = C.c_int()
+        o = opts if opts is not None else self.gmres_opts()
+        self._c(self.lib.cfd_gmres(self._h, _dp(U), _dp(R), _dp(RHS), _dp(x), C.byref(o), C.byref(it)),
+                "cfd_gmres")
+        return x, it.value
+
+    # Solver::solveImplicit
+    def solve_implicit(self, U, opts: NewtonOpts | None = None):
+        U = _state(U, self.N).copy()
+        o = opts if opts is not None else self.newton_opts()
+        n = max(1, o.max_steps)
+        hvm, hit, hcfl = np.zeros(n), np.zeros(n, np.int32), np.zeros(n)
+        res = NewtonResult()
+        self._c(self.lib.cfd_solve_implicit(self._h, _dp(U), C.byref(o), C.byref(res), _dp(hvm), _ip(hit), _dp(hcfl)),
+                "cfd_solve_implicit")
+        k = res.steps + (1 if res.converged else 0)
+        return U, res, {"vm": hvm[:k], "iters": hit[:k], "cfl": hcfl[:k]}
+
+    # Solver::timeStepExplicit
+    def time_step_explicit(self, U, dt: float) -> np.ndarray:
+        U = _state(U, self.N).copy()
+        self._c(self.lib.cfd_explicit_step(self._h, _dp(U), dt), "cfd_explicit_step")
+        return U
+
+    def rk3_step(self, U, dt: float) -> np.ndarray:
+        U = _state(U, self.N).copy()
+        self._c(self.lib.cfd_rk3_step(self._h, _dp(U), dt), "cfd_rk3_step")
+        return U
+
+    def wall_outputs(self, U):
+        U = _state(U, self.N)
+        F = self.mesh.n_faces
+        Cp, Cf, J = np.zeros(F), np.zeros(F), C.c_double()
+        self._c(self.lib.cfd_wall_outputs(self._h, _dp(U), _dp(Cp), _dp(Cf), C.byref(J)), "cfd_wall_outputs")
+        return Cp, Cf, J.value
+
+    # device-handle variants (torch tensors or raw pointers)
+    def dev_residual(self, U_ptr: int, R_ptr: int, update_limiters: bool = True):
+        self._c(self.lib.cfd_dev_residual(self._h, C.c_void_p(U_ptr), C.c_void_p(R_ptr), int(update_limiters)),
+                "cfd_dev_residual")
+
+    def dev_pack(self, U6_ptr: int, U_ptr: int):
+        self._c(self.lib.cfd_dev_pack(self._h, C.c_void_p(U6_ptr), C.c_void_p(U_ptr)), "cfd_dev_pack")
+
+    def dev_unpack(self, U_ptr: int, U6_ptr: int):
+        self._c(self.lib.cfd_dev_unpack(self._h, C.c_void_p(U_ptr), C.c_void_p(U6_ptr)), "cfd_dev_unpack")
+
+    def dev_local_timestep(self, U_ptr: int, cfl: float):
+        self._c(self.lib.cfd_dev_local_timestep(self._h, C.c_void_p(U_ptr), cfl), "cfd_dev_local_timestep")
+
+    def dev_jv_fd(self, U_ptr, Rb_ptr, v_ptr, Jv_ptr):
+        self._c(self.lib.cfd_dev_jv_fd(self._h, C.c_void_p(U_ptr), C.c_void_p(Rb_ptr), C.c_void_p(v_ptr),
+                                       C.c_void_p(Jv_ptr)), "cfd_dev_jv_fd")
+
+    def dev_time_residual(self, U_ptr: int, R_ptr: int, reps: int, update_limiters: bool = False):
+        """Average ms per residual of (gradients, face flux, assembly, total),
+        timed with HIP events on the context stream, and flux launches per residual."""
+        ms = np.zeros(4)
+        nl = C.c_int()
+        self._c(self.lib.cfd_dev_time_residual(self._h, C.c_void_p(U_ptr), C.c_void_p(R_ptr), reps,
+                                               int(update_limiters), _dp(ms), C.byref(nl)), "cfd_dev_time_residual")
+        return ms, nl.value
+
+    def synchronize(self):
+        self._c(self.lib.cfd_dev_synchronize(self._h), "cfd_dev_synchronize")
+
+    @property
+    def pitch(self) -> int:
+        return int(self.lib.cfd_pitch(self._h))
+
+    @property
+    def stream(self) -> int:
+        return int(self.lib.cfd_stream(self._h) or 0)

@@ -1,0 +1,794 @@
+// C-ABI entry points (include/cfd_gpu.h): context set-up, the host-pointer
+// drop-ins for the reference Solver members, and the host side of GMRES and
+// the Newton driver.  Only scalars (norms, one Hessenberg column per Arnoldi
+// step) cross PCIe inside a solve; every vector stays in HBM.
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "ctx.hpp"
+
+using namespace cfd;
+
+namespace {
+
+// scal[] slots
+enum { S_UNORM = 0, S_VNORM = 1, S_BETA = 2, S_VW = 3, S_VWVOL = 4, S_L2 = 5, S_DOT = 6, S_W = 7 };
+
+#define HIP_TRY(ctx, expr)                                                          \
+  do {                                                                              \
+    hipError_t e_ = (expr);                                                         \
+    if (e_ != hipSuccess) {                                                         \
+      (ctx)->err = std::string(#expr) + ": " + hipGetErrorString(e_);               \
+      return CFD_ERR_HIP;                                                           \
+    }                                                                               \
+  } while (0)
+
+int fail(cfd_ctx* c, int code, const std::string& msg) {
+  if (c) c->err = msg;
+  return code;
+}
+
+template <class T> hipError_t dalloc(T** p, size_t n) { return hipMalloc((void**)p, std::max<size_t>(n, 1) * sizeof(T)); }
+
+void fill_phys(cfd_ctx* c) {
+  const cfd_params& p = c->P;
+  Phys& q = c->phys;
+  q.gamma = p.gamma;
+  q.R = p.R_gas;
+  q.P0 = p.P_total_inlet;
+  q.T0 = p.T_total_inlet;
+  q.pexit = p.P_static_exit;
+  q.nu = p.nu;
+  q.Pr = p.Pr;
+  q.Cp = p.gamma * p.R_gas / (p.gamma - 1.0);
+  // frozen freestream, Solver.cpp:1094-1100
+  const double g = p.gamma;
+  const double fT = 1.0 + 0.5 * (g - 1.0) * p.M_inf * p.M_inf;
+  const double Ts = p.T_total_inlet / fT;
+  const double rho = p.P_static_exit / (p.R_gas * Ts);
+  const double cs = std::sqrt(g * p.R_gas * Ts);
+  const double u = p.M_inf * cs;
+  const double v = 0.0;
+  q.fs[0] = rho;
+  q.fs[1] = rho * u;
+  q.fs[2] = rho * v;
+  q.fs[3] = p.P_static_exit / (g - 1.0) + 0.5 * rho * (u * u + v * v);
+  q.fs[4] = rho * 0.0;
+  q.fs[5] = rho * 0.0;
+  q.sh[0] = p.shock_rho;
+  q.sh[1] = p.shock_rho * p.shock_u;
+  q.sh[2] = p.shock_rho * p.shock_v;
+  q.sh[3] = p.shock_p / (g - 1.0) + 0.5 * p.shock_rho * (p.shock_u * p.shock_u + p.shock_v * p.shock_v);
+  q.sh[4] = p.shock_rho * 0.0;
+  q.sh[5] = p.shock_rho * 0.0;
+  q.flux = p.flux;
+  q.flux_alt = p.flux_alt;
+  q.spatial = p.spatial;
+  q.bcset = p.bcset;
+  q.viscous = p.viscous;
+  q.nvar = p.nvar;
+}
+
+int check_params(const cfd_params* p) {
+  if (!p) return CFD_ERR_ARG;
+  if (p->nvar != 4 && p->nvar != 6) return CFD_ERR_ARG;
+  if (p->flux < 0 || p->flux > 3 || p->flux_alt < 0 || p->flux_alt > 3) return CFD_ERR_ARG;
+  if (p->spatial < 0 || p->spatial > 2 || p->bcset < 0 || p->bcset > 3) return CFD_ERR_ARG;
+  return CFD_OK;
+}
+
+void free_krylov(cfd_ctx* c) {
+  Krylov& k = c->kry;
+  (void)hipFree(k.V);
+  (void)hipFree(k.D);
+  (void)hipFree(k.blk);
+  (void)hipFree(k.z);
+  (void)hipFree(k.w);
+  (void)hipFree(k.h);
+  (void)hipHostFree(k.hpin);
+  k = Krylov();
+}
+
+int ensure_krylov(cfd_ctx* c, int m, int prec) {
+  Krylov& k = c->kry;
+  if (k.V && k.m >= m && k.prec == prec) return CFD_OK;
+  free_krylov(c);
+  const size_t es = prec == 32 ? 4 : 8;
+  const size_t vs = (size_t)c->P.nvar * c->pitch;
+  HIP_TRY(c, hipMalloc(&k.V, (size_t)(m + 1) * vs * es));
+  HIP_TRY(c, hipMalloc((void**)&k.D, (size_t)c->pitch * 8));
+  HIP_TRY(c, hipMalloc((void**)&k.z, vs * 8));
+  HIP_TRY(c, hipMalloc((void**)&k.w, vs * 8));
+  HIP_TRY(c, hipMalloc((void**)&k.h, 256 * 8));
+  HIP_TRY(c, hipHostMalloc((void**)&k.hpin, 256 * 8, hipHostMallocDefault));
+  k.m = m;
+  k.prec = prec;
+  return CFD_OK;
+}
+
+// upload a host AoS state vector (reference order) into SoA vector dst
+int upload_state(cfd_ctx* c, const double* U6, double* dst) {
+  HIP_TRY(c, hipMemcpyAsync(c->io6, U6, (size_t)c->N * 6 * 8, hipMemcpyHostToDevice, c->stream));
+  launch_pack(c, c->io6, dst);
+  return CFD_OK;
+}
+int download_state(cfd_ctx* c, const double* src, double* U6) {
+  launch_unpack(c, src, c->io6);
+  HIP_TRY(c, hipMemcpyAsync(U6, c->io6, (size_t)c->N * 6 * 8, hipMemcpyDeviceToHost, c->stream));
+  HIP_TRY(c, hipStreamSynchronize(c->stream));
+  return CFD_OK;
+}
+int sync(cfd_ctx* c) {
+  HIP_TRY(c, hipGetLastError());
+  HIP_TRY(c, hipStreamSynchronize(c->stream));
+  return CFD_OK;
+}
+// copy n device scalars scal[slot..] to host
+int read_scal(cfd_ctx* c, int slot, int n, double* out) {
+  HIP_TRY(c, hipMemcpyAsync(c->scal_host, c->scal + slot, n * 8, hipMemcpyDeviceToHost, c->stream));
+  HIP_TRY(c, hipStreamSynchronize(c->stream));
+  std::memcpy(out, c->scal_host, n * 8);
+  return CFD_OK;
+}
+// per-cell scalar (RCM order) -> host array in reference order
+int download_cellscalar(cfd_ctx* c, const double* d, double* host) {
+  std::vector<double> tmp(c->N);
+  HIP_TRY(c, hipMemcpyAsync(tmp.data(), d, (size_t)c->N * 8, hipMemcpyDeviceToHost, c->stream));
+  HIP_TRY(c, hipStreamSynchronize(c->stream));
+  for (int i = 0; i < c->N; ++i) host[c->iperm[i]] = tmp[i];
+  return CFD_OK;
+}
+
+}  // namespace
+
+// ============================================================================
+extern "C" {
+
+int cfd_params_default(cfd_params* p) {
+  if (!p) return CFD_ERR_ARG;
+  std::memset(p, 0, sizeof *p);
+  p->gamma = 1.4;
+  p->R_gas = 287.0;
+  p->nu = 0.01;
+  p->Pr = 0.72;
+  p->flux = CFD_FLUX_HLLC;
+  p->flux_alt = CFD_FLUX_LLF;
+  p->spatial = CFD_SECOND_ORDER_LIMITED;
+  p->bcset = CFD_BC_BUMP;
+  p->viscous = 0;
+  p->nvar = 4;
+  return CFD_OK;
+}
+
+int cfd_params_flow_conditions(cfd_params* p, double M, double P, double T) {
+  if (!p) return CFD_ERR_ARG;
+  const double g = p->gamma;
+  p->M_inf = M;
+  const double fT = 1.0 + 0.5 * (g - 1.0) * M * M;
+  const double fP = std::pow(fT, g / (g - 1.0));
+  p->T_total_inlet = T * fT;
+  p->P_total_inlet = P * fP;
+  p->P_static_exit = P;
+  return CFD_OK;
+}
+
+int cfd_params_freestream_static(cfd_params* p, double M, double P, double T) {
+  return cfd_params_flow_conditions(p, M, P, T);  // Solver.cpp:2017-2025 computes the same fields
+}
+
+int cfd_params_kinematic_viscosity(cfd_params* p, double Re, double L) {
+  if (!p || !(Re > 0)) return CFD_ERR_ARG;
+  const double g = p->gamma;
+  const double fT = 1.0 + 0.5 * (g - 1.0) * p->M_inf * p->M_inf;
+  const double Ts = p->T_total_inlet / fT;
+  const double c = std::sqrt(g * p->R_gas * Ts);
+  const double u = p->M_inf * c;
+  p->nu = (u * L) / Re;
+  return CFD_OK;
+}
+
+int cfd_params_initial_state(const cfd_params* p, double* U6) {
+  if (!p || !U6) return CFD_ERR_ARG;
+  const double g = p->gamma, R = p->R_gas;
+  const double pr = p->P_static_exit;
+  const double fT = std::pow(p->P_total_inlet / pr, (g - 1.0) / g);
+  const double T = p->T_total_inlet / fT;
+  const double rho = pr / (R * T);
+  const double V = std::sqrt(2.0 * (g * R / (g - 1.0)) * (p->T_total_inlet - T));
+  const double u = V, v = 0.0;
+  U6[0] = rho;
+  U6[1] = rho * u;
+  U6[2] = rho * v;
+  U6[3] = pr / (g - 1.0) + 0.5 * rho * (u * u + v * v);
+  U6[4] = rho * 0.0;
+  U6[5] = rho * 0.0;
+  return CFD_OK;
+}
+
+int cfd_gmres_opts_default(cfd_gmres_opts* o) {
+  if (!o) return CFD_ERR_ARG;
+  std::memset(o, 0, sizeof *o);
+  o->m = 40;
+  o->restarts = 0;
+  o->tol = 1e-2;
+  o->jv = 0;
+  o->precision = 64;
+  o->precond = 0;
+  return CFD_OK;
+}
+
+int cfd_newton_opts_default(cfd_newton_opts* o) {
+  if (!o) return CFD_ERR_ARG;
+  std::memset(o, 0, sizeof *o);
+  o->tolerance = 1e-6;
+  o->rel_drop = 0.0;
+  o->max_steps = 50;
+  o->log = 0;
+  o->cfl0 = 0.1;
+  cfd_gmres_opts_default(&o->gmres);
+  return CFD_OK;
+}
+
+int cfd_ctx_create(const cfd_mesh* m, const cfd_params* p, cfd_ctx** out) {
+  if (!m || !out || check_params(p)) return CFD_ERR_ARG;
+  cfd_ctx* c = new cfd_ctx();
+  *out = c;
+  c->P = *p;
+  fill_phys(c);
+  HIP_TRY(c, hipGetDevice(&c->device));
+  HIP_TRY(c, hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+  const int N = m->ncells(), F = m->nfaces();
+  c->N = N;
+  c->F = F;
+  c->pitch = (N + 63) / 64 * 64;
+  c->ncolors = m->ncolors;
+  c->color_ptr = m->color_ptr;
+  c->face_order = m->face_order;
+  c->perm = m->perm;
+  c->iperm = m->iperm;
+  const int P_ = c->pitch;
+  // cells in RCM order
+  std::vector<double> vol(P_, 1.0), xc(P_, 0.0), yc(P_, 0.0);
+  for (int i = 0; i < N; ++i) {
+    const int o = m->iperm[i];
+    vol[i] = m->vol[o];
+    xc[i] = m->xc[o];
+    yc[i] = m->yc[o];
+  }
+  // faces in device order
+  std::vector<int> fpos(F);  // reference face -> device face
+  for (int d = 0; d < F; ++d) fpos[m->face_order[d]] = d;
+  std::vector<int> own(F), nei(F), tag(F);
+  std::vector<double> nx(F), ny(F), area(F), xm(F), ym(F);
+  for (int d = 0; d < F; ++d) {
+    const int f = m->face_order[d];
+    own[d] = m->perm[m->own[f]];
+    nei[d] = m->nei[f] >= 0 ? m->perm[m->nei[f]] : -1;
+    tag[d] = m->tag[f];
+    nx[d] = m->nx[f];
+    ny[d] = m->ny[f];
+    area[d] = m->area[f];
+    xm[d] = m->xm[f];
+    ym[d] = m->ym[f];
+  }
+  // per-cell face slots in ascending reference face index
+  std::vector<int4> slots(P_, make_int4(-1, -1, -1, -1));
+  std::vector<int> cnt(N, 0);
+  for (int f = 0; f < F; ++f) {  // ascending reference index
+    const int d = fpos[f];
+    const int co = m->perm[m->own[f]];
+    int* so = &slots[co].x;
+    if (cnt[co] >= 4) return fail(c, CFD_ERR_MESH, "cell with more than 4 faces");
+    so[cnt[co]++] = d << 1;
+    if (m->nei[f] >= 0) {
+      const int cn = m->perm[m->nei[f]];
+      int* sn = &slots[cn].x;
+      if (cnt[cn] >= 4) return fail(c, CFD_ERR_MESH, "cell with more than 4 faces");
+      sn[cnt[cn]++] = (d << 1) | 1;
+    }
+  }
+  HIP_TRY(c, dalloc(&c->d_vol, P_));
+  HIP_TRY(c, dalloc(&c->d_xc, P_));
+  HIP_TRY(c, dalloc(&c->d_yc, P_));
+  HIP_TRY(c, dalloc(&c->d_slots, P_));
+  HIP_TRY(c, dalloc(&c->d_own, F));
+  HIP_TRY(c, dalloc(&c->d_nei, F));
+  HIP_TRY(c, dalloc(&c->d_tag, F));
+  HIP_TRY(c, dalloc(&c->d_nx, F));
+  HIP_TRY(c, dalloc(&c->d_ny, F));
+  HIP_TRY(c, dalloc(&c->d_area, F));
+  HIP_TRY(c, dalloc(&c->d_xm, F));
+  HIP_TRY(c, dalloc(&c->d_ym, F));
+  HIP_TRY(c, dalloc(&c->d_iperm, N));
+  HIP_TRY(c, dalloc(&c->d_facemap, F));
+  HIP_TRY(c, hipMemcpy(c->d_vol, vol.data(), P_ * 8, hipMemcpyHostToDevice));
+  HIP_TRY(c, hipMemcpy(c->d_xc, xc.data(), P_ * 8, hipMemcpyHostToDevice));
+  HIP_TRY(c, hipMemcpy(c->d_yc, yc.data(), P_ * 8, hipMemcpyHostToDevice));
+  HIP_TRY(c, hipMemcpy(c->d_slots, slots.data(), P_ * sizeof(int4), hipMemcpyHostToDevice));
+  HIP_TRY(c, hipMemcpy(c->d_own, own.data(), F * 4, hipMemcpyHostToDevice));
+  HIP_TRY(c, hipMemcpy(c->d_nei, nei.data(), F * 4, hipMemcpyHostToDevice));
+  HIP_TRY(c, hipMemcpy(c->d_tag, tag.data(), F * 4, hipMemcpyHostToDevice));
+  HIP_TRY(c, hipMemcpy(c->d_nx, nx.data(), F * 8, hipMemcpyHostToDevice));
+  HIP_TRY(c, hipMemcpy(c->d_ny, ny.data(), F * 8, hipMemcpyHostToDevice));
+  HIP_TRY(c, hipMemcpy(c->d_area, area.data(), F * 8, hipMemcpyHostToDevice));
+  HIP_TRY(c, hipMemcpy(c->d_xm, xm.data(), F * 8, hipMemcpyHostToDevice));
+  HIP_TRY(c, hipMemcpy(c->d_ym, ym.data(), F * 8, hipMemcpyHostToDevice));
+  HIP_TRY(c, hipMemcpy(c->d_iperm, m->iperm.data(), N * 4, hipMemcpyHostToDevice));
+  HIP_TRY(c, hipMemcpy(c->d_facemap, m->face_order.data(), F * 4, hipMemcpyHostToDevice));
+  DMesh& d = c->dm;
+  d.N = N;
+  d.F = F;
+  d.pitch = P_;
+  d.vol = c->d_vol;
+  d.xc = c->d_xc;
+  d.yc = c->d_yc;
+  d.slots = c->d_slots;
+  d.own = c->d_own;
+  d.nei = c->d_nei;
+  d.tag = c->d_tag;
+  d.nx = c->d_nx;
+  d.ny = c->d_ny;
+  d.area = c->d_area;
+  d.xm = c->d_xm;
+  d.ym = c->d_ym;
+  d.alt = nullptr;
+  // solver buffers
+  HIP_TRY(c, dalloc(&c->grad, (size_t)10 * P_));
+  HIP_TRY(c, dalloc(&c->phi, P_));
+  HIP_TRY(c, dalloc(&c->dt, P_));
+  HIP_TRY(c, dalloc(&c->Fm, (size_t)6 * F));
+  HIP_TRY(c, dalloc(&c->red, 65536));
+  HIP_TRY(c, dalloc(&c->scal, 64));
+  HIP_TRY(c, dalloc(&c->gradT, (size_t)20 * P_));
+  HIP_TRY(c, hipHostMalloc((void**)&c->scal_host, 64 * 8, hipHostMallocDefault));
+  HIP_TRY(c, dalloc(&c->io6, (size_t)6 * N));
+  for (int i = 0; i < 6; ++i) {
+    HIP_TRY(c, dalloc(&c->vec[i], (size_t)6 * P_));
+    HIP_TRY(c, hipMemset(c->vec[i], 0, (size_t)6 * P_ * 8));
+  }
+  HIP_TRY(c, hipMemset(c->phi, 0, (size_t)P_ * 8));
+  HIP_TRY(c, hipDeviceSynchronize());
+  return CFD_OK;
+}
+
+int cfd_ctx_destroy(cfd_ctx* c) {
+  if (!c) return CFD_OK;
+  (void)hipStreamSynchronize(c->stream);
+  free_krylov(c);
+  void* bufs[] = {c->d_vol, c->d_xc,  c->d_yc,  c->d_slots, c->d_own, c->d_nei, c->d_tag,  c->d_nx,
+                  c->d_ny,  c->d_area, c->d_xm, c->d_ym,    c->d_iperm, c->d_facemap, c->d_alt, c->grad,
+                  c->phi,   c->dt,    c->Fm,    c->red,     c->scal, c->gradT, c->io6};
+  for (void* b : bufs) (void)hipFree(b);
+  for (int i = 0; i < 6; ++i) (void)hipFree(c->vec[i]);
+  (void)hipHostFree(c->scal_host);
+  if (c->stream) (void)hipStreamDestroy(c->stream);
+  delete c;
+  return CFD_OK;
+}
+
+const char* cfd_last_error(const cfd_ctx* c) { return c ? c->err.c_str() : "null context"; }
+
+int cfd_ctx_set_params(cfd_ctx* c, const cfd_params* p) {
+  if (!c || check_params(p)) return CFD_ERR_ARG;
+  if (p->nvar != c->P.nvar) return fail(c, CFD_ERR_ARG, "nvar cannot change on a live context");
+  c->P = *p;
+  fill_phys(c);
+  return CFD_OK;
+}
+int cfd_ctx_get_params(const cfd_ctx* c, cfd_params* p) {
+  if (!c || !p) return CFD_ERR_ARG;
+  *p = c->P;
+  return CFD_OK;
+}
+int cfd_pitch(const cfd_ctx* c) { return c ? c->pitch : -1; }
+int cfd_ncells(const cfd_ctx* c) { return c ? c->N : -1; }
+void* cfd_stream(const cfd_ctx* c) { return c ? (void*)c->stream : nullptr; }
+
+int cfd_set_face_flux(cfd_ctx* c, const unsigned char* use_alt) {
+  if (!c) return CFD_ERR_ARG;
+  if (!use_alt) {
+    c->dm.alt = nullptr;
+    return CFD_OK;
+  }
+  std::vector<unsigned char> a(c->F);
+  for (int d = 0; d < c->F; ++d) a[d] = use_alt[c->face_order[d]] ? 1 : 0;
+  if (!c->d_alt) HIP_TRY(c, dalloc(&c->d_alt, c->F));
+  HIP_TRY(c, hipMemcpy(c->d_alt, a.data(), c->F, hipMemcpyHostToDevice));
+  c->dm.alt = c->d_alt;
+  return CFD_OK;
+}
+
+int cfd_set_assembly(cfd_ctx* c, int mode) {
+  if (!c || (mode != 0 && mode != 1)) return CFD_ERR_ARG;
+  c->assembly = mode;
+  return CFD_OK;
+}
+
+// ---- host-pointer drop-ins -----------------------------------------------------
+int cfd_residual(cfd_ctx* c, const double* U6, double* R6, int update_limiters) {
+  if (!c || !U6 || !R6) return CFD_ERR_ARG;
+  int rc = upload_state(c, U6, c->vec[0]);
+  if (rc) return rc;
+  launch_residual(c, c->vec[0], c->vec[1], update_limiters != 0);
+  return download_state(c, c->vec[1], R6);
+}
+
+int cfd_get_gradients(cfd_ctx* c, double* g10) {
+  if (!c || !g10) return CFD_ERR_ARG;
+  std::vector<double> tmp((size_t)10 * c->pitch);
+  HIP_TRY(c, hipMemcpyAsync(tmp.data(), c->grad, tmp.size() * 8, hipMemcpyDeviceToHost, c->stream));
+  HIP_TRY(c, hipStreamSynchronize(c->stream));
+  for (int i = 0; i < c->N; ++i)
+    for (int q = 0; q < 10; ++q) g10[(size_t)c->iperm[i] * 10 + q] = tmp[(size_t)q * c->pitch + i];
+  return CFD_OK;
+}
+
+int cfd_get_limiters(cfd_ctx* c, double* phi) {
+  if (!c || !phi) return CFD_ERR_ARG;
+  return download_cellscalar(c, c->phi, phi);
+}
+
+int cfd_set_limiters(cfd_ctx* c, const double* phi) {
+  if (!c || !phi) return CFD_ERR_ARG;
+  std::vector<double> tmp(c->pitch, 1.0);
+  for (int i = 0; i < c->N; ++i) tmp[i] = phi[c->iperm[i]];
+  HIP_TRY(c, hipMemcpy(c->phi, tmp.data(), (size_t)c->pitch * 8, hipMemcpyHostToDevice));
+  c->phi_valid = true;
+  return CFD_OK;
+}
+
+int cfd_local_timestep(cfd_ctx* c, const double* U6, double cfl) {
+  if (!c || !U6) return CFD_ERR_ARG;
+  int rc = upload_state(c, U6, c->vec[0]);
+  if (rc) return rc;
+  launch_timestep(c, c->vec[0], cfl);
+  return sync(c);
+}
+
+int cfd_get_dt(cfd_ctx* c, double* dt) {
+  if (!c || !dt) return CFD_ERR_ARG;
+  if (!c->dt_valid) return fail(c, CFD_ERR_STATE, "no local time step computed");
+  return download_cellscalar(c, c->dt, dt);
+}
+
+int cfd_diag_preconditioner(cfd_ctx* c, const double* U6, double* D) {
+  if (!c || !U6 || !D) return CFD_ERR_ARG;
+  if (!c->dt_valid) return fail(c, CFD_ERR_STATE, "no local time step computed");
+  int rc = upload_state(c, U6, c->vec[0]);
+  if (rc) return rc;
+  rc = ensure_krylov(c, 1, 64);
+  if (rc) return rc;
+  launch_diag_precond(c, c->vec[0], c->kry.D);
+  return download_cellscalar(c, c->kry.D, D);
+}
+
+int cfd_jv_fd(cfd_ctx* c, const double* U6, const double* R6, const double* v6, double* Jv6) {
+  if (!c || !U6 || !R6 || !v6 || !Jv6) return CFD_ERR_ARG;
+  if (!c->dt_valid) return fail(c, CFD_ERR_STATE, "no local time step computed");
+  int rc;
+  if ((rc = upload_state(c, U6, c->vec[0]))) return rc;
+  if ((rc = upload_state(c, R6, c->vec[1]))) return rc;
+  if ((rc = upload_state(c, v6, c->vec[2]))) return rc;
+  launch_reduce(c, RED_L2, c->vec[0], nullptr, S_UNORM);
+  launch_reduce(c, RED_L2, c->vec[2], nullptr, S_VNORM);
+  launch_jv_fd(c, c->vec[0], c->vec[1], c->vec[2], c->vec[3], S_UNORM, S_VNORM, c->vec[4], c->vec[5]);
+  return download_state(c, c->vec[3], Jv6);
+}
+
+int cfd_norm_l2(cfd_ctx* c, const double* v6, double* out) {
+  if (!c || !v6 || !out) return CFD_ERR_ARG;
+  int rc = upload_state(c, v6, c->vec[0]);
+  if (rc) return rc;
+  launch_reduce(c, RED_L2, c->vec[0], nullptr, S_L2);
+  double s;
+  if ((rc = read_scal(c, S_L2, 1, &s))) return rc;
+  *out = std::sqrt(s);
+  return CFD_OK;
+}
+
+int cfd_dot(cfd_ctx* c, const double* a6, const double* b6, double* out) {
+  if (!c || !a6 || !b6 || !out) return CFD_ERR_ARG;
+  int rc;
+  if ((rc = upload_state(c, a6, c->vec[0]))) return rc;
+  if ((rc = upload_state(c, b6, c->vec[1]))) return rc;
+  launch_reduce(c, RED_DOT, c->vec[0], c->vec[1], S_DOT);
+  return read_scal(c, S_DOT, 1, out);
+}
+
+int cfd_norm_vw(cfd_ctx* c, const double* R6, double* out) {
+  if (!c || !R6 || !out) return CFD_ERR_ARG;
+  int rc = upload_state(c, R6, c->vec[0]);
+  if (rc) return rc;
+  launch_reduce(c, RED_VW, c->vec[0], nullptr, S_VW);
+  double s[2];
+  if ((rc = read_scal(c, S_VW, 2, s))) return rc;
+  *out = s[1] == 0 ? 0.0 : std::sqrt(s[0] / s[1]);
+  return CFD_OK;
+}
+
+// ---- GMRES (Solver.cpp:1472-1618) on device vectors -------------------------------
+// U, Rb: SoA device vectors; rhs = sign * r (sign -1 passes the residual itself)
+static int gmres_dev(cfd_ctx* c, const double* U, const double* Rb, const double* r, double sign, double* x,
+                     const cfd_gmres_opts* o, int* iters, double* tmpU, double* tmpR) {
+  const int m = o->m, prec = o->precision;
+  if (m < 1 || m > 60) return fail(c, CFD_ERR_ARG, "GMRES m must be in [1, 60]");
+  if (prec != 64) return fail(c, CFD_ERR_ARG, "only FP64 GMRES is available");
+  if (o->jv != 0) return fail(c, CFD_ERR_ARG, "tangent Jv is not available");
+  int rc = ensure_krylov(c, m, prec);
+  if (rc) return rc;
+  Krylov& k = c->kry;
+  const size_t es = prec == 32 ? 4 : 8;
+  const size_t vs = (size_t)c->P.nvar * c->pitch;
+  char* V = (char*)k.V;
+  launch_diag_precond(c, U, k.D);
+  launch_reduce(c, RED_L2, U, nullptr, S_UNORM);
+  launch_reduce(c, RED_L2, r, nullptr, S_BETA);
+  double beta2;
+  if ((rc = read_scal(c, S_BETA, 1, &beta2))) return rc;
+  const double beta = std::sqrt(beta2);
+  launch_fill(c, x, 0.0, vs);
+  if (beta < 1e-12) {
+    *iters = 0;
+    return sync(c);
+  }
+  launch_krylov_init(c, V, r, sign, S_BETA, prec);
+  std::vector<std::vector<double>> H(m + 1, std::vector<double>(m, 0.0));
+  std::vector<double> sn(m, 0.0), cs(m, 0.0), g(m + 1, 0.0);
+  g[0] = beta;
+  int kk = 0;
+  for (int j = 0; j < m; ++j) {
+    kk = j;
+    void* Vj = V + (size_t)j * vs * es;
+    void* Vj1 = V + (size_t)(j + 1) * vs * es;
+    launch_precond_apply(c, k.z, Vj, prec, S_VNORM);
+    launch_jv_fd(c, U, Rb, k.z, (double*)Vj1, S_UNORM, S_VNORM, tmpU, tmpR);
+    launch_orth(c, V, j, nullptr, prec, k.h);
+    launch_store_basis(c, Vj1, nullptr, k.h, j, prec);
+    HIP_TRY(c, hipMemcpyAsync(k.hpin, k.h, (size_t)(j + 2) * 8, hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    for (int i = 0; i <= j + 1; ++i) H[i][j] = k.hpin[i];
+    for (int i = 0; i < j; ++i) {
+      const double t = cs[i] * H[i][j] + sn[i] * H[i + 1][j];
+      H[i + 1][j] = -sn[i] * H[i][j] + cs[i] * H[i + 1][j];
+      H[i][j] = t;
+    }
+    const double a = H[j][j], b = H[j + 1][j];
+    if (std::abs(b) < 1e-14) {
+      cs[j] = 1.0;
+      sn[j] = 0.0;
+    } else {
+      const double t = std::sqrt(a * a + b * b);
+      cs[j] = a / t;
+      sn[j] = b / t;
+    }
+    H[j][j] = cs[j] * a + sn[j] * b;
+    H[j + 1][j] = 0.0;
+    g[j + 1] = -sn[j] * g[j];
+    g[j] = cs[j] * g[j];
+    if (std::abs(g[j + 1]) < o->tol * beta) break;
+  }
+  std::vector<double> y(kk + 1);
+  for (int i = kk; i >= 0; --i) {
+    y[i] = g[i];
+    for (int l = i + 1; l <= kk; ++l) y[i] -= H[i][l] * y[l];
+    y[i] /= H[i][i];
+  }
+  std::memcpy(k.hpin, y.data(), (kk + 1) * 8);
+  HIP_TRY(c, hipMemcpyAsync(k.h + 128, k.hpin, (kk + 1) * 8, hipMemcpyHostToDevice, c->stream));
+  launch_solution(c, x, V, k.h + 128, kk, prec);
+  *iters = kk + 1;
+  return sync(c);
+}
+
+int cfd_gmres(cfd_ctx* c, const double* U6, const double* R6, const double* RHS6, double* x6,
+              const cfd_gmres_opts* opts, int* iters_out) {
+  if (!c || !U6 || !R6 || !RHS6 || !x6) return CFD_ERR_ARG;
+  if (!c->dt_valid) return fail(c, CFD_ERR_STATE, "no local time step computed");
+  cfd_gmres_opts o;
+  if (opts) o = *opts;
+  else cfd_gmres_opts_default(&o);
+  int rc;
+  if ((rc = upload_state(c, U6, c->vec[0]))) return rc;
+  if ((rc = upload_state(c, R6, c->vec[1]))) return rc;
+  if ((rc = upload_state(c, RHS6, c->vec[2]))) return rc;
+  int it = 0;
+  if ((rc = gmres_dev(c, c->vec[0], c->vec[1], c->vec[2], 1.0, c->vec[3], &o, &it, c->vec[4], c->vec[5]))) return rc;
+  if (iters_out) *iters_out = it;
+  return download_state(c, c->vec[3], x6);
+}
+
+// ---- Newton / pseudo-transient driver (Solver.cpp:1696-1799) ---------------------
+int cfd_solve_implicit(cfd_ctx* c, double* U6, const cfd_newton_opts* opts, cfd_newton_result* res,
+                       double* hist_vm, int* hist_iters, double* hist_cfl) {
+  if (!c || !U6) return CFD_ERR_ARG;
+  cfd_newton_opts o;
+  if (opts) o = *opts;
+  else cfd_newton_opts_default(&o);
+  const auto t0 = std::chrono::steady_clock::now();
+  double* U = c->vec[0];
+  double* R = c->vec[1];
+  double* dU = c->vec[2];
+  int rc = upload_state(c, U6, U);
+  if (rc) return rc;
+  double cfl = o.cfl0 > 0 ? o.cfl0 : 0.1;
+  double vm0 = -1.0, vm = 0.0;
+  int step = 0, conv = 0, total_it = 0;
+  for (step = 0; step < o.max_steps; ++step) {
+    launch_timestep(c, U, cfl);
+    const bool upd = !(step > 150 || cfl > 10.0);
+    launch_residual(c, U, R, upd);
+    launch_reduce(c, RED_VW, R, nullptr, S_VW);
+    double s[2];
+    if ((rc = read_scal(c, S_VW, 2, s))) return rc;
+    vm = s[1] == 0 ? 0.0 : std::sqrt(s[0] / s[1]);
+    if (vm0 < 0) vm0 = vm;
+    if (hist_vm) hist_vm[step] = vm;
+    if (hist_cfl) hist_cfl[step] = cfl;
+    if (!(vm == vm)) return fail(c, CFD_ERR_NUMERIC, "residual norm is NaN");
+    if (vm < o.tolerance || (o.rel_drop > 0 && vm <= o.rel_drop * vm0)) {
+      conv = 1;
+      if (hist_iters) hist_iters[step] = 0;
+      break;
+    }
+    int it = 0;
+    if ((rc = gmres_dev(c, U, R, R, -1.0, dU, &o.gmres, &it, c->vec[4], c->vec[5]))) return rc;
+    total_it += it;
+    if (hist_iters) hist_iters[step] = it;
+    if (o.log) std::printf("{\"step\": %d, \"vm\": %.6e, \"cfl\": %.4g, \"gmres\": %d}\n", step, vm, cfl, it);
+    if (it >= 29) {
+      cfl *= 0.5;
+      if (cfl < 0.05) cfl = 0.05;
+    }
+    const double relax = vm > 1e6 ? 0.5 : 1.0;
+    launch_newton_update(c, U, dU, relax);
+    if (step > 5) {
+      cfl *= vm > 1e6 ? 1.2 : 1.5;
+      if (it >= 25) cfl *= 0.5;
+      if (cfl > 10000) cfl = 10000;
+      if (cfl < 0.1) cfl = 0.1;
+    }
+  }
+  if ((rc = download_state(c, U, U6))) return rc;
+  if (res) {
+    res->steps = step;
+    res->converged = conv;
+    res->res_vm0 = vm0;
+    res->res_vm = vm;
+    res->total_gmres_iters = total_it;
+    res->seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+  }
+  return CFD_OK;
+}
+
+int cfd_explicit_step(cfd_ctx* c, double* U6, double dt) {
+  if (!c || !U6) return CFD_ERR_ARG;
+  int rc = upload_state(c, U6, c->vec[0]);
+  if (rc) return rc;
+  launch_residual(c, c->vec[0], c->vec[1], true);
+  launch_explicit(c, c->vec[0], c->vec[0], 0.0, c->vec[0], 1.0, c->vec[1], dt);
+  return download_state(c, c->vec[0], U6);
+}
+
+int cfd_rk3_step(cfd_ctx* c, double* U6, double dt) {
+  if (!c || !U6) return CFD_ERR_ARG;
+  int rc = upload_state(c, U6, c->vec[0]);
+  if (rc) return rc;
+  double *U0 = c->vec[0], *R = c->vec[1], *U1 = c->vec[2], *U2 = c->vec[3];
+  launch_residual(c, U0, R, true);
+  launch_explicit(c, U1, U0, 0.0, U0, 1.0, R, dt);
+  launch_residual(c, U1, R, true);
+  launch_explicit(c, U2, U0, 0.75, U1, 0.25, R, dt);
+  launch_residual(c, U2, R, true);
+  launch_explicit(c, U0, U0, 1.0 / 3.0, U2, 2.0 / 3.0, R, dt);
+  return download_state(c, U0, U6);
+}
+
+int cfd_wall_outputs(cfd_ctx* c, const double* U6, double* Cp, double* Cf, double* J) {
+  if (!c || !U6) return CFD_ERR_ARG;
+  int rc = upload_state(c, U6, c->vec[0]);
+  if (rc) return rc;
+  double* out3 = c->Fm;  // 3 x F scratch (face buffer is free between residuals)
+  launch_wall(c, c->vec[0], out3);
+  std::vector<double> h((size_t)3 * c->F);
+  HIP_TRY(c, hipMemcpyAsync(h.data(), out3, h.size() * 8, hipMemcpyDeviceToHost, c->stream));
+  HIP_TRY(c, hipStreamSynchronize(c->stream));
+  std::vector<int> fpos(c->F);
+  for (int d = 0; d < c->F; ++d) fpos[c->face_order[d]] = d;
+  double j = 0.0;
+  for (int f = 0; f < c->F; ++f) {
+    const int d = fpos[f];
+    if (Cp) Cp[f] = h[d];
+    if (Cf) Cf[f] = h[c->F + d];
+    j += h[2 * c->F + d];
+  }
+  if (J) *J = j;
+  return CFD_OK;
+}
+
+// ---- device-handle variants --------------------------------------------------------
+int cfd_dev_pack(cfd_ctx* c, const double* U6_dev, double* U) {
+  if (!c || !U6_dev || !U) return CFD_ERR_ARG;
+  launch_pack(c, U6_dev, U);
+  HIP_TRY(c, hipGetLastError());
+  return CFD_OK;
+}
+int cfd_dev_unpack(cfd_ctx* c, const double* U, double* U6_dev) {
+  if (!c || !U6_dev || !U) return CFD_ERR_ARG;
+  launch_unpack(c, U, U6_dev);
+  HIP_TRY(c, hipGetLastError());
+  return CFD_OK;
+}
+int cfd_dev_residual(cfd_ctx* c, const double* U, double* R, int update_limiters) {
+  if (!c || !U || !R) return CFD_ERR_ARG;
+  launch_residual(c, U, R, update_limiters != 0);
+  HIP_TRY(c, hipGetLastError());
+  return CFD_OK;
+}
+int cfd_dev_local_timestep(cfd_ctx* c, const double* U, double cfl) {
+  if (!c || !U) return CFD_ERR_ARG;
+  launch_timestep(c, U, cfl);
+  HIP_TRY(c, hipGetLastError());
+  return CFD_OK;
+}
+int cfd_dev_jv_fd(cfd_ctx* c, const double* U, const double* Rb, const double* v, double* Jv) {
+  if (!c || !U || !Rb || !v || !Jv) return CFD_ERR_ARG;
+  if (!c->dt_valid) return fail(c, CFD_ERR_STATE, "no local time step computed");
+  launch_reduce(c, RED_L2, U, nullptr, S_UNORM);
+  launch_reduce(c, RED_L2, v, nullptr, S_VNORM);
+  launch_jv_fd(c, U, Rb, v, Jv, S_UNORM, S_VNORM, c->vec[4], c->vec[5]);
+  HIP_TRY(c, hipGetLastError());
+  return CFD_OK;
+}
+int cfd_dev_synchronize(cfd_ctx* c) {
+  if (!c) return CFD_ERR_ARG;
+  return sync(c);
+}
+
+int cfd_dev_time_residual(cfd_ctx* c, const double* U, double* R, int reps, int update_limiters, double* ms_out,
+                          int* launches_out) {
+  if (!c || !U || !R || reps < 1 || reps > 100000 || !ms_out) return CFD_ERR_ARG;
+  // five events per residual, recorded back to back on the context stream;
+  // the host waits once, after the last residual
+  std::vector<hipEvent_t> ev((size_t)5 * reps, nullptr);
+  int rc = CFD_OK;
+  for (auto& e : ev) {
+    hipError_t er = hipEventCreate(&e);
+    if (er != hipSuccess) {
+      rc = fail(c, CFD_ERR_HIP, hipGetErrorString(er));
+      break;
+    }
+  }
+  if (rc == CFD_OK) {
+    for (int r = 0; r < reps; ++r) launch_residual_timed(c, U, R, update_limiters != 0, &ev[(size_t)5 * r]);
+    hipError_t er = hipEventSynchronize(ev.back());
+    if (er != hipSuccess) rc = fail(c, CFD_ERR_HIP, hipGetErrorString(er));
+  }
+  double acc[4] = {0, 0, 0, 0};
+  if (rc == CFD_OK) {
+    for (int r = 0; r < reps; ++r) {
+      hipEvent_t* e = &ev[(size_t)5 * r];
+      float t01 = 0, t12 = 0, t23 = 0, t34 = 0, t04 = 0;
+      (void)hipEventElapsedTime(&t01, e[0], e[1]);
+      (void)hipEventElapsedTime(&t12, e[1], e[2]);
+      (void)hipEventElapsedTime(&t23, e[2], e[3]);
+      (void)hipEventElapsedTime(&t34, e[3], e[4]);
+      (void)hipEventElapsedTime(&t04, e[0], e[4]);
+      acc[0] += t01;
+      acc[1] += t23;
+      acc[2] += t12 + t34;
+      acc[3] += t04;
+    }
+  }
+  for (auto& e : ev)
+    if (e) (void)hipEventDestroy(e);
+  for (int i = 0; i < 4; ++i) ms_out[i] = acc[i] / reps;
+  if (launches_out) *launches_out = c->assembly == 0 ? c->ncolors : 1;
+  return rc;
+}
+
+}  // extern "C"

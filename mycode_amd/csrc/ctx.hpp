@@ -1,0 +1,111 @@
+// Device context: SoA mesh arrays in RCM order, per-context buffers, and the
+// launchers the C-ABI entry points are built from.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <string>
+#include <vector>
+
+#include "mesh.hpp"
+#include "physics.hpp"
+
+namespace cfd {
+
+// Device view of the mesh.  Cells are in RCM order; faces are grouped by
+// edge color (color-major, then owner position).  slots[c] lists the faces of
+// cell c in ascending reference face index, each as (device face << 1) | side
+// with side 1 when c is the neighbor; -1 pads triangles.
+struct DMesh {
+  int N, F, pitch;
+  const double *vol, *xc, *yc;
+  const int4* slots;
+  const int *own, *nei, *tag;
+  const double *nx, *ny, *area, *xm, *ym;
+  const unsigned char* alt;  // per device face: 1 = use Phys::flux_alt (may be null)
+};
+
+// Workspace of one GMRES solve.
+struct Krylov {
+  int m = 0, prec = 64;        // basis size, element precision (64 or 32)
+  void* V = nullptr;           // (m + 1) vectors, nv x pitch each
+  double* D = nullptr;         // scalar preconditioner
+  void* blk = nullptr;         // block-Jacobi inverses (nv x nv per cell)
+  double* z = nullptr;         // preconditioned direction (FP64)
+  double* w = nullptr;         // Jv output (FP64)
+  double* h = nullptr;         // device copy of a Hessenberg column (+ norm)
+  double* hpin = nullptr;      // pinned host mirror of h
+  size_t bytes = 0;
+};
+
+}  // namespace cfd
+
+struct cfd_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  cfd_params P{};
+  cfd::Phys phys{};
+  int N = 0, F = 0, pitch = 0, ncolors = 0;
+  std::vector<int> color_ptr;
+  std::vector<int> face_order, perm, iperm;  // host copies
+  cfd::DMesh dm{};
+  int assembly = 0;  // 0 colored scatter, 1 face buffer + gather
+  // device mesh storage
+  double *d_vol = nullptr, *d_xc = nullptr, *d_yc = nullptr;
+  int4* d_slots = nullptr;
+  int *d_own = nullptr, *d_nei = nullptr, *d_tag = nullptr, *d_iperm = nullptr, *d_facemap = nullptr;
+  double *d_nx = nullptr, *d_ny = nullptr, *d_area = nullptr, *d_xm = nullptr, *d_ym = nullptr;
+  unsigned char* d_alt = nullptr;
+  // solver state
+  double* grad = nullptr;   // 10 x pitch
+  double* phi = nullptr;    // pitch
+  double* dt = nullptr;     // pitch
+  double* Fm = nullptr;     // nv x F face buffer (assembly 1)
+  double* red = nullptr;    // reduction partials
+  double* scal = nullptr;   // device scalars (norms, eps, ...)
+  double* scal_host = nullptr;  // pinned
+  double* gradT = nullptr;  // tangent gradients (10 x pitch, FP64 or FP32 dual parts)
+  bool phi_valid = false, dt_valid = false;
+  // host-pointer staging: AoS device copy + SoA vectors
+  double* io6 = nullptr;     // 6 x N (AoS)
+  double* vec[6] = {};       // 6 SoA vectors of nv x pitch
+  cfd::Krylov kry;
+  std::string err;
+  int nv() const { return P.nvar; }
+  size_t vbytes() const { return (size_t)P.nvar * pitch * sizeof(double); }
+};
+
+namespace cfd {
+// ---- launchers (kernels.hip) -------------------------------------------------
+// All launch on ctx->stream.  Vectors are SoA (nv x pitch, RCM order).
+void launch_residual(cfd_ctx* c, const double* U, double* R, bool update_limiters);
+// same, recording ev[0..4] around: gradients | fill-or-nothing | flux | gather-or-nothing
+void launch_residual_timed(cfd_ctx* c, const double* U, double* R, bool update_limiters, hipEvent_t* ev);
+void launch_pack(cfd_ctx* c, const double* U6, double* U);
+void launch_unpack(cfd_ctx* c, const double* U, double* U6);
+void launch_timestep(cfd_ctx* c, const double* U, double cfl);
+void launch_diag_precond(cfd_ctx* c, const double* U, double* D);
+// reductions: result written to c->scal[slot] (device)
+enum RedKind { RED_L2 = 0, RED_DOT = 1, RED_VW = 2 };
+void launch_reduce(cfd_ctx* c, int kind, const double* a, const double* b, int slot);
+void launch_fill(cfd_ctx* c, double* x, double v, size_t n);
+// FD Jv: Jv = (R(U + eps v) - Rb) / eps + (vol/dt) v, eps from scal[su] (|U|^2) and scal[sv] (|v|^2)
+void launch_jv_fd(cfd_ctx* c, const double* U, const double* Rb, const double* v, double* Jv, int su, int sv,
+                  double* tmpU, double* tmpR);
+// forward tangent (limiter frozen): Jv = dR/dU v + (vol/dt) v; prec 64 or 32; Jv is double or float
+void launch_jv_tangent(cfd_ctx* c, const double* U, const void* v, void* Jv, int prec);
+// reverse: JTw = (dR/dU)^T w + (vol/dt) w
+void launch_vjp(cfd_ctx* c, const double* U, const double* w, double* JTw);
+// vector kernels
+void launch_axpby(cfd_ctx* c, double* y, double a, const double* x, double b, size_t n);
+void launch_scale_by_scal(cfd_ctx* c, double* y, const double* x, int slot, int mode);
+void launch_newton_update(cfd_ctx* c, double* U, const double* dU, double relax);
+void launch_explicit(cfd_ctx* c, double* Unew, const double* U0, double a0, const double* U1, double a1,
+                     const double* R, double dtc);
+void launch_wall(cfd_ctx* c, const double* U, double* out3);
+// Krylov (precision-generic basis)
+void launch_krylov_init(cfd_ctx* c, void* V0, const double* rhs, double sign, int slot, int prec);
+void launch_precond_apply(cfd_ctx* c, double* z, const void* Vj, int prec, int slot_norm);
+void launch_orth(cfd_ctx* c, void* V, int j, double* w, int prec, double* hdev);
+void launch_store_basis(cfd_ctx* c, void* Vj1, const double* w, const double* hdev, int j, int prec);
+void launch_solution(cfd_ctx* c, double* x, const void* V, const double* y, int k, int prec);
+}  // namespace cfd

@@ -1,0 +1,881 @@
+// HIP kernels (gfx950) of the residual + JFNK-GMRES path and their launchers.
+//
+// Residual evaluation (Solver::computeResiduals, Solver.cpp:529-801) runs as
+//   k_gradlim  one thread per cell: Green-Gauss gradients (Solver.cpp:81-149)
+//              and the Barth-Jespersen limiter (Solver.cpp:152-253), gathered
+//              over the cell's faces in reference face order, so the sums are
+//              formed in the reference's order;
+//   k_flux     one thread per face: MUSCL reconstruction, Riemann flux,
+//              wall pressure flux, viscous flux.  Assembly mode 0 launches it
+//              once per edge color and scatters +-F*A straight into R (no two
+//              faces of a color share a cell, so no atomics); mode 1 writes
+//              F*A to a face buffer and
+//   k_gather   sums it per cell in reference face order.
+// Every block index goes through xcd_block(), which gives each of the eight
+// XCDs a contiguous range of cells / faces so neighbouring gathers hit its L2.
+#include "ctx.hpp"
+
+namespace cfd {
+
+constexpr int BS = 256;
+
+// Blocks are dispatched round-robin over the 8 XCDs: remap so XCD x walks a
+// contiguous run of logical blocks.
+__device__ __forceinline__ int xcd_block() {
+  int b = blockIdx.x, nb = gridDim.x;
+  int q = nb >> 3, r = nb & 7, x = b & 7;
+  return x * q + min(x, r) + (b >> 3);
+}
+
+inline int nblocks(size_t n) { return (int)((n + BS - 1) / BS); }
+
+template <int NV, class T>
+__device__ __forceinline__ S6<double> ld(const T* __restrict__ U, int p, int c) {
+  S6<double> s;
+  s[0] = (double)U[c];
+  s[1] = (double)U[p + c];
+  s[2] = (double)U[2 * p + c];
+  s[3] = (double)U[3 * p + c];
+  if (NV == 6) {
+    s[4] = (double)U[4 * p + c];
+    s[5] = (double)U[5 * p + c];
+  }
+  return s;
+}
+
+// ---- gradients + limiter ------------------------------------------------------
+template <int NV>
+__global__ __launch_bounds__(BS) void k_gradlim(DMesh m, Phys P, const double* __restrict__ U,
+                                                 double* __restrict__ G, double* __restrict__ phi, int do_lim) {
+  const int c = xcd_block() * BS + threadIdx.x;
+  if (c >= m.N) return;
+  const int p = m.pitch;
+  const double Rg = P.R;
+  S6<double> Uc = ld<NV>(U, p, c);
+  const double uc = Uc[1] / Uc[0], vc = Uc[2] / Uc[0], pc = pressure(P, Uc), Tc = pc / (Uc[0] * Rg);
+  double g[10];
+#pragma unroll
+  for (int q = 0; q < 10; ++q) g[q] = 0.0;
+  double pmax = pc, pmin = pc;
+  const int4 s4 = m.slots[c];
+  const int sl[4] = {s4.x, s4.y, s4.z, s4.w};
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    const int e = sl[k];
+    if (e < 0) continue;
+    const int f = e >> 1;
+    const bool as_nei = e & 1;
+    const int other = as_nei ? m.own[f] : m.nei[f];
+    const double nx = m.nx[f], ny = m.ny[f], A = m.area[f];
+    S6<double> Uo = other >= 0 ? ld<NV>(U, p, other) : ghost(P, Uc, m.tag[f], nx, ny);
+    const double uo = Uo[1] / Uo[0], vo = Uo[2] / Uo[0], po = pressure(P, Uo), To = po / (Uo[0] * Rg);
+    const double fv[5] = {0.5 * (Uc[0] + Uo[0]), 0.5 * (uc + uo), 0.5 * (vc + vo), 0.5 * (pc + po),
+                          0.5 * (Tc + To)};
+    if (!as_nei) {
+#pragma unroll
+      for (int q = 0; q < 5; ++q) {
+        g[2 * q] += fv[q] * nx * A;
+        g[2 * q + 1] += fv[q] * ny * A;
+      }
+    } else {
+#pragma unroll
+      for (int q = 0; q < 5; ++q) {
+        g[2 * q] -= fv[q] * nx * A;
+        g[2 * q + 1] -= fv[q] * ny * A;
+      }
+    }
+    pmax = mx(pmax, po);
+    pmin = mn(pmin, po);
+  }
+  const double vol = m.vol[c];
+#pragma unroll
+  for (int q = 0; q < 10; ++q) {
+    g[q] /= vol;
+    G[q * p + c] = g[q];
+  }
+  if (!do_lim) return;
+  const double xc = m.xc[c], yc = m.yc[c];
+  double ph = 1.0;
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    const int e = sl[k];
+    if (e < 0) continue;
+    const int f = e >> 1;
+    const double dx = m.xm[f] - xc, dy = m.ym[f] - yc;
+    const double Tf = Tc + 1.0 * (g[8] * dx + g[9] * dy);
+    const double Pf = Uc[0] * Rg * Tf;
+    const double d = Pf - pc;
+    double a = 1.0;
+    if (d > 0.0) a = mn(1.0, (pmax - pc) / (d + 1e-12));
+    else if (d < 0.0) a = mn(1.0, (pmin - pc) / (d - 1e-12));
+    ph = mn(ph, a);
+  }
+  phi[c] = ph;
+}
+
+// ---- face flux ------------------------------------------------------------------------
+struct Grad8 {
+  double v[10];
+};
+
+template <int NV, int VISC>
+__device__ __forceinline__ S6<double> face_flux(const DMesh& m, const Phys& P, const double* __restrict__ U,
+                                                const double* __restrict__ G, const double* __restrict__ phi,
+                                                int f, int& o, int& n) {
+  const int p = m.pitch;
+  o = m.own[f];
+  n = m.nei[f];
+  const int tg = m.tag[f];
+  const double nx = m.nx[f], ny = m.ny[f];
+  const S6<double> L = ld<NV>(U, p, o);
+  const S6<double> R = n >= 0 ? ld<NV>(U, p, n) : ghost(P, L, tg, nx, ny);
+  S6<double> Lr = L, Rr = R;
+  const bool second = P.spatial != CFD_FIRST_ORDER && n >= 0;
+  double gL[10], gR[10];
+  if (second || VISC) {
+#pragma unroll
+    for (int q = 0; q < 10; ++q) gL[q] = G[q * p + o];
+    if (n >= 0) {
+#pragma unroll
+      for (int q = 0; q < 10; ++q) gR[q] = G[q * p + n];
+    } else {
+#pragma unroll
+      for (int q = 0; q < 10; ++q) gR[q] = gL[q];
+    }
+  }
+  double xco = 0, yco = 0, xcn = 0, ycn = 0;
+  if (second || VISC) {
+    xco = m.xc[o];
+    yco = m.yc[o];
+    if (n >= 0) {
+      xcn = m.xc[n];
+      ycn = m.yc[n];
+    }
+  }
+  if (second) {
+    const double xm = m.xm[f], ym = m.ym[f];
+    const double dxL = xm - xco, dyL = ym - yco, dxR = xm - xcn, dyR = ym - ycn;
+    const bool lim = P.spatial == CFD_SECOND_ORDER_LIMITED;
+    const double phL = lim ? phi[o] : 0.5, phR = lim ? phi[n] : 0.5;
+    const double uL = L[1] / L[0], vL = L[2] / L[0], pL = pressure(P, L);
+    double rLf = L[0] + phL * (gL[0] * dxL + gL[1] * dyL);
+    double pLf = pL + phL * (gL[6] * dxL + gL[7] * dyL);
+    const double uLf = uL + phL * (gL[2] * dxL + gL[3] * dyL);
+    const double vLf = vL + phL * (gL[4] * dxL + gL[5] * dyL);
+    if (rLf < 1e-4) rLf = L[0];
+    if (pLf < 1e-4) pLf = pL;
+    Lr = prim2cons(P, rLf, uLf, vLf, pLf);
+    const double uR = R[1] / R[0], vR = R[2] / R[0], pR = pressure(P, R);
+    double rRf = R[0] + phR * (gR[0] * dxR + gR[1] * dyR);
+    double pRf = pR + phR * (gR[6] * dxR + gR[7] * dyR);
+    const double uRf = uR + phR * (gR[2] * dxR + gR[3] * dyR);
+    const double vRf = vR + phR * (gR[4] * dxR + gR[5] * dyR);
+    if (rRf < 1e-4) rRf = R[0];
+    if (pRf < 1e-4) pRf = pR;
+    Rr = prim2cons(P, rRf, uRf, vRf, pRf);
+  }
+  S6<double> Fl;
+  if (n < 0 && (tg == 3 || tg == 4)) {
+    const double pw_ = pressure(P, Lr);
+    Fl[0] = 0.0;
+    Fl[1] = pw_ * nx;
+    Fl[2] = pw_ * ny;
+    Fl[3] = 0.0;
+  } else {
+    const int scheme = (m.alt && m.alt[f]) ? P.flux_alt : P.flux;
+    Fl = riemann(P, Lr, Rr, nx, ny, scheme);
+  }
+  if (VISC) {
+    const double rf = 0.5 * (L[0] + R[0]);
+    const double uf = 0.5 * (L[1] / L[0] + R[1] / R[0]);
+    const double vf = 0.5 * (L[2] / L[0] + R[2] / R[0]);
+    double ux, uy, vx, vy, Tx, Ty;
+    if (n < 0 && tg == 4) {
+      const double dx = m.xm[f] - xco, dy = m.ym[f] - yco;
+      const double d2 = dx * dx + dy * dy;
+      const double uc = L[1] / L[0], vc = L[2] / L[0];
+      ux = -uc * dx / d2;
+      uy = -uc * dy / d2;
+      vx = -vc * dx / d2;
+      vy = -vc * dy / d2;
+      Tx = 0.0;
+      Ty = 0.0;
+    } else {
+      ux = 0.5 * (gL[2] + gR[2]);
+      uy = 0.5 * (gL[3] + gR[3]);
+      vx = 0.5 * (gL[4] + gR[4]);
+      vy = 0.5 * (gL[5] + gR[5]);
+      Tx = 0.5 * (gL[8] + gR[8]);
+      Ty = 0.5 * (gL[9] + gR[9]);
+      if (n >= 0) {
+        const double dxLR = xcn - xco, dyLR = ycn - yco;
+        const double d2 = dxLR * dxLR + dyLR * dyLR;
+        if (d2 > 1e-14) {
+          const double du = (R[1] / R[0]) - (L[1] / L[0]);
+          const double dv = (R[2] / R[0]) - (L[2] / L[0]);
+          const double ux0 = ux, uy0 = uy, vx0 = vx, vy0 = vy;
+          ux = ux0 - (ux0 * dxLR + uy0 * dyLR - du) * dxLR / d2;
+          uy = uy0 - (ux0 * dxLR + uy0 * dyLR - du) * dyLR / d2;
+          vx = vx0 - (vx0 * dxLR + vy0 * dyLR - dv) * dxLR / d2;
+          vy = vy0 - (vx0 * dxLR + vy0 * dyLR - dv) * dyLR / d2;
+        }
+      }
+    }
+    const double mu = rf * P.nu;
+    const double kc = mu * P.Cp / P.Pr;
+    const double div = ux + vy;
+    const double txx = mu * (2.0 * ux - (2.0 / 3.0) * div);
+    const double tyy = mu * (2.0 * vy - (2.0 / 3.0) * div);
+    const double txy = mu * (uy + vx);
+    const double qx = -kc * Tx, qy = -kc * Ty;
+    const double wx = uf * txx + vf * txy - qx;
+    const double wy = uf * txy + vf * tyy - qy;
+    Fl[0] = Fl[0] - 0.0;
+    Fl[1] = Fl[1] - (txx * nx + txy * ny);
+    Fl[2] = Fl[2] - (txy * nx + tyy * ny);
+    Fl[3] = Fl[3] - (wx * nx + wy * ny);
+    Fl[4] = Fl[4] - 0.0;
+    Fl[5] = Fl[5] - 0.0;
+  }
+  const double A = m.area[f];
+  return scale(Fl, A);
+}
+
+// MODE 0: scatter into R for faces [f0, f1) of one color.  MODE 1: face buffer.
+template <int NV, int VISC, int MODE>
+__global__ __launch_bounds__(BS) void k_flux(DMesh m, Phys P, const double* __restrict__ U,
+                                              const double* __restrict__ G, const double* __restrict__ phi,
+                                              double* __restrict__ out, int f0, int f1) {
+  const int f = f0 + xcd_block() * BS + threadIdx.x;
+  if (f >= f1) return;
+  int o, n;
+  const S6<double> TA = face_flux<NV, VISC>(m, P, U, G, phi, f, o, n);
+  if (MODE == 0) {
+    const int p = m.pitch;
+#pragma unroll
+    for (int k = 0; k < NV; ++k) out[k * p + o] = out[k * p + o] + TA[k];
+    if (n >= 0) {
+#pragma unroll
+      for (int k = 0; k < NV; ++k) out[k * p + n] = out[k * p + n] - TA[k];
+    }
+  } else {
+#pragma unroll
+    for (int k = 0; k < NV; ++k) out[(size_t)k * m.F + f] = TA[k];
+  }
+}
+
+template <int NV>
+__global__ __launch_bounds__(BS) void k_gather(DMesh m, const double* __restrict__ Fm, double* __restrict__ R) {
+  const int c = xcd_block() * BS + threadIdx.x;
+  if (c >= m.N) return;
+  const int4 s4 = m.slots[c];
+  const int sl[4] = {s4.x, s4.y, s4.z, s4.w};
+  double r[NV];
+#pragma unroll
+  for (int k = 0; k < NV; ++k) r[k] = 0.0;
+#pragma unroll
+  for (int j = 0; j < 4; ++j) {
+    const int e = sl[j];
+    if (e < 0) continue;
+    const int f = e >> 1;
+    if (e & 1) {
+#pragma unroll
+      for (int k = 0; k < NV; ++k) r[k] = r[k] - Fm[(size_t)k * m.F + f];
+    } else {
+#pragma unroll
+      for (int k = 0; k < NV; ++k) r[k] = r[k] + Fm[(size_t)k * m.F + f];
+    }
+  }
+#pragma unroll
+  for (int k = 0; k < NV; ++k) R[k * m.pitch + c] = r[k];
+}
+
+__global__ void k_fill(double* __restrict__ x, double v, size_t n) {
+  size_t i = (size_t)blockIdx.x * BS + threadIdx.x;
+  if (i < n) x[i] = v;
+}
+
+// ---- local time step (Solver.cpp:1388-1422) ----------------------------------------
+template <int NV>
+__global__ __launch_bounds__(BS) void k_timestep(DMesh m, Phys P, const double* __restrict__ U, double cfl,
+                                                  double* __restrict__ dt) {
+  const int c = xcd_block() * BS + threadIdx.x;
+  if (c >= m.N) return;
+  const S6<double> Uc = ld<NV>(U, m.pitch, c);
+  const double pc = pressure(P, Uc);
+  const double cc = sqrt(P.gamma * pc / Uc[0]);
+  double lam = 1e-12;
+  const int4 s4 = m.slots[c];
+  const int sl[4] = {s4.x, s4.y, s4.z, s4.w};
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    const int e = sl[k];
+    if (e < 0) continue;
+    const int f = e >> 1;
+    double nx = m.nx[f], ny = m.ny[f];
+    if (e & 1) {
+      nx = -nx;
+      ny = -ny;
+    }
+    const double un = (Uc[1] * nx + Uc[2] * ny) / Uc[0];
+    lam += (fabs(un) + cc) * m.area[f];
+  }
+  dt[c] = cfl * m.vol[c] / lam;
+}
+
+// ---- scalar diagonal preconditioner (Solver.cpp:1424-1465) --------------------------
+template <int NV>
+__global__ __launch_bounds__(BS) void k_diagprec(DMesh m, Phys P, const double* __restrict__ U,
+                                                  const double* __restrict__ dt, double* __restrict__ D) {
+  const int c = xcd_block() * BS + threadIdx.x;
+  if (c >= m.N) return;
+  const int p = m.pitch;
+  const double g = P.gamma;
+  double acc = 0.0;
+  const int4 s4 = m.slots[c];
+  const int sl[4] = {s4.x, s4.y, s4.z, s4.w};
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    const int e = sl[k];
+    if (e < 0) continue;
+    const int f = e >> 1;
+    const int o = m.own[f], n = m.nei[f];
+    const double nx = m.nx[f], ny = m.ny[f];
+    const S6<double> L = ld<NV>(U, p, o);
+    const S6<double> R = n >= 0 ? ld<NV>(U, p, n) : ghost(P, L, m.tag[f], nx, ny);
+    const double cL = sqrt(g * pressure(P, L) / L[0]);
+    const double pRr = pressure(P, R);
+    const double cR = sqrt(g * (pRr > 0 ? pRr : 1e-3) / (R[0] > 0 ? R[0] : 1e-3));
+    const double cf = 0.5 * (cL + cR);
+    const double uf = 0.5 * (L[1] / L[0] + R[1] / R[0]);
+    const double vf = 0.5 * (L[2] / L[0] + R[2] / R[0]);
+    const double vn = fabs(uf * nx + vf * ny);
+    const double li = vn + cf;
+    const double dx = sqrt(m.vol[o]);
+    const double mu = 0.5 * (L[0] + R[0]) * P.nu;
+    const double lv = 2.0 * mu / (0.5 * (L[0] + R[0]) * dx + 1e-12);
+    const double lt = (li + lv) * m.area[f];
+    acc += lt * 0.5;
+  }
+  double d = (m.vol[c] / (dt[c] + 1e-30)) + acc;
+  if (d < 1e-12) d = 1e-12;
+  D[c] = d;
+}
+
+// ---- reductions (deterministic: fixed grid, fixed tree) ------------------------------
+constexpr int RED_BLOCKS = 1024;
+
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+  return v;
+}
+
+// block-wide sum; result valid in thread 0
+__device__ __forceinline__ double block_sum(double v, double* sh) {
+  v = wave_sum(v);
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  __syncthreads();
+  if (lane == 0) sh[w] = v;
+  __syncthreads();
+  double r = 0.0;
+  if (threadIdx.x == 0) {
+    for (int i = 0; i < BS / 64; ++i) r += sh[i];
+  }
+  return r;
+}
+
+template <int NV, int KIND, class TA, class TB>
+__global__ __launch_bounds__(BS) void k_red(int N, int p, const TA* __restrict__ a, const TB* __restrict__ b,
+                                             const double* __restrict__ vol, double* __restrict__ part) {
+  __shared__ double sh[BS / 64];
+  double s0 = 0.0, s1 = 0.0;
+  for (int c = blockIdx.x * BS + threadIdx.x; c < N; c += gridDim.x * BS) {
+    if (KIND == RED_L2) {
+      double t = 0.0;
+#pragma unroll
+      for (int k = 0; k < NV; ++k) {
+        const double x = (double)a[k * p + c];
+        t += x * x;
+      }
+      s0 += t;
+    } else if (KIND == RED_DOT) {
+      double t = 0.0;
+#pragma unroll
+      for (int k = 0; k < NV; ++k) t += (double)a[k * p + c] * (double)b[k * p + c];
+      s0 += t;
+    } else {
+      const double v = vol[c];
+      double t = 0.0;
+#pragma unroll
+      for (int k = 0; k < NV; ++k) {
+        const double r = (double)a[k * p + c] / v;
+        t += r * r;
+      }
+      s0 += t * v;
+      s1 += v;
+    }
+  }
+  s0 = block_sum(s0, sh);
+  if (threadIdx.x == 0) part[blockIdx.x] = s0;
+  if (KIND == RED_VW) {
+    s1 = block_sum(s1, sh);
+    if (threadIdx.x == 0) part[gridDim.x + blockIdx.x] = s1;
+  }
+}
+
+// sums nsum groups of nb partials; group i -> out[i]
+__global__ __launch_bounds__(BS) void k_red_final(const double* __restrict__ part, int nb, int nsum,
+                                                   double* __restrict__ out) {
+  __shared__ double sh[BS / 64];
+  for (int i = 0; i < nsum; ++i) {
+    double s = 0.0;
+    for (int k = threadIdx.x; k < nb; k += BS) s += part[(size_t)i * nb + k];
+    s = block_sum(s, sh);
+    if (threadIdx.x == 0) out[i] = s;
+  }
+}
+
+// ---- FD Jacobian-vector product (Solver.cpp:1331-1386) -------------------------------
+__device__ __forceinline__ double fd_eps(const double* scal, int su, int sv) {
+  return 1e-7 * (1.0 + sqrt(scal[su])) / (sqrt(scal[sv]) + 1e-30);
+}
+
+template <int NV>
+__global__ __launch_bounds__(BS) void k_perturb(int N, int p, const double* __restrict__ U,
+                                                 const double* __restrict__ v, const double* __restrict__ scal,
+                                                 int su, int sv, double* __restrict__ Up) {
+  const int c = xcd_block() * BS + threadIdx.x;
+  if (c >= N) return;
+  const double eps = fd_eps(scal, su, sv);
+#pragma unroll
+  for (int k = 0; k < NV; ++k) Up[k * p + c] = U[k * p + c] + v[k * p + c] * eps;
+}
+
+template <int NV>
+__global__ __launch_bounds__(BS) void k_fd_combine(int N, int p, const double* __restrict__ Rp,
+                                                    const double* __restrict__ Rb, const double* __restrict__ v,
+                                                    const double* __restrict__ vol, const double* __restrict__ dt,
+                                                    const double* __restrict__ scal, int su, int sv,
+                                                    double* __restrict__ Jv) {
+  const int c = xcd_block() * BS + threadIdx.x;
+  if (c >= N) return;
+  const double eps = fd_eps(scal, su, sv);
+  const double tc = vol[c] / (dt[c] + 1e-30);
+#pragma unroll
+  for (int k = 0; k < NV; ++k) Jv[k * p + c] = (Rp[k * p + c] - Rb[k * p + c]) * (1.0 / eps) + v[k * p + c] * tc;
+}
+
+// ---- pack / unpack between reference-order AoS and RCM-order SoA ---------------------
+template <int NV>
+__global__ __launch_bounds__(BS) void k_pack(int N, int p, const int* __restrict__ iperm,
+                                              const double* __restrict__ U6, double* __restrict__ U) {
+  const int i = blockIdx.x * BS + threadIdx.x;
+  if (i >= N) return;
+  const size_t o = (size_t)iperm[i] * 6;
+#pragma unroll
+  for (int k = 0; k < NV; ++k) U[k * p + i] = U6[o + k];
+}
+template <int NV>
+__global__ __launch_bounds__(BS) void k_unpack(int N, int p, const int* __restrict__ iperm,
+                                                const double* __restrict__ U, double* __restrict__ U6) {
+  const int i = blockIdx.x * BS + threadIdx.x;
+  if (i >= N) return;
+  const size_t o = (size_t)iperm[i] * 6;
+#pragma unroll
+  for (int k = 0; k < 6; ++k) U6[o + k] = k < NV ? U[k * p + i] : 0.0;
+}
+
+// ---- vector kernels ----------------------------------------------------------------------
+__global__ void k_axpby(double* __restrict__ y, double a, const double* __restrict__ x, double b, size_t n) {
+  size_t i = (size_t)blockIdx.x * BS + threadIdx.x;
+  if (i < n) y[i] = a * y[i] + b * x[i];
+}
+
+// Newton update with the positivity clips of Solver.cpp:1760-1778
+template <int NV>
+__global__ __launch_bounds__(BS) void k_newton_update(int N, int p, Phys P, double* __restrict__ U,
+                                                       const double* __restrict__ dU, double relax) {
+  const int c = blockIdx.x * BS + threadIdx.x;
+  if (c >= N) return;
+  double u[NV];
+#pragma unroll
+  for (int k = 0; k < NV; ++k) u[k] = U[k * p + c] + dU[k * p + c] * relax;
+  const double ke = 0.5 * (u[1] * u[1] + u[2] * u[2]) / (u[0] + 1e-12);
+  double pn = (P.gamma - 1.0) * (u[3] - ke);
+  if (u[0] < 1e-5) u[0] = 1e-5;
+  if (pn < 1e-5) {
+    pn = 1e-5;
+    u[3] = pn / (P.gamma - 1.0) + ke;
+  }
+#pragma unroll
+  for (int k = 0; k < NV; ++k) U[k * p + c] = u[k];
+}
+
+// Explicit stage: Unew = a0 U0 + a1 (U1 - R dt / vol), with the clips of
+// Solver.cpp:1233-1241.  a0 == 0 gives Solver::timeStepExplicit.
+template <int NV>
+__global__ __launch_bounds__(BS) void k_explicit(int N, int p, const double* __restrict__ vol,
+                                                  double* __restrict__ Unew, const double* __restrict__ U0, double a0,
+                                                  const double* __restrict__ U1, double a1,
+                                                  const double* __restrict__ R, double dtc) {
+  const int c = blockIdx.x * BS + threadIdx.x;
+  if (c >= N) return;
+  const double f = dtc / vol[c];
+  double u[NV];
+#pragma unroll
+  for (int k = 0; k < NV; ++k) {
+    const double e = U1[k * p + c] - R[k * p + c] * f;
+    u[k] = a0 == 0.0 ? e : U0[k * p + c] * a0 + e * a1;
+  }
+  if (u[0] < 1e-3) u[0] = 1e-3;
+  const double ke = 0.5 * (u[1] * u[1] + u[2] * u[2]) / u[0];
+  if (u[3] - ke < 1e-3) u[3] = ke + 1e-3;
+#pragma unroll
+  for (int k = 0; k < NV; ++k) Unew[k * p + c] = u[k];
+}
+
+// Wall outputs per device face: Cp on tags 3/4, Cf on tag 4
+// (Solver.cpp:1919-1967), p A on tag 3.
+template <int NV>
+__global__ __launch_bounds__(BS) void k_wall(DMesh m, Phys P, const double* __restrict__ U, double qinf,
+                                              double* __restrict__ out3) {
+  const int f = blockIdx.x * BS + threadIdx.x;
+  if (f >= m.F) return;
+  double cp = 0.0, cf = 0.0, jp = 0.0;
+  const int tg = m.tag[f];
+  if (m.nei[f] < 0 && (tg == 3 || tg == 4)) {
+    const int o = m.own[f];
+    const S6<double> L = ld<NV>(U, m.pitch, o);
+    const double pr = pressure(P, L);
+    cp = (pr - P.pexit) / qinf;
+    if (tg == 3) jp = pr * m.area[f];
+    if (tg == 4) {
+      const double uc = L[1] / L[0], vc = L[2] / L[0];
+      double tx = -m.ny[f], ty = m.nx[f];
+      if (tx < 0) {
+        tx = -tx;
+        ty = -ty;
+      }
+      const double ut = uc * tx + vc * ty;
+      const double dx = m.xm[f] - m.xc[o], dy = m.ym[f] - m.yc[o];
+      double dn = fabs(dx * m.nx[f] + dy * m.ny[f]);
+      if (dn < 1e-12) dn = 1e-12;
+      const double mu = L[0] * P.nu;
+      cf = mu * (ut / dn) / qinf;
+    }
+  }
+  out3[f] = cp;
+  out3[m.F + f] = cf;
+  out3[2 * m.F + f] = jp;
+}
+
+// ---- Krylov vector kernels (basis element type TV: double or float) -------------------
+template <int NV, class TV>
+__global__ __launch_bounds__(BS) void k_kinit(int N, int p, TV* __restrict__ V0, const double* __restrict__ r,
+                                               double sign, const double* __restrict__ scal, int slot) {
+  const int c = blockIdx.x * BS + threadIdx.x;
+  if (c >= N) return;
+  const double ib = 1.0 / sqrt(scal[slot]);
+#pragma unroll
+  for (int k = 0; k < NV; ++k) V0[k * p + c] = (TV)(sign * (r[k * p + c] * ib));
+}
+
+template <int NV, class TV, class TZ>
+__global__ __launch_bounds__(BS) void k_precond(int N, int p, TZ* __restrict__ z, const TV* __restrict__ Vj,
+                                                 const double* __restrict__ D) {
+  const int c = blockIdx.x * BS + threadIdx.x;
+  if (c >= N) return;
+  const double id = 1.0 / D[c];
+#pragma unroll
+  for (int k = 0; k < NV; ++k) z[k * p + c] = (TZ)((double)Vj[k * p + c] * id);
+}
+
+// partial dots of w against up to 8 basis vectors (chunk = blockIdx.y)
+template <int NV, class TV>
+__global__ __launch_bounds__(BS) void k_mdot(int N, int p, size_t vstride, const TV* __restrict__ V, int nvec,
+                                              const TV* __restrict__ w, double* __restrict__ part) {
+  __shared__ double sh[BS / 64];
+  const int i0 = blockIdx.y * 8;
+  const int cnt = min(8, nvec - i0);
+  double acc[8];
+#pragma unroll
+  for (int i = 0; i < 8; ++i) acc[i] = 0.0;
+  for (int c = blockIdx.x * BS + threadIdx.x; c < N; c += gridDim.x * BS) {
+    double wv[NV];
+#pragma unroll
+    for (int k = 0; k < NV; ++k) wv[k] = (double)w[k * p + c];
+#pragma unroll
+    for (int i = 0; i < 8; ++i) {
+      if (i < cnt) {
+        const TV* Vi = V + (size_t)(i0 + i) * vstride;
+        double t = 0.0;
+#pragma unroll
+        for (int k = 0; k < NV; ++k) t += wv[k] * (double)Vi[k * p + c];
+        acc[i] += t;
+      }
+    }
+  }
+  for (int i = 0; i < cnt; ++i) {
+    const double s = block_sum(acc[i], sh);
+    if (threadIdx.x == 0) part[(size_t)(i0 + i) * gridDim.x + blockIdx.x] = s;
+  }
+}
+
+// w -= sum_i h[i] V_i ; hacc[i] += h[i] (second pass of CGS2 accumulates)
+template <int NV, class TV>
+__global__ __launch_bounds__(BS) void k_maxpy(int N, int p, size_t vstride, const TV* __restrict__ V, int nvec,
+                                               TV* __restrict__ w, const double* __restrict__ h) {
+  const int c = xcd_block() * BS + threadIdx.x;
+  if (c >= N) return;
+#pragma unroll
+  for (int k = 0; k < NV; ++k) {
+    double s = (double)w[k * p + c];
+    for (int i = 0; i < nvec; ++i) s -= h[i] * (double)V[(size_t)i * vstride + k * p + c];
+    w[k * p + c] = (TV)s;
+  }
+}
+
+__global__ void k_hacc(double* __restrict__ h, const double* __restrict__ h2, int n, double* __restrict__ nrm2,
+                       double* __restrict__ nrm_out) {
+  const int i = threadIdx.x;
+  if (i < n) h[i] += h2[i];
+  if (i == 0) *nrm_out = sqrt(*nrm2);
+}
+
+template <int NV, class TV>
+__global__ __launch_bounds__(BS) void k_normalize(int N, int p, TV* __restrict__ w, const double* __restrict__ nrm) {
+  const int c = blockIdx.x * BS + threadIdx.x;
+  if (c >= N) return;
+  const double h = *nrm;
+  if (!(h > 1e-12)) return;
+  const double ih = 1.0 / h;
+#pragma unroll
+  for (int k = 0; k < NV; ++k) w[k * p + c] = (TV)((double)w[k * p + c] * ih);
+}
+
+// x = sum_{i<=k} y_i V_i / D, accumulated in i order (Solver.cpp:1610-1615)
+template <int NV, class TV>
+__global__ __launch_bounds__(BS) void k_solution(int N, int p, size_t vstride, const TV* __restrict__ V,
+                                                  const double* __restrict__ y, int nvec,
+                                                  const double* __restrict__ D, double* __restrict__ x) {
+  const int c = xcd_block() * BS + threadIdx.x;
+  if (c >= N) return;
+  const double id = 1.0 / D[c];
+#pragma unroll
+  for (int k = 0; k < NV; ++k) {
+    double s = 0.0;
+    for (int i = 0; i < nvec; ++i) s = s + (double)V[(size_t)i * vstride + k * p + c] * y[i] * id;
+    x[k * p + c] = s;
+  }
+}
+
+// ---- launchers --------------------------------------------------------------------------------
+#define NV_DISPATCH(nv, ...) \
+  do {                       \
+    if ((nv) == 6) {         \
+      constexpr int NVt = 6; \
+      __VA_ARGS__;           \
+    } else {                 \
+      constexpr int NVt = 4; \
+      __VA_ARGS__;           \
+    }                        \
+  } while (0)
+
+void launch_fill(cfd_ctx* c, double* x, double v, size_t n) {
+  if (n == 0) return;
+  hipLaunchKernelGGL(k_fill, dim3(nblocks(n)), dim3(BS), 0, c->stream, x, v, n);
+}
+
+template <int NV, int VISC>
+static void residual_nv(cfd_ctx* c, const double* U, double* R, bool upd, hipEvent_t* ev) {
+  const DMesh& m = c->dm;
+  const Phys& P = c->phys;
+  const bool lim = P.spatial == CFD_SECOND_ORDER_LIMITED;
+  const int do_lim = lim && (upd || !c->phi_valid);
+  if (ev) (void)hipEventRecord(ev[0], c->stream);
+  hipLaunchKernelGGL(k_gradlim<NV>, dim3(nblocks(m.N)), dim3(BS), 0, c->stream, m, P, U, c->grad, c->phi, do_lim);
+  if (do_lim) c->phi_valid = true;
+  if (ev) (void)hipEventRecord(ev[1], c->stream);
+  if (c->assembly == 0) {
+    launch_fill(c, R, 0.0, (size_t)NV * m.pitch);
+    if (ev) (void)hipEventRecord(ev[2], c->stream);
+    for (int col = 0; col < c->ncolors; ++col) {
+      const int f0 = c->color_ptr[col], f1 = c->color_ptr[col + 1];
+      if (f1 > f0)
+        hipLaunchKernelGGL((k_flux<NV, VISC, 0>), dim3(nblocks(f1 - f0)), dim3(BS), 0, c->stream, m, P, U, c->grad,
+                           c->phi, R, f0, f1);
+    }
+    if (ev) (void)hipEventRecord(ev[3], c->stream);
+  } else {
+    if (ev) (void)hipEventRecord(ev[2], c->stream);
+    hipLaunchKernelGGL((k_flux<NV, VISC, 1>), dim3(nblocks(m.F)), dim3(BS), 0, c->stream, m, P, U, c->grad, c->phi,
+                       c->Fm, 0, m.F);
+    if (ev) (void)hipEventRecord(ev[3], c->stream);
+    hipLaunchKernelGGL(k_gather<NV>, dim3(nblocks(m.N)), dim3(BS), 0, c->stream, m, c->Fm, R);
+  }
+  if (ev) (void)hipEventRecord(ev[4], c->stream);
+}
+
+void launch_residual_timed(cfd_ctx* c, const double* U, double* R, bool upd, hipEvent_t* ev) {
+  const bool visc = c->phys.viscous != 0;
+  if (c->P.nvar == 6) {
+    if (visc) residual_nv<6, 1>(c, U, R, upd, ev);
+    else residual_nv<6, 0>(c, U, R, upd, ev);
+  } else {
+    if (visc) residual_nv<4, 1>(c, U, R, upd, ev);
+    else residual_nv<4, 0>(c, U, R, upd, ev);
+  }
+}
+
+void launch_residual(cfd_ctx* c, const double* U, double* R, bool upd) { launch_residual_timed(c, U, R, upd, nullptr); }
+
+void launch_pack(cfd_ctx* c, const double* U6, double* U) {
+  NV_DISPATCH(c->P.nvar, hipLaunchKernelGGL(k_pack<NVt>, dim3(nblocks(c->N)), dim3(BS), 0, c->stream, c->N,
+                                            c->pitch, c->d_iperm, U6, U));
+}
+void launch_unpack(cfd_ctx* c, const double* U, double* U6) {
+  NV_DISPATCH(c->P.nvar, hipLaunchKernelGGL(k_unpack<NVt>, dim3(nblocks(c->N)), dim3(BS), 0, c->stream, c->N,
+                                            c->pitch, c->d_iperm, U, U6));
+}
+void launch_timestep(cfd_ctx* c, const double* U, double cfl) {
+  NV_DISPATCH(c->P.nvar, hipLaunchKernelGGL(k_timestep<NVt>, dim3(nblocks(c->N)), dim3(BS), 0, c->stream, c->dm,
+                                            c->phys, U, cfl, c->dt));
+  c->dt_valid = true;
+}
+void launch_diag_precond(cfd_ctx* c, const double* U, double* D) {
+  NV_DISPATCH(c->P.nvar, hipLaunchKernelGGL(k_diagprec<NVt>, dim3(nblocks(c->N)), dim3(BS), 0, c->stream, c->dm,
+                                            c->phys, U, c->dt, D));
+}
+
+static int red_grid(int N) { return std::min(RED_BLOCKS, std::max(1, nblocks(N))); }
+
+template <class TA, class TB>
+static void reduce_t(cfd_ctx* c, int kind, const TA* a, const TB* b, int slot) {
+  const int nb = red_grid(c->N);
+  NV_DISPATCH(c->P.nvar, {
+    if (kind == RED_L2)
+      hipLaunchKernelGGL((k_red<NVt, RED_L2, TA, TB>), dim3(nb), dim3(BS), 0, c->stream, c->N, c->pitch, a, b,
+                         c->d_vol, c->red);
+    else if (kind == RED_DOT)
+      hipLaunchKernelGGL((k_red<NVt, RED_DOT, TA, TB>), dim3(nb), dim3(BS), 0, c->stream, c->N, c->pitch, a, b,
+                         c->d_vol, c->red);
+    else
+      hipLaunchKernelGGL((k_red<NVt, RED_VW, TA, TB>), dim3(nb), dim3(BS), 0, c->stream, c->N, c->pitch, a, b,
+                         c->d_vol, c->red);
+  });
+  hipLaunchKernelGGL(k_red_final, dim3(1), dim3(BS), 0, c->stream, c->red, nb, kind == RED_VW ? 2 : 1,
+                     c->scal + slot);
+}
+
+void launch_reduce(cfd_ctx* c, int kind, const double* a, const double* b, int slot) {
+  reduce_t<double, double>(c, kind, a, b ? b : a, slot);
+}
+
+void launch_jv_fd(cfd_ctx* c, const double* U, const double* Rb, const double* v, double* Jv, int su, int sv,
+                  double* tmpU, double* tmpR) {
+  NV_DISPATCH(c->P.nvar, hipLaunchKernelGGL(k_perturb<NVt>, dim3(nblocks(c->N)), dim3(BS), 0, c->stream, c->N,
+                                            c->pitch, U, v, c->scal, su, sv, tmpU));
+  launch_residual(c, tmpU, tmpR, false);
+  NV_DISPATCH(c->P.nvar, hipLaunchKernelGGL(k_fd_combine<NVt>, dim3(nblocks(c->N)), dim3(BS), 0, c->stream, c->N,
+                                            c->pitch, tmpR, Rb, v, c->d_vol, c->dt, c->scal, su, sv, Jv));
+}
+
+void launch_axpby(cfd_ctx* c, double* y, double a, const double* x, double b, size_t n) {
+  if (n == 0) return;
+  hipLaunchKernelGGL(k_axpby, dim3(nblocks(n)), dim3(BS), 0, c->stream, y, a, x, b, n);
+}
+
+void launch_newton_update(cfd_ctx* c, double* U, const double* dU, double relax) {
+  NV_DISPATCH(c->P.nvar, hipLaunchKernelGGL(k_newton_update<NVt>, dim3(nblocks(c->N)), dim3(BS), 0, c->stream,
+                                            c->N, c->pitch, c->phys, U, dU, relax));
+}
+
+void launch_explicit(cfd_ctx* c, double* Unew, const double* U0, double a0, const double* U1, double a1,
+                     const double* R, double dtc) {
+  NV_DISPATCH(c->P.nvar, hipLaunchKernelGGL(k_explicit<NVt>, dim3(nblocks(c->N)), dim3(BS), 0, c->stream, c->N,
+                                            c->pitch, c->d_vol, Unew, U0, a0, U1, a1, R, dtc));
+}
+
+void launch_wall(cfd_ctx* c, const double* U, double* out3) {
+  const Phys& P = c->phys;
+  const double g = P.gamma;
+  const double fT = 1.0 + 0.5 * (g - 1.0) * c->P.M_inf * c->P.M_inf;
+  const double Ts = c->P.T_total_inlet / fT;
+  const double cs = std::sqrt(g * c->P.R_gas * Ts);
+  const double uinf = c->P.M_inf * cs;
+  const double rinf = c->P.P_static_exit / (c->P.R_gas * Ts);
+  const double qinf = 0.5 * rinf * uinf * uinf;
+  NV_DISPATCH(c->P.nvar, hipLaunchKernelGGL(k_wall<NVt>, dim3(nblocks(c->F)), dim3(BS), 0, c->stream, c->dm,
+                                            c->phys, U, qinf, out3));
+}
+
+// ---- Krylov launchers ---------------------------------------------------------------------
+template <class TV> static void kinit_t(cfd_ctx* c, TV* V0, const double* rhs, double sign, int slot) {
+  NV_DISPATCH(c->P.nvar, hipLaunchKernelGGL((k_kinit<NVt, TV>), dim3(nblocks(c->N)), dim3(BS), 0, c->stream, c->N,
+                                            c->pitch, V0, rhs, sign, c->scal, slot));
+}
+void launch_krylov_init(cfd_ctx* c, void* V0, const double* rhs, double sign, int slot, int prec) {
+  if (prec == 32) kinit_t<float>(c, (float*)V0, rhs, sign, slot);
+  else kinit_t<double>(c, (double*)V0, rhs, sign, slot);
+}
+
+template <class TV, class TZ> static void precond_t(cfd_ctx* c, TZ* z, const TV* Vj, int slot_norm) {
+  NV_DISPATCH(c->P.nvar, hipLaunchKernelGGL((k_precond<NVt, TV, TZ>), dim3(nblocks(c->N)), dim3(BS), 0, c->stream,
+                                            c->N, c->pitch, z, Vj, c->kry.D));
+  if (slot_norm >= 0) reduce_t<TZ, TZ>(c, RED_L2, z, z, slot_norm);
+}
+void launch_precond_apply(cfd_ctx* c, double* z, const void* Vj, int prec, int slot_norm) {
+  if (prec == 32) precond_t<float, float>(c, (float*)z, (const float*)Vj, slot_norm);
+  else precond_t<double, double>(c, z, (const double*)Vj, slot_norm);
+}
+
+// Classical Gram-Schmidt with one reorthogonalisation pass (CGS2), FP64
+// accumulation.  V_{j+1} is orthogonalised in place; hdev[0..j] receives the
+// Hessenberg column and hdev[j+1] the norm of the result.
+template <class TV> static void orth_t(cfd_ctx* c, TV* V, int j, double* hdev) {
+  const int N = c->N, p = c->pitch, nv = c->P.nvar;
+  const size_t vs = (size_t)nv * p;
+  TV* w = V + (size_t)(j + 1) * vs;
+  const int nvec = j + 1;
+  const int nb = std::min(256, std::max(1, nblocks(N)));
+  const int nchunk = (nvec + 7) / 8;
+  double* h2 = hdev + 64;
+  for (int pass = 0; pass < 2; ++pass) {
+    double* hout = pass == 0 ? hdev : h2;
+    NV_DISPATCH(nv, hipLaunchKernelGGL((k_mdot<NVt, TV>), dim3(nb, nchunk), dim3(BS), 0, c->stream, N, p, vs, V,
+                                       nvec, w, c->red));
+    hipLaunchKernelGGL(k_red_final, dim3(1), dim3(BS), 0, c->stream, c->red, nb, nvec, hout);
+    NV_DISPATCH(nv, hipLaunchKernelGGL((k_maxpy<NVt, TV>), dim3(nblocks(N)), dim3(BS), 0, c->stream, N, p, vs, V,
+                                       nvec, w, hout));
+  }
+  reduce_t<TV, TV>(c, RED_L2, w, w, 7);
+  hipLaunchKernelGGL(k_hacc, dim3(1), dim3(64), 0, c->stream, hdev, h2, nvec, c->scal + 7, hdev + j + 1);
+}
+void launch_orth(cfd_ctx* c, void* V, int j, double* w, int prec, double* hdev) {
+  (void)w;
+  if (prec == 32) orth_t<float>(c, (float*)V, j, hdev);
+  else orth_t<double>(c, (double*)V, j, hdev);
+}
+
+template <class TV> static void store_t(cfd_ctx* c, TV* Vj1, const double* nrm) {
+  NV_DISPATCH(c->P.nvar, hipLaunchKernelGGL((k_normalize<NVt, TV>), dim3(nblocks(c->N)), dim3(BS), 0, c->stream,
+                                            c->N, c->pitch, Vj1, nrm));
+}
+void launch_store_basis(cfd_ctx* c, void* Vj1, const double* w, const double* hdev, int j, int prec) {
+  (void)w;
+  if (prec == 32) store_t<float>(c, (float*)Vj1, hdev + j + 1);
+  else store_t<double>(c, (double*)Vj1, hdev + j + 1);
+}
+
+template <class TV> static void solution_t(cfd_ctx* c, double* x, const TV* V, const double* y, int k) {
+  const size_t vs = (size_t)c->P.nvar * c->pitch;
+  NV_DISPATCH(c->P.nvar, hipLaunchKernelGGL((k_solution<NVt, TV>), dim3(nblocks(c->N)), dim3(BS), 0, c->stream,
+                                            c->N, c->pitch, vs, V, y, k + 1, c->kry.D, x));
+}
+void launch_solution(cfd_ctx* c, double* x, const void* V, const double* y, int k, int prec) {
+  if (prec == 32) solution_t<float>(c, x, (const float*)V, y, k);
+  else solution_t<double>(c, x, (const double*)V, y, k);
+}
+
+}  // namespace cfd

@@ -1,0 +1,581 @@
+// Mesh pipeline on the host: Gmsh v2.2 reader, reference-order connectivity,
+// geometry, RCM reordering, edge coloring, and the seeded synthetic meshes of
+// the benchmark configurations.
+//
+// Connectivity and geometry reproduce Mesh::buildConnectivity
+// (Mesh.cpp:102-143) and Mesh::computeGeometry (Mesh.cpp:148-202) exactly —
+// same face order, owner/neighbor, tags and the same double arithmetic — but
+// with an open-addressing hash table instead of std::map so the 8.4M-face
+// meshes build in well under a second.
+#include "mesh.hpp"
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <fstream>
+#include <memory>
+#include <new>
+#include <numeric>
+#include <sstream>
+#include <string>
+
+namespace {
+
+// 64-bit key of an undirected edge (sorted node ids)
+inline uint64_t edge_key(int a, int b) {
+  uint32_t lo = (uint32_t)std::min(a, b), hi = (uint32_t)std::max(a, b);
+  return ((uint64_t)lo << 32) | hi;
+}
+
+// Open-addressing map uint64 -> int (insert / find), linear probing.
+struct EdgeMap {
+  std::vector<uint64_t> keys;
+  std::vector<int> vals;
+  uint64_t mask = 0;
+  static constexpr uint64_t EMPTY = ~0ull;
+  explicit EdgeMap(size_t expected) {
+    size_t cap = 16;
+    while (cap < 2 * expected + 16) cap <<= 1;
+    keys.assign(cap, EMPTY);
+    vals.assign(cap, -1);
+    mask = cap - 1;
+  }
+  static uint64_t mix(uint64_t k) {
+    k ^= k >> 33;
+    k *= 0xff51afd7ed558ccdull;
+    k ^= k >> 33;
+    return k;
+  }
+  // returns pointer to the value slot; *fresh tells whether it was inserted
+  int* slot(uint64_t k, bool* fresh) {
+    uint64_t h = mix(k) & mask;
+    while (true) {
+      if (keys[h] == k) { *fresh = false; return &vals[h]; }
+      if (keys[h] == EMPTY) { keys[h] = k; *fresh = true; return &vals[h]; }
+      h = (h + 1) & mask;
+    }
+  }
+  const int* find(uint64_t k) const {
+    uint64_t h = mix(k) & mask;
+    while (true) {
+      if (keys[h] == k) return &vals[h];
+      if (keys[h] == EMPTY) return nullptr;
+      h = (h + 1) & mask;
+    }
+  }
+};
+
+int build_connectivity(cfd_mesh& m, std::string& err) {
+  int N = m.ncells();
+  size_t nedges = m.cell_nodes.size();
+  EdgeMap bmap(m.bedges.size() / 3 + 1);
+  for (size_t e = 0; e + 2 < m.bedges.size(); e += 3) {
+    bool fresh;
+    *bmap.slot(edge_key(m.bedges[e], m.bedges[e + 1]), &fresh) = m.bedges[e + 2];  // last tag wins, like map[]=
+  }
+  EdgeMap fmap(nedges);
+  m.n1.clear(); m.n2.clear(); m.own.clear(); m.nei.clear(); m.tag.clear();
+  m.n1.reserve(nedges); m.n2.reserve(nedges); m.own.reserve(nedges); m.nei.reserve(nedges); m.tag.reserve(nedges);
+  for (int c = 0; c < N; ++c) {
+    int b = m.cell_ptr[c], n = m.cell_ptr[c + 1] - b;
+    if (n != 3 && n != 4) { err = "cell with " + std::to_string(n) + " nodes"; return CFD_ERR_MESH; }
+    for (int j = 0; j < n; ++j) {
+      int a1 = m.cell_nodes[b + j], a2 = m.cell_nodes[b + (j + 1) % n];
+      if (a1 < 1 || a1 > m.n_nodes || a2 < 1 || a2 > m.n_nodes) { err = "node id out of range"; return CFD_ERR_MESH; }
+      bool fresh;
+      int* v = fmap.slot(edge_key(a1, a2), &fresh);
+      if (!fresh) {
+        int f = *v;
+        if (m.nei[f] >= 0) { err = "edge shared by more than two cells"; return CFD_ERR_MESH; }
+        m.nei[f] = c;
+      } else {
+        int f = (int)m.own.size();
+        *v = f;
+        m.n1.push_back(std::min(a1, a2));
+        m.n2.push_back(std::max(a1, a2));
+        m.own.push_back(c);
+        m.nei.push_back(-1);
+        const int* t = bmap.find(edge_key(a1, a2));
+        m.tag.push_back(t ? *t : 0);
+      }
+    }
+  }
+  return CFD_OK;
+}
+
+void compute_geometry(cfd_mesh& m) {
+  int N = m.ncells(), F = m.nfaces();
+  const double* X = m.x.data();
+  const double* Y = m.y.data();
+  m.vol.assign(N, 0.0); m.xc.assign(N, 0.0); m.yc.assign(N, 0.0);
+  for (int c = 0; c < N; ++c) {
+    const int* nd = &m.cell_nodes[m.cell_ptr[c]];
+    int n = m.cell_ptr[c + 1] - m.cell_ptr[c];
+    if (n == 3) {
+      double x1 = X[nd[0]], y1 = Y[nd[0]], x2 = X[nd[1]], y2 = Y[nd[1]], x3 = X[nd[2]], y3 = Y[nd[2]];
+      m.vol[c] = 0.5 * std::abs((x2 - x1) * (y3 - y1) - (x3 - x1) * (y2 - y1));
+      m.xc[c] = (x1 + x2 + x3) / 3.0;
+      m.yc[c] = (y1 + y2 + y3) / 3.0;
+    } else {
+      double sx = 0, sy = 0;
+      for (int k = 0; k < 4; ++k) { sx += X[nd[k]]; sy += Y[nd[k]]; }
+      m.xc[c] = sx / 4.0;
+      m.yc[c] = sy / 4.0;
+      double a = 0.0;
+      for (int k = 0; k < 4; ++k) {
+        int p = nd[k], q = nd[(k + 1) % 4];
+        a += (X[p] * Y[q] - X[q] * Y[p]);
+      }
+      m.vol[c] = 0.5 * std::abs(a);
+    }
+  }
+  m.nx.assign(F, 0); m.ny.assign(F, 0); m.area.assign(F, 0); m.xm.assign(F, 0); m.ym.assign(F, 0);
+  for (int f = 0; f < F; ++f) {
+    double dx = X[m.n2[f]] - X[m.n1[f]];
+    double dy = Y[m.n2[f]] - Y[m.n1[f]];
+    double A = std::sqrt(dx * dx + dy * dy);
+    m.area[f] = A;
+    m.xm[f] = 0.5 * (X[m.n1[f]] + X[m.n2[f]]);
+    m.ym[f] = 0.5 * (Y[m.n1[f]] + Y[m.n2[f]]);
+    double nx = dy / A, ny = -dx / A;
+    double vx = m.xm[f] - m.xc[m.own[f]], vy = m.ym[f] - m.yc[m.own[f]];
+    if (vx * nx + vy * ny < 0) { nx = -nx; ny = -ny; }
+    m.nx[f] = nx;
+    m.ny[f] = ny;
+  }
+}
+
+// Reverse Cuthill-McKee over the cell graph (interior faces).  Deterministic:
+// each component starts at its lowest-degree unvisited cell (lowest index on
+// ties) and neighbours are queued by ascending degree, then index.
+void rcm(cfd_mesh& m) {
+  int N = m.ncells(), F = m.nfaces();
+  std::vector<int> deg(N, 0);
+  for (int f = 0; f < F; ++f)
+    if (m.nei[f] >= 0) { deg[m.own[f]]++; deg[m.nei[f]]++; }
+  std::vector<int> aptr(N + 1, 0);
+  for (int c = 0; c < N; ++c) aptr[c + 1] = aptr[c] + deg[c];
+  std::vector<int> adj(aptr[N]), fill(aptr.begin(), aptr.end() - 1);
+  for (int f = 0; f < F; ++f)
+    if (m.nei[f] >= 0) { adj[fill[m.own[f]]++] = m.nei[f]; adj[fill[m.nei[f]]++] = m.own[f]; }
+  std::vector<int> start(N);
+  std::iota(start.begin(), start.end(), 0);
+  std::stable_sort(start.begin(), start.end(), [&](int a, int b) { return deg[a] < deg[b]; });
+  std::vector<char> seen(N, 0);
+  std::vector<int> order;
+  order.reserve(N);
+  size_t si = 0;
+  int nb[8];
+  while ((int)order.size() < N) {
+    while (seen[start[si]]) ++si;
+    int s = start[si];
+    seen[s] = 1;
+    size_t head = order.size();
+    order.push_back(s);
+    while (head < order.size()) {
+      int c = order[head++];
+      int k = 0;
+      for (int a = aptr[c]; a < aptr[c + 1]; ++a)
+        if (!seen[adj[a]]) { nb[k++] = adj[a]; seen[adj[a]] = 1; }
+      std::sort(nb, nb + k, [&](int a, int b) { return deg[a] != deg[b] ? deg[a] < deg[b] : a < b; });
+      for (int i = 0; i < k; ++i) order.push_back(nb[i]);
+    }
+  }
+  std::reverse(order.begin(), order.end());
+  m.iperm = order;
+  m.perm.assign(N, 0);
+  for (int i = 0; i < N; ++i) m.perm[order[i]] = i;
+}
+
+// Greedy edge coloring: faces visited by ascending RCM owner position (then
+// reference index); each takes the lowest color not yet used at its owner or
+// neighbor.  No two faces of one color share a cell, so a color's scatter
+// needs no atomics.  Device order: by color, then owner position, then
+// reference index.
+void color_faces(cfd_mesh& m) {
+  int N = m.ncells(), F = m.nfaces();
+  std::vector<int> ord(F);
+  std::iota(ord.begin(), ord.end(), 0);
+  std::stable_sort(ord.begin(), ord.end(), [&](int a, int b) { return m.perm[m.own[a]] < m.perm[m.own[b]]; });
+  std::vector<uint32_t> used(N, 0);
+  m.color.assign(F, 0);
+  int nc = 0;
+  for (int f : ord) {
+    uint32_t u = used[m.own[f]] | (m.nei[f] >= 0 ? used[m.nei[f]] : 0u);
+    int c = 0;
+    while (u & (1u << c)) ++c;
+    m.color[f] = c;
+    used[m.own[f]] |= 1u << c;
+    if (m.nei[f] >= 0) used[m.nei[f]] |= 1u << c;
+    nc = std::max(nc, c + 1);
+  }
+  m.ncolors = nc;
+  m.face_order = ord;
+  std::stable_sort(m.face_order.begin(), m.face_order.end(), [&](int a, int b) { return m.color[a] < m.color[b]; });
+  m.color_ptr.assign(nc + 1, 0);
+  for (int f = 0; f < F; ++f) m.color_ptr[m.color[f] + 1]++;
+  for (int c = 0; c < nc; ++c) m.color_ptr[c + 1] += m.color_ptr[c];
+}
+
+// ---- synthetic meshes -------------------------------------------------------
+struct SplitMix64 {  // Vigna's SplitMix64
+  uint64_t s;
+  explicit SplitMix64(uint64_t seed) : s(seed) {}
+  uint64_t next() {
+    uint64_t z = (s += 0x9E3779B97F4A7C15ull);
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    return z ^ (z >> 31);
+  }
+  double uniform() { return (double)(next() >> 11) * 0x1.0p-53; }
+};
+inline uint64_t stream_seed(int cfg, int purpose) { return 0x5EED0000ull + 16ull * (uint64_t)cfg + (uint64_t)purpose; }
+
+// wall-normal spacing: dy_k = min(d1 r^k, dmax) with dmax chosen so the ny
+// spacings sum to H (bisection), returned as node ordinates 0..H
+std::vector<double> stretched(int ny, double H, double d1, double r) {
+  auto total = [&](double dmax) {
+    double s = 0, d = d1;
+    for (int k = 0; k < ny; ++k) { s += std::min(d, dmax); d *= r; }
+    return s;
+  };
+  std::vector<double> y(ny + 1, 0.0);
+  if (d1 * ny >= H) {  // cannot stretch: uniform
+    for (int j = 0; j <= ny; ++j) y[j] = H * j / ny;
+    return y;
+  }
+  double lo = d1, hi = H;
+  for (int it = 0; it < 200; ++it) {
+    double mid = 0.5 * (lo + hi);
+    if (total(mid) < H) lo = mid; else hi = mid;
+  }
+  double d = d1, s = 0;
+  for (int k = 0; k < ny; ++k) { s += std::min(d, hi); y[k + 1] = s; d *= r; }
+  double sc = H / y[ny];
+  for (int j = 1; j < ny; ++j) y[j] *= sc;
+  y[ny] = H;
+  return y;
+}
+
+struct GenSpec {
+  int nx, ny;
+  double split_p;
+};
+
+}  // namespace
+
+namespace cfdmesh {
+int finish(cfd_mesh& m, std::string& err) {
+  int rc = build_connectivity(m, err);
+  if (rc) return rc;
+  compute_geometry(m);
+  rcm(m);
+  color_faces(m);
+  return CFD_OK;
+}
+}  // namespace cfdmesh
+
+extern "C" {
+
+int cfd_mesh_build(const cfd_mesh_desc* d, cfd_mesh** out) {
+  if (!d || !out || d->n_nodes < 1 || d->n_cells < 1 || !d->x || !d->y || !d->cell_ptr || !d->cell_nodes)
+    return CFD_ERR_ARG;
+  std::unique_ptr<cfd_mesh> m(new (std::nothrow) cfd_mesh());
+  if (!m) return CFD_ERR_ARG;
+  m->n_nodes = d->n_nodes;
+  m->x.assign(d->x, d->x + d->n_nodes + 1);
+  m->y.assign(d->y, d->y + d->n_nodes + 1);
+  m->cell_ptr.assign(d->cell_ptr, d->cell_ptr + d->n_cells + 1);
+  m->cell_nodes.assign(d->cell_nodes, d->cell_nodes + d->cell_ptr[d->n_cells]);
+  if (d->n_bedges > 0 && d->bedges) m->bedges.assign(d->bedges, d->bedges + 3 * d->n_bedges);
+  std::string err;
+  int rc = cfdmesh::finish(*m, err);
+  if (rc) return rc;
+  *out = m.release();
+  return CFD_OK;
+}
+
+// Mesh::readGmsh (Mesh.cpp:30-97)
+int cfd_mesh_read_gmsh(const char* path, cfd_mesh** out) {
+  if (!path || !out) return CFD_ERR_ARG;
+  std::ifstream in(path);
+  if (!in.is_open()) return CFD_ERR_IO;
+  std::unique_ptr<cfd_mesh> m(new cfd_mesh());
+  std::string line;
+  while (std::getline(in, line)) {
+    if (!line.empty() && line.back() == '\r') line.pop_back();  // CRLF files (Windows text mode)
+    if (line == "$Nodes") {
+      int nn = 0;
+      in >> nn;
+      m->n_nodes = nn;
+      m->x.assign(nn + 1, 0.0);
+      m->y.assign(nn + 1, 0.0);
+      for (int i = 1; i <= nn; ++i) {
+        int id;
+        double z;
+        in >> id >> m->x[i] >> m->y[i] >> z;
+      }
+      break;
+    }
+  }
+  while (std::getline(in, line)) {
+    if (!line.empty() && line.back() == '\r') line.pop_back();  // CRLF files (Windows text mode)
+    if (line == "$Elements") {
+      int ne = 0;
+      in >> ne;
+      for (int e = 0; e < ne; ++e) {
+        int id, type, ntags, t, phys = 0;
+        in >> id >> type >> ntags;
+        for (int j = 0; j < ntags; ++j) {
+          in >> t;
+          if (j == 0) phys = t;
+        }
+        if (type == 1) {
+          int a, b;
+          in >> a >> b;
+          m->bedges.push_back(a);
+          m->bedges.push_back(b);
+          m->bedges.push_back(phys);
+        } else if (type == 2 || type == 3) {
+          int nv = type == 2 ? 3 : 4;
+          for (int k = 0; k < nv; ++k) {
+            int nd;
+            in >> nd;
+            m->cell_nodes.push_back(nd);
+          }
+          m->cell_ptr.push_back((int)m->cell_nodes.size());
+        }
+        std::getline(in, line);
+      }
+      break;
+    }
+  }
+  if (m->ncells() < 1) return CFD_ERR_IO;
+  std::string err;
+  int rc = cfdmesh::finish(*m, err);
+  if (rc) return rc;
+  *out = m.release();
+  return CFD_OK;
+}
+
+int cfd_mesh_destroy(cfd_mesh* m) {
+  delete m;
+  return CFD_OK;
+}
+
+int cfd_mesh_sizes(const cfd_mesh* m, int* nn, int* nc, int* nf, int* ncol) {
+  if (!m) return CFD_ERR_ARG;
+  if (nn) *nn = m->n_nodes;
+  if (nc) *nc = m->ncells();
+  if (nf) *nf = m->nfaces();
+  if (ncol) *ncol = m->ncolors;
+  return CFD_OK;
+}
+
+int cfd_mesh_get_desc(const cfd_mesh* m, cfd_mesh_desc* d) {
+  if (!m || !d) return CFD_ERR_ARG;
+  d->n_nodes = m->n_nodes;
+  d->x = m->x.data();
+  d->y = m->y.data();
+  d->n_cells = m->ncells();
+  d->cell_ptr = m->cell_ptr.data();
+  d->cell_nodes = m->cell_nodes.data();
+  d->n_bedges = (int)m->bedges.size() / 3;
+  d->bedges = m->bedges.data();
+  return CFD_OK;
+}
+
+int cfd_mesh_get_faces(const cfd_mesh* m, int* n1, int* n2, int* own, int* nei, int* tag, double* nx, double* ny,
+                       double* area, double* xm, double* ym) {
+  if (!m) return CFD_ERR_ARG;
+  size_t F = m->nfaces();
+  auto cp = [F](auto* dst, const auto& src) {
+    if (dst) std::memcpy(dst, src.data(), F * sizeof(src[0]));
+  };
+  cp(n1, m->n1); cp(n2, m->n2); cp(own, m->own); cp(nei, m->nei); cp(tag, m->tag);
+  cp(nx, m->nx); cp(ny, m->ny); cp(area, m->area); cp(xm, m->xm); cp(ym, m->ym);
+  return CFD_OK;
+}
+
+int cfd_mesh_get_cells(const cfd_mesh* m, double* vol, double* xc, double* yc) {
+  if (!m) return CFD_ERR_ARG;
+  size_t N = m->ncells();
+  if (vol) std::memcpy(vol, m->vol.data(), N * 8);
+  if (xc) std::memcpy(xc, m->xc.data(), N * 8);
+  if (yc) std::memcpy(yc, m->yc.data(), N * 8);
+  return CFD_OK;
+}
+
+int cfd_mesh_get_reorder(const cfd_mesh* m, int* perm, int* color) {
+  if (!m) return CFD_ERR_ARG;
+  if (perm) std::memcpy(perm, m->perm.data(), m->perm.size() * 4);
+  if (color) std::memcpy(color, m->color.data(), m->color.size() * 4);
+  return CFD_OK;
+}
+
+uint64_t cfd_mesh_hash(const cfd_mesh* m) {
+  if (!m) return 0;
+  uint64_t h = 0xcbf29ce484222325ull;
+  auto feed = [&](const void* p, size_t n) {
+    const unsigned char* b = (const unsigned char*)p;
+    for (size_t i = 0; i < n; ++i) { h ^= b[i]; h *= 0x100000001b3ull; }
+  };
+  feed(m->x.data(), m->x.size() * 8);
+  feed(m->y.data(), m->y.size() * 8);
+  feed(m->cell_ptr.data(), m->cell_ptr.size() * 4);
+  feed(m->cell_nodes.data(), m->cell_nodes.size() * 4);
+  feed(m->bedges.data(), m->bedges.size() * 4);
+  return h;
+}
+
+int cfd_mesh_write_gmsh(const cfd_mesh* m, const char* path) {
+  if (!m || !path) return CFD_ERR_ARG;
+  FILE* f = std::fopen(path, "w");
+  if (!f) return CFD_ERR_IO;
+  std::fprintf(f, "$MeshFormat\n2.2 0 8\n$EndMeshFormat\n$Nodes\n%d\n", m->n_nodes);
+  for (int i = 1; i <= m->n_nodes; ++i) std::fprintf(f, "%d %.17g %.17g 0\n", i, m->x[i], m->y[i]);
+  int nb = (int)m->bedges.size() / 3, nc = m->ncells();
+  std::fprintf(f, "$EndNodes\n$Elements\n%d\n", nb + nc);
+  int id = 1;
+  for (int e = 0; e < nb; ++e)
+    std::fprintf(f, "%d 1 2 %d %d %d %d\n", id++, m->bedges[3 * e + 2], m->bedges[3 * e + 2], m->bedges[3 * e],
+                 m->bedges[3 * e + 1]);
+  for (int c = 0; c < nc; ++c) {
+    int b = m->cell_ptr[c], n = m->cell_ptr[c + 1] - b;
+    std::fprintf(f, "%d %d 2 100 100", id++, n == 3 ? 2 : 3);
+    for (int k = 0; k < n; ++k) std::fprintf(f, " %d", m->cell_nodes[b + k]);
+    std::fprintf(f, "\n");
+  }
+  std::fprintf(f, "$EndElements\n");
+  std::fclose(f);
+  return CFD_OK;
+}
+
+// Seeded synthetic meshes.  Nodes on an (nx+1) x (ny+1) grid, id = 1 + j (nx+1) + i.
+// Quads visited row-major (j outer, i inner); stream 1 draws u per quad, the quad
+// is split when u < p and a second draw's top bit picks the diagonal; a split
+// quad is emitted as two consecutive counter-clockwise triangles.
+int cfd_mesh_generate(int config, double scale, cfd_mesh** out) {
+  if (!out || config < 1 || config > 5 || !(scale > 0)) return CFD_ERR_ARG;
+  static const GenSpec specs[6] = {{0, 0, 0}, {160, 80, 1.0}, {400, 160, 0.2}, {1200, 1000, 0.1},
+                                   {1280, 560, 0.15}, {2000, 2000, 0.1}};
+  GenSpec s = specs[config];
+  int nx = std::max(2, (int)std::lround(s.nx * scale));
+  int ny = std::max(2, (int)std::lround(s.ny * scale));
+  std::unique_ptr<cfd_mesh> m(new cfd_mesh());
+  m->config = config;
+  m->n_nodes = (nx + 1) * (ny + 1);
+  m->x.assign(m->n_nodes + 1, 0.0);
+  m->y.assign(m->n_nodes + 1, 0.0);
+  auto nid = [&](int i, int j) { return 1 + j * (nx + 1) + i; };
+  const double PI = 3.14159265358979323846;
+  std::vector<double> xs(nx + 1), eta(ny + 1);
+  // per-config node coordinates and boundary tags
+  // tags: bottom(i) / top(i) / left / right
+  std::vector<int> tb(nx), tt(nx);
+  int tl = 1, tr = 2;
+  if (config == 1) {  // 15-degree wedge ramp in [0,1] x [0,0.5]
+    for (int i = 0; i <= nx; ++i) xs[i] = (double)i / nx;
+    for (int j = 0; j <= ny; ++j) eta[j] = (double)j / ny;
+    double tn = std::tan(15.0 * PI / 180.0);
+    for (int j = 0; j <= ny; ++j)
+      for (int i = 0; i <= nx; ++i) {
+        double x = xs[i], yb = std::max(0.0, (x - 0.25) * tn);
+        m->x[nid(i, j)] = x;
+        m->y[nid(i, j)] = yb + (0.5 - yb) * eta[j];
+      }
+    for (int i = 0; i < nx; ++i) { tb[i] = 3; tt[i] = 1; }
+  } else if (config == 2 || config == 3 || config == 5) {  // flat plate, x in [-0.1, 1]
+    double H = 0.5, d1, r;
+    if (config == 2) { d1 = 1.0 / 3240.0; r = 1.08; }
+    else if (config == 3) { d1 = 4e-6; r = 1.1; }
+    else { d1 = 2e-5; r = 1.05; }
+    std::vector<double> yn = stretched(ny, H, d1, r);
+    for (int i = 0; i <= nx; ++i) xs[i] = -0.1 + 1.1 * i / nx;
+    for (int j = 0; j <= ny; ++j)
+      for (int i = 0; i <= nx; ++i) {
+        m->x[nid(i, j)] = xs[i];
+        m->y[nid(i, j)] = yn[j];
+      }
+    for (int i = 0; i < nx; ++i) {
+      double xmid = 0.5 * (xs[i] + xs[i + 1]);
+      tb[i] = xmid < 0 ? 3 : 4;
+      tt[i] = 5;
+    }
+    if (config == 5) {
+      // oblique shock of 8 degrees deflection, Mach 2, hits the wall at x = 0.5:
+      // top faces downstream of the shock foot get the post-shock state (tag 6)
+      double beta = 0;
+      {
+        double M = 2.0, th = 8.0 * PI / 180.0, g = 1.4, lo = std::asin(1.0 / M) + 1e-9, hi = PI / 2;
+        // weak-shock branch of tan(th) = 2 cot(b) (M^2 sin^2 b - 1) / (M^2 (g + cos 2b) + 2)
+        auto fn = [&](double b) {
+          return 2.0 / std::tan(b) * (M * M * std::sin(b) * std::sin(b) - 1.0) /
+                     (M * M * (g + std::cos(2 * b)) + 2.0) - std::tan(th);
+        };
+        double bmax = lo, fmax = -1e30;
+        for (int k = 0; k <= 2000; ++k) {
+          double b = lo + (hi - lo) * k / 2000.0;
+          if (fn(b) > fmax) { fmax = fn(b); bmax = b; }
+        }
+        double a = lo, b = bmax;
+        for (int it = 0; it < 200; ++it) {
+          double mid = 0.5 * (a + b);
+          if (fn(mid) < 0) a = mid; else b = mid;
+        }
+        beta = 0.5 * (a + b);
+      }
+      double xtop = 0.5 - H / std::tan(beta);
+      for (int i = 0; i < nx; ++i) {
+        double xmid = 0.5 * (xs[i] + xs[i + 1]);
+        tt[i] = xmid >= xtop ? 6 : 5;
+      }
+    }
+  } else {  // config 4: 3 x 1 channel with a 5%-height circular-arc bump on [1,2]
+    double h = 0.05, R = (0.25 + h * h) / (2 * h);
+    for (int i = 0; i <= nx; ++i) xs[i] = 3.0 * i / nx;
+    for (int j = 0; j <= ny; ++j) eta[j] = (double)j / ny;
+    for (int j = 0; j <= ny; ++j)
+      for (int i = 0; i <= nx; ++i) {
+        double x = xs[i], yb = 0.0;
+        if (x > 1.0 && x < 2.0) yb = std::sqrt(R * R - (x - 1.5) * (x - 1.5)) - (R - h);
+        m->x[nid(i, j)] = x;
+        m->y[nid(i, j)] = yb + (1.0 - yb) * eta[j];
+      }
+    for (int i = 0; i < nx; ++i) { tb[i] = 3; tt[i] = 3; }
+  }
+  SplitMix64 rng(stream_seed(config, 1));
+  m->cell_nodes.reserve((size_t)nx * ny * 6);
+  for (int j = 0; j < ny; ++j)
+    for (int i = 0; i < nx; ++i) {
+      int a = nid(i, j), b = nid(i + 1, j), c = nid(i + 1, j + 1), d = nid(i, j + 1);
+      double u = rng.uniform();
+      if (u < s.split_p) {
+        bool diag = (rng.next() >> 63) != 0;
+        int t1[3], t2[3];
+        if (!diag) { t1[0] = a; t1[1] = b; t1[2] = c; t2[0] = a; t2[1] = c; t2[2] = d; }
+        else { t1[0] = a; t1[1] = b; t1[2] = d; t2[0] = b; t2[1] = c; t2[2] = d; }
+        m->cell_nodes.insert(m->cell_nodes.end(), t1, t1 + 3);
+        m->cell_ptr.push_back((int)m->cell_nodes.size());
+        m->cell_nodes.insert(m->cell_nodes.end(), t2, t2 + 3);
+        m->cell_ptr.push_back((int)m->cell_nodes.size());
+      } else {
+        int q[4] = {a, b, c, d};
+        m->cell_nodes.insert(m->cell_nodes.end(), q, q + 4);
+        m->cell_ptr.push_back((int)m->cell_nodes.size());
+      }
+    }
+  auto addb = [&](int p, int q, int t) { m->bedges.push_back(p); m->bedges.push_back(q); m->bedges.push_back(t); };
+  for (int i = 0; i < nx; ++i) addb(nid(i, 0), nid(i + 1, 0), tb[i]);
+  for (int j = 0; j < ny; ++j) addb(nid(nx, j), nid(nx, j + 1), tr);
+  for (int i = 0; i < nx; ++i) addb(nid(i, ny), nid(i + 1, ny), tt[i]);
+  for (int j = 0; j < ny; ++j) addb(nid(0, j), nid(0, j + 1), tl);
+  std::string err;
+  int rc = cfdmesh::finish(*m, err);
+  if (rc) return rc;
+  *out = m.release();
+  return CFD_OK;
+}
+
+}  // extern "C"

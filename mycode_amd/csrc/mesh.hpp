@@ -1,0 +1,41 @@
+// Host-side mesh representation shared by the mesh pipeline and the device
+// context.  Reference-order arrays reproduce Mesh::faces / Mesh::cells
+// (Types.hpp:16-35) field for field; the reorder arrays describe the device
+// layout (RCM cell order, faces grouped by edge color).
+#pragma once
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "../../include/cfd_gpu.h"
+
+struct cfd_mesh {
+  // description (Gmsh-like, 1-based node ids)
+  int n_nodes = 0;
+  std::vector<double> x, y;  // n_nodes + 1
+  std::vector<int> cell_ptr{0}, cell_nodes;
+  std::vector<int> bedges;  // n1, n2, tag triples
+  // faces in reference order
+  std::vector<int> n1, n2, own, nei, tag;
+  std::vector<double> nx, ny, area, xm, ym;
+  // cells in reference order
+  std::vector<double> vol, xc, yc;
+  // device layout
+  std::vector<int> perm;        // reference cell -> RCM position
+  std::vector<int> iperm;       // RCM position -> reference cell
+  std::vector<int> color;       // reference face -> color
+  int ncolors = 0;
+  std::vector<int> face_order;  // device face slot -> reference face
+  std::vector<int> color_ptr;   // ncolors + 1 offsets into face_order
+  // synthetic-mesh extras (valid after cfd_mesh_generate)
+  int config = 0;
+  std::vector<double> init_U6;  // seeded initial state, reference order (may be empty)
+
+  int ncells() const { return (int)cell_ptr.size() - 1; }
+  int nfaces() const { return (int)own.size(); }
+};
+
+namespace cfdmesh {
+// Mesh::buildConnectivity + Mesh::computeGeometry + RCM + coloring
+int finish(cfd_mesh& m, std::string& err);
+}

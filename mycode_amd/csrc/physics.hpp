@@ -1,0 +1,285 @@
+// Pointwise numerics of the residual, written once for any scalar type T
+// (double for the residual, float for the FP32 path, Dual<> for the forward
+// tangent).  Every expression keeps the operand order of the reference so
+// that, built without multiply-add contraction, the FP64 results agree with
+// the reference bit for bit.
+//
+//   pressure / normal flux      Flux.hpp:12-35
+//   HLLC                        Flux.hpp:36-74
+//   LLF (Rusanov)               Flux.hpp:81-110
+//   Roe + Harten entropy fix    Flux.hpp:112-213
+//   ghost states (per BC set)   Solver.cpp:806-863, 973-1011, 1088-1139
+//   primitive -> conservative   Solver.cpp:498-501
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <type_traits>
+
+#include "../../include/cfd_gpu.h"
+
+#define HD __host__ __device__ __forceinline__
+
+namespace cfd {
+
+// ---- dual numbers for forward-mode differentiation -------------------------
+template <class F> struct Dual {
+  F v, d;
+  HD Dual() : v(0), d(0) {}
+  HD Dual(F a) : v(a), d(0) {}
+  HD Dual(F a, F b) : v(a), d(b) {}
+};
+template <class F> HD Dual<F> operator+(Dual<F> a, Dual<F> b) { return Dual<F>(a.v + b.v, a.d + b.d); }
+template <class F> HD Dual<F> operator-(Dual<F> a, Dual<F> b) { return Dual<F>(a.v - b.v, a.d - b.d); }
+template <class F> HD Dual<F> operator-(Dual<F> a) { return Dual<F>(-a.v, -a.d); }
+template <class F> HD Dual<F> operator*(Dual<F> a, Dual<F> b) { return Dual<F>(a.v * b.v, a.d * b.v + a.v * b.d); }
+template <class F> HD Dual<F> operator/(Dual<F> a, Dual<F> b) {
+  F q = a.v / b.v;
+  return Dual<F>(q, (a.d - q * b.d) / b.v);
+}
+// mixed operations with plain arithmetic constants: the constant is first
+// converted to the dual's component type, so an FP32 dual stays in FP32
+#define CFD_ARITH template <class F, class A, class = typename std::enable_if<std::is_arithmetic<A>::value>::type>
+CFD_ARITH HD Dual<F> operator+(Dual<F> a, A b) { return Dual<F>(a.v + F(b), a.d); }
+CFD_ARITH HD Dual<F> operator+(A a, Dual<F> b) { return Dual<F>(F(a) + b.v, b.d); }
+CFD_ARITH HD Dual<F> operator-(Dual<F> a, A b) { return Dual<F>(a.v - F(b), a.d); }
+CFD_ARITH HD Dual<F> operator-(A a, Dual<F> b) { return Dual<F>(F(a) - b.v, -b.d); }
+CFD_ARITH HD Dual<F> operator*(Dual<F> a, A b) { return Dual<F>(a.v * F(b), a.d * F(b)); }
+CFD_ARITH HD Dual<F> operator*(A a, Dual<F> b) { return Dual<F>(F(a) * b.v, F(a) * b.d); }
+CFD_ARITH HD Dual<F> operator/(Dual<F> a, A b) { return Dual<F>(a.v / F(b), a.d / F(b)); }
+CFD_ARITH HD Dual<F> operator/(A a, Dual<F> b) {
+  F q = F(a) / b.v;
+  return Dual<F>(q, -q * b.d / b.v);
+}
+
+template <class T> struct Real { typedef T type; };
+template <class F> struct Real<Dual<F>> { typedef F type; };
+
+HD double val(double x) { return x; }
+HD float val(float x) { return x; }
+template <class F> HD F val(Dual<F> x) { return x.v; }
+
+HD double sq_rt(double x) { return sqrt(x); }
+HD float sq_rt(float x) { return sqrtf(x); }
+template <class F> HD Dual<F> sq_rt(Dual<F> x) {
+  F s = sq_rt(x.v);
+  return Dual<F>(s, x.d / (F(2) * s));
+}
+HD double pw(double a, double b) { return pow(a, b); }
+HD float pw(float a, float b) { return powf(a, b); }
+template <class F, class A> HD Dual<F> pw(Dual<F> a, A bb) {
+  F b = F(bb);
+  F p = pw(a.v, b);
+  return Dual<F>(p, b * pw(a.v, b - F(1)) * a.d);
+}
+// |x|, std::min, std::max with the reference's tie behaviour
+template <class T> HD T ab(T x) { return val(x) < 0 ? T(-x) : x; }
+template <class T> HD T mn(T a, T b) { return val(b) < val(a) ? b : a; }
+template <class T> HD T mx(T a, T b) { return val(a) < val(b) ? b : a; }
+
+// ---- state ---------------------------------------------------------------------
+template <class T> struct S6 {
+  T q[6];
+  HD S6() {
+#pragma unroll
+    for (int i = 0; i < 6; ++i) q[i] = T(0);
+  }
+  HD S6(T a, T b, T c, T d, T e = T(0), T f = T(0)) {
+    q[0] = a; q[1] = b; q[2] = c; q[3] = d; q[4] = e; q[5] = f;
+  }
+  HD T& operator[](int i) { return q[i]; }
+  HD const T& operator[](int i) const { return q[i]; }
+};
+template <class T> HD S6<T> add(const S6<T>& a, const S6<T>& b) {
+  S6<T> s;
+#pragma unroll
+  for (int i = 0; i < 6; ++i) s.q[i] = a.q[i] + b.q[i];
+  return s;
+}
+template <class T> HD S6<T> sub(const S6<T>& a, const S6<T>& b) {
+  S6<T> s;
+#pragma unroll
+  for (int i = 0; i < 6; ++i) s.q[i] = a.q[i] - b.q[i];
+  return s;
+}
+template <class T> HD S6<T> scale(const S6<T>& a, T k) {
+  S6<T> s;
+#pragma unroll
+  for (int i = 0; i < 6; ++i) s.q[i] = a.q[i] * k;
+  return s;
+}
+
+// ---- parameters as the kernels see them ---------------------------------------------
+struct Phys {
+  double gamma, R, P0, T0, pexit, nu, Pr, Cp;
+  double fs[6];  // frozen freestream (SUPERSONIC tag 1, PLATE tags 1 and 5)
+  double sh[6];  // frozen post-shock state (SWBLI tag 6)
+  int flux, flux_alt, spatial, bcset, viscous, nvar;
+};
+
+template <class T> HD T pressure(const Phys& P, const S6<T>& U) {
+  T V2 = (U[1] * U[1] + U[2] * U[2]) / (U[0] * U[0]);
+  return (P.gamma - 1.0) * (U[3] - 0.5 * U[0] * V2);
+}
+template <class T> HD S6<T> prim2cons(const Phys& P, T rho, T u, T v, T p) {
+  T rE = p / (P.gamma - 1.0) + 0.5 * rho * (u * u + v * v);
+  return S6<T>(rho, rho * u, rho * v, rE, rho * T(0.0), rho * T(0.0));
+}
+template <class T> HD S6<T> frozen(const double* s) {
+  return S6<T>(T(s[0]), T(s[1]), T(s[2]), T(s[3]), T(s[4]), T(s[5]));
+}
+template <class T> HD S6<T> slip_ghost(const S6<T>& U, double nx, double ny) {
+  T u = U[1] / U[0], v = U[2] / U[0];
+  T vn = u * nx + v * ny;
+  T ug = u - 2.0 * vn * nx, vg = v - 2.0 * vn * ny;
+  S6<T> G = U;
+  G[1] = U[0] * ug;
+  G[2] = U[0] * vg;
+  return G;
+}
+
+// Ghost state of a boundary face with tag `tag` seen from interior state U.
+template <class T> HD S6<T> ghost(const Phys& P, const S6<T>& U, int tag, double nx, double ny) {
+  if (P.bcset == CFD_BC_BUMP) {  // Solver.cpp:806-863
+    if (tag == 1) {
+      T pin = pressure(P, U);
+      T puse = mn(pin, T(P.P0 - 1e-4));
+      T Tg = P.T0 * pw(puse / P.P0, (P.gamma - 1.0) / P.gamma);
+      T rg = puse / (P.R * Tg);
+      T V2 = 2.0 * P.Cp * (P.T0 - Tg);
+      T V = val(V2) > 0 ? sq_rt(V2) : T(0.0);
+      return prim2cons(P, rg, V, T(0.0), puse);
+    }
+    if (tag == 2) {
+      T rho = U[0], u = U[1] / U[0], v = U[2] / U[0];
+      return prim2cons(P, rho, u, v, T(P.pexit));
+    }
+    if (tag == 3) return slip_ghost(U, nx, ny);
+    return U;
+  }
+  if (P.bcset == CFD_BC_SUPERSONIC) {  // Solver.cpp:973-1011
+    if (tag == 1) return frozen<T>(P.fs);
+    if (tag == 3) return slip_ghost(U, nx, ny);
+    return U;  // tag 2 extrapolates; untagged faces mirror the interior
+  }
+  // PLATE (Solver.cpp:1088-1139) and SWBLI (PLATE + tag 6)
+  if (tag == 1 || tag == 5) return frozen<T>(P.fs);
+  if (tag == 3) return slip_ghost(U, nx, ny);
+  if (tag == 4) {
+    T ug = -(U[1] / U[0]), vg = -(U[2] / U[0]);
+    T ek = 0.5 * (U[1] * U[1] + U[2] * U[2]) / U[0];
+    T pin = (P.gamma - 1.0) * (U[3] - ek);
+    T Tin = pin / (U[0] * P.R);
+    T rg = pin / (P.R * Tin);
+    return prim2cons(P, rg, ug, vg, pin);
+  }
+  if (tag == 6 && P.bcset == CFD_BC_SWBLI) return frozen<T>(P.sh);
+  return U;
+}
+
+template <class T> HD S6<T> normal_flux(const Phys& P, const S6<T>& U, double nx, double ny) {
+  T p = pressure(P, U);
+  T u = U[1] / U[0], v = U[2] / U[0];
+  T un = u * nx + v * ny;
+  return S6<T>(U[0] * un, U[1] * un + p * nx, U[2] * un + p * ny, (U[3] + p) * un, T(0.0), T(0.0));
+}
+
+template <class T> HD S6<T> flux_hllc(const Phys& P, const S6<T>& L, const S6<T>& R, double nx, double ny) {
+  const double g = P.gamma;
+  T pL = pressure(P, L), pR = pressure(P, R);
+  T uL = L[1] / L[0], uR = R[1] / R[0];
+  T vL = L[2] / L[0], vR = R[2] / R[0];
+  T unL = uL * nx + vL * ny, unR = uR * nx + vR * ny;
+  T cL = sq_rt(g * pL / L[0]), cR = sq_rt(g * pR / R[0]);
+  T hL = (L[3] + pL) / L[0], hR = (R[3] + pR) / R[0];
+  T sL = sq_rt(L[0]), sR = sq_rt(R[0]);
+  T ut = (sL * uL + sR * uR) / (sL + sR);
+  T vt = (sL * vL + sR * vR) / (sL + sR);
+  T ht = (sL * hL + sR * hR) / (sL + sR);
+  T unt = ut * nx + vt * ny;
+  T ct = sq_rt((g - 1.0) * (ht - 0.5 * (ut * ut + vt * vt)));
+  T SL = mn(unL - cL, unt - ct);
+  T SR = mx(unR + cR, unt + ct);
+  T Ss = (pR - pL + L[0] * unL * (SL - unL) - R[0] * unR * (SR - unR)) / (L[0] * (SL - unL) - R[0] * (SR - unR));
+  if (val(SL) >= 0.0) return normal_flux(P, L, nx, ny);
+  if (val(SR) <= 0.0) return normal_flux(P, R, nx, ny);
+  // star region: upwind side K with wave speed S
+  const bool left = val(Ss) >= 0.0;
+  const S6<T>& K = left ? L : R;
+  T S = left ? SL : SR;
+  T pK = left ? pL : pR, uK = left ? uL : uR, vK = left ? vL : vR, unK = left ? unL : unR;
+  T coef = K[0] * (S - unK) / (S - Ss);
+  S6<T> Us(coef, coef * (uK + (Ss - unK) * nx), coef * (vK + (Ss - unK) * ny),
+           coef * (K[3] / K[0] + (Ss - unK) * (Ss + pK / (K[0] * (S - unK)))), T(0.0), T(0.0));
+  return add(normal_flux(P, K, nx, ny), scale(sub(Us, K), S));
+}
+
+template <class T> HD S6<T> flux_llf(const Phys& P, const S6<T>& L, const S6<T>& R, double nx, double ny) {
+  const double g = P.gamma;
+  S6<T> FL = normal_flux(P, L, nx, ny), FR = normal_flux(P, R, nx, ny);
+  T pL = pressure(P, L);
+  T uL = L[1] / L[0], vL = L[2] / L[0];
+  T unL = uL * nx + vL * ny;
+  T cL = sq_rt(g * pL / L[0]);
+  T pR = pressure(P, R);
+  T uR = R[1] / R[0], vR = R[2] / R[0];
+  T unR = uR * nx + vR * ny;
+  T cR = sq_rt(g * pR / R[0]);
+  T a = mx(ab(unL) + cL, ab(unR) + cR);
+  S6<T> D = scale(sub(R, L), T(0.5 * a));
+  S6<T> A = scale(add(FL, FR), T(0.5));
+  return sub(A, D);
+}
+
+template <class T> HD S6<T> flux_roe(const Phys& P, const S6<T>& L, const S6<T>& R, double nx, double ny) {
+  const double g = P.gamma;
+  T pL = pressure(P, L);
+  T uL = L[1] / L[0], vL = L[2] / L[0];
+  T hL = (L[3] + pL) / L[0];
+  T pR = pressure(P, R);
+  T uR = R[1] / R[0], vR = R[2] / R[0];
+  T hR = (R[3] + pR) / R[0];
+  T sL = sq_rt(L[0]), sR = sq_rt(R[0]);
+  T inv = 1.0 / (sL + sR);
+  T ut = (sL * uL + sR * uR) * inv;
+  T vt = (sL * vL + sR * vR) * inv;
+  T ht = (sL * hL + sR * hR) * inv;
+  T V2 = ut * ut + vt * vt;
+  T c2 = (g - 1.0) * (ht - 0.5 * V2);
+  T ct = sq_rt(mx(c2, T(1e-8)));
+  T unt = ut * nx + vt * ny;
+  T utt = -ut * ny + vt * nx;
+  T lam[4] = {unt - ct, unt, unt, unt + ct};
+  T eps = 0.1 * ct;
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+    if (val(ab(lam[i])) < val(eps)) lam[i] = 0.5 * (lam[i] * lam[i] / eps + eps);
+    lam[i] = ab(lam[i]);
+  }
+  T drho = R[0] - L[0], du = uR - uL, dv = vR - vL, dp = pR - pL;
+  T dun = du * nx + dv * ny;
+  T dut = -du * ny + dv * nx;
+  T rt = sL * sR;
+  T a0 = 0.5 / c2 * (dp - rt * ct * dun);
+  T a1 = drho - dp / c2;
+  T a2 = rt * dut;
+  T a3 = 0.5 / c2 * (dp + rt * ct * dun);
+  T w1 = lam[0] * a0, w2 = lam[1] * a1, w3 = lam[2] * a2, w4 = lam[3] * a3;
+  S6<T> D;
+  D[0] = w1 * 1.0 + w2 * 1.0 + w3 * 0.0 + w4 * 1.0;
+  D[1] = w1 * (ut - ct * nx) + w2 * ut + w3 * (-ny) + w4 * (ut + ct * nx);
+  D[2] = w1 * (vt - ct * ny) + w2 * vt + w3 * nx + w4 * (vt + ct * ny);
+  D[3] = w1 * (ht - unt * ct) + w2 * (0.5 * V2) + w3 * utt + w4 * (ht + unt * ct);
+  D[4] = ab(unt) * (R[4] / R[0] - L[4] / L[0]) * rt;
+  D[5] = ab(unt) * (R[5] / R[0] - L[5] / L[0]) * rt;
+  S6<T> FL = normal_flux(P, L, nx, ny), FR = normal_flux(P, R, nx, ny);
+  return sub(scale(add(FL, FR), T(0.5)), scale(D, T(0.5)));
+}
+
+template <class T> HD S6<T> riemann(const Phys& P, const S6<T>& L, const S6<T>& R, double nx, double ny, int scheme) {
+  if (scheme == CFD_FLUX_ROE) return flux_roe(P, L, R, nx, ny);
+  if (scheme == CFD_FLUX_HLLC) return flux_hllc(P, L, R, nx, ny);
+  return flux_llf(P, L, R, nx, ny);  // LLF, and AUSM falls back to LLF (Flux.hpp:224-225)
+}
+
+}  // namespace cfd

@@ -1,0 +1,1069 @@
+// ORACLE — test infrastructure only.
+//
+// CPU restatement of the reference solver's residual + JFNK-GMRES path
+// (AmberMond/mycode).  It is the checker for the HIP library: only tests/,
+// __graft_entry__.smoke() and bench.py's cpu_baseline leg load it, and never
+// as the thing measured or shipped.
+//
+// Every routine keeps the reference's operation order, so in double it
+// reproduces the reference bit for bit (pinned against the compiled reference
+// in oracle/_ref and against the committed fixtures in tests/golden).  The
+// scalar type is a template parameter: double for parity,
+// std::complex<double> for complex-step derivative checks of the tangent.
+//
+// Routines the reference does not contain are marked "extension": the viscous
+// flux with the boundary-face read defined (the reference reads cells[-1]
+// there, Solver.cpp:729-730), the extra boundary-condition sets the reference
+// keeps commented out, per-face flux selection, SSP-RK3 and the tag-6 shock
+// boundary.  They are pinned by closed forms and invariants in tests/.
+#include <algorithm>
+#include <array>
+#include <cmath>
+#include <complex>
+#include <cstdint>
+#include <cstdio>
+#include <cstring>
+#include <fstream>
+#include <map>
+#include <sstream>
+#include <string>
+#include <utility>
+#include <vector>
+
+#include "cfd_oracle.h"
+
+namespace orc {
+
+using cplx = std::complex<double>;
+
+// ---- scalar helpers: comparisons on the real part, so the same code runs
+// under complex-step -----------------------------------------------------------
+inline double re(double x) { return x; }
+inline double re(const cplx& x) { return x.real(); }
+template <class T> inline T sabs(const T& x) { return re(x) < 0.0 ? T(-x) : x; }
+// std::min(a, b) yields a unless b < a; std::max(a, b) yields a unless a < b.
+template <class T> inline T smin(const T& a, const T& b) { return re(b) < re(a) ? b : a; }
+template <class T> inline T smax(const T& a, const T& b) { return re(a) < re(b) ? b : a; }
+inline double ssqrt(double x) { return std::sqrt(x); }
+inline cplx ssqrt(const cplx& x) {
+  double r = std::sqrt(x.real());
+  return cplx(r, x.imag() / (2.0 * r));
+}
+inline double spow(double a, double b) { return std::pow(a, b); }
+inline cplx spow(const cplx& a, double b) {
+  double r = std::pow(a.real(), b);
+  return cplx(r, b * std::pow(a.real(), b - 1.0) * a.imag());
+}
+
+// ---- State (Types.hpp:55-82) -------------------------------------------------
+template <class T> struct St {
+  T q[6];
+  St() { for (auto& v : q) v = T(0); }
+  St(T a, T b, T c, T d, T e = T(0), T f = T(0)) { q[0] = a; q[1] = b; q[2] = c; q[3] = d; q[4] = e; q[5] = f; }
+  St operator+(const St& o) const { St s; for (int i = 0; i < 6; ++i) s.q[i] = q[i] + o.q[i]; return s; }
+  St operator-(const St& o) const { St s; for (int i = 0; i < 6; ++i) s.q[i] = q[i] - o.q[i]; return s; }
+  St operator*(T a) const { St s; for (int i = 0; i < 6; ++i) s.q[i] = q[i] * a; return s; }
+  T& operator[](int i) { return q[i]; }
+  const T& operator[](int i) const { return q[i]; }
+};
+
+// ---- Mesh (Mesh.cpp) -----------------------------------------------------------
+struct Topo {
+  std::vector<double> x, y;  // index 0 unused: Gmsh node ids start at 1
+  std::vector<int> cptr{0}, cnodes;
+  std::map<std::pair<int, int>, int> bedges;
+  std::vector<int> n1, n2, own, nei, tag;  // faces
+  int ncells() const { return (int)cptr.size() - 1; }
+  int nfaces() const { return (int)own.size(); }
+};
+
+template <class T> struct Geo {
+  std::vector<T> vol, xc, yc;
+  std::vector<T> nx, ny, area, xm, ym;
+};
+
+// Mesh::readGmsh (Mesh.cpp:30-97): nodes, then element types 1 (boundary
+// line -> tag of the sorted node pair), 2 (triangle), 3 (quad); any other
+// element's remaining fields are skipped to the end of its line.
+static bool read_gmsh(const char* path, Topo& t) {
+  std::ifstream in(path);
+  if (!in.is_open()) return false;
+  std::string line;
+  while (std::getline(in, line)) {
+    if (!line.empty() && line.back() == '\r') line.pop_back();  // CRLF files (Windows text mode)
+    if (line == "$Nodes") {
+      int nn = 0;
+      in >> nn;
+      t.x.assign(nn + 1, 0.0);
+      t.y.assign(nn + 1, 0.0);
+      for (int i = 1; i <= nn; ++i) {
+        int id;
+        double z;
+        in >> id >> t.x[i] >> t.y[i] >> z;
+      }
+      break;
+    }
+  }
+  while (std::getline(in, line)) {
+    if (!line.empty() && line.back() == '\r') line.pop_back();  // CRLF files (Windows text mode)
+    if (line == "$Elements") {
+      int ne = 0;
+      in >> ne;
+      for (int e = 0; e < ne; ++e) {
+        int id, type, ntags, tg, phys = 0;
+        in >> id >> type >> ntags;
+        for (int j = 0; j < ntags; ++j) {
+          in >> tg;
+          if (j == 0) phys = tg;
+        }
+        if (type == 1) {
+          int a, b;
+          in >> a >> b;
+          t.bedges[{std::min(a, b), std::max(a, b)}] = phys;
+        } else if (type == 2 || type == 3) {
+          int nv = type == 2 ? 3 : 4;
+          for (int k = 0; k < nv; ++k) {
+            int nd;
+            in >> nd;
+            t.cnodes.push_back(nd);
+          }
+          t.cptr.push_back((int)t.cnodes.size());
+        }
+        std::getline(in, line);
+      }
+      break;
+    }
+  }
+  return true;
+}
+
+// Mesh::buildConnectivity (Mesh.cpp:102-143): edges walked cell by cell, edge
+// j = (n_j, n_{j+1 mod n}); first visit creates the face (owner = visiting
+// cell, nodes stored sorted, tag from the boundary map else 0), second visit
+// sets the neighbor.
+static void build_connectivity(Topo& t) {
+  std::map<std::pair<int, int>, int> e2f;
+  for (int c = 0; c < t.ncells(); ++c) {
+    int b = t.cptr[c], n = t.cptr[c + 1] - b;
+    for (int j = 0; j < n; ++j) {
+      int a1 = t.cnodes[b + j], a2 = t.cnodes[b + (j + 1) % n];
+      std::pair<int, int> key{std::min(a1, a2), std::max(a1, a2)};
+      auto it = e2f.find(key);
+      if (it != e2f.end()) {
+        t.nei[it->second] = c;
+      } else {
+        t.n1.push_back(key.first);
+        t.n2.push_back(key.second);
+        t.own.push_back(c);
+        t.nei.push_back(-1);
+        auto bt = t.bedges.find(key);
+        t.tag.push_back(bt != t.bedges.end() ? bt->second : 0);
+        e2f[key] = (int)t.own.size() - 1;
+      }
+    }
+  }
+}
+
+// Mesh::computeGeometry (Mesh.cpp:148-202): triangle area from the cross
+// product and vertex-mean centroid; quad vertex-mean centroid and shoelace
+// area in node order; face length, midpoint and unit normal (dy, -dx)/A
+// flipped to point away from the owner centroid.
+template <class T>
+static void compute_geometry(const Topo& t, const std::vector<T>& X, const std::vector<T>& Y, Geo<T>& g) {
+  int N = t.ncells(), F = t.nfaces();
+  g.vol.assign(N, T(0)); g.xc.assign(N, T(0)); g.yc.assign(N, T(0));
+  for (int c = 0; c < N; ++c) {
+    const int* nd = &t.cnodes[t.cptr[c]];
+    int n = t.cptr[c + 1] - t.cptr[c];
+    if (n == 3) {
+      T x1 = X[nd[0]], y1 = Y[nd[0]], x2 = X[nd[1]], y2 = Y[nd[1]], x3 = X[nd[2]], y3 = Y[nd[2]];
+      g.vol[c] = 0.5 * sabs((x2 - x1) * (y3 - y1) - (x3 - x1) * (y2 - y1));
+      g.xc[c] = (x1 + x2 + x3) / 3.0;
+      g.yc[c] = (y1 + y2 + y3) / 3.0;
+    } else if (n == 4) {
+      T sx = 0, sy = 0;
+      for (int k = 0; k < 4; ++k) { sx += X[nd[k]]; sy += Y[nd[k]]; }
+      g.xc[c] = sx / 4.0;
+      g.yc[c] = sy / 4.0;
+      T a = 0;
+      for (int k = 0; k < 4; ++k) {
+        int p = nd[k], q = nd[(k + 1) % 4];
+        a += (X[p] * Y[q] - X[q] * Y[p]);
+      }
+      g.vol[c] = 0.5 * sabs(a);
+    }
+  }
+  g.nx.assign(F, T(0)); g.ny.assign(F, T(0)); g.area.assign(F, T(0)); g.xm.assign(F, T(0)); g.ym.assign(F, T(0));
+  for (int f = 0; f < F; ++f) {
+    T dx = X[t.n2[f]] - X[t.n1[f]];
+    T dy = Y[t.n2[f]] - Y[t.n1[f]];
+    T A = ssqrt(dx * dx + dy * dy);
+    g.area[f] = A;
+    g.xm[f] = 0.5 * (X[t.n1[f]] + X[t.n2[f]]);
+    g.ym[f] = 0.5 * (Y[t.n1[f]] + Y[t.n2[f]]);
+    T nx = dy / A, ny = -dx / A;
+    T vx = g.xm[f] - g.xc[t.own[f]], vy = g.ym[f] - g.yc[t.own[f]];
+    if (re(vx * nx + vy * ny) < 0) { nx = -nx; ny = -ny; }
+    g.nx[f] = nx;
+    g.ny[f] = ny;
+  }
+}
+
+// ---- the discretisation ---------------------------------------------------------
+template <class T> struct Disc {
+  const Topo* topo = nullptr;
+  const Geo<T>* geo = nullptr;
+  cfd_params P{};
+  const unsigned char* face_alt = nullptr;  // extension: per-face flux choice
+  std::vector<std::array<T, 10>> grads;     // drhodx drhody dudx dudy dvdx dvdy dpdx dpdy dTdx dTdy
+  std::vector<T> limiters;
+
+  // Flux.hpp:12-15 (GAMMA of Flux.hpp:9 equals Solver::gamma, 1.4)
+  T pressure(const St<T>& U) const {
+    T V2 = (U[1] * U[1] + U[2] * U[2]) / (U[0] * U[0]);
+    return (P.gamma - 1.0) * (U[3] - 0.5 * U[0] * V2);
+  }
+  // Solver::primitiveToConservative (Solver.cpp:498-501)
+  St<T> p2c(T rho, T u, T v, T p, T k = T(0), T w = T(0)) const {
+    T rE = p / (P.gamma - 1.0) + 0.5 * rho * (u * u + v * v);
+    return St<T>(rho, rho * u, rho * v, rE, rho * k, rho * w);
+  }
+  // frozen freestream of Solver.cpp:979-985 / 1094-1100
+  St<T> freestream() const {
+    double g = P.gamma;
+    double fT = 1.0 + 0.5 * (g - 1.0) * P.M_inf * P.M_inf;
+    double Ts = P.T_total_inlet / fT;
+    double rho = P.P_static_exit / (P.R_gas * Ts);
+    double c = std::sqrt(g * P.R_gas * Ts);
+    double u = P.M_inf * c;
+    return p2c(T(rho), T(u), T(0.0), T(P.P_static_exit));
+  }
+  St<T> slip(const St<T>& U, int f) const {  // tag 3 everywhere (e.g. Solver.cpp:845-860)
+    T u = U[1] / U[0], v = U[2] / U[0];
+    T nx = geo->nx[f], ny = geo->ny[f];
+    T vn = u * nx + v * ny;
+    T ug = u - 2.0 * vn * nx, vg = v - 2.0 * vn * ny;
+    St<T> G = U;
+    G[1] = U[0] * ug;
+    G[2] = U[0] * vg;
+    return G;
+  }
+  // Solver::getBoundaryGhostState; the set is chosen by P.bcset
+  St<T> ghost(const St<T>& U, int f) const {
+    int tg = topo->tag[f];
+    double g = P.gamma, Rg = P.R_gas;
+    switch (P.bcset) {
+      case CFD_BC_BUMP:  // Solver.cpp:806-863
+        if (tg == 1) {
+          T pin = pressure(U);
+          T puse = smin(pin, T(P.P_total_inlet - 1e-4));
+          T Tg = P.T_total_inlet * spow(puse / P.P_total_inlet, (g - 1.0) / g);
+          T rg = puse / (Rg * Tg);
+          double Cp = g * Rg / (g - 1.0);
+          T V2 = 2.0 * Cp * (P.T_total_inlet - Tg);
+          T V = re(V2) > 0 ? ssqrt(V2) : T(0.0);
+          return p2c(rg, V, T(0.0), puse);
+        } else if (tg == 2) {
+          T rho = U[0], u = U[1] / U[0], v = U[2] / U[0];
+          return p2c(rho, u, v, T(P.P_static_exit));
+        } else if (tg == 3) {
+          return slip(U, f);
+        }
+        return U;
+      case CFD_BC_SUPERSONIC:  // Solver.cpp:973-1011
+        if (tg == 1) return freestream();
+        if (tg == 2) return U;
+        if (tg == 3) return slip(U, f);
+        return U;
+      case CFD_BC_PLATE:  // Solver.cpp:1088-1139
+      case CFD_BC_SWBLI:  // extension: tag 6 = frozen post-shock state
+        if (tg == 1 || tg == 5) return freestream();
+        if (tg == 2) return U;
+        if (tg == 3) return slip(U, f);
+        if (tg == 4) {
+          T ug = -(U[1] / U[0]), vg = -(U[2] / U[0]);
+          T ek = 0.5 * (U[1] * U[1] + U[2] * U[2]) / U[0];
+          T pin = (g - 1.0) * (U[3] - ek);
+          T Tin = pin / (U[0] * Rg);
+          T rg = pin / (Rg * Tin);
+          return p2c(rg, ug, vg, pin, T(0.0), T(0.0));
+        }
+        if (tg == 6 && P.bcset == CFD_BC_SWBLI)
+          return p2c(T(P.shock_rho), T(P.shock_u), T(P.shock_v), T(P.shock_p));
+        return U;
+    }
+    return U;
+  }
+
+  // Flux.hpp:23-35
+  St<T> normal_flux(const St<T>& U, T nx, T ny) const {
+    T p = pressure(U);
+    T u = U[1] / U[0], v = U[2] / U[0];
+    T un = u * nx + v * ny;
+    return St<T>(U[0] * un, U[1] * un + p * nx, U[2] * un + p * ny, (U[3] + p) * un);
+  }
+  // Flux.hpp:36-74 (HLLC with Roe-averaged wave-speed bounds)
+  St<T> flux_hllc(const St<T>& L, const St<T>& R, T nx, T ny) const {
+    double g = P.gamma;
+    T pL = pressure(L), pR = pressure(R);
+    T uL = L[1] / L[0], uR = R[1] / R[0];
+    T vL = L[2] / L[0], vR = R[2] / R[0];
+    T unL = uL * nx + vL * ny, unR = uR * nx + vR * ny;
+    T cL = ssqrt(g * pL / L[0]), cR = ssqrt(g * pR / R[0]);
+    T hL = (L[3] + pL) / L[0], hR = (R[3] + pR) / R[0];
+    T sL = ssqrt(L[0]), sR = ssqrt(R[0]);
+    T ut = (sL * uL + sR * uR) / (sL + sR);
+    T vt = (sL * vL + sR * vR) / (sL + sR);
+    T ht = (sL * hL + sR * hR) / (sL + sR);
+    T unt = ut * nx + vt * ny;
+    T ct = ssqrt((g - 1.0) * (ht - 0.5 * (ut * ut + vt * vt)));
+    T SL = smin(unL - cL, unt - ct);
+    T SR = smax(unR + cR, unt + ct);
+    T Ss = (pR - pL + L[0] * unL * (SL - unL) - R[0] * unR * (SR - unR)) /
+           (L[0] * (SL - unL) - R[0] * (SR - unR));
+    if (re(SL) >= 0.0) return normal_flux(L, nx, ny);
+    if (re(SR) <= 0.0) return normal_flux(R, nx, ny);
+    if (re(Ss) >= 0.0) {
+      T coef = L[0] * (SL - unL) / (SL - Ss);
+      St<T> Us(coef, coef * (uL + (Ss - unL) * nx), coef * (vL + (Ss - unL) * ny),
+               coef * (L[3] / L[0] + (Ss - unL) * (Ss + pL / (L[0] * (SL - unL)))));
+      return normal_flux(L, nx, ny) + (Us - L) * SL;
+    }
+    T coef = R[0] * (SR - unR) / (SR - Ss);
+    St<T> Us(coef, coef * (uR + (Ss - unR) * nx), coef * (vR + (Ss - unR) * ny),
+             coef * (R[3] / R[0] + (Ss - unR) * (Ss + pR / (R[0] * (SR - unR)))));
+    return normal_flux(R, nx, ny) + (Us - R) * SR;
+  }
+  // Flux.hpp:81-110 (local Lax-Friedrichs / Rusanov)
+  St<T> flux_llf(const St<T>& L, const St<T>& R, T nx, T ny) const {
+    double g = P.gamma;
+    St<T> FL = normal_flux(L, nx, ny), FR = normal_flux(R, nx, ny);
+    T pL = pressure(L);
+    T uL = L[1] / L[0], vL = L[2] / L[0];
+    T unL = uL * nx + vL * ny;
+    T cL = ssqrt(g * pL / L[0]);
+    T pR = pressure(R);
+    T uR = R[1] / R[0], vR = R[2] / R[0];
+    T unR = uR * nx + vR * ny;
+    T cR = ssqrt(g * pR / R[0]);
+    T a = smax(sabs(unL) + cL, sabs(unR) + cR);
+    St<T> D = (R - L) * (0.5 * a);
+    St<T> A = (FL + FR) * T(0.5);
+    return A - D;
+  }
+  // Flux.hpp:112-213 (Roe with Harten's entropy fix, eps = 0.1 c~)
+  St<T> flux_roe(const St<T>& L, const St<T>& R, T nx, T ny) const {
+    double g = P.gamma;
+    T pL = pressure(L);
+    T uL = L[1] / L[0], vL = L[2] / L[0];
+    T hL = (L[3] + pL) / L[0];
+    T pR = pressure(R);
+    T uR = R[1] / R[0], vR = R[2] / R[0];
+    T hR = (R[3] + pR) / R[0];
+    T sL = ssqrt(L[0]), sR = ssqrt(R[0]);
+    T inv = 1.0 / (sL + sR);
+    T ut = (sL * uL + sR * uR) * inv;
+    T vt = (sL * vL + sR * vR) * inv;
+    T ht = (sL * hL + sR * hR) * inv;
+    T V2 = ut * ut + vt * vt;
+    T c2 = (g - 1.0) * (ht - 0.5 * V2);
+    T ct = ssqrt(smax(c2, T(1e-8)));
+    T unt = ut * nx + vt * ny;
+    T utt = -ut * ny + vt * nx;
+    T lam[4] = {unt - ct, unt, unt, unt + ct};
+    T eps = 0.1 * ct;
+    for (int i = 0; i < 4; ++i) {
+      if (re(sabs(lam[i])) < re(eps)) lam[i] = 0.5 * (lam[i] * lam[i] / eps + eps);
+      lam[i] = sabs(lam[i]);
+    }
+    T drho = R[0] - L[0], du = uR - uL, dv = vR - vL, dp = pR - pL;
+    T dun = du * nx + dv * ny;
+    T dut = -du * ny + dv * nx;
+    T rt = sL * sR;
+    T al0 = 0.5 / c2 * (dp - rt * ct * dun);
+    T al1 = drho - dp / c2;
+    T al2 = rt * dut;
+    T al3 = 0.5 / c2 * (dp + rt * ct * dun);
+    T w1 = lam[0] * al0, w2 = lam[1] * al1, w3 = lam[2] * al2, w4 = lam[3] * al3;
+    St<T> D;
+    D[0] = w1 * 1.0 + w2 * 1.0 + w3 * 0.0 + w4 * 1.0;
+    D[1] = w1 * (ut - ct * nx) + w2 * ut + w3 * (-ny) + w4 * (ut + ct * nx);
+    D[2] = w1 * (vt - ct * ny) + w2 * vt + w3 * nx + w4 * (vt + ct * ny);
+    D[3] = w1 * (ht - unt * ct) + w2 * (0.5 * V2) + w3 * utt + w4 * (ht + unt * ct);
+    D[4] = sabs(unt) * (R[4] / R[0] - L[4] / L[0]) * rt;
+    D[5] = sabs(unt) * (R[5] / R[0] - L[5] / L[0]) * rt;
+    St<T> FL = normal_flux(L, nx, ny), FR = normal_flux(R, nx, ny);
+    return (FL + FR) * T(0.5) - (D * T(0.5));
+  }
+  // Flux.hpp:216-228
+  St<T> flux(const St<T>& L, const St<T>& R, T nx, T ny, int scheme) const {
+    switch (scheme) {
+      case CFD_FLUX_LLF: return flux_llf(L, R, nx, ny);
+      case CFD_FLUX_ROE: return flux_roe(L, R, nx, ny);
+      case CFD_FLUX_HLLC: return flux_hllc(L, R, nx, ny);
+      default: return flux_llf(L, R, nx, ny);
+    }
+  }
+
+  // Solver::computeGradients (Solver.cpp:81-149): Green-Gauss with the
+  // arithmetic face mean of rho, p, u, v, T; owner += phi n A, neighbor -= .
+  void gradients(const St<T>* U) {
+    const Topo& t = *topo;
+    const Geo<T>& G = *geo;
+    int N = t.ncells(), F = t.nfaces();
+    double Rg = P.R_gas;
+    grads.assign(N, std::array<T, 10>{});
+    for (auto& a : grads) a.fill(T(0));
+    for (int f = 0; f < F; ++f) {
+      int o = t.own[f], n = t.nei[f];
+      St<T> L = U[o];
+      St<T> R = n >= 0 ? U[n] : ghost(L, f);
+      T uL = L[1] / L[0], vL = L[2] / L[0], pL = pressure(L), TL = pL / (L[0] * Rg);
+      T uR = R[1] / R[0], vR = R[2] / R[0], pR = pressure(R), TR = pR / (R[0] * Rg);
+      T phi[5] = {0.5 * (L[0] + R[0]), 0.5 * (uL + uR), 0.5 * (vL + vR), 0.5 * (pL + pR), 0.5 * (TL + TR)};
+      T nxA_ = G.nx[f], nyA_ = G.ny[f], A = G.area[f];
+      for (int k = 0; k < 5; ++k) {
+        grads[o][2 * k] += phi[k] * nxA_ * A;
+        grads[o][2 * k + 1] += phi[k] * nyA_ * A;
+      }
+      if (n >= 0)
+        for (int k = 0; k < 5; ++k) {
+          grads[n][2 * k] -= phi[k] * nxA_ * A;
+          grads[n][2 * k + 1] -= phi[k] * nyA_ * A;
+        }
+    }
+    for (int c = 0; c < N; ++c)
+      for (int k = 0; k < 10; ++k) grads[c][k] /= G.vol[c];
+  }
+  // slot order of the ten gradients: rho(0,1) u(2,3) v(4,5) p(6,7) T(8,9)
+
+  // Solver::computeLimiters (Solver.cpp:152-253): Barth-Jespersen on pressure,
+  // with the unlimited face pressure rebuilt from the temperature gradient at
+  // frozen density.
+  void limiter(const St<T>* U) {
+    const Topo& t = *topo;
+    const Geo<T>& G = *geo;
+    int N = t.ncells(), F = t.nfaces();
+    double Rg = P.R_gas;
+    limiters.assign(N, T(1.0));
+    std::vector<T> Pc(N), Pmax(N), Pmin(N);
+    for (int c = 0; c < N; ++c) Pc[c] = pressure(U[c]);
+    Pmax = Pc;
+    Pmin = Pc;
+    for (int f = 0; f < F; ++f) {
+      int o = t.own[f], n = t.nei[f];
+      T pg = n >= 0 ? Pc[n] : pressure(ghost(U[o], f));
+      Pmax[o] = smax(Pmax[o], pg);
+      Pmin[o] = smin(Pmin[o], pg);
+      if (n >= 0) {
+        Pmax[n] = smax(Pmax[n], Pc[o]);
+        Pmin[n] = smin(Pmin[n], Pc[o]);
+      }
+    }
+    const double eps = 1e-12;
+    auto side = [&](int c, int f) {
+      T dx = G.xm[f] - G.xc[c], dy = G.ym[f] - G.yc[c];
+      T Tc = Pc[c] / (U[c][0] * Rg);
+      T Tf = Tc + 1.0 * (grads[c][8] * dx + grads[c][9] * dy);
+      T Pf = U[c][0] * Rg * Tf;
+      T d = Pf - Pc[c];
+      T a = T(1.0);
+      if (re(d) > 0.0) a = smin(T(1.0), (Pmax[c] - Pc[c]) / (d + eps));
+      else if (re(d) < 0.0) a = smin(T(1.0), (Pmin[c] - Pc[c]) / (d - eps));
+      limiters[c] = smin(limiters[c], a);
+    };
+    for (int f = 0; f < F; ++f) {
+      side(t.own[f], f);
+      if (t.nei[f] >= 0) side(t.nei[f], f);
+    }
+  }
+
+  // Solver::computeResiduals (Solver.cpp:529-801)
+  void residual(const St<T>* U, St<T>* Rout, bool update_limiters, St<T>* face_out = nullptr) {
+    const Topo& t = *topo;
+    const Geo<T>& G = *geo;
+    int N = t.ncells(), F = t.nfaces();
+    double g = P.gamma, Rg = P.R_gas;
+    for (int c = 0; c < N; ++c) Rout[c] = St<T>();
+    gradients(U);
+    if (P.spatial == CFD_SECOND_ORDER_LIMITED) {
+      if (update_limiters || (int)limiters.size() != N) limiter(U);
+    }
+    double Cp = g * Rg / (g - 1.0);
+    double Pr = P.Pr;
+    for (int f = 0; f < F; ++f) {
+      int o = t.own[f], n = t.nei[f], tg = t.tag[f];
+      T nx = G.nx[f], ny = G.ny[f];
+      St<T> L = U[o];
+      St<T> R = n >= 0 ? U[n] : ghost(L, f);
+      St<T> Lr, Rr;
+      if (P.spatial == CFD_FIRST_ORDER || n < 0) {
+        Lr = L;
+        Rr = R;
+      } else {
+        T dxL = G.xm[f] - G.xc[o], dyL = G.ym[f] - G.yc[o];
+        T dxR = G.xm[f] - G.xc[n], dyR = G.ym[f] - G.yc[n];
+        const auto& gL = grads[o];
+        const auto& gR = grads[n];
+        bool lim = P.spatial == CFD_SECOND_ORDER_LIMITED;
+        T phL = lim ? limiters[o] : T(0.5);
+        T phR = lim ? limiters[n] : T(0.5);
+        T uL = L[1] / L[0], vL = L[2] / L[0];
+        T pL = pressure(L);
+        T rLf = L[0] + phL * (gL[0] * dxL + gL[1] * dyL);
+        T pLf = pL + phL * (gL[6] * dxL + gL[7] * dyL);
+        T uLf = uL + phL * (gL[2] * dxL + gL[3] * dyL);
+        T vLf = vL + phL * (gL[4] * dxL + gL[5] * dyL);
+        if (re(rLf) < 1e-4) rLf = L[0];
+        if (re(pLf) < 1e-4) pLf = pL;
+        Lr = p2c(rLf, uLf, vLf, pLf);
+        T uR = R[1] / R[0], vR = R[2] / R[0];
+        T pR = pressure(R);
+        T rRf = R[0] + phR * (gR[0] * dxR + gR[1] * dyR);
+        T pRf = pR + phR * (gR[6] * dxR + gR[7] * dyR);
+        T uRf = uR + phR * (gR[2] * dxR + gR[3] * dyR);
+        T vRf = vR + phR * (gR[4] * dxR + gR[5] * dyR);
+        if (re(rRf) < 1e-4) rRf = R[0];
+        if (re(pRf) < 1e-4) pRf = pR;
+        Rr = p2c(rRf, uRf, vRf, pRf);
+      }
+      St<T> FE;
+      if (n < 0 && (tg == 3 || tg == 4)) {
+        T pw = pressure(Lr);
+        FE[0] = 0.0;
+        FE[1] = pw * nx;
+        FE[2] = pw * ny;
+        FE[3] = 0.0;
+      } else {
+        int scheme = (face_alt && face_alt[f]) ? P.flux_alt : P.flux;
+        FE = flux(Lr, Rr, nx, ny, scheme);
+      }
+      St<T> Tot = FE;
+      if (P.viscous) {
+        // Solver.cpp:685-787 with the boundary-face branch defined: faces
+        // without a neighbor skip the centroid-line correction.
+        T rf = 0.5 * (L[0] + R[0]);
+        T uf = 0.5 * (L[1] / L[0] + R[1] / R[0]);
+        T vf = 0.5 * (L[2] / L[0] + R[2] / R[0]);
+        const auto& gL = grads[o];
+        const auto& gR = n >= 0 ? grads[n] : grads[o];
+        T ux, uy, vx, vy, Tx, Ty;
+        if (n < 0 && tg == 4) {
+          T dx = G.xm[f] - G.xc[o], dy = G.ym[f] - G.yc[o];
+          T d2 = dx * dx + dy * dy;
+          T uc = L[1] / L[0], vc = L[2] / L[0];
+          ux = -uc * dx / d2;
+          uy = -uc * dy / d2;
+          vx = -vc * dx / d2;
+          vy = -vc * dy / d2;
+          Tx = 0.0;
+          Ty = 0.0;
+        } else {
+          ux = 0.5 * (gL[2] + gR[2]);
+          uy = 0.5 * (gL[3] + gR[3]);
+          vx = 0.5 * (gL[4] + gR[4]);
+          vy = 0.5 * (gL[5] + gR[5]);
+          Tx = 0.5 * (gL[8] + gR[8]);
+          Ty = 0.5 * (gL[9] + gR[9]);
+          if (n >= 0) {
+            T dxLR = G.xc[n] - G.xc[o], dyLR = G.yc[n] - G.yc[o];
+            T d2 = dxLR * dxLR + dyLR * dyLR;
+            if (re(d2) > 1e-14) {
+              T du = (R[1] / R[0]) - (L[1] / L[0]);
+              T dv = (R[2] / R[0]) - (L[2] / L[0]);
+              T ux0 = ux, uy0 = uy, vx0 = vx, vy0 = vy;
+              ux = ux0 - (ux0 * dxLR + uy0 * dyLR - du) * dxLR / d2;
+              uy = uy0 - (ux0 * dxLR + uy0 * dyLR - du) * dyLR / d2;
+              vx = vx0 - (vx0 * dxLR + vy0 * dyLR - dv) * dxLR / d2;
+              vy = vy0 - (vx0 * dxLR + vy0 * dyLR - dv) * dyLR / d2;
+            }
+          }
+        }
+        T mu = rf * P.nu;
+        T k = mu * Cp / Pr;
+        T div = ux + vy;
+        T txx = mu * (2.0 * ux - (2.0 / 3.0) * div);
+        T tyy = mu * (2.0 * vy - (2.0 / 3.0) * div);
+        T txy = mu * (uy + vx);
+        T qx = -k * Tx, qy = -k * Ty;
+        St<T> FV;
+        FV[0] = 0.0;
+        FV[1] = txx * nx + txy * ny;
+        FV[2] = txy * nx + tyy * ny;
+        T wx = uf * txx + vf * txy - qx;
+        T wy = uf * txy + vf * tyy - qy;
+        FV[3] = wx * nx + wy * ny;
+        Tot = FE - FV;
+      }
+      T A = G.area[f];
+      if (face_out) face_out[f] = Tot * A;
+      Rout[o] = Rout[o] + Tot * A;
+      if (n >= 0) Rout[n] = Rout[n] - Tot * A;
+    }
+  }
+};
+
+// ---- solver object: members the reference keeps in Solver ------------------
+struct Solver {
+  Topo topo;
+  Geo<double> geo;
+  cfd_params P{};
+  Disc<double> D;
+  std::vector<double> dt;
+  std::vector<unsigned char> face_alt;
+  int N() const { return topo.ncells(); }
+
+  void bind() {
+    D.topo = &topo;
+    D.geo = &geo;
+    D.P = P;
+    D.face_alt = face_alt.empty() ? nullptr : face_alt.data();
+  }
+  static const St<double>* S(const double* p) { return reinterpret_cast<const St<double>*>(p); }
+  static St<double>* S(double* p) { return reinterpret_cast<St<double>*>(p); }
+
+  // Solver::computeLocalTimeStep (Solver.cpp:1388-1422), U_current = U
+  void local_timestep(const St<double>* U, double cfl) {
+    int n = N();
+    std::vector<double> lam(n, 1e-12);
+    double g = P.gamma;
+    for (int f = 0; f < topo.nfaces(); ++f) {
+      int o = topo.own[f], nb = topo.nei[f];
+      const St<double>& L = U[o];
+      double pL = D.pressure(L);
+      double cL = std::sqrt(g * pL / L[0]);
+      double unL = (L[1] * geo.nx[f] + L[2] * geo.ny[f]) / L[0];
+      lam[o] += (std::abs(unL) + cL) * geo.area[f];
+      if (nb >= 0) {
+        const St<double>& R = U[nb];
+        double pR = D.pressure(R);
+        double cR = std::sqrt(g * pR / R[0]);
+        double unR = (R[1] * (-geo.nx[f]) + R[2] * (-geo.ny[f])) / R[0];
+        lam[nb] += (std::abs(unR) + cR) * geo.area[f];
+      }
+    }
+    dt.resize(n);
+    for (int c = 0; c < n; ++c) dt[c] = cfl * geo.vol[c] / lam[c];
+  }
+  // Solver::computeDiagonalPreconditioner (Solver.cpp:1424-1465)
+  void diag_precond(const St<double>* U, std::vector<double>& Dg) {
+    int n = N();
+    double g = P.gamma;
+    Dg.assign(n, 0.0);
+    for (int f = 0; f < topo.nfaces(); ++f) {
+      int o = topo.own[f], nb = topo.nei[f];
+      St<double> L = U[o];
+      St<double> R = nb >= 0 ? U[nb] : D.ghost(L, f);
+      double cL = std::sqrt(g * D.pressure(L) / L[0]);
+      double cR = std::sqrt(g * (D.pressure(R) > 0 ? D.pressure(R) : 1e-3) / (R[0] > 0 ? R[0] : 1e-3));
+      double cf = 0.5 * (cL + cR);
+      double uf = 0.5 * (L[1] / L[0] + R[1] / R[0]);
+      double vf = 0.5 * (L[2] / L[0] + R[2] / R[0]);
+      double vn = std::abs(uf * geo.nx[f] + vf * geo.ny[f]);
+      double li = vn + cf;
+      double dx = std::sqrt(geo.vol[o]);
+      double mu = 0.5 * (L[0] + R[0]) * P.nu;
+      double lv = 2.0 * mu / (0.5 * (L[0] + R[0]) * dx + 1e-12);
+      double lt = (li + lv) * geo.area[f];
+      Dg[o] += lt * 0.5;
+      if (nb >= 0) Dg[nb] += lt * 0.5;
+    }
+    for (int c = 0; c < n; ++c) {
+      Dg[c] = (geo.vol[c] / (dt[c] + 1e-30)) + Dg[c];
+      if (Dg[c] < 1e-12) Dg[c] = 1e-12;
+    }
+  }
+  // Solver::norm_L2 / dot_product / norm_L2_VolumeWeighted (Solver.cpp:1250-1303)
+  double norm_l2(const St<double>* v) const {
+    double s = 0.0;
+    for (int c = 0; c < N(); ++c) {
+      const auto& x = v[c];
+      s += x[0] * x[0] + x[1] * x[1] + x[2] * x[2] + x[3] * x[3] + x[4] * x[4] + x[5] * x[5];
+    }
+    return std::sqrt(s);
+  }
+  double dot(const St<double>* a, const St<double>* b) const {
+    double s = 0.0;
+    for (int c = 0; c < N(); ++c)
+      s += a[c][0] * b[c][0] + a[c][1] * b[c][1] + a[c][2] * b[c][2] + a[c][3] * b[c][3] + a[c][4] * b[c][4] +
+           a[c][5] * b[c][5];
+    return s;
+  }
+  double norm_vw(const St<double>* R) const {
+    double sq = 0.0, tv = 0.0;
+    for (int c = 0; c < N(); ++c) {
+      double vol = geo.vol[c];
+      double a = R[c][0] / vol, b = R[c][1] / vol, d = R[c][2] / vol, e = R[c][3] / vol, k = R[c][4] / vol,
+             w = R[c][5] / vol;
+      double loc = a * a + b * b + d * d + e * e + k * k + w * w;
+      sq += loc * vol;
+      tv += vol;
+    }
+    if (tv == 0) return 0.0;
+    return std::sqrt(sq / tv);
+  }
+  // Solver::applyJacobian (Solver.cpp:1331-1386)
+  void jv_fd(const St<double>* U, const St<double>* Rb, const St<double>* v, St<double>* Jv) {
+    int n = N();
+    double un = norm_l2(U), vn = norm_l2(v);
+    double eps = 1e-7 * (1.0 + un) / (vn + 1e-30);
+    std::vector<St<double>> Up(n), Rp(n);
+    for (int c = 0; c < n; ++c) Up[c] = U[c] + v[c] * eps;
+    D.residual(Up.data(), Rp.data(), false);
+    for (int c = 0; c < n; ++c) {
+      St<double> t1 = (Rp[c] - Rb[c]) * (1.0 / eps);
+      double tc = geo.vol[c] / (dt[c] + 1e-30);
+      Jv[c] = t1 + v[c] * tc;
+    }
+  }
+  // Solver::solveGMRES (Solver.cpp:1472-1618): right-preconditioned by the
+  // scalar diagonal, modified Gram-Schmidt, Givens rotations, one cycle of m.
+  int gmres(const St<double>* U, const St<double>* Rb, const St<double>* rhs, St<double>* x, int m, double tol) {
+    int n = N();
+    std::vector<double> Dm;
+    diag_precond(U, Dm);
+    std::vector<std::vector<St<double>>> V(m + 1, std::vector<St<double>>(n));
+    std::vector<std::vector<double>> H(m + 1, std::vector<double>(m, 0.0));
+    std::vector<double> sn(m, 0.0), cs(m, 0.0);
+    for (int c = 0; c < n; ++c) x[c] = St<double>();
+    double beta = norm_l2(rhs);
+    if (beta < 1e-12) return 0;
+    double ib = 1.0 / beta;
+    for (int c = 0; c < n; ++c) V[0][c] = rhs[c] * ib;
+    std::vector<double> gv(m + 1, 0.0);
+    gv[0] = beta;
+    int k = 0;
+    std::vector<St<double>> z(n);
+    for (int j = 0; j < m; ++j) {
+      k = j;
+      for (int c = 0; c < n; ++c) z[c] = V[j][c] * (1.0 / Dm[c]);
+      jv_fd(U, Rb, z.data(), V[j + 1].data());
+      for (int i = 0; i <= j; ++i) {
+        H[i][j] = dot(V[j + 1].data(), V[i].data());
+        for (int c = 0; c < n; ++c) V[j + 1][c] = V[j + 1][c] - V[i][c] * H[i][j];
+      }
+      H[j + 1][j] = norm_l2(V[j + 1].data());
+      if (H[j + 1][j] > 1e-12) {
+        double ih = 1.0 / H[j + 1][j];
+        for (int c = 0; c < n; ++c) V[j + 1][c] = V[j + 1][c] * ih;
+      }
+      for (int i = 0; i < j; ++i) {
+        double tmp = cs[i] * H[i][j] + sn[i] * H[i + 1][j];
+        H[i + 1][j] = -sn[i] * H[i][j] + cs[i] * H[i + 1][j];
+        H[i][j] = tmp;
+      }
+      double a = H[j][j], b = H[j + 1][j];
+      if (std::abs(b) < 1e-14) {
+        cs[j] = 1.0;
+        sn[j] = 0.0;
+      } else {
+        double t = std::sqrt(a * a + b * b);
+        cs[j] = a / t;
+        sn[j] = b / t;
+      }
+      H[j][j] = cs[j] * a + sn[j] * b;
+      H[j + 1][j] = 0.0;
+      gv[j + 1] = -sn[j] * gv[j];
+      gv[j] = cs[j] * gv[j];
+      if (std::abs(gv[j + 1]) < tol * beta) break;
+    }
+    std::vector<double> y(k + 1);
+    for (int i = k; i >= 0; --i) {
+      y[i] = gv[i];
+      for (int l = i + 1; l <= k; ++l) y[i] -= H[i][l] * y[l];
+      y[i] /= H[i][i];
+    }
+    for (int c = 0; c < n; ++c) x[c] = St<double>();
+    for (int i = 0; i <= k; ++i)
+      for (int c = 0; c < n; ++c) x[c] = x[c] + V[i][c] * y[i] * (1.0 / Dm[c]);
+    return k + 1;
+  }
+  // Solver::solveImplicit (Solver.cpp:1696-1799)
+  int solve_implicit(St<double>* U, double tol, int maxsteps, double rel_drop, int m, double* hvm, int* hit,
+                     double* hcfl, int* converged) {
+    int n = N();
+    double cfl = 0.1;
+    std::vector<St<double>> Res(n), rhs(n), dU(n);
+    double vm0 = -1;
+    *converged = 0;
+    int step = 0;
+    for (step = 0; step < maxsteps; ++step) {
+      local_timestep(U, cfl);
+      bool upd = !(step > 150 || cfl > 10.0);
+      D.residual(U, Res.data(), upd);
+      double vm = norm_vw(Res.data());
+      if (vm0 < 0) vm0 = vm;
+      if (hvm) hvm[step] = vm;
+      if (hcfl) hcfl[step] = cfl;
+      if (vm < tol || (rel_drop > 0 && vm <= rel_drop * vm0)) {
+        *converged = 1;
+        if (hit) hit[step] = 0;
+        break;
+      }
+      for (int c = 0; c < n; ++c) rhs[c] = Res[c] * (-1.0);
+      int it = gmres(U, Res.data(), rhs.data(), dU.data(), m, 1e-2);
+      if (hit) hit[step] = it;
+      if (it >= 29) {
+        cfl *= 0.5;
+        if (cfl < 0.05) cfl = 0.05;
+      }
+      double relax = 1.0;
+      if (vm > 1e6) relax = 0.5;
+      double g = P.gamma;
+      for (int c = 0; c < n; ++c) {
+        St<double> Un = U[c] + dU[c] * relax;
+        double ke = 0.5 * (Un[1] * Un[1] + Un[2] * Un[2]) / (Un[0] + 1e-12);
+        double pn = (g - 1.0) * (Un[3] - ke);
+        if (Un[0] < 1e-5) Un[0] = 1e-5;
+        if (pn < 1e-5) {
+          pn = 1e-5;
+          Un[3] = pn / (g - 1.0) + ke;
+        }
+        U[c] = Un;
+      }
+      if (step > 5) {
+        if (vm > 1e6) cfl *= 1.2;
+        else cfl *= 1.5;
+        if (it >= 25) cfl *= 0.5;
+        if (cfl > 10000) cfl = 10000;
+        if (cfl < 0.1) cfl = 0.1;
+      }
+    }
+    return step;
+  }
+  // Solver::timeStepExplicit (Solver.cpp:1219-1243)
+  void explicit_step(St<double>* U, double dtc) {
+    std::vector<St<double>> Res(N());
+    D.residual(U, Res.data(), true);
+    for (int c = 0; c < N(); ++c) {
+      double vol = geo.vol[c];
+      U[c] = U[c] - Res[c] * (dtc / vol);
+      clip_explicit(U[c]);
+    }
+  }
+  static void clip_explicit(St<double>& u) {
+    if (u[0] < 1e-3) u[0] = 1e-3;
+    double ke = 0.5 * (u[1] * u[1] + u[2] * u[2]) / u[0];
+    double ie = u[3] - ke;
+    if (ie < 1e-3) u[3] = ke + 1e-3;
+  }
+  // extension: SSP-RK3 (Shu-Osher) from three residual calls, same clip per stage
+  void rk3_step(St<double>* U, double dtc) {
+    int n = N();
+    std::vector<St<double>> R(n), U1(n), U2(n);
+    D.residual(U, R.data(), true);
+    for (int c = 0; c < n; ++c) {
+      U1[c] = U[c] - R[c] * (dtc / geo.vol[c]);
+      clip_explicit(U1[c]);
+    }
+    D.residual(U1.data(), R.data(), true);
+    for (int c = 0; c < n; ++c) {
+      U2[c] = U[c] * 0.75 + (U1[c] - R[c] * (dtc / geo.vol[c])) * 0.25;
+      clip_explicit(U2[c]);
+    }
+    D.residual(U2.data(), R.data(), true);
+    for (int c = 0; c < n; ++c) {
+      U[c] = U[c] * (1.0 / 3.0) + (U2[c] - R[c] * (dtc / geo.vol[c])) * (2.0 / 3.0);
+      clip_explicit(U[c]);
+    }
+  }
+};
+
+}  // namespace orc
+
+using namespace orc;
+
+struct orc_mesh {
+  Topo topo;
+  Geo<double> geo;
+};
+struct orc_solver {
+  Solver s;
+};
+
+static void finish_mesh(orc_mesh* m) {
+  build_connectivity(m->topo);
+  compute_geometry<double>(m->topo, m->topo.x, m->topo.y, m->geo);
+}
+
+extern "C" {
+
+orc_mesh* orc_mesh_from_desc(const cfd_mesh_desc* d) {
+  auto* m = new orc_mesh();
+  Topo& t = m->topo;
+  t.x.assign(d->x, d->x + d->n_nodes + 1);
+  t.y.assign(d->y, d->y + d->n_nodes + 1);
+  t.cptr.assign(d->cell_ptr, d->cell_ptr + d->n_cells + 1);
+  t.cnodes.assign(d->cell_nodes, d->cell_nodes + d->cell_ptr[d->n_cells]);
+  for (int e = 0; e < d->n_bedges; ++e) {
+    int a = d->bedges[3 * e], b = d->bedges[3 * e + 1];
+    t.bedges[{std::min(a, b), std::max(a, b)}] = d->bedges[3 * e + 2];
+  }
+  finish_mesh(m);
+  return m;
+}
+
+orc_mesh* orc_mesh_read_gmsh(const char* path) {
+  auto* m = new orc_mesh();
+  if (!read_gmsh(path, m->topo)) {
+    delete m;
+    return nullptr;
+  }
+  finish_mesh(m);
+  return m;
+}
+
+void orc_mesh_destroy(orc_mesh* m) { delete m; }
+
+void orc_mesh_sizes(const orc_mesh* m, int* nn, int* nc, int* nf) {
+  *nn = (int)m->topo.x.size() - 1;
+  *nc = m->topo.ncells();
+  *nf = m->topo.nfaces();
+}
+
+void orc_mesh_faces(const orc_mesh* m, int* n1, int* n2, int* own, int* nei, int* tag, double* nx, double* ny,
+                    double* area, double* xm, double* ym) {
+  const Topo& t = m->topo;
+  const Geo<double>& g = m->geo;
+  for (int f = 0; f < t.nfaces(); ++f) {
+    n1[f] = t.n1[f]; n2[f] = t.n2[f]; own[f] = t.own[f]; nei[f] = t.nei[f]; tag[f] = t.tag[f];
+    nx[f] = g.nx[f]; ny[f] = g.ny[f]; area[f] = g.area[f]; xm[f] = g.xm[f]; ym[f] = g.ym[f];
+  }
+}
+
+void orc_mesh_cells(const orc_mesh* m, double* vol, double* xc, double* yc) {
+  for (int c = 0; c < m->topo.ncells(); ++c) {
+    vol[c] = m->geo.vol[c]; xc[c] = m->geo.xc[c]; yc[c] = m->geo.yc[c];
+  }
+}
+
+orc_solver* orc_solver_create(const orc_mesh* m, const cfd_params* p) {
+  auto* s = new orc_solver();
+  s->s.topo = m->topo;
+  s->s.geo = m->geo;
+  s->s.P = *p;
+  s->s.bind();
+  return s;
+}
+void orc_solver_destroy(orc_solver* s) { delete s; }
+void orc_set_params(orc_solver* s, const cfd_params* p) {
+  s->s.P = *p;
+  s->s.D.P = *p;
+}
+void orc_set_face_flux(orc_solver* s, const unsigned char* alt) {
+  s->s.face_alt.assign(alt, alt + s->s.topo.nfaces());
+  s->s.bind();
+}
+
+void orc_residual(orc_solver* s, const double* U6, double* R6, int update) {
+  s->s.D.residual(Solver::S(U6), Solver::S(R6), update != 0);
+}
+
+// complex step: R(U + i h v); returns real part and imaginary part / h
+void orc_residual_cs(orc_solver* s, const double* U6, const double* v6, double h, double* Rre, double* dR,
+                     int update) {
+  Solver& S = s->s;
+  int n = S.N();
+  Geo<cplx> gc;
+  std::vector<cplx> X(S.topo.x.begin(), S.topo.x.end()), Y(S.topo.y.begin(), S.topo.y.end());
+  compute_geometry<cplx>(S.topo, X, Y, gc);
+  Disc<cplx> D;
+  D.topo = &S.topo;
+  D.geo = &gc;
+  D.P = S.P;
+  D.face_alt = S.face_alt.empty() ? nullptr : S.face_alt.data();
+  if (!update && !S.D.limiters.empty()) {
+    D.limiters.resize(n);
+    for (int c = 0; c < n; ++c) D.limiters[c] = S.D.limiters[c];
+  }
+  std::vector<St<cplx>> U(n), R(n);
+  for (int c = 0; c < n; ++c)
+    for (int k = 0; k < 6; ++k) U[c][k] = cplx(U6[6 * c + k], h * v6[6 * c + k]);
+  D.residual(U.data(), R.data(), update != 0);
+  for (int c = 0; c < n; ++c)
+    for (int k = 0; k < 6; ++k) {
+      Rre[6 * c + k] = R[c][k].real();
+      dR[6 * c + k] = R[c][k].imag() / h;
+    }
+}
+
+// per-face contribution F*A (reference face order) of the residual at U
+void orc_face_flux(orc_solver* s, const double* U6, double* F6, int update) {
+  std::vector<St<double>> R(s->s.N());
+  s->s.D.residual(Solver::S(U6), R.data(), update != 0, Solver::S(F6));
+}
+
+void orc_get_gradients(orc_solver* s, double* g10) {
+  for (int c = 0; c < s->s.N(); ++c)
+    for (int k = 0; k < 10; ++k) g10[10 * c + k] = s->s.D.grads[c][k];
+}
+void orc_get_limiters(orc_solver* s, double* phi) {
+  auto& L = s->s.D.limiters;
+  for (int c = 0; c < s->s.N(); ++c) phi[c] = c < (int)L.size() ? L[c] : 1.0;
+}
+void orc_set_limiters(orc_solver* s, const double* phi) { s->s.D.limiters.assign(phi, phi + s->s.N()); }
+void orc_local_timestep(orc_solver* s, const double* U6, double cfl) { s->s.local_timestep(Solver::S(U6), cfl); }
+void orc_get_dt(orc_solver* s, double* dt) { std::copy(s->s.dt.begin(), s->s.dt.end(), dt); }
+void orc_set_dt(orc_solver* s, const double* dt) { s->s.dt.assign(dt, dt + s->s.N()); }
+void orc_diag_precond(orc_solver* s, const double* U6, double* D) {
+  std::vector<double> d;
+  s->s.diag_precond(Solver::S(U6), d);
+  std::copy(d.begin(), d.end(), D);
+}
+void orc_jv_fd(orc_solver* s, const double* U6, const double* R6, const double* v6, double* Jv6) {
+  s->s.jv_fd(Solver::S(U6), Solver::S(R6), Solver::S(v6), Solver::S(Jv6));
+}
+double orc_norm_l2(orc_solver* s, const double* v6) { return s->s.norm_l2(Solver::S(v6)); }
+double orc_dot(orc_solver* s, const double* a6, const double* b6) { return s->s.dot(Solver::S(a6), Solver::S(b6)); }
+double orc_norm_vw(orc_solver* s, const double* R6) { return s->s.norm_vw(Solver::S(R6)); }
+int orc_gmres(orc_solver* s, const double* U6, const double* R6, const double* RHS6, double* x6, int m, double tol) {
+  return s->s.gmres(Solver::S(U6), Solver::S(R6), Solver::S(RHS6), Solver::S(x6), m, tol);
+}
+int orc_solve_implicit(orc_solver* s, double* U6, double tol, int maxsteps, double rel_drop, int m, double* hvm,
+                       int* hit, double* hcfl, int* converged) {
+  return s->s.solve_implicit(Solver::S(U6), tol, maxsteps, rel_drop, m, hvm, hit, hcfl, converged);
+}
+void orc_explicit_step(orc_solver* s, double* U6, double dt) { s->s.explicit_step(Solver::S(U6), dt); }
+void orc_rk3_step(orc_solver* s, double* U6, double dt) { s->s.rk3_step(Solver::S(U6), dt); }
+
+// Solver::extractSkinFriction (Solver.cpp:1899-1979) per tag-4 face, Cp per
+// wall face (extension: (p_owner - p_inf)/q_inf), J = sum over tag-3 faces of
+// p_owner * A.
+void orc_wall_outputs(orc_solver* s, const double* U6, double* Cp, double* Cf, double* J) {
+  Solver& S = s->s;
+  const cfd_params& P = S.P;
+  double g = P.gamma;
+  double fT = 1.0 + 0.5 * (g - 1.0) * P.M_inf * P.M_inf;
+  double Ts = P.T_total_inlet / fT;
+  double c = std::sqrt(g * P.R_gas * Ts);
+  double uinf = P.M_inf * c;
+  double rinf = P.P_static_exit / (P.R_gas * Ts);
+  double qinf = 0.5 * rinf * uinf * uinf;
+  const St<double>* U = Solver::S(U6);
+  double j = 0.0;
+  for (int f = 0; f < S.topo.nfaces(); ++f) {
+    Cp[f] = 0.0;
+    Cf[f] = 0.0;
+    int o = S.topo.own[f], tg = S.topo.tag[f];
+    if (S.topo.nei[f] >= 0) continue;
+    if (tg == 3 || tg == 4) Cp[f] = (S.D.pressure(U[o]) - P.P_static_exit) / qinf;
+    if (tg == 3) j += S.D.pressure(U[o]) * S.geo.area[f];
+    if (tg == 4) {
+      const St<double>& L = U[o];
+      double uc = L[1] / L[0], vc = L[2] / L[0];
+      double tx = -S.geo.ny[f], ty = S.geo.nx[f];
+      if (tx < 0) { tx = -tx; ty = -ty; }
+      double ut = uc * tx + vc * ty;
+      double dx = S.geo.xm[f] - S.geo.xc[o], dy = S.geo.ym[f] - S.geo.yc[o];
+      double dn = std::abs(dx * S.geo.nx[f] + dy * S.geo.ny[f]);
+      if (dn < 1e-12) dn = 1e-12;
+      double dudn = ut / dn;
+      double mu = L[0] * P.nu;
+      double tw = mu * dudn;
+      Cf[f] = tw / qinf;
+    }
+  }
+  *J = j;
+}
+
+}  // extern "C"

@@ -1,0 +1,127 @@
+"""GPU norms, FD Jacobian-vector product, GMRES and the Newton driver against
+the reference fixtures and the oracle.
+
+Reductions on the device sum in a fixed tree order (deterministic, not the
+reference's sequential order), so norms agree to 1e-14 relative.  The FD Jv
+step eps is built from two such norms; an eps that differs in the last bit
+shifts (R(U + eps v) - R(U)) / eps by the finite-difference round-off,
+~1e-16 |R| / (eps |Jv|), so FD Jv parity is held to 1e-7 relative in the L2
+norm — the level at which two exact FD evaluations with eps one ulp apart
+already differ (checked on the oracle itself below).
+"""
+import numpy as np
+import pytest
+
+from helpers import bump_params
+from mycode_amd import cfd, configs
+from oracle import oracle
+
+pytestmark = pytest.mark.gpu
+
+
+def _bump(g, fl, sp):
+    m = cfd.Mesh.from_arrays(g["x"], g["y"], g["cell_ptr"], g["cell_nodes"], g["bedges"])
+    om = oracle.OracleMesh(g["x"], g["y"], g["cell_ptr"], g["cell_nodes"], g["bedges"])
+    p = bump_params(g, fl, sp)
+    return cfd.Solver(m, p, assembly=1), oracle.OracleSolver(om, p)
+
+
+def rel(a, b):
+    return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300))
+
+
+def test_norms(bump_golden):
+    g = bump_golden
+    s, o = _bump(g, 2, 2)
+    U, v = g["U"], g["v"]
+    assert abs(s.norm_L2(U) / o.norm_L2(U) - 1) < 1e-14
+    assert abs(s.dot_product(U, v) / o.dot_product(U, v) - 1) < 1e-12
+    R = o.compute_residuals(U)
+    assert abs(s.norm_L2_volume_weighted(R) / o.norm_L2_volume_weighted(R) - 1) < 1e-14
+
+
+@pytest.mark.parametrize("fs", ["00", "11", "22"])
+def test_fd_jv_vs_reference(bump_golden, fs):
+    g = bump_golden
+    s, o = _bump(g, int(fs[0]), int(fs[1]))
+    U, v = g["U"], g["v"]
+    R = s.compute_residuals(U)
+    s.compute_local_time_step(U, 5.0)
+    J = s.apply_jacobian(U, R, v)
+    assert rel(J[:, :4], g[f"Jv_{fs}"]) < 1e-7
+
+
+@pytest.mark.parametrize("fs", ["00", "11", "22"])
+def test_gmres_vs_reference(bump_golden, fs):
+    g = bump_golden
+    s, o = _bump(g, int(fs[0]), int(fs[1]))
+    U = g["U"]
+    R = s.compute_residuals(U)
+    s.compute_local_time_step(U, 5.0)
+    x, it = s.solve_gmres(U, R, -R, s.gmres_opts(m=40, tol=1e-2))
+    assert abs(it - int(g[f"gmres_it_{fs}"])) <= 1
+    assert rel(x[:, :4], g[f"gmres_x_{fs}"]) < 1e-5
+
+
+def test_gmres_plate_vs_oracle():
+    mesh, p, U = configs.setup(2, 0.1)
+    U = U.copy()
+    U[:, :4] *= 1 + 0.01 * np.random.default_rng(3).standard_normal((U.shape[0], 4))
+    s = cfd.Solver(mesh, p)
+    o = oracle.OracleSolver(oracle.OracleMesh.from_mesh(mesh), p)
+    R = s.compute_residuals(U)
+    Ro = o.compute_residuals(U)
+    s.compute_local_time_step(U, 2.0)
+    o.compute_local_time_step(U, 2.0)
+    x, it = s.solve_gmres(U, R, -R, s.gmres_opts(m=30, tol=1e-3))
+    xo, ito = o.solve_gmres(U, Ro, -Ro, m=30, tol=1e-3)
+    assert abs(it - ito) <= 1
+    assert rel(x, xo) < 1e-5
+
+
+@pytest.mark.parametrize("fs", ["00", "22"])
+def test_newton_trajectory_vs_reference(bump_golden, fs):
+    g = bump_golden
+    s, o = _bump(g, int(fs[0]), int(fs[1]))
+    steps = int(g[f"newton_{fs}_steps"])
+    U, res, hist = s.solve_implicit(g["U0"], s.newton_opts(tolerance=1e-12, max_steps=steps))
+    _, info = o.solve_implicit(g["U0"], tol=1e-12, max_steps=steps)
+    assert res.steps == steps
+    assert np.array_equal(hist["iters"], info["iters"][:steps])
+    assert np.max(np.abs(hist["vm"] / info["vm"][:steps] - 1)) < 1e-6
+    assert rel(U[:, :4], g[f"newton_{fs}_U"]) < 1e-8
+
+
+def test_explicit_and_rk3(bump_golden):
+    g = bump_golden
+    s, o = _bump(g, 2, 2)
+    Ue = s.time_step_explicit(g["U"], 1e-6)
+    assert rel(Ue[:, :4], g["explicit_U"]) < 1e-14
+    U3 = s.rk3_step(g["U"], 1e-6)
+    assert rel(U3, o.rk3_step(g["U"], 1e-6)) < 1e-14
+
+
+def test_wall_outputs_vs_oracle():
+    mesh, p, U = configs.setup(2, 0.1)
+    U = U.copy()
+    U[:, 1] *= 1 + 0.1 * np.random.default_rng(5).random(U.shape[0])
+    s = cfd.Solver(mesh, p)
+    o = oracle.OracleSolver(oracle.OracleMesh.from_mesh(mesh), p)
+    Cp, Cf, J = s.wall_outputs(U)
+    Cpo, Cfo, Jo = o.wall_outputs(U)
+    assert np.array_equal(Cp, Cpo)
+    assert np.array_equal(Cf, Cfo)
+    assert J == Jo
+
+
+def test_error_paths():
+    mesh, p, U = configs.setup(2, 0.1)
+    s = cfd.Solver(mesh, p)
+    with pytest.raises(cfd.CfdError):  # Jv before a local time step
+        s.apply_jacobian(U, U, U)
+    with pytest.raises(ValueError):
+        s.compute_residuals(U[:10])
+    bad = p.copy()
+    bad.nvar = 5
+    with pytest.raises(cfd.CfdError):
+        cfd.Solver(mesh, bad)
