@@ -174,6 +174,21 @@ int cfd_get_dt(cfd_ctx* ctx, double* dt);
 int cfd_diag_preconditioner(cfd_ctx* ctx, const double* U6, double* D);
 /* Solver::applyJacobian(U_base, R_base, v, Jv) — Solver.cpp:1331-1386 */
 int cfd_jv_fd(cfd_ctx* ctx, const double* U6, const double* R6, const double* v6, double* Jv6);
+/* Forward-mode tangent J v = dR/dU v + (V/dtau) v with the limiter frozen —
+ * the exact derivative of what Solver::applyJacobian (Solver.cpp:1331-1386)
+ * approximates by a finite difference.  No reference counterpart.
+ * precision: 64 = FP64 tangent; 32 = FP32 tangent (values and derivatives). */
+int cfd_jv_tangent(cfd_ctx* ctx, const double* U6, const double* v6, double* Jv6, int precision);
+/* Reverse-mode product J^T w = (dR/dU)^T w + (V/dtau) w with the limiter
+ * frozen: the transpose of cfd_jv_tangent, for discrete-adjoint solves.
+ * Gradients are recomputed, not stored.  No reference counterpart. */
+int cfd_vjp_adjoint(cfd_ctx* ctx, const double* U6, const double* w6, double* JTw6);
+/* Block-Jacobi preconditioner (no reference counterpart; the reference uses
+ * the scalar diagonal of Solver.cpp:1424-1465): per cell the nv x nv diagonal
+ * block B of the first-order Jacobian plus V/dtau and a viscous diagonal,
+ * and its inverse stored in `precision`.  B and Binv (N x nv x nv, row
+ * major, reference cell order, may be NULL) are copied out for inspection. */
+int cfd_block_jacobi(cfd_ctx* ctx, const double* U6, int precision, double* B, double* Binv);
 /* Solver::norm_L2, dot_product, norm_L2_VolumeWeighted — Solver.cpp:1250-1303 */
 int cfd_norm_l2(cfd_ctx* ctx, const double* v6, double* out);
 int cfd_dot(cfd_ctx* ctx, const double* a6, const double* b6, double* out);
@@ -242,6 +257,9 @@ int cfd_dev_unpack(cfd_ctx* ctx, const double* Usoa_dev, double* U6_dev);
 int cfd_dev_residual(cfd_ctx* ctx, const double* U, double* R, int update_limiters);
 int cfd_dev_jv_fd(cfd_ctx* ctx, const double* U, const double* Rb, const double* v, double* Jv);
 int cfd_dev_local_timestep(cfd_ctx* ctx, const double* U, double cfl);
+/* v and Jv are FP64 (precision 64) or FP32 (precision 32) SoA vectors. */
+int cfd_dev_jv_tangent(cfd_ctx* ctx, const double* U, const void* v, void* Jv, int precision);
+int cfd_dev_vjp_adjoint(cfd_ctx* ctx, const double* U, const double* w, double* JTw);
 int cfd_dev_synchronize(cfd_ctx* ctx);
 
 /* ---- measurement -------------------------------------------------------- */

@@ -109,6 +109,7 @@ SIGNATURES = [
     ("cfd_jv_fd", C.c_int, [_VP, _DP, _DP, _DP, _DP]),
     ("cfd_jv_tangent", C.c_int, [_VP, _DP, _DP, _DP, C.c_int]),
     ("cfd_vjp_adjoint", C.c_int, [_VP, _DP, _DP, _DP]),
+    ("cfd_block_jacobi", C.c_int, [_VP, _DP, C.c_int, _DP, _DP]),
     ("cfd_norm_l2", C.c_int, [_VP, _DP, _DP]),
     ("cfd_dot", C.c_int, [_VP, _DP, _DP, _DP]),
     ("cfd_norm_vw", C.c_int, [_VP, _DP, _DP]),
@@ -125,6 +126,7 @@ SIGNATURES = [
     ("cfd_dev_jv_fd", C.c_int, [_VP, _VP, _VP, _VP, _VP]),
     ("cfd_dev_jv_tangent", C.c_int, [_VP, _VP, _VP, _VP, C.c_int]),
     ("cfd_dev_local_timestep", C.c_int, [_VP, _VP, C.c_double]),
+    ("cfd_dev_vjp_adjoint", C.c_int, [_VP, _VP, _VP, _VP]),
     ("cfd_dev_synchronize", C.c_int, [_VP]),
     ("cfd_dev_time_residual", C.c_int, [_VP, _VP, _VP, C.c_int, C.c_int, _DP, _IP]),
 ]
@@ -386,6 +388,15 @@ class Solver:
         J = np.zeros((self.N, 6))
         self._c(self.lib.cfd_vjp_adjoint(self._h, _dp(U), _dp(w), _dp(J)), "cfd_vjp_adjoint")
         return J
+
+    def block_jacobi(self, U, precision: int = 64):
+        """(B, Binv): per-cell nv x nv blocks of the block-Jacobi preconditioner."""
+        U = _state(U, self.N)
+        nv = self.params().nvar
+        B = np.zeros((self.N, nv, nv))
+        Bi = np.zeros((self.N, nv, nv))
+        self._c(self.lib.cfd_block_jacobi(self._h, _dp(U), precision, _dp(B), _dp(Bi)), "cfd_block_jacobi")
+        return B, Bi
 
     def norm_L2(self, v) -> float:
         v = _state(v, self.N)

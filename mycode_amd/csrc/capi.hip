@@ -102,6 +102,7 @@ int ensure_krylov(cfd_ctx* c, int m, int prec) {
   const size_t vs = (size_t)c->P.nvar * c->pitch;
   HIP_TRY(c, hipMalloc(&k.V, (size_t)(m + 1) * vs * es));
   HIP_TRY(c, hipMalloc((void**)&k.D, (size_t)c->pitch * 8));
+  HIP_TRY(c, hipMalloc(&k.blk, (size_t)c->P.nvar * c->P.nvar * c->pitch * es));
   HIP_TRY(c, hipMalloc((void**)&k.z, vs * 8));
   HIP_TRY(c, hipMalloc((void**)&k.w, vs * 8));
   HIP_TRY(c, hipMalloc((void**)&k.h, 256 * 8));
@@ -345,6 +346,7 @@ int cfd_ctx_create(const cfd_mesh* m, const cfd_params* p, cfd_ctx** out) {
   HIP_TRY(c, dalloc(&c->red, 65536));
   HIP_TRY(c, dalloc(&c->scal, 64));
   HIP_TRY(c, dalloc(&c->gradT, (size_t)20 * P_));
+  HIP_TRY(c, dalloc(&c->adj, (size_t)10 * P_));
   HIP_TRY(c, hipHostMalloc((void**)&c->scal_host, 64 * 8, hipHostMallocDefault));
   HIP_TRY(c, dalloc(&c->io6, (size_t)6 * N));
   for (int i = 0; i < 6; ++i) {
@@ -362,7 +364,7 @@ int cfd_ctx_destroy(cfd_ctx* c) {
   free_krylov(c);
   void* bufs[] = {c->d_vol, c->d_xc,  c->d_yc,  c->d_slots, c->d_own, c->d_nei, c->d_tag,  c->d_nx,
                   c->d_ny,  c->d_area, c->d_xm, c->d_ym,    c->d_iperm, c->d_facemap, c->d_alt, c->grad,
-                  c->phi,   c->dt,    c->Fm,    c->red,     c->scal, c->gradT, c->io6};
+                  c->phi,   c->dt,    c->Fm,    c->red,     c->scal, c->gradT, c->adj, c->io6};
   for (void* b : bufs) (void)hipFree(b);
   for (int i = 0; i < 6; ++i) (void)hipFree(c->vec[i]);
   (void)hipHostFree(c->scal_host);
@@ -480,6 +482,76 @@ int cfd_jv_fd(cfd_ctx* c, const double* U6, const double* R6, const double* v6, 
   return download_state(c, c->vec[3], Jv6);
 }
 
+int cfd_jv_tangent(cfd_ctx* c, const double* U6, const double* v6, double* Jv6, int precision) {
+  if (!c || !U6 || !v6 || !Jv6 || (precision != 64 && precision != 32)) return CFD_ERR_ARG;
+  if (!c->dt_valid) return fail(c, CFD_ERR_STATE, "no local time step computed");
+  if (c->P.spatial == CFD_SECOND_ORDER_LIMITED && !c->phi_valid)
+    return fail(c, CFD_ERR_STATE, "limiter not formed: evaluate the residual first");
+  int rc;
+  if ((rc = upload_state(c, U6, c->vec[0]))) return rc;
+  if ((rc = upload_state(c, v6, c->vec[1]))) return rc;
+  const size_t n = (size_t)c->P.nvar * c->pitch;
+  if (precision == 64) {
+    launch_jv_tangent(c, c->vec[0], c->vec[1], c->vec[2], 64);
+  } else {
+    float* vf = (float*)c->vec[3];
+    float* jf = (float*)c->vec[4];
+    launch_d2f(c, c->vec[1], vf, n);
+    launch_jv_tangent(c, c->vec[0], vf, jf, 32);
+    launch_f2d(c, jf, c->vec[2], n);
+  }
+  return download_state(c, c->vec[2], Jv6);
+}
+
+int cfd_vjp_adjoint(cfd_ctx* c, const double* U6, const double* w6, double* JTw6) {
+  if (!c || !U6 || !w6 || !JTw6) return CFD_ERR_ARG;
+  if (!c->dt_valid) return fail(c, CFD_ERR_STATE, "no local time step computed");
+  if (c->P.spatial == CFD_SECOND_ORDER_LIMITED && !c->phi_valid)
+    return fail(c, CFD_ERR_STATE, "limiter not formed: evaluate the residual first");
+  int rc;
+  if ((rc = upload_state(c, U6, c->vec[0]))) return rc;
+  if ((rc = upload_state(c, w6, c->vec[1]))) return rc;
+  launch_vjp(c, c->vec[0], c->vec[1], c->vec[2]);
+  return download_state(c, c->vec[2], JTw6);
+}
+
+int cfd_block_jacobi(cfd_ctx* c, const double* U6, int precision, double* B, double* Binv) {
+  if (!c || !U6 || (precision != 64 && precision != 32)) return CFD_ERR_ARG;
+  if (!c->dt_valid) return fail(c, CFD_ERR_STATE, "no local time step computed");
+  int rc;
+  if ((rc = upload_state(c, U6, c->vec[0]))) return rc;
+  if ((rc = ensure_krylov(c, c->kry.m > 0 ? c->kry.m : 1, precision))) return rc;
+  const int nv = c->P.nvar;
+  const size_t nb = (size_t)nv * nv * c->pitch;
+  double* Bd = nullptr;
+  HIP_TRY(c, hipMalloc((void**)&Bd, nb * 8));
+  launch_block_precond(c, c->vec[0], c->kry.blk, precision, Bd);
+  std::vector<double> hB(nb), hI(nb);
+  std::vector<float> hIf;
+  hipError_t e = hipMemcpyAsync(hB.data(), Bd, nb * 8, hipMemcpyDeviceToHost, c->stream);
+  if (e == hipSuccess) {
+    if (precision == 64) {
+      e = hipMemcpyAsync(hI.data(), c->kry.blk, nb * 8, hipMemcpyDeviceToHost, c->stream);
+    } else {
+      hIf.resize(nb);
+      e = hipMemcpyAsync(hIf.data(), c->kry.blk, nb * 4, hipMemcpyDeviceToHost, c->stream);
+    }
+  }
+  if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+  (void)hipFree(Bd);
+  if (e != hipSuccess) return fail(c, CFD_ERR_HIP, hipGetErrorString(e));
+  if (precision == 32)
+    for (size_t i = 0; i < nb; ++i) hI[i] = hIf[i];
+  for (int i = 0; i < c->N; ++i) {
+    const size_t o = (size_t)c->iperm[i] * nv * nv;
+    for (int q = 0; q < nv * nv; ++q) {
+      if (B) B[o + q] = hB[(size_t)q * c->pitch + i];
+      if (Binv) Binv[o + q] = hI[(size_t)q * c->pitch + i];
+    }
+  }
+  return CFD_OK;
+}
+
 int cfd_norm_l2(cfd_ctx* c, const double* v6, double* out) {
   if (!c || !v6 || !out) return CFD_ERR_ARG;
   int rc = upload_state(c, v6, c->vec[0]);
@@ -517,15 +589,19 @@ static int gmres_dev(cfd_ctx* c, const double* U, const double* Rb, const double
                      const cfd_gmres_opts* o, int* iters, double* tmpU, double* tmpR) {
   const int m = o->m, prec = o->precision;
   if (m < 1 || m > 60) return fail(c, CFD_ERR_ARG, "GMRES m must be in [1, 60]");
-  if (prec != 64) return fail(c, CFD_ERR_ARG, "only FP64 GMRES is available");
-  if (o->jv != 0) return fail(c, CFD_ERR_ARG, "tangent Jv is not available");
+  if (prec != 64 && prec != 32) return fail(c, CFD_ERR_ARG, "precision must be 64 or 32");
+  if (o->jv == 0 && prec != 64) return fail(c, CFD_ERR_ARG, "the finite-difference Jv runs in FP64 only");
+  if (o->precond != 0 && o->precond != 1) return fail(c, CFD_ERR_ARG, "unknown preconditioner");
+  if (o->precond == 1 && o->jv == 0 && prec != 64) return fail(c, CFD_ERR_ARG, "FD Jv runs in FP64 only");
   int rc = ensure_krylov(c, m, prec);
   if (rc) return rc;
   Krylov& k = c->kry;
   const size_t es = prec == 32 ? 4 : 8;
   const size_t vs = (size_t)c->P.nvar * c->pitch;
   char* V = (char*)k.V;
-  launch_diag_precond(c, U, k.D);
+  const bool bj = o->precond == 1;
+  if (bj) launch_block_precond(c, U, k.blk, prec);
+  else launch_diag_precond(c, U, k.D);
   launch_reduce(c, RED_L2, U, nullptr, S_UNORM);
   launch_reduce(c, RED_L2, r, nullptr, S_BETA);
   double beta2;
@@ -545,8 +621,14 @@ static int gmres_dev(cfd_ctx* c, const double* U, const double* Rb, const double
     kk = j;
     void* Vj = V + (size_t)j * vs * es;
     void* Vj1 = V + (size_t)(j + 1) * vs * es;
-    launch_precond_apply(c, k.z, Vj, prec, S_VNORM);
-    launch_jv_fd(c, U, Rb, k.z, (double*)Vj1, S_UNORM, S_VNORM, tmpU, tmpR);
+    if (bj) launch_block_apply(c, k.blk, Vj, k.z, prec);
+    else launch_precond_apply(c, k.z, Vj, prec, o->jv == 0 ? S_VNORM : -1);
+    if (o->jv == 0) {
+      if (bj) launch_reduce(c, RED_L2, (const double*)k.z, nullptr, S_VNORM);
+      launch_jv_fd(c, U, Rb, (const double*)k.z, (double*)Vj1, S_UNORM, S_VNORM, tmpU, tmpR);
+    } else {
+      launch_jv_tangent(c, U, k.z, Vj1, prec);
+    }
     launch_orth(c, V, j, nullptr, prec, k.h);
     launch_store_basis(c, Vj1, nullptr, k.h, j, prec);
     HIP_TRY(c, hipMemcpyAsync(k.hpin, k.h, (size_t)(j + 2) * 8, hipMemcpyDeviceToHost, c->stream));
@@ -580,7 +662,8 @@ static int gmres_dev(cfd_ctx* c, const double* U, const double* Rb, const double
   }
   std::memcpy(k.hpin, y.data(), (kk + 1) * 8);
   HIP_TRY(c, hipMemcpyAsync(k.h + 128, k.hpin, (kk + 1) * 8, hipMemcpyHostToDevice, c->stream));
-  launch_solution(c, x, V, k.h + 128, kk, prec);
+  if (bj) launch_solution_bj(c, x, V, k.h + 128, kk, k.blk, prec);
+  else launch_solution(c, x, V, k.h + 128, kk, prec);
   *iters = kk + 1;
   return sync(c);
 }
@@ -744,6 +827,22 @@ int cfd_dev_jv_fd(cfd_ctx* c, const double* U, const double* Rb, const double* v
   HIP_TRY(c, hipGetLastError());
   return CFD_OK;
 }
+int cfd_dev_jv_tangent(cfd_ctx* c, const double* U, const void* v, void* Jv, int precision) {
+  if (!c || !U || !v || !Jv || (precision != 64 && precision != 32)) return CFD_ERR_ARG;
+  if (!c->dt_valid) return fail(c, CFD_ERR_STATE, "no local time step computed");
+  launch_jv_tangent(c, U, v, Jv, precision);
+  HIP_TRY(c, hipGetLastError());
+  return CFD_OK;
+}
+
+int cfd_dev_vjp_adjoint(cfd_ctx* c, const double* U, const double* w, double* JTw) {
+  if (!c || !U || !w || !JTw) return CFD_ERR_ARG;
+  if (!c->dt_valid) return fail(c, CFD_ERR_STATE, "no local time step computed");
+  launch_vjp(c, U, w, JTw);
+  HIP_TRY(c, hipGetLastError());
+  return CFD_OK;
+}
+
 int cfd_dev_synchronize(cfd_ctx* c) {
   if (!c) return CFD_ERR_ARG;
   return sync(c);

@@ -64,6 +64,7 @@ struct cfd_ctx {
   double* scal = nullptr;   // device scalars (norms, eps, ...)
   double* scal_host = nullptr;  // pinned
   double* gradT = nullptr;  // tangent gradients (10 x pitch, FP64 or FP32 dual parts)
+  double* adj = nullptr;    // adjoint of the gradients (10 x pitch)
   bool phi_valid = false, dt_valid = false;
   // host-pointer staging: AoS device copy + SoA vectors
   double* io6 = nullptr;     // 6 x N (AoS)
@@ -91,9 +92,9 @@ void launch_fill(cfd_ctx* c, double* x, double v, size_t n);
 // FD Jv: Jv = (R(U + eps v) - Rb) / eps + (vol/dt) v, eps from scal[su] (|U|^2) and scal[sv] (|v|^2)
 void launch_jv_fd(cfd_ctx* c, const double* U, const double* Rb, const double* v, double* Jv, int su, int sv,
                   double* tmpU, double* tmpR);
-// forward tangent (limiter frozen): Jv = dR/dU v + (vol/dt) v; prec 64 or 32; Jv is double or float
+// forward tangent (limiter frozen): Jv = dR/dU v + (vol/dt) v; prec 64 or 32; v, Jv double or float
 void launch_jv_tangent(cfd_ctx* c, const double* U, const void* v, void* Jv, int prec);
-// reverse: JTw = (dR/dU)^T w + (vol/dt) w
+// reverse product (limiter frozen): JTw = (dR/dU)^T w + (vol/dt) w
 void launch_vjp(cfd_ctx* c, const double* U, const double* w, double* JTw);
 // vector kernels
 void launch_axpby(cfd_ctx* c, double* y, double a, const double* x, double b, size_t n);
@@ -104,8 +105,14 @@ void launch_explicit(cfd_ctx* c, double* Unew, const double* U0, double a0, cons
 void launch_wall(cfd_ctx* c, const double* U, double* out3);
 // Krylov (precision-generic basis)
 void launch_krylov_init(cfd_ctx* c, void* V0, const double* rhs, double sign, int slot, int prec);
-void launch_precond_apply(cfd_ctx* c, double* z, const void* Vj, int prec, int slot_norm);
+void launch_precond_apply(cfd_ctx* c, void* z, const void* Vj, int prec, int slot_norm);
+void launch_d2f(cfd_ctx* c, const double* a, float* b, size_t n);
+void launch_f2d(cfd_ctx* c, const float* a, double* b, size_t n);
 void launch_orth(cfd_ctx* c, void* V, int j, double* w, int prec, double* hdev);
 void launch_store_basis(cfd_ctx* c, void* Vj1, const double* w, const double* hdev, int j, int prec);
 void launch_solution(cfd_ctx* c, double* x, const void* V, const double* y, int k, int prec);
+// block-Jacobi: assemble + invert (Binv: nv*nv x pitch of the basis type), apply, solution
+void launch_block_precond(cfd_ctx* c, const double* U, void* Binv, int prec, double* Bout = nullptr);
+void launch_block_apply(cfd_ctx* c, const void* Binv, const void* x, void* z, int prec);
+void launch_solution_bj(cfd_ctx* c, double* x, const void* V, const double* y, int k, const void* Binv, int prec);
 }  // namespace cfd

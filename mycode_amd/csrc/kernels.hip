@@ -43,20 +43,56 @@ __device__ __forceinline__ S6<double> ld(const T* __restrict__ U, int p, int c) 
   return s;
 }
 
+// ---- scalar-type plumbing: T = double (residual) or Dual<F> (tangent) -----------
+// A state is loaded from U (FP64) and, for the tangent, the direction v (type F).
+template <class T> struct Scal {
+  typedef double F;
+  static __device__ __forceinline__ T mk(double u, F) { return u; }
+  static __device__ __forceinline__ double dpart(T x) { return x; }
+  static __device__ __forceinline__ void put(F* v, F*, size_t i, T x) { v[i] = x; }
+  static __device__ __forceinline__ T get(const F* v, const F*, size_t i) { return v[i]; }
+};
+template <class F_> struct Scal<Dual<F_>> {
+  typedef F_ F;
+  static __device__ __forceinline__ Dual<F> mk(double u, F d) { return Dual<F>(F(u), d); }
+  static __device__ __forceinline__ F dpart(Dual<F> x) { return x.d; }
+  static __device__ __forceinline__ void put(F* v, F* d, size_t i, Dual<F> x) {
+    v[i] = x.v;
+    d[i] = x.d;
+  }
+  static __device__ __forceinline__ Dual<F> get(const F* v, const F* d, size_t i) { return Dual<F>(v[i], d[i]); }
+};
+
+template <int NV, class T>
+__device__ __forceinline__ S6<T> ldT(const double* __restrict__ U, const typename Scal<T>::F* __restrict__ v, int p,
+                                     int c) {
+  S6<T> s;
+#pragma unroll
+  for (int k = 0; k < NV; ++k) s[k] = Scal<T>::mk(U[k * p + c], v ? v[k * p + c] : typename Scal<T>::F(0));
+  return s;
+}
+
 // ---- gradients + limiter ------------------------------------------------------
-template <int NV>
+// Gv / Gd: 10 x pitch values (and, for the tangent, derivatives) of the
+// Green-Gauss gradients in CellGradient order.  The limiter is only formed in
+// the FP64 residual; the tangent keeps it frozen.
+template <int NV, class T>
 __global__ __launch_bounds__(BS) void k_gradlim(DMesh m, Phys P, const double* __restrict__ U,
-                                                 double* __restrict__ G, double* __restrict__ phi, int do_lim) {
+                                                 const typename Scal<T>::F* __restrict__ v,
+                                                 typename Scal<T>::F* __restrict__ Gv,
+                                                 typename Scal<T>::F* __restrict__ Gd, double* __restrict__ phi,
+                                                 int do_lim) {
+  typedef Scal<T> Sc;
   const int c = xcd_block() * BS + threadIdx.x;
   if (c >= m.N) return;
   const int p = m.pitch;
   const double Rg = P.R;
-  S6<double> Uc = ld<NV>(U, p, c);
-  const double uc = Uc[1] / Uc[0], vc = Uc[2] / Uc[0], pc = pressure(P, Uc), Tc = pc / (Uc[0] * Rg);
-  double g[10];
+  const S6<T> Uc = ldT<NV, T>(U, v, p, c);
+  const T uc = Uc[1] / Uc[0], vc = Uc[2] / Uc[0], pc = pressure(P, Uc), Tc = pc / (Uc[0] * Rg);
+  T g[10];
 #pragma unroll
-  for (int q = 0; q < 10; ++q) g[q] = 0.0;
-  double pmax = pc, pmin = pc;
+  for (int q = 0; q < 10; ++q) g[q] = T(0.0);
+  double pmax = val(pc), pmin = val(pc);
   const int4 s4 = m.slots[c];
   const int sl[4] = {s4.x, s4.y, s4.z, s4.w};
 #pragma unroll
@@ -67,139 +103,143 @@ __global__ __launch_bounds__(BS) void k_gradlim(DMesh m, Phys P, const double* _
     const bool as_nei = e & 1;
     const int other = as_nei ? m.own[f] : m.nei[f];
     const double nx = m.nx[f], ny = m.ny[f], A = m.area[f];
-    S6<double> Uo = other >= 0 ? ld<NV>(U, p, other) : ghost(P, Uc, m.tag[f], nx, ny);
-    const double uo = Uo[1] / Uo[0], vo = Uo[2] / Uo[0], po = pressure(P, Uo), To = po / (Uo[0] * Rg);
-    const double fv[5] = {0.5 * (Uc[0] + Uo[0]), 0.5 * (uc + uo), 0.5 * (vc + vo), 0.5 * (pc + po),
-                          0.5 * (Tc + To)};
+    const S6<T> Uo = other >= 0 ? ldT<NV, T>(U, v, p, other) : ghost(P, Uc, m.tag[f], nx, ny);
+    const T uo = Uo[1] / Uo[0], vo = Uo[2] / Uo[0], po = pressure(P, Uo), To = po / (Uo[0] * Rg);
+    const T fv[5] = {0.5 * (Uc[0] + Uo[0]), 0.5 * (uc + uo), 0.5 * (vc + vo), 0.5 * (pc + po), 0.5 * (Tc + To)};
     if (!as_nei) {
 #pragma unroll
       for (int q = 0; q < 5; ++q) {
-        g[2 * q] += fv[q] * nx * A;
-        g[2 * q + 1] += fv[q] * ny * A;
+        g[2 * q] = g[2 * q] + fv[q] * nx * A;
+        g[2 * q + 1] = g[2 * q + 1] + fv[q] * ny * A;
       }
     } else {
 #pragma unroll
       for (int q = 0; q < 5; ++q) {
-        g[2 * q] -= fv[q] * nx * A;
-        g[2 * q + 1] -= fv[q] * ny * A;
+        g[2 * q] = g[2 * q] - fv[q] * nx * A;
+        g[2 * q + 1] = g[2 * q + 1] - fv[q] * ny * A;
       }
     }
-    pmax = mx(pmax, po);
-    pmin = mn(pmin, po);
+    pmax = mx(pmax, (double)val(po));
+    pmin = mn(pmin, (double)val(po));
   }
   const double vol = m.vol[c];
 #pragma unroll
   for (int q = 0; q < 10; ++q) {
-    g[q] /= vol;
-    G[q * p + c] = g[q];
+    g[q] = g[q] / vol;
+    Sc::put(Gv, Gd, (size_t)q * p + c, g[q]);
   }
-  if (!do_lim) return;
-  const double xc = m.xc[c], yc = m.yc[c];
-  double ph = 1.0;
+  if constexpr (std::is_same<T, double>::value) {
+    if (!do_lim) return;
+    const double xc = m.xc[c], yc = m.yc[c];
+    double ph = 1.0;
 #pragma unroll
-  for (int k = 0; k < 4; ++k) {
-    const int e = sl[k];
-    if (e < 0) continue;
-    const int f = e >> 1;
-    const double dx = m.xm[f] - xc, dy = m.ym[f] - yc;
-    const double Tf = Tc + 1.0 * (g[8] * dx + g[9] * dy);
-    const double Pf = Uc[0] * Rg * Tf;
-    const double d = Pf - pc;
-    double a = 1.0;
-    if (d > 0.0) a = mn(1.0, (pmax - pc) / (d + 1e-12));
-    else if (d < 0.0) a = mn(1.0, (pmin - pc) / (d - 1e-12));
-    ph = mn(ph, a);
+    for (int k = 0; k < 4; ++k) {
+      const int e = sl[k];
+      if (e < 0) continue;
+      const int f = e >> 1;
+      const double dx = m.xm[f] - xc, dy = m.ym[f] - yc;
+      const double Tf = Tc + 1.0 * (g[8] * dx + g[9] * dy);
+      const double Pf = Uc[0] * Rg * Tf;
+      const double d = Pf - pc;
+      double a = 1.0;
+      if (d > 0.0) a = mn(1.0, (pmax - pc) / (d + 1e-12));
+      else if (d < 0.0) a = mn(1.0, (pmin - pc) / (d - 1e-12));
+      ph = mn(ph, a);
+    }
+    phi[c] = ph;
   }
-  phi[c] = ph;
 }
 
 // ---- face flux ------------------------------------------------------------------------
-struct Grad8 {
-  double v[10];
+// Geometry and frozen quantities of one face, as the flux core sees them.
+struct FaceGeo {
+  double nx, ny, A, xm, ym, xco, yco, xcn, ycn, phL, phR;
+  int tg, scheme;
+  bool boundary;
 };
 
-template <int NV, int VISC>
-__device__ __forceinline__ S6<double> face_flux(const DMesh& m, const Phys& P, const double* __restrict__ U,
-                                                const double* __restrict__ G, const double* __restrict__ phi,
-                                                int f, int& o, int& n) {
-  const int p = m.pitch;
-  o = m.own[f];
-  n = m.nei[f];
-  const int tg = m.tag[f];
-  const double nx = m.nx[f], ny = m.ny[f];
-  const S6<double> L = ld<NV>(U, p, o);
-  const S6<double> R = n >= 0 ? ld<NV>(U, p, n) : ghost(P, L, tg, nx, ny);
-  S6<double> Lr = L, Rr = R;
-  const bool second = P.spatial != CFD_FIRST_ORDER && n >= 0;
-  double gL[10], gR[10];
-  if (second || VISC) {
-#pragma unroll
-    for (int q = 0; q < 10; ++q) gL[q] = G[q * p + o];
+__device__ __forceinline__ FaceGeo face_geo(const DMesh& m, const Phys& P, const double* __restrict__ phi, int f,
+                                            int o, int n, bool need_cells) {
+  FaceGeo g;
+  g.nx = m.nx[f];
+  g.ny = m.ny[f];
+  g.A = m.area[f];
+  g.tg = m.tag[f];
+  g.boundary = n < 0;
+  g.scheme = (m.alt && m.alt[f]) ? P.flux_alt : P.flux;
+  g.xm = g.ym = g.xco = g.yco = g.xcn = g.ycn = 0.0;
+  g.phL = g.phR = 0.5;
+  if (need_cells) {
+    g.xm = m.xm[f];
+    g.ym = m.ym[f];
+    g.xco = m.xc[o];
+    g.yco = m.yc[o];
     if (n >= 0) {
-#pragma unroll
-      for (int q = 0; q < 10; ++q) gR[q] = G[q * p + n];
-    } else {
-#pragma unroll
-      for (int q = 0; q < 10; ++q) gR[q] = gL[q];
+      g.xcn = m.xc[n];
+      g.ycn = m.yc[n];
+    }
+    if (P.spatial == CFD_SECOND_ORDER_LIMITED) {
+      g.phL = phi[o];
+      g.phR = n >= 0 ? phi[n] : 0.5;
     }
   }
-  double xco = 0, yco = 0, xcn = 0, ycn = 0;
-  if (second || VISC) {
-    xco = m.xc[o];
-    yco = m.yc[o];
-    if (n >= 0) {
-      xcn = m.xc[n];
-      ycn = m.yc[n];
-    }
-  }
-  if (second) {
-    const double xm = m.xm[f], ym = m.ym[f];
-    const double dxL = xm - xco, dyL = ym - yco, dxR = xm - xcn, dyR = ym - ycn;
-    const bool lim = P.spatial == CFD_SECOND_ORDER_LIMITED;
-    const double phL = lim ? phi[o] : 0.5, phR = lim ? phi[n] : 0.5;
-    const double uL = L[1] / L[0], vL = L[2] / L[0], pL = pressure(P, L);
-    double rLf = L[0] + phL * (gL[0] * dxL + gL[1] * dyL);
-    double pLf = pL + phL * (gL[6] * dxL + gL[7] * dyL);
-    const double uLf = uL + phL * (gL[2] * dxL + gL[3] * dyL);
-    const double vLf = vL + phL * (gL[4] * dxL + gL[5] * dyL);
-    if (rLf < 1e-4) rLf = L[0];
-    if (pLf < 1e-4) pLf = pL;
+  return g;
+}
+
+// F * A of one face from its owner state L, right state R (neighbor or
+// ghost), and the two cells' gradients (Solver.cpp:543-799): MUSCL
+// reconstruction with the reconstruction clips, wall pressure flux or Riemann
+// flux, viscous flux with the centroid-line correction.
+template <int VISC, class T>
+__device__ __forceinline__ S6<T> flux_core(const Phys& P, const FaceGeo& g, const S6<T>& L, const S6<T>& R,
+                                           const T* gL, const T* gR) {
+  const double nx = g.nx, ny = g.ny;
+  S6<T> Lr = L, Rr = R;
+  if (P.spatial != CFD_FIRST_ORDER && !g.boundary) {
+    const double dxL = g.xm - g.xco, dyL = g.ym - g.yco, dxR = g.xm - g.xcn, dyR = g.ym - g.ycn;
+    const double phL = g.phL, phR = g.phR;
+    const T uL = L[1] / L[0], vL = L[2] / L[0], pL = pressure(P, L);
+    T rLf = L[0] + phL * (gL[0] * dxL + gL[1] * dyL);
+    T pLf = pL + phL * (gL[6] * dxL + gL[7] * dyL);
+    const T uLf = uL + phL * (gL[2] * dxL + gL[3] * dyL);
+    const T vLf = vL + phL * (gL[4] * dxL + gL[5] * dyL);
+    if (val(rLf) < 1e-4) rLf = L[0];
+    if (val(pLf) < 1e-4) pLf = pL;
     Lr = prim2cons(P, rLf, uLf, vLf, pLf);
-    const double uR = R[1] / R[0], vR = R[2] / R[0], pR = pressure(P, R);
-    double rRf = R[0] + phR * (gR[0] * dxR + gR[1] * dyR);
-    double pRf = pR + phR * (gR[6] * dxR + gR[7] * dyR);
-    const double uRf = uR + phR * (gR[2] * dxR + gR[3] * dyR);
-    const double vRf = vR + phR * (gR[4] * dxR + gR[5] * dyR);
-    if (rRf < 1e-4) rRf = R[0];
-    if (pRf < 1e-4) pRf = pR;
+    const T uR = R[1] / R[0], vR = R[2] / R[0], pR = pressure(P, R);
+    T rRf = R[0] + phR * (gR[0] * dxR + gR[1] * dyR);
+    T pRf = pR + phR * (gR[6] * dxR + gR[7] * dyR);
+    const T uRf = uR + phR * (gR[2] * dxR + gR[3] * dyR);
+    const T vRf = vR + phR * (gR[4] * dxR + gR[5] * dyR);
+    if (val(rRf) < 1e-4) rRf = R[0];
+    if (val(pRf) < 1e-4) pRf = pR;
     Rr = prim2cons(P, rRf, uRf, vRf, pRf);
   }
-  S6<double> Fl;
-  if (n < 0 && (tg == 3 || tg == 4)) {
-    const double pw_ = pressure(P, Lr);
-    Fl[0] = 0.0;
+  S6<T> Fl;
+  if (g.boundary && (g.tg == 3 || g.tg == 4)) {
+    const T pw_ = pressure(P, Lr);
+    Fl[0] = T(0.0);
     Fl[1] = pw_ * nx;
     Fl[2] = pw_ * ny;
-    Fl[3] = 0.0;
+    Fl[3] = T(0.0);
   } else {
-    const int scheme = (m.alt && m.alt[f]) ? P.flux_alt : P.flux;
-    Fl = riemann(P, Lr, Rr, nx, ny, scheme);
+    Fl = riemann(P, Lr, Rr, nx, ny, g.scheme);
   }
   if (VISC) {
-    const double rf = 0.5 * (L[0] + R[0]);
-    const double uf = 0.5 * (L[1] / L[0] + R[1] / R[0]);
-    const double vf = 0.5 * (L[2] / L[0] + R[2] / R[0]);
-    double ux, uy, vx, vy, Tx, Ty;
-    if (n < 0 && tg == 4) {
-      const double dx = m.xm[f] - xco, dy = m.ym[f] - yco;
+    const T rf = 0.5 * (L[0] + R[0]);
+    const T uf = 0.5 * (L[1] / L[0] + R[1] / R[0]);
+    const T vf = 0.5 * (L[2] / L[0] + R[2] / R[0]);
+    T ux, uy, vx, vy, Tx, Ty;
+    if (g.boundary && g.tg == 4) {
+      const double dx = g.xm - g.xco, dy = g.ym - g.yco;
       const double d2 = dx * dx + dy * dy;
-      const double uc = L[1] / L[0], vc = L[2] / L[0];
+      const T uc = L[1] / L[0], vc = L[2] / L[0];
       ux = -uc * dx / d2;
       uy = -uc * dy / d2;
       vx = -vc * dx / d2;
       vy = -vc * dy / d2;
-      Tx = 0.0;
-      Ty = 0.0;
+      Tx = T(0.0);
+      Ty = T(0.0);
     } else {
       ux = 0.5 * (gL[2] + gR[2]);
       uy = 0.5 * (gL[3] + gR[3]);
@@ -207,13 +247,13 @@ __device__ __forceinline__ S6<double> face_flux(const DMesh& m, const Phys& P, c
       vy = 0.5 * (gL[5] + gR[5]);
       Tx = 0.5 * (gL[8] + gR[8]);
       Ty = 0.5 * (gL[9] + gR[9]);
-      if (n >= 0) {
-        const double dxLR = xcn - xco, dyLR = ycn - yco;
+      if (!g.boundary) {
+        const double dxLR = g.xcn - g.xco, dyLR = g.ycn - g.yco;
         const double d2 = dxLR * dxLR + dyLR * dyLR;
         if (d2 > 1e-14) {
-          const double du = (R[1] / R[0]) - (L[1] / L[0]);
-          const double dv = (R[2] / R[0]) - (L[2] / L[0]);
-          const double ux0 = ux, uy0 = uy, vx0 = vx, vy0 = vy;
+          const T du = (R[1] / R[0]) - (L[1] / L[0]);
+          const T dv = (R[2] / R[0]) - (L[2] / L[0]);
+          const T ux0 = ux, uy0 = uy, vx0 = vx, vy0 = vy;
           ux = ux0 - (ux0 * dxLR + uy0 * dyLR - du) * dxLR / d2;
           uy = uy0 - (ux0 * dxLR + uy0 * dyLR - du) * dyLR / d2;
           vx = vx0 - (vx0 * dxLR + vy0 * dyLR - dv) * dxLR / d2;
@@ -221,15 +261,15 @@ __device__ __forceinline__ S6<double> face_flux(const DMesh& m, const Phys& P, c
         }
       }
     }
-    const double mu = rf * P.nu;
-    const double kc = mu * P.Cp / P.Pr;
-    const double div = ux + vy;
-    const double txx = mu * (2.0 * ux - (2.0 / 3.0) * div);
-    const double tyy = mu * (2.0 * vy - (2.0 / 3.0) * div);
-    const double txy = mu * (uy + vx);
-    const double qx = -kc * Tx, qy = -kc * Ty;
-    const double wx = uf * txx + vf * txy - qx;
-    const double wy = uf * txy + vf * tyy - qy;
+    const T mu = rf * P.nu;
+    const T kc = mu * P.Cp / P.Pr;
+    const T div = ux + vy;
+    const T txx = mu * (2.0 * ux - (2.0 / 3.0) * div);
+    const T tyy = mu * (2.0 * vy - (2.0 / 3.0) * div);
+    const T txy = mu * (uy + vx);
+    const T qx = -kc * Tx, qy = -kc * Ty;
+    const T wx = uf * txx + vf * txy - qx;
+    const T wy = uf * txy + vf * tyy - qy;
     Fl[0] = Fl[0] - 0.0;
     Fl[1] = Fl[1] - (txx * nx + txy * ny);
     Fl[2] = Fl[2] - (txy * nx + tyy * ny);
@@ -237,42 +277,75 @@ __device__ __forceinline__ S6<double> face_flux(const DMesh& m, const Phys& P, c
     Fl[4] = Fl[4] - 0.0;
     Fl[5] = Fl[5] - 0.0;
   }
-  const double A = m.area[f];
-  return scale(Fl, A);
+  return scale(Fl, T(g.A));
 }
 
-// MODE 0: scatter into R for faces [f0, f1) of one color.  MODE 1: face buffer.
-template <int NV, int VISC, int MODE>
+template <int NV, int VISC, class T>
+__device__ __forceinline__ S6<T> face_flux(const DMesh& m, const Phys& P, const double* __restrict__ U,
+                                           const typename Scal<T>::F* __restrict__ v,
+                                           const typename Scal<T>::F* __restrict__ Gv,
+                                           const typename Scal<T>::F* __restrict__ Gd,
+                                           const double* __restrict__ phi, int f, int& o, int& n) {
+  typedef Scal<T> Sc;
+  const int p = m.pitch;
+  o = m.own[f];
+  n = m.nei[f];
+  const bool second = P.spatial != CFD_FIRST_ORDER && n >= 0;
+  const FaceGeo g = face_geo(m, P, phi, f, o, n, second || VISC);
+  const S6<T> L = ldT<NV, T>(U, v, p, o);
+  const S6<T> R = n >= 0 ? ldT<NV, T>(U, v, p, n) : ghost(P, L, g.tg, g.nx, g.ny);
+  T gL[10], gR[10];
+  if (second || VISC) {
+#pragma unroll
+    for (int q = 0; q < 10; ++q) gL[q] = Sc::get(Gv, Gd, (size_t)q * p + o);
+    if (n >= 0) {
+#pragma unroll
+      for (int q = 0; q < 10; ++q) gR[q] = Sc::get(Gv, Gd, (size_t)q * p + n);
+    } else {
+#pragma unroll
+      for (int q = 0; q < 10; ++q) gR[q] = gL[q];
+    }
+  }
+  return flux_core<VISC, T>(P, g, L, R, gL, gR);
+}
+
+// MODE 0: scatter into `out` (the residual) for faces [f0, f1) of one color.
+// MODE 1: write the face buffer.  The tangent writes derivative parts only.
+template <int NV, int VISC, int MODE, class T>
 __global__ __launch_bounds__(BS) void k_flux(DMesh m, Phys P, const double* __restrict__ U,
-                                              const double* __restrict__ G, const double* __restrict__ phi,
-                                              double* __restrict__ out, int f0, int f1) {
+                                              const typename Scal<T>::F* __restrict__ v,
+                                              const typename Scal<T>::F* __restrict__ Gv,
+                                              const typename Scal<T>::F* __restrict__ Gd,
+                                              const double* __restrict__ phi, typename Scal<T>::F* __restrict__ out,
+                                              int f0, int f1) {
+  typedef Scal<T> Sc;
   const int f = f0 + xcd_block() * BS + threadIdx.x;
   if (f >= f1) return;
   int o, n;
-  const S6<double> TA = face_flux<NV, VISC>(m, P, U, G, phi, f, o, n);
+  const S6<T> TA = face_flux<NV, VISC, T>(m, P, U, v, Gv, Gd, phi, f, o, n);
   if (MODE == 0) {
     const int p = m.pitch;
 #pragma unroll
-    for (int k = 0; k < NV; ++k) out[k * p + o] = out[k * p + o] + TA[k];
+    for (int k = 0; k < NV; ++k) out[k * p + o] = out[k * p + o] + Sc::dpart(TA[k]);
     if (n >= 0) {
 #pragma unroll
-      for (int k = 0; k < NV; ++k) out[k * p + n] = out[k * p + n] - TA[k];
+      for (int k = 0; k < NV; ++k) out[k * p + n] = out[k * p + n] - Sc::dpart(TA[k]);
     }
   } else {
 #pragma unroll
-    for (int k = 0; k < NV; ++k) out[(size_t)k * m.F + f] = TA[k];
+    for (int k = 0; k < NV; ++k) out[(size_t)k * m.F + f] = Sc::dpart(TA[k]);
   }
 }
 
-template <int NV>
-__global__ __launch_bounds__(BS) void k_gather(DMesh m, const double* __restrict__ Fm, double* __restrict__ R) {
+template <int NV, class F>
+__global__ __launch_bounds__(BS) void k_gather(DMesh m, const F* __restrict__ Fm, F* __restrict__ R) {
   const int c = xcd_block() * BS + threadIdx.x;
   if (c >= m.N) return;
   const int4 s4 = m.slots[c];
   const int sl[4] = {s4.x, s4.y, s4.z, s4.w};
-  double r[NV];
+  F r[NV];
 #pragma unroll
-  for (int k = 0; k < NV; ++k) r[k] = 0.0;
+  for (int k = 0; k < NV; ++k) r[k] = F(0);
 #pragma unroll
   for (int j = 0; j < 4; ++j) {
     const int e = sl[j];
@@ -290,7 +363,172 @@ __global__ __launch_bounds__(BS) void k_gather(DMesh m, const double* __restrict
   for (int k = 0; k < NV; ++k) R[k * m.pitch + c] = r[k];
 }
 
-__global__ void k_fill(double* __restrict__ x, double v, size_t n) {
+// Tangent Jv: J v = dR/dU v + (vol / dt) v  (Solver.cpp:1381-1384 time term)
+template <int NV, class F>
+__global__ __launch_bounds__(BS) void k_tcombine(int N, int p, F* __restrict__ Jv, const F* __restrict__ v,
+                                                  const double* __restrict__ vol, const double* __restrict__ dt) {
+  const int c = blockIdx.x * BS + threadIdx.x;
+  if (c >= N) return;
+  const double tc = vol[c] / (dt[c] + 1e-30);
+#pragma unroll
+  for (int k = 0; k < NV; ++k) Jv[k * p + c] = (F)((double)Jv[k * p + c] + (double)v[k * p + c] * tc);
+}
+
+// ---- reverse product J^T w ------------------------------------------------------------
+// With the limiter frozen, R_c = sum_f s_cf F_f A_f(U_o, U_n, G_o, G_n) and
+// G_c = (1/V_c) sum_f s_cf A_f n_f q_f(U_o, U_n), q = face mean of (rho, u, v,
+// p, T).  The transpose product is formed in two colored face sweeps:
+//   k_vjp_face  w^T dF/d(U_o, U_n, G_o, G_n) per face, lambda = w_o - w_n,
+//               by VK-direction forward duals over the face's inputs
+//               (gradients recomputed, never stored per face);
+//   k_vjp_gg    the Green-Gauss transpose: aG_c / V_c back onto the face means
+//               and through the primitive (and ghost) maps onto U.
+constexpr int VK = 4;
+
+template <int NV, int VISC>
+__global__ __launch_bounds__(BS) void k_vjp_face(DMesh m, Phys P, const double* __restrict__ U,
+                                                  const double* __restrict__ G, const double* __restrict__ phi,
+                                                  const double* __restrict__ w, double* __restrict__ aU,
+                                                  double* __restrict__ aG, int f0, int f1) {
+  typedef DualV<double, VK> T;
+  const int f = f0 + xcd_block() * BS + threadIdx.x;
+  if (f >= f1) return;
+  const int p = m.pitch;
+  const int o = m.own[f], n = m.nei[f];
+  const bool interior = n >= 0;
+  const bool need = (P.spatial != CFD_FIRST_ORDER && interior) || VISC;
+  const FaceGeo g = face_geo(m, P, phi, f, o, n, need);
+  double Uo[NV], Un[NV], lam[NV];
+#pragma unroll
+  for (int k = 0; k < NV; ++k) {
+    Uo[k] = U[k * p + o];
+    Un[k] = interior ? U[k * p + n] : 0.0;
+    lam[k] = w[k * p + o] - (interior ? w[k * p + n] : 0.0);
+  }
+  const int nU = interior ? 2 * NV : NV;
+  const int nin = nU + (need ? (interior ? 20 : 10) : 0);
+  for (int base = 0; base < nin; base += VK) {
+    S6<T> L, R;
+    T gL[10], gR[10];
+#pragma unroll
+    for (int j = 0; j < NV; ++j) {
+      L[j] = T(Uo[j]);
+#pragma unroll
+      for (int d = 0; d < VK; ++d) L[j].d[d] = (j == base + d) ? 1.0 : 0.0;
+    }
+    if (interior) {
+#pragma unroll
+      for (int j = 0; j < NV; ++j) {
+        R[j] = T(Un[j]);
+#pragma unroll
+        for (int d = 0; d < VK; ++d) R[j].d[d] = (NV + j == base + d) ? 1.0 : 0.0;
+      }
+    } else {
+      R = ghost(P, L, g.tg, g.nx, g.ny);
+    }
+    if (need) {
+#pragma unroll
+      for (int q = 0; q < 10; ++q) {
+        gL[q] = T(G[(size_t)q * p + o]);
+#pragma unroll
+        for (int d = 0; d < VK; ++d) gL[q].d[d] = (nU + q == base + d) ? 1.0 : 0.0;
+      }
+      if (interior) {
+#pragma unroll
+        for (int q = 0; q < 10; ++q) {
+          gR[q] = T(G[(size_t)q * p + n]);
+#pragma unroll
+          for (int d = 0; d < VK; ++d) gR[q].d[d] = (nU + 10 + q == base + d) ? 1.0 : 0.0;
+        }
+      } else {
+#pragma unroll
+        for (int q = 0; q < 10; ++q) gR[q] = gL[q];
+      }
+    }
+    const S6<T> F = flux_core<VISC, T>(P, g, L, R, gL, gR);
+#pragma unroll
+    for (int d = 0; d < VK; ++d) {
+      const int idx = base + d;
+      if (idx >= nin) break;
+      double s = 0.0;
+#pragma unroll
+      for (int k = 0; k < NV; ++k) s += lam[k] * F[k].d[d];
+      double* dst;
+      if (idx < NV) dst = aU + (size_t)idx * p + o;
+      else if (idx < nU) dst = aU + (size_t)(idx - NV) * p + n;
+      else if (idx < nU + 10) dst = aG + (size_t)(idx - nU) * p + o;
+      else dst = aG + (size_t)(idx - nU - 10) * p + n;
+      *dst = *dst + s;
+    }
+  }
+}
+
+template <class T> __device__ __forceinline__ void prims(const Phys& P, const S6<T>& U, T* q) {
+  q[0] = U[0];
+  q[1] = U[1] / U[0];
+  q[2] = U[2] / U[0];
+  q[3] = pressure(P, U);
+  q[4] = q[3] / (U[0] * P.R);
+}
+
+template <int NV>
+__global__ __launch_bounds__(BS) void k_vjp_gg(DMesh m, Phys P, const double* __restrict__ U,
+                                                const double* __restrict__ aG, double* __restrict__ aU, int f0,
+                                                int f1) {
+  typedef DualV<double, NV> T;
+  const int f = f0 + xcd_block() * BS + threadIdx.x;
+  if (f >= f1) return;
+  const int p = m.pitch;
+  const int o = m.own[f], n = m.nei[f];
+  const double nx = m.nx[f], ny = m.ny[f], A = m.area[f];
+  const double vo = m.vol[o];
+  double aq[5];
+#pragma unroll
+  for (int k = 0; k < 5; ++k) {
+    aq[k] = A * ((aG[(size_t)(2 * k) * p + o] * nx + aG[(size_t)(2 * k + 1) * p + o] * ny) / vo);
+    if (n >= 0)
+      aq[k] -= A * ((aG[(size_t)(2 * k) * p + n] * nx + aG[(size_t)(2 * k + 1) * p + n] * ny) / m.vol[n]);
+  }
+  S6<T> L;
+#pragma unroll
+  for (int j = 0; j < NV; ++j) {
+    L[j] = T(U[j * p + o]);
+#pragma unroll
+    for (int d = 0; d < NV; ++d) L[j].d[d] = j == d ? 1.0 : 0.0;
+  }
+  T qL[5], qR[5];
+  prims(P, L, qL);
+  if (n >= 0) {
+    S6<T> R;
+#pragma unroll
+    for (int j = 0; j < NV; ++j) {
+      R[j] = T(U[j * p + n]);
+#pragma unroll
+      for (int d = 0; d < NV; ++d) R[j].d[d] = j == d ? 1.0 : 0.0;
+    }
+    prims(P, R, qR);
+  } else {
+    prims(P, ghost(P, L, m.tag[f], nx, ny), qR);
+  }
+#pragma unroll
+  for (int j = 0; j < NV; ++j) {
+    double so = 0.0, sn = 0.0;
+#pragma unroll
+    for (int k = 0; k < 5; ++k) {
+      so += aq[k] * 0.5 * qL[k].d[j];
+      sn += aq[k] * 0.5 * qR[k].d[j];
+    }
+    if (n >= 0) {
+      aU[(size_t)j * p + o] += so;
+      aU[(size_t)j * p + n] += sn;
+    } else {
+      aU[(size_t)j * p + o] += so + sn;
+    }
+  }
+}
+
+template <class F>
+__global__ void k_fill(F* __restrict__ x, F v, size_t n) {
   size_t i = (size_t)blockIdx.x * BS + threadIdx.x;
   if (i < n) x[i] = v;
 }
@@ -360,6 +598,171 @@ __global__ __launch_bounds__(BS) void k_diagprec(DMesh m, Phys P, const double* 
   double d = (m.vol[c] / (dt[c] + 1e-30)) + acc;
   if (d < 1e-12) d = 1e-12;
   D[c] = d;
+}
+
+// ---- block-Jacobi preconditioner ------------------------------------------------------
+// Diagonal block of the first-order Jacobian of cell c:
+//   B_c = (V/dtau) I + sum_f (+-) A dF(U_L, U_R)/dU_c + nu A / d I,
+// each column k formed by a forward-mode dual with direction e_k on U_c (the
+// ghost state of a boundary face is differentiated with it), d the normal
+// distance between the centroids (twice the centroid-face distance at a
+// boundary).  B_c is inverted by Gauss-Jordan with partial pivoting in FP64
+// and stored as TV, NV*NV rows of pitch.
+template <int NV, class TV>
+__global__ __launch_bounds__(BS) void k_block(DMesh m, Phys P, const double* __restrict__ U,
+                                               const double* __restrict__ dt, TV* __restrict__ Binv,
+                                               double* __restrict__ Bout) {
+  typedef Dual<double> D;
+  const int c = xcd_block() * BS + threadIdx.x;
+  if (c >= m.N) return;
+  const int p = m.pitch;
+  double B[NV][NV];
+#pragma unroll
+  for (int i = 0; i < NV; ++i)
+#pragma unroll
+    for (int j = 0; j < NV; ++j) B[i][j] = 0.0;
+  const S6<double> Uc = ld<NV>(U, p, c);
+  const int4 s4 = m.slots[c];
+  const int sl[4] = {s4.x, s4.y, s4.z, s4.w};
+  double visc = 0.0;
+  for (int q = 0; q < 4; ++q) {
+    const int e = sl[q];
+    if (e < 0) continue;
+    const int f = e >> 1;
+    const bool as_nei = e & 1;
+    const int o = m.own[f], n = m.nei[f], tg = m.tag[f];
+    const double nx = m.nx[f], ny = m.ny[f], A = m.area[f];
+    const int other = as_nei ? o : n;
+    const S6<double> Uo = other >= 0 ? ld<NV>(U, p, other) : Uc;
+    const int scheme = (m.alt && m.alt[f]) ? P.flux_alt : P.flux;
+#pragma unroll
+    for (int k = 0; k < NV; ++k) {
+      S6<D> Ud, Od;
+#pragma unroll
+      for (int i = 0; i < 6; ++i) {
+        Ud[i] = D(Uc[i], i == k ? 1.0 : 0.0);
+        Od[i] = D(Uo[i], 0.0);
+      }
+      S6<D> Fl;
+      if (n < 0) {
+        if (tg == 3 || tg == 4) {
+          const D pw_ = pressure(P, Ud);
+          Fl[1] = pw_ * nx;
+          Fl[2] = pw_ * ny;
+        } else {
+          Fl = riemann(P, Ud, ghost(P, Ud, tg, nx, ny), nx, ny, scheme);
+        }
+      } else if (!as_nei) {
+        Fl = riemann(P, Ud, Od, nx, ny, scheme);
+      } else {
+        Fl = riemann(P, Od, Ud, nx, ny, scheme);
+      }
+      const double sg = as_nei ? -A : A;
+#pragma unroll
+      for (int i = 0; i < NV; ++i) B[i][k] += sg * Fl[i].d;
+    }
+    if (P.viscous) {
+      double d;
+      if (other >= 0) {
+        d = fabs((m.xc[other] - m.xc[c]) * nx + (m.yc[other] - m.yc[c]) * ny);
+      } else {
+        d = 2.0 * fabs((m.xm[f] - m.xc[c]) * nx + (m.ym[f] - m.yc[c]) * ny);
+      }
+      visc += P.nu * A / fmax(d, 1e-300);
+    }
+  }
+  const double tc = m.vol[c] / (dt[c] + 1e-30);
+#pragma unroll
+  for (int i = 0; i < NV; ++i) B[i][i] += tc + (i > 0 ? visc : 0.0);
+  if (Bout) {
+#pragma unroll
+    for (int i = 0; i < NV; ++i)
+#pragma unroll
+      for (int j = 0; j < NV; ++j) Bout[(size_t)(i * NV + j) * p + c] = B[i][j];
+  }
+  // Gauss-Jordan inverse
+  double Inv[NV][NV];
+#pragma unroll
+  for (int i = 0; i < NV; ++i)
+#pragma unroll
+    for (int j = 0; j < NV; ++j) Inv[i][j] = i == j ? 1.0 : 0.0;
+  for (int col = 0; col < NV; ++col) {
+    int piv = col;
+    double best = fabs(B[col][col]);
+    for (int r = col + 1; r < NV; ++r)
+      if (fabs(B[r][col]) > best) {
+        best = fabs(B[r][col]);
+        piv = r;
+      }
+    if (piv != col) {
+      for (int j = 0; j < NV; ++j) {
+        double t = B[col][j];
+        B[col][j] = B[piv][j];
+        B[piv][j] = t;
+        t = Inv[col][j];
+        Inv[col][j] = Inv[piv][j];
+        Inv[piv][j] = t;
+      }
+    }
+    const double ip = 1.0 / B[col][col];
+    for (int j = 0; j < NV; ++j) {
+      B[col][j] *= ip;
+      Inv[col][j] *= ip;
+    }
+    for (int r = 0; r < NV; ++r) {
+      if (r == col) continue;
+      const double fct = B[r][col];
+      for (int j = 0; j < NV; ++j) {
+        B[r][j] -= fct * B[col][j];
+        Inv[r][j] -= fct * Inv[col][j];
+      }
+    }
+  }
+#pragma unroll
+  for (int i = 0; i < NV; ++i)
+#pragma unroll
+    for (int j = 0; j < NV; ++j) Binv[(size_t)(i * NV + j) * p + c] = (TV)Inv[i][j];
+}
+
+// z = Binv_c x_c per cell (x, z of the basis / matvec type)
+template <int NV, class TV, class TZ>
+__global__ __launch_bounds__(BS) void k_block_apply(int N, int p, const TV* __restrict__ Binv,
+                                                     const TV* __restrict__ x, TZ* __restrict__ z) {
+  const int c = blockIdx.x * BS + threadIdx.x;
+  if (c >= N) return;
+  double xv[NV];
+#pragma unroll
+  for (int j = 0; j < NV; ++j) xv[j] = (double)x[j * p + c];
+#pragma unroll
+  for (int i = 0; i < NV; ++i) {
+    double s = 0.0;
+#pragma unroll
+    for (int j = 0; j < NV; ++j) s += (double)Binv[(size_t)(i * NV + j) * p + c] * xv[j];
+    z[i * p + c] = (TZ)s;
+  }
+}
+
+// x = Binv (sum_{i<=k} y_i V_i)
+template <int NV, class TV>
+__global__ __launch_bounds__(BS) void k_solution_bj(int N, int p, size_t vstride, const TV* __restrict__ V,
+                                                     const double* __restrict__ y, int nvec,
+                                                     const TV* __restrict__ Binv, double* __restrict__ x) {
+  const int c = xcd_block() * BS + threadIdx.x;
+  if (c >= N) return;
+  double s[NV];
+#pragma unroll
+  for (int k = 0; k < NV; ++k) {
+    double a = 0.0;
+    for (int i = 0; i < nvec; ++i) a = a + (double)V[(size_t)i * vstride + k * p + c] * y[i];
+    s[k] = a;
+  }
+#pragma unroll
+  for (int i = 0; i < NV; ++i) {
+    double a = 0.0;
+#pragma unroll
+    for (int j = 0; j < NV; ++j) a += (double)Binv[(size_t)(i * NV + j) * p + c] * s[j];
+    x[i * p + c] = a;
+  }
 }
 
 // ---- reductions (deterministic: fixed grid, fixed tree) ------------------------------
@@ -464,6 +867,12 @@ __global__ __launch_bounds__(BS) void k_fd_combine(int N, int p, const double* _
   const double tc = vol[c] / (dt[c] + 1e-30);
 #pragma unroll
   for (int k = 0; k < NV; ++k) Jv[k * p + c] = (Rp[k * p + c] - Rb[k * p + c]) * (1.0 / eps) + v[k * p + c] * tc;
+}
+
+template <class A, class B>
+__global__ void k_convert(const A* __restrict__ a, B* __restrict__ b, size_t n) {
+  size_t i = (size_t)blockIdx.x * BS + threadIdx.x;
+  if (i < n) b[i] = (B)a[i];
 }
 
 // ---- pack / unpack between reference-order AoS and RCM-order SoA ---------------------
@@ -684,7 +1093,7 @@ __global__ __launch_bounds__(BS) void k_solution(int N, int p, size_t vstride, c
 
 void launch_fill(cfd_ctx* c, double* x, double v, size_t n) {
   if (n == 0) return;
-  hipLaunchKernelGGL(k_fill, dim3(nblocks(n)), dim3(BS), 0, c->stream, x, v, n);
+  hipLaunchKernelGGL(k_fill<double>, dim3(nblocks(n)), dim3(BS), 0, c->stream, x, v, n);
 }
 
 template <int NV, int VISC>
@@ -694,7 +1103,8 @@ static void residual_nv(cfd_ctx* c, const double* U, double* R, bool upd, hipEve
   const bool lim = P.spatial == CFD_SECOND_ORDER_LIMITED;
   const int do_lim = lim && (upd || !c->phi_valid);
   if (ev) (void)hipEventRecord(ev[0], c->stream);
-  hipLaunchKernelGGL(k_gradlim<NV>, dim3(nblocks(m.N)), dim3(BS), 0, c->stream, m, P, U, c->grad, c->phi, do_lim);
+  hipLaunchKernelGGL((k_gradlim<NV, double>), dim3(nblocks(m.N)), dim3(BS), 0, c->stream, m, P, U,
+                     (const double*)nullptr, c->grad, (double*)nullptr, c->phi, do_lim);
   if (do_lim) c->phi_valid = true;
   if (ev) (void)hipEventRecord(ev[1], c->stream);
   if (c->assembly == 0) {
@@ -703,18 +1113,98 @@ static void residual_nv(cfd_ctx* c, const double* U, double* R, bool upd, hipEve
     for (int col = 0; col < c->ncolors; ++col) {
       const int f0 = c->color_ptr[col], f1 = c->color_ptr[col + 1];
       if (f1 > f0)
-        hipLaunchKernelGGL((k_flux<NV, VISC, 0>), dim3(nblocks(f1 - f0)), dim3(BS), 0, c->stream, m, P, U, c->grad,
-                           c->phi, R, f0, f1);
+        hipLaunchKernelGGL((k_flux<NV, VISC, 0, double>), dim3(nblocks(f1 - f0)), dim3(BS), 0, c->stream, m, P, U,
+                           (const double*)nullptr, c->grad, (const double*)nullptr, c->phi, R, f0, f1);
     }
     if (ev) (void)hipEventRecord(ev[3], c->stream);
   } else {
     if (ev) (void)hipEventRecord(ev[2], c->stream);
-    hipLaunchKernelGGL((k_flux<NV, VISC, 1>), dim3(nblocks(m.F)), dim3(BS), 0, c->stream, m, P, U, c->grad, c->phi,
-                       c->Fm, 0, m.F);
+    hipLaunchKernelGGL((k_flux<NV, VISC, 1, double>), dim3(nblocks(m.F)), dim3(BS), 0, c->stream, m, P, U,
+                       (const double*)nullptr, c->grad, (const double*)nullptr, c->phi, c->Fm, 0, m.F);
     if (ev) (void)hipEventRecord(ev[3], c->stream);
-    hipLaunchKernelGGL(k_gather<NV>, dim3(nblocks(m.N)), dim3(BS), 0, c->stream, m, c->Fm, R);
+    hipLaunchKernelGGL((k_gather<NV, double>), dim3(nblocks(m.N)), dim3(BS), 0, c->stream, m, c->Fm, R);
   }
   if (ev) (void)hipEventRecord(ev[4], c->stream);
+}
+
+// Forward tangent of the residual with the limiter frozen, element type F.
+template <int NV, int VISC, class F>
+static void tangent_nv(cfd_ctx* c, const double* U, const F* v, F* Jv) {
+  typedef Dual<F> T;
+  const DMesh& m = c->dm;
+  const Phys& P = c->phys;
+  F* Gv = (F*)c->gradT;
+  F* Gd = Gv + (size_t)10 * m.pitch;
+  hipLaunchKernelGGL((k_gradlim<NV, T>), dim3(nblocks(m.N)), dim3(BS), 0, c->stream, m, P, U, v, Gv, Gd, c->phi, 0);
+  if (c->assembly == 0) {
+    hipLaunchKernelGGL(k_fill<F>, dim3(nblocks((size_t)NV * m.pitch)), dim3(BS), 0, c->stream, Jv, F(0),
+                       (size_t)NV * m.pitch);
+    for (int col = 0; col < c->ncolors; ++col) {
+      const int f0 = c->color_ptr[col], f1 = c->color_ptr[col + 1];
+      if (f1 > f0)
+        hipLaunchKernelGGL((k_flux<NV, VISC, 0, T>), dim3(nblocks(f1 - f0)), dim3(BS), 0, c->stream, m, P, U, v,
+                           (const F*)Gv, (const F*)Gd, c->phi, Jv, f0, f1);
+    }
+  } else {
+    F* Fm = (F*)c->Fm;
+    hipLaunchKernelGGL((k_flux<NV, VISC, 1, T>), dim3(nblocks(m.F)), dim3(BS), 0, c->stream, m, P, U, v,
+                       (const F*)Gv, (const F*)Gd, c->phi, Fm, 0, m.F);
+    hipLaunchKernelGGL((k_gather<NV, F>), dim3(nblocks(m.N)), dim3(BS), 0, c->stream, m, (const F*)Fm, Jv);
+  }
+  hipLaunchKernelGGL((k_tcombine<NV, F>), dim3(nblocks(m.N)), dim3(BS), 0, c->stream, m.N, m.pitch, Jv, v, c->d_vol,
+                     c->dt);
+}
+
+template <int NV, int VISC>
+static void vjp_nv(cfd_ctx* c, const double* U, const double* w, double* JTw) {
+  const DMesh& m = c->dm;
+  const Phys& P = c->phys;
+  double* aG = c->adj;
+  hipLaunchKernelGGL((k_gradlim<NV, double>), dim3(nblocks(m.N)), dim3(BS), 0, c->stream, m, P, U,
+                     (const double*)nullptr, c->grad, (double*)nullptr, c->phi, 0);
+  launch_fill(c, JTw, 0.0, (size_t)NV * m.pitch);
+  launch_fill(c, aG, 0.0, (size_t)10 * m.pitch);
+  for (int col = 0; col < c->ncolors; ++col) {
+    const int f0 = c->color_ptr[col], f1 = c->color_ptr[col + 1];
+    if (f1 > f0)
+      hipLaunchKernelGGL((k_vjp_face<NV, VISC>), dim3(nblocks(f1 - f0)), dim3(BS), 0, c->stream, m, P, U,
+                         (const double*)c->grad, (const double*)c->phi, w, JTw, aG, f0, f1);
+  }
+  for (int col = 0; col < c->ncolors; ++col) {
+    const int f0 = c->color_ptr[col], f1 = c->color_ptr[col + 1];
+    if (f1 > f0)
+      hipLaunchKernelGGL((k_vjp_gg<NV>), dim3(nblocks(f1 - f0)), dim3(BS), 0, c->stream, m, P, U,
+                         (const double*)aG, JTw, f0, f1);
+  }
+  hipLaunchKernelGGL((k_tcombine<NV, double>), dim3(nblocks(m.N)), dim3(BS), 0, c->stream, m.N, m.pitch, JTw, w,
+                     c->d_vol, c->dt);
+}
+
+void launch_vjp(cfd_ctx* c, const double* U, const double* w, double* JTw) {
+  const bool visc = c->phys.viscous != 0;
+  if (c->P.nvar == 6) {
+    if (visc) vjp_nv<6, 1>(c, U, w, JTw);
+    else vjp_nv<6, 0>(c, U, w, JTw);
+  } else {
+    if (visc) vjp_nv<4, 1>(c, U, w, JTw);
+    else vjp_nv<4, 0>(c, U, w, JTw);
+  }
+}
+
+template <class F> static void tangent_t(cfd_ctx* c, const double* U, const F* v, F* Jv) {
+  const bool visc = c->phys.viscous != 0;
+  if (c->P.nvar == 6) {
+    if (visc) tangent_nv<6, 1, F>(c, U, v, Jv);
+    else tangent_nv<6, 0, F>(c, U, v, Jv);
+  } else {
+    if (visc) tangent_nv<4, 1, F>(c, U, v, Jv);
+    else tangent_nv<4, 0, F>(c, U, v, Jv);
+  }
+}
+
+void launch_jv_tangent(cfd_ctx* c, const double* U, const void* v, void* Jv, int prec) {
+  if (prec == 32) tangent_t<float>(c, U, (const float*)v, (float*)Jv);
+  else tangent_t<double>(c, U, (const double*)v, (double*)Jv);
 }
 
 void launch_residual_timed(cfd_ctx* c, const double* U, double* R, bool upd, hipEvent_t* ev) {
@@ -781,6 +1271,13 @@ void launch_jv_fd(cfd_ctx* c, const double* U, const double* Rb, const double* v
                                             c->pitch, tmpR, Rb, v, c->d_vol, c->dt, c->scal, su, sv, Jv));
 }
 
+void launch_d2f(cfd_ctx* c, const double* a, float* b, size_t n) {
+  if (n) hipLaunchKernelGGL((k_convert<double, float>), dim3(nblocks(n)), dim3(BS), 0, c->stream, a, b, n);
+}
+void launch_f2d(cfd_ctx* c, const float* a, double* b, size_t n) {
+  if (n) hipLaunchKernelGGL((k_convert<float, double>), dim3(nblocks(n)), dim3(BS), 0, c->stream, a, b, n);
+}
+
 void launch_axpby(cfd_ctx* c, double* y, double a, const double* x, double b, size_t n) {
   if (n == 0) return;
   hipLaunchKernelGGL(k_axpby, dim3(nblocks(n)), dim3(BS), 0, c->stream, y, a, x, b, n);
@@ -825,9 +1322,9 @@ template <class TV, class TZ> static void precond_t(cfd_ctx* c, TZ* z, const TV*
                                             c->N, c->pitch, z, Vj, c->kry.D));
   if (slot_norm >= 0) reduce_t<TZ, TZ>(c, RED_L2, z, z, slot_norm);
 }
-void launch_precond_apply(cfd_ctx* c, double* z, const void* Vj, int prec, int slot_norm) {
+void launch_precond_apply(cfd_ctx* c, void* z, const void* Vj, int prec, int slot_norm) {
   if (prec == 32) precond_t<float, float>(c, (float*)z, (const float*)Vj, slot_norm);
-  else precond_t<double, double>(c, z, (const double*)Vj, slot_norm);
+  else precond_t<double, double>(c, (double*)z, (const double*)Vj, slot_norm);
 }
 
 // Classical Gram-Schmidt with one reorthogonalisation pass (CGS2), FP64
@@ -866,6 +1363,32 @@ void launch_store_basis(cfd_ctx* c, void* Vj1, const double* w, const double* hd
   (void)w;
   if (prec == 32) store_t<float>(c, (float*)Vj1, hdev + j + 1);
   else store_t<double>(c, (double*)Vj1, hdev + j + 1);
+}
+
+template <class TV> static void block_t(cfd_ctx* c, const double* U, TV* Binv, double* Bout) {
+  NV_DISPATCH(c->P.nvar, hipLaunchKernelGGL((k_block<NVt, TV>), dim3(nblocks(c->N)), dim3(BS), 0, c->stream, c->dm,
+                                            c->phys, U, c->dt, Binv, Bout));
+}
+void launch_block_precond(cfd_ctx* c, const double* U, void* Binv, int prec, double* Bout) {
+  if (prec == 32) block_t<float>(c, U, (float*)Binv, Bout);
+  else block_t<double>(c, U, (double*)Binv, Bout);
+}
+template <class TV> static void bapply_t(cfd_ctx* c, const TV* Binv, const TV* x, TV* z) {
+  NV_DISPATCH(c->P.nvar, hipLaunchKernelGGL((k_block_apply<NVt, TV, TV>), dim3(nblocks(c->N)), dim3(BS), 0,
+                                            c->stream, c->N, c->pitch, Binv, x, z));
+}
+void launch_block_apply(cfd_ctx* c, const void* Binv, const void* x, void* z, int prec) {
+  if (prec == 32) bapply_t<float>(c, (const float*)Binv, (const float*)x, (float*)z);
+  else bapply_t<double>(c, (const double*)Binv, (const double*)x, (double*)z);
+}
+template <class TV> static void solbj_t(cfd_ctx* c, double* x, const TV* V, const double* y, int k, const TV* B) {
+  const size_t vs = (size_t)c->P.nvar * c->pitch;
+  NV_DISPATCH(c->P.nvar, hipLaunchKernelGGL((k_solution_bj<NVt, TV>), dim3(nblocks(c->N)), dim3(BS), 0, c->stream,
+                                            c->N, c->pitch, vs, V, y, k + 1, B, x));
+}
+void launch_solution_bj(cfd_ctx* c, double* x, const void* V, const double* y, int k, const void* Binv, int prec) {
+  if (prec == 32) solbj_t<float>(c, x, (const float*)V, y, k, (const float*)Binv);
+  else solbj_t<double>(c, x, (const double*)V, y, k, (const double*)Binv);
 }
 
 template <class TV> static void solution_t(cfd_ctx* c, double* x, const TV* V, const double* y, int k) {

@@ -52,6 +52,96 @@ CFD_ARITH HD Dual<F> operator/(A a, Dual<F> b) {
   return Dual<F>(q, -q * b.d / b.v);
 }
 
+// K-direction forward dual: value plus K directional derivatives, used for
+// face-local Jacobian-transpose products (one pass covers K inputs).
+template <class F, int K> struct DualV {
+  F v;
+  F d[K];
+  HD DualV() : v(0) {
+#pragma unroll
+    for (int i = 0; i < K; ++i) d[i] = 0;
+  }
+  HD DualV(F a) : v(a) {
+#pragma unroll
+    for (int i = 0; i < K; ++i) d[i] = 0;
+  }
+};
+#define DV_BIN(op, dexpr)                                                      \
+  template <class F, int K> HD DualV<F, K> operator op(const DualV<F, K>& a, const DualV<F, K>& b) { \
+    DualV<F, K> r;                                                             \
+    r.v = a.v op b.v;                                                          \
+    _Pragma("unroll") for (int i = 0; i < K; ++i) r.d[i] = dexpr;              \
+    return r;                                                                  \
+  }
+DV_BIN(+, a.d[i] + b.d[i])
+DV_BIN(-, a.d[i] - b.d[i])
+DV_BIN(*, a.d[i] * b.v + a.v * b.d[i])
+#undef DV_BIN
+template <class F, int K> HD DualV<F, K> operator/(const DualV<F, K>& a, const DualV<F, K>& b) {
+  DualV<F, K> r;
+  r.v = a.v / b.v;
+#pragma unroll
+  for (int i = 0; i < K; ++i) r.d[i] = (a.d[i] - r.v * b.d[i]) / b.v;
+  return r;
+}
+template <class F, int K> HD DualV<F, K> operator-(const DualV<F, K>& a) {
+  DualV<F, K> r;
+  r.v = -a.v;
+#pragma unroll
+  for (int i = 0; i < K; ++i) r.d[i] = -a.d[i];
+  return r;
+}
+#define CFD_ARITHV template <class F, int K, class A, class = typename std::enable_if<std::is_arithmetic<A>::value>::type>
+CFD_ARITHV HD DualV<F, K> operator+(const DualV<F, K>& a, A b) { DualV<F, K> r = a; r.v = a.v + F(b); return r; }
+CFD_ARITHV HD DualV<F, K> operator+(A b, const DualV<F, K>& a) { DualV<F, K> r = a; r.v = F(b) + a.v; return r; }
+CFD_ARITHV HD DualV<F, K> operator-(const DualV<F, K>& a, A b) { DualV<F, K> r = a; r.v = a.v - F(b); return r; }
+CFD_ARITHV HD DualV<F, K> operator-(A b, const DualV<F, K>& a) {
+  DualV<F, K> r;
+  r.v = F(b) - a.v;
+#pragma unroll
+  for (int i = 0; i < K; ++i) r.d[i] = -a.d[i];
+  return r;
+}
+CFD_ARITHV HD DualV<F, K> operator*(const DualV<F, K>& a, A b) {
+  DualV<F, K> r;
+  r.v = a.v * F(b);
+#pragma unroll
+  for (int i = 0; i < K; ++i) r.d[i] = a.d[i] * F(b);
+  return r;
+}
+CFD_ARITHV HD DualV<F, K> operator*(A b, const DualV<F, K>& a) { return a * b; }
+CFD_ARITHV HD DualV<F, K> operator/(const DualV<F, K>& a, A b) {
+  DualV<F, K> r;
+  r.v = a.v / F(b);
+#pragma unroll
+  for (int i = 0; i < K; ++i) r.d[i] = a.d[i] / F(b);
+  return r;
+}
+CFD_ARITHV HD DualV<F, K> operator/(A b, const DualV<F, K>& a) {
+  DualV<F, K> r;
+  r.v = F(b) / a.v;
+#pragma unroll
+  for (int i = 0; i < K; ++i) r.d[i] = -r.v * a.d[i] / a.v;
+  return r;
+}
+template <class F, int K> HD F val(const DualV<F, K>& x) { return x.v; }
+template <class F, int K> HD DualV<F, K> sq_rt(const DualV<F, K>& x) {
+  DualV<F, K> r;
+  r.v = sqrt(x.v);
+#pragma unroll
+  for (int i = 0; i < K; ++i) r.d[i] = x.d[i] / (F(2) * r.v);
+  return r;
+}
+template <class F, int K, class A> HD DualV<F, K> pw(const DualV<F, K>& a, A bb) {
+  const F b = F(bb);
+  DualV<F, K> r;
+  r.v = pow(a.v, b);
+  const F dp = b * pow(a.v, b - F(1));
+#pragma unroll
+  for (int i = 0; i < K; ++i) r.d[i] = dp * a.d[i];
+  return r;
+}
+
 template <class T> struct Real { typedef T type; };
 template <class F> struct Real<Dual<F>> { typedef F type; };
 
