@@ -68,6 +68,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--assembly", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--config", type=int, default=CONFIG, help="benchmark configuration to run (1-5)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (got --gpus {args.gpus}); the multi-GPU layout is design-only",
@@ -77,7 +78,7 @@ def main():
     import torch
     from mycode_amd import cfd, configs
 
-    mesh, p, U0 = configs.setup(CONFIG)
+    mesh, p, U0 = configs.setup(args.config)
     rng = np.random.default_rng(12345)
     U = U0.copy()
     U[:, :4] *= 1.0 + 1e-3 * rng.standard_normal((U.shape[0], 4))

@@ -81,7 +81,9 @@ typedef struct cfd_params {
   int pad0;
   /* post-shock state for the SWBLI tag-6 boundary (primitive) */
   double shock_rho, shock_u, shock_v, shock_p;
-  double reserved[8];
+  /* SST k-omega: freestream specific k and omega of the frozen inflow */
+  double turb_k_inf, turb_w_inf;
+  double reserved[6];
 } cfd_params;
 
 /* Helpers that fill the freestream fields exactly as the reference does. */

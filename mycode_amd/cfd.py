@@ -34,7 +34,8 @@ class Params(C.Structure):
         ("flux", C.c_int), ("spatial", C.c_int), ("bcset", C.c_int), ("viscous", C.c_int),
         ("nvar", C.c_int), ("turb", C.c_int), ("flux_alt", C.c_int), ("pad0", C.c_int),
         ("shock_rho", C.c_double), ("shock_u", C.c_double), ("shock_v", C.c_double),
-        ("shock_p", C.c_double), ("reserved", C.c_double * 8),
+        ("shock_p", C.c_double), ("turb_k_inf", C.c_double), ("turb_w_inf", C.c_double),
+        ("reserved", C.c_double * 6),
     ]
 
     def copy(self) -> "Params":

@@ -10,7 +10,9 @@ case needs — applied to a seeded synthetic mesh (``cfd_mesh_generate``).
   2  laminar flat plate, Mach 0.2, Re_L = 1e5, 400 x 160 (20 % split),
      Roe + viscous, second-order central, plate BC set, JFNK-GMRES(30).
   3  turbulent flat plate, Mach 0.2, Re = 5e6, 1200 x 1000 (10 % split),
-     Roe + viscous, six conserved variables.
+     Roe + viscous + SST k-omega (six conserved variables), freestream
+     k = 9e-9 a^2 and omega = 1e-6 a^2 / nu (mu_t / mu = 0.009, the NASA
+     Turbulence Modeling Resource's flat-plate values).
   4  Mach 0.7 inviscid flow over a 5 % circular-arc bump in a 3 x 1 channel,
      1280 x 560 (15 % split), HLLC central, bump BC set.
   5  Mach 2 laminar shock / boundary-layer interaction, 2000 x 2000 (10 %
@@ -76,7 +78,8 @@ def freestream_state(p: cfd.Params) -> np.ndarray:
     Ts = p.T_total_inlet / fT
     rho = p.P_static_exit / (p.R_gas * Ts)
     u = p.M_inf * math.sqrt(g * p.R_gas * Ts)
-    return np.array([rho, rho * u, 0.0, p.P_static_exit / (g - 1.0) + 0.5 * rho * u * u, 0.0, 0.0])
+    k, w = (p.turb_k_inf, p.turb_w_inf) if p.turb else (0.0, 0.0)
+    return np.array([rho, rho * u, 0.0, p.P_static_exit / (g - 1.0) + 0.5 * rho * u * u, rho * k, rho * w])
 
 
 def params_for(config: int) -> cfd.Params:
@@ -92,6 +95,8 @@ def params_for(config: int) -> cfd.Params:
         cfd.flow_conditions(p, 0.2, 101325.0, 300.0)
         cfd.kinematic_viscosity(p, 5e6, 1.0)
         p.bcset, p.flux, p.spatial, p.viscous, p.nvar = cfd.BC_PLATE, cfd.ROE, cfd.SECOND_ORDER_CENTRAL, 1, 6
+        a2 = p.gamma * p.R_gas * p.T_total_inlet / (1.0 + 0.5 * (p.gamma - 1.0) * p.M_inf ** 2)
+        p.turb, p.turb_k_inf, p.turb_w_inf = 1, 9e-9 * a2, 1e-6 * a2 / p.nu
     elif config == 4:
         cfd.flow_conditions(p, 0.7, 101325.0, 300.0)
         p.bcset, p.flux, p.spatial, p.viscous, p.nvar = cfd.BC_BUMP, cfd.HLLC, cfd.SECOND_ORDER_CENTRAL, 0, 4

@@ -66,12 +66,19 @@ void fill_phys(cfd_ctx* c) {
   q.sh[3] = p.shock_p / (g - 1.0) + 0.5 * p.shock_rho * (p.shock_u * p.shock_u + p.shock_v * p.shock_v);
   q.sh[4] = p.shock_rho * 0.0;
   q.sh[5] = p.shock_rho * 0.0;
+  if (p.turb) {  // frozen freestream carries k_inf, omega_inf
+    q.fs[4] = rho * p.turb_k_inf;
+    q.fs[5] = rho * p.turb_w_inf;
+  }
+  q.kmin = 1e-8 * p.turb_k_inf;
+  q.wmin = 1e-3 * p.turb_w_inf;
   q.flux = p.flux;
   q.flux_alt = p.flux_alt;
   q.spatial = p.spatial;
   q.bcset = p.bcset;
   q.viscous = p.viscous;
   q.nvar = p.nvar;
+  q.turb = p.turb;
 }
 
 int check_params(const cfd_params* p) {
@@ -79,6 +86,7 @@ int check_params(const cfd_params* p) {
   if (p->nvar != 4 && p->nvar != 6) return CFD_ERR_ARG;
   if (p->flux < 0 || p->flux > 3 || p->flux_alt < 0 || p->flux_alt > 3) return CFD_ERR_ARG;
   if (p->spatial < 0 || p->spatial > 2 || p->bcset < 0 || p->bcset > 3) return CFD_ERR_ARG;
+  if (p->turb && (p->nvar != 6 || !p->viscous || !(p->turb_k_inf > 0) || !(p->turb_w_inf > 0))) return CFD_ERR_ARG;
   return CFD_OK;
 }
 
@@ -265,7 +273,8 @@ int cfd_ctx_create(const cfd_mesh* m, const cfd_params* p, cfd_ctx** out) {
   std::vector<int> fpos(F);  // reference face -> device face
   for (int d = 0; d < F; ++d) fpos[m->face_order[d]] = d;
   std::vector<int> own(F), nei(F), tag(F);
-  std::vector<double> nx(F), ny(F), area(F), xm(F), ym(F);
+  std::vector<double> nx(F), ny(F), area(F), xm(F), ym(F), dnw(F);
+  std::vector<double> wxy;  // no-slip face midpoints (Solver.cpp:1986-1991)
   for (int d = 0; d < F; ++d) {
     const int f = m->face_order[d];
     own[d] = m->perm[m->own[f]];
@@ -276,7 +285,14 @@ int cfd_ctx_create(const cfd_mesh* m, const cfd_params* p, cfd_ctx** out) {
     area[d] = m->area[f];
     xm[d] = m->xm[f];
     ym[d] = m->ym[f];
+    const int o = m->own[f];
+    dnw[d] = std::abs((m->xm[f] - m->xc[o]) * m->nx[f] + (m->ym[f] - m->yc[o]) * m->ny[f]);
   }
+  for (int f = 0; f < F; ++f)
+    if (m->nei[f] < 0 && m->tag[f] == 4) {
+      wxy.push_back(m->xm[f]);
+      wxy.push_back(m->ym[f]);
+    }
   // per-cell face slots in ascending reference face index
   std::vector<int4> slots(P_, make_int4(-1, -1, -1, -1));
   std::vector<int> cnt(N, 0);
@@ -306,6 +322,8 @@ int cfd_ctx_create(const cfd_mesh* m, const cfd_params* p, cfd_ctx** out) {
   HIP_TRY(c, dalloc(&c->d_xm, F));
   HIP_TRY(c, dalloc(&c->d_ym, F));
   HIP_TRY(c, dalloc(&c->d_iperm, N));
+  HIP_TRY(c, dalloc(&c->d_wd, P_));
+  HIP_TRY(c, dalloc(&c->d_dnw, F));
   HIP_TRY(c, dalloc(&c->d_facemap, F));
   HIP_TRY(c, hipMemcpy(c->d_vol, vol.data(), P_ * 8, hipMemcpyHostToDevice));
   HIP_TRY(c, hipMemcpy(c->d_xc, xc.data(), P_ * 8, hipMemcpyHostToDevice));
@@ -320,6 +338,7 @@ int cfd_ctx_create(const cfd_mesh* m, const cfd_params* p, cfd_ctx** out) {
   HIP_TRY(c, hipMemcpy(c->d_xm, xm.data(), F * 8, hipMemcpyHostToDevice));
   HIP_TRY(c, hipMemcpy(c->d_ym, ym.data(), F * 8, hipMemcpyHostToDevice));
   HIP_TRY(c, hipMemcpy(c->d_iperm, m->iperm.data(), N * 4, hipMemcpyHostToDevice));
+  HIP_TRY(c, hipMemcpy(c->d_dnw, dnw.data(), F * 8, hipMemcpyHostToDevice));
   HIP_TRY(c, hipMemcpy(c->d_facemap, m->face_order.data(), F * 4, hipMemcpyHostToDevice));
   DMesh& d = c->dm;
   d.N = N;
@@ -338,15 +357,25 @@ int cfd_ctx_create(const cfd_mesh* m, const cfd_params* p, cfd_ctx** out) {
   d.xm = c->d_xm;
   d.ym = c->d_ym;
   d.alt = nullptr;
+  d.wd = c->d_wd;
+  d.dnw = c->d_dnw;
+  {  // wall distances on the device (brute force, as the reference)
+    double* dw = nullptr;
+    HIP_TRY(c, dalloc(&dw, std::max<size_t>(wxy.size(), 2)));
+    if (!wxy.empty()) HIP_TRY(c, hipMemcpy(dw, wxy.data(), wxy.size() * 8, hipMemcpyHostToDevice));
+    launch_wall_distance(c, dw, (int)wxy.size() / 2, c->d_wd);
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    HIP_TRY(c, hipFree(dw));
+  }
   // solver buffers
-  HIP_TRY(c, dalloc(&c->grad, (size_t)10 * P_));
+  HIP_TRY(c, dalloc(&c->grad, (size_t)14 * P_));
   HIP_TRY(c, dalloc(&c->phi, P_));
   HIP_TRY(c, dalloc(&c->dt, P_));
   HIP_TRY(c, dalloc(&c->Fm, (size_t)6 * F));
   HIP_TRY(c, dalloc(&c->red, 65536));
   HIP_TRY(c, dalloc(&c->scal, 64));
-  HIP_TRY(c, dalloc(&c->gradT, (size_t)20 * P_));
-  HIP_TRY(c, dalloc(&c->adj, (size_t)10 * P_));
+  HIP_TRY(c, dalloc(&c->gradT, (size_t)28 * P_));
+  HIP_TRY(c, dalloc(&c->adj, (size_t)14 * P_));
   HIP_TRY(c, hipHostMalloc((void**)&c->scal_host, 64 * 8, hipHostMallocDefault));
   HIP_TRY(c, dalloc(&c->io6, (size_t)6 * N));
   for (int i = 0; i < 6; ++i) {
@@ -363,7 +392,7 @@ int cfd_ctx_destroy(cfd_ctx* c) {
   (void)hipStreamSynchronize(c->stream);
   free_krylov(c);
   void* bufs[] = {c->d_vol, c->d_xc,  c->d_yc,  c->d_slots, c->d_own, c->d_nei, c->d_tag,  c->d_nx,
-                  c->d_ny,  c->d_area, c->d_xm, c->d_ym,    c->d_iperm, c->d_facemap, c->d_alt, c->grad,
+                  c->d_ny,  c->d_area, c->d_xm, c->d_ym,    c->d_iperm, c->d_facemap, c->d_alt, c->grad, c->d_wd, c->d_dnw,
                   c->phi,   c->dt,    c->Fm,    c->red,     c->scal, c->gradT, c->adj, c->io6};
   for (void* b : bufs) (void)hipFree(b);
   for (int i = 0; i < 6; ++i) (void)hipFree(c->vec[i]);
@@ -378,6 +407,7 @@ const char* cfd_last_error(const cfd_ctx* c) { return c ? c->err.c_str() : "null
 int cfd_ctx_set_params(cfd_ctx* c, const cfd_params* p) {
   if (!c || check_params(p)) return CFD_ERR_ARG;
   if (p->nvar != c->P.nvar) return fail(c, CFD_ERR_ARG, "nvar cannot change on a live context");
+  if (p->turb != c->P.turb) c->phi_valid = false;
   c->P = *p;
   fill_phys(c);
   return CFD_OK;

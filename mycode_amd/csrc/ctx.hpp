@@ -22,6 +22,8 @@ struct DMesh {
   const int *own, *nei, *tag;
   const double *nx, *ny, *area, *xm, *ym;
   const unsigned char* alt;  // per device face: 1 = use Phys::flux_alt (may be null)
+  const double* wd;          // per cell: distance to the nearest no-slip face midpoint
+  const double* dnw;         // per face: normal distance of the owner centroid to the face
 };
 
 // Workspace of one GMRES solve.
@@ -54,17 +56,18 @@ struct cfd_ctx {
   int4* d_slots = nullptr;
   int *d_own = nullptr, *d_nei = nullptr, *d_tag = nullptr, *d_iperm = nullptr, *d_facemap = nullptr;
   double *d_nx = nullptr, *d_ny = nullptr, *d_area = nullptr, *d_xm = nullptr, *d_ym = nullptr;
+  double *d_wd = nullptr, *d_dnw = nullptr;
   unsigned char* d_alt = nullptr;
   // solver state
-  double* grad = nullptr;   // 10 x pitch
+  double* grad = nullptr;   // 14 x pitch (rho u v p T gradients; k, omega with SST)
   double* phi = nullptr;    // pitch
   double* dt = nullptr;     // pitch
   double* Fm = nullptr;     // nv x F face buffer (assembly 1)
   double* red = nullptr;    // reduction partials
   double* scal = nullptr;   // device scalars (norms, eps, ...)
   double* scal_host = nullptr;  // pinned
-  double* gradT = nullptr;  // tangent gradients (10 x pitch, FP64 or FP32 dual parts)
-  double* adj = nullptr;    // adjoint of the gradients (10 x pitch)
+  double* gradT = nullptr;  // tangent gradients (2 x 14 x pitch: values, derivatives)
+  double* adj = nullptr;    // adjoint of the gradients (14 x pitch)
   bool phi_valid = false, dt_valid = false;
   // host-pointer staging: AoS device copy + SoA vectors
   double* io6 = nullptr;     // 6 x N (AoS)
@@ -103,6 +106,8 @@ void launch_newton_update(cfd_ctx* c, double* U, const double* dU, double relax)
 void launch_explicit(cfd_ctx* c, double* Unew, const double* U0, double a0, const double* U1, double a1,
                      const double* R, double dtc);
 void launch_wall(cfd_ctx* c, const double* U, double* out3);
+// wall distance of every cell to the nw no-slip face midpoints (xy interleaved)
+void launch_wall_distance(cfd_ctx* c, const double* wxy, int nw, double* wd);
 // Krylov (precision-generic basis)
 void launch_krylov_init(cfd_ctx* c, void* V0, const double* rhs, double sign, int slot, int prec);
 void launch_precond_apply(cfd_ctx* c, void* z, const void* Vj, int prec, int slot_norm);

@@ -83,15 +83,18 @@ __global__ __launch_bounds__(BS) void k_gradlim(DMesh m, Phys P, const double* _
                                                  typename Scal<T>::F* __restrict__ Gd, double* __restrict__ phi,
                                                  int do_lim) {
   typedef Scal<T> Sc;
+  constexpr int NQM = NV == 6 ? 7 : 5;  // gradient variables: rho u v p T (+ k omega with SST)
   const int c = xcd_block() * BS + threadIdx.x;
   if (c >= m.N) return;
   const int p = m.pitch;
   const double Rg = P.R;
+  const bool turb = NV == 6 && P.turb;
+  const int nq = turb ? 7 : 5;
   const S6<T> Uc = ldT<NV, T>(U, v, p, c);
   const T uc = Uc[1] / Uc[0], vc = Uc[2] / Uc[0], pc = pressure(P, Uc), Tc = pc / (Uc[0] * Rg);
-  T g[10];
+  T g[2 * NQM];
 #pragma unroll
-  for (int q = 0; q < 10; ++q) g[q] = T(0.0);
+  for (int q = 0; q < 2 * NQM; ++q) g[q] = T(0.0);
   double pmax = val(pc), pmin = val(pc);
   const int4 s4 = m.slots[c];
   const int sl[4] = {s4.x, s4.y, s4.z, s4.w};
@@ -103,18 +106,32 @@ __global__ __launch_bounds__(BS) void k_gradlim(DMesh m, Phys P, const double* _
     const bool as_nei = e & 1;
     const int other = as_nei ? m.own[f] : m.nei[f];
     const double nx = m.nx[f], ny = m.ny[f], A = m.area[f];
-    const S6<T> Uo = other >= 0 ? ldT<NV, T>(U, v, p, other) : ghost(P, Uc, m.tag[f], nx, ny);
+    const S6<T> Uo = other >= 0 ? ldT<NV, T>(U, v, p, other)
+                                : ghost(P, Uc, m.tag[f], nx, ny, turb ? m.dnw[f] : 0.0);
     const T uo = Uo[1] / Uo[0], vo = Uo[2] / Uo[0], po = pressure(P, Uo), To = po / (Uo[0] * Rg);
-    const T fv[5] = {0.5 * (Uc[0] + Uo[0]), 0.5 * (uc + uo), 0.5 * (vc + vo), 0.5 * (pc + po), 0.5 * (Tc + To)};
+    T fv[NQM];
+    fv[0] = 0.5 * (Uc[0] + Uo[0]);
+    fv[1] = 0.5 * (uc + uo);
+    fv[2] = 0.5 * (vc + vo);
+    fv[3] = 0.5 * (pc + po);
+    fv[4] = 0.5 * (Tc + To);
+    if constexpr (NQM == 7) {
+      if (turb) {
+        fv[5] = 0.5 * (Uc[4] / Uc[0] + Uo[4] / Uo[0]);
+        fv[6] = 0.5 * (Uc[5] / Uc[0] + Uo[5] / Uo[0]);
+      } else {
+        fv[5] = fv[6] = T(0.0);
+      }
+    }
     if (!as_nei) {
 #pragma unroll
-      for (int q = 0; q < 5; ++q) {
+      for (int q = 0; q < NQM; ++q) {
         g[2 * q] = g[2 * q] + fv[q] * nx * A;
         g[2 * q + 1] = g[2 * q + 1] + fv[q] * ny * A;
       }
     } else {
 #pragma unroll
-      for (int q = 0; q < 5; ++q) {
+      for (int q = 0; q < NQM; ++q) {
         g[2 * q] = g[2 * q] - fv[q] * nx * A;
         g[2 * q + 1] = g[2 * q + 1] - fv[q] * ny * A;
       }
@@ -124,7 +141,8 @@ __global__ __launch_bounds__(BS) void k_gradlim(DMesh m, Phys P, const double* _
   }
   const double vol = m.vol[c];
 #pragma unroll
-  for (int q = 0; q < 10; ++q) {
+  for (int q = 0; q < 2 * NQM; ++q) {
+    if (q >= 2 * nq) break;
     g[q] = g[q] / vol;
     Sc::put(Gv, Gd, (size_t)q * p + c, g[q]);
   }
@@ -150,10 +168,32 @@ __global__ __launch_bounds__(BS) void k_gradlim(DMesh m, Phys P, const double* _
   }
 }
 
+// SST sources: R_c(k, omega) -= V_c S(U_c, G_c, d_c)  (the tangent subtracts dS)
+template <int NV, class T>
+__global__ __launch_bounds__(BS) void k_turb_src(DMesh m, Phys P, const double* __restrict__ U,
+                                                  const typename Scal<T>::F* __restrict__ v,
+                                                  const typename Scal<T>::F* __restrict__ Gv,
+                                                  const typename Scal<T>::F* __restrict__ Gd,
+                                                  typename Scal<T>::F* __restrict__ out) {
+  typedef Scal<T> Sc;
+  const int c = xcd_block() * BS + threadIdx.x;
+  if (c >= m.N) return;
+  const int p = m.pitch;
+  const S6<T> Uc = ldT<NV, T>(U, v, p, c);
+  T g[14];
+#pragma unroll
+  for (int q = 0; q < 14; ++q) g[q] = Sc::get(Gv, Gd, (size_t)q * p + c);
+  const TurbQ<T> tq = turb_cell(P, Uc, g, m.wd[c]);
+  const double V = m.vol[c];
+  out[4 * p + c] = out[4 * p + c] - Sc::dpart(V * tq.Sk);
+  out[5 * p + c] = out[5 * p + c] - Sc::dpart(V * tq.Sw);
+}
+
 // ---- face flux ------------------------------------------------------------------------
 // Geometry and frozen quantities of one face, as the flux core sees them.
 struct FaceGeo {
   double nx, ny, A, xm, ym, xco, yco, xcn, ycn, phL, phR;
+  double wdo, wdn, dnw;  // SST: wall distances of the two cells, owner normal distance
   int tg, scheme;
   bool boundary;
 };
@@ -169,6 +209,12 @@ __device__ __forceinline__ FaceGeo face_geo(const DMesh& m, const Phys& P, const
   g.scheme = (m.alt && m.alt[f]) ? P.flux_alt : P.flux;
   g.xm = g.ym = g.xco = g.yco = g.xcn = g.ycn = 0.0;
   g.phL = g.phR = 0.5;
+  g.wdo = g.wdn = g.dnw = 1.0;
+  if (P.turb) {
+    g.wdo = m.wd[o];
+    g.wdn = n >= 0 ? m.wd[n] : g.wdo;
+    g.dnw = m.dnw[f];
+  }
   if (need_cells) {
     g.xm = m.xm[f];
     g.ym = m.ym[f];
@@ -214,6 +260,19 @@ __device__ __forceinline__ S6<T> flux_core(const Phys& P, const FaceGeo& g, cons
     if (val(rRf) < 1e-4) rRf = R[0];
     if (val(pRf) < 1e-4) pRf = pR;
     Rr = prim2cons(P, rRf, uRf, vRf, pRf);
+    if (P.turb) {  // MUSCL on specific k and omega, clipped at zero
+      const T kL = L[4] / L[0], wL = L[5] / L[0], kR = R[4] / R[0], wR = R[5] / R[0];
+      T kLf = kL + phL * (gL[10] * dxL + gL[11] * dyL), wLf = wL + phL * (gL[12] * dxL + gL[13] * dyL);
+      T kRf = kR + phR * (gR[10] * dxR + gR[11] * dyR), wRf = wR + phR * (gR[12] * dxR + gR[13] * dyR);
+      if (val(kLf) < 0.0) kLf = kL;
+      if (val(wLf) < 0.0) wLf = wL;
+      if (val(kRf) < 0.0) kRf = kR;
+      if (val(wRf) < 0.0) wRf = wR;
+      Lr[4] = rLf * kLf;
+      Lr[5] = rLf * wLf;
+      Rr[4] = rRf * kRf;
+      Rr[5] = rRf * wRf;
+    }
   }
   S6<T> Fl;
   if (g.boundary && (g.tg == 3 || g.tg == 4)) {
@@ -224,6 +283,12 @@ __device__ __forceinline__ S6<T> flux_core(const Phys& P, const FaceGeo& g, cons
     Fl[3] = T(0.0);
   } else {
     Fl = riemann(P, Lr, Rr, nx, ny, g.scheme);
+  }
+  if (P.turb) {  // k and omega carried by the mass flux, upwinded
+    const T mf = Fl[0];
+    const T mp = val(mf) > 0.0 ? mf : T(0.0), mm = val(mf) > 0.0 ? T(0.0) : mf;
+    Fl[4] = mp * (Lr[4] / Lr[0]) + mm * (Rr[4] / Rr[0]);
+    Fl[5] = mp * (Lr[5] / Lr[0]) + mm * (Rr[5] / Rr[0]);
   }
   if (VISC) {
     const T rf = 0.5 * (L[0] + R[0]);
@@ -261,8 +326,22 @@ __device__ __forceinline__ S6<T> flux_core(const Phys& P, const FaceGeo& g, cons
         }
       }
     }
-    const T mu = rf * P.nu;
-    const T kc = mu * P.Cp / P.Pr;
+    T mu = rf * P.nu;
+    T kc = mu * P.Cp / P.Pr;
+    T mut = T(0.0), F1f = T(0.0);
+    if (P.turb) {  // eddy viscosity in stress and heat flux
+      const TurbQ<T> qo = turb_cell(P, L, gL, g.wdo);
+      if (!g.boundary) {
+        const TurbQ<T> qn = turb_cell(P, R, gR, g.wdn);
+        mut = 0.5 * (qo.mut + qn.mut);
+        F1f = 0.5 * (qo.F1 + qn.F1);
+      } else {
+        mut = g.tg == 4 ? T(0.0) : qo.mut;
+        F1f = qo.F1;
+      }
+      kc = P.Cp * (mu / P.Pr + mut / sst::Prt);
+      mu = mu + mut;
+    }
     const T div = ux + vy;
     const T txx = mu * (2.0 * ux - (2.0 / 3.0) * div);
     const T tyy = mu * (2.0 * vy - (2.0 / 3.0) * div);
@@ -274,8 +353,43 @@ __device__ __forceinline__ S6<T> flux_core(const Phys& P, const FaceGeo& g, cons
     Fl[1] = Fl[1] - (txx * nx + txy * ny);
     Fl[2] = Fl[2] - (txy * nx + tyy * ny);
     Fl[3] = Fl[3] - (wx * nx + wy * ny);
-    Fl[4] = Fl[4] - 0.0;
-    Fl[5] = Fl[5] - 0.0;
+    if (P.turb) {  // (mu + sigma mu_t) grad(k, omega) . n
+      T kx, ky, wx_, wy_;
+      const T kcl = L[4] / L[0], wcl = L[5] / L[0];
+      if (g.boundary && g.tg == 4) {
+        const double dx = g.xm - g.xco, dy = g.ym - g.yco;
+        const double d2 = dx * dx + dy * dy;
+        const double ww = omega_wall(P, g.dnw);
+        kx = -kcl * dx / d2;
+        ky = -kcl * dy / d2;
+        wx_ = (ww - wcl) * dx / d2;
+        wy_ = (ww - wcl) * dy / d2;
+      } else {
+        kx = 0.5 * (gL[10] + gR[10]);
+        ky = 0.5 * (gL[11] + gR[11]);
+        wx_ = 0.5 * (gL[12] + gR[12]);
+        wy_ = 0.5 * (gL[13] + gR[13]);
+        if (!g.boundary) {
+          const double dxLR = g.xcn - g.xco, dyLR = g.ycn - g.yco;
+          const double d2 = dxLR * dxLR + dyLR * dyLR;
+          if (d2 > 1e-14) {
+            const T dk = (R[4] / R[0]) - kcl, dw = (R[5] / R[0]) - wcl;
+            const T kx0 = kx, ky0 = ky, wx0 = wx_, wy0 = wy_;
+            kx = kx0 - (kx0 * dxLR + ky0 * dyLR - dk) * dxLR / d2;
+            ky = ky0 - (kx0 * dxLR + ky0 * dyLR - dk) * dyLR / d2;
+            wx_ = wx0 - (wx0 * dxLR + wy0 * dyLR - dw) * dxLR / d2;
+            wy_ = wy0 - (wx0 * dxLR + wy0 * dyLR - dw) * dyLR / d2;
+          }
+        }
+      }
+      const T mul = rf * P.nu;
+      const T sgk = F1f * sst::sk1 + (1.0 - F1f) * sst::sk2, sgw = F1f * sst::sw1 + (1.0 - F1f) * sst::sw2;
+      Fl[4] = Fl[4] - (mul + sgk * mut) * (kx * nx + ky * ny);
+      Fl[5] = Fl[5] - (mul + sgw * mut) * (wx_ * nx + wy_ * ny);
+    } else {
+      Fl[4] = Fl[4] - 0.0;
+      Fl[5] = Fl[5] - 0.0;
+    }
   }
   return scale(Fl, T(g.A));
 }
@@ -292,18 +406,20 @@ __device__ __forceinline__ S6<T> face_flux(const DMesh& m, const Phys& P, const 
   n = m.nei[f];
   const bool second = P.spatial != CFD_FIRST_ORDER && n >= 0;
   const FaceGeo g = face_geo(m, P, phi, f, o, n, second || VISC);
+  constexpr int NG = NV == 6 ? 14 : 10;
+  const int ng = (NV == 6 && P.turb) ? 14 : 10;
   const S6<T> L = ldT<NV, T>(U, v, p, o);
-  const S6<T> R = n >= 0 ? ldT<NV, T>(U, v, p, n) : ghost(P, L, g.tg, g.nx, g.ny);
-  T gL[10], gR[10];
+  const S6<T> R = n >= 0 ? ldT<NV, T>(U, v, p, n) : ghost(P, L, g.tg, g.nx, g.ny, g.dnw);
+  T gL[NG], gR[NG];
   if (second || VISC) {
 #pragma unroll
-    for (int q = 0; q < 10; ++q) gL[q] = Sc::get(Gv, Gd, (size_t)q * p + o);
+    for (int q = 0; q < NG; ++q) gL[q] = q < ng ? Sc::get(Gv, Gd, (size_t)q * p + o) : T(0.0);
     if (n >= 0) {
 #pragma unroll
-      for (int q = 0; q < 10; ++q) gR[q] = Sc::get(Gv, Gd, (size_t)q * p + n);
+      for (int q = 0; q < NG; ++q) gR[q] = q < ng ? Sc::get(Gv, Gd, (size_t)q * p + n) : T(0.0);
     } else {
 #pragma unroll
-      for (int q = 0; q < 10; ++q) gR[q] = gL[q];
+      for (int q = 0; q < NG; ++q) gR[q] = gL[q];
     }
   }
   return flux_core<VISC, T>(P, g, L, R, gL, gR);
@@ -391,11 +507,13 @@ __global__ __launch_bounds__(BS) void k_vjp_face(DMesh m, Phys P, const double* 
                                                   const double* __restrict__ w, double* __restrict__ aU,
                                                   double* __restrict__ aG, int f0, int f1) {
   typedef DualV<double, VK> T;
+  constexpr int NGM = NV == 6 ? 14 : 10;
   const int f = f0 + xcd_block() * BS + threadIdx.x;
   if (f >= f1) return;
   const int p = m.pitch;
   const int o = m.own[f], n = m.nei[f];
   const bool interior = n >= 0;
+  const int ng = (NV == 6 && P.turb) ? 14 : 10;
   const bool need = (P.spatial != CFD_FIRST_ORDER && interior) || VISC;
   const FaceGeo g = face_geo(m, P, phi, f, o, n, need);
   double Uo[NV], Un[NV], lam[NV];
@@ -406,10 +524,10 @@ __global__ __launch_bounds__(BS) void k_vjp_face(DMesh m, Phys P, const double* 
     lam[k] = w[k * p + o] - (interior ? w[k * p + n] : 0.0);
   }
   const int nU = interior ? 2 * NV : NV;
-  const int nin = nU + (need ? (interior ? 20 : 10) : 0);
+  const int nin = nU + (need ? (interior ? 2 * ng : ng) : 0);
   for (int base = 0; base < nin; base += VK) {
     S6<T> L, R;
-    T gL[10], gR[10];
+    T gL[NGM], gR[NGM];
 #pragma unroll
     for (int j = 0; j < NV; ++j) {
       L[j] = T(Uo[j]);
@@ -424,25 +542,25 @@ __global__ __launch_bounds__(BS) void k_vjp_face(DMesh m, Phys P, const double* 
         for (int d = 0; d < VK; ++d) R[j].d[d] = (NV + j == base + d) ? 1.0 : 0.0;
       }
     } else {
-      R = ghost(P, L, g.tg, g.nx, g.ny);
+      R = ghost(P, L, g.tg, g.nx, g.ny, g.dnw);
     }
     if (need) {
 #pragma unroll
-      for (int q = 0; q < 10; ++q) {
-        gL[q] = T(G[(size_t)q * p + o]);
+      for (int q = 0; q < NGM; ++q) {
+        gL[q] = T(q < ng ? G[(size_t)q * p + o] : 0.0);
 #pragma unroll
-        for (int d = 0; d < VK; ++d) gL[q].d[d] = (nU + q == base + d) ? 1.0 : 0.0;
+        for (int d = 0; d < VK; ++d) gL[q].d[d] = (q < ng && nU + q == base + d) ? 1.0 : 0.0;
       }
       if (interior) {
 #pragma unroll
-        for (int q = 0; q < 10; ++q) {
-          gR[q] = T(G[(size_t)q * p + n]);
+        for (int q = 0; q < NGM; ++q) {
+          gR[q] = T(q < ng ? G[(size_t)q * p + n] : 0.0);
 #pragma unroll
-          for (int d = 0; d < VK; ++d) gR[q].d[d] = (nU + 10 + q == base + d) ? 1.0 : 0.0;
+          for (int d = 0; d < VK; ++d) gR[q].d[d] = (q < ng && nU + ng + q == base + d) ? 1.0 : 0.0;
         }
       } else {
 #pragma unroll
-        for (int q = 0; q < 10; ++q) gR[q] = gL[q];
+        for (int q = 0; q < NGM; ++q) gR[q] = gL[q];
       }
     }
     const S6<T> F = flux_core<VISC, T>(P, g, L, R, gL, gR);
@@ -456,19 +574,61 @@ __global__ __launch_bounds__(BS) void k_vjp_face(DMesh m, Phys P, const double* 
       double* dst;
       if (idx < NV) dst = aU + (size_t)idx * p + o;
       else if (idx < nU) dst = aU + (size_t)(idx - NV) * p + n;
-      else if (idx < nU + 10) dst = aG + (size_t)(idx - nU) * p + o;
-      else dst = aG + (size_t)(idx - nU - 10) * p + n;
+      else if (idx < nU + ng) dst = aG + (size_t)(idx - nU) * p + o;
+      else dst = aG + (size_t)(idx - nU - ng) * p + n;
       *dst = *dst + s;
     }
   }
 }
 
-template <class T> __device__ __forceinline__ void prims(const Phys& P, const S6<T>& U, T* q) {
+// SST sources: w_c^T d(-V S)/d(U_c, G_c) onto aU_c and aG_c (one thread per cell)
+template <int NV>
+__global__ __launch_bounds__(BS) void k_vjp_src(DMesh m, Phys P, const double* __restrict__ U,
+                                                 const double* __restrict__ G, const double* __restrict__ w,
+                                                 double* __restrict__ aU, double* __restrict__ aG) {
+  typedef DualV<double, VK> T;
+  const int c = xcd_block() * BS + threadIdx.x;
+  if (c >= m.N) return;
+  const int p = m.pitch;
+  const double V = m.vol[c];
+  const double l4 = -V * w[4 * p + c], l5 = -V * w[5 * p + c];
+  for (int base = 0; base < NV + 14; base += VK) {
+    S6<T> Uc;
+    T g[14];
+#pragma unroll
+    for (int j = 0; j < NV; ++j) {
+      Uc[j] = T(U[j * p + c]);
+#pragma unroll
+      for (int d = 0; d < VK; ++d) Uc[j].d[d] = (j == base + d) ? 1.0 : 0.0;
+    }
+#pragma unroll
+    for (int q = 0; q < 14; ++q) {
+      g[q] = T(G[(size_t)q * p + c]);
+#pragma unroll
+      for (int d = 0; d < VK; ++d) g[q].d[d] = (NV + q == base + d) ? 1.0 : 0.0;
+    }
+    const TurbQ<T> tq = turb_cell(P, Uc, g, m.wd[c]);
+#pragma unroll
+    for (int d = 0; d < VK; ++d) {
+      const int idx = base + d;
+      if (idx >= NV + 14) break;
+      const double s = l4 * tq.Sk.d[d] + l5 * tq.Sw.d[d];
+      double* dst = idx < NV ? aU + (size_t)idx * p + c : aG + (size_t)(idx - NV) * p + c;
+      *dst = *dst + s;
+    }
+  }
+}
+
+template <class T> __device__ __forceinline__ void prims(const Phys& P, const S6<T>& U, T* q, bool turb) {
   q[0] = U[0];
   q[1] = U[1] / U[0];
   q[2] = U[2] / U[0];
   q[3] = pressure(P, U);
   q[4] = q[3] / (U[0] * P.R);
+  if (turb) {
+    q[5] = U[4] / U[0];
+    q[6] = U[5] / U[0];
+  }
 }
 
 template <int NV>
@@ -476,15 +636,22 @@ __global__ __launch_bounds__(BS) void k_vjp_gg(DMesh m, Phys P, const double* __
                                                 const double* __restrict__ aG, double* __restrict__ aU, int f0,
                                                 int f1) {
   typedef DualV<double, NV> T;
+  constexpr int NQM = NV == 6 ? 7 : 5;
   const int f = f0 + xcd_block() * BS + threadIdx.x;
   if (f >= f1) return;
   const int p = m.pitch;
   const int o = m.own[f], n = m.nei[f];
+  const bool turb = NV == 6 && P.turb;
+  const int nq = turb ? 7 : 5;
   const double nx = m.nx[f], ny = m.ny[f], A = m.area[f];
   const double vo = m.vol[o];
-  double aq[5];
+  double aq[NQM];
 #pragma unroll
-  for (int k = 0; k < 5; ++k) {
+  for (int k = 0; k < NQM; ++k) {
+    if (k >= nq) {
+      aq[k] = 0.0;
+      continue;
+    }
     aq[k] = A * ((aG[(size_t)(2 * k) * p + o] * nx + aG[(size_t)(2 * k + 1) * p + o] * ny) / vo);
     if (n >= 0)
       aq[k] -= A * ((aG[(size_t)(2 * k) * p + n] * nx + aG[(size_t)(2 * k + 1) * p + n] * ny) / m.vol[n]);
@@ -496,8 +663,8 @@ __global__ __launch_bounds__(BS) void k_vjp_gg(DMesh m, Phys P, const double* __
 #pragma unroll
     for (int d = 0; d < NV; ++d) L[j].d[d] = j == d ? 1.0 : 0.0;
   }
-  T qL[5], qR[5];
-  prims(P, L, qL);
+  T qL[NQM], qR[NQM];
+  prims(P, L, qL, turb);
   if (n >= 0) {
     S6<T> R;
 #pragma unroll
@@ -506,15 +673,16 @@ __global__ __launch_bounds__(BS) void k_vjp_gg(DMesh m, Phys P, const double* __
 #pragma unroll
       for (int d = 0; d < NV; ++d) R[j].d[d] = j == d ? 1.0 : 0.0;
     }
-    prims(P, R, qR);
+    prims(P, R, qR, turb);
   } else {
-    prims(P, ghost(P, L, m.tag[f], nx, ny), qR);
+    prims(P, ghost(P, L, m.tag[f], nx, ny, turb ? m.dnw[f] : 0.0), qR, turb);
   }
 #pragma unroll
   for (int j = 0; j < NV; ++j) {
     double so = 0.0, sn = 0.0;
 #pragma unroll
-    for (int k = 0; k < 5; ++k) {
+    for (int k = 0; k < NQM; ++k) {
+      if (k >= nq) break;
       so += aq[k] * 0.5 * qL[k].d[j];
       sn += aq[k] * 0.5 * qR[k].d[j];
     }
@@ -531,6 +699,32 @@ template <class F>
 __global__ void k_fill(F* __restrict__ x, F v, size_t n) {
   size_t i = (size_t)blockIdx.x * BS + threadIdx.x;
   if (i < n) x[i] = v;
+}
+
+// ---- wall distance (Solver.cpp:1981-2015) -------------------------------------------
+// min over the no-slip face midpoints of the centroid distance; wall points are
+// staged through LDS in tiles shared by the block.
+__global__ __launch_bounds__(BS) void k_walldist(int N, const double* __restrict__ xc, const double* __restrict__ yc,
+                                                  const double* __restrict__ wxy, int nw, double* __restrict__ wd) {
+  __shared__ double sx[BS], sy[BS];
+  const int c = blockIdx.x * BS + threadIdx.x;
+  const double cx = c < N ? xc[c] : 0.0, cy = c < N ? yc[c] : 0.0;
+  double md = 1e20;
+  for (int t0 = 0; t0 < nw; t0 += BS) {
+    __syncthreads();
+    if (t0 + (int)threadIdx.x < nw) {
+      sx[threadIdx.x] = wxy[2 * (t0 + threadIdx.x)];
+      sy[threadIdx.x] = wxy[2 * (t0 + threadIdx.x) + 1];
+    }
+    __syncthreads();
+    const int cnt = min(BS, nw - t0);
+    for (int j = 0; j < cnt; ++j) {
+      const double dx = cx - sx[j], dy = cy - sy[j];
+      const double d2 = dx * dx + dy * dy;
+      if (d2 < md) md = d2;
+    }
+  }
+  if (c < N) wd[c] = nw > 0 ? sqrt(md) : 1e10;
 }
 
 // ---- local time step (Solver.cpp:1388-1422) ----------------------------------------
@@ -610,12 +804,14 @@ __global__ __launch_bounds__(BS) void k_diagprec(DMesh m, Phys P, const double* 
 // and stored as TV, NV*NV rows of pitch.
 template <int NV, class TV>
 __global__ __launch_bounds__(BS) void k_block(DMesh m, Phys P, const double* __restrict__ U,
-                                               const double* __restrict__ dt, TV* __restrict__ Binv,
-                                               double* __restrict__ Bout) {
+                                               const double* __restrict__ G, const double* __restrict__ dt,
+                                               TV* __restrict__ Binv, double* __restrict__ Bout) {
   typedef Dual<double> D;
   const int c = xcd_block() * BS + threadIdx.x;
   if (c >= m.N) return;
   const int p = m.pitch;
+  Phys P1 = P;  // first-order inviscid flux for the blocks
+  P1.spatial = CFD_FIRST_ORDER;
   double B[NV][NV];
 #pragma unroll
   for (int i = 0; i < NV; ++i)
@@ -630,11 +826,10 @@ __global__ __launch_bounds__(BS) void k_block(DMesh m, Phys P, const double* __r
     if (e < 0) continue;
     const int f = e >> 1;
     const bool as_nei = e & 1;
-    const int o = m.own[f], n = m.nei[f], tg = m.tag[f];
-    const double nx = m.nx[f], ny = m.ny[f], A = m.area[f];
+    const int o = m.own[f], n = m.nei[f];
+    const FaceGeo g = face_geo(m, P1, nullptr, f, o, n, false);
     const int other = as_nei ? o : n;
     const S6<double> Uo = other >= 0 ? ld<NV>(U, p, other) : Uc;
-    const int scheme = (m.alt && m.alt[f]) ? P.flux_alt : P.flux;
 #pragma unroll
     for (int k = 0; k < NV; ++k) {
       S6<D> Ud, Od;
@@ -644,31 +839,38 @@ __global__ __launch_bounds__(BS) void k_block(DMesh m, Phys P, const double* __r
         Od[i] = D(Uo[i], 0.0);
       }
       S6<D> Fl;
-      if (n < 0) {
-        if (tg == 3 || tg == 4) {
-          const D pw_ = pressure(P, Ud);
-          Fl[1] = pw_ * nx;
-          Fl[2] = pw_ * ny;
-        } else {
-          Fl = riemann(P, Ud, ghost(P, Ud, tg, nx, ny), nx, ny, scheme);
-        }
-      } else if (!as_nei) {
-        Fl = riemann(P, Ud, Od, nx, ny, scheme);
-      } else {
-        Fl = riemann(P, Od, Ud, nx, ny, scheme);
-      }
-      const double sg = as_nei ? -A : A;
+      if (n < 0) Fl = flux_core<0, D>(P1, g, Ud, ghost(P1, Ud, g.tg, g.nx, g.ny, g.dnw), nullptr, nullptr);
+      else if (!as_nei) Fl = flux_core<0, D>(P1, g, Ud, Od, nullptr, nullptr);
+      else Fl = flux_core<0, D>(P1, g, Od, Ud, nullptr, nullptr);
+      const double sg = as_nei ? -1.0 : 1.0;  // flux_core returns F * A
 #pragma unroll
       for (int i = 0; i < NV; ++i) B[i][k] += sg * Fl[i].d;
     }
     if (P.viscous) {
       double d;
       if (other >= 0) {
-        d = fabs((m.xc[other] - m.xc[c]) * nx + (m.yc[other] - m.yc[c]) * ny);
+        d = fabs((m.xc[other] - m.xc[c]) * g.nx + (m.yc[other] - m.yc[c]) * g.ny);
       } else {
-        d = 2.0 * fabs((m.xm[f] - m.xc[c]) * nx + (m.ym[f] - m.yc[c]) * ny);
+        d = 2.0 * fabs((m.xm[f] - m.xc[c]) * g.nx + (m.ym[f] - m.yc[c]) * g.ny);
       }
-      visc += P.nu * A / fmax(d, 1e-300);
+      visc += P.nu * g.A / fmax(d, 1e-300);
+    }
+  }
+  if constexpr (NV == 6) {
+    if (P.turb) {  // source Jacobian with the gradients frozen
+      D gd[14];
+#pragma unroll
+      for (int q = 0; q < 14; ++q) gd[q] = D(G[(size_t)q * p + c], 0.0);
+      const double V = m.vol[c];
+#pragma unroll
+      for (int k = 0; k < NV; ++k) {
+        S6<D> Ud;
+#pragma unroll
+        for (int i = 0; i < 6; ++i) Ud[i] = D(Uc[i], i == k ? 1.0 : 0.0);
+        const TurbQ<D> tq = turb_cell(P, Ud, gd, m.wd[c]);
+        B[4][k] -= V * tq.Sk.d;
+        B[5][k] -= V * tq.Sw.d;
+      }
     }
   }
   const double tc = m.vol[c] / (dt[c] + 1e-30);
@@ -917,6 +1119,12 @@ __global__ __launch_bounds__(BS) void k_newton_update(int N, int p, Phys P, doub
     pn = 1e-5;
     u[3] = pn / (P.gamma - 1.0) + ke;
   }
+  if constexpr (NV == 6) {
+    if (P.turb) {  // SST: keep k and omega positive
+      if (u[4] < u[0] * P.kmin) u[4] = u[0] * P.kmin;
+      if (u[5] < u[0] * P.wmin) u[5] = u[0] * P.wmin;
+    }
+  }
 #pragma unroll
   for (int k = 0; k < NV; ++k) U[k * p + c] = u[k];
 }
@@ -1124,6 +1332,9 @@ static void residual_nv(cfd_ctx* c, const double* U, double* R, bool upd, hipEve
     if (ev) (void)hipEventRecord(ev[3], c->stream);
     hipLaunchKernelGGL((k_gather<NV, double>), dim3(nblocks(m.N)), dim3(BS), 0, c->stream, m, c->Fm, R);
   }
+  if (NV == 6 && P.turb)
+    hipLaunchKernelGGL((k_turb_src<NV, double>), dim3(nblocks(m.N)), dim3(BS), 0, c->stream, m, P, U,
+                       (const double*)nullptr, (const double*)c->grad, (const double*)nullptr, R);
   if (ev) (void)hipEventRecord(ev[4], c->stream);
 }
 
@@ -1134,7 +1345,7 @@ static void tangent_nv(cfd_ctx* c, const double* U, const F* v, F* Jv) {
   const DMesh& m = c->dm;
   const Phys& P = c->phys;
   F* Gv = (F*)c->gradT;
-  F* Gd = Gv + (size_t)10 * m.pitch;
+  F* Gd = Gv + (size_t)14 * m.pitch;
   hipLaunchKernelGGL((k_gradlim<NV, T>), dim3(nblocks(m.N)), dim3(BS), 0, c->stream, m, P, U, v, Gv, Gd, c->phi, 0);
   if (c->assembly == 0) {
     hipLaunchKernelGGL(k_fill<F>, dim3(nblocks((size_t)NV * m.pitch)), dim3(BS), 0, c->stream, Jv, F(0),
@@ -1151,6 +1362,9 @@ static void tangent_nv(cfd_ctx* c, const double* U, const F* v, F* Jv) {
                        (const F*)Gv, (const F*)Gd, c->phi, Fm, 0, m.F);
     hipLaunchKernelGGL((k_gather<NV, F>), dim3(nblocks(m.N)), dim3(BS), 0, c->stream, m, (const F*)Fm, Jv);
   }
+  if (NV == 6 && P.turb)
+    hipLaunchKernelGGL((k_turb_src<NV, T>), dim3(nblocks(m.N)), dim3(BS), 0, c->stream, m, P, U, v, (const F*)Gv,
+                       (const F*)Gd, Jv);
   hipLaunchKernelGGL((k_tcombine<NV, F>), dim3(nblocks(m.N)), dim3(BS), 0, c->stream, m.N, m.pitch, Jv, v, c->d_vol,
                      c->dt);
 }
@@ -1163,13 +1377,16 @@ static void vjp_nv(cfd_ctx* c, const double* U, const double* w, double* JTw) {
   hipLaunchKernelGGL((k_gradlim<NV, double>), dim3(nblocks(m.N)), dim3(BS), 0, c->stream, m, P, U,
                      (const double*)nullptr, c->grad, (double*)nullptr, c->phi, 0);
   launch_fill(c, JTw, 0.0, (size_t)NV * m.pitch);
-  launch_fill(c, aG, 0.0, (size_t)10 * m.pitch);
+  launch_fill(c, aG, 0.0, (size_t)14 * m.pitch);
   for (int col = 0; col < c->ncolors; ++col) {
     const int f0 = c->color_ptr[col], f1 = c->color_ptr[col + 1];
     if (f1 > f0)
       hipLaunchKernelGGL((k_vjp_face<NV, VISC>), dim3(nblocks(f1 - f0)), dim3(BS), 0, c->stream, m, P, U,
                          (const double*)c->grad, (const double*)c->phi, w, JTw, aG, f0, f1);
   }
+  if (NV == 6 && P.turb)
+    hipLaunchKernelGGL((k_vjp_src<NV>), dim3(nblocks(m.N)), dim3(BS), 0, c->stream, m, P, U,
+                       (const double*)c->grad, w, JTw, aG);
   for (int col = 0; col < c->ncolors; ++col) {
     const int f0 = c->color_ptr[col], f1 = c->color_ptr[col + 1];
     if (f1 > f0)
@@ -1294,6 +1511,10 @@ void launch_explicit(cfd_ctx* c, double* Unew, const double* U0, double a0, cons
                                             c->pitch, c->d_vol, Unew, U0, a0, U1, a1, R, dtc));
 }
 
+void launch_wall_distance(cfd_ctx* c, const double* wxy, int nw, double* wd) {
+  hipLaunchKernelGGL(k_walldist, dim3(nblocks(c->N)), dim3(BS), 0, c->stream, c->N, c->d_xc, c->d_yc, wxy, nw, wd);
+}
+
 void launch_wall(cfd_ctx* c, const double* U, double* out3) {
   const Phys& P = c->phys;
   const double g = P.gamma;
@@ -1366,8 +1587,12 @@ void launch_store_basis(cfd_ctx* c, void* Vj1, const double* w, const double* hd
 }
 
 template <class TV> static void block_t(cfd_ctx* c, const double* U, TV* Binv, double* Bout) {
-  NV_DISPATCH(c->P.nvar, hipLaunchKernelGGL((k_block<NVt, TV>), dim3(nblocks(c->N)), dim3(BS), 0, c->stream, c->dm,
-                                            c->phys, U, c->dt, Binv, Bout));
+  NV_DISPATCH(c->P.nvar, {
+    hipLaunchKernelGGL((k_gradlim<NVt, double>), dim3(nblocks(c->N)), dim3(BS), 0, c->stream, c->dm, c->phys, U,
+                       (const double*)nullptr, c->grad, (double*)nullptr, c->phi, 0);
+    hipLaunchKernelGGL((k_block<NVt, TV>), dim3(nblocks(c->N)), dim3(BS), 0, c->stream, c->dm, c->phys, U,
+                       (const double*)c->grad, c->dt, Binv, Bout);
+  });
 }
 void launch_block_precond(cfd_ctx* c, const double* U, void* Binv, int prec, double* Bout) {
   if (prec == 32) block_t<float>(c, U, (float*)Binv, Bout);

@@ -204,7 +204,8 @@ struct Phys {
   double gamma, R, P0, T0, pexit, nu, Pr, Cp;
   double fs[6];  // frozen freestream (SUPERSONIC tag 1, PLATE tags 1 and 5)
   double sh[6];  // frozen post-shock state (SWBLI tag 6)
-  int flux, flux_alt, spatial, bcset, viscous, nvar;
+  double kmin, wmin;  // SST positivity floors of specific k and omega
+  int flux, flux_alt, spatial, bcset, viscous, nvar, turb;
 };
 
 template <class T> HD T pressure(const Phys& P, const S6<T>& U) {
@@ -215,6 +216,70 @@ template <class T> HD S6<T> prim2cons(const Phys& P, T rho, T u, T v, T p) {
   T rE = p / (P.gamma - 1.0) + 0.5 * rho * (u * u + v * v);
   return S6<T>(rho, rho * u, rho * v, rE, rho * T(0.0), rho * T(0.0));
 }
+template <class T> HD S6<T> prim2cons(const Phys& P, T rho, T u, T v, T p, T k, T w) {
+  T rE = p / (P.gamma - 1.0) + 0.5 * rho * (u * u + v * v);
+  return S6<T>(rho, rho * u, rho * v, rE, rho * k, rho * w);
+}
+
+// ---- SST k-omega (no reference counterpart: Menter SST-2003 as defined by
+// the NASA Turbulence Modeling Resource; the reference reserves the rhok, rhow
+// slots, Types.hpp:61-64, and the wall distance, Solver.cpp:1981-2015) -------
+namespace sst {
+constexpr double sk1 = 0.85, sw1 = 0.5, b1 = 0.075, sk2 = 1.0, sw2 = 0.856, b2 = 0.0828, bs = 0.09, a1 = 0.31,
+                 kap = 0.41, Prt = 0.9;
+HD double g1() { return b1 / bs - sw1 * kap * kap / sqrt(bs); }
+HD double g2() { return b2 / bs - sw2 * kap * kap / sqrt(bs); }
+}  // namespace sst
+
+HD double th(double x) { return tanh(x); }
+HD float th(float x) { return tanhf(x); }
+template <class F> HD Dual<F> th(Dual<F> x) {
+  F t = th(x.v);
+  return Dual<F>(t, (F(1) - t * t) * x.d);
+}
+template <class F, int K> HD DualV<F, K> th(const DualV<F, K>& x) {
+  DualV<F, K> r;
+  r.v = tanh(x.v);
+  const F dv = F(1) - r.v * r.v;
+#pragma unroll
+  for (int i = 0; i < K; ++i) r.d[i] = dv * x.d[i];
+  return r;
+}
+
+// Eddy viscosity, blending F1 and the k / omega sources (per unit volume) of
+// a cell from its state, its 14 gradients (rho u v p T k w, x then y) and its
+// wall distance d.
+template <class T> struct TurbQ {
+  T mut, F1, Sk, Sw;
+};
+template <class T> HD TurbQ<T> turb_cell(const Phys& P, const S6<T>& U, const T* g, double d) {
+  using namespace sst;
+  const double nu = P.nu;
+  const T rho = U[0];
+  const T k = mx(U[4] / rho, T(1e-20)), w = mx(U[5] / rho, T(1e-20));
+  const T ux = g[2], uy = g[3], vx = g[4], vy = g[5];
+  const T S2 = 2.0 * ux * ux + 2.0 * vy * vy + (uy + vx) * (uy + vx);
+  const T S = sq_rt(S2 + 1e-20);
+  const T gkw = g[10] * g[12] + g[11] * g[13];
+  const T CD = mx(2.0 * rho * sw2 * gkw / w, T(1e-10));
+  const T sk = sq_rt(k);
+  const T arg1 = mn(mx(sk / (bs * w * d), 500.0 * nu / (d * d * w)), 4.0 * rho * sw2 * k / (CD * d * d));
+  const T F1 = th(arg1 * arg1 * arg1 * arg1);
+  const T arg2 = mx(2.0 * sk / (bs * w * d), 500.0 * nu / (d * d * w));
+  const T F2 = th(arg2 * arg2);
+  const T mut = rho * a1 * k / mx(a1 * w, S * F2);
+  const T Pk = mn(mut * S2, 10.0 * bs * rho * w * k);
+  const T beta = F1 * b1 + (1.0 - F1) * b2;
+  const T gam = F1 * g1() + (1.0 - F1) * g2();
+  TurbQ<T> q;
+  q.mut = mut;
+  q.F1 = F1;
+  q.Sk = Pk - bs * rho * w * k;
+  q.Sw = gam * rho * S2 - beta * rho * w * w + 2.0 * (1.0 - F1) * rho * sw2 * gkw / w;
+  return q;
+}
+// omega at a no-slip wall: 10 * 6 nu / (beta1 dn^2), dn the owner centroid's normal distance
+HD double omega_wall(const Phys& P, double dn) { return 60.0 * P.nu / (sst::b1 * dn * dn); }
 template <class T> HD S6<T> frozen(const double* s) {
   return S6<T>(T(s[0]), T(s[1]), T(s[2]), T(s[3]), T(s[4]), T(s[5]));
 }
@@ -229,7 +294,7 @@ template <class T> HD S6<T> slip_ghost(const S6<T>& U, double nx, double ny) {
 }
 
 // Ghost state of a boundary face with tag `tag` seen from interior state U.
-template <class T> HD S6<T> ghost(const Phys& P, const S6<T>& U, int tag, double nx, double ny) {
+template <class T> HD S6<T> ghost(const Phys& P, const S6<T>& U, int tag, double nx, double ny, double dn = 0.0) {
   if (P.bcset == CFD_BC_BUMP) {  // Solver.cpp:806-863
     if (tag == 1) {
       T pin = pressure(P, U);
@@ -261,6 +326,11 @@ template <class T> HD S6<T> ghost(const Phys& P, const S6<T>& U, int tag, double
     T pin = (P.gamma - 1.0) * (U[3] - ek);
     T Tin = pin / (U[0] * P.R);
     T rg = pin / (P.R * Tin);
+    if (P.turb) {  // k = 0 and omega = omega_wall on the wall face
+      T kg = -(U[4] / U[0]);
+      T wg = 2.0 * omega_wall(P, dn) - U[5] / U[0];
+      return prim2cons(P, rg, ug, vg, pin, kg, wg);
+    }
     return prim2cons(P, rg, ug, vg, pin);
   }
   if (tag == 6 && P.bcset == CFD_BC_SWBLI) return frozen<T>(P.sh);

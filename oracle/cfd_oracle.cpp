@@ -210,13 +210,65 @@ static void compute_geometry(const Topo& t, const std::vector<T>& X, const std::
 }
 
 // ---- the discretisation ---------------------------------------------------------
+template <class T> struct Disc;
+inline double ttanh_(double x) { return std::tanh(x); }
+inline cplx ttanh_(const cplx& x) {
+  double t = std::tanh(x.real());
+  return cplx(t, (1.0 - t * t) * x.imag());
+}
+
 template <class T> struct Disc {
   const Topo* topo = nullptr;
   const Geo<T>* geo = nullptr;
   cfd_params P{};
   const unsigned char* face_alt = nullptr;  // extension: per-face flux choice
-  std::vector<std::array<T, 10>> grads;     // drhodx drhody dudx dudy dvdx dvdy dpdx dpdy dTdx dTdy
+  const double* wdist = nullptr;            // extension (SST): wall distance per cell
+  // drhodx drhody dudx dudy dvdx dvdy dpdx dpdy dTdx dTdy (+ dkdx dkdy dwdx dwdy with SST)
+  std::vector<std::array<T, 14>> grads;
   std::vector<T> limiters;
+
+  // ---- SST k-omega (extension; Menter SST-2003 as defined by the NASA
+  // Turbulence Modeling Resource), per-cell quantities from the cell state,
+  // its gradients and its wall distance d.
+  struct Turb {
+    T mut, F1, Sk, Sw;
+  };
+  static constexpr double sk1 = 0.85, sw1 = 0.5, b1 = 0.075, sk2 = 1.0, sw2 = 0.856, b2 = 0.0828, bs = 0.09,
+                          a1 = 0.31, kap = 0.41, Prt = 0.9;
+  static double g1() { return b1 / bs - sw1 * kap * kap / std::sqrt(bs); }
+  static double g2() { return b2 / bs - sw2 * kap * kap / std::sqrt(bs); }
+  static T ttanh(const T& x) { return ttanh_(x); }
+  Turb turb_cell(const St<T>& U, const std::array<T, 14>& g, double d) const {
+    const double nu = P.nu;
+    T rho = U[0];
+    T k = smax(U[4] / rho, T(1e-20)), w = smax(U[5] / rho, T(1e-20));
+    T ux = g[2], uy = g[3], vx = g[4], vy = g[5];
+    T S2 = 2.0 * ux * ux + 2.0 * vy * vy + (uy + vx) * (uy + vx);
+    T S = ssqrt(S2 + 1e-20);
+    T gkw = g[10] * g[12] + g[11] * g[13];
+    T CD = smax(2.0 * rho * sw2 * gkw / w, T(1e-10));
+    T sk = ssqrt(k);
+    T arg1 = smin(smax(sk / (bs * w * d), 500.0 * nu / (d * d * w)), 4.0 * rho * sw2 * k / (CD * d * d));
+    T F1 = ttanh(arg1 * arg1 * arg1 * arg1);
+    T arg2 = smax(2.0 * sk / (bs * w * d), 500.0 * nu / (d * d * w));
+    T F2 = ttanh(arg2 * arg2);
+    T mut = rho * a1 * k / smax(a1 * w, S * F2);
+    T Pk = smin(mut * S2, 10.0 * bs * rho * w * k);
+    T beta = F1 * b1 + (1.0 - F1) * b2;
+    T gam = F1 * g1() + (1.0 - F1) * g2();
+    Turb t;
+    t.mut = mut;
+    t.F1 = F1;
+    t.Sk = Pk - bs * rho * w * k;
+    t.Sw = gam * rho * S2 - beta * rho * w * w + 2.0 * (1.0 - F1) * rho * sw2 * gkw / w;
+    return t;
+  }
+  // omega at a no-slip wall, 10 * 6 nu / (beta1 d^2) with d the centroid-to-wall normal distance
+  T omega_wall(int f) const {
+    int o = topo->own[f];
+    T dn = sabs((geo->xm[f] - geo->xc[o]) * geo->nx[f] + (geo->ym[f] - geo->yc[o]) * geo->ny[f]);
+    return 60.0 * P.nu / (b1 * dn * dn);
+  }
 
   // Flux.hpp:12-15 (GAMMA of Flux.hpp:9 equals Solver::gamma, 1.4)
   T pressure(const St<T>& U) const {
@@ -236,6 +288,7 @@ template <class T> struct Disc {
     double rho = P.P_static_exit / (P.R_gas * Ts);
     double c = std::sqrt(g * P.R_gas * Ts);
     double u = P.M_inf * c;
+    if (P.turb) return p2c(T(rho), T(u), T(0.0), T(P.P_static_exit), T(P.turb_k_inf), T(P.turb_w_inf));
     return p2c(T(rho), T(u), T(0.0), T(P.P_static_exit));
   }
   St<T> slip(const St<T>& U, int f) const {  // tag 3 everywhere (e.g. Solver.cpp:845-860)
@@ -286,6 +339,11 @@ template <class T> struct Disc {
           T pin = (g - 1.0) * (U[3] - ek);
           T Tin = pin / (U[0] * Rg);
           T rg = pin / (Rg * Tin);
+          if (P.turb) {  // extension: k = 0 and omega = omega_wall on the wall face
+            T kg = -(U[4] / U[0]);
+            T wg = 2.0 * omega_wall(f) - U[5] / U[0];
+            return p2c(rg, ug, vg, pin, kg, wg);
+          }
           return p2c(rg, ug, vg, pin, T(0.0), T(0.0));
         }
         if (tg == 6 && P.bcset == CFD_BC_SWBLI)
@@ -412,28 +470,34 @@ template <class T> struct Disc {
     const Geo<T>& G = *geo;
     int N = t.ncells(), F = t.nfaces();
     double Rg = P.R_gas;
-    grads.assign(N, std::array<T, 10>{});
+    grads.assign(N, std::array<T, 14>{});
     for (auto& a : grads) a.fill(T(0));
+    const int nq = P.turb ? 7 : 5;
     for (int f = 0; f < F; ++f) {
       int o = t.own[f], n = t.nei[f];
       St<T> L = U[o];
       St<T> R = n >= 0 ? U[n] : ghost(L, f);
       T uL = L[1] / L[0], vL = L[2] / L[0], pL = pressure(L), TL = pL / (L[0] * Rg);
       T uR = R[1] / R[0], vR = R[2] / R[0], pR = pressure(R), TR = pR / (R[0] * Rg);
-      T phi[5] = {0.5 * (L[0] + R[0]), 0.5 * (uL + uR), 0.5 * (vL + vR), 0.5 * (pL + pR), 0.5 * (TL + TR)};
+      T phi[7] = {0.5 * (L[0] + R[0]), 0.5 * (uL + uR), 0.5 * (vL + vR), 0.5 * (pL + pR), 0.5 * (TL + TR),
+                  T(0), T(0)};
+      if (P.turb) {
+        phi[5] = 0.5 * (L[4] / L[0] + R[4] / R[0]);
+        phi[6] = 0.5 * (L[5] / L[0] + R[5] / R[0]);
+      }
       T nxA_ = G.nx[f], nyA_ = G.ny[f], A = G.area[f];
-      for (int k = 0; k < 5; ++k) {
+      for (int k = 0; k < nq; ++k) {
         grads[o][2 * k] += phi[k] * nxA_ * A;
         grads[o][2 * k + 1] += phi[k] * nyA_ * A;
       }
       if (n >= 0)
-        for (int k = 0; k < 5; ++k) {
+        for (int k = 0; k < nq; ++k) {
           grads[n][2 * k] -= phi[k] * nxA_ * A;
           grads[n][2 * k + 1] -= phi[k] * nyA_ * A;
         }
     }
     for (int c = 0; c < N; ++c)
-      for (int k = 0; k < 10; ++k) grads[c][k] /= G.vol[c];
+      for (int k = 0; k < 2 * nq; ++k) grads[c][k] /= G.vol[c];
   }
   // slot order of the ten gradients: rho(0,1) u(2,3) v(4,5) p(6,7) T(8,9)
 
@@ -491,6 +555,11 @@ template <class T> struct Disc {
     }
     double Cp = g * Rg / (g - 1.0);
     double Pr = P.Pr;
+    std::vector<Turb> tc;
+    if (P.turb) {
+      tc.resize(N);
+      for (int c = 0; c < N; ++c) tc[c] = turb_cell(U[c], grads[c], wdist[c]);
+    }
     for (int f = 0; f < F; ++f) {
       int o = t.own[f], n = t.nei[f], tg = t.tag[f];
       T nx = G.nx[f], ny = G.ny[f];
@@ -526,6 +595,19 @@ template <class T> struct Disc {
         if (re(rRf) < 1e-4) rRf = R[0];
         if (re(pRf) < 1e-4) pRf = pR;
         Rr = p2c(rRf, uRf, vRf, pRf);
+        if (P.turb) {  // extension: MUSCL on specific k and omega, clipped like rho and p
+          T kL = L[4] / L[0], wL = L[5] / L[0], kR = R[4] / R[0], wR = R[5] / R[0];
+          T kLf = kL + phL * (gL[10] * dxL + gL[11] * dyL), wLf = wL + phL * (gL[12] * dxL + gL[13] * dyL);
+          T kRf = kR + phR * (gR[10] * dxR + gR[11] * dyR), wRf = wR + phR * (gR[12] * dxR + gR[13] * dyR);
+          if (re(kLf) < 0.0) kLf = kL;
+          if (re(wLf) < 0.0) wLf = wL;
+          if (re(kRf) < 0.0) kRf = kR;
+          if (re(wRf) < 0.0) wRf = wR;
+          Lr[4] = rLf * kLf;
+          Lr[5] = rLf * wLf;
+          Rr[4] = rRf * kRf;
+          Rr[5] = rRf * wRf;
+        }
       }
       St<T> FE;
       if (n < 0 && (tg == 3 || tg == 4)) {
@@ -537,6 +619,12 @@ template <class T> struct Disc {
       } else {
         int scheme = (face_alt && face_alt[f]) ? P.flux_alt : P.flux;
         FE = flux(Lr, Rr, nx, ny, scheme);
+      }
+      if (P.turb) {  // extension: k and omega carried by the mass flux, upwinded
+        T m = FE[0];
+        T mp = re(m) > 0.0 ? m : T(0.0), mm = re(m) > 0.0 ? T(0.0) : m;
+        FE[4] = mp * (Lr[4] / Lr[0]) + mm * (Rr[4] / Rr[0]);
+        FE[5] = mp * (Lr[5] / Lr[0]) + mm * (Rr[5] / Rr[0]);
       }
       St<T> Tot = FE;
       if (P.viscous) {
@@ -581,6 +669,18 @@ template <class T> struct Disc {
         }
         T mu = rf * P.nu;
         T k = mu * Cp / Pr;
+        T mut = T(0.0), F1f = T(0.0);
+        if (P.turb) {  // extension: eddy viscosity in stress and heat flux
+          if (n >= 0) {
+            mut = 0.5 * (tc[o].mut + tc[n].mut);
+            F1f = 0.5 * (tc[o].F1 + tc[n].F1);
+          } else {
+            mut = tg == 4 ? T(0.0) : tc[o].mut;
+            F1f = tc[o].F1;
+          }
+          k = Cp * (mu / Pr + mut / Prt);
+          mu = mu + mut;
+        }
         T div = ux + vy;
         T txx = mu * (2.0 * ux - (2.0 / 3.0) * div);
         T tyy = mu * (2.0 * vy - (2.0 / 3.0) * div);
@@ -593,6 +693,40 @@ template <class T> struct Disc {
         T wx = uf * txx + vf * txy - qx;
         T wy = uf * txy + vf * tyy - qy;
         FV[3] = wx * nx + wy * ny;
+        if (P.turb) {  // extension: (mu + sigma mu_t) grad(k, omega) . n
+          T kx, ky, wx_, wy_;
+          T kc = L[4] / L[0], wc = L[5] / L[0];
+          if (n < 0 && tg == 4) {
+            T dx = G.xm[f] - G.xc[o], dy = G.ym[f] - G.yc[o];
+            T d2 = dx * dx + dy * dy;
+            T ww = omega_wall(f);
+            kx = -kc * dx / d2;
+            ky = -kc * dy / d2;
+            wx_ = (ww - wc) * dx / d2;
+            wy_ = (ww - wc) * dy / d2;
+          } else {
+            kx = 0.5 * (gL[10] + gR[10]);
+            ky = 0.5 * (gL[11] + gR[11]);
+            wx_ = 0.5 * (gL[12] + gR[12]);
+            wy_ = 0.5 * (gL[13] + gR[13]);
+            if (n >= 0) {
+              T dxLR = G.xc[n] - G.xc[o], dyLR = G.yc[n] - G.yc[o];
+              T d2 = dxLR * dxLR + dyLR * dyLR;
+              if (re(d2) > 1e-14) {
+                T dk = (R[4] / R[0]) - kc, dw = (R[5] / R[0]) - wc;
+                T kx0 = kx, ky0 = ky, wx0 = wx_, wy0 = wy_;
+                kx = kx0 - (kx0 * dxLR + ky0 * dyLR - dk) * dxLR / d2;
+                ky = ky0 - (kx0 * dxLR + ky0 * dyLR - dk) * dyLR / d2;
+                wx_ = wx0 - (wx0 * dxLR + wy0 * dyLR - dw) * dxLR / d2;
+                wy_ = wy0 - (wx0 * dxLR + wy0 * dyLR - dw) * dyLR / d2;
+              }
+            }
+          }
+          T mul = rf * P.nu;
+          T sgk = F1f * sk1 + (1.0 - F1f) * sk2, sgw = F1f * sw1 + (1.0 - F1f) * sw2;
+          FV[4] = (mul + sgk * mut) * (kx * nx + ky * ny);
+          FV[5] = (mul + sgw * mut) * (wx_ * nx + wy_ * ny);
+        }
         Tot = FE - FV;
       }
       T A = G.area[f];
@@ -600,6 +734,11 @@ template <class T> struct Disc {
       Rout[o] = Rout[o] + Tot * A;
       if (n >= 0) Rout[n] = Rout[n] - Tot * A;
     }
+    if (P.turb)  // extension: SST sources enter R with a minus sign (R = fluxes out - V S)
+      for (int c = 0; c < N; ++c) {
+        Rout[c][4] = Rout[c][4] - G.vol[c] * tc[c].Sk;
+        Rout[c][5] = Rout[c][5] - G.vol[c] * tc[c].Sw;
+      }
   }
 };
 
@@ -611,13 +750,36 @@ struct Solver {
   Disc<double> D;
   std::vector<double> dt;
   std::vector<unsigned char> face_alt;
+  std::vector<double> wdist;
   int N() const { return topo.ncells(); }
 
+  // Solver::computeWallDistances (Solver.cpp:1981-2015): brute-force minimum
+  // distance from each centroid to the midpoints of the no-slip (tag 4) faces
+  void wall_distance() {
+    int n = N();
+    wdist.assign(n, 1e10);
+    std::vector<int> walls;
+    for (int f = 0; f < topo.nfaces(); ++f)
+      if (topo.nei[f] < 0 && topo.tag[f] == 4) walls.push_back(f);
+    if (walls.empty()) return;
+    for (int c = 0; c < n; ++c) {
+      double md = 1e20, cx = geo.xc[c], cy = geo.yc[c];
+      for (int f : walls) {
+        double dx = cx - geo.xm[f], dy = cy - geo.ym[f];
+        double d2 = dx * dx + dy * dy;
+        if (d2 < md) md = d2;
+      }
+      wdist[c] = std::sqrt(md);
+    }
+  }
+
   void bind() {
+    if ((int)wdist.size() != N()) wall_distance();
     D.topo = &topo;
     D.geo = &geo;
     D.P = P;
     D.face_alt = face_alt.empty() ? nullptr : face_alt.data();
+    D.wdist = wdist.data();
   }
   static const St<double>* S(const double* p) { return reinterpret_cast<const St<double>*>(p); }
   static St<double>* S(double* p) { return reinterpret_cast<St<double>*>(p); }
@@ -819,6 +981,7 @@ struct Solver {
           pn = 1e-5;
           Un[3] = pn / (g - 1.0) + ke;
         }
+        if (P.turb) turb_floor(Un);
         U[c] = Un;
       }
       if (step > 5) {
@@ -830,6 +993,12 @@ struct Solver {
       }
     }
     return step;
+  }
+  // extension (SST): k and omega kept positive after an update
+  void turb_floor(St<double>& u) const {
+    double kmin = 1e-8 * P.turb_k_inf, wmin = 1e-3 * P.turb_w_inf;
+    if (u[4] < u[0] * kmin) u[4] = u[0] * kmin;
+    if (u[5] < u[0] * wmin) u[5] = u[0] * wmin;
   }
   // Solver::timeStepExplicit (Solver.cpp:1219-1243)
   void explicit_step(St<double>* U, double dtc) {
@@ -950,6 +1119,7 @@ void orc_set_params(orc_solver* s, const cfd_params* p) {
   s->s.P = *p;
   s->s.D.P = *p;
 }
+void orc_wall_distance(orc_solver* s, double* d) { std::copy(s->s.wdist.begin(), s->s.wdist.end(), d); }
 void orc_set_face_flux(orc_solver* s, const unsigned char* alt) {
   s->s.face_alt.assign(alt, alt + s->s.topo.nfaces());
   s->s.bind();
@@ -972,6 +1142,7 @@ void orc_residual_cs(orc_solver* s, const double* U6, const double* v6, double h
   D.geo = &gc;
   D.P = S.P;
   D.face_alt = S.face_alt.empty() ? nullptr : S.face_alt.data();
+  D.wdist = S.wdist.data();
   if (!update && !S.D.limiters.empty()) {
     D.limiters.resize(n);
     for (int c = 0; c < n; ++c) D.limiters[c] = S.D.limiters[c];

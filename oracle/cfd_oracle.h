@@ -22,6 +22,7 @@ orc_solver* orc_solver_create(const orc_mesh* m, const cfd_params* p);
 void orc_solver_destroy(orc_solver* s);
 void orc_set_params(orc_solver* s, const cfd_params* p);
 void orc_set_face_flux(orc_solver* s, const unsigned char* alt);
+void orc_wall_distance(orc_solver* s, double* d);
 void orc_residual(orc_solver* s, const double* U6, double* R6, int update);
 void orc_residual_cs(orc_solver* s, const double* U6, const double* v6, double h, double* Rre, double* dR,
                      int update);

@@ -61,6 +61,7 @@ def oracle_lib():
             "orc_solver_destroy": (None, [_VP]),
             "orc_set_params": (None, [_VP, P]),
             "orc_set_face_flux": (None, [_VP, C.POINTER(C.c_ubyte)]),
+            "orc_wall_distance": (None, [_VP, _DP]),
             "orc_residual": (None, [_VP, _DP, _DP, C.c_int]),
             "orc_residual_cs": (None, [_VP, _DP, _DP, C.c_double, _DP, _DP, C.c_int]),
             "orc_face_flux": (None, [_VP, _DP, _DP, C.c_int]),
@@ -206,6 +207,11 @@ class OracleSolver:
     def set_face_flux(self, alt):
         a = np.ascontiguousarray(alt, np.uint8)
         self.lib.orc_set_face_flux(self.h, a.ctypes.data_as(C.POINTER(C.c_ubyte)))
+
+    def wall_distance(self):
+        d = np.zeros(self.N)
+        self.lib.orc_wall_distance(self.h, _dp(d))
+        return d
 
     def compute_residuals(self, U, update_limiters=True):
         U = np.ascontiguousarray(U, np.float64)

@@ -33,9 +33,9 @@ def _setup(config, scale, flux, assembly=0):
     U[:, :4] *= 1 + 0.01 * rng.standard_normal((U.shape[0], 4))
     # a small vertical velocity keeps |u_n| away from the kink at zero
     U[:, 2] += 0.01 * U[:, 1] * rng.standard_normal(U.shape[0])
-    if p.nvar == 6:
-        U[:, 4] = U[:, 0] * 1e-3 * (1 + rng.random(U.shape[0]))
-        U[:, 5] = U[:, 0] * 1e3 * (1 + rng.random(U.shape[0]))
+    if p.nvar == 6:  # SST: perturbed freestream k and omega
+        U[:, 4] *= 1 + 0.5 * rng.random(U.shape[0])
+        U[:, 5] *= 1 + 0.5 * rng.random(U.shape[0])
     else:
         U[:, 4:] = 0
     v = rng.standard_normal(U.shape) * np.abs(U).mean(0)
