@@ -1,19 +1,23 @@
-"""Residual throughput of the MI355X library on the laminar flat-plate case.
+"""Residual throughput and JFNK time-to-converge of the MI355X library on the
+turbulent flat plate (the configuration the headline metric is quoted on).
 
-Workload (one "step" = one full second-order residual evaluation
-R(U) = sum of Roe + viscous face fluxes, FP64, on device-resident state):
-laminar flat plate, Mach 0.2, Re_L = 1e5, 400 x 160 stretched quads with a
-seeded 20 % split into triangles (76,790 cells), plate boundary conditions.
+Workload: turbulent flat plate, Mach 0.2, Re = 5e6, 1200 x 1000 stretched
+quads with a seeded 10 % split into triangles (1,320,146 cells, 2,522,346
+faces), Roe + viscous + SST k-omega, second-order central, six conserved
+variables, plate boundary conditions.  One "step" is one full FP64 residual
+evaluation on device-resident state.
 
-Prints one JSON line:  value = Mcell-residuals/s (cells x residuals / s,
-wall clock around K residuals, inputs already in HBM), plus
-  roofline      the dominant kernel (face flux), algorithmic bytes of its
-                inputs/outputs per launch over its HIP-event duration;
-  roofline_residual   the whole residual against B_res = N(16 n_v + 24) + 48 F;
-  cpu_baseline  the CPU restatement of the reference residual (g++ -O2,
-                one core) on the same mesh and state.
-
-Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--assembly 0|1]
+The JSON line carries:
+  value              Mcell-residuals/s (cells x residuals / wall second);
+  roofline           the dominant residual kernel: algorithmic bytes of its own
+                     inputs and outputs per residual over its HIP-event time;
+  roofline_residual  the whole residual against B_res = N(16 n_v + 24) + 48 F;
+  jfnk               wall time to a 1e-10 drop of the volume-weighted residual
+                     norm for FD Jv + FP64 Krylov and for the tangent Jv with
+                     FP32 matvec, block-Jacobi and basis (both block-Jacobi,
+                     GMRES(30)), their ratio and the max |Cf| difference;
+  cpu_baseline       the CPU restatement of the reference residual (g++ -O2,
+                     one core) on the same mesh and state.
 """
 from __future__ import annotations
 
@@ -29,46 +33,74 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
-CONFIG = 2
+CONFIG = 3
+WORKLOADS = {
+    2: "laminar flat plate M0.2 Re_L=1e5, 400x160 stretched quads 20% split, Roe + viscous, 2nd-order central",
+    3: "turbulent flat plate M0.2 Re=5e6, 1200x1000 stretched quads 10% split, Roe + viscous + SST k-omega, "
+       "2nd-order central",
+}
 
 
-def stage_bytes(N, F, nv, assembly, limiter):
+def stage_bytes(N, F, nv, assembly, limiter, turb):
     """Compulsory bytes of each residual stage's own inputs and outputs."""
-    grad = N * (8 * nv + 16 + 80 + (8 + 16 if limiter else 0)) + F * (8 + 24 + (16 if limiter else 0))
-    flux = N * (8 * nv + 80 + 16 + (8 if limiter else 0)) + F * (12 + 40)
+    ng = 14 if turb else 10
+    grad = N * (8 * nv + 16 + 8 * ng + (8 + 16 if limiter else 0)) + F * (8 + 24 + (16 if limiter else 0))
+    flux = N * (8 * nv + 8 * ng + 16 + (8 if limiter else 0) + (8 if turb else 0)) + F * (12 + 40)
     if assembly == 0:
-        flux += N * 16 * nv  # read-modify-write of R
-        asm = N * 8 * nv      # zero fill
+        flux += N * 16 * nv
+        asm = N * 8 * nv
     else:
-        flux += F * 8 * nv    # face buffer write
+        flux += F * 8 * nv
         asm = F * 8 * nv + N * (16 + 8 * nv)
+    if turb:
+        asm += N * (8 * nv + 8 * ng + 16 + 16)
     return grad, flux, asm
 
 
-def cpu_baseline(mesh, p, U, budget_s=20.0):
+def cpu_baseline(mesh, p, U, budget_s=25.0):
     from oracle import oracle
     o = oracle.OracleSolver(oracle.OracleMesh.from_mesh(mesh), p)
-    o.compute_residuals(U)  # warm-up
     times = []
     t_start = time.perf_counter()
-    while len(times) < 5 and time.perf_counter() - t_start < budget_s:
+    while len(times) < 3 and time.perf_counter() - t_start < budget_s:
         t0 = time.perf_counter()
         o.compute_residuals(U)
         times.append(time.perf_counter() - t0)
     t = float(np.median(times))
     return {"value": mesh.n_cells / t / 1e6, "unit": "Mcell-residuals/s", "cores": 1, "kind": "port",
-            "sample": f"{len(times)} residual evaluations of the same {mesh.n_cells}-cell mesh and state "
-                      f"(median {t * 1e3:.2f} ms), oracle/cfd_oracle.cpp built g++ -O2"}
+            "sample": f"{len(times)} full residual evaluations of the same {mesh.n_cells}-cell mesh and state "
+                      f"(median {t:.3f} s), oracle/cfd_oracle.cpp built g++ -O2"}
+
+
+def jfnk(mesh, p, U0, assembly, max_steps):
+    from mycode_amd import cfd
+    out = {}
+    runs = {"fd_fp64": dict(jv=0, precision=64, precond=1), "tangent_fp32": dict(jv=1, precision=32, precond=1)}
+    Cf = {}
+    for name, kw in runs.items():
+        s = cfd.Solver(mesh, p, assembly=assembly)
+        g = s.gmres_opts(m=30, tol=1e-2, **kw)
+        o = s.newton_opts(gmres=g, tolerance=0.0, rel_drop=1e-10, max_steps=max_steps)
+        U, res, hist = s.solve_implicit(U0, o)
+        Cf[name] = s.wall_outputs(U)[1]
+        out[name] = {"seconds": res.seconds, "newton_steps": res.steps, "converged": bool(res.converged),
+                     "drop": float(hist["vm"][-1] / hist["vm"][0]), "gmres_iters": res.total_gmres_iters}
+        print(f"# jfnk {name}: {out[name]}", file=sys.stderr, flush=True)
+    out["speedup_tangent_fp32_vs_fd_fp64"] = out["fd_fp64"]["seconds"] / out["tangent_fp32"]["seconds"]
+    out["max_abs_Cf_diff"] = float(np.max(np.abs(Cf["fd_fp64"] - Cf["tangent_fp32"])))
+    return out
 
 
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=200)
-    ap.add_argument("--warmup", type=int, default=20)
-    ap.add_argument("--assembly", type=int, default=0)
+    ap.add_argument("--steps", type=int, default=100)
+    ap.add_argument("--warmup", type=int, default=10)
+    ap.add_argument("--assembly", type=int, default=1)
+    ap.add_argument("--config", type=int, default=CONFIG)
+    ap.add_argument("--scale", type=float, default=1.0, help="mesh coarsening (1.0 = the named configuration)")
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--config", type=int, default=CONFIG, help="benchmark configuration to run (1-5)")
+    ap.add_argument("--jfnk-steps", type=int, default=0, help="Newton step cap of the JFNK comparison (0 = skip)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (got --gpus {args.gpus}); the multi-GPU layout is design-only",
@@ -78,7 +110,7 @@ def main():
     import torch
     from mycode_amd import cfd, configs
 
-    mesh, p, U0 = configs.setup(args.config)
+    mesh, p, U0 = configs.setup(args.config, args.scale)
     rng = np.random.default_rng(12345)
     U = U0.copy()
     U[:, :4] *= 1.0 + 1e-3 * rng.standard_normal((U.shape[0], 4))
@@ -100,7 +132,6 @@ def main():
     solver.synchronize()
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
-    # sanity: the device residual equals the host-pointer path
     R6 = torch.zeros(N * 6, dtype=torch.float64, device=dev)
     solver.dev_unpack(Rd.data_ptr(), R6.data_ptr())
     solver.synchronize()
@@ -109,13 +140,16 @@ def main():
     assert np.array_equal(R_dev, R_host), "device and host residual paths disagree"
 
     limiter = p.spatial == cfd.SECOND_ORDER_LIMITED
-    bg, bf, ba = stage_bytes(N, F, nv, args.assembly, limiter)
+    bg, bf, ba = stage_bytes(N, F, nv, args.assembly, limiter, bool(p.turb))
     stages = {"gradients": (bg, ms[0]), "face_flux": (bf, ms[1]), "assembly": (ba, ms[2])}
     dom = max(stages, key=lambda k: stages[k][1])
     b_dom, t_dom = stages[dom]
     achieved = b_dom / (t_dom * 1e-3) / 1e9
     B_res = N * (16 * nv + 24) + 48 * F
     ach_res = B_res / (ms[3] * 1e-3) / 1e9
+    wl = WORKLOADS.get(args.config, f"configuration {args.config}") + ", FP64 residual"
+    if args.scale != 1.0:
+        wl += f" (mesh scaled by {args.scale})"
     out = {
         "metric": "Mcell-residuals/s",
         "value": N * args.steps / wall / 1e6,
@@ -129,10 +163,9 @@ def main():
         "vs_baseline": None,
         "dtype": "f64",
         "data": "synthetic (seeded mesh and state)",
-        "config": {"workload": "laminar flat plate M0.2 Re_L=1e5, 400x160 stretched quads 20% split, "
-                               "Roe + viscous, 2nd-order central, FP64 residual",
-                   "cells": N, "faces": F, "nvar": nv, "assembly": ["colored-scatter", "face-buffer-gather"][
-                       args.assembly], "flux_launches_per_residual": launches, "mesh_hash": hex(mesh.hash())},
+        "config": {"workload": wl, "cells": N, "faces": F, "nvar": nv,
+                   "assembly": ["colored-scatter", "face-buffer-gather"][args.assembly],
+                   "flux_launches_per_residual": launches, "mesh_hash": hex(mesh.hash())},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": None, "kernel": dom,
                      "algorithmic_bytes_per_residual": b_dom, "ms_per_residual": t_dom},
@@ -140,6 +173,8 @@ def main():
                               "frac": ach_res / HBM_PEAK_GBS, "B_res": B_res, "ms_event": ms[3]},
         "stage_ms": {"gradients": ms[0], "face_flux": ms[1], "assembly": ms[2], "residual": ms[3]},
     }
+    if args.jfnk_steps > 0:
+        out["jfnk"] = jfnk(mesh, p, U0, args.assembly, args.jfnk_steps)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(mesh, p, U)
     print(json.dumps(out), flush=True)

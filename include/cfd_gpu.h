@@ -135,6 +135,10 @@ int cfd_mesh_get_cells(const cfd_mesh* m, double* volume, double* x_c, double* y
 int cfd_mesh_get_reorder(const cfd_mesh* m, int* perm, int* face_color);
 /* FNV-1a hash of the mesh arrays, logged by benchmarks. */
 uint64_t cfd_mesh_hash(const cfd_mesh* m);
+/* Move the nodes (n_nodes + 1 coordinates, index 0 unused), keeping the
+ * topology, face order and colors; geometry is recomputed exactly as
+ * Mesh::computeGeometry.  Used for design sensitivities. */
+int cfd_mesh_set_nodes(cfd_mesh* m, const double* x, const double* y);
 /* Write Gmsh v2.2 ASCII with %.17g coordinates. */
 int cfd_mesh_write_gmsh(const cfd_mesh* m, const char* path);
 
@@ -145,6 +149,9 @@ typedef struct cfd_ctx cfd_ctx;
  * uploads the SoA mesh, RCM permutation and colors to the current device. */
 int cfd_ctx_create(const cfd_mesh* m, const cfd_params* p, cfd_ctx** out);
 int cfd_ctx_destroy(cfd_ctx* ctx);
+/* Re-upload the geometry of `m` (same topology as the context's mesh, e.g.
+ * after cfd_mesh_set_nodes); the local time step must be recomputed. */
+int cfd_ctx_update_geometry(cfd_ctx* ctx, const cfd_mesh* m);
 const char* cfd_last_error(const cfd_ctx* ctx);
 int cfd_ctx_set_params(cfd_ctx* ctx, const cfd_params* p);
 int cfd_ctx_get_params(const cfd_ctx* ctx, cfd_params* p);
@@ -201,7 +208,8 @@ typedef struct cfd_gmres_opts {
   int m;            /* Krylov dimension (reference: 40, Solver.cpp:1479)          */
   int restarts;     /* extra restart cycles (reference: 0)                         */
   double tol;       /* relative tolerance on |g_{j+1}| (reference: 1e-2, 1587)     */
-  int jv;           /* 0 finite difference (reference), 1 tangent                  */
+  int jv;           /* 0 finite difference (reference), 1 tangent, 2 transpose:
+                       solve J^T x = RHS with the reverse product (adjoint)      */
   int precision;    /* 64: FP64 basis/matvec; 32: FP32 basis, matvec, precond      */
   int precond;      /* 0 scalar diagonal (reference, 1424-1465), 1 block-Jacobi    */
   int pad;
@@ -221,6 +229,11 @@ typedef struct cfd_newton_opts {
   int log;              /* print one JSON line per step to stdout                  */
   double cfl0;          /* 0.1                                                     */
   cfd_gmres_opts gmres;
+  int controller;       /* 0: the reference CFL controller (Solver.cpp:1747-1796);
+                           1: switched evolution relaxation, CFL = cfl0 (VM_0/VM_k)^1
+                              clamped to [cfl_min, cfl_max] (no reference counterpart) */
+  int pad;
+  double cfl_max;       /* SER upper bound (0 = 1e12)                              */
 } cfd_newton_opts;
 int cfd_newton_opts_default(cfd_newton_opts* o);
 

@@ -59,7 +59,8 @@ class GmresOpts(C.Structure):
 
 class NewtonOpts(C.Structure):
     _fields_ = [("tolerance", C.c_double), ("rel_drop", C.c_double), ("max_steps", C.c_int),
-                ("log", C.c_int), ("cfl0", C.c_double), ("gmres", GmresOpts)]
+                ("log", C.c_int), ("cfl0", C.c_double), ("gmres", GmresOpts), ("controller", C.c_int),
+                ("pad", C.c_int), ("cfl_max", C.c_double)]
 
 
 class NewtonResult(C.Structure):
@@ -90,6 +91,8 @@ SIGNATURES = [
     ("cfd_mesh_get_reorder", C.c_int, [_VP, _IP, _IP]),
     ("cfd_mesh_hash", C.c_uint64, [_VP]),
     ("cfd_mesh_write_gmsh", C.c_int, [_VP, C.c_char_p]),
+    ("cfd_mesh_set_nodes", C.c_int, [_VP, _DP, _DP]),
+    ("cfd_ctx_update_geometry", C.c_int, [_VP, _VP]),
     ("cfd_ctx_create", C.c_int, [_VP, C.POINTER(Params), C.POINTER(_VP)]),
     ("cfd_ctx_destroy", C.c_int, [_VP]),
     ("cfd_last_error", C.c_char_p, [_VP]),
@@ -274,6 +277,14 @@ class Mesh:
     def hash(self) -> int:
         return int(self.lib.cfd_mesh_hash(self._h))
 
+    def set_nodes(self, x, y):
+        """Move the nodes (arrays of n_nodes + 1, index 0 unused); geometry is recomputed."""
+        x = np.ascontiguousarray(x, np.float64)
+        y = np.ascontiguousarray(y, np.float64)
+        if x.shape != (self.n_nodes + 1,) or y.shape != (self.n_nodes + 1,):
+            raise ValueError("node arrays must have n_nodes + 1 entries")
+        _check(self.lib.cfd_mesh_set_nodes(self._h, _dp(x), _dp(y)), what="cfd_mesh_set_nodes")
+
     def write_gmsh(self, path: str):
         _check(self.lib.cfd_mesh_write_gmsh(self._h, path.encode()))
 
@@ -325,6 +336,10 @@ class Solver:
         p = Params()
         self._c(self.lib.cfd_ctx_get_params(self._h, C.byref(p)), "cfd_ctx_get_params")
         return p
+
+    def update_geometry(self, mesh: "Mesh"):
+        """Re-upload geometry after mesh.set_nodes (same topology)."""
+        self._c(self.lib.cfd_ctx_update_geometry(self._h, mesh.handle), "cfd_ctx_update_geometry")
 
     def set_assembly(self, mode: int):
         self._c(self.lib.cfd_set_assembly(self._h, mode), "cfd_set_assembly")

@@ -97,6 +97,7 @@ void free_krylov(cfd_ctx* c) {
   (void)hipFree(k.blk);
   (void)hipFree(k.z);
   (void)hipFree(k.w);
+  (void)hipFree(k.rr);
   (void)hipFree(k.h);
   (void)hipHostFree(k.hpin);
   k = Krylov();
@@ -113,6 +114,7 @@ int ensure_krylov(cfd_ctx* c, int m, int prec) {
   HIP_TRY(c, hipMalloc(&k.blk, (size_t)c->P.nvar * c->P.nvar * c->pitch * es));
   HIP_TRY(c, hipMalloc((void**)&k.z, vs * 8));
   HIP_TRY(c, hipMalloc((void**)&k.w, vs * 8));
+  HIP_TRY(c, hipMalloc((void**)&k.rr, vs * 8));
   HIP_TRY(c, hipMalloc((void**)&k.h, 256 * 8));
   HIP_TRY(c, hipHostMalloc((void**)&k.hpin, 256 * 8, hipHostMallocDefault));
   k.m = m;
@@ -150,6 +152,54 @@ int download_cellscalar(cfd_ctx* c, const double* d, double* host) {
   HIP_TRY(c, hipMemcpyAsync(tmp.data(), d, (size_t)c->N * 8, hipMemcpyDeviceToHost, c->stream));
   HIP_TRY(c, hipStreamSynchronize(c->stream));
   for (int i = 0; i < c->N; ++i) host[c->iperm[i]] = tmp[i];
+  return CFD_OK;
+}
+
+// Geometry of mesh m (reference order) into the context's device arrays (RCM /
+// color order), and the wall distances computed on the device.
+int upload_geometry(cfd_ctx* c, const cfd_mesh* m) {
+  const int N = c->N, F = c->F, P_ = c->pitch;
+  std::vector<double> vol(P_, 1.0), xc(P_, 0.0), yc(P_, 0.0);
+  for (int i = 0; i < N; ++i) {
+    const int o = m->iperm[i];
+    vol[i] = m->vol[o];
+    xc[i] = m->xc[o];
+    yc[i] = m->yc[o];
+  }
+  std::vector<double> nx(F), ny(F), area(F), xm(F), ym(F), dnw(F);
+  std::vector<double> wxy;  // no-slip face midpoints (Solver.cpp:1986-1991)
+  for (int d = 0; d < F; ++d) {
+    const int f = m->face_order[d];
+    nx[d] = m->nx[f];
+    ny[d] = m->ny[f];
+    area[d] = m->area[f];
+    xm[d] = m->xm[f];
+    ym[d] = m->ym[f];
+    const int o = m->own[f];
+    dnw[d] = std::abs((m->xm[f] - m->xc[o]) * m->nx[f] + (m->ym[f] - m->yc[o]) * m->ny[f]);
+  }
+  for (int f = 0; f < F; ++f)
+    if (m->nei[f] < 0 && m->tag[f] == 4) {
+      wxy.push_back(m->xm[f]);
+      wxy.push_back(m->ym[f]);
+    }
+  HIP_TRY(c, hipStreamSynchronize(c->stream));
+  HIP_TRY(c, hipMemcpy(c->d_vol, vol.data(), P_ * 8, hipMemcpyHostToDevice));
+  HIP_TRY(c, hipMemcpy(c->d_xc, xc.data(), P_ * 8, hipMemcpyHostToDevice));
+  HIP_TRY(c, hipMemcpy(c->d_yc, yc.data(), P_ * 8, hipMemcpyHostToDevice));
+  HIP_TRY(c, hipMemcpy(c->d_nx, nx.data(), F * 8, hipMemcpyHostToDevice));
+  HIP_TRY(c, hipMemcpy(c->d_ny, ny.data(), F * 8, hipMemcpyHostToDevice));
+  HIP_TRY(c, hipMemcpy(c->d_area, area.data(), F * 8, hipMemcpyHostToDevice));
+  HIP_TRY(c, hipMemcpy(c->d_xm, xm.data(), F * 8, hipMemcpyHostToDevice));
+  HIP_TRY(c, hipMemcpy(c->d_ym, ym.data(), F * 8, hipMemcpyHostToDevice));
+  HIP_TRY(c, hipMemcpy(c->d_dnw, dnw.data(), F * 8, hipMemcpyHostToDevice));
+  double* dw = nullptr;
+  HIP_TRY(c, hipMalloc((void**)&dw, std::max<size_t>(wxy.size(), 2) * 8));
+  if (!wxy.empty()) HIP_TRY(c, hipMemcpy(dw, wxy.data(), wxy.size() * 8, hipMemcpyHostToDevice));
+  launch_wall_distance(c, dw, (int)wxy.size() / 2, c->d_wd);
+  HIP_TRY(c, hipStreamSynchronize(c->stream));
+  HIP_TRY(c, hipFree(dw));
+  c->dt_valid = false;
   return CFD_OK;
 }
 
@@ -261,39 +311,16 @@ int cfd_ctx_create(const cfd_mesh* m, const cfd_params* p, cfd_ctx** out) {
   c->perm = m->perm;
   c->iperm = m->iperm;
   const int P_ = c->pitch;
-  // cells in RCM order
-  std::vector<double> vol(P_, 1.0), xc(P_, 0.0), yc(P_, 0.0);
-  for (int i = 0; i < N; ++i) {
-    const int o = m->iperm[i];
-    vol[i] = m->vol[o];
-    xc[i] = m->xc[o];
-    yc[i] = m->yc[o];
-  }
-  // faces in device order
+  // per-cell face slots in ascending reference face index
   std::vector<int> fpos(F);  // reference face -> device face
   for (int d = 0; d < F; ++d) fpos[m->face_order[d]] = d;
   std::vector<int> own(F), nei(F), tag(F);
-  std::vector<double> nx(F), ny(F), area(F), xm(F), ym(F), dnw(F);
-  std::vector<double> wxy;  // no-slip face midpoints (Solver.cpp:1986-1991)
   for (int d = 0; d < F; ++d) {
     const int f = m->face_order[d];
     own[d] = m->perm[m->own[f]];
     nei[d] = m->nei[f] >= 0 ? m->perm[m->nei[f]] : -1;
     tag[d] = m->tag[f];
-    nx[d] = m->nx[f];
-    ny[d] = m->ny[f];
-    area[d] = m->area[f];
-    xm[d] = m->xm[f];
-    ym[d] = m->ym[f];
-    const int o = m->own[f];
-    dnw[d] = std::abs((m->xm[f] - m->xc[o]) * m->nx[f] + (m->ym[f] - m->yc[o]) * m->ny[f]);
   }
-  for (int f = 0; f < F; ++f)
-    if (m->nei[f] < 0 && m->tag[f] == 4) {
-      wxy.push_back(m->xm[f]);
-      wxy.push_back(m->ym[f]);
-    }
-  // per-cell face slots in ascending reference face index
   std::vector<int4> slots(P_, make_int4(-1, -1, -1, -1));
   std::vector<int> cnt(N, 0);
   for (int f = 0; f < F; ++f) {  // ascending reference index
@@ -322,23 +349,14 @@ int cfd_ctx_create(const cfd_mesh* m, const cfd_params* p, cfd_ctx** out) {
   HIP_TRY(c, dalloc(&c->d_xm, F));
   HIP_TRY(c, dalloc(&c->d_ym, F));
   HIP_TRY(c, dalloc(&c->d_iperm, N));
+  HIP_TRY(c, dalloc(&c->d_facemap, F));
   HIP_TRY(c, dalloc(&c->d_wd, P_));
   HIP_TRY(c, dalloc(&c->d_dnw, F));
-  HIP_TRY(c, dalloc(&c->d_facemap, F));
-  HIP_TRY(c, hipMemcpy(c->d_vol, vol.data(), P_ * 8, hipMemcpyHostToDevice));
-  HIP_TRY(c, hipMemcpy(c->d_xc, xc.data(), P_ * 8, hipMemcpyHostToDevice));
-  HIP_TRY(c, hipMemcpy(c->d_yc, yc.data(), P_ * 8, hipMemcpyHostToDevice));
   HIP_TRY(c, hipMemcpy(c->d_slots, slots.data(), P_ * sizeof(int4), hipMemcpyHostToDevice));
   HIP_TRY(c, hipMemcpy(c->d_own, own.data(), F * 4, hipMemcpyHostToDevice));
   HIP_TRY(c, hipMemcpy(c->d_nei, nei.data(), F * 4, hipMemcpyHostToDevice));
   HIP_TRY(c, hipMemcpy(c->d_tag, tag.data(), F * 4, hipMemcpyHostToDevice));
-  HIP_TRY(c, hipMemcpy(c->d_nx, nx.data(), F * 8, hipMemcpyHostToDevice));
-  HIP_TRY(c, hipMemcpy(c->d_ny, ny.data(), F * 8, hipMemcpyHostToDevice));
-  HIP_TRY(c, hipMemcpy(c->d_area, area.data(), F * 8, hipMemcpyHostToDevice));
-  HIP_TRY(c, hipMemcpy(c->d_xm, xm.data(), F * 8, hipMemcpyHostToDevice));
-  HIP_TRY(c, hipMemcpy(c->d_ym, ym.data(), F * 8, hipMemcpyHostToDevice));
   HIP_TRY(c, hipMemcpy(c->d_iperm, m->iperm.data(), N * 4, hipMemcpyHostToDevice));
-  HIP_TRY(c, hipMemcpy(c->d_dnw, dnw.data(), F * 8, hipMemcpyHostToDevice));
   HIP_TRY(c, hipMemcpy(c->d_facemap, m->face_order.data(), F * 4, hipMemcpyHostToDevice));
   DMesh& d = c->dm;
   d.N = N;
@@ -359,13 +377,9 @@ int cfd_ctx_create(const cfd_mesh* m, const cfd_params* p, cfd_ctx** out) {
   d.alt = nullptr;
   d.wd = c->d_wd;
   d.dnw = c->d_dnw;
-  {  // wall distances on the device (brute force, as the reference)
-    double* dw = nullptr;
-    HIP_TRY(c, dalloc(&dw, std::max<size_t>(wxy.size(), 2)));
-    if (!wxy.empty()) HIP_TRY(c, hipMemcpy(dw, wxy.data(), wxy.size() * 8, hipMemcpyHostToDevice));
-    launch_wall_distance(c, dw, (int)wxy.size() / 2, c->d_wd);
-    HIP_TRY(c, hipStreamSynchronize(c->stream));
-    HIP_TRY(c, hipFree(dw));
+  {
+    int rc = upload_geometry(c, m);
+    if (rc) return rc;
   }
   // solver buffers
   HIP_TRY(c, dalloc(&c->grad, (size_t)14 * P_));
@@ -385,6 +399,11 @@ int cfd_ctx_create(const cfd_mesh* m, const cfd_params* p, cfd_ctx** out) {
   HIP_TRY(c, hipMemset(c->phi, 0, (size_t)P_ * 8));
   HIP_TRY(c, hipDeviceSynchronize());
   return CFD_OK;
+}
+
+int cfd_ctx_update_geometry(cfd_ctx* c, const cfd_mesh* m) {
+  if (!c || !m || m->ncells() != c->N || m->nfaces() != c->F) return CFD_ERR_ARG;
+  return upload_geometry(c, m);
 }
 
 int cfd_ctx_destroy(cfd_ctx* c) {
@@ -614,30 +633,26 @@ int cfd_norm_vw(cfd_ctx* c, const double* R6, double* out) {
 }
 
 // ---- GMRES (Solver.cpp:1472-1618) on device vectors -------------------------------
-// U, Rb: SoA device vectors; rhs = sign * r (sign -1 passes the residual itself)
-static int gmres_dev(cfd_ctx* c, const double* U, const double* Rb, const double* r, double sign, double* x,
-                     const cfd_gmres_opts* o, int* iters, double* tmpU, double* tmpR) {
+// One GMRES(m) cycle from x = 0 on the right-hand side sign * r, written to x.
+// Returns the number of Arnoldi steps (k + 1) and the cycle's final
+// least-squares residual |g_{k+1}| in *res_out.
+static int gmres_cycle(cfd_ctx* c, const double* U, const double* Rb, const double* r, double sign, double* x,
+                       const cfd_gmres_opts* o, double tol_abs, int* iters, double* res_out, double* tmpU,
+                       double* tmpR) {
   const int m = o->m, prec = o->precision;
-  if (m < 1 || m > 60) return fail(c, CFD_ERR_ARG, "GMRES m must be in [1, 60]");
-  if (prec != 64 && prec != 32) return fail(c, CFD_ERR_ARG, "precision must be 64 or 32");
-  if (o->jv == 0 && prec != 64) return fail(c, CFD_ERR_ARG, "the finite-difference Jv runs in FP64 only");
-  if (o->precond != 0 && o->precond != 1) return fail(c, CFD_ERR_ARG, "unknown preconditioner");
-  if (o->precond == 1 && o->jv == 0 && prec != 64) return fail(c, CFD_ERR_ARG, "FD Jv runs in FP64 only");
-  int rc = ensure_krylov(c, m, prec);
-  if (rc) return rc;
   Krylov& k = c->kry;
   const size_t es = prec == 32 ? 4 : 8;
   const size_t vs = (size_t)c->P.nvar * c->pitch;
   char* V = (char*)k.V;
   const bool bj = o->precond == 1;
-  if (bj) launch_block_precond(c, U, k.blk, prec);
-  else launch_diag_precond(c, U, k.D);
-  launch_reduce(c, RED_L2, U, nullptr, S_UNORM);
+  const bool tr = o->jv == 2;
+  int rc;
   launch_reduce(c, RED_L2, r, nullptr, S_BETA);
   double beta2;
   if ((rc = read_scal(c, S_BETA, 1, &beta2))) return rc;
   const double beta = std::sqrt(beta2);
   launch_fill(c, x, 0.0, vs);
+  *res_out = beta;
   if (beta < 1e-12) {
     *iters = 0;
     return sync(c);
@@ -651,13 +666,15 @@ static int gmres_dev(cfd_ctx* c, const double* U, const double* Rb, const double
     kk = j;
     void* Vj = V + (size_t)j * vs * es;
     void* Vj1 = V + (size_t)(j + 1) * vs * es;
-    if (bj) launch_block_apply(c, k.blk, Vj, k.z, prec);
+    if (bj) launch_block_apply(c, k.blk, Vj, k.z, prec, tr);
     else launch_precond_apply(c, k.z, Vj, prec, o->jv == 0 ? S_VNORM : -1);
     if (o->jv == 0) {
       if (bj) launch_reduce(c, RED_L2, (const double*)k.z, nullptr, S_VNORM);
       launch_jv_fd(c, U, Rb, (const double*)k.z, (double*)Vj1, S_UNORM, S_VNORM, tmpU, tmpR);
-    } else {
+    } else if (o->jv == 1) {
       launch_jv_tangent(c, U, k.z, Vj1, prec);
+    } else {
+      launch_vjp(c, U, (const double*)k.z, (double*)Vj1);
     }
     launch_orth(c, V, j, nullptr, prec, k.h);
     launch_store_basis(c, Vj1, nullptr, k.h, j, prec);
@@ -682,7 +699,7 @@ static int gmres_dev(cfd_ctx* c, const double* U, const double* Rb, const double
     H[j + 1][j] = 0.0;
     g[j + 1] = -sn[j] * g[j];
     g[j] = cs[j] * g[j];
-    if (std::abs(g[j + 1]) < o->tol * beta) break;
+    if (std::abs(g[j + 1]) < tol_abs) break;
   }
   std::vector<double> y(kk + 1);
   for (int i = kk; i >= 0; --i) {
@@ -692,9 +709,69 @@ static int gmres_dev(cfd_ctx* c, const double* U, const double* Rb, const double
   }
   std::memcpy(k.hpin, y.data(), (kk + 1) * 8);
   HIP_TRY(c, hipMemcpyAsync(k.h + 128, k.hpin, (kk + 1) * 8, hipMemcpyHostToDevice, c->stream));
-  if (bj) launch_solution_bj(c, x, V, k.h + 128, kk, k.blk, prec);
+  if (bj) launch_solution_bj(c, x, V, k.h + 128, kk, k.blk, prec, tr);
   else launch_solution(c, x, V, k.h + 128, kk, prec);
   *iters = kk + 1;
+  *res_out = std::abs(g[kk + 1]);
+  return sync(c);
+}
+
+// GMRES(m) with o->restarts extra cycles; rhs = sign * r.  Restart cycles
+// solve for the correction on the true residual rhs - A x.
+static int gmres_dev(cfd_ctx* c, const double* U, const double* Rb, const double* r, double sign, double* x,
+                     const cfd_gmres_opts* o, int* iters, double* tmpU, double* tmpR) {
+  const int m = o->m, prec = o->precision;
+  if (m < 1 || m > 60) return fail(c, CFD_ERR_ARG, "GMRES m must be in [1, 60]");
+  if (prec != 64 && prec != 32) return fail(c, CFD_ERR_ARG, "precision must be 64 or 32");
+  if (o->jv < 0 || o->jv > 2) return fail(c, CFD_ERR_ARG, "jv must be 0 (FD), 1 (tangent) or 2 (transpose)");
+  if (o->jv != 1 && prec != 64) return fail(c, CFD_ERR_ARG, "FD Jv and the transpose run in FP64 only");
+  if (o->precond != 0 && o->precond != 1) return fail(c, CFD_ERR_ARG, "unknown preconditioner");
+  if (o->restarts < 0) return fail(c, CFD_ERR_ARG, "restarts must be >= 0");
+  int rc = ensure_krylov(c, m, prec);
+  if (rc) return rc;
+  Krylov& k = c->kry;
+  const size_t vs = (size_t)c->P.nvar * c->pitch;
+  if (o->precond == 1) launch_block_precond(c, U, k.blk, prec);
+  else launch_diag_precond(c, U, k.D);
+  launch_reduce(c, RED_L2, U, nullptr, S_UNORM);
+  launch_reduce(c, RED_L2, r, nullptr, S_BETA);
+  double b2;
+  if ((rc = read_scal(c, S_BETA, 1, &b2))) return rc;
+  const double tol_abs = o->tol * std::sqrt(b2);
+  int total = 0, it = 0;
+  double res = 0.0;
+  if ((rc = gmres_cycle(c, U, Rb, r, sign, x, o, tol_abs, &it, &res, tmpU, tmpR))) return rc;
+  total += it;
+  if (o->restarts > 0) {
+    double* xs = k.w;  // accumulated solution
+    HIP_TRY(c, hipMemcpyAsync(xs, x, vs * 8, hipMemcpyDeviceToDevice, c->stream));
+    for (int cyc = 0; cyc < o->restarts && res >= tol_abs && it > 0; ++cyc) {
+      // true residual: t = sign * r - A xs  (into tmpR, via the unpreconditioned operator)
+      double* Ax = k.rr;
+      if (o->jv == 0) {
+        launch_reduce(c, RED_L2, xs, nullptr, S_VNORM);
+        launch_jv_fd(c, U, Rb, xs, Ax, S_UNORM, S_VNORM, tmpU, tmpR);
+      } else if (o->jv == 1) {
+        if (prec == 32) {
+          float* xf = (float*)k.z;
+          float* af = (float*)tmpU;
+          launch_d2f(c, xs, xf, vs);
+          launch_jv_tangent(c, U, xf, af, 32);
+          launch_f2d(c, af, Ax, vs);
+        } else {
+          launch_jv_tangent(c, U, xs, Ax, 64);
+        }
+      } else {
+        launch_vjp(c, U, xs, Ax);
+      }
+      launch_axpby(c, Ax, -1.0, r, sign, vs);  // Ax <- sign r - A xs
+      if ((rc = gmres_cycle(c, U, Rb, Ax, 1.0, x, o, tol_abs, &it, &res, tmpU, tmpR))) return rc;
+      total += it;
+      launch_axpby(c, xs, 1.0, x, 1.0, vs);
+    }
+    HIP_TRY(c, hipMemcpyAsync(x, xs, vs * 8, hipMemcpyDeviceToDevice, c->stream));
+  }
+  *iters = total;
   return sync(c);
 }
 
@@ -753,17 +830,29 @@ int cfd_solve_implicit(cfd_ctx* c, double* U6, const cfd_newton_opts* opts, cfd_
     total_it += it;
     if (hist_iters) hist_iters[step] = it;
     if (o.log) std::printf("{\"step\": %d, \"vm\": %.6e, \"cfl\": %.4g, \"gmres\": %d}\n", step, vm, cfl, it);
-    if (it >= 29) {
-      cfl *= 0.5;
-      if (cfl < 0.05) cfl = 0.05;
-    }
-    const double relax = vm > 1e6 ? 0.5 : 1.0;
-    launch_newton_update(c, U, dU, relax);
-    if (step > 5) {
-      cfl *= vm > 1e6 ? 1.2 : 1.5;
-      if (it >= 25) cfl *= 0.5;
-      if (cfl > 10000) cfl = 10000;
-      if (cfl < 0.1) cfl = 0.1;
+    if (o.controller == 0) {  // Solver.cpp:1747-1796
+      if (it >= 29) {
+        cfl *= 0.5;
+        if (cfl < 0.05) cfl = 0.05;
+      }
+      const double relax = vm > 1e6 ? 0.5 : 1.0;
+      launch_newton_update(c, U, dU, relax);
+      if (step > 5) {
+        cfl *= vm > 1e6 ? 1.2 : 1.5;
+        if (it >= 25) cfl *= 0.5;
+        if (cfl > 10000) cfl = 10000;
+        if (cfl < 0.1) cfl = 0.1;
+      }
+    } else {  // switched evolution relaxation
+      launch_newton_update(c, U, dU, 1.0);
+      const double cmax = o.cfl_max > 0 ? o.cfl_max : 1e12;
+      const double c0 = o.cfl0 > 0 ? o.cfl0 : 0.1;
+      double next = c0 * (vm0 / vm);
+      if (next > 2.0 * cfl) next = 2.0 * cfl;   // at most doubling per step
+      if (next < 0.1 * cfl) next = 0.1 * cfl;
+      if (next < c0) next = c0;
+      if (next > cmax) next = cmax;
+      cfl = next;
     }
   }
   if ((rc = download_state(c, U, U6))) return rc;

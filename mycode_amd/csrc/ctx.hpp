@@ -33,7 +33,8 @@ struct Krylov {
   double* D = nullptr;         // scalar preconditioner
   void* blk = nullptr;         // block-Jacobi inverses (nv x nv per cell)
   double* z = nullptr;         // preconditioned direction (FP64)
-  double* w = nullptr;         // Jv output (FP64)
+  double* w = nullptr;         // accumulated solution of restarted GMRES (FP64)
+  double* rr = nullptr;        // true residual of a restart cycle (FP64)
   double* h = nullptr;         // device copy of a Hessenberg column (+ norm)
   double* hpin = nullptr;      // pinned host mirror of h
   size_t bytes = 0;
@@ -118,6 +119,7 @@ void launch_store_basis(cfd_ctx* c, void* Vj1, const double* w, const double* hd
 void launch_solution(cfd_ctx* c, double* x, const void* V, const double* y, int k, int prec);
 // block-Jacobi: assemble + invert (Binv: nv*nv x pitch of the basis type), apply, solution
 void launch_block_precond(cfd_ctx* c, const double* U, void* Binv, int prec, double* Bout = nullptr);
-void launch_block_apply(cfd_ctx* c, const void* Binv, const void* x, void* z, int prec);
-void launch_solution_bj(cfd_ctx* c, double* x, const void* V, const double* y, int k, const void* Binv, int prec);
+void launch_block_apply(cfd_ctx* c, const void* Binv, const void* x, void* z, int prec, bool transpose = false);
+void launch_solution_bj(cfd_ctx* c, double* x, const void* V, const double* y, int k, const void* Binv, int prec,
+                        bool transpose = false);
 }  // namespace cfd

@@ -926,8 +926,8 @@ __global__ __launch_bounds__(BS) void k_block(DMesh m, Phys P, const double* __r
     for (int j = 0; j < NV; ++j) Binv[(size_t)(i * NV + j) * p + c] = (TV)Inv[i][j];
 }
 
-// z = Binv_c x_c per cell (x, z of the basis / matvec type)
-template <int NV, class TV, class TZ>
+// z = Binv_c x_c (or Binv_c^T x_c for the transposed system) per cell
+template <int NV, class TV, class TZ, bool TR>
 __global__ __launch_bounds__(BS) void k_block_apply(int N, int p, const TV* __restrict__ Binv,
                                                      const TV* __restrict__ x, TZ* __restrict__ z) {
   const int c = blockIdx.x * BS + threadIdx.x;
@@ -939,13 +939,13 @@ __global__ __launch_bounds__(BS) void k_block_apply(int N, int p, const TV* __re
   for (int i = 0; i < NV; ++i) {
     double s = 0.0;
 #pragma unroll
-    for (int j = 0; j < NV; ++j) s += (double)Binv[(size_t)(i * NV + j) * p + c] * xv[j];
+    for (int j = 0; j < NV; ++j) s += (double)Binv[(size_t)(TR ? j * NV + i : i * NV + j) * p + c] * xv[j];
     z[i * p + c] = (TZ)s;
   }
 }
 
-// x = Binv (sum_{i<=k} y_i V_i)
-template <int NV, class TV>
+// x = Binv (sum_{i<=k} y_i V_i), or Binv^T for the transposed system
+template <int NV, class TV, bool TR>
 __global__ __launch_bounds__(BS) void k_solution_bj(int N, int p, size_t vstride, const TV* __restrict__ V,
                                                      const double* __restrict__ y, int nvec,
                                                      const TV* __restrict__ Binv, double* __restrict__ x) {
@@ -962,7 +962,7 @@ __global__ __launch_bounds__(BS) void k_solution_bj(int N, int p, size_t vstride
   for (int i = 0; i < NV; ++i) {
     double a = 0.0;
 #pragma unroll
-    for (int j = 0; j < NV; ++j) a += (double)Binv[(size_t)(i * NV + j) * p + c] * s[j];
+    for (int j = 0; j < NV; ++j) a += (double)Binv[(size_t)(TR ? j * NV + i : i * NV + j) * p + c] * s[j];
     x[i * p + c] = a;
   }
 }
@@ -1598,22 +1598,34 @@ void launch_block_precond(cfd_ctx* c, const double* U, void* Binv, int prec, dou
   if (prec == 32) block_t<float>(c, U, (float*)Binv, Bout);
   else block_t<double>(c, U, (double*)Binv, Bout);
 }
-template <class TV> static void bapply_t(cfd_ctx* c, const TV* Binv, const TV* x, TV* z) {
-  NV_DISPATCH(c->P.nvar, hipLaunchKernelGGL((k_block_apply<NVt, TV, TV>), dim3(nblocks(c->N)), dim3(BS), 0,
+template <class TV, bool TR> static void bapply_t(cfd_ctx* c, const TV* Binv, const TV* x, TV* z) {
+  NV_DISPATCH(c->P.nvar, hipLaunchKernelGGL((k_block_apply<NVt, TV, TV, TR>), dim3(nblocks(c->N)), dim3(BS), 0,
                                             c->stream, c->N, c->pitch, Binv, x, z));
 }
-void launch_block_apply(cfd_ctx* c, const void* Binv, const void* x, void* z, int prec) {
-  if (prec == 32) bapply_t<float>(c, (const float*)Binv, (const float*)x, (float*)z);
-  else bapply_t<double>(c, (const double*)Binv, (const double*)x, (double*)z);
+void launch_block_apply(cfd_ctx* c, const void* Binv, const void* x, void* z, int prec, bool transpose) {
+  if (prec == 32) {
+    if (transpose) bapply_t<float, true>(c, (const float*)Binv, (const float*)x, (float*)z);
+    else bapply_t<float, false>(c, (const float*)Binv, (const float*)x, (float*)z);
+  } else {
+    if (transpose) bapply_t<double, true>(c, (const double*)Binv, (const double*)x, (double*)z);
+    else bapply_t<double, false>(c, (const double*)Binv, (const double*)x, (double*)z);
+  }
 }
-template <class TV> static void solbj_t(cfd_ctx* c, double* x, const TV* V, const double* y, int k, const TV* B) {
+template <class TV, bool TR>
+static void solbj_t(cfd_ctx* c, double* x, const TV* V, const double* y, int k, const TV* B) {
   const size_t vs = (size_t)c->P.nvar * c->pitch;
-  NV_DISPATCH(c->P.nvar, hipLaunchKernelGGL((k_solution_bj<NVt, TV>), dim3(nblocks(c->N)), dim3(BS), 0, c->stream,
-                                            c->N, c->pitch, vs, V, y, k + 1, B, x));
+  NV_DISPATCH(c->P.nvar, hipLaunchKernelGGL((k_solution_bj<NVt, TV, TR>), dim3(nblocks(c->N)), dim3(BS), 0,
+                                            c->stream, c->N, c->pitch, vs, V, y, k + 1, B, x));
 }
-void launch_solution_bj(cfd_ctx* c, double* x, const void* V, const double* y, int k, const void* Binv, int prec) {
-  if (prec == 32) solbj_t<float>(c, x, (const float*)V, y, k, (const float*)Binv);
-  else solbj_t<double>(c, x, (const double*)V, y, k, (const double*)Binv);
+void launch_solution_bj(cfd_ctx* c, double* x, const void* V, const double* y, int k, const void* Binv, int prec,
+                        bool transpose) {
+  if (prec == 32) {
+    if (transpose) solbj_t<float, true>(c, x, (const float*)V, y, k, (const float*)Binv);
+    else solbj_t<float, false>(c, x, (const float*)V, y, k, (const float*)Binv);
+  } else {
+    if (transpose) solbj_t<double, true>(c, x, (const double*)V, y, k, (const double*)Binv);
+    else solbj_t<double, false>(c, x, (const double*)V, y, k, (const double*)Binv);
+  }
 }
 
 template <class TV> static void solution_t(cfd_ctx* c, double* x, const TV* V, const double* y, int k) {

@@ -359,6 +359,18 @@ int cfd_mesh_read_gmsh(const char* path, cfd_mesh** out) {
   return CFD_OK;
 }
 
+// New node coordinates, same topology: geometry recomputed exactly as
+// Mesh::computeGeometry; face order, RCM order and colors are kept.
+int cfd_mesh_set_nodes(cfd_mesh* m, const double* x, const double* y) {
+  if (!m || !x || !y) return CFD_ERR_ARG;
+  std::memcpy(m->x.data(), x, m->x.size() * sizeof(double));
+  std::memcpy(m->y.data(), y, m->y.size() * sizeof(double));
+  compute_geometry(*m);
+  for (int c = 0; c < m->ncells(); ++c)
+    if (!(m->vol[c] > 0)) return CFD_ERR_MESH;
+  return CFD_OK;
+}
+
 int cfd_mesh_destroy(cfd_mesh* m) {
   delete m;
   return CFD_OK;

@@ -39,7 +39,7 @@ def test_binding_covers_header():
 def test_struct_sizes():
     assert ctypes.sizeof(cfd.Params) == 192
     assert ctypes.sizeof(cfd.GmresOpts) == 32
-    assert ctypes.sizeof(cfd.NewtonOpts) == 64
+    assert ctypes.sizeof(cfd.NewtonOpts) == 80
     assert ctypes.sizeof(cfd.NewtonResult) == 40
 
 

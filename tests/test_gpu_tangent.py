@@ -122,10 +122,13 @@ def test_block_jacobi_blocks(config, scale):
     nv = p.nvar
     s.compute_residuals(U, True)
     s.compute_local_time_step(U, 3.0)
-    for prec, tol in ((64, 1e-12), (32, 1e-5)):
+    # backward-error bar: |B Binv - I| <= c * eps * cond(B); the blocks mix SI
+    # units (density ~1, energy ~1e5), so cond(B) is large
+    for prec, eps in ((64, 2.2e-16), (32, 1.2e-7)):
         B, Bi = s.block_jacobi(U, prec)
-        err = np.max(np.abs(B @ Bi - np.eye(nv)))
-        assert err < tol, (prec, err)
+        err = np.max(np.abs(B @ Bi - np.eye(nv)), axis=(1, 2))
+        kappa = np.linalg.cond(B)
+        assert np.all(err <= 50 * eps * kappa), (prec, float(np.max(err / (eps * kappa))))
     p1 = p.copy()
     p1.spatial, p1.viscous = cfd.FIRST_ORDER, 0
     s1 = cfd.Solver(s.mesh, p1)
@@ -154,10 +157,25 @@ def test_gmres_block_jacobi(prec):
     U[:, :4] *= 1 + 0.01 * np.random.default_rng(3).standard_normal((U.shape[0], 4))
     s = cfd.Solver(mesh, p)
     R = s.compute_residuals(U)
-    s.compute_local_time_step(U, 50.0)
-    xd, itd = s.solve_gmres(U, R, -R, s.gmres_opts(m=30, tol=1e-3, jv=1, precision=64, precond=0))
-    xb, itb = s.solve_gmres(U, R, -R, s.gmres_opts(m=30, tol=1e-3, jv=1, precision=prec, precond=1))
+    s.compute_local_time_step(U, 2.0)
+    xd, itd = s.solve_gmres(U, R, -R, s.gmres_opts(m=40, tol=1e-3, jv=1, precision=64, precond=0))
+    xb, itb = s.solve_gmres(U, R, -R, s.gmres_opts(m=40, tol=1e-3, jv=1, precision=prec, precond=1))
     print(f"GMRES iterations: diagonal {itd}, block-Jacobi FP{prec} {itb}")
-    assert itb <= itd
+    assert itb <= itd and itb < 40
     Jx = s.jv_tangent(U, xb, 64)
     assert np.linalg.norm(Jx + R) <= 2e-3 * np.linalg.norm(R)
+
+
+@pytest.mark.parametrize("jv,prec", [(0, 64), (1, 64), (1, 32)])
+def test_gmres_restarts_reach_tight_tolerance(jv, prec):
+    mesh, p, U = configs.setup(2, 0.1)
+    U = U.copy()
+    U[:, :4] *= 1 + 0.01 * np.random.default_rng(4).standard_normal((U.shape[0], 4))
+    s = cfd.Solver(mesh, p)
+    R = s.compute_residuals(U)
+    s.compute_local_time_step(U, 5.0)
+    x, it = s.solve_gmres(U, R, -R, s.gmres_opts(m=20, tol=1e-8, jv=jv, precision=prec, precond=1, restarts=30))
+    Jx = s.jv_tangent(U, x, 64)
+    rel = np.linalg.norm(Jx + R) / np.linalg.norm(R)
+    print(f"restarted GMRES jv={jv} FP{prec}: {it} iterations, relative residual {rel:.2e}")
+    assert rel < (1e-6 if jv == 0 else 1e-7 if prec == 64 else 1e-5)
