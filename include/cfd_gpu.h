@@ -146,7 +146,9 @@ int cfd_mesh_write_gmsh(const cfd_mesh* m, const char* path);
 typedef struct cfd_ctx cfd_ctx;
 
 /* Solver::Solver (Solver.cpp:7-10) + setNumericalMethod (Solver.hpp:33-36):
- * uploads the SoA mesh, RCM permutation and colors to the current device. */
+ * uploads the SoA mesh, RCM permutation and colors to the current device.
+ * The mesh is borrowed and must outlive the context, as the reference's
+ * Solver holds a const Mesh& (Solver.hpp:71). */
 int cfd_ctx_create(const cfd_mesh* m, const cfd_params* p, cfd_ctx** out);
 int cfd_ctx_destroy(cfd_ctx* ctx);
 /* Re-upload the geometry of `m` (same topology as the context's mesh, e.g.
@@ -162,8 +164,9 @@ int cfd_ncells(const cfd_ctx* ctx);
 void* cfd_stream(const cfd_ctx* ctx);
 /* Per-face flux override (0 = params.flux, 1 = params.flux_alt), reference face order. */
 int cfd_set_face_flux(cfd_ctx* ctx, const unsigned char* use_alt);
-/* Residual assembly strategy: 0 = edge-colored scatter (default),
- * 1 = face-flux buffer + per-cell gather in reference face order. */
+/* Residual assembly strategy: 0 = edge-colored scatter (faces laid out color
+ * by color), 1 = face-flux buffer + per-cell gather in reference face order
+ * (faces laid out by owner; the default).  Changing it re-lays the faces. */
 int cfd_set_assembly(cfd_ctx* ctx, int mode);
 
 /* ---- host-pointer drop-ins (reference order, AoS stride 6) -------------- */
@@ -212,7 +215,9 @@ typedef struct cfd_gmres_opts {
                        solve J^T x = RHS with the reverse product (adjoint)      */
   int precision;    /* 64: FP64 basis/matvec; 32: FP32 basis, matvec, precond      */
   int precond;      /* 0 scalar diagonal (reference, 1424-1465), 1 block-Jacobi    */
-  int pad;
+  int scaling;      /* 0: unscaled 6-component inner product (reference);
+                       1: rows scaled by the freestream state, so every equation
+                          weighs alike in the Krylov norm (no reference counterpart) */
 } cfd_gmres_opts;
 int cfd_gmres_opts_default(cfd_gmres_opts* o);
 

@@ -54,7 +54,7 @@ class MeshDesc(C.Structure):
 
 class GmresOpts(C.Structure):
     _fields_ = [("m", C.c_int), ("restarts", C.c_int), ("tol", C.c_double), ("jv", C.c_int),
-                ("precision", C.c_int), ("precond", C.c_int), ("pad", C.c_int)]
+                ("precision", C.c_int), ("precond", C.c_int), ("scaling", C.c_int)]
 
 
 class NewtonOpts(C.Structure):
@@ -300,7 +300,7 @@ def _state(a, N):
 class Solver:
     """The reference Solver's residual / Jacobian / Krylov members on the GPU."""
 
-    def __init__(self, mesh: Mesh, params: Params, assembly: int = 0):
+    def __init__(self, mesh: Mesh, params: Params, assembly: int = 1):
         self.lib = load_library()
         self.mesh = mesh
         self.N = mesh.n_cells

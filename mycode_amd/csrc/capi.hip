@@ -98,6 +98,7 @@ void free_krylov(cfd_ctx* c) {
   (void)hipFree(k.z);
   (void)hipFree(k.w);
   (void)hipFree(k.rr);
+  (void)hipFree(k.w2);
   (void)hipFree(k.h);
   (void)hipHostFree(k.hpin);
   k = Krylov();
@@ -115,6 +116,7 @@ int ensure_krylov(cfd_ctx* c, int m, int prec) {
   HIP_TRY(c, hipMalloc((void**)&k.z, vs * 8));
   HIP_TRY(c, hipMalloc((void**)&k.w, vs * 8));
   HIP_TRY(c, hipMalloc((void**)&k.rr, vs * 8));
+  HIP_TRY(c, hipMalloc((void**)&k.w2, vs * 8));
   HIP_TRY(c, hipMalloc((void**)&k.h, 256 * 8));
   HIP_TRY(c, hipHostMalloc((void**)&k.hpin, 256 * 8, hipHostMallocDefault));
   k.m = m;
@@ -169,7 +171,7 @@ int upload_geometry(cfd_ctx* c, const cfd_mesh* m) {
   std::vector<double> nx(F), ny(F), area(F), xm(F), ym(F), dnw(F);
   std::vector<double> wxy;  // no-slip face midpoints (Solver.cpp:1986-1991)
   for (int d = 0; d < F; ++d) {
-    const int f = m->face_order[d];
+    const int f = c->face_order[d];
     nx[d] = m->nx[f];
     ny[d] = m->ny[f];
     area[d] = m->area[f];
@@ -201,6 +203,62 @@ int upload_geometry(cfd_ctx* c, const cfd_mesh* m) {
   HIP_TRY(c, hipFree(dw));
   c->dt_valid = false;
   return CFD_OK;
+}
+
+// Device face order for the context's assembly mode: edge-color-major (then
+// owner position) for the colored scatter, owner position for the face
+// buffer, so that in buffer mode the faces of a cell are processed together
+// and its gathered data is reused from cache.  corder lists the device faces
+// color by color for the kernels that need color ranges in either mode.
+int layout_faces(cfd_ctx* c) {
+  const cfd_mesh* m = c->mesh;
+  const int N = c->N, F = c->F, P_ = c->pitch;
+  if (c->assembly == 0) {
+    c->face_order = m->face_order;
+  } else {
+    c->face_order.resize(F);
+    for (int f = 0; f < F; ++f) c->face_order[f] = f;
+    std::stable_sort(c->face_order.begin(), c->face_order.end(),
+                     [&](int a, int b) { return m->perm[m->own[a]] < m->perm[m->own[b]]; });
+  }
+  std::vector<int> fpos(F);  // reference face -> device face
+  for (int d = 0; d < F; ++d) fpos[c->face_order[d]] = d;
+  std::vector<int> corder(F);
+  for (int i = 0; i < F; ++i) corder[i] = fpos[m->face_order[i]];
+  std::vector<int> own(F), nei(F), tag(F);
+  for (int d = 0; d < F; ++d) {
+    const int f = c->face_order[d];
+    own[d] = m->perm[m->own[f]];
+    nei[d] = m->nei[f] >= 0 ? m->perm[m->nei[f]] : -1;
+    tag[d] = m->tag[f];
+  }
+  // per-cell face slots in ascending reference face index
+  std::vector<int4> slots(P_, make_int4(-1, -1, -1, -1));
+  std::vector<int> cnt(N, 0);
+  for (int f = 0; f < F; ++f) {
+    const int d = fpos[f];
+    const int co = m->perm[m->own[f]];
+    if (cnt[co] >= 4) return fail(c, CFD_ERR_MESH, "cell with more than 4 faces");
+    (&slots[co].x)[cnt[co]++] = d << 1;
+    if (m->nei[f] >= 0) {
+      const int cn = m->perm[m->nei[f]];
+      if (cnt[cn] >= 4) return fail(c, CFD_ERR_MESH, "cell with more than 4 faces");
+      (&slots[cn].x)[cnt[cn]++] = (d << 1) | 1;
+    }
+  }
+  HIP_TRY(c, hipStreamSynchronize(c->stream));
+  HIP_TRY(c, hipMemcpy(c->d_slots, slots.data(), P_ * sizeof(int4), hipMemcpyHostToDevice));
+  HIP_TRY(c, hipMemcpy(c->d_own, own.data(), F * 4, hipMemcpyHostToDevice));
+  HIP_TRY(c, hipMemcpy(c->d_nei, nei.data(), F * 4, hipMemcpyHostToDevice));
+  HIP_TRY(c, hipMemcpy(c->d_tag, tag.data(), F * 4, hipMemcpyHostToDevice));
+  HIP_TRY(c, hipMemcpy(c->d_facemap, c->face_order.data(), F * 4, hipMemcpyHostToDevice));
+  HIP_TRY(c, hipMemcpy(c->d_corder, corder.data(), F * 4, hipMemcpyHostToDevice));
+  if (!c->alt_ref.empty()) {
+    std::vector<unsigned char> al(F);
+    for (int d = 0; d < F; ++d) al[d] = c->alt_ref[c->face_order[d]];
+    HIP_TRY(c, hipMemcpy(c->d_alt, al.data(), F, hipMemcpyHostToDevice));
+  }
+  return upload_geometry(c, m);
 }
 
 }  // namespace
@@ -311,31 +369,7 @@ int cfd_ctx_create(const cfd_mesh* m, const cfd_params* p, cfd_ctx** out) {
   c->perm = m->perm;
   c->iperm = m->iperm;
   const int P_ = c->pitch;
-  // per-cell face slots in ascending reference face index
-  std::vector<int> fpos(F);  // reference face -> device face
-  for (int d = 0; d < F; ++d) fpos[m->face_order[d]] = d;
-  std::vector<int> own(F), nei(F), tag(F);
-  for (int d = 0; d < F; ++d) {
-    const int f = m->face_order[d];
-    own[d] = m->perm[m->own[f]];
-    nei[d] = m->nei[f] >= 0 ? m->perm[m->nei[f]] : -1;
-    tag[d] = m->tag[f];
-  }
-  std::vector<int4> slots(P_, make_int4(-1, -1, -1, -1));
-  std::vector<int> cnt(N, 0);
-  for (int f = 0; f < F; ++f) {  // ascending reference index
-    const int d = fpos[f];
-    const int co = m->perm[m->own[f]];
-    int* so = &slots[co].x;
-    if (cnt[co] >= 4) return fail(c, CFD_ERR_MESH, "cell with more than 4 faces");
-    so[cnt[co]++] = d << 1;
-    if (m->nei[f] >= 0) {
-      const int cn = m->perm[m->nei[f]];
-      int* sn = &slots[cn].x;
-      if (cnt[cn] >= 4) return fail(c, CFD_ERR_MESH, "cell with more than 4 faces");
-      sn[cnt[cn]++] = (d << 1) | 1;
-    }
-  }
+  c->mesh = m;
   HIP_TRY(c, dalloc(&c->d_vol, P_));
   HIP_TRY(c, dalloc(&c->d_xc, P_));
   HIP_TRY(c, dalloc(&c->d_yc, P_));
@@ -350,14 +384,10 @@ int cfd_ctx_create(const cfd_mesh* m, const cfd_params* p, cfd_ctx** out) {
   HIP_TRY(c, dalloc(&c->d_ym, F));
   HIP_TRY(c, dalloc(&c->d_iperm, N));
   HIP_TRY(c, dalloc(&c->d_facemap, F));
+  HIP_TRY(c, dalloc(&c->d_corder, F));
   HIP_TRY(c, dalloc(&c->d_wd, P_));
   HIP_TRY(c, dalloc(&c->d_dnw, F));
-  HIP_TRY(c, hipMemcpy(c->d_slots, slots.data(), P_ * sizeof(int4), hipMemcpyHostToDevice));
-  HIP_TRY(c, hipMemcpy(c->d_own, own.data(), F * 4, hipMemcpyHostToDevice));
-  HIP_TRY(c, hipMemcpy(c->d_nei, nei.data(), F * 4, hipMemcpyHostToDevice));
-  HIP_TRY(c, hipMemcpy(c->d_tag, tag.data(), F * 4, hipMemcpyHostToDevice));
   HIP_TRY(c, hipMemcpy(c->d_iperm, m->iperm.data(), N * 4, hipMemcpyHostToDevice));
-  HIP_TRY(c, hipMemcpy(c->d_facemap, m->face_order.data(), F * 4, hipMemcpyHostToDevice));
   DMesh& d = c->dm;
   d.N = N;
   d.F = F;
@@ -377,8 +407,10 @@ int cfd_ctx_create(const cfd_mesh* m, const cfd_params* p, cfd_ctx** out) {
   d.alt = nullptr;
   d.wd = c->d_wd;
   d.dnw = c->d_dnw;
+  d.corder = c->d_corder;
+  c->assembly = 1;
   {
-    int rc = upload_geometry(c, m);
+    int rc = layout_faces(c);
     if (rc) return rc;
   }
   // solver buffers
@@ -390,6 +422,7 @@ int cfd_ctx_create(const cfd_mesh* m, const cfd_params* p, cfd_ctx** out) {
   HIP_TRY(c, dalloc(&c->scal, 64));
   HIP_TRY(c, dalloc(&c->gradT, (size_t)28 * P_));
   HIP_TRY(c, dalloc(&c->adj, (size_t)14 * P_));
+  HIP_TRY(c, dalloc(&c->tq, (size_t)8 * P_));
   HIP_TRY(c, hipHostMalloc((void**)&c->scal_host, 64 * 8, hipHostMallocDefault));
   HIP_TRY(c, dalloc(&c->io6, (size_t)6 * N));
   for (int i = 0; i < 6; ++i) {
@@ -403,6 +436,7 @@ int cfd_ctx_create(const cfd_mesh* m, const cfd_params* p, cfd_ctx** out) {
 
 int cfd_ctx_update_geometry(cfd_ctx* c, const cfd_mesh* m) {
   if (!c || !m || m->ncells() != c->N || m->nfaces() != c->F) return CFD_ERR_ARG;
+  if (m != c->mesh) return fail(c, CFD_ERR_ARG, "geometry must come from the context's own mesh");
   return upload_geometry(c, m);
 }
 
@@ -411,8 +445,8 @@ int cfd_ctx_destroy(cfd_ctx* c) {
   (void)hipStreamSynchronize(c->stream);
   free_krylov(c);
   void* bufs[] = {c->d_vol, c->d_xc,  c->d_yc,  c->d_slots, c->d_own, c->d_nei, c->d_tag,  c->d_nx,
-                  c->d_ny,  c->d_area, c->d_xm, c->d_ym,    c->d_iperm, c->d_facemap, c->d_alt, c->grad, c->d_wd, c->d_dnw,
-                  c->phi,   c->dt,    c->Fm,    c->red,     c->scal, c->gradT, c->adj, c->io6};
+                  c->d_ny,  c->d_area, c->d_xm, c->d_ym,    c->d_iperm, c->d_facemap, c->d_alt, c->grad, c->d_wd, c->d_dnw, c->d_corder,
+                  c->phi,   c->dt,    c->Fm,    c->red,     c->scal, c->gradT, c->adj, c->tq, c->io6};
   for (void* b : bufs) (void)hipFree(b);
   for (int i = 0; i < 6; ++i) (void)hipFree(c->vec[i]);
   (void)hipHostFree(c->scal_host);
@@ -444,10 +478,13 @@ int cfd_set_face_flux(cfd_ctx* c, const unsigned char* use_alt) {
   if (!c) return CFD_ERR_ARG;
   if (!use_alt) {
     c->dm.alt = nullptr;
+    c->alt_ref.clear();
     return CFD_OK;
   }
+  c->alt_ref.assign(use_alt, use_alt + c->F);
+  for (auto& v : c->alt_ref) v = v ? 1 : 0;
   std::vector<unsigned char> a(c->F);
-  for (int d = 0; d < c->F; ++d) a[d] = use_alt[c->face_order[d]] ? 1 : 0;
+  for (int d = 0; d < c->F; ++d) a[d] = c->alt_ref[c->face_order[d]];
   if (!c->d_alt) HIP_TRY(c, dalloc(&c->d_alt, c->F));
   HIP_TRY(c, hipMemcpy(c->d_alt, a.data(), c->F, hipMemcpyHostToDevice));
   c->dm.alt = c->d_alt;
@@ -456,8 +493,9 @@ int cfd_set_face_flux(cfd_ctx* c, const unsigned char* use_alt) {
 
 int cfd_set_assembly(cfd_ctx* c, int mode) {
   if (!c || (mode != 0 && mode != 1)) return CFD_ERR_ARG;
+  if (mode == c->assembly) return CFD_OK;
   c->assembly = mode;
-  return CFD_OK;
+  return layout_faces(c);
 }
 
 // ---- host-pointer drop-ins -----------------------------------------------------
@@ -632,6 +670,24 @@ int cfd_norm_vw(cfd_ctx* c, const double* R6, double* out) {
   return CFD_OK;
 }
 
+// Row scale factors 1 / |reference magnitude| of each conserved equation,
+// from the frozen freestream (momentum uses rho (|u| + a)).
+static void row_scales(const cfd_ctx* c, double* s) {
+  const cfd_params& p = c->P;
+  const double g = p.gamma;
+  const double fT = 1.0 + 0.5 * (g - 1.0) * p.M_inf * p.M_inf;
+  const double Ts = p.T_total_inlet / fT;
+  const double rho = p.P_static_exit / (p.R_gas * Ts);
+  const double a = std::sqrt(g * p.R_gas * Ts);
+  const double u = p.M_inf * a;
+  const double E = p.P_static_exit / (g - 1.0) + 0.5 * rho * u * u;
+  s[0] = 1.0 / rho;
+  s[1] = s[2] = 1.0 / (rho * (u + a));
+  s[3] = 1.0 / E;
+  s[4] = p.turb ? 1.0 / (rho * p.turb_k_inf) : 1.0;
+  s[5] = p.turb ? 1.0 / (rho * p.turb_w_inf) : 1.0;
+}
+
 // ---- GMRES (Solver.cpp:1472-1618) on device vectors -------------------------------
 // One GMRES(m) cycle from x = 0 on the right-hand side sign * r, written to x.
 // Returns the number of Arnoldi steps (k + 1) and the cycle's final
@@ -646,7 +702,14 @@ static int gmres_cycle(cfd_ctx* c, const double* U, const double* Rb, const doub
   char* V = (char*)k.V;
   const bool bj = o->precond == 1;
   const bool tr = o->jv == 2;
+  double sc[6];
+  row_scales(c, sc);
   int rc;
+  if (o->scaling) {  // scaled copy of the right-hand side
+    HIP_TRY(c, hipMemcpyAsync(k.w2, r, vs * 8, hipMemcpyDeviceToDevice, c->stream));
+    launch_rowscale(c, k.w2, sc, 64);
+    r = k.w2;
+  }
   launch_reduce(c, RED_L2, r, nullptr, S_BETA);
   double beta2;
   if ((rc = read_scal(c, S_BETA, 1, &beta2))) return rc;
@@ -676,6 +739,7 @@ static int gmres_cycle(cfd_ctx* c, const double* U, const double* Rb, const doub
     } else {
       launch_vjp(c, U, (const double*)k.z, (double*)Vj1);
     }
+    if (o->scaling) launch_rowscale(c, Vj1, sc, prec);
     launch_orth(c, V, j, nullptr, prec, k.h);
     launch_store_basis(c, Vj1, nullptr, k.h, j, prec);
     HIP_TRY(c, hipMemcpyAsync(k.hpin, k.h, (size_t)(j + 2) * 8, hipMemcpyDeviceToHost, c->stream));
@@ -734,8 +798,16 @@ static int gmres_dev(cfd_ctx* c, const double* U, const double* Rb, const double
   if (o->precond == 1) launch_block_precond(c, U, k.blk, prec);
   else launch_diag_precond(c, U, k.D);
   launch_reduce(c, RED_L2, U, nullptr, S_UNORM);
-  launch_reduce(c, RED_L2, r, nullptr, S_BETA);
   double b2;
+  if (o->scaling) {
+    double sc[6];
+    row_scales(c, sc);
+    HIP_TRY(c, hipMemcpyAsync(k.w2, r, vs * 8, hipMemcpyDeviceToDevice, c->stream));
+    launch_rowscale(c, k.w2, sc, 64);
+    launch_reduce(c, RED_L2, k.w2, nullptr, S_BETA);
+  } else {
+    launch_reduce(c, RED_L2, r, nullptr, S_BETA);
+  }
   if ((rc = read_scal(c, S_BETA, 1, &b2))) return rc;
   const double tol_abs = o->tol * std::sqrt(b2);
   int total = 0, it = 0;
@@ -772,6 +844,7 @@ static int gmres_dev(cfd_ctx* c, const double* U, const double* Rb, const double
     HIP_TRY(c, hipMemcpyAsync(x, xs, vs * 8, hipMemcpyDeviceToDevice, c->stream));
   }
   *iters = total;
+  c->kry.last_rel = tol_abs > 0 ? res / (tol_abs / o->tol) : 0.0;
   return sync(c);
 }
 
@@ -843,14 +916,15 @@ int cfd_solve_implicit(cfd_ctx* c, double* U6, const cfd_newton_opts* opts, cfd_
         if (cfl > 10000) cfl = 10000;
         if (cfl < 0.1) cfl = 0.1;
       }
-    } else {  // switched evolution relaxation
+    } else {  // switched evolution relaxation, backed off when GMRES misses its tolerance
       launch_newton_update(c, U, dU, 1.0);
       const double cmax = o.cfl_max > 0 ? o.cfl_max : 1e12;
       const double c0 = o.cfl0 > 0 ? o.cfl0 : 0.1;
       double next = c0 * (vm0 / vm);
       if (next > 2.0 * cfl) next = 2.0 * cfl;   // at most doubling per step
       if (next < 0.1 * cfl) next = 0.1 * cfl;
-      if (next < c0) next = c0;
+      if (c->kry.last_rel > 1.5 * o.gmres.tol) next = 0.5 * cfl;
+      if (next < 1e-3 * c0) next = 1e-3 * c0;
       if (next > cmax) next = cmax;
       cfl = next;
     }

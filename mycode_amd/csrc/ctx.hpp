@@ -24,6 +24,7 @@ struct DMesh {
   const unsigned char* alt;  // per device face: 1 = use Phys::flux_alt (may be null)
   const double* wd;          // per cell: distance to the nearest no-slip face midpoint
   const double* dnw;         // per face: normal distance of the owner centroid to the face
+  const int* corder;         // device faces listed color by color (ranges: cfd_ctx::color_ptr)
 };
 
 // Workspace of one GMRES solve.
@@ -35,6 +36,8 @@ struct Krylov {
   double* z = nullptr;         // preconditioned direction (FP64)
   double* w = nullptr;         // accumulated solution of restarted GMRES (FP64)
   double* rr = nullptr;        // true residual of a restart cycle (FP64)
+  double* w2 = nullptr;        // row-scaled right-hand side (FP64)
+  double last_rel = 0.0;       // relative residual reached by the last solve
   double* h = nullptr;         // device copy of a Hessenberg column (+ norm)
   double* hpin = nullptr;      // pinned host mirror of h
   size_t bytes = 0;
@@ -51,11 +54,14 @@ struct cfd_ctx {
   std::vector<int> color_ptr;
   std::vector<int> face_order, perm, iperm;  // host copies
   cfd::DMesh dm{};
-  int assembly = 0;  // 0 colored scatter, 1 face buffer + gather
+  int assembly = 1;  // 0 colored scatter, 1 face buffer + gather
+  const cfd_mesh* mesh = nullptr;   // borrowed: must outlive the context (Solver holds const Mesh&)
+  std::vector<unsigned char> alt_ref;  // per reference face flux choice
   // device mesh storage
   double *d_vol = nullptr, *d_xc = nullptr, *d_yc = nullptr;
   int4* d_slots = nullptr;
   int *d_own = nullptr, *d_nei = nullptr, *d_tag = nullptr, *d_iperm = nullptr, *d_facemap = nullptr;
+  int* d_corder = nullptr;
   double *d_nx = nullptr, *d_ny = nullptr, *d_area = nullptr, *d_xm = nullptr, *d_ym = nullptr;
   double *d_wd = nullptr, *d_dnw = nullptr;
   unsigned char* d_alt = nullptr;
@@ -69,6 +75,7 @@ struct cfd_ctx {
   double* scal_host = nullptr;  // pinned
   double* gradT = nullptr;  // tangent gradients (2 x 14 x pitch: values, derivatives)
   double* adj = nullptr;    // adjoint of the gradients (14 x pitch)
+  double* tq = nullptr;     // SST per cell: mu_t, F1, V S_k, V S_w (values; + derivatives in the tangent)
   bool phi_valid = false, dt_valid = false;
   // host-pointer staging: AoS device copy + SoA vectors
   double* io6 = nullptr;     // 6 x N (AoS)
@@ -102,6 +109,8 @@ void launch_jv_tangent(cfd_ctx* c, const double* U, const void* v, void* Jv, int
 void launch_vjp(cfd_ctx* c, const double* U, const double* w, double* JTw);
 // vector kernels
 void launch_axpby(cfd_ctx* c, double* y, double a, const double* x, double b, size_t n);
+// w_k *= s_k for the nv rows of an SoA vector of the basis type
+void launch_rowscale(cfd_ctx* c, void* w, const double* s, int prec);
 void launch_scale_by_scal(cfd_ctx* c, double* y, const double* x, int slot, int mode);
 void launch_newton_update(cfd_ctx* c, double* U, const double* dU, double relax);
 void launch_explicit(cfd_ctx* c, double* Unew, const double* U0, double a0, const double* U1, double a1,

@@ -76,12 +76,15 @@ __device__ __forceinline__ S6<T> ldT(const double* __restrict__ U, const typenam
 // Gv / Gd: 10 x pitch values (and, for the tangent, derivatives) of the
 // Green-Gauss gradients in CellGradient order.  The limiter is only formed in
 // the FP64 residual; the tangent keeps it frozen.
+// With SST the cell's eddy viscosity, blending F1 and V * (k, omega sources)
+// are formed here once per cell (rows 0..3 of Qv / Qd) for the face fluxes.
 template <int NV, class T>
 __global__ __launch_bounds__(BS) void k_gradlim(DMesh m, Phys P, const double* __restrict__ U,
                                                  const typename Scal<T>::F* __restrict__ v,
                                                  typename Scal<T>::F* __restrict__ Gv,
                                                  typename Scal<T>::F* __restrict__ Gd, double* __restrict__ phi,
-                                                 int do_lim) {
+                                                 int do_lim, typename Scal<T>::F* __restrict__ Qv = nullptr,
+                                                 typename Scal<T>::F* __restrict__ Qd = nullptr) {
   typedef Scal<T> Sc;
   constexpr int NQM = NV == 6 ? 7 : 5;  // gradient variables: rho u v p T (+ k omega with SST)
   const int c = xcd_block() * BS + threadIdx.x;
@@ -146,6 +149,15 @@ __global__ __launch_bounds__(BS) void k_gradlim(DMesh m, Phys P, const double* _
     g[q] = g[q] / vol;
     Sc::put(Gv, Gd, (size_t)q * p + c, g[q]);
   }
+  if constexpr (NQM == 7) {
+    if (turb && Qv) {
+      const TurbQ<T> tq = turb_cell(P, Uc, g, m.wd[c]);
+      Sc::put(Qv, Qd, (size_t)c, tq.mut);
+      Sc::put(Qv, Qd, (size_t)p + c, tq.F1);
+      Sc::put(Qv, Qd, (size_t)2 * p + c, vol * tq.Sk);
+      Sc::put(Qv, Qd, (size_t)3 * p + c, vol * tq.Sw);
+    }
+  }
   if constexpr (std::is_same<T, double>::value) {
     if (!do_lim) return;
     const double xc = m.xc[c], yc = m.yc[c];
@@ -168,25 +180,13 @@ __global__ __launch_bounds__(BS) void k_gradlim(DMesh m, Phys P, const double* _
   }
 }
 
-// SST sources: R_c(k, omega) -= V_c S(U_c, G_c, d_c)  (the tangent subtracts dS)
-template <int NV, class T>
-__global__ __launch_bounds__(BS) void k_turb_src(DMesh m, Phys P, const double* __restrict__ U,
-                                                  const typename Scal<T>::F* __restrict__ v,
-                                                  const typename Scal<T>::F* __restrict__ Gv,
-                                                  const typename Scal<T>::F* __restrict__ Gd,
-                                                  typename Scal<T>::F* __restrict__ out) {
-  typedef Scal<T> Sc;
-  const int c = xcd_block() * BS + threadIdx.x;
-  if (c >= m.N) return;
-  const int p = m.pitch;
-  const S6<T> Uc = ldT<NV, T>(U, v, p, c);
-  T g[14];
-#pragma unroll
-  for (int q = 0; q < 14; ++q) g[q] = Sc::get(Gv, Gd, (size_t)q * p + c);
-  const TurbQ<T> tq = turb_cell(P, Uc, g, m.wd[c]);
-  const double V = m.vol[c];
-  out[4 * p + c] = out[4 * p + c] - Sc::dpart(V * tq.Sk);
-  out[5 * p + c] = out[5 * p + c] - Sc::dpart(V * tq.Sw);
+// SST sources: R_c(k, omega) -= V_c S_c, with V_c S_c formed by k_gradlim
+template <class F>
+__global__ __launch_bounds__(BS) void k_turb_src(int N, int p, const F* __restrict__ Q, F* __restrict__ out) {
+  const int c = blockIdx.x * BS + threadIdx.x;
+  if (c >= N) return;
+  out[4 * p + c] = out[4 * p + c] - Q[(size_t)2 * p + c];
+  out[5 * p + c] = out[5 * p + c] - Q[(size_t)3 * p + c];
 }
 
 // ---- face flux ------------------------------------------------------------------------
@@ -238,7 +238,8 @@ __device__ __forceinline__ FaceGeo face_geo(const DMesh& m, const Phys& P, const
 // flux, viscous flux with the centroid-line correction.
 template <int VISC, class T>
 __device__ __forceinline__ S6<T> flux_core(const Phys& P, const FaceGeo& g, const S6<T>& L, const S6<T>& R,
-                                           const T* gL, const T* gR) {
+                                           const T* gL, const T* gR, const TurbQ<T>* tqo = nullptr,
+                                           const TurbQ<T>* tqn = nullptr) {
   const double nx = g.nx, ny = g.ny;
   S6<T> Lr = L, Rr = R;
   if (P.spatial != CFD_FIRST_ORDER && !g.boundary) {
@@ -330,9 +331,9 @@ __device__ __forceinline__ S6<T> flux_core(const Phys& P, const FaceGeo& g, cons
     T kc = mu * P.Cp / P.Pr;
     T mut = T(0.0), F1f = T(0.0);
     if (P.turb) {  // eddy viscosity in stress and heat flux
-      const TurbQ<T> qo = turb_cell(P, L, gL, g.wdo);
+      const TurbQ<T> qo = tqo ? *tqo : turb_cell(P, L, gL, g.wdo);
       if (!g.boundary) {
-        const TurbQ<T> qn = turb_cell(P, R, gR, g.wdn);
+        const TurbQ<T> qn = tqn ? *tqn : turb_cell(P, R, gR, g.wdn);
         mut = 0.5 * (qo.mut + qn.mut);
         F1f = 0.5 * (qo.F1 + qn.F1);
       } else {
@@ -399,7 +400,9 @@ __device__ __forceinline__ S6<T> face_flux(const DMesh& m, const Phys& P, const 
                                            const typename Scal<T>::F* __restrict__ v,
                                            const typename Scal<T>::F* __restrict__ Gv,
                                            const typename Scal<T>::F* __restrict__ Gd,
-                                           const double* __restrict__ phi, int f, int& o, int& n) {
+                                           const double* __restrict__ phi, int f, int& o, int& n,
+                                           const typename Scal<T>::F* __restrict__ Qv,
+                                           const typename Scal<T>::F* __restrict__ Qd) {
   typedef Scal<T> Sc;
   const int p = m.pitch;
   o = m.own[f];
@@ -422,6 +425,18 @@ __device__ __forceinline__ S6<T> face_flux(const DMesh& m, const Phys& P, const 
       for (int q = 0; q < NG; ++q) gR[q] = gL[q];
     }
   }
+  if (NV == 6 && VISC && P.turb) {
+    TurbQ<T> qo, qn;
+    qo.mut = Sc::get(Qv, Qd, (size_t)o);
+    qo.F1 = Sc::get(Qv, Qd, (size_t)p + o);
+    if (n >= 0) {
+      qn.mut = Sc::get(Qv, Qd, (size_t)n);
+      qn.F1 = Sc::get(Qv, Qd, (size_t)p + n);
+    } else {
+      qn = qo;
+    }
+    return flux_core<VISC, T>(P, g, L, R, gL, gR, &qo, &qn);
+  }
   return flux_core<VISC, T>(P, g, L, R, gL, gR);
 }
 
@@ -433,12 +448,14 @@ __global__ __launch_bounds__(BS) void k_flux(DMesh m, Phys P, const double* __re
                                               const typename Scal<T>::F* __restrict__ Gv,
                                               const typename Scal<T>::F* __restrict__ Gd,
                                               const double* __restrict__ phi, typename Scal<T>::F* __restrict__ out,
-                                              int f0, int f1) {
+                                              int f0, int f1, const typename Scal<T>::F* __restrict__ Qv,
+                                              const typename Scal<T>::F* __restrict__ Qd) {
   typedef Scal<T> Sc;
-  const int f = f0 + xcd_block() * BS + threadIdx.x;
-  if (f >= f1) return;
+  const int i = f0 + xcd_block() * BS + threadIdx.x;
+  if (i >= f1) return;
+  const int f = MODE == 0 ? m.corder[i] : i;
   int o, n;
-  const S6<T> TA = face_flux<NV, VISC, T>(m, P, U, v, Gv, Gd, phi, f, o, n);
+  const S6<T> TA = face_flux<NV, VISC, T>(m, P, U, v, Gv, Gd, phi, f, o, n, Qv, Qd);
   if (MODE == 0) {
     const int p = m.pitch;
 #pragma unroll
@@ -508,8 +525,9 @@ __global__ __launch_bounds__(BS) void k_vjp_face(DMesh m, Phys P, const double* 
                                                   double* __restrict__ aG, int f0, int f1) {
   typedef DualV<double, VK> T;
   constexpr int NGM = NV == 6 ? 14 : 10;
-  const int f = f0 + xcd_block() * BS + threadIdx.x;
-  if (f >= f1) return;
+  const int i = f0 + xcd_block() * BS + threadIdx.x;
+  if (i >= f1) return;
+  const int f = m.corder[i];
   const int p = m.pitch;
   const int o = m.own[f], n = m.nei[f];
   const bool interior = n >= 0;
@@ -637,8 +655,9 @@ __global__ __launch_bounds__(BS) void k_vjp_gg(DMesh m, Phys P, const double* __
                                                 int f1) {
   typedef DualV<double, NV> T;
   constexpr int NQM = NV == 6 ? 7 : 5;
-  const int f = f0 + xcd_block() * BS + threadIdx.x;
-  if (f >= f1) return;
+  const int i = f0 + xcd_block() * BS + threadIdx.x;
+  if (i >= f1) return;
+  const int f = m.corder[i];
   const int p = m.pitch;
   const int o = m.own[f], n = m.nei[f];
   const bool turb = NV == 6 && P.turb;
@@ -1208,6 +1227,18 @@ __global__ __launch_bounds__(BS) void k_precond(int N, int p, TZ* __restrict__ z
   for (int k = 0; k < NV; ++k) z[k * p + c] = (TZ)((double)Vj[k * p + c] * id);
 }
 
+// row scaling of a vector: w_k *= s_k (six factors passed by value)
+struct Scale6 {
+  double s[6];
+};
+template <int NV, class TV>
+__global__ __launch_bounds__(BS) void k_rowscale(int N, int p, TV* __restrict__ w, Scale6 s) {
+  const int c = blockIdx.x * BS + threadIdx.x;
+  if (c >= N) return;
+#pragma unroll
+  for (int k = 0; k < NV; ++k) w[k * p + c] = (TV)((double)w[k * p + c] * s.s[k]);
+}
+
 // partial dots of w against up to 8 basis vectors (chunk = blockIdx.y)
 template <int NV, class TV>
 __global__ __launch_bounds__(BS) void k_mdot(int N, int p, size_t vstride, const TV* __restrict__ V, int nvec,
@@ -1312,7 +1343,7 @@ static void residual_nv(cfd_ctx* c, const double* U, double* R, bool upd, hipEve
   const int do_lim = lim && (upd || !c->phi_valid);
   if (ev) (void)hipEventRecord(ev[0], c->stream);
   hipLaunchKernelGGL((k_gradlim<NV, double>), dim3(nblocks(m.N)), dim3(BS), 0, c->stream, m, P, U,
-                     (const double*)nullptr, c->grad, (double*)nullptr, c->phi, do_lim);
+                     (const double*)nullptr, c->grad, (double*)nullptr, c->phi, do_lim, c->tq, (double*)nullptr);
   if (do_lim) c->phi_valid = true;
   if (ev) (void)hipEventRecord(ev[1], c->stream);
   if (c->assembly == 0) {
@@ -1322,19 +1353,21 @@ static void residual_nv(cfd_ctx* c, const double* U, double* R, bool upd, hipEve
       const int f0 = c->color_ptr[col], f1 = c->color_ptr[col + 1];
       if (f1 > f0)
         hipLaunchKernelGGL((k_flux<NV, VISC, 0, double>), dim3(nblocks(f1 - f0)), dim3(BS), 0, c->stream, m, P, U,
-                           (const double*)nullptr, c->grad, (const double*)nullptr, c->phi, R, f0, f1);
+                           (const double*)nullptr, c->grad, (const double*)nullptr, c->phi, R, f0, f1,
+                           (const double*)c->tq, (const double*)nullptr);
     }
     if (ev) (void)hipEventRecord(ev[3], c->stream);
   } else {
     if (ev) (void)hipEventRecord(ev[2], c->stream);
     hipLaunchKernelGGL((k_flux<NV, VISC, 1, double>), dim3(nblocks(m.F)), dim3(BS), 0, c->stream, m, P, U,
-                       (const double*)nullptr, c->grad, (const double*)nullptr, c->phi, c->Fm, 0, m.F);
+                       (const double*)nullptr, c->grad, (const double*)nullptr, c->phi, c->Fm, 0, m.F,
+                       (const double*)c->tq, (const double*)nullptr);
     if (ev) (void)hipEventRecord(ev[3], c->stream);
     hipLaunchKernelGGL((k_gather<NV, double>), dim3(nblocks(m.N)), dim3(BS), 0, c->stream, m, c->Fm, R);
   }
   if (NV == 6 && P.turb)
-    hipLaunchKernelGGL((k_turb_src<NV, double>), dim3(nblocks(m.N)), dim3(BS), 0, c->stream, m, P, U,
-                       (const double*)nullptr, (const double*)c->grad, (const double*)nullptr, R);
+    hipLaunchKernelGGL((k_turb_src<double>), dim3(nblocks(m.N)), dim3(BS), 0, c->stream, m.N, m.pitch,
+                       (const double*)c->tq, R);
   if (ev) (void)hipEventRecord(ev[4], c->stream);
 }
 
@@ -1346,7 +1379,10 @@ static void tangent_nv(cfd_ctx* c, const double* U, const F* v, F* Jv) {
   const Phys& P = c->phys;
   F* Gv = (F*)c->gradT;
   F* Gd = Gv + (size_t)14 * m.pitch;
-  hipLaunchKernelGGL((k_gradlim<NV, T>), dim3(nblocks(m.N)), dim3(BS), 0, c->stream, m, P, U, v, Gv, Gd, c->phi, 0);
+  F* Qv = (F*)c->tq;
+  F* Qd = Qv + (size_t)4 * m.pitch;
+  hipLaunchKernelGGL((k_gradlim<NV, T>), dim3(nblocks(m.N)), dim3(BS), 0, c->stream, m, P, U, v, Gv, Gd, c->phi, 0,
+                     Qv, Qd);
   if (c->assembly == 0) {
     hipLaunchKernelGGL(k_fill<F>, dim3(nblocks((size_t)NV * m.pitch)), dim3(BS), 0, c->stream, Jv, F(0),
                        (size_t)NV * m.pitch);
@@ -1354,17 +1390,16 @@ static void tangent_nv(cfd_ctx* c, const double* U, const F* v, F* Jv) {
       const int f0 = c->color_ptr[col], f1 = c->color_ptr[col + 1];
       if (f1 > f0)
         hipLaunchKernelGGL((k_flux<NV, VISC, 0, T>), dim3(nblocks(f1 - f0)), dim3(BS), 0, c->stream, m, P, U, v,
-                           (const F*)Gv, (const F*)Gd, c->phi, Jv, f0, f1);
+                           (const F*)Gv, (const F*)Gd, c->phi, Jv, f0, f1, (const F*)Qv, (const F*)Qd);
     }
   } else {
     F* Fm = (F*)c->Fm;
     hipLaunchKernelGGL((k_flux<NV, VISC, 1, T>), dim3(nblocks(m.F)), dim3(BS), 0, c->stream, m, P, U, v,
-                       (const F*)Gv, (const F*)Gd, c->phi, Fm, 0, m.F);
+                       (const F*)Gv, (const F*)Gd, c->phi, Fm, 0, m.F, (const F*)Qv, (const F*)Qd);
     hipLaunchKernelGGL((k_gather<NV, F>), dim3(nblocks(m.N)), dim3(BS), 0, c->stream, m, (const F*)Fm, Jv);
   }
   if (NV == 6 && P.turb)
-    hipLaunchKernelGGL((k_turb_src<NV, T>), dim3(nblocks(m.N)), dim3(BS), 0, c->stream, m, P, U, v, (const F*)Gv,
-                       (const F*)Gd, Jv);
+    hipLaunchKernelGGL((k_turb_src<F>), dim3(nblocks(m.N)), dim3(BS), 0, c->stream, m.N, m.pitch, (const F*)Qd, Jv);
   hipLaunchKernelGGL((k_tcombine<NV, F>), dim3(nblocks(m.N)), dim3(BS), 0, c->stream, m.N, m.pitch, Jv, v, c->d_vol,
                      c->dt);
 }
@@ -1493,6 +1528,17 @@ void launch_d2f(cfd_ctx* c, const double* a, float* b, size_t n) {
 }
 void launch_f2d(cfd_ctx* c, const float* a, double* b, size_t n) {
   if (n) hipLaunchKernelGGL((k_convert<float, double>), dim3(nblocks(n)), dim3(BS), 0, c->stream, a, b, n);
+}
+
+void launch_rowscale(cfd_ctx* c, void* w, const double* s, int prec) {
+  Scale6 sc;
+  for (int i = 0; i < 6; ++i) sc.s[i] = s[i];
+  if (prec == 32)
+    NV_DISPATCH(c->P.nvar, hipLaunchKernelGGL((k_rowscale<NVt, float>), dim3(nblocks(c->N)), dim3(BS), 0, c->stream,
+                                              c->N, c->pitch, (float*)w, sc));
+  else
+    NV_DISPATCH(c->P.nvar, hipLaunchKernelGGL((k_rowscale<NVt, double>), dim3(nblocks(c->N)), dim3(BS), 0,
+                                              c->stream, c->N, c->pitch, (double*)w, sc));
 }
 
 void launch_axpby(cfd_ctx* c, double* y, double a, const double* x, double b, size_t n) {
