@@ -91,6 +91,32 @@ def jfnk(mesh, p, U0, assembly, max_steps):
     return out
 
 
+def copy_calibration():
+    import torch
+    n = 32 * 1024 * 1024  # doubles: 256 MiB read and 256 MiB written per copy
+    a = torch.ones(n, dtype=torch.float64, device="cuda:0")
+    b = torch.empty_like(a)
+    for _ in range(3):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_bytes": n * 8, "copies": 3}), flush=True)
+
+
+def measured_traffic(kernel, mesh_hash):
+    """HBM bytes per launch of `kernel` from the committed counter summary of
+    the same mesh (FETCH_SIZE and WRITE_SIZE scaled by a plain copy), or None."""
+    path = os.path.join(ROOT, "profiles", "traffic.json")
+    try:
+        with open(path) as f:
+            rec = json.load(f)
+    except OSError:
+        return None
+    for r in rec.get("kernels", []):
+        if r.get("stage") == kernel and r.get("mesh_hash") == mesh_hash:
+            return r["bytes_per_launch"]
+    return None
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -101,11 +127,16 @@ def main():
     ap.add_argument("--scale", type=float, default=1.0, help="mesh coarsening (1.0 = the named configuration)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--jfnk-steps", type=int, default=0, help="Newton step cap of the JFNK comparison (0 = skip)")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only run a plain 256 MiB device copy three times (scales the FETCH_SIZE/WRITE_SIZE counters)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (got --gpus {args.gpus}); the multi-GPU layout is design-only",
               file=sys.stderr)
         sys.exit(2)
+    if args.copy_calibration:
+        copy_calibration()
+        return
 
     import torch
     from mycode_amd import cfd, configs
@@ -167,7 +198,8 @@ def main():
                    "assembly": ["colored-scatter", "face-buffer-gather"][args.assembly],
                    "flux_launches_per_residual": launches, "mesh_hash": hex(mesh.hash())},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None, "kernel": dom,
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(dom, hex(mesh.hash())),
+                     "kernel": dom,
                      "algorithmic_bytes_per_residual": b_dom, "ms_per_residual": t_dom},
         "roofline_residual": {"bound": "hbm", "achieved": ach_res, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                               "frac": ach_res / HBM_PEAK_GBS, "B_res": B_res, "ms_event": ms[3]},

@@ -201,6 +201,16 @@ int cfd_vjp_adjoint(cfd_ctx* ctx, const double* U6, const double* w6, double* JT
  * and its inverse stored in `precision`.  B and Binv (N x nv x nv, row
  * major, reference cell order, may be NULL) are copied out for inspection. */
 int cfd_block_jacobi(cfd_ctx* ctx, const double* U6, int precision, double* B, double* Binv);
+/* Multicolor block symmetric Gauss-Seidel preconditioner (no reference
+ * counterpart): M = (D + L) D^-1 (D + U) of the same first-order Jacobian,
+ * D the blocks of cfd_block_jacobi and L, U the couplings to the face
+ * neighbors of lower and higher color of a distance-1 cell coloring.
+ * z6 = M^-1 x6 (M^-T x6 when transpose != 0) if both are given.  Aoff
+ * (N x 4 x nv x nv: per cell, its faces in ascending face index, row-major
+ * coupling block dR_c/dU_nbr, zero where the face has no neighbor), nbr
+ * (N x 4 neighbor cell or -1) and color (N) may be NULL. */
+int cfd_block_sgs(cfd_ctx* ctx, const double* U6, int precision, int transpose, const double* x6, double* z6,
+                  double* Aoff, int* nbr, int* color);
 /* Solver::norm_L2, dot_product, norm_L2_VolumeWeighted — Solver.cpp:1250-1303 */
 int cfd_norm_l2(cfd_ctx* ctx, const double* v6, double* out);
 int cfd_dot(cfd_ctx* ctx, const double* a6, const double* b6, double* out);
@@ -214,7 +224,8 @@ typedef struct cfd_gmres_opts {
   int jv;           /* 0 finite difference (reference), 1 tangent, 2 transpose:
                        solve J^T x = RHS with the reverse product (adjoint)      */
   int precision;    /* 64: FP64 basis/matvec; 32: FP32 basis, matvec, precond      */
-  int precond;      /* 0 scalar diagonal (reference, 1424-1465), 1 block-Jacobi    */
+  int precond;      /* 0 scalar diagonal (reference, 1424-1465), 1 block-Jacobi,   *
+                     * 2 multicolor block symmetric Gauss-Seidel                   */
   int scaling;      /* 0: unscaled 6-component inner product (reference);
                        1: rows scaled by the freestream state, so every equation
                           weighs alike in the Krylov norm (no reference counterpart) */

@@ -114,6 +114,7 @@ SIGNATURES = [
     ("cfd_jv_tangent", C.c_int, [_VP, _DP, _DP, _DP, C.c_int]),
     ("cfd_vjp_adjoint", C.c_int, [_VP, _DP, _DP, _DP]),
     ("cfd_block_jacobi", C.c_int, [_VP, _DP, C.c_int, _DP, _DP]),
+    ("cfd_block_sgs", C.c_int, [_VP, _DP, C.c_int, C.c_int, _DP, _DP, _DP, _IP, _IP]),
     ("cfd_norm_l2", C.c_int, [_VP, _DP, _DP]),
     ("cfd_dot", C.c_int, [_VP, _DP, _DP, _DP]),
     ("cfd_norm_vw", C.c_int, [_VP, _DP, _DP]),
@@ -413,6 +414,26 @@ class Solver:
         Bi = np.zeros((self.N, nv, nv))
         self._c(self.lib.cfd_block_jacobi(self._h, _dp(U), precision, _dp(B), _dp(Bi)), "cfd_block_jacobi")
         return B, Bi
+
+    def block_sgs(self, U, x=None, precision: int = 64, transpose: bool = False, blocks: bool = False):
+        """Multicolor block SGS preconditioner at U.
+
+        Returns (z, Aoff, nbr, color): z = M^-1 x (M^-T x with transpose) or
+        None; with blocks=True the coupling blocks (N x 4 x nv x nv, slots in
+        ascending face index), neighbor cells (N x 4, -1 for none) and the cell
+        colors (N), else None for each."""
+        U = _state(U, self.N)
+        nv = self.params().nvar
+        xz = _state(x, self.N) if x is not None else None
+        z = np.zeros((self.N, 6)) if x is not None else None
+        A = np.zeros((self.N, 4, nv, nv)) if blocks else None
+        nb = np.zeros((self.N, 4), dtype=np.int32) if blocks else None
+        col = np.zeros(self.N, dtype=np.int32) if blocks else None
+        dp = lambda a: _dp(a) if a is not None else None
+        ip = lambda a: _ip(a) if a is not None else None
+        self._c(self.lib.cfd_block_sgs(self._h, _dp(U), precision, int(transpose), dp(xz), dp(z), dp(A),
+                                       ip(nb), ip(col)), "cfd_block_sgs")
+        return z, A, nb, col
 
     def norm_L2(self, v) -> float:
         v = _state(v, self.N)

@@ -95,6 +95,8 @@ void free_krylov(cfd_ctx* c) {
   (void)hipFree(k.V);
   (void)hipFree(k.D);
   (void)hipFree(k.blk);
+  (void)hipFree(k.aoff);
+  (void)hipFree(k.xt);
   (void)hipFree(k.z);
   (void)hipFree(k.w);
   (void)hipFree(k.rr);
@@ -121,6 +123,16 @@ int ensure_krylov(cfd_ctx* c, int m, int prec) {
   HIP_TRY(c, hipHostMalloc((void**)&k.hpin, 256 * 8, hipHostMallocDefault));
   k.m = m;
   k.prec = prec;
+  return CFD_OK;
+}
+
+// coupling blocks and the solution staging vector of the block SGS preconditioner
+int ensure_sgs(cfd_ctx* c) {
+  Krylov& k = c->kry;
+  if (k.aoff) return CFD_OK;
+  const size_t es = k.prec == 32 ? 4 : 8;
+  HIP_TRY(c, hipMalloc(&k.aoff, (size_t)4 * c->P.nvar * c->P.nvar * c->pitch * es));
+  HIP_TRY(c, hipMalloc((void**)&k.xt, (size_t)c->P.nvar * c->pitch * 8));
   return CFD_OK;
 }
 
@@ -413,6 +425,37 @@ int cfd_ctx_create(const cfd_mesh* m, const cfd_params* p, cfd_ctx** out) {
     int rc = layout_faces(c);
     if (rc) return rc;
   }
+  {  // cell coloring and the neighbor/back-slot table of the block SGS sweeps
+    std::vector<int4> nq(P_, make_int4(-1, -1, -1, -1));
+    std::vector<int> fl((size_t)4 * N, -1), cnt(N, 0);
+    for (int f = 0; f < F; ++f) {
+      const int co = m->perm[m->own[f]];
+      fl[(size_t)4 * co + cnt[co]++] = f;
+      if (m->nei[f] >= 0) {
+        const int cn = m->perm[m->nei[f]];
+        fl[(size_t)4 * cn + cnt[cn]++] = f;
+      }
+    }
+    for (int i = 0; i < N; ++i)
+      for (int q = 0; q < cnt[i]; ++q) {
+        const int f = fl[(size_t)4 * i + q];
+        if (m->nei[f] < 0) continue;
+        const int co = m->perm[m->own[f]], cn = m->perm[m->nei[f]];
+        const int o = co == i ? cn : co;
+        int r = 0;
+        while (fl[(size_t)4 * o + r] != f) ++r;
+        (&nq[i].x)[q] = (o << 2) | r;
+      }
+    std::vector<unsigned char> col(P_, 0);
+    for (int i = 0; i < N; ++i) col[i] = (unsigned char)m->cell_color[i];
+    c->ccell_ptr = m->ccell_ptr;
+    HIP_TRY(c, dalloc(&c->d_ccol, P_));
+    HIP_TRY(c, dalloc(&c->d_ccells, std::max(N, 1)));
+    HIP_TRY(c, dalloc(&c->d_nq, P_));
+    HIP_TRY(c, hipMemcpy(c->d_ccol, col.data(), P_, hipMemcpyHostToDevice));
+    HIP_TRY(c, hipMemcpy(c->d_ccells, m->ccells.data(), (size_t)N * 4, hipMemcpyHostToDevice));
+    HIP_TRY(c, hipMemcpy(c->d_nq, nq.data(), (size_t)P_ * sizeof(int4), hipMemcpyHostToDevice));
+  }
   // solver buffers
   HIP_TRY(c, dalloc(&c->grad, (size_t)14 * P_));
   HIP_TRY(c, dalloc(&c->phi, P_));
@@ -445,7 +488,7 @@ int cfd_ctx_destroy(cfd_ctx* c) {
   (void)hipStreamSynchronize(c->stream);
   free_krylov(c);
   void* bufs[] = {c->d_vol, c->d_xc,  c->d_yc,  c->d_slots, c->d_own, c->d_nei, c->d_tag,  c->d_nx,
-                  c->d_ny,  c->d_area, c->d_xm, c->d_ym,    c->d_iperm, c->d_facemap, c->d_alt, c->grad, c->d_wd, c->d_dnw, c->d_corder,
+                  c->d_ny,  c->d_area, c->d_xm, c->d_ym,    c->d_iperm, c->d_facemap, c->d_alt, c->grad, c->d_wd, c->d_dnw, c->d_corder, c->d_ccol, c->d_ccells, c->d_nq,
                   c->phi,   c->dt,    c->Fm,    c->red,     c->scal, c->gradT, c->adj, c->tq, c->io6};
   for (void* b : bufs) (void)hipFree(b);
   for (int i = 0; i < 6; ++i) (void)hipFree(c->vec[i]);
@@ -639,6 +682,55 @@ int cfd_block_jacobi(cfd_ctx* c, const double* U6, int precision, double* B, dou
   return CFD_OK;
 }
 
+int cfd_block_sgs(cfd_ctx* c, const double* U6, int precision, int transpose, const double* x6, double* z6,
+                  double* Aoff, int* nbr, int* color) {
+  if (!c || !U6 || (precision != 64 && precision != 32) || (!x6) != (!z6)) return CFD_ERR_ARG;
+  if (!c->dt_valid) return fail(c, CFD_ERR_STATE, "no local time step computed");
+  int rc;
+  if ((rc = upload_state(c, U6, c->vec[0]))) return rc;
+  if ((rc = ensure_krylov(c, c->kry.m > 0 ? c->kry.m : 1, precision))) return rc;
+  if ((rc = ensure_sgs(c))) return rc;
+  const int nv = c->P.nvar, P_ = c->pitch;
+  launch_block_precond(c, c->vec[0], c->kry.blk, precision, nullptr, c->kry.aoff);
+  if (x6) {
+    if ((rc = upload_state(c, x6, c->vec[1]))) return rc;
+    launch_sgs_apply(c, c->kry.blk, c->kry.aoff, c->vec[1], 64, c->vec[2], 64, precision, transpose != 0);
+    if ((rc = download_state(c, c->vec[2], z6))) return rc;
+  }
+  if (Aoff || nbr) {
+    const size_t na = (size_t)4 * nv * nv * P_;
+    std::vector<double> hA(Aoff ? na : 0);
+    std::vector<float> hAf;
+    std::vector<int4> hq(P_);
+    hipError_t e = hipMemcpyAsync(hq.data(), c->d_nq, (size_t)P_ * sizeof(int4), hipMemcpyDeviceToHost, c->stream);
+    if (e == hipSuccess && Aoff) {
+      if (precision == 64) {
+        e = hipMemcpyAsync(hA.data(), c->kry.aoff, na * 8, hipMemcpyDeviceToHost, c->stream);
+      } else {
+        hAf.resize(na);
+        e = hipMemcpyAsync(hAf.data(), c->kry.aoff, na * 4, hipMemcpyDeviceToHost, c->stream);
+      }
+    }
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    if (e != hipSuccess) return fail(c, CFD_ERR_HIP, hipGetErrorString(e));
+    if (precision == 32)
+      for (size_t i = 0; i < hAf.size(); ++i) hA[i] = hAf[i];
+    for (int i = 0; i < c->N; ++i) {
+      const int r = c->iperm[i];
+      for (int q = 0; q < 4; ++q) {
+        const int e4 = (&hq[i].x)[q];
+        if (nbr) nbr[(size_t)r * 4 + q] = e4 < 0 ? -1 : c->iperm[e4 >> 2];
+        if (!Aoff) continue;
+        for (int k = 0; k < nv * nv; ++k)
+          Aoff[((size_t)r * 4 + q) * nv * nv + k] = e4 < 0 ? 0.0 : hA[(size_t)(q * nv * nv + k) * P_ + i];
+      }
+    }
+  }
+  if (color)
+    for (int i = 0; i < c->N; ++i) color[c->iperm[i]] = c->mesh->cell_color[i];
+  return sync(c);
+}
+
 int cfd_norm_l2(cfd_ctx* c, const double* v6, double* out) {
   if (!c || !v6 || !out) return CFD_ERR_ARG;
   int rc = upload_state(c, v6, c->vec[0]);
@@ -700,7 +792,7 @@ static int gmres_cycle(cfd_ctx* c, const double* U, const double* Rb, const doub
   const size_t es = prec == 32 ? 4 : 8;
   const size_t vs = (size_t)c->P.nvar * c->pitch;
   char* V = (char*)k.V;
-  const bool bj = o->precond == 1;
+  const bool bj = o->precond == 1, sgs = o->precond == 2;
   const bool tr = o->jv == 2;
   double sc[6];
   row_scales(c, sc);
@@ -730,9 +822,10 @@ static int gmres_cycle(cfd_ctx* c, const double* U, const double* Rb, const doub
     void* Vj = V + (size_t)j * vs * es;
     void* Vj1 = V + (size_t)(j + 1) * vs * es;
     if (bj) launch_block_apply(c, k.blk, Vj, k.z, prec, tr);
+    else if (sgs) launch_sgs_apply(c, k.blk, k.aoff, Vj, prec, k.z, prec, prec, tr);
     else launch_precond_apply(c, k.z, Vj, prec, o->jv == 0 ? S_VNORM : -1);
     if (o->jv == 0) {
-      if (bj) launch_reduce(c, RED_L2, (const double*)k.z, nullptr, S_VNORM);
+      if (bj || sgs) launch_reduce(c, RED_L2, (const double*)k.z, nullptr, S_VNORM);
       launch_jv_fd(c, U, Rb, (const double*)k.z, (double*)Vj1, S_UNORM, S_VNORM, tmpU, tmpR);
     } else if (o->jv == 1) {
       launch_jv_tangent(c, U, k.z, Vj1, prec);
@@ -773,8 +866,14 @@ static int gmres_cycle(cfd_ctx* c, const double* U, const double* Rb, const doub
   }
   std::memcpy(k.hpin, y.data(), (kk + 1) * 8);
   HIP_TRY(c, hipMemcpyAsync(k.h + 128, k.hpin, (kk + 1) * 8, hipMemcpyHostToDevice, c->stream));
-  if (bj) launch_solution_bj(c, x, V, k.h + 128, kk, k.blk, prec, tr);
-  else launch_solution(c, x, V, k.h + 128, kk, prec);
+  if (bj) {
+    launch_solution_bj(c, x, V, k.h + 128, kk, k.blk, prec, tr);
+  } else if (sgs) {
+    launch_solution(c, k.xt, V, k.h + 128, kk, prec, false);
+    launch_sgs_apply(c, k.blk, k.aoff, k.xt, 64, x, 64, prec, tr);
+  } else {
+    launch_solution(c, x, V, k.h + 128, kk, prec);
+  }
   *iters = kk + 1;
   *res_out = std::abs(g[kk + 1]);
   return sync(c);
@@ -789,14 +888,20 @@ static int gmres_dev(cfd_ctx* c, const double* U, const double* Rb, const double
   if (prec != 64 && prec != 32) return fail(c, CFD_ERR_ARG, "precision must be 64 or 32");
   if (o->jv < 0 || o->jv > 2) return fail(c, CFD_ERR_ARG, "jv must be 0 (FD), 1 (tangent) or 2 (transpose)");
   if (o->jv != 1 && prec != 64) return fail(c, CFD_ERR_ARG, "FD Jv and the transpose run in FP64 only");
-  if (o->precond != 0 && o->precond != 1) return fail(c, CFD_ERR_ARG, "unknown preconditioner");
+  if (o->precond < 0 || o->precond > 2) return fail(c, CFD_ERR_ARG, "unknown preconditioner");
   if (o->restarts < 0) return fail(c, CFD_ERR_ARG, "restarts must be >= 0");
   int rc = ensure_krylov(c, m, prec);
   if (rc) return rc;
   Krylov& k = c->kry;
   const size_t vs = (size_t)c->P.nvar * c->pitch;
-  if (o->precond == 1) launch_block_precond(c, U, k.blk, prec);
-  else launch_diag_precond(c, U, k.D);
+  if (o->precond == 1) {
+    launch_block_precond(c, U, k.blk, prec);
+  } else if (o->precond == 2) {
+    if ((rc = ensure_sgs(c))) return rc;
+    launch_block_precond(c, U, k.blk, prec, nullptr, k.aoff);
+  } else {
+    launch_diag_precond(c, U, k.D);
+  }
   launch_reduce(c, RED_L2, U, nullptr, S_UNORM);
   double b2;
   if (o->scaling) {

@@ -33,6 +33,8 @@ struct Krylov {
   void* V = nullptr;           // (m + 1) vectors, nv x pitch each
   double* D = nullptr;         // scalar preconditioner
   void* blk = nullptr;         // block-Jacobi inverses (nv x nv per cell)
+  void* aoff = nullptr;        // neighbor coupling blocks (4 x nv x nv per cell), block SGS only
+  double* xt = nullptr;        // basis combination before the block SGS solve (FP64)
   double* z = nullptr;         // preconditioned direction (FP64)
   double* w = nullptr;         // accumulated solution of restarted GMRES (FP64)
   double* rr = nullptr;        // true residual of a restart cycle (FP64)
@@ -53,6 +55,7 @@ struct cfd_ctx {
   int N = 0, F = 0, pitch = 0, ncolors = 0;
   std::vector<int> color_ptr;
   std::vector<int> face_order, perm, iperm;  // host copies
+  std::vector<int> ccell_ptr;                 // cell color ranges into d_ccells
   cfd::DMesh dm{};
   int assembly = 1;  // 0 colored scatter, 1 face buffer + gather
   const cfd_mesh* mesh = nullptr;   // borrowed: must outlive the context (Solver holds const Mesh&)
@@ -65,6 +68,9 @@ struct cfd_ctx {
   double *d_nx = nullptr, *d_ny = nullptr, *d_area = nullptr, *d_xm = nullptr, *d_ym = nullptr;
   double *d_wd = nullptr, *d_dnw = nullptr;
   unsigned char* d_alt = nullptr;
+  unsigned char* d_ccol = nullptr;  // per cell: color of the distance-1 cell coloring
+  int* d_ccells = nullptr;          // cells grouped by color
+  int4* d_nq = nullptr;             // per cell slot: (neighbor << 2) | back slot, or -1
   // solver state
   double* grad = nullptr;   // 14 x pitch (rho u v p T gradients; k, omega with SST)
   double* phi = nullptr;    // pitch
@@ -125,9 +131,15 @@ void launch_d2f(cfd_ctx* c, const double* a, float* b, size_t n);
 void launch_f2d(cfd_ctx* c, const float* a, double* b, size_t n);
 void launch_orth(cfd_ctx* c, void* V, int j, double* w, int prec, double* hdev);
 void launch_store_basis(cfd_ctx* c, void* Vj1, const double* w, const double* hdev, int j, int prec);
-void launch_solution(cfd_ctx* c, double* x, const void* V, const double* y, int k, int prec);
+// x = D^-1 sum_{i<=k} y_i V_i (diag) or the plain combination
+void launch_solution(cfd_ctx* c, double* x, const void* V, const double* y, int k, int prec, bool diag = true);
 // block-Jacobi: assemble + invert (Binv: nv*nv x pitch of the basis type), apply, solution
-void launch_block_precond(cfd_ctx* c, const double* U, void* Binv, int prec, double* Bout = nullptr);
+void launch_block_precond(cfd_ctx* c, const double* U, void* Binv, int prec, double* Bout = nullptr,
+                          void* Aoff = nullptr);
+// multicolor block SGS: z = M^-1 x (M^-T with transpose); blocks of precision bprec, x/z of xprec/zprec
+// (bprec 64 needs FP64 x and z; bprec 32 takes FP32 x and z or FP64 x and z)
+void launch_sgs_apply(cfd_ctx* c, const void* Binv, const void* Aoff, const void* x, int xprec, void* z,
+                      int zprec, int bprec, bool transpose);
 void launch_block_apply(cfd_ctx* c, const void* Binv, const void* x, void* z, int prec, bool transpose = false);
 void launch_solution_bj(cfd_ctx* c, double* x, const void* V, const double* y, int k, const void* Binv, int prec,
                         bool transpose = false);

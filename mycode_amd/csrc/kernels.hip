@@ -820,11 +820,15 @@ __global__ __launch_bounds__(BS) void k_diagprec(DMesh m, Phys P, const double* 
 // ghost state of a boundary face is differentiated with it), d the normal
 // distance between the centroids (twice the centroid-face distance at a
 // boundary).  B_c is inverted by Gauss-Jordan with partial pivoting in FP64
-// and stored as TV, NV*NV rows of pitch.
+// and stored as TV, NV*NV rows of pitch.  With Aoff the coupling blocks of
+// the same first-order Jacobian to the interior neighbors are stored too,
+// A_{c,q} = (+-) A dF/dU_other - nu A / d I for slot q of cell c, in rows
+// (q * NV + i) * NV + j of pitch (slots without a neighbor are not written).
 template <int NV, class TV>
 __global__ __launch_bounds__(BS) void k_block(DMesh m, Phys P, const double* __restrict__ U,
                                                const double* __restrict__ G, const double* __restrict__ dt,
-                                               TV* __restrict__ Binv, double* __restrict__ Bout) {
+                                               TV* __restrict__ Binv, double* __restrict__ Bout,
+                                               TV* __restrict__ Aoff) {
   typedef Dual<double> D;
   const int c = xcd_block() * BS + threadIdx.x;
   if (c >= m.N) return;
@@ -865,6 +869,7 @@ __global__ __launch_bounds__(BS) void k_block(DMesh m, Phys P, const double* __r
 #pragma unroll
       for (int i = 0; i < NV; ++i) B[i][k] += sg * Fl[i].d;
     }
+    double vf = 0.0;
     if (P.viscous) {
       double d;
       if (other >= 0) {
@@ -872,7 +877,24 @@ __global__ __launch_bounds__(BS) void k_block(DMesh m, Phys P, const double* __r
       } else {
         d = 2.0 * fabs((m.xm[f] - m.xc[c]) * g.nx + (m.ym[f] - m.yc[c]) * g.ny);
       }
-      visc += P.nu * g.A / fmax(d, 1e-300);
+      vf = P.nu * g.A / fmax(d, 1e-300);
+      visc += vf;
+    }
+    if (Aoff && other >= 0) {
+      const double sg = as_nei ? -1.0 : 1.0;
+      for (int k = 0; k < NV; ++k) {
+        S6<D> Ud, Od;
+#pragma unroll
+        for (int i = 0; i < 6; ++i) {
+          Ud[i] = D(Uc[i], 0.0);
+          Od[i] = D(Uo[i], i == k ? 1.0 : 0.0);
+        }
+        const S6<D> Fl = as_nei ? flux_core<0, D>(P1, g, Od, Ud, nullptr, nullptr)
+                                : flux_core<0, D>(P1, g, Ud, Od, nullptr, nullptr);
+#pragma unroll
+        for (int i = 0; i < NV; ++i)
+          Aoff[(size_t)((q * NV + i) * NV + k) * p + c] = (TV)(sg * Fl[i].d - (i == k && i > 0 ? vf : 0.0));
+      }
     }
   }
   if constexpr (NV == 6) {
@@ -983,6 +1005,84 @@ __global__ __launch_bounds__(BS) void k_solution_bj(int N, int p, size_t vstride
 #pragma unroll
     for (int j = 0; j < NV; ++j) a += (double)Binv[(size_t)(TR ? j * NV + i : i * NV + j) * p + c] * s[j];
     x[i * p + c] = a;
+  }
+}
+
+// ---- multicolor block symmetric Gauss-Seidel --------------------------------------
+// M = (D + L) D^-1 (D + U) of the first-order Jacobian, L and U the couplings
+// to neighbors of lower and higher cell color (no two neighbors share a
+// color).  M z = x is two sweeps over the colors, each a launch per color of
+//   z_c = D_c^-1 (x_c - sum_q A_{c,q} z_{nbr(c,q)}),
+// ascending then descending (the last color of the ascending sweep is final
+// and not repeated); z starts at zero.  M^T z = x (the transposed system)
+// runs the same sweeps with D_c^-T and A_{nbr,q'}^T, q' the slot of the
+// neighbor that points back to c.  nq[c] holds (neighbor << 2) | q' per slot,
+// -1 where the slot has no interior neighbor.
+
+// first launch: z = D^-1 x on color 0, zero elsewhere
+template <int NV, class TB, class TX, class TZ, bool TR>
+__global__ __launch_bounds__(BS) void k_sgs_first(int N, int p, const unsigned char* __restrict__ col,
+                                                   const TB* __restrict__ Binv, const TX* __restrict__ x,
+                                                   TZ* __restrict__ z) {
+  const int c = xcd_block() * BS + threadIdx.x;
+  if (c >= N) return;
+  if (col[c] != 0) {
+#pragma unroll
+    for (int i = 0; i < NV; ++i) z[i * p + c] = (TZ)0.0;
+    return;
+  }
+  double xv[NV];
+#pragma unroll
+  for (int j = 0; j < NV; ++j) xv[j] = (double)x[j * p + c];
+#pragma unroll
+  for (int i = 0; i < NV; ++i) {
+    double s = 0.0;
+#pragma unroll
+    for (int j = 0; j < NV; ++j) s += (double)Binv[(size_t)(TR ? j * NV + i : i * NV + j) * p + c] * xv[j];
+    z[i * p + c] = (TZ)s;
+  }
+}
+
+template <int NV, class TB, class TX, class TZ, bool TR>
+__global__ __launch_bounds__(BS) void k_sgs_color(int n, const int* __restrict__ cells, int p,
+                                                   const int4* __restrict__ nq, const TB* __restrict__ Binv,
+                                                   const TB* __restrict__ Aoff, const TX* __restrict__ x,
+                                                   TZ* __restrict__ z) {
+  const int t = xcd_block() * BS + threadIdx.x;
+  if (t >= n) return;
+  const int c = cells[t];
+  double s[NV];
+#pragma unroll
+  for (int i = 0; i < NV; ++i) s[i] = (double)x[i * p + c];
+  const int4 e4 = nq[c];
+  const int es[4] = {e4.x, e4.y, e4.z, e4.w};
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    const int e = es[q];
+    if (e < 0) continue;
+    const int o = e >> 2;
+    double zo[NV];
+#pragma unroll
+    for (int j = 0; j < NV; ++j) zo[j] = (double)z[j * p + o];
+    if (TR) {
+      const int r = e & 3;
+#pragma unroll
+      for (int i = 0; i < NV; ++i)
+#pragma unroll
+        for (int j = 0; j < NV; ++j) s[i] -= (double)Aoff[(size_t)((r * NV + j) * NV + i) * p + o] * zo[j];
+    } else {
+#pragma unroll
+      for (int i = 0; i < NV; ++i)
+#pragma unroll
+        for (int j = 0; j < NV; ++j) s[i] -= (double)Aoff[(size_t)((q * NV + i) * NV + j) * p + c] * zo[j];
+    }
+  }
+#pragma unroll
+  for (int i = 0; i < NV; ++i) {
+    double a = 0.0;
+#pragma unroll
+    for (int j = 0; j < NV; ++j) a += (double)Binv[(size_t)(TR ? j * NV + i : i * NV + j) * p + c] * s[j];
+    z[i * p + c] = (TZ)a;
   }
 }
 
@@ -1309,7 +1409,7 @@ __global__ __launch_bounds__(BS) void k_solution(int N, int p, size_t vstride, c
                                                   const double* __restrict__ D, double* __restrict__ x) {
   const int c = xcd_block() * BS + threadIdx.x;
   if (c >= N) return;
-  const double id = 1.0 / D[c];
+  const double id = D ? 1.0 / D[c] : 1.0;
 #pragma unroll
   for (int k = 0; k < NV; ++k) {
     double s = 0.0;
@@ -1632,17 +1732,45 @@ void launch_store_basis(cfd_ctx* c, void* Vj1, const double* w, const double* hd
   else store_t<double>(c, (double*)Vj1, hdev + j + 1);
 }
 
-template <class TV> static void block_t(cfd_ctx* c, const double* U, TV* Binv, double* Bout) {
+template <class TV> static void block_t(cfd_ctx* c, const double* U, TV* Binv, double* Bout, TV* Aoff) {
   NV_DISPATCH(c->P.nvar, {
     hipLaunchKernelGGL((k_gradlim<NVt, double>), dim3(nblocks(c->N)), dim3(BS), 0, c->stream, c->dm, c->phys, U,
                        (const double*)nullptr, c->grad, (double*)nullptr, c->phi, 0);
     hipLaunchKernelGGL((k_block<NVt, TV>), dim3(nblocks(c->N)), dim3(BS), 0, c->stream, c->dm, c->phys, U,
-                       (const double*)c->grad, c->dt, Binv, Bout);
+                       (const double*)c->grad, c->dt, Binv, Bout, Aoff);
   });
 }
-void launch_block_precond(cfd_ctx* c, const double* U, void* Binv, int prec, double* Bout) {
-  if (prec == 32) block_t<float>(c, U, (float*)Binv, Bout);
-  else block_t<double>(c, U, (double*)Binv, Bout);
+void launch_block_precond(cfd_ctx* c, const double* U, void* Binv, int prec, double* Bout, void* Aoff) {
+  if (prec == 32) block_t<float>(c, U, (float*)Binv, Bout, (float*)Aoff);
+  else block_t<double>(c, U, (double*)Binv, Bout, (double*)Aoff);
+}
+template <class TB, class TX, class TZ, bool TR>
+static void sgs_t(cfd_ctx* c, const TB* Binv, const TB* Aoff, const TX* x, TZ* z) {
+  const int nc = (int)c->ccell_ptr.size() - 1;
+  NV_DISPATCH(c->P.nvar, {
+    hipLaunchKernelGGL((k_sgs_first<NVt, TB, TX, TZ, TR>), dim3(nblocks(c->N)), dim3(BS), 0, c->stream, c->N,
+                       c->pitch, c->d_ccol, Binv, x, z);
+    for (int s = 1; s < 2 * nc - 1; ++s) {
+      const int col = s < nc ? s : 2 * nc - 2 - s;
+      const int b = c->ccell_ptr[col], n = c->ccell_ptr[col + 1] - b;
+      if (n == 0) continue;
+      hipLaunchKernelGGL((k_sgs_color<NVt, TB, TX, TZ, TR>), dim3(nblocks(n)), dim3(BS), 0, c->stream, n,
+                         c->d_ccells + b, c->pitch, c->d_nq, Binv, Aoff, x, z);
+    }
+  });
+}
+void launch_sgs_apply(cfd_ctx* c, const void* Binv, const void* Aoff, const void* x, int xprec, void* z,
+                      int zprec, int bprec, bool transpose) {
+#define SGS_CASE(TB, TX, TZ)                                                                              \
+  if (transpose) sgs_t<TB, TX, TZ, true>(c, (const TB*)Binv, (const TB*)Aoff, (const TX*)x, (TZ*)z);    \
+  else sgs_t<TB, TX, TZ, false>(c, (const TB*)Binv, (const TB*)Aoff, (const TX*)x, (TZ*)z);
+  if (bprec == 32) {
+    if (xprec == 32 && zprec == 32) { SGS_CASE(float, float, float) }
+    else { SGS_CASE(float, double, double) }
+  } else {
+    SGS_CASE(double, double, double)
+  }
+#undef SGS_CASE
 }
 template <class TV, bool TR> static void bapply_t(cfd_ctx* c, const TV* Binv, const TV* x, TV* z) {
   NV_DISPATCH(c->P.nvar, hipLaunchKernelGGL((k_block_apply<NVt, TV, TV, TR>), dim3(nblocks(c->N)), dim3(BS), 0,
@@ -1674,14 +1802,16 @@ void launch_solution_bj(cfd_ctx* c, double* x, const void* V, const double* y, i
   }
 }
 
-template <class TV> static void solution_t(cfd_ctx* c, double* x, const TV* V, const double* y, int k) {
+template <class TV>
+static void solution_t(cfd_ctx* c, double* x, const TV* V, const double* y, int k, const double* D) {
   const size_t vs = (size_t)c->P.nvar * c->pitch;
   NV_DISPATCH(c->P.nvar, hipLaunchKernelGGL((k_solution<NVt, TV>), dim3(nblocks(c->N)), dim3(BS), 0, c->stream,
-                                            c->N, c->pitch, vs, V, y, k + 1, c->kry.D, x));
+                                            c->N, c->pitch, vs, V, y, k + 1, D, x));
 }
-void launch_solution(cfd_ctx* c, double* x, const void* V, const double* y, int k, int prec) {
-  if (prec == 32) solution_t<float>(c, x, (const float*)V, y, k);
-  else solution_t<double>(c, x, (const double*)V, y, k);
+void launch_solution(cfd_ctx* c, double* x, const void* V, const double* y, int k, int prec, bool diag) {
+  const double* D = diag ? c->kry.D : nullptr;
+  if (prec == 32) solution_t<float>(c, x, (const float*)V, y, k, D);
+  else solution_t<double>(c, x, (const double*)V, y, k, D);
 }
 
 }  // namespace cfd

@@ -218,6 +218,41 @@ void color_faces(cfd_mesh& m) {
   for (int c = 0; c < nc; ++c) m.color_ptr[c + 1] += m.color_ptr[c];
 }
 
+// Greedy distance-1 coloring of the cells in RCM order (each cell takes the
+// lowest color not used by an already colored neighbor): the color classes
+// of the multicolor Gauss-Seidel sweeps.
+void color_cells(cfd_mesh& m) {
+  int N = m.ncells(), F = m.nfaces();
+  std::vector<int> deg(N + 1, 0);
+  for (int f = 0; f < F; ++f)
+    if (m.nei[f] >= 0) { deg[m.perm[m.own[f]] + 1]++; deg[m.perm[m.nei[f]] + 1]++; }
+  for (int i = 0; i < N; ++i) deg[i + 1] += deg[i];
+  std::vector<int> adj(deg[N]), fill(deg.begin(), deg.end() - 1);
+  for (int f = 0; f < F; ++f)
+    if (m.nei[f] >= 0) {
+      int a = m.perm[m.own[f]], b = m.perm[m.nei[f]];
+      adj[fill[a]++] = b;
+      adj[fill[b]++] = a;
+    }
+  m.cell_color.assign(N, -1);
+  int nc = 0;
+  for (int i = 0; i < N; ++i) {
+    uint32_t used = 0;
+    for (int k = deg[i]; k < deg[i + 1]; ++k)
+      if (m.cell_color[adj[k]] >= 0) used |= 1u << m.cell_color[adj[k]];
+    int c = 0;
+    while (used & (1u << c)) ++c;
+    m.cell_color[i] = c;
+    nc = std::max(nc, c + 1);
+  }
+  m.ccell_ptr.assign(nc + 1, 0);
+  for (int i = 0; i < N; ++i) m.ccell_ptr[m.cell_color[i] + 1]++;
+  for (int c = 0; c < nc; ++c) m.ccell_ptr[c + 1] += m.ccell_ptr[c];
+  std::vector<int> pos(m.ccell_ptr.begin(), m.ccell_ptr.end() - 1);
+  m.ccells.assign(N, 0);
+  for (int i = 0; i < N; ++i) m.ccells[pos[m.cell_color[i]]++] = i;
+}
+
 // ---- synthetic meshes -------------------------------------------------------
 struct SplitMix64 {  // Vigna's SplitMix64
   uint64_t s;
@@ -272,6 +307,7 @@ int finish(cfd_mesh& m, std::string& err) {
   compute_geometry(m);
   rcm(m);
   color_faces(m);
+  color_cells(m);
   return CFD_OK;
 }
 }  // namespace cfdmesh

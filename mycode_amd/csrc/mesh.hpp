@@ -27,6 +27,10 @@ struct cfd_mesh {
   int ncolors = 0;
   std::vector<int> face_order;  // device face slot -> reference face
   std::vector<int> color_ptr;   // ncolors + 1 offsets into face_order
+  // distance-1 cell coloring (no two cells sharing a face alike), RCM positions
+  std::vector<int> cell_color;  // per RCM position
+  std::vector<int> ccells;      // RCM positions grouped by cell color, ascending within a color
+  std::vector<int> ccell_ptr;   // n_cell_colors + 1
   // synthetic-mesh extras (valid after cfd_mesh_generate)
   int config = 0;
   std::vector<double> init_U6;  // seeded initial state, reference order (may be empty)
