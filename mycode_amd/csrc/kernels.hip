@@ -354,39 +354,32 @@ __device__ __forceinline__ S6<T> flux_core(const Phys& P, const FaceGeo& g, cons
     Fl[1] = Fl[1] - (txx * nx + txy * ny);
     Fl[2] = Fl[2] - (txy * nx + tyy * ny);
     Fl[3] = Fl[3] - (wx * nx + wy * ny);
-    if (P.turb) {  // (mu + sigma mu_t) grad(k, omega) . n
-      T kx, ky, wx_, wy_;
+    if (P.turb) {  // (mu + sigma mu_t) dk/dn, domega/dn by two-point normal differences
+      // wall: to k = 0 and omega_wall over the owner's normal distance;
+      // interior: between the centroids over their normal separation (a
+      // positive coupling on the stretched split cells, where the averaged
+      // gradients decouple the two triangles of a quad); averaged gradients on
+      // the other boundary faces
+      T kn, wn;
       const T kcl = L[4] / L[0], wcl = L[5] / L[0];
       if (g.boundary && g.tg == 4) {
-        const double dx = g.xm - g.xco, dy = g.ym - g.yco;
-        const double d2 = dx * dx + dy * dy;
+        const double dn = g.dnw;
         const double ww = omega_wall(P, g.dnw);
-        kx = -kcl * dx / d2;
-        ky = -kcl * dy / d2;
-        wx_ = (ww - wcl) * dx / d2;
-        wy_ = (ww - wcl) * dy / d2;
+        kn = (0.0 - kcl) / dn;
+        wn = (ww - wcl) / dn;
+      } else if (!g.boundary) {
+        double dn = fabs((g.xcn - g.xco) * nx + (g.ycn - g.yco) * ny);
+        if (dn < 1e-300) dn = 1e-300;
+        kn = ((R[4] / R[0]) - kcl) / dn;
+        wn = ((R[5] / R[0]) - wcl) / dn;
       } else {
-        kx = 0.5 * (gL[10] + gR[10]);
-        ky = 0.5 * (gL[11] + gR[11]);
-        wx_ = 0.5 * (gL[12] + gR[12]);
-        wy_ = 0.5 * (gL[13] + gR[13]);
-        if (!g.boundary) {
-          const double dxLR = g.xcn - g.xco, dyLR = g.ycn - g.yco;
-          const double d2 = dxLR * dxLR + dyLR * dyLR;
-          if (d2 > 1e-14) {
-            const T dk = (R[4] / R[0]) - kcl, dw = (R[5] / R[0]) - wcl;
-            const T kx0 = kx, ky0 = ky, wx0 = wx_, wy0 = wy_;
-            kx = kx0 - (kx0 * dxLR + ky0 * dyLR - dk) * dxLR / d2;
-            ky = ky0 - (kx0 * dxLR + ky0 * dyLR - dk) * dyLR / d2;
-            wx_ = wx0 - (wx0 * dxLR + wy0 * dyLR - dw) * dxLR / d2;
-            wy_ = wy0 - (wx0 * dxLR + wy0 * dyLR - dw) * dyLR / d2;
-          }
-        }
+        kn = 0.5 * (gL[10] + gR[10]) * nx + 0.5 * (gL[11] + gR[11]) * ny;
+        wn = 0.5 * (gL[12] + gR[12]) * nx + 0.5 * (gL[13] + gR[13]) * ny;
       }
       const T mul = rf * P.nu;
       const T sgk = F1f * sst::sk1 + (1.0 - F1f) * sst::sk2, sgw = F1f * sst::sw1 + (1.0 - F1f) * sst::sw2;
-      Fl[4] = Fl[4] - (mul + sgk * mut) * (kx * nx + ky * ny);
-      Fl[5] = Fl[5] - (mul + sgw * mut) * (wx_ * nx + wy_ * ny);
+      Fl[4] = Fl[4] - (mul + sgk * mut) * kn;
+      Fl[5] = Fl[5] - (mul + sgw * mut) * wn;
     } else {
       Fl[4] = Fl[4] - 0.0;
       Fl[5] = Fl[5] - 0.0;
@@ -828,7 +821,7 @@ template <int NV, class TV>
 __global__ __launch_bounds__(BS) void k_block(DMesh m, Phys P, const double* __restrict__ U,
                                                const double* __restrict__ G, const double* __restrict__ dt,
                                                TV* __restrict__ Binv, double* __restrict__ Bout,
-                                               TV* __restrict__ Aoff) {
+                                               TV* __restrict__ Aoff, const double* __restrict__ Q) {
   typedef Dual<double> D;
   const int c = xcd_block() * BS + threadIdx.x;
   if (c >= m.N) return;
@@ -843,7 +836,9 @@ __global__ __launch_bounds__(BS) void k_block(DMesh m, Phys P, const double* __r
   const S6<double> Uc = ld<NV>(U, p, c);
   const int4 s4 = m.slots[c];
   const int sl[4] = {s4.x, s4.y, s4.z, s4.w};
-  double visc = 0.0;
+  double visc[NV];
+#pragma unroll
+  for (int i = 0; i < NV; ++i) visc[i] = 0.0;
   for (int q = 0; q < 4; ++q) {
     const int e = sl[q];
     if (e < 0) continue;
@@ -869,16 +864,34 @@ __global__ __launch_bounds__(BS) void k_block(DMesh m, Phys P, const double* __r
 #pragma unroll
       for (int i = 0; i < NV; ++i) B[i][k] += sg * Fl[i].d;
     }
-    double vf = 0.0;
-    if (P.viscous) {
-      double d;
+    // viscous diagonal: the two-point diffusion coefficient of each row,
+    // (nu + nu_t) for momentum, gamma (nu / Pr + nu_t / Pr_t) for energy,
+    // (nu + sigma nu_t) for k and omega, times A over the normal distance
+    // (interior: between the centroids; no-slip wall: centroid to face)
+    double vf[NV];
+#pragma unroll
+    for (int i = 0; i < NV; ++i) vf[i] = 0.0;
+    if (P.viscous && (other >= 0 || g.tg == 4)) {
+      double d, nut = 0.0, F1 = 0.0;
       if (other >= 0) {
         d = fabs((m.xc[other] - m.xc[c]) * g.nx + (m.yc[other] - m.yc[c]) * g.ny);
+        if (Q) {
+          nut = (Q[c] + Q[other]) / (Uc[0] + Uo[0]);
+          F1 = 0.5 * (Q[p + c] + Q[p + other]);
+        }
       } else {
-        d = 2.0 * fabs((m.xm[f] - m.xc[c]) * g.nx + (m.ym[f] - m.yc[c]) * g.ny);
+        d = fabs((m.xm[f] - m.xc[c]) * g.nx + (m.ym[f] - m.yc[c]) * g.ny);
+        if (Q) F1 = Q[p + c];
       }
-      vf = P.nu * g.A / fmax(d, 1e-300);
-      visc += vf;
+      const double a = g.A / fmax(d, 1e-300);
+      vf[1] = vf[2] = (P.nu + nut) * a;
+      vf[3] = P.gamma * (P.nu / P.Pr + nut / sst::Prt) * a;
+      if constexpr (NV == 6) {
+        vf[4] = (P.nu + (F1 * sst::sk1 + (1.0 - F1) * sst::sk2) * nut) * a;
+        vf[5] = (P.nu + (F1 * sst::sw1 + (1.0 - F1) * sst::sw2) * nut) * a;
+      }
+#pragma unroll
+      for (int i = 0; i < NV; ++i) visc[i] += vf[i];
     }
     if (Aoff && other >= 0) {
       const double sg = as_nei ? -1.0 : 1.0;
@@ -893,7 +906,7 @@ __global__ __launch_bounds__(BS) void k_block(DMesh m, Phys P, const double* __r
                                 : flux_core<0, D>(P1, g, Ud, Od, nullptr, nullptr);
 #pragma unroll
         for (int i = 0; i < NV; ++i)
-          Aoff[(size_t)((q * NV + i) * NV + k) * p + c] = (TV)(sg * Fl[i].d - (i == k && i > 0 ? vf : 0.0));
+          Aoff[(size_t)((q * NV + i) * NV + k) * p + c] = (TV)(sg * Fl[i].d - (i == k ? vf[i] : 0.0));
       }
     }
   }
@@ -916,7 +929,7 @@ __global__ __launch_bounds__(BS) void k_block(DMesh m, Phys P, const double* __r
   }
   const double tc = m.vol[c] / (dt[c] + 1e-30);
 #pragma unroll
-  for (int i = 0; i < NV; ++i) B[i][i] += tc + (i > 0 ? visc : 0.0);
+  for (int i = 0; i < NV; ++i) B[i][i] += tc + visc[i];
   if (Bout) {
 #pragma unroll
     for (int i = 0; i < NV; ++i)
@@ -1734,10 +1747,12 @@ void launch_store_basis(cfd_ctx* c, void* Vj1, const double* w, const double* hd
 
 template <class TV> static void block_t(cfd_ctx* c, const double* U, TV* Binv, double* Bout, TV* Aoff) {
   NV_DISPATCH(c->P.nvar, {
+    const bool turb = NVt == 6 && c->P.turb;
     hipLaunchKernelGGL((k_gradlim<NVt, double>), dim3(nblocks(c->N)), dim3(BS), 0, c->stream, c->dm, c->phys, U,
-                       (const double*)nullptr, c->grad, (double*)nullptr, c->phi, 0);
+                       (const double*)nullptr, c->grad, (double*)nullptr, c->phi, 0,
+                       turb ? c->tq : (double*)nullptr, (double*)nullptr);
     hipLaunchKernelGGL((k_block<NVt, TV>), dim3(nblocks(c->N)), dim3(BS), 0, c->stream, c->dm, c->phys, U,
-                       (const double*)c->grad, c->dt, Binv, Bout, Aoff);
+                       (const double*)c->grad, c->dt, Binv, Bout, Aoff, turb ? (const double*)c->tq : nullptr);
   });
 }
 void launch_block_precond(cfd_ctx* c, const double* U, void* Binv, int prec, double* Bout, void* Aoff) {

@@ -693,39 +693,31 @@ template <class T> struct Disc {
         T wx = uf * txx + vf * txy - qx;
         T wy = uf * txy + vf * tyy - qy;
         FV[3] = wx * nx + wy * ny;
-        if (P.turb) {  // extension: (mu + sigma mu_t) grad(k, omega) . n
-          T kx, ky, wx_, wy_;
+        if (P.turb) {  // extension: (mu + sigma mu_t) dk/dn, domega/dn
+          // two-point normal differences (wall: to the wall values k = 0,
+          // omega_wall over the owner's normal distance; interior: between the
+          // centroids over their normal separation); averaged gradients only on
+          // the other boundary faces
+          T kn, wn;
           T kc = L[4] / L[0], wc = L[5] / L[0];
           if (n < 0 && tg == 4) {
-            T dx = G.xm[f] - G.xc[o], dy = G.ym[f] - G.yc[o];
-            T d2 = dx * dx + dy * dy;
+            T dn = sabs((G.xm[f] - G.xc[o]) * nx + (G.ym[f] - G.yc[o]) * ny);
             T ww = omega_wall(f);
-            kx = -kc * dx / d2;
-            ky = -kc * dy / d2;
-            wx_ = (ww - wc) * dx / d2;
-            wy_ = (ww - wc) * dy / d2;
+            kn = (0.0 - kc) / dn;
+            wn = (ww - wc) / dn;
+          } else if (n >= 0) {
+            T dn = sabs((G.xc[n] - G.xc[o]) * nx + (G.yc[n] - G.yc[o]) * ny);
+            if (re(dn) < 1e-300) dn = 1e-300;
+            kn = ((R[4] / R[0]) - kc) / dn;
+            wn = ((R[5] / R[0]) - wc) / dn;
           } else {
-            kx = 0.5 * (gL[10] + gR[10]);
-            ky = 0.5 * (gL[11] + gR[11]);
-            wx_ = 0.5 * (gL[12] + gR[12]);
-            wy_ = 0.5 * (gL[13] + gR[13]);
-            if (n >= 0) {
-              T dxLR = G.xc[n] - G.xc[o], dyLR = G.yc[n] - G.yc[o];
-              T d2 = dxLR * dxLR + dyLR * dyLR;
-              if (re(d2) > 1e-14) {
-                T dk = (R[4] / R[0]) - kc, dw = (R[5] / R[0]) - wc;
-                T kx0 = kx, ky0 = ky, wx0 = wx_, wy0 = wy_;
-                kx = kx0 - (kx0 * dxLR + ky0 * dyLR - dk) * dxLR / d2;
-                ky = ky0 - (kx0 * dxLR + ky0 * dyLR - dk) * dyLR / d2;
-                wx_ = wx0 - (wx0 * dxLR + wy0 * dyLR - dw) * dxLR / d2;
-                wy_ = wy0 - (wx0 * dxLR + wy0 * dyLR - dw) * dyLR / d2;
-              }
-            }
+            kn = 0.5 * (gL[10] + gR[10]) * nx + 0.5 * (gL[11] + gR[11]) * ny;
+            wn = 0.5 * (gL[12] + gR[12]) * nx + 0.5 * (gL[13] + gR[13]) * ny;
           }
           T mul = rf * P.nu;
           T sgk = F1f * sk1 + (1.0 - F1f) * sk2, sgw = F1f * sw1 + (1.0 - F1f) * sw2;
-          FV[4] = (mul + sgk * mut) * (kx * nx + ky * ny);
-          FV[5] = (mul + sgw * mut) * (wx_ * nx + wy_ * ny);
+          FV[4] = (mul + sgk * mut) * kn;
+          FV[5] = (mul + sgw * mut) * wn;
         }
         Tot = FE - FV;
       }
