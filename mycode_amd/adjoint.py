@@ -70,7 +70,7 @@ def design_modes(mesh: cfd.Mesh, n_design: int = 64) -> np.ndarray:
     return modes
 
 
-def adjoint_solve(solver: cfd.Solver, U, rhs, tol=1e-10, m=40, restarts=60, precond=1):
+def adjoint_solve(solver: cfd.Solver, U, rhs, tol=1e-10, m=60, restarts=40, precond=2):
     """psi with (dR/dU)^T psi = rhs at the steady state (pseudo-time term removed)."""
     solver.compute_residuals(U, True)
     solver.compute_local_time_step(U, 1e30)
@@ -103,9 +103,14 @@ def design_gradient(mesh: cfd.Mesh, solver: cfd.Solver, p: cfd.Params, U, modes,
     return grad, psi, it
 
 
-def converge(solver: cfd.Solver, U0, rel_drop=1e-12, max_steps=400, jv=1, precision=64):
-    """Steady state by JFNK with the SER controller and block-Jacobi."""
-    g = solver.gmres_opts(m=40, tol=1e-4, jv=jv, precision=precision, precond=1, restarts=2)
-    o = solver.newton_opts(gmres=g, tolerance=0.0, rel_drop=rel_drop, max_steps=max_steps, controller=1,
-                           cfl0=1.0, cfl_max=1e12)
+def converge(solver: cfd.Solver, U0, rel_drop=1e-12, max_steps=400, jv=1, precision=64, tolerance=0.0,
+             cfl0=1.0):
+    """Steady state by JFNK with the SER controller and the block SGS preconditioner.
+
+    Stops at a drop of `rel_drop` of the volume-weighted residual norm or below
+    the absolute `tolerance` (a re-solve from a nearby converged state needs
+    the absolute test: its own initial residual is already small)."""
+    g = solver.gmres_opts(m=60, tol=1e-4, jv=jv, precision=precision, precond=2, restarts=2)
+    o = solver.newton_opts(gmres=g, tolerance=tolerance, rel_drop=rel_drop, max_steps=max_steps, controller=1,
+                           cfl0=cfl0, cfl_max=1e12)
     return solver.solve_implicit(U0, o)
