@@ -983,8 +983,12 @@ int cfd_solve_implicit(cfd_ctx* c, double* U6, const cfd_newton_opts* opts, cfd_
   double* dU = c->vec[2];
   int rc = upload_state(c, U6, U);
   if (rc) return rc;
+  if (o.controller < 0 || o.controller > 2) return fail(c, CFD_ERR_ARG, "controller must be 0, 1 or 2");
   double cfl = o.cfl0 > 0 ? o.cfl0 : 0.1;
-  double vm0 = -1.0, vm = 0.0;
+  double vm0 = -1.0, vm = 0.0, vm_prev = -1.0;
+  bool gmres_ok = true;
+  double* Ubak = c->vec[3];  // adaptive controller: state before the last update
+  const size_t vs = (size_t)c->P.nvar * c->pitch;
   int step = 0, conv = 0, total_it = 0;
   for (step = 0; step < o.max_steps; ++step) {
     launch_timestep(c, U, cfl);
@@ -994,6 +998,23 @@ int cfd_solve_implicit(cfd_ctx* c, double* U6, const cfd_newton_opts* opts, cfd_
     double s[2];
     if ((rc = read_scal(c, S_VW, 2, s))) return rc;
     vm = s[1] == 0 ? 0.0 : std::sqrt(s[0] / s[1]);
+    if (o.controller == 2 && vm_prev > 0) {
+      if (!(vm == vm) || vm > 10.0 * vm_prev) {  // reject: back to the previous state, smaller step
+        HIP_TRY(c, hipMemcpyAsync(U, Ubak, vs * 8, hipMemcpyDeviceToDevice, c->stream));
+        cfl *= 0.1;
+        const double cmin = 1e-3 * (o.cfl0 > 0 ? o.cfl0 : 0.1);
+        if (cfl < cmin) cfl = cmin;
+        launch_timestep(c, U, cfl);
+        launch_residual(c, U, R, upd);
+        vm = vm_prev;
+      } else {
+        const double cmax = o.cfl_max > 0 ? o.cfl_max : 1e12;
+        if (vm > vm_prev) cfl *= 0.5;
+        else if (gmres_ok) cfl *= 2.0;
+        if (cfl > cmax) cfl = cmax;
+        launch_timestep(c, U, cfl);
+      }
+    }
     if (vm0 < 0) vm0 = vm;
     if (hist_vm) hist_vm[step] = vm;
     if (hist_cfl) hist_cfl[step] = cfl;
@@ -1021,6 +1042,11 @@ int cfd_solve_implicit(cfd_ctx* c, double* U6, const cfd_newton_opts* opts, cfd_
         if (cfl > 10000) cfl = 10000;
         if (cfl < 0.1) cfl = 0.1;
       }
+    } else if (o.controller == 2) {
+      gmres_ok = c->kry.last_rel <= 1.5 * o.gmres.tol;
+      vm_prev = vm;
+      HIP_TRY(c, hipMemcpyAsync(Ubak, U, vs * 8, hipMemcpyDeviceToDevice, c->stream));
+      launch_newton_update(c, U, dU, 1.0);
     } else {  // switched evolution relaxation, backed off when GMRES misses its tolerance
       launch_newton_update(c, U, dU, 1.0);
       const double cmax = o.cfl_max > 0 ? o.cfl_max : 1e12;

@@ -464,7 +464,9 @@ __global__ __launch_bounds__(BS) void k_flux(DMesh m, Phys P, const double* __re
 }
 
 template <int NV, class F>
-__global__ __launch_bounds__(BS) void k_gather(DMesh m, const F* __restrict__ Fm, F* __restrict__ R) {
+__global__ __launch_bounds__(BS) void k_gather(DMesh m, const F* __restrict__ Fm, F* __restrict__ R,
+                                                const F* __restrict__ Q = nullptr, const F* __restrict__ v = nullptr,
+                                                const double* __restrict__ dt = nullptr) {
   const int c = xcd_block() * BS + threadIdx.x;
   if (c >= m.N) return;
   const int4 s4 = m.slots[c];
@@ -485,8 +487,20 @@ __global__ __launch_bounds__(BS) void k_gather(DMesh m, const F* __restrict__ Fm
       for (int k = 0; k < NV; ++k) r[k] = r[k] + Fm[(size_t)k * m.F + f];
     }
   }
+  if constexpr (NV == 6) {
+    if (Q) {  // SST sources: R(k, omega) -= V S (rows 2 and 3 of the per-cell turbulence quantities)
+      r[4] = r[4] - Q[(size_t)2 * m.pitch + c];
+      r[5] = r[5] - Q[(size_t)3 * m.pitch + c];
+    }
+  }
+  if (v) {  // tangent: + (vol / dt) v  (Solver.cpp:1381-1384 time term)
+    const double tc = m.vol[c] / (dt[c] + 1e-30);
 #pragma unroll
-  for (int k = 0; k < NV; ++k) R[k * m.pitch + c] = r[k];
+    for (int k = 0; k < NV; ++k) R[k * m.pitch + c] = (F)((double)r[k] + (double)v[k * m.pitch + c] * tc);
+  } else {
+#pragma unroll
+    for (int k = 0; k < NV; ++k) R[k * m.pitch + c] = r[k];
+  }
 }
 
 // Tangent Jv: J v = dR/dU v + (vol / dt) v  (Solver.cpp:1381-1384 time term)
@@ -1476,9 +1490,11 @@ static void residual_nv(cfd_ctx* c, const double* U, double* R, bool upd, hipEve
                        (const double*)nullptr, c->grad, (const double*)nullptr, c->phi, c->Fm, 0, m.F,
                        (const double*)c->tq, (const double*)nullptr);
     if (ev) (void)hipEventRecord(ev[3], c->stream);
-    hipLaunchKernelGGL((k_gather<NV, double>), dim3(nblocks(m.N)), dim3(BS), 0, c->stream, m, c->Fm, R);
+    hipLaunchKernelGGL((k_gather<NV, double>), dim3(nblocks(m.N)), dim3(BS), 0, c->stream, m, c->Fm, R,
+                       NV == 6 && P.turb ? (const double*)c->tq : nullptr, (const double*)nullptr,
+                       (const double*)nullptr);
   }
-  if (NV == 6 && P.turb)
+  if (c->assembly == 0 && NV == 6 && P.turb)
     hipLaunchKernelGGL((k_turb_src<double>), dim3(nblocks(m.N)), dim3(BS), 0, c->stream, m.N, m.pitch,
                        (const double*)c->tq, R);
   if (ev) (void)hipEventRecord(ev[4], c->stream);
@@ -1509,7 +1525,9 @@ static void tangent_nv(cfd_ctx* c, const double* U, const F* v, F* Jv) {
     F* Fm = (F*)c->Fm;
     hipLaunchKernelGGL((k_flux<NV, VISC, 1, T>), dim3(nblocks(m.F)), dim3(BS), 0, c->stream, m, P, U, v,
                        (const F*)Gv, (const F*)Gd, c->phi, Fm, 0, m.F, (const F*)Qv, (const F*)Qd);
-    hipLaunchKernelGGL((k_gather<NV, F>), dim3(nblocks(m.N)), dim3(BS), 0, c->stream, m, (const F*)Fm, Jv);
+    hipLaunchKernelGGL((k_gather<NV, F>), dim3(nblocks(m.N)), dim3(BS), 0, c->stream, m, (const F*)Fm, Jv,
+                       NV == 6 && P.turb ? (const F*)Qd : nullptr, v, (const double*)c->dt);
+    return;
   }
   if (NV == 6 && P.turb)
     hipLaunchKernelGGL((k_turb_src<F>), dim3(nblocks(m.N)), dim3(BS), 0, c->stream, m.N, m.pitch, (const F*)Qd, Jv);
