@@ -999,21 +999,23 @@ int cfd_solve_implicit(cfd_ctx* c, double* U6, const cfd_newton_opts* opts, cfd_
     if ((rc = read_scal(c, S_VW, 2, s))) return rc;
     vm = s[1] == 0 ? 0.0 : std::sqrt(s[0] / s[1]);
     if (o.controller == 2 && vm_prev > 0) {
+      const double cmin = 1e-2 * (o.cfl0 > 0 ? o.cfl0 : 0.1);
+      const double cmax = o.cfl_max > 0 ? o.cfl_max : 1e12;
       if (!(vm == vm) || vm > 10.0 * vm_prev) {  // reject: back to the previous state, smaller step
         HIP_TRY(c, hipMemcpyAsync(U, Ubak, vs * 8, hipMemcpyDeviceToDevice, c->stream));
         cfl *= 0.1;
-        const double cmin = 1e-3 * (o.cfl0 > 0 ? o.cfl0 : 0.1);
-        if (cfl < cmin) cfl = cmin;
-        launch_timestep(c, U, cfl);
+        launch_timestep(c, U, std::max(cfl, cmin));
         launch_residual(c, U, R, upd);
         vm = vm_prev;
-      } else {
-        const double cmax = o.cfl_max > 0 ? o.cfl_max : 1e12;
-        if (vm > vm_prev) cfl *= 0.5;
-        else if (gmres_ok) cfl *= 2.0;
-        if (cfl > cmax) cfl = cmax;
-        launch_timestep(c, U, cfl);
-      }
+      } else if (vm > 1.2 * vm_prev) {
+        cfl *= 0.5;
+      } else if (vm <= 0.95 * vm_prev) {
+        cfl *= gmres_ok ? 2.0 : 1.25;
+      } else if (!gmres_ok) {  // stalled with an unconverged linear solve: an easier system
+        cfl *= 0.7;
+      }  // a stall or small rise with GMRES at its tolerance holds the CFL
+      cfl = std::min(std::max(cfl, cmin), cmax);
+      launch_timestep(c, U, cfl);
     }
     if (vm0 < 0) vm0 = vm;
     if (hist_vm) hist_vm[step] = vm;

@@ -145,9 +145,10 @@ __global__ __launch_bounds__(BS) void k_gradlim(DMesh m, Phys P, const double* _
   const double vol = m.vol[c];
 #pragma unroll
   for (int q = 0; q < 2 * NQM; ++q) {
-    if (q >= 2 * nq) break;
-    g[q] = g[q] / vol;
-    Sc::put(Gv, Gd, (size_t)q * p + c, g[q]);
+    if (q < 2 * nq) {
+      g[q] = g[q] / vol;
+      Sc::put(Gv, Gd, (size_t)q * p + c, g[q]);
+    }
   }
   if constexpr (NQM == 7) {
     if (turb && Qv) {
