@@ -248,7 +248,7 @@ typedef struct cfd_newton_opts {
   int controller;       /* 0: the reference CFL controller (Solver.cpp:1747-1796);
                            1: switched evolution relaxation, CFL = cfl0 (VM_0/VM_k)^1
                               clamped to [cfl_min, cfl_max] (no reference counterpart);
-                           2: adaptive: after a step that lowered VM by 5 % or more,
+                           2: adaptive: after a step that lowered VM by 1 % or more,
                               x2 (GMRES at its tolerance) or x1.25 (GMRES missed it);
                               after a smaller change up to a 20 % rise, held, or x0.7
                               when GMRES missed its tolerance; x0.5 after a larger rise;

@@ -96,6 +96,7 @@ void free_krylov(cfd_ctx* c) {
   (void)hipFree(k.D);
   (void)hipFree(k.blk);
   (void)hipFree(k.aoff);
+  (void)hipFree(k.fjac);
   (void)hipFree(k.xt);
   (void)hipFree(k.z);
   (void)hipFree(k.w);
@@ -115,6 +116,7 @@ int ensure_krylov(cfd_ctx* c, int m, int prec) {
   HIP_TRY(c, hipMalloc(&k.V, (size_t)(m + 1) * vs * es));
   HIP_TRY(c, hipMalloc((void**)&k.D, (size_t)c->pitch * 8));
   HIP_TRY(c, hipMalloc(&k.blk, (size_t)c->P.nvar * c->P.nvar * c->pitch * es));
+  HIP_TRY(c, hipMalloc((void**)&k.fjac, (size_t)2 * c->P.nvar * c->P.nvar * std::max(c->F, 1) * 8));
   HIP_TRY(c, hipMalloc((void**)&k.z, vs * 8));
   HIP_TRY(c, hipMalloc((void**)&k.w, vs * 8));
   HIP_TRY(c, hipMalloc((void**)&k.rr, vs * 8));
@@ -452,6 +454,10 @@ int cfd_ctx_create(const cfd_mesh* m, const cfd_params* p, cfd_ctx** out) {
     HIP_TRY(c, dalloc(&c->d_ccol, P_));
     HIP_TRY(c, dalloc(&c->d_ccells, std::max(N, 1)));
     HIP_TRY(c, dalloc(&c->d_nq, P_));
+    std::vector<int> cpos(P_, 0);
+    for (int t = 0; t < N; ++t) cpos[m->ccells[t]] = t;
+    HIP_TRY(c, dalloc(&c->d_cpos, P_));
+    HIP_TRY(c, hipMemcpy(c->d_cpos, cpos.data(), (size_t)P_ * 4, hipMemcpyHostToDevice));
     HIP_TRY(c, hipMemcpy(c->d_ccol, col.data(), P_, hipMemcpyHostToDevice));
     HIP_TRY(c, hipMemcpy(c->d_ccells, m->ccells.data(), (size_t)N * 4, hipMemcpyHostToDevice));
     HIP_TRY(c, hipMemcpy(c->d_nq, nq.data(), (size_t)P_ * sizeof(int4), hipMemcpyHostToDevice));
@@ -488,7 +494,7 @@ int cfd_ctx_destroy(cfd_ctx* c) {
   (void)hipStreamSynchronize(c->stream);
   free_krylov(c);
   void* bufs[] = {c->d_vol, c->d_xc,  c->d_yc,  c->d_slots, c->d_own, c->d_nei, c->d_tag,  c->d_nx,
-                  c->d_ny,  c->d_area, c->d_xm, c->d_ym,    c->d_iperm, c->d_facemap, c->d_alt, c->grad, c->d_wd, c->d_dnw, c->d_corder, c->d_ccol, c->d_ccells, c->d_nq,
+                  c->d_ny,  c->d_area, c->d_xm, c->d_ym,    c->d_iperm, c->d_facemap, c->d_alt, c->grad, c->d_wd, c->d_dnw, c->d_corder, c->d_ccol, c->d_ccells, c->d_nq, c->d_cpos,
                   c->phi,   c->dt,    c->Fm,    c->red,     c->scal, c->gradT, c->adj, c->tq, c->io6};
   for (void* b : bufs) (void)hipFree(b);
   for (int i = 0; i < 6; ++i) (void)hipFree(c->vec[i]);
@@ -715,14 +721,16 @@ int cfd_block_sgs(cfd_ctx* c, const double* U6, int precision, int transpose, co
     if (e != hipSuccess) return fail(c, CFD_ERR_HIP, hipGetErrorString(e));
     if (precision == 32)
       for (size_t i = 0; i < hAf.size(); ++i) hA[i] = hAf[i];
+    std::vector<int> cpos(c->N);
+    for (int t = 0; t < c->N; ++t) cpos[c->mesh->ccells[t]] = t;
     for (int i = 0; i < c->N; ++i) {
-      const int r = c->iperm[i];
+      const int r = c->iperm[i], b = cpos[i];  // blocks are stored color-major
       for (int q = 0; q < 4; ++q) {
         const int e4 = (&hq[i].x)[q];
         if (nbr) nbr[(size_t)r * 4 + q] = e4 < 0 ? -1 : c->iperm[e4 >> 2];
         if (!Aoff) continue;
         for (int k = 0; k < nv * nv; ++k)
-          Aoff[((size_t)r * 4 + q) * nv * nv + k] = e4 < 0 ? 0.0 : hA[(size_t)(q * nv * nv + k) * P_ + i];
+          Aoff[((size_t)r * 4 + q) * nv * nv + k] = e4 < 0 ? 0.0 : hA[(size_t)(q * nv * nv + k) * P_ + b];
       }
     }
   }
@@ -1009,7 +1017,7 @@ int cfd_solve_implicit(cfd_ctx* c, double* U6, const cfd_newton_opts* opts, cfd_
         vm = vm_prev;
       } else if (vm > 1.2 * vm_prev) {
         cfl *= 0.5;
-      } else if (vm <= 0.95 * vm_prev) {
+      } else if (vm <= 0.99 * vm_prev) {
         cfl *= gmres_ok ? 2.0 : 1.25;
       } else if (!gmres_ok) {  // stalled with an unconverged linear solve: an easier system
         cfl *= 0.7;

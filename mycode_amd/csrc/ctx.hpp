@@ -35,6 +35,7 @@ struct Krylov {
   void* blk = nullptr;         // block-Jacobi inverses (nv x nv per cell)
   void* aoff = nullptr;        // neighbor coupling blocks (4 x nv x nv per cell), block SGS only
   double* xt = nullptr;        // basis combination before the block SGS solve (FP64)
+  double* fjac = nullptr;      // face Jacobians of the first-order flux (2 nv^2 x F), block preconditioners
   double* z = nullptr;         // preconditioned direction (FP64)
   double* w = nullptr;         // accumulated solution of restarted GMRES (FP64)
   double* rr = nullptr;        // true residual of a restart cycle (FP64)
@@ -70,6 +71,7 @@ struct cfd_ctx {
   unsigned char* d_alt = nullptr;
   unsigned char* d_ccol = nullptr;  // per cell: color of the distance-1 cell coloring
   int* d_ccells = nullptr;          // cells grouped by color
+  int* d_cpos = nullptr;            // per cell: its position in d_ccells (column of its SGS blocks)
   int4* d_nq = nullptr;             // per cell slot: (neighbor << 2) | back slot, or -1
   // solver state
   double* grad = nullptr;   // 14 x pitch (rho u v p T gradients; k, omega with SST)
@@ -134,6 +136,7 @@ void launch_store_basis(cfd_ctx* c, void* Vj1, const double* w, const double* hd
 // x = D^-1 sum_{i<=k} y_i V_i (diag) or the plain combination
 void launch_solution(cfd_ctx* c, double* x, const void* V, const double* y, int k, int prec, bool diag = true);
 // block-Jacobi: assemble + invert (Binv: nv*nv x pitch of the basis type), apply, solution
+// with Aoff (block SGS) both Binv and Aoff are stored color-major (column = d_cpos[cell])
 void launch_block_precond(cfd_ctx* c, const double* U, void* Binv, int prec, double* Bout = nullptr,
                           void* Aoff = nullptr);
 // multicolor block SGS: z = M^-1 x (M^-T with transpose); blocks of precision bprec, x/z of xprec/zprec

@@ -832,17 +832,52 @@ __global__ __launch_bounds__(BS) void k_diagprec(DMesh m, Phys P, const double* 
 // the same first-order Jacobian to the interior neighbors are stored too,
 // A_{c,q} = (+-) A dF/dU_other - nu A / d I for slot q of cell c, in rows
 // (q * NV + i) * NV + j of pitch (slots without a neighbor are not written).
+// Face Jacobians of the first-order inviscid flux (times A): dF/dU_L and, on
+// interior faces, dF/dU_R, one forward-dual pass per direction, stored as
+// Jf[(s * NV * NV + i * NV + k) * F + f] for s = 0 (L) and 1 (R).  On a
+// boundary face the ghost state is differentiated with U_L.  Each face is
+// differentiated once; the cell kernel below assembles both of its cells.
+template <int NV>
+__global__ __launch_bounds__(BS) void k_face_jac(DMesh m, Phys P, const double* __restrict__ U,
+                                                  double* __restrict__ Jf) {
+  typedef Dual<double> D;
+  const int f = xcd_block() * BS + threadIdx.x;
+  if (f >= m.F) return;
+  const int p = m.pitch;
+  Phys P1 = P;
+  P1.spatial = CFD_FIRST_ORDER;
+  const int o = m.own[f], n = m.nei[f];
+  const FaceGeo g = face_geo(m, P1, nullptr, f, o, n, false);
+  const S6<double> UL = ld<NV>(U, p, o);
+  const S6<double> UR = n >= 0 ? ld<NV>(U, p, n) : UL;
+  const int nside = n >= 0 ? 2 : 1;
+#pragma unroll 1
+  for (int sk = 0; sk < NV * nside; ++sk) {
+    const int side = sk / NV, k = sk - side * NV;
+    S6<D> Ld, Rd;
+#pragma unroll
+    for (int i = 0; i < 6; ++i) {
+      Ld[i] = D(UL[i], side == 0 && i == k ? 1.0 : 0.0);
+      Rd[i] = D(UR[i], side == 1 && i == k ? 1.0 : 0.0);
+    }
+    if (n < 0) Rd = ghost(P1, Ld, g.tg, g.nx, g.ny, g.dnw);
+    const S6<D> Fl = flux_core<0, D>(P1, g, Ld, Rd, nullptr, nullptr);
+#pragma unroll
+    for (int i = 0; i < NV; ++i) Jf[(size_t)(side * NV * NV + i * NV + k) * m.F + f] = Fl[i].d;
+  }
+}
+
 template <int NV, class TV>
 __global__ __launch_bounds__(BS) void k_block(DMesh m, Phys P, const double* __restrict__ U,
                                                const double* __restrict__ G, const double* __restrict__ dt,
-                                               TV* __restrict__ Binv, double* __restrict__ Bout,
-                                               TV* __restrict__ Aoff, const double* __restrict__ Q) {
+                                               const double* __restrict__ Jf, TV* __restrict__ Binv,
+                                               double* __restrict__ Bout, TV* __restrict__ Aoff,
+                                               const double* __restrict__ Q, const int* __restrict__ pos) {
   typedef Dual<double> D;
   const int c = xcd_block() * BS + threadIdx.x;
   if (c >= m.N) return;
   const int p = m.pitch;
-  Phys P1 = P;  // first-order inviscid flux for the blocks
-  P1.spatial = CFD_FIRST_ORDER;
+  const int oc = pos ? pos[c] : c;  // output column: color-major position for the block SGS
   double B[NV][NV];
 #pragma unroll
   for (int i = 0; i < NV; ++i)
@@ -860,25 +895,15 @@ __global__ __launch_bounds__(BS) void k_block(DMesh m, Phys P, const double* __r
     const int f = e >> 1;
     const bool as_nei = e & 1;
     const int o = m.own[f], n = m.nei[f];
-    const FaceGeo g = face_geo(m, P1, nullptr, f, o, n, false);
     const int other = as_nei ? o : n;
-    const S6<double> Uo = other >= 0 ? ld<NV>(U, p, other) : Uc;
+    // this cell's side of the face Jacobian, with the sign of its residual
+    // (R_o += F A, R_n -= F A); the other side couples to the neighbor
+    const double sg = as_nei ? -1.0 : 1.0;
+    const double* Js = Jf + (size_t)(as_nei ? NV * NV : 0) * m.F + f;
 #pragma unroll
-    for (int k = 0; k < NV; ++k) {
-      S6<D> Ud, Od;
+    for (int i = 0; i < NV; ++i)
 #pragma unroll
-      for (int i = 0; i < 6; ++i) {
-        Ud[i] = D(Uc[i], i == k ? 1.0 : 0.0);
-        Od[i] = D(Uo[i], 0.0);
-      }
-      S6<D> Fl;
-      if (n < 0) Fl = flux_core<0, D>(P1, g, Ud, ghost(P1, Ud, g.tg, g.nx, g.ny, g.dnw), nullptr, nullptr);
-      else if (!as_nei) Fl = flux_core<0, D>(P1, g, Ud, Od, nullptr, nullptr);
-      else Fl = flux_core<0, D>(P1, g, Od, Ud, nullptr, nullptr);
-      const double sg = as_nei ? -1.0 : 1.0;  // flux_core returns F * A
-#pragma unroll
-      for (int i = 0; i < NV; ++i) B[i][k] += sg * Fl[i].d;
-    }
+      for (int k = 0; k < NV; ++k) B[i][k] += sg * Js[(size_t)(i * NV + k) * m.F];
     // viscous diagonal: the two-point diffusion coefficient of each row,
     // (nu + nu_t) for momentum, gamma (nu / Pr + nu_t / Pr_t) for energy,
     // (nu + sigma nu_t) for k and omega, times A over the normal distance
@@ -886,19 +911,20 @@ __global__ __launch_bounds__(BS) void k_block(DMesh m, Phys P, const double* __r
     double vf[NV];
 #pragma unroll
     for (int i = 0; i < NV; ++i) vf[i] = 0.0;
-    if (P.viscous && (other >= 0 || g.tg == 4)) {
+    if (P.viscous && (other >= 0 || m.tag[f] == 4)) {
+      const double nx = m.nx[f], ny = m.ny[f];
       double d, nut = 0.0, F1 = 0.0;
       if (other >= 0) {
-        d = fabs((m.xc[other] - m.xc[c]) * g.nx + (m.yc[other] - m.yc[c]) * g.ny);
+        d = fabs((m.xc[other] - m.xc[c]) * nx + (m.yc[other] - m.yc[c]) * ny);
         if (Q) {
-          nut = (Q[c] + Q[other]) / (Uc[0] + Uo[0]);
+          nut = (Q[c] + Q[other]) / (Uc[0] + U[other]);
           F1 = 0.5 * (Q[p + c] + Q[p + other]);
         }
       } else {
-        d = fabs((m.xm[f] - m.xc[c]) * g.nx + (m.ym[f] - m.yc[c]) * g.ny);
+        d = fabs((m.xm[f] - m.xc[c]) * nx + (m.ym[f] - m.yc[c]) * ny);
         if (Q) F1 = Q[p + c];
       }
-      const double a = g.A / fmax(d, 1e-300);
+      const double a = m.area[f] / fmax(d, 1e-300);
       vf[1] = vf[2] = (P.nu + nut) * a;
       vf[3] = P.gamma * (P.nu / P.Pr + nut / sst::Prt) * a;
       if constexpr (NV == 6) {
@@ -909,20 +935,13 @@ __global__ __launch_bounds__(BS) void k_block(DMesh m, Phys P, const double* __r
       for (int i = 0; i < NV; ++i) visc[i] += vf[i];
     }
     if (Aoff && other >= 0) {
-      const double sg = as_nei ? -1.0 : 1.0;
-      for (int k = 0; k < NV; ++k) {
-        S6<D> Ud, Od;
+      const double* Jo = Jf + (size_t)(as_nei ? 0 : NV * NV) * m.F + f;
 #pragma unroll
-        for (int i = 0; i < 6; ++i) {
-          Ud[i] = D(Uc[i], 0.0);
-          Od[i] = D(Uo[i], i == k ? 1.0 : 0.0);
-        }
-        const S6<D> Fl = as_nei ? flux_core<0, D>(P1, g, Od, Ud, nullptr, nullptr)
-                                : flux_core<0, D>(P1, g, Ud, Od, nullptr, nullptr);
+      for (int i = 0; i < NV; ++i)
 #pragma unroll
-        for (int i = 0; i < NV; ++i)
-          Aoff[(size_t)((q * NV + i) * NV + k) * p + c] = (TV)(sg * Fl[i].d - (i == k ? vf[i] : 0.0));
-      }
+        for (int k = 0; k < NV; ++k)
+          Aoff[(size_t)((q * NV + i) * NV + k) * p + oc] =
+              (TV)(sg * Jo[(size_t)(i * NV + k) * m.F] - (i == k ? vf[i] : 0.0));
     }
   }
   if constexpr (NV == 6) {
@@ -957,32 +976,43 @@ __global__ __launch_bounds__(BS) void k_block(DMesh m, Phys P, const double* __r
   for (int i = 0; i < NV; ++i)
 #pragma unroll
     for (int j = 0; j < NV; ++j) Inv[i][j] = i == j ? 1.0 : 0.0;
+  // fully unrolled with the row swap done by compare-and-select, so that
+  // every index is static and both matrices stay in registers
+#pragma unroll
   for (int col = 0; col < NV; ++col) {
     int piv = col;
     double best = fabs(B[col][col]);
+#pragma unroll
     for (int r = col + 1; r < NV; ++r)
       if (fabs(B[r][col]) > best) {
         best = fabs(B[r][col]);
         piv = r;
       }
-    if (piv != col) {
-      for (int j = 0; j < NV; ++j) {
-        double t = B[col][j];
-        B[col][j] = B[piv][j];
-        B[piv][j] = t;
-        t = Inv[col][j];
-        Inv[col][j] = Inv[piv][j];
-        Inv[piv][j] = t;
+#pragma unroll
+    for (int r = col + 1; r < NV; ++r) {
+      if (piv == r) {
+#pragma unroll
+        for (int j = 0; j < NV; ++j) {
+          double t = B[col][j];
+          B[col][j] = B[r][j];
+          B[r][j] = t;
+          t = Inv[col][j];
+          Inv[col][j] = Inv[r][j];
+          Inv[r][j] = t;
+        }
       }
     }
     const double ip = 1.0 / B[col][col];
+#pragma unroll
     for (int j = 0; j < NV; ++j) {
       B[col][j] *= ip;
       Inv[col][j] *= ip;
     }
+#pragma unroll
     for (int r = 0; r < NV; ++r) {
       if (r == col) continue;
       const double fct = B[r][col];
+#pragma unroll
       for (int j = 0; j < NV; ++j) {
         B[r][j] -= fct * B[col][j];
         Inv[r][j] -= fct * Inv[col][j];
@@ -992,7 +1022,7 @@ __global__ __launch_bounds__(BS) void k_block(DMesh m, Phys P, const double* __r
 #pragma unroll
   for (int i = 0; i < NV; ++i)
 #pragma unroll
-    for (int j = 0; j < NV; ++j) Binv[(size_t)(i * NV + j) * p + c] = (TV)Inv[i][j];
+    for (int j = 0; j < NV; ++j) Binv[(size_t)(i * NV + j) * p + oc] = (TV)Inv[i][j];
 }
 
 // z = Binv_c x_c (or Binv_c^T x_c for the transposed system) per cell
@@ -1047,11 +1077,15 @@ __global__ __launch_bounds__(BS) void k_solution_bj(int N, int p, size_t vstride
 // neighbor that points back to c.  nq[c] holds (neighbor << 2) | q' per slot,
 // -1 where the slot has no interior neighbor.
 
+// The blocks are stored color-major: the blocks of the cell at position t of
+// the color-grouped cell list are in column t, so that a color's launch reads
+// its own contiguous range instead of every cache line of the arrays.
+
 // first launch: z = D^-1 x on color 0, zero elsewhere
 template <int NV, class TB, class TX, class TZ, bool TR>
 __global__ __launch_bounds__(BS) void k_sgs_first(int N, int p, const unsigned char* __restrict__ col,
-                                                   const TB* __restrict__ Binv, const TX* __restrict__ x,
-                                                   TZ* __restrict__ z) {
+                                                   const int* __restrict__ pos, const TB* __restrict__ Binv,
+                                                   const TX* __restrict__ x, TZ* __restrict__ z) {
   const int c = xcd_block() * BS + threadIdx.x;
   if (c >= N) return;
   if (col[c] != 0) {
@@ -1059,6 +1093,7 @@ __global__ __launch_bounds__(BS) void k_sgs_first(int N, int p, const unsigned c
     for (int i = 0; i < NV; ++i) z[i * p + c] = (TZ)0.0;
     return;
   }
+  const int b = pos[c];
   double xv[NV];
 #pragma unroll
   for (int j = 0; j < NV; ++j) xv[j] = (double)x[j * p + c];
@@ -1066,19 +1101,21 @@ __global__ __launch_bounds__(BS) void k_sgs_first(int N, int p, const unsigned c
   for (int i = 0; i < NV; ++i) {
     double s = 0.0;
 #pragma unroll
-    for (int j = 0; j < NV; ++j) s += (double)Binv[(size_t)(TR ? j * NV + i : i * NV + j) * p + c] * xv[j];
+    for (int j = 0; j < NV; ++j) s += (double)Binv[(size_t)(TR ? j * NV + i : i * NV + j) * p + b] * xv[j];
     z[i * p + c] = (TZ)s;
   }
 }
 
+// cells [b0, b0 + n) of the color-grouped list
 template <int NV, class TB, class TX, class TZ, bool TR>
-__global__ __launch_bounds__(BS) void k_sgs_color(int n, const int* __restrict__ cells, int p,
-                                                   const int4* __restrict__ nq, const TB* __restrict__ Binv,
-                                                   const TB* __restrict__ Aoff, const TX* __restrict__ x,
-                                                   TZ* __restrict__ z) {
+__global__ __launch_bounds__(BS) void k_sgs_color(int n, int b0, const int* __restrict__ cells, int p,
+                                                   const int4* __restrict__ nq, const int* __restrict__ pos,
+                                                   const TB* __restrict__ Binv, const TB* __restrict__ Aoff,
+                                                   const TX* __restrict__ x, TZ* __restrict__ z) {
   const int t = xcd_block() * BS + threadIdx.x;
   if (t >= n) return;
-  const int c = cells[t];
+  const int b = b0 + t;
+  const int c = cells[b];
   double s[NV];
 #pragma unroll
   for (int i = 0; i < NV; ++i) s[i] = (double)x[i * p + c];
@@ -1093,23 +1130,23 @@ __global__ __launch_bounds__(BS) void k_sgs_color(int n, const int* __restrict__
 #pragma unroll
     for (int j = 0; j < NV; ++j) zo[j] = (double)z[j * p + o];
     if (TR) {
-      const int r = e & 3;
+      const int r = e & 3, bo = pos[o];
 #pragma unroll
       for (int i = 0; i < NV; ++i)
 #pragma unroll
-        for (int j = 0; j < NV; ++j) s[i] -= (double)Aoff[(size_t)((r * NV + j) * NV + i) * p + o] * zo[j];
+        for (int j = 0; j < NV; ++j) s[i] -= (double)Aoff[(size_t)((r * NV + j) * NV + i) * p + bo] * zo[j];
     } else {
 #pragma unroll
       for (int i = 0; i < NV; ++i)
 #pragma unroll
-        for (int j = 0; j < NV; ++j) s[i] -= (double)Aoff[(size_t)((q * NV + i) * NV + j) * p + c] * zo[j];
+        for (int j = 0; j < NV; ++j) s[i] -= (double)Aoff[(size_t)((q * NV + i) * NV + j) * p + b] * zo[j];
     }
   }
 #pragma unroll
   for (int i = 0; i < NV; ++i) {
     double a = 0.0;
 #pragma unroll
-    for (int j = 0; j < NV; ++j) a += (double)Binv[(size_t)(TR ? j * NV + i : i * NV + j) * p + c] * s[j];
+    for (int j = 0; j < NV; ++j) a += (double)Binv[(size_t)(TR ? j * NV + i : i * NV + j) * p + b] * s[j];
     z[i * p + c] = (TZ)a;
   }
 }
@@ -1179,13 +1216,15 @@ __global__ __launch_bounds__(BS) void k_red(int N, int p, const TA* __restrict__
 // sums nsum groups of nb partials; group i -> out[i]
 __global__ __launch_bounds__(BS) void k_red_final(const double* __restrict__ part, int nb, int nsum,
                                                    double* __restrict__ out) {
+  // one block per sum (launched with nsum blocks); each block adds its nb
+  // partials in a fixed order, so the result is deterministic
   __shared__ double sh[BS / 64];
-  for (int i = 0; i < nsum; ++i) {
-    double s = 0.0;
-    for (int k = threadIdx.x; k < nb; k += BS) s += part[(size_t)i * nb + k];
-    s = block_sum(s, sh);
-    if (threadIdx.x == 0) out[i] = s;
-  }
+  const int i = blockIdx.x;
+  if (i >= nsum) return;
+  double s = 0.0;
+  for (int k = threadIdx.x; k < nb; k += BS) s += part[(size_t)i * nb + k];
+  s = block_sum(s, sh);
+  if (threadIdx.x == 0) out[i] = s;
 }
 
 // ---- FD Jacobian-vector product (Solver.cpp:1331-1386) -------------------------------
@@ -1638,7 +1677,8 @@ static void reduce_t(cfd_ctx* c, int kind, const TA* a, const TB* b, int slot) {
       hipLaunchKernelGGL((k_red<NVt, RED_VW, TA, TB>), dim3(nb), dim3(BS), 0, c->stream, c->N, c->pitch, a, b,
                          c->d_vol, c->red);
   });
-  hipLaunchKernelGGL(k_red_final, dim3(1), dim3(BS), 0, c->stream, c->red, nb, kind == RED_VW ? 2 : 1,
+  hipLaunchKernelGGL(k_red_final, dim3(kind == RED_VW ? 2 : 1), dim3(BS), 0, c->stream, c->red, nb,
+                     kind == RED_VW ? 2 : 1,
                      c->scal + slot);
 }
 
@@ -1734,14 +1774,14 @@ template <class TV> static void orth_t(cfd_ctx* c, TV* V, int j, double* hdev) {
   const size_t vs = (size_t)nv * p;
   TV* w = V + (size_t)(j + 1) * vs;
   const int nvec = j + 1;
-  const int nb = std::min(256, std::max(1, nblocks(N)));
+  const int nb = std::min(RED_BLOCKS, std::max(1, nblocks(N)));  // nvec * nb partials fit c->red
   const int nchunk = (nvec + 7) / 8;
   double* h2 = hdev + 64;
   for (int pass = 0; pass < 2; ++pass) {
     double* hout = pass == 0 ? hdev : h2;
     NV_DISPATCH(nv, hipLaunchKernelGGL((k_mdot<NVt, TV>), dim3(nb, nchunk), dim3(BS), 0, c->stream, N, p, vs, V,
                                        nvec, w, c->red));
-    hipLaunchKernelGGL(k_red_final, dim3(1), dim3(BS), 0, c->stream, c->red, nb, nvec, hout);
+    hipLaunchKernelGGL(k_red_final, dim3(nvec), dim3(BS), 0, c->stream, c->red, nb, nvec, hout);
     NV_DISPATCH(nv, hipLaunchKernelGGL((k_maxpy<NVt, TV>), dim3(nblocks(N)), dim3(BS), 0, c->stream, N, p, vs, V,
                                        nvec, w, hout));
   }
@@ -1770,8 +1810,11 @@ template <class TV> static void block_t(cfd_ctx* c, const double* U, TV* Binv, d
     hipLaunchKernelGGL((k_gradlim<NVt, double>), dim3(nblocks(c->N)), dim3(BS), 0, c->stream, c->dm, c->phys, U,
                        (const double*)nullptr, c->grad, (double*)nullptr, c->phi, 0,
                        turb ? c->tq : (double*)nullptr, (double*)nullptr);
+    hipLaunchKernelGGL((k_face_jac<NVt>), dim3(nblocks(c->F)), dim3(BS), 0, c->stream, c->dm, c->phys, U,
+                       c->kry.fjac);
     hipLaunchKernelGGL((k_block<NVt, TV>), dim3(nblocks(c->N)), dim3(BS), 0, c->stream, c->dm, c->phys, U,
-                       (const double*)c->grad, c->dt, Binv, Bout, Aoff, turb ? (const double*)c->tq : nullptr);
+                       (const double*)c->grad, c->dt, (const double*)c->kry.fjac, Binv, Bout, Aoff,
+                       turb ? (const double*)c->tq : nullptr, Aoff ? (const int*)c->d_cpos : nullptr);
   });
 }
 void launch_block_precond(cfd_ctx* c, const double* U, void* Binv, int prec, double* Bout, void* Aoff) {
@@ -1783,13 +1826,13 @@ static void sgs_t(cfd_ctx* c, const TB* Binv, const TB* Aoff, const TX* x, TZ* z
   const int nc = (int)c->ccell_ptr.size() - 1;
   NV_DISPATCH(c->P.nvar, {
     hipLaunchKernelGGL((k_sgs_first<NVt, TB, TX, TZ, TR>), dim3(nblocks(c->N)), dim3(BS), 0, c->stream, c->N,
-                       c->pitch, c->d_ccol, Binv, x, z);
+                       c->pitch, c->d_ccol, c->d_cpos, Binv, x, z);
     for (int s = 1; s < 2 * nc - 1; ++s) {
       const int col = s < nc ? s : 2 * nc - 2 - s;
       const int b = c->ccell_ptr[col], n = c->ccell_ptr[col + 1] - b;
       if (n == 0) continue;
-      hipLaunchKernelGGL((k_sgs_color<NVt, TB, TX, TZ, TR>), dim3(nblocks(n)), dim3(BS), 0, c->stream, n,
-                         c->d_ccells + b, c->pitch, c->d_nq, Binv, Aoff, x, z);
+      hipLaunchKernelGGL((k_sgs_color<NVt, TB, TX, TZ, TR>), dim3(nblocks(n)), dim3(BS), 0, c->stream, n, b,
+                         c->d_ccells, c->pitch, c->d_nq, c->d_cpos, Binv, Aoff, x, z);
     }
   });
 }
