@@ -833,10 +833,12 @@ __global__ __launch_bounds__(BS) void k_diagprec(DMesh m, Phys P, const double* 
 // A_{c,q} = (+-) A dF/dU_other - nu A / d I for slot q of cell c, in rows
 // (q * NV + i) * NV + j of pitch (slots without a neighbor are not written).
 // Face Jacobians of the first-order inviscid flux (times A): dF/dU_L and, on
-// interior faces, dF/dU_R, one forward-dual pass per direction, stored as
-// Jf[(s * NV * NV + i * NV + k) * F + f] for s = 0 (L) and 1 (R).  On a
-// boundary face the ghost state is differentiated with U_L.  Each face is
-// differentiated once; the cell kernel below assembles both of its cells.
+// interior faces, dF/dU_R, one forward-dual pass per direction, stored per
+// face as Jf[f * 2 NV^2 + s * NV^2 + i * NV + k] for s = 0 (L) and 1 (R) (the
+// cell kernel gathers whole faces, so each face's 2 NV^2 values are
+// contiguous).  On a boundary face the ghost state is differentiated with
+// U_L.  Each face is differentiated once; the cell kernel below assembles
+// both of its cells.
 template <int NV>
 __global__ __launch_bounds__(BS) void k_face_jac(DMesh m, Phys P, const double* __restrict__ U,
                                                   double* __restrict__ Jf) {
@@ -863,7 +865,7 @@ __global__ __launch_bounds__(BS) void k_face_jac(DMesh m, Phys P, const double* 
     if (n < 0) Rd = ghost(P1, Ld, g.tg, g.nx, g.ny, g.dnw);
     const S6<D> Fl = flux_core<0, D>(P1, g, Ld, Rd, nullptr, nullptr);
 #pragma unroll
-    for (int i = 0; i < NV; ++i) Jf[(size_t)(side * NV * NV + i * NV + k) * m.F + f] = Fl[i].d;
+    for (int i = 0; i < NV; ++i) Jf[(size_t)f * 2 * NV * NV + side * NV * NV + i * NV + k] = Fl[i].d;
   }
 }
 
@@ -899,11 +901,11 @@ __global__ __launch_bounds__(BS) void k_block(DMesh m, Phys P, const double* __r
     // this cell's side of the face Jacobian, with the sign of its residual
     // (R_o += F A, R_n -= F A); the other side couples to the neighbor
     const double sg = as_nei ? -1.0 : 1.0;
-    const double* Js = Jf + (size_t)(as_nei ? NV * NV : 0) * m.F + f;
+    const double* Js = Jf + (size_t)f * 2 * NV * NV + (as_nei ? NV * NV : 0);
 #pragma unroll
     for (int i = 0; i < NV; ++i)
 #pragma unroll
-      for (int k = 0; k < NV; ++k) B[i][k] += sg * Js[(size_t)(i * NV + k) * m.F];
+      for (int k = 0; k < NV; ++k) B[i][k] += sg * Js[i * NV + k];
     // viscous diagonal: the two-point diffusion coefficient of each row,
     // (nu + nu_t) for momentum, gamma (nu / Pr + nu_t / Pr_t) for energy,
     // (nu + sigma nu_t) for k and omega, times A over the normal distance
@@ -935,13 +937,12 @@ __global__ __launch_bounds__(BS) void k_block(DMesh m, Phys P, const double* __r
       for (int i = 0; i < NV; ++i) visc[i] += vf[i];
     }
     if (Aoff && other >= 0) {
-      const double* Jo = Jf + (size_t)(as_nei ? 0 : NV * NV) * m.F + f;
+      const double* Jo = Jf + (size_t)f * 2 * NV * NV + (as_nei ? 0 : NV * NV);
 #pragma unroll
       for (int i = 0; i < NV; ++i)
 #pragma unroll
         for (int k = 0; k < NV; ++k)
-          Aoff[(size_t)((q * NV + i) * NV + k) * p + oc] =
-              (TV)(sg * Jo[(size_t)(i * NV + k) * m.F] - (i == k ? vf[i] : 0.0));
+          Aoff[(size_t)((q * NV + i) * NV + k) * p + oc] = (TV)(sg * Jo[i * NV + k] - (i == k ? vf[i] : 0.0));
     }
   }
   if constexpr (NV == 6) {
@@ -1215,9 +1216,10 @@ __global__ __launch_bounds__(BS) void k_red(int N, int p, const TA* __restrict__
 
 // sums nsum groups of nb partials; group i -> out[i]
 __global__ __launch_bounds__(BS) void k_red_final(const double* __restrict__ part, int nb, int nsum,
-                                                   double* __restrict__ out) {
+                                                   double* __restrict__ out, const double* __restrict__ gate = nullptr) {
   // one block per sum (launched with nsum blocks); each block adds its nb
   // partials in a fixed order, so the result is deterministic
+  if (gate && *gate == 0.0) return;
   __shared__ double sh[BS / 64];
   const int i = blockIdx.x;
   if (i >= nsum) return;
@@ -1406,13 +1408,17 @@ __global__ __launch_bounds__(BS) void k_rowscale(int N, int p, TV* __restrict__ 
   for (int k = 0; k < NV; ++k) w[k * p + c] = (TV)((double)w[k * p + c] * s.s[k]);
 }
 
-// partial dots of w against up to 8 basis vectors (chunk = blockIdx.y)
+// partial dots of w against up to 8 basis vectors (chunk = blockIdx.y);
+// indices nvec .. ntot-1 (at most one) dot w with itself.  With `gate`, the
+// launch does nothing unless *gate is nonzero.
 template <int NV, class TV>
 __global__ __launch_bounds__(BS) void k_mdot(int N, int p, size_t vstride, const TV* __restrict__ V, int nvec,
-                                              const TV* __restrict__ w, double* __restrict__ part) {
+                                              int ntot, const TV* __restrict__ w, double* __restrict__ part,
+                                              const double* __restrict__ gate) {
+  if (gate && *gate == 0.0) return;
   __shared__ double sh[BS / 64];
   const int i0 = blockIdx.y * 8;
-  const int cnt = min(8, nvec - i0);
+  const int cnt = min(8, ntot - i0);
   double acc[8];
 #pragma unroll
   for (int i = 0; i < 8; ++i) acc[i] = 0.0;
@@ -1423,7 +1429,7 @@ __global__ __launch_bounds__(BS) void k_mdot(int N, int p, size_t vstride, const
 #pragma unroll
     for (int i = 0; i < 8; ++i) {
       if (i < cnt) {
-        const TV* Vi = V + (size_t)(i0 + i) * vstride;
+        const TV* Vi = i0 + i < nvec ? V + (size_t)(i0 + i) * vstride : w;
         double t = 0.0;
 #pragma unroll
         for (int k = 0; k < NV; ++k) t += wv[k] * (double)Vi[k * p + c];
@@ -1440,15 +1446,41 @@ __global__ __launch_bounds__(BS) void k_mdot(int N, int p, size_t vstride, const
 // w -= sum_i h[i] V_i ; hacc[i] += h[i] (second pass of CGS2 accumulates)
 template <int NV, class TV>
 __global__ __launch_bounds__(BS) void k_maxpy(int N, int p, size_t vstride, const TV* __restrict__ V, int nvec,
-                                               TV* __restrict__ w, const double* __restrict__ h) {
+                                               TV* __restrict__ w, const double* __restrict__ h,
+                                               const double* __restrict__ gate) {
+  if (gate && *gate == 0.0) return;
   const int c = xcd_block() * BS + threadIdx.x;
   if (c >= N) return;
+  // all NV rows per basis vector, and the basis loop unrolled so that several
+  // vectors' loads are in flight at once
+  double s[NV];
 #pragma unroll
-  for (int k = 0; k < NV; ++k) {
-    double s = (double)w[k * p + c];
-    for (int i = 0; i < nvec; ++i) s -= h[i] * (double)V[(size_t)i * vstride + k * p + c];
-    w[k * p + c] = (TV)s;
+  for (int k = 0; k < NV; ++k) s[k] = (double)w[k * p + c];
+#pragma unroll 4
+  for (int i = 0; i < nvec; ++i) {
+    const double hi = h[i];
+    const TV* Vi = V + (size_t)i * vstride + c;
+#pragma unroll
+    for (int k = 0; k < NV; ++k) s[k] -= hi * (double)Vi[k * p];
   }
+#pragma unroll
+  for (int k = 0; k < NV; ++k) w[k * p + c] = (TV)s[k];
+}
+
+// Reorthogonalisation test after the first Gram-Schmidt pass ("twice is
+// enough"): with h the projections and h[n] = |w|^2 before the pass,
+// |w'|^2 = |w|^2 - |h|^2; a second pass runs only if |w'| < 0.7 |w|.
+// Otherwise its projections h2 are zero and *flag = 0 gates its launches off.
+__global__ void k_reorth_test(const double* __restrict__ h, int n, double* __restrict__ h2,
+                              double* __restrict__ flag) {
+  if (threadIdx.x != 0) return;
+  double hn = 0.0;
+  for (int i = 0; i < n; ++i) hn += h[i] * h[i];
+  const double w0 = h[n], w1 = w0 - hn;
+  const bool again = !(w1 >= 0.49 * w0);
+  *flag = again ? 1.0 : 0.0;
+  if (!again)
+    for (int i = 0; i < n; ++i) h2[i] = 0.0;
 }
 
 __global__ void k_hacc(double* __restrict__ h, const double* __restrict__ h2, int n, double* __restrict__ nrm2,
@@ -1777,14 +1809,22 @@ template <class TV> static void orth_t(cfd_ctx* c, TV* V, int j, double* hdev) {
   const int nb = std::min(RED_BLOCKS, std::max(1, nblocks(N)));  // nvec * nb partials fit c->red
   const int nchunk = (nvec + 7) / 8;
   double* h2 = hdev + 64;
-  for (int pass = 0; pass < 2; ++pass) {
-    double* hout = pass == 0 ? hdev : h2;
-    NV_DISPATCH(nv, hipLaunchKernelGGL((k_mdot<NVt, TV>), dim3(nb, nchunk), dim3(BS), 0, c->stream, N, p, vs, V,
-                                       nvec, w, c->red));
-    hipLaunchKernelGGL(k_red_final, dim3(nvec), dim3(BS), 0, c->stream, c->red, nb, nvec, hout);
-    NV_DISPATCH(nv, hipLaunchKernelGGL((k_maxpy<NVt, TV>), dim3(nblocks(N)), dim3(BS), 0, c->stream, N, p, vs, V,
-                                       nvec, w, hout));
-  }
+  double* flag = c->scal + 8;
+  // pass 1 also forms |w|^2 (into hdev[nvec], overwritten by the final norm)
+  const int nchunk1 = (nvec + 1 + 7) / 8;
+  NV_DISPATCH(nv, hipLaunchKernelGGL((k_mdot<NVt, TV>), dim3(nb, nchunk1), dim3(BS), 0, c->stream, N, p, vs, V, nvec,
+                                     nvec + 1, w, c->red, (const double*)nullptr));
+  hipLaunchKernelGGL(k_red_final, dim3(nvec + 1), dim3(BS), 0, c->stream, c->red, nb, nvec + 1, hdev,
+                     (const double*)nullptr);
+  NV_DISPATCH(nv, hipLaunchKernelGGL((k_maxpy<NVt, TV>), dim3(nblocks(N)), dim3(BS), 0, c->stream, N, p, vs, V, nvec,
+                                     w, (const double*)hdev, (const double*)nullptr));
+  hipLaunchKernelGGL(k_reorth_test, dim3(1), dim3(64), 0, c->stream, (const double*)hdev, nvec, h2, flag);
+  // pass 2 only where the first lost too much of w
+  NV_DISPATCH(nv, hipLaunchKernelGGL((k_mdot<NVt, TV>), dim3(nb, nchunk), dim3(BS), 0, c->stream, N, p, vs, V, nvec,
+                                     nvec, w, c->red, (const double*)flag));
+  hipLaunchKernelGGL(k_red_final, dim3(nvec), dim3(BS), 0, c->stream, c->red, nb, nvec, h2, (const double*)flag);
+  NV_DISPATCH(nv, hipLaunchKernelGGL((k_maxpy<NVt, TV>), dim3(nblocks(N)), dim3(BS), 0, c->stream, N, p, vs, V, nvec,
+                                     w, (const double*)h2, (const double*)flag));
   reduce_t<TV, TV>(c, RED_L2, w, w, 7);
   hipLaunchKernelGGL(k_hacc, dim3(1), dim3(64), 0, c->stream, hdev, h2, nvec, c->scal + 7, hdev + j + 1);
 }
