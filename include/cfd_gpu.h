@@ -298,6 +298,10 @@ int cfd_dev_local_timestep(cfd_ctx* ctx, const double* U, double cfl);
 /* v and Jv are FP64 (precision 64) or FP32 (precision 32) SoA vectors. */
 int cfd_dev_jv_tangent(cfd_ctx* ctx, const double* U, const void* v, void* Jv, int precision);
 int cfd_dev_vjp_adjoint(cfd_ctx* ctx, const double* U, const double* w, double* JTw);
+/* cfd_gmres on device SoA vectors (x = solution); returns when the solve has
+ * finished.  rhs and x must not alias U or R. */
+int cfd_dev_gmres(cfd_ctx* ctx, const double* U, const double* R, const double* rhs, double* x,
+                  const cfd_gmres_opts* opts, int* iters_out);
 int cfd_dev_synchronize(cfd_ctx* ctx);
 
 /* ---- measurement -------------------------------------------------------- */

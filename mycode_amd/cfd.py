@@ -115,6 +115,7 @@ SIGNATURES = [
     ("cfd_vjp_adjoint", C.c_int, [_VP, _DP, _DP, _DP]),
     ("cfd_block_jacobi", C.c_int, [_VP, _DP, C.c_int, _DP, _DP]),
     ("cfd_block_sgs", C.c_int, [_VP, _DP, C.c_int, C.c_int, _DP, _DP, _DP, _IP, _IP]),
+    ("cfd_dev_gmres", C.c_int, [_VP, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.POINTER(GmresOpts), _IP]),
     ("cfd_norm_l2", C.c_int, [_VP, _DP, _DP]),
     ("cfd_dot", C.c_int, [_VP, _DP, _DP, _DP]),
     ("cfd_norm_vw", C.c_int, [_VP, _DP, _DP]),
@@ -140,12 +141,21 @@ _lib = None
 
 
 def load_library(path: str = LIB_PATH):
-    """Load the HIP library; raises if it has not been built."""
+    """Load the HIP library; raises if it has not been built.
+
+    PyTorch ships its own HIP runtime.  When torch is installed it is imported
+    first, so that the library binds to the runtime torch loads and device
+    pointers from torch tensors are valid in both (loading the library first
+    leaves torch without a usable device)."""
     global _lib
     if _lib is not None:
         return _lib
     if not os.path.exists(path):
         raise CfdError(f"{path} is missing: run __graft_entry__.build() (hipcc, gfx950) first")
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
     lib = C.CDLL(path)
     for name, res, args in SIGNATURES:
         fn = getattr(lib, name, None)
@@ -537,6 +547,13 @@ class Solver:
         self._c(self.lib.cfd_dev_time_residual(self._h, C.c_void_p(U_ptr), C.c_void_p(R_ptr), reps,
                                                int(update_limiters), _dp(ms), C.byref(nl)), "cfd_dev_time_residual")
         return ms, nl.value
+
+    def dev_gmres(self, U_ptr, R_ptr, rhs_ptr, x_ptr, opts: GmresOpts) -> int:
+        """GMRES on device SoA vectors; returns the iteration count."""
+        it = C.c_int()
+        self._c(self.lib.cfd_dev_gmres(self._h, C.c_void_p(U_ptr), C.c_void_p(R_ptr), C.c_void_p(rhs_ptr),
+                                       C.c_void_p(x_ptr), C.byref(opts), C.byref(it)), "cfd_dev_gmres")
+        return it.value
 
     def synchronize(self):
         self._c(self.lib.cfd_dev_synchronize(self._h), "cfd_dev_synchronize")

@@ -116,7 +116,7 @@ int ensure_krylov(cfd_ctx* c, int m, int prec) {
   HIP_TRY(c, hipMalloc(&k.V, (size_t)(m + 1) * vs * es));
   HIP_TRY(c, hipMalloc((void**)&k.D, (size_t)c->pitch * 8));
   HIP_TRY(c, hipMalloc(&k.blk, (size_t)c->P.nvar * c->P.nvar * c->pitch * es));
-  HIP_TRY(c, hipMalloc((void**)&k.fjac, (size_t)2 * c->P.nvar * c->P.nvar * std::max(c->F, 1) * 8));
+  HIP_TRY(c, hipMalloc((void**)&k.fjac, (size_t)2 * c->P.nvar * c->P.nvar * ((c->F + 63) / 64 * 64) * 8));
   HIP_TRY(c, hipMalloc((void**)&k.z, vs * 8));
   HIP_TRY(c, hipMalloc((void**)&k.w, vs * 8));
   HIP_TRY(c, hipMalloc((void**)&k.rr, vs * 8));
@@ -1175,6 +1175,19 @@ int cfd_dev_vjp_adjoint(cfd_ctx* c, const double* U, const double* w, double* JT
   launch_vjp(c, U, w, JTw);
   HIP_TRY(c, hipGetLastError());
   return CFD_OK;
+}
+
+int cfd_dev_gmres(cfd_ctx* c, const double* U, const double* R, const double* rhs, double* x,
+                  const cfd_gmres_opts* opts, int* iters_out) {
+  if (!c || !U || !R || !rhs || !x) return CFD_ERR_ARG;
+  if (!c->dt_valid) return fail(c, CFD_ERR_STATE, "no local time step computed");
+  cfd_gmres_opts o;
+  if (opts) o = *opts;
+  else cfd_gmres_opts_default(&o);
+  int it = 0;
+  const int rc = gmres_dev(c, U, R, rhs, 1.0, x, &o, &it, c->vec[4], c->vec[5]);
+  if (iters_out) *iters_out = it;
+  return rc;
 }
 
 int cfd_dev_synchronize(cfd_ctx* c) {

@@ -833,12 +833,16 @@ __global__ __launch_bounds__(BS) void k_diagprec(DMesh m, Phys P, const double* 
 // A_{c,q} = (+-) A dF/dU_other - nu A / d I for slot q of cell c, in rows
 // (q * NV + i) * NV + j of pitch (slots without a neighbor are not written).
 // Face Jacobians of the first-order inviscid flux (times A): dF/dU_L and, on
-// interior faces, dF/dU_R, one forward-dual pass per direction, stored per
-// face as Jf[f * 2 NV^2 + s * NV^2 + i * NV + k] for s = 0 (L) and 1 (R) (the
-// cell kernel gathers whole faces, so each face's 2 NV^2 values are
-// contiguous).  On a boundary face the ghost state is differentiated with
-// U_L.  Each face is differentiated once; the cell kernel below assembles
-// both of its cells.
+// interior faces, dF/dU_R, one forward-dual pass per direction.  Entry
+// e = s * NV^2 + i * NV + k (s = 0 for L, 1 for R) of face f sits at
+// jf_index: faces in tiles of 64, entry-major inside a tile, so that a wave's
+// stores are contiguous and the cell kernel's gather of a face stays within
+// one tile.  On a boundary face the ghost state is differentiated with U_L.
+// Each face is differentiated once; the cell kernel below assembles both of
+// its cells.
+template <int NV> __device__ __forceinline__ size_t jf_index(int f, int e) {
+  return ((size_t)(f >> 6) * (2 * NV * NV) + e) * 64 + (f & 63);
+}
 template <int NV>
 __global__ __launch_bounds__(BS) void k_face_jac(DMesh m, Phys P, const double* __restrict__ U,
                                                   double* __restrict__ Jf) {
@@ -865,7 +869,7 @@ __global__ __launch_bounds__(BS) void k_face_jac(DMesh m, Phys P, const double* 
     if (n < 0) Rd = ghost(P1, Ld, g.tg, g.nx, g.ny, g.dnw);
     const S6<D> Fl = flux_core<0, D>(P1, g, Ld, Rd, nullptr, nullptr);
 #pragma unroll
-    for (int i = 0; i < NV; ++i) Jf[(size_t)f * 2 * NV * NV + side * NV * NV + i * NV + k] = Fl[i].d;
+    for (int i = 0; i < NV; ++i) Jf[jf_index<NV>(f, side * NV * NV + i * NV + k)] = Fl[i].d;
   }
 }
 
@@ -901,11 +905,11 @@ __global__ __launch_bounds__(BS) void k_block(DMesh m, Phys P, const double* __r
     // this cell's side of the face Jacobian, with the sign of its residual
     // (R_o += F A, R_n -= F A); the other side couples to the neighbor
     const double sg = as_nei ? -1.0 : 1.0;
-    const double* Js = Jf + (size_t)f * 2 * NV * NV + (as_nei ? NV * NV : 0);
+    const int es = as_nei ? NV * NV : 0;
 #pragma unroll
     for (int i = 0; i < NV; ++i)
 #pragma unroll
-      for (int k = 0; k < NV; ++k) B[i][k] += sg * Js[i * NV + k];
+      for (int k = 0; k < NV; ++k) B[i][k] += sg * Jf[jf_index<NV>(f, es + i * NV + k)];
     // viscous diagonal: the two-point diffusion coefficient of each row,
     // (nu + nu_t) for momentum, gamma (nu / Pr + nu_t / Pr_t) for energy,
     // (nu + sigma nu_t) for k and omega, times A over the normal distance
@@ -937,12 +941,13 @@ __global__ __launch_bounds__(BS) void k_block(DMesh m, Phys P, const double* __r
       for (int i = 0; i < NV; ++i) visc[i] += vf[i];
     }
     if (Aoff && other >= 0) {
-      const double* Jo = Jf + (size_t)f * 2 * NV * NV + (as_nei ? 0 : NV * NV);
+      const int eo = as_nei ? 0 : NV * NV;
 #pragma unroll
       for (int i = 0; i < NV; ++i)
 #pragma unroll
         for (int k = 0; k < NV; ++k)
-          Aoff[(size_t)((q * NV + i) * NV + k) * p + oc] = (TV)(sg * Jo[i * NV + k] - (i == k ? vf[i] : 0.0));
+          Aoff[(size_t)((q * NV + i) * NV + k) * p + oc] =
+              (TV)(sg * Jf[jf_index<NV>(f, eo + i * NV + k)] - (i == k ? vf[i] : 0.0));
     }
   }
   if constexpr (NV == 6) {
