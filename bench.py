@@ -45,9 +45,18 @@ WORKLOADS = {
 
 
 def stage_bytes(N, F, nv, assembly, limiter, turb):
-    """Compulsory bytes of each residual stage's own inputs and outputs."""
+    """Compulsory bytes of each residual stage's own inputs and outputs.
+
+    gradients: U, V and slots per cell, face geometry and indices, the
+    gradients (and limiter) written, and with SST the wall distance read and
+    mu_t, F1, V S_k, V S_w written; flux: both cells' U, gradients (limiter,
+    mu_t and F1) and the centroids, face geometry, and the face buffer (or the
+    scatter's read-modify-write of R); assembly: the face buffer, slots and R
+    (and the two SST source rows)."""
     ng = 14 if turb else 10
     grad = N * (8 * nv + 16 + 8 * ng + (8 + 16 if limiter else 0)) + F * (8 + 24 + (16 if limiter else 0))
+    if turb:
+        grad += N * (8 + 32)
     flux = N * (8 * nv + 8 * ng + 16 + (8 if limiter else 0) + (8 if turb else 0)) + F * (12 + 40)
     if assembly == 0:
         flux += N * 16 * nv
@@ -55,8 +64,8 @@ def stage_bytes(N, F, nv, assembly, limiter, turb):
     else:
         flux += F * 8 * nv
         asm = F * 8 * nv + N * (16 + 8 * nv)
-    if turb:
-        asm += N * (8 * nv + 8 * ng + 16 + 16)
+        if turb:
+            asm += N * 16
     return grad, flux, asm
 
 
