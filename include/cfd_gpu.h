@@ -141,6 +141,11 @@ uint64_t cfd_mesh_hash(const cfd_mesh* m);
 int cfd_mesh_set_nodes(cfd_mesh* m, const double* x, const double* y);
 /* Write Gmsh v2.2 ASCII with %.17g coordinates. */
 int cfd_mesh_write_gmsh(const cfd_mesh* m, const char* path);
+/* main.cpp saveResults (main.cpp:8-51): legacy ASCII VTK of the mesh with the
+ * cell fields Mach, Pressure and Density of U6 (reference cell order),
+ * pressure with the given gamma (the reference uses Flux.hpp's 1.4), numbers
+ * at the reference stream's default 6 significant digits. */
+int cfd_mesh_write_vtk(const cfd_mesh* m, const double* U6, double gamma, const char* path);
 
 /* ---- context ------------------------------------------------------------ */
 typedef struct cfd_ctx cfd_ctx;
@@ -277,6 +282,17 @@ int cfd_solve_implicit(cfd_ctx* ctx, double* U6, const cfd_newton_opts* opts,
                        cfd_newton_result* res, double* hist_vm, int* hist_iters,
                        double* hist_cfl);
 
+/* Solver::solveUnsteady(dt_phys, maxTimeSteps) — Solver.cpp:1801-1864: BDF1
+ * dual time stepping.  The Jacobian's time term becomes vol / dt_phys; per
+ * physical step, Newton on F = (U - U_old) vol / dt_phys + R(U) with GMRES
+ * (gopts, NULL = the reference's FD Jv + scalar diagonal, m = 40, tol 1e-2),
+ * stopping when VM||F|| over rho, rho u, rho v, rho E < tolerance after at
+ * least one update, U += relax dU without clips.  max_newton <= 0, tolerance
+ * <= 0 and relax <= 0 select the reference's 15, 1e-6 and 0.7.  hist_vm
+ * (n_steps x max_newton, may be NULL) receives VM||F|| per Newton step,
+ * hist_newton (n_steps, may be NULL) the updates taken per physical step. */
+int cfd_solve_unsteady(cfd_ctx* ctx, double* U6, double dt_phys, int n_steps, const cfd_gmres_opts* gopts,
+                       int max_newton, double tolerance, double relax, double* hist_vm, int* hist_newton);
 /* Solver::timeStepExplicit(dt) — Solver.cpp:1219-1243 (forward Euler). */
 int cfd_explicit_step(cfd_ctx* ctx, double* U6, double dt);
 /* SSP-RK3 (Shu-Osher) built from three residual calls (no reference counterpart). */
@@ -287,6 +303,19 @@ int cfd_rk3_step(cfd_ctx* ctx, double* U6, double dt);
  * order (0 elsewhere), and J = sum over tag-3 faces of p_owner * area
  * (cf. calculateDrag, Solver.cpp:1307-1325). */
 int cfd_wall_outputs(cfd_ctx* ctx, const double* U6, double* Cp, double* Cf, double* J);
+
+/* Solver::calculateEntropyError (Solver.cpp:1620-1646): the volume-weighted L2
+ * norm of (p / rho^gamma - S_ref) / S_ref over the cells with p, rho > 0, with
+ * S_ref from the freestream implied by the total conditions and M_inf.  The
+ * reference prints it; here it is returned. */
+int cfd_entropy_error(cfd_ctx* ctx, const double* U6, double* err);
+/* Solver::extractBoundaryLayerProfile (Solver.cpp:1866-1897): one row
+ * (y_c, u / u_inf, v / u_inf, eta) per cell with |x_c - x_target| < 0.005, in
+ * reference cell order; eta = y_c sqrt(u_inf / (nu max(x_target, 1e-6))).
+ * The reference writes the rows to a CSV file; here they fill rows4 (up to
+ * `capacity` rows) and *n_rows receives their number (an error if it exceeds
+ * the capacity). */
+int cfd_bl_profile(cfd_ctx* ctx, const double* U6, double x_target, int capacity, double* rows4, int* n_rows);
 
 /* ---- device-handle variants (internal SoA layout, RCM order, async) ----- */
 /* Reference-order AoS host/device buffer <-> internal SoA device vector. */

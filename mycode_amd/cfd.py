@@ -116,6 +116,11 @@ SIGNATURES = [
     ("cfd_block_jacobi", C.c_int, [_VP, _DP, C.c_int, _DP, _DP]),
     ("cfd_block_sgs", C.c_int, [_VP, _DP, C.c_int, C.c_int, _DP, _DP, _DP, _IP, _IP]),
     ("cfd_dev_gmres", C.c_int, [_VP, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.POINTER(GmresOpts), _IP]),
+    ("cfd_entropy_error", C.c_int, [_VP, _DP, _DP]),
+    ("cfd_bl_profile", C.c_int, [_VP, _DP, C.c_double, C.c_int, _DP, _IP]),
+    ("cfd_mesh_write_vtk", C.c_int, [_VP, _DP, C.c_double, C.c_char_p]),
+    ("cfd_solve_unsteady", C.c_int, [_VP, _DP, C.c_double, C.c_int, C.POINTER(GmresOpts), C.c_int, C.c_double,
+                                     C.c_double, _DP, _IP]),
     ("cfd_norm_l2", C.c_int, [_VP, _DP, _DP]),
     ("cfd_dot", C.c_int, [_VP, _DP, _DP, _DP]),
     ("cfd_norm_vw", C.c_int, [_VP, _DP, _DP]),
@@ -298,6 +303,13 @@ class Mesh:
 
     def write_gmsh(self, path: str):
         _check(self.lib.cfd_mesh_write_gmsh(self._h, path.encode()))
+
+    def write_vtk(self, U, path: str, gamma: float = 1.4):
+        """main.cpp saveResults: legacy VTK with Mach, Pressure, Density cell fields."""
+        U = np.ascontiguousarray(U, np.float64)
+        if U.shape != (self.n_cells, 6):
+            raise ValueError(f"state must have shape ({self.n_cells}, 6)")
+        _check(self.lib.cfd_mesh_write_vtk(self._h, _dp(U), gamma, path.encode()))
 
 
 # ---- solver ------------------------------------------------------------------------
@@ -547,6 +559,32 @@ class Solver:
         self._c(self.lib.cfd_dev_time_residual(self._h, C.c_void_p(U_ptr), C.c_void_p(R_ptr), reps,
                                                int(update_limiters), _dp(ms), C.byref(nl)), "cfd_dev_time_residual")
         return ms, nl.value
+
+    def calculate_entropy_error(self, U) -> float:
+        """Solver::calculateEntropyError: volume-weighted L2 entropy error."""
+        out = C.c_double()
+        self._c(self.lib.cfd_entropy_error(self._h, _dp(_state(U, self.N)), C.byref(out)), "cfd_entropy_error")
+        return out.value
+
+    def extract_boundary_layer_profile(self, U, x_target: float) -> np.ndarray:
+        """Solver::extractBoundaryLayerProfile rows (y, u/u_inf, v/u_inf, eta), reference cell order."""
+        U = _state(U, self.N)
+        n = C.c_int()
+        rows = np.zeros((self.N, 4))
+        self._c(self.lib.cfd_bl_profile(self._h, _dp(U), x_target, self.N, _dp(rows), C.byref(n)), "cfd_bl_profile")
+        return rows[:n.value]
+
+    def solve_unsteady(self, U, dt_phys: float, n_steps: int, gmres: GmresOpts | None = None, max_newton: int = 15,
+                       tolerance: float = 1e-6, relax: float = 0.7):
+        """Solver::solveUnsteady (BDF1 dual time stepping): (U, VM||F|| per Newton
+        step (n_steps x max_newton, zero after the stop), updates per step)."""
+        U = _state(U, self.N).copy()
+        hvm = np.zeros((max(1, n_steps), max_newton))
+        hn = np.zeros(max(1, n_steps), dtype=np.int32)
+        g = C.byref(gmres) if gmres is not None else None
+        self._c(self.lib.cfd_solve_unsteady(self._h, _dp(U), dt_phys, n_steps, g, max_newton, tolerance, relax,
+                                            _dp(hvm), _ip(hn)), "cfd_solve_unsteady")
+        return U, hvm[:n_steps], hn[:n_steps]
 
     def dev_gmres(self, U_ptr, R_ptr, rhs_ptr, x_ptr, opts: GmresOpts) -> int:
         """GMRES on device SoA vectors; returns the iteration count."""

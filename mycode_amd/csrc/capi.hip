@@ -498,6 +498,7 @@ int cfd_ctx_destroy(cfd_ctx* c) {
                   c->phi,   c->dt,    c->Fm,    c->red,     c->scal, c->gradT, c->adj, c->tq, c->io6};
   for (void* b : bufs) (void)hipFree(b);
   for (int i = 0; i < 6; ++i) (void)hipFree(c->vec[i]);
+  (void)hipFree(c->aux);
   (void)hipHostFree(c->scal_host);
   if (c->stream) (void)hipStreamDestroy(c->stream);
   delete c;
@@ -1080,6 +1081,92 @@ int cfd_solve_implicit(cfd_ctx* c, double* U6, const cfd_newton_opts* opts, cfd_
     res->seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
   }
   return CFD_OK;
+}
+
+// ---- dual time stepping (Solver.cpp:1801-1864) ---------------------------------
+int cfd_solve_unsteady(cfd_ctx* c, double* U6, double dt_phys, int n_steps, const cfd_gmres_opts* gopts,
+                       int max_newton, double tolerance, double relax, double* hist_vm, int* hist_newton) {
+  if (!c || !U6 || !(dt_phys > 0) || n_steps < 0) return CFD_ERR_ARG;
+  cfd_gmres_opts g;
+  if (gopts) g = *gopts;
+  else cfd_gmres_opts_default(&g);
+  if (max_newton <= 0) max_newton = 15;  // Solver.cpp:1819-1820
+  if (!(tolerance > 0)) tolerance = 1e-6;
+  if (!(relax > 0)) relax = 0.7;         // Solver.cpp:1858
+  const size_t vs = (size_t)c->P.nvar * c->pitch;
+  if (!c->aux) HIP_TRY(c, hipMalloc((void**)&c->aux, (size_t)6 * c->pitch * 8));
+  double *U = c->vec[0], *R = c->vec[1], *dU = c->vec[2], *Uold = c->vec[3], *rhs = c->aux;
+  int rc = upload_state(c, U6, U);
+  if (rc) return rc;
+  // the pseudo-time term of the Jacobian becomes vol / dt_phys (Solver.cpp:1808-1809)
+  launch_fill(c, c->dt, dt_phys, (size_t)c->pitch);
+  c->dt_valid = true;
+  for (int t = 0; t < n_steps; ++t) {
+    HIP_TRY(c, hipMemcpyAsync(Uold, U, vs * 8, hipMemcpyDeviceToDevice, c->stream));
+    int step = 0;
+    for (step = 0; step < max_newton; ++step) {
+      launch_residual(c, U, R, true);
+      launch_unsteady_rhs(c, U, Uold, R, dt_phys, rhs);
+      launch_reduce(c, RED_VW4, rhs, nullptr, S_VW);
+      double sv[2];
+      if ((rc = read_scal(c, S_VW, 2, sv))) return rc;
+      const double vm = sv[1] == 0 ? 0.0 : std::sqrt(sv[0] / sv[1]);
+      if (hist_vm) hist_vm[(size_t)t * max_newton + step] = vm;
+      if (!(vm == vm)) return fail(c, CFD_ERR_NUMERIC, "unsteady residual norm is NaN");
+      if (vm < tolerance && step > 0) break;
+      int it = 0;
+      if ((rc = gmres_dev(c, U, R, rhs, 1.0, dU, &g, &it, c->vec[4], c->vec[5]))) return rc;
+      launch_axpby(c, U, 1.0, dU, relax, vs);  // U <- U + relax dU, no clips (Solver.cpp:1856-1859)
+    }
+    if (hist_newton) hist_newton[t] = step;
+  }
+  return download_state(c, U, U6);
+}
+
+// ---- verification outputs (Solver.cpp:1620-1646, 1866-1897) -----------------------
+int cfd_entropy_error(cfd_ctx* c, const double* U6, double* err) {
+  if (!c || !U6 || !err) return CFD_ERR_ARG;
+  int rc = upload_state(c, U6, c->vec[0]);
+  if (rc) return rc;
+  const cfd_params& P = c->P;
+  const double f = 1.0 + 0.5 * (P.gamma - 1.0) * P.M_inf * P.M_inf;
+  const double T_inf = P.T_total_inlet / f;
+  const double P_inf = P.P_total_inlet / std::pow(f, P.gamma / (P.gamma - 1.0));
+  const double rho_inf = P_inf / (P.R_gas * T_inf);
+  const double S_ref = P_inf / std::pow(rho_inf, P.gamma);
+  launch_entropy(c, c->vec[0], S_ref, S_L2);
+  double s[2];
+  if ((rc = read_scal(c, S_L2, 2, s))) return rc;
+  *err = std::sqrt(s[0] / s[1]);
+  return CFD_OK;
+}
+
+int cfd_bl_profile(cfd_ctx* c, const double* U6, double x_target, int capacity, double* rows4, int* n_rows) {
+  if (!c || !U6 || !n_rows || capacity < 0 || (capacity > 0 && !rows4)) return CFD_ERR_ARG;
+  int rc = upload_state(c, U6, c->vec[0]);
+  if (rc) return rc;
+  const cfd_params& P = c->P;
+  const double T_static = P.T_total_inlet / (1.0 + 0.5 * (P.gamma - 1.0) * P.M_inf * P.M_inf);
+  const double u_inf = P.M_inf * std::sqrt(P.gamma * P.R_gas * T_static);
+  double* out4 = c->vec[1];
+  unsigned char* flag = (unsigned char*)c->vec[2];
+  launch_bl_profile(c, c->vec[0], x_target, u_inf, P.nu, out4, flag);
+  const int P_ = c->pitch;
+  std::vector<double> h((size_t)4 * P_);
+  std::vector<unsigned char> fl(c->N);
+  HIP_TRY(c, hipMemcpyAsync(h.data(), out4, h.size() * 8, hipMemcpyDeviceToHost, c->stream));
+  HIP_TRY(c, hipMemcpyAsync(fl.data(), flag, c->N, hipMemcpyDeviceToHost, c->stream));
+  HIP_TRY(c, hipStreamSynchronize(c->stream));
+  int n = 0;
+  for (int i = 0; i < c->N; ++i) {  // rows in reference cell order
+    const int d = c->perm[i];
+    if (!fl[d]) continue;
+    if (n < capacity)
+      for (int k = 0; k < 4; ++k) rows4[(size_t)n * 4 + k] = h[(size_t)k * P_ + d];
+    ++n;
+  }
+  *n_rows = n;
+  return n > capacity ? fail(c, CFD_ERR_ARG, "boundary-layer profile has more rows than the capacity") : CFD_OK;
 }
 
 int cfd_explicit_step(cfd_ctx* c, double* U6, double dt) {

@@ -88,6 +88,7 @@ struct cfd_ctx {
   // host-pointer staging: AoS device copy + SoA vectors
   double* io6 = nullptr;     // 6 x N (AoS)
   double* vec[6] = {};       // 6 SoA vectors of nv x pitch
+  double* aux = nullptr;     // one more SoA vector (6 x pitch), allocated by the unsteady driver
   cfd::Krylov kry;
   std::string err;
   int nv() const { return P.nvar; }
@@ -105,7 +106,9 @@ void launch_unpack(cfd_ctx* c, const double* U, double* U6);
 void launch_timestep(cfd_ctx* c, const double* U, double cfl);
 void launch_diag_precond(cfd_ctx* c, const double* U, double* D);
 // reductions: result written to c->scal[slot] (device)
-enum RedKind { RED_L2 = 0, RED_DOT = 1, RED_VW = 2 };
+// RED_VW: sum (a_k / vol)^2 vol over all rows and sum vol (two results);
+// RED_VW4: the same over rows 0-3 (rho, rho u, rho v, rho E)
+enum RedKind { RED_L2 = 0, RED_DOT = 1, RED_VW = 2, RED_VW4 = 3 };
 void launch_reduce(cfd_ctx* c, int kind, const double* a, const double* b, int slot);
 void launch_fill(cfd_ctx* c, double* x, double v, size_t n);
 // FD Jv: Jv = (R(U + eps v) - Rb) / eps + (vol/dt) v, eps from scal[su] (|U|^2) and scal[sv] (|v|^2)
@@ -124,6 +127,14 @@ void launch_newton_update(cfd_ctx* c, double* U, const double* dU, double relax)
 void launch_explicit(cfd_ctx* c, double* Unew, const double* U0, double a0, const double* U1, double a1,
                      const double* R, double dtc);
 void launch_wall(cfd_ctx* c, const double* U, double* out3);
+// entropy-error sums (sum e^2 V, sum V) into scal[slot], scal[slot + 1]
+void launch_entropy(cfd_ctx* c, const double* U, double S_ref, int slot);
+// boundary-layer profile rows (4 x pitch) and per-cell selection flags
+void launch_bl_profile(cfd_ctx* c, const double* U, double x_target, double u_inf, double nu, double* out4,
+                       unsigned char* flag);
+// rhs = -((U - Uold) vol / dtp + R)
+void launch_unsteady_rhs(cfd_ctx* c, const double* U, const double* Uold, const double* R, double dtp,
+                         double* rhs);
 // wall distance of every cell to the nw no-slip face midpoints (xy interleaved)
 void launch_wall_distance(cfd_ctx* c, const double* wxy, int nw, double* wd);
 // Krylov (precision-generic basis)

@@ -1199,11 +1199,12 @@ __global__ __launch_bounds__(BS) void k_red(int N, int p, const TA* __restrict__
 #pragma unroll
       for (int k = 0; k < NV; ++k) t += (double)a[k * p + c] * (double)b[k * p + c];
       s0 += t;
-    } else {
+    } else {  // RED_VW over all rows, RED_VW4 over rho, rho u, rho v, rho E only
       const double v = vol[c];
+      constexpr int NK = KIND == RED_VW4 ? 4 : NV;
       double t = 0.0;
 #pragma unroll
-      for (int k = 0; k < NV; ++k) {
+      for (int k = 0; k < NK; ++k) {
         const double r = (double)a[k * p + c] / v;
         t += r * r;
       }
@@ -1213,7 +1214,7 @@ __global__ __launch_bounds__(BS) void k_red(int N, int p, const TA* __restrict__
   }
   s0 = block_sum(s0, sh);
   if (threadIdx.x == 0) part[blockIdx.x] = s0;
-  if (KIND == RED_VW) {
+  if (KIND == RED_VW || KIND == RED_VW4) {
     s1 = block_sum(s1, sh);
     if (threadIdx.x == 0) part[gridDim.x + blockIdx.x] = s1;
   }
@@ -1320,6 +1321,68 @@ __global__ __launch_bounds__(BS) void k_newton_update(int N, int p, Phys P, doub
   }
 #pragma unroll
   for (int k = 0; k < NV; ++k) U[k * p + c] = u[k];
+}
+
+// Entropy error partials (Solver.cpp:1620-1646): per block, sum of
+// ((p / rho^gamma - S_ref) / S_ref)^2 V and of V over the cells with p, rho > 0.
+template <int NV>
+__global__ __launch_bounds__(BS) void k_entropy(int N, int p, Phys P, const double* __restrict__ vol,
+                                                 const double* __restrict__ U, double S_ref,
+                                                 double* __restrict__ part) {
+  __shared__ double sh[BS / 64];
+  double s0 = 0.0, s1 = 0.0;
+  for (int c = blockIdx.x * BS + threadIdx.x; c < N; c += gridDim.x * BS) {
+    const S6<double> u = ld<NV>(U, p, c);
+    const double pr = pressure(P, u), rho = u[0];
+    if (pr <= 0 || rho <= 0) continue;
+    const double S = pr / pow(rho, P.gamma);
+    const double e = (S - S_ref) / S_ref;
+    s0 += (e * e) * vol[c];
+    s1 += vol[c];
+  }
+  s0 = block_sum(s0, sh);
+  if (threadIdx.x == 0) part[blockIdx.x] = s0;
+  s1 = block_sum(s1, sh);
+  if (threadIdx.x == 0) part[gridDim.x + blockIdx.x] = s1;
+}
+
+// Boundary-layer profile (Solver.cpp:1866-1897): for cells with
+// |x_c - x_target| < 0.005, (y_c, u / u_inf, v / u_inf, eta) with
+// eta = y_c sqrt(u_inf / (nu max(x_target, 1e-6))); flag marks them.
+template <int NV>
+__global__ __launch_bounds__(BS) void k_bl_profile(int N, int p, const double* __restrict__ xc,
+                                                    const double* __restrict__ yc, const double* __restrict__ U,
+                                                    double x_target, double u_inf, double nu,
+                                                    double* __restrict__ out4, unsigned char* __restrict__ flag) {
+  const int c = blockIdx.x * BS + threadIdx.x;
+  if (c >= N) return;
+  const double x = xc[c], y = yc[c];
+  const bool in = fabs(x - x_target) < 0.005;
+  flag[c] = in ? 1 : 0;
+  if (!in) return;
+  const double u = U[p + c] / U[c], v = U[2 * p + c] / U[c];
+  const double x_safe = fmax(x_target, 1e-6);
+  out4[c] = y;
+  out4[p + c] = u / u_inf;
+  out4[2 * p + c] = v / u_inf;
+  out4[3 * p + c] = y * sqrt(u_inf / (nu * x_safe));
+}
+
+// Dual time stepping (Solver.cpp:1831-1843): the unsteady residual
+// F = (U - U_old) vol / dt_phys + R, stored negated as the GMRES right-hand side.
+template <int NV>
+__global__ __launch_bounds__(BS) void k_unsteady_rhs(int N, int p, const double* __restrict__ vol,
+                                                      const double* __restrict__ U, const double* __restrict__ Uold,
+                                                      const double* __restrict__ R, double dtp,
+                                                      double* __restrict__ rhs) {
+  const int c = blockIdx.x * BS + threadIdx.x;
+  if (c >= N) return;
+  const double f = vol[c] / dtp;
+#pragma unroll
+  for (int k = 0; k < NV; ++k) {
+    const double F = (U[k * p + c] - Uold[k * p + c]) * f + R[k * p + c];
+    rhs[k * p + c] = F * (-1.0);
+  }
 }
 
 // Explicit stage: Unew = a0 U0 + a1 (U1 - R dt / vol), with the clips of
@@ -1710,13 +1773,15 @@ static void reduce_t(cfd_ctx* c, int kind, const TA* a, const TB* b, int slot) {
     else if (kind == RED_DOT)
       hipLaunchKernelGGL((k_red<NVt, RED_DOT, TA, TB>), dim3(nb), dim3(BS), 0, c->stream, c->N, c->pitch, a, b,
                          c->d_vol, c->red);
-    else
+    else if (kind == RED_VW)
       hipLaunchKernelGGL((k_red<NVt, RED_VW, TA, TB>), dim3(nb), dim3(BS), 0, c->stream, c->N, c->pitch, a, b,
                          c->d_vol, c->red);
+    else
+      hipLaunchKernelGGL((k_red<NVt, RED_VW4, TA, TB>), dim3(nb), dim3(BS), 0, c->stream, c->N, c->pitch, a, b,
+                         c->d_vol, c->red);
   });
-  hipLaunchKernelGGL(k_red_final, dim3(kind == RED_VW ? 2 : 1), dim3(BS), 0, c->stream, c->red, nb,
-                     kind == RED_VW ? 2 : 1,
-                     c->scal + slot);
+  const int nsum = kind == RED_VW || kind == RED_VW4 ? 2 : 1;
+  hipLaunchKernelGGL(k_red_final, dim3(nsum), dim3(BS), 0, c->stream, c->red, nb, nsum, c->scal + slot);
 }
 
 void launch_reduce(cfd_ctx* c, int kind, const double* a, const double* b, int slot) {
@@ -1764,6 +1829,26 @@ void launch_explicit(cfd_ctx* c, double* Unew, const double* U0, double a0, cons
                      const double* R, double dtc) {
   NV_DISPATCH(c->P.nvar, hipLaunchKernelGGL(k_explicit<NVt>, dim3(nblocks(c->N)), dim3(BS), 0, c->stream, c->N,
                                             c->pitch, c->d_vol, Unew, U0, a0, U1, a1, R, dtc));
+}
+
+void launch_unsteady_rhs(cfd_ctx* c, const double* U, const double* Uold, const double* R, double dtp,
+                         double* rhs) {
+  NV_DISPATCH(c->P.nvar, hipLaunchKernelGGL(k_unsteady_rhs<NVt>, dim3(nblocks(c->N)), dim3(BS), 0, c->stream, c->N,
+                                            c->pitch, c->d_vol, U, Uold, R, dtp, rhs));
+}
+
+void launch_entropy(cfd_ctx* c, const double* U, double S_ref, int slot) {
+  const int nb = red_grid(c->N);
+  NV_DISPATCH(c->P.nvar, hipLaunchKernelGGL(k_entropy<NVt>, dim3(nb), dim3(BS), 0, c->stream, c->N, c->pitch,
+                                            c->phys, c->d_vol, U, S_ref, c->red));
+  hipLaunchKernelGGL(k_red_final, dim3(2), dim3(BS), 0, c->stream, c->red, nb, 2, c->scal + slot,
+                     (const double*)nullptr);
+}
+
+void launch_bl_profile(cfd_ctx* c, const double* U, double x_target, double u_inf, double nu, double* out4,
+                       unsigned char* flag) {
+  NV_DISPATCH(c->P.nvar, hipLaunchKernelGGL(k_bl_profile<NVt>, dim3(nblocks(c->N)), dim3(BS), 0, c->stream, c->N,
+                                            c->pitch, c->d_xc, c->d_yc, U, x_target, u_inf, nu, out4, flag));
 }
 
 void launch_wall_distance(cfd_ctx* c, const double* wxy, int nw, double* wd) {

@@ -500,6 +500,49 @@ int cfd_mesh_write_gmsh(const cfd_mesh* m, const char* path) {
   return CFD_OK;
 }
 
+// main.cpp saveResults: the stream's default formatting of a double is "%g"
+// at 6 significant digits, reproduced with printf.
+int cfd_mesh_write_vtk(const cfd_mesh* m, const double* U6, double gamma, const char* path) {
+  if (!m || !U6 || !path) return CFD_ERR_ARG;
+  FILE* f = std::fopen(path, "w");
+  if (!f) return CFD_ERR_IO;
+  const int nc = m->ncells();
+  std::fprintf(f, "# vtk DataFile Version 3.0\n2D Bump\nASCII\nDATASET UNSTRUCTURED_GRID\n");
+  std::fprintf(f, "POINTS %d double\n", m->n_nodes);
+  for (int i = 1; i <= m->n_nodes; ++i) std::fprintf(f, "%.6g %.6g 0.0\n", m->x[i], m->y[i]);
+  long list = 0;
+  for (int c = 0; c < nc; ++c) list += m->cell_ptr[c + 1] - m->cell_ptr[c] + 1;
+  std::fprintf(f, "CELLS %d %ld\n", nc, list);
+  for (int c = 0; c < nc; ++c) {
+    const int b = m->cell_ptr[c], n = m->cell_ptr[c + 1] - b;
+    std::fprintf(f, "%d", n);
+    for (int k = 0; k < n; ++k) std::fprintf(f, " %d", m->cell_nodes[b + k] - 1);
+    std::fprintf(f, "\n");
+  }
+  std::fprintf(f, "CELL_TYPES %d\n", nc);
+  for (int c = 0; c < nc; ++c) std::fprintf(f, m->cell_ptr[c + 1] - m->cell_ptr[c] == 3 ? "5\n" : "9\n");
+  std::fprintf(f, "CELL_DATA %d\n", nc);
+  auto pres = [&](const double* u) {  // Flux.hpp getPressure
+    const double V2 = (u[1] * u[1] + u[2] * u[2]) / (u[0] * u[0]);
+    return (gamma - 1.0) * (u[3] - 0.5 * u[0] * V2);
+  };
+  std::fprintf(f, "SCALARS Mach double 1\nLOOKUP_TABLE default\n");
+  for (int c = 0; c < nc; ++c) {
+    const double* u = U6 + (size_t)6 * c;
+    const double p = pres(u);
+    const double a = std::sqrt(gamma * p / u[0]);
+    const double v = std::sqrt(u[1] * u[1] + u[2] * u[2]) / u[0];
+    std::fprintf(f, "%.6g\n", v / a);
+  }
+  std::fprintf(f, "SCALARS Pressure double 1\nLOOKUP_TABLE default\n");
+  for (int c = 0; c < nc; ++c) std::fprintf(f, "%.6g\n", pres(U6 + (size_t)6 * c));
+  std::fprintf(f, "SCALARS Density double 1\nLOOKUP_TABLE default\n");
+  for (int c = 0; c < nc; ++c) std::fprintf(f, "%.6g\n", U6[(size_t)6 * c]);
+  const bool ok = std::ferror(f) == 0;
+  std::fclose(f);
+  return ok ? CFD_OK : CFD_ERR_IO;
+}
+
 // Seeded synthetic meshes.  Nodes on an (nx+1) x (ny+1) grid, id = 1 + j (nx+1) + i.
 // Quads visited row-major (j outer, i inner); stream 1 draws u per quad, the quad
 // is split when u < p and a second draw's top bit picks the diagonal; a split

@@ -986,6 +986,80 @@ struct Solver {
     }
     return step;
   }
+  // Solver::solveUnsteady (Solver.cpp:1801-1864): BDF1 dual time stepping,
+  // the scalar-preconditioned FD GMRES on F = (U - U_old) vol / dt + R.
+  void solve_unsteady(St<double>* U, double dtp, int nsteps, int m, double* hvm, int* hnewton) {
+    const int n = N(), maxNewton = 15;
+    const double tolerance = 1e-6;
+    dt.assign(n, dtp);
+    std::vector<St<double>> Uold(n), Res(n), rhs(n), dU(n);
+    for (int t = 1; t <= nsteps; ++t) {
+      for (int c = 0; c < n; ++c) Uold[c] = U[c];
+      int step = 0;
+      for (step = 0; step < maxNewton; ++step) {
+        D.residual(U, Res.data(), true);
+        double norm_sq = 0.0, total_vol = 0.0;
+        for (int c = 0; c < n; ++c) {
+          double vol = geo.vol[c];
+          St<double> dUdt = (U[c] - Uold[c]) * (vol / dtp);
+          St<double> F = dUdt + Res[c];
+          rhs[c] = F * (-1.0);
+          double a = F[0] / vol, b = F[1] / vol, d = F[2] / vol, e = F[3] / vol;
+          norm_sq += (a * a + b * b + d * d + e * e) * vol;
+          total_vol += vol;
+        }
+        double resVM = std::sqrt(norm_sq / total_vol);
+        if (hvm) hvm[(size_t)(t - 1) * maxNewton + step] = resVM;
+        if (resVM < tolerance && step > 0) break;
+        gmres(U, Res.data(), rhs.data(), dU.data(), m, 1e-2);
+        for (int c = 0; c < n; ++c) U[c] = U[c] + dU[c] * 0.7;
+      }
+      if (hnewton) hnewton[t - 1] = step;
+    }
+  }
+  // Solver::calculateEntropyError (Solver.cpp:1620-1646), returned instead of printed
+  double entropy_error(const St<double>* U) const {
+    const double g = P.gamma;
+    double T_inf = P.T_total_inlet / (1.0 + 0.5 * (g - 1.0) * P.M_inf * P.M_inf);
+    double P_inf = P.P_total_inlet / std::pow(1.0 + 0.5 * (g - 1.0) * P.M_inf * P.M_inf, g / (g - 1.0));
+    double rho_inf = P_inf / (P.R_gas * T_inf);
+    double S_ref = P_inf / std::pow(rho_inf, g);
+    double se = 0.0, tv = 0.0;
+    for (int c = 0; c < N(); ++c) {
+      double p = D.pressure(U[c]), rho = U[c][0];
+      if (p <= 0 || rho <= 0) continue;
+      double S = p / std::pow(rho, g);
+      double e = (S - S_ref) / S_ref;
+      se += (e * e) * geo.vol[c];
+      tv += geo.vol[c];
+    }
+    return std::sqrt(se / tv);
+  }
+  // Solver::extractBoundaryLayerProfile (Solver.cpp:1866-1897), rows instead of a CSV
+  int bl_profile(const St<double>* U, double x_target, double* rows) const {
+    const double g = P.gamma;
+    double factor_T = 1.0 + 0.5 * (g - 1.0) * P.M_inf * P.M_inf;
+    double T_static = P.T_total_inlet / factor_T;
+    double c_inf = std::sqrt(g * P.R_gas * T_static);
+    double u_inf = P.M_inf * c_inf;
+    int n = 0;
+    for (int i = 0; i < N(); ++i) {
+      double x_c = geo.xc[i], y_c = geo.yc[i];
+      if (std::abs(x_c - x_target) < 0.005) {
+        double u = U[i][1] / U[i][0], v = U[i][2] / U[i][0];
+        double x_safe = std::max(x_target, 1e-6);
+        double eta = y_c * std::sqrt(u_inf / (P.nu * x_safe));
+        if (rows) {
+          rows[4 * n] = y_c;
+          rows[4 * n + 1] = u / u_inf;
+          rows[4 * n + 2] = v / u_inf;
+          rows[4 * n + 3] = eta;
+        }
+        ++n;
+      }
+    }
+    return n;
+  }
   // extension (SST): k and omega kept positive after an update
   void turb_floor(St<double>& u) const {
     double kmin = 1e-8 * P.turb_k_inf, wmin = 1e-3 * P.turb_w_inf;
@@ -1187,6 +1261,13 @@ int orc_solve_implicit(orc_solver* s, double* U6, double tol, int maxsteps, doub
   return s->s.solve_implicit(Solver::S(U6), tol, maxsteps, rel_drop, m, hvm, hit, hcfl, converged);
 }
 void orc_explicit_step(orc_solver* s, double* U6, double dt) { s->s.explicit_step(Solver::S(U6), dt); }
+double orc_entropy_error(orc_solver* s, const double* U6) { return s->s.entropy_error(Solver::S(U6)); }
+int orc_bl_profile(orc_solver* s, const double* U6, double x_target, double* rows) {
+  return s->s.bl_profile(Solver::S(U6), x_target, rows);
+}
+void orc_solve_unsteady(orc_solver* s, double* U6, double dtp, int nsteps, double* hvm, int* hnewton) {
+  s->s.solve_unsteady(Solver::S(U6), dtp, nsteps, 40, hvm, hnewton);
+}
 void orc_rk3_step(orc_solver* s, double* U6, double dt) { s->s.rk3_step(Solver::S(U6), dt); }
 
 // Solver::extractSkinFriction (Solver.cpp:1899-1979) per tag-4 face, Cp per

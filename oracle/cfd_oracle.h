@@ -42,6 +42,9 @@ int orc_gmres(orc_solver* s, const double* U6, const double* R6, const double* R
 int orc_solve_implicit(orc_solver* s, double* U6, double tol, int maxsteps, double rel_drop, int m, double* hvm,
                        int* hit, double* hcfl, int* converged);
 void orc_explicit_step(orc_solver* s, double* U6, double dt);
+void orc_solve_unsteady(orc_solver* s, double* U6, double dtp, int nsteps, double* hvm, int* hnewton);
+double orc_entropy_error(orc_solver* s, const double* U6);
+int orc_bl_profile(orc_solver* s, const double* U6, double x_target, double* rows);
 void orc_rk3_step(orc_solver* s, double* U6, double dt);
 void orc_wall_outputs(orc_solver* s, const double* U6, double* Cp, double* Cf, double* J);
 #ifdef __cplusplus

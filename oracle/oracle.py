@@ -79,6 +79,9 @@ def oracle_lib():
             "orc_gmres": (C.c_int, [_VP, _DP, _DP, _DP, _DP, C.c_int, C.c_double]),
             "orc_solve_implicit": (C.c_int, [_VP, _DP, C.c_double, C.c_int, C.c_double, C.c_int, _DP, _IP, _DP, _IP]),
             "orc_explicit_step": (None, [_VP, _DP, C.c_double]),
+            "orc_solve_unsteady": (None, [_VP, _DP, C.c_double, C.c_int, _DP, _IP]),
+            "orc_entropy_error": (C.c_double, [_VP, _DP]),
+            "orc_bl_profile": (C.c_int, [_VP, _DP, C.c_double, _DP]),
             "orc_rk3_step": (None, [_VP, _DP, C.c_double]),
             "orc_wall_outputs": (None, [_VP, _DP, _DP, _DP, _DP]),
         }
@@ -124,6 +127,10 @@ def ref_lib():
             "ref_norm_vw": (C.c_double, [_VP, _DP]),
             "ref_solve_implicit": (None, [_VP, _DP, C.c_double, C.c_int]),
             "ref_explicit_step": (None, [_VP, _DP, C.c_double]),
+            "ref_solve_unsteady": (None, [_VP, _DP, C.c_double, C.c_int]),
+            "ref_entropy_error": (C.c_double, [_VP, _DP]),
+            "ref_bl_profile": (None, [_VP, _DP, C.c_double, C.c_char_p]),
+            "ref_save_vtk": (None, [_VP, _DP, C.c_int]),
             "ref_set_nu": (None, [_VP, C.c_double]),
             "ref_kinematic_viscosity": (None, [_VP, C.c_double, C.c_double]),
             "ref_get_nu": (C.c_double, [_VP]),
@@ -312,6 +319,24 @@ class OracleSolver:
         self.lib.orc_explicit_step(self.h, _dp(U), dt)
         return U
 
+    def entropy_error(self, U):
+        return self.lib.orc_entropy_error(self.h, _dp(np.ascontiguousarray(U, np.float64)))
+
+    def bl_profile(self, U, x_target):
+        U = np.ascontiguousarray(U, np.float64)
+        n = self.lib.orc_bl_profile(self.h, _dp(U), x_target, None)
+        rows = np.zeros((max(n, 1), 4))
+        self.lib.orc_bl_profile(self.h, _dp(U), x_target, _dp(rows))
+        return rows[:n]
+
+    def solve_unsteady(self, U, dt_phys, n_steps):
+        """Solver::solveUnsteady: (U, VM||F|| per Newton step (n_steps x 15), updates per step)."""
+        U = np.ascontiguousarray(U, np.float64).copy()
+        hvm = np.zeros((max(1, n_steps), 15))
+        hn = np.zeros(max(1, n_steps), np.int32)
+        self.lib.orc_solve_unsteady(self.h, _dp(U), dt_phys, n_steps, _dp(hvm), _ip(hn))
+        return U, hvm[:n_steps], hn[:n_steps]
+
     def rk3_step(self, U, dt):
         U = np.ascontiguousarray(U, np.float64).copy()
         self.lib.orc_rk3_step(self.h, _dp(U), dt)
@@ -337,6 +362,10 @@ class RefMesh:
 
     def faces(self):
         return _faces_from(self.lib.ref_mesh_faces, self.h, self.n_faces)
+
+    def save_vtk(self, U, step):
+        """main.cpp saveResults: writes results/sol_<step>.vtk under the working directory."""
+        self.lib.ref_save_vtk(self.h, _dp(np.ascontiguousarray(U, np.float64)), step)
 
     def cells(self):
         out = {k: np.zeros(self.n_cells) for k in ("volume", "x_c", "y_c")}
@@ -419,6 +448,18 @@ class RefSolver:
         U = np.ascontiguousarray(U, np.float64).copy()
         self.lib.ref_explicit_step(self.h, _dp(U), dt)
         return U
+
+    def solve_unsteady(self, U, dt_phys, n_steps):
+        U = np.ascontiguousarray(U, np.float64).copy()
+        self.lib.ref_solve_unsteady(self.h, _dp(U), dt_phys, n_steps)
+        return U
+
+    def entropy_error(self, U):
+        """The value calculateEntropyError prints (6 significant digits)."""
+        return self.lib.ref_entropy_error(self.h, _dp(np.ascontiguousarray(U, np.float64)))
+
+    def bl_profile_csv(self, U, x_target, path):
+        self.lib.ref_bl_profile(self.h, _dp(np.ascontiguousarray(U, np.float64)), x_target, path.encode())
 
     def set_nu(self, nu):
         self.lib.ref_set_nu(self.h, nu)

@@ -11,8 +11,16 @@
 #include "Solver.hpp"
 #undef private
 
+#include <cstdlib>
 #include <cstring>
+#include <iostream>
+#include <sstream>
+#include <string>
 #include <vector>
+
+// main.cpp's VTK writer, compiled from main.cpp with its main() renamed
+// (oracle/Makefile); it writes results/sol_<step>.vtk under the working directory.
+void saveResults(const Mesh& mesh, const std::vector<State>& U, int step);
 
 extern "C" {
 
@@ -147,6 +155,41 @@ void ref_explicit_step(void* sp, double* U6, double dt) {
   s->U_current = to_states(U6, s->mesh.cells.size());
   s->timeStepExplicit(dt);
   from_states(s->U_current, U6);
+}
+// Solver::solveUnsteady on U6 (in/out); the reference prints per physical step.
+void ref_solve_unsteady(void* sp, double* U6, double dt_phys, int nsteps) {
+  Solver* s = static_cast<Solver*>(sp);
+  s->U_current = to_states(U6, s->mesh.cells.size());
+  s->solveUnsteady(dt_phys, nsteps);
+  from_states(s->U_current, U6);
+}
+// Solver::calculateEntropyError prints its result; the value is read back
+// from what it prints (default stream precision, 6 significant digits).
+double ref_entropy_error(void* sp, const double* U6) {
+  Solver* s = static_cast<Solver*>(sp);
+  s->U_current = to_states(U6, s->mesh.cells.size());
+  std::ostringstream cap;
+  std::streambuf* old = std::cout.rdbuf(cap.rdbuf());
+  s->calculateEntropyError();
+  std::cout.rdbuf(old);
+  const std::string t = cap.str();
+  const std::string key = "L2 Entropy Error:";
+  const size_t k = t.find(key);
+  return k == std::string::npos ? -1.0 : std::strtod(t.c_str() + k + key.size(), nullptr);
+}
+// Solver::extractBoundaryLayerProfile(x, path) on U6
+void ref_bl_profile(void* sp, const double* U6, double x_target, const char* path) {
+  Solver* s = static_cast<Solver*>(sp);
+  s->U_current = to_states(U6, s->mesh.cells.size());
+  std::ostringstream cap;
+  std::streambuf* old = std::cout.rdbuf(cap.rdbuf());
+  s->extractBoundaryLayerProfile(x_target, path);
+  std::cout.rdbuf(old);
+}
+// main.cpp saveResults(mesh, U6, step)
+void ref_save_vtk(void* mp, const double* U6, int step) {
+  const Mesh* m = static_cast<Mesh*>(mp);
+  saveResults(*m, to_states(U6, m->cells.size()), step);
 }
 void ref_set_nu(void* sp, double nu) { static_cast<Solver*>(sp)->nu = nu; }
 void ref_kinematic_viscosity(void* sp, double Re, double L) { static_cast<Solver*>(sp)->setKinematicViscosity(Re, L); }

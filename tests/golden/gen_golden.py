@@ -12,7 +12,12 @@ reference's own output (Solver.cpp compiled with its makefile flags) on:
                          trajectory;
   ref_wedge_q.npz        a seeded synthetic wedge mesh (configuration 1 at
                          quarter scale) written by cfd_mesh_write_gmsh and
-                         read back by the reference's Mesh::load.
+                         read back by the reference's Mesh::load;
+  ref_bump_1230_next.npz on the bump_1230_th mesh, Roe + second-order central:
+                         two BDF1 dual-time steps (Solver::solveUnsteady, dt
+                         1e-5) from a seeded perturbed state, the entropy
+                         error it prints, the boundary-layer profile rows it
+                         writes, and the SHA-256 of main.cpp's VTK output.
 
 The shipped meshes have CRLF line ends, which the reference's readGmsh only
 accepts in Windows text mode; a CR-stripped temporary copy is fed to it.
@@ -110,6 +115,31 @@ def gen_wedge(tmp: str):
     np.savez_compressed(os.path.join(OUT, "ref_wedge_q.npz"), **out)
 
 
+def gen_next(tmp: str):
+    import hashlib
+    src = os.path.join(oracle.REFERENCE, "mesh", "bump_1230_th")
+    path = lf_copy(src, tmp)
+    rm = oracle.RefMesh(path)
+    rs = oracle.RefSolver(rm, 0.5, 101325.0, 300.0, 1, 1)
+    U = rs.initialize()
+    U[:, :4] *= 1 + 0.01 * np.random.default_rng(3).standard_normal((U.shape[0], 4))
+    out = {"U_in": U.copy(), "dt_phys": np.float64(1e-5), "n_steps": np.int32(2)}
+    Uu = rs.solve_unsteady(U, 1e-5, 2)
+    out["unsteady_U"] = Uu
+    out["entropy_printed"] = np.float64(rs.entropy_error(Uu))
+    x_target = float(rm.cells()["x_c"][100])
+    csv = os.path.join(tmp, "bl.csv")
+    rs.bl_profile_csv(Uu, x_target, csv)
+    out["bl_x"] = np.float64(x_target)
+    out["bl_rows"] = np.loadtxt(csv, delimiter=",", skiprows=1, ndmin=2)
+    os.makedirs(os.path.join(tmp, "results"), exist_ok=True)
+    rm.save_vtk(Uu, 7)
+    data = open(os.path.join(tmp, "results", "sol_7.vtk"), "rb").read()
+    out["vtk_sha256"] = np.array(hashlib.sha256(data).hexdigest())
+    out["vtk_bytes"] = np.int64(len(data))
+    np.savez_compressed(os.path.join(OUT, "ref_bump_1230_next.npz"), **out)
+
+
 def main():
     if not oracle.ref_available():
         oracle.build(ref=True)
@@ -119,6 +149,7 @@ def main():
         try:
             gen_bump(tmp)
             gen_wedge(tmp)
+            gen_next(tmp)
         finally:
             os.chdir(cwd)
     for f in sorted(os.listdir(OUT)):
