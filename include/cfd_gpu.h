@@ -141,6 +141,15 @@ uint64_t cfd_mesh_hash(const cfd_mesh* m);
 int cfd_mesh_set_nodes(cfd_mesh* m, const double* x, const double* y);
 /* Write Gmsh v2.2 ASCII with %.17g coordinates. */
 int cfd_mesh_write_gmsh(const cfd_mesh* m, const char* path);
+/* Line segments of the line-implicit preconditioner (no reference
+ * counterpart): chains of strongly coupled cells (coupling A_f / |dx . n_f|,
+ * cells whose largest coupling is at least 4x their smallest), at most 64
+ * cells each, every other cell a segment of its own; segments listed color
+ * by color (adjacent segments differ in color).  seg_ptr (n_segments + 1),
+ * seg_cells (N, reference cell ids in chain order) and segcol_ptr
+ * (n_colors + 1) may be NULL to query the sizes. */
+int cfd_mesh_get_lines(const cfd_mesh* m, int* n_segments, int* n_colors, int* seg_ptr, int* seg_cells,
+                       int* segcol_ptr);
 /* main.cpp saveResults (main.cpp:8-51): legacy ASCII VTK of the mesh with the
  * cell fields Mach, Pressure and Density of U6 (reference cell order),
  * pressure with the given gamma (the reference uses Flux.hpp's 1.4), numbers
@@ -216,6 +225,12 @@ int cfd_block_jacobi(cfd_ctx* ctx, const double* U6, int precision, double* B, d
  * (N x 4 neighbor cell or -1) and color (N) may be NULL. */
 int cfd_block_sgs(cfd_ctx* ctx, const double* U6, int precision, int transpose, const double* x6, double* z6,
                   double* Aoff, int* nbr, int* color);
+/* Line-implicit symmetric block Gauss-Seidel (no reference counterpart): the
+ * same first-order blocks, with the cells of each line segment
+ * (cfd_mesh_get_lines) solved together by a block-tridiagonal elimination and
+ * the segments swept color by color, ascending then descending.
+ * z6 = M^-1 x6 at U6 (for inspection; GMRES uses it with precond = 3). */
+int cfd_line_precond(cfd_ctx* ctx, const double* U6, int precision, const double* x6, double* z6);
 /* Solver::norm_L2, dot_product, norm_L2_VolumeWeighted — Solver.cpp:1250-1303 */
 int cfd_norm_l2(cfd_ctx* ctx, const double* v6, double* out);
 int cfd_dot(cfd_ctx* ctx, const double* a6, const double* b6, double* out);
@@ -230,7 +245,8 @@ typedef struct cfd_gmres_opts {
                        solve J^T x = RHS with the reverse product (adjoint)      */
   int precision;    /* 64: FP64 basis/matvec; 32: FP32 basis, matvec, precond      */
   int precond;      /* 0 scalar diagonal (reference, 1424-1465), 1 block-Jacobi,   *
-                     * 2 multicolor block symmetric Gauss-Seidel                   */
+                     * 2 multicolor block symmetric Gauss-Seidel, 3 line-implicit *
+                     * symmetric Gauss-Seidel (jv 0 and 1 only)                    */
   int scaling;      /* 0: unscaled 6-component inner product (reference);
                        1: rows scaled by the freestream state, so every equation
                           weighs alike in the Krylov norm (no reference counterpart) */

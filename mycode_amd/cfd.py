@@ -119,6 +119,8 @@ SIGNATURES = [
     ("cfd_entropy_error", C.c_int, [_VP, _DP, _DP]),
     ("cfd_bl_profile", C.c_int, [_VP, _DP, C.c_double, C.c_int, _DP, _IP]),
     ("cfd_mesh_write_vtk", C.c_int, [_VP, _DP, C.c_double, C.c_char_p]),
+    ("cfd_mesh_get_lines", C.c_int, [_VP, _IP, _IP, _IP, _IP, _IP]),
+    ("cfd_line_precond", C.c_int, [_VP, _DP, C.c_int, _DP, _DP]),
     ("cfd_solve_unsteady", C.c_int, [_VP, _DP, C.c_double, C.c_int, C.POINTER(GmresOpts), C.c_int, C.c_double,
                                      C.c_double, _DP, _IP]),
     ("cfd_norm_l2", C.c_int, [_VP, _DP, _DP]),
@@ -304,6 +306,16 @@ class Mesh:
     def write_gmsh(self, path: str):
         _check(self.lib.cfd_mesh_write_gmsh(self._h, path.encode()))
 
+    def lines(self):
+        """(seg_ptr, seg_cells, segcol_ptr) of the line-implicit preconditioner's segments."""
+        ns, nc = C.c_int(), C.c_int()
+        _check(self.lib.cfd_mesh_get_lines(self._h, C.byref(ns), C.byref(nc), None, None, None))
+        sp = np.zeros(ns.value + 1, np.int32)
+        sc = np.zeros(self.n_cells, np.int32)
+        cp = np.zeros(nc.value + 1, np.int32)
+        _check(self.lib.cfd_mesh_get_lines(self._h, C.byref(ns), C.byref(nc), _ip(sp), _ip(sc), _ip(cp)))
+        return sp, sc, cp
+
     def write_vtk(self, U, path: str, gamma: float = 1.4):
         """main.cpp saveResults: legacy VTK with Mach, Pressure, Density cell fields."""
         U = np.ascontiguousarray(U, np.float64)
@@ -456,6 +468,13 @@ class Solver:
         self._c(self.lib.cfd_block_sgs(self._h, _dp(U), precision, int(transpose), dp(xz), dp(z), dp(A),
                                        ip(nb), ip(col)), "cfd_block_sgs")
         return z, A, nb, col
+
+    def line_precond(self, U, x, precision: int = 64) -> np.ndarray:
+        """z = M^-1 x of the line-implicit preconditioner at U."""
+        U, x = _state(U, self.N), _state(x, self.N)
+        z = np.zeros((self.N, 6))
+        self._c(self.lib.cfd_line_precond(self._h, _dp(U), precision, _dp(x), _dp(z)), "cfd_line_precond")
+        return z
 
     def norm_L2(self, v) -> float:
         v = _state(v, self.N)

@@ -97,6 +97,9 @@ void free_krylov(cfd_ctx* c) {
   (void)hipFree(k.blk);
   (void)hipFree(k.aoff);
   (void)hipFree(k.fjac);
+  (void)hipFree(k.bd);
+  (void)hipFree(k.lineG);
+  (void)hipFree(k.lineP);
   (void)hipFree(k.xt);
   (void)hipFree(k.z);
   (void)hipFree(k.w);
@@ -135,6 +138,20 @@ int ensure_sgs(cfd_ctx* c) {
   const size_t es = k.prec == 32 ? 4 : 8;
   HIP_TRY(c, hipMalloc(&k.aoff, (size_t)4 * c->P.nvar * c->P.nvar * c->pitch * es));
   HIP_TRY(c, hipMalloc((void**)&k.xt, (size_t)c->P.nvar * c->pitch * 8));
+  return CFD_OK;
+}
+
+// line preconditioner: the SGS buffers plus FP64 diagonal blocks and the line factors
+int ensure_line(cfd_ctx* c) {
+  int rc = ensure_sgs(c);
+  if (rc) return rc;
+  Krylov& k = c->kry;
+  if (k.bd) return CFD_OK;
+  const size_t es = k.prec == 32 ? 4 : 8;
+  const size_t nb = (size_t)c->P.nvar * c->P.nvar * c->pitch;
+  HIP_TRY(c, hipMalloc((void**)&k.bd, nb * 8));
+  HIP_TRY(c, hipMalloc(&k.lineG, nb * es));
+  HIP_TRY(c, hipMalloc(&k.lineP, nb * es));
   return CFD_OK;
 }
 
@@ -458,6 +475,59 @@ int cfd_ctx_create(const cfd_mesh* m, const cfd_params* p, cfd_ctx** out) {
     for (int t = 0; t < N; ++t) cpos[m->ccells[t]] = t;
     HIP_TRY(c, dalloc(&c->d_cpos, P_));
     HIP_TRY(c, hipMemcpy(c->d_cpos, cpos.data(), (size_t)P_ * 4, hipMemcpyHostToDevice));
+    // line segments, checked before any kernel indexes with them: a permutation
+    // of the cells, at most 64 per segment, and the previous / next slots of each
+    // cell lead to its chain neighbours
+    {
+      std::vector<char> seen(N, 0);
+      if ((int)m->seg_cells.size() != N || m->seg_ptr.empty() || m->seg_ptr.back() != N)
+        return fail(c, CFD_ERR_MESH, "line segments do not cover the cells");
+      for (size_t k = 0; k + 1 < m->seg_ptr.size(); ++k) {
+        const int b = m->seg_ptr[k], e = m->seg_ptr[k + 1];
+        if (e <= b || e - b > 64) return fail(c, CFD_ERR_MESH, "bad line segment length");
+        for (int t = b; t < e; ++t) {
+          const int cc = m->seg_cells[t];
+          if (cc < 0 || cc >= N || seen[cc]) return fail(c, CFD_ERR_MESH, "line segments are not a permutation");
+          seen[cc] = 1;
+          const int qp = m->lslot_prev[cc], qn = m->lslot_next[cc];
+          const bool ok_p = t == b ? qp == -1 : (qp >= 0 && qp < 4 && (&nq[cc].x)[qp] >= 0 &&
+                                                  ((&nq[cc].x)[qp] >> 2) == m->seg_cells[t - 1]);
+          const bool ok_n = t == e - 1 ? qn == -1 : (qn >= 0 && qn < 4 && (&nq[cc].x)[qn] >= 0 &&
+                                                      ((&nq[cc].x)[qn] >> 2) == m->seg_cells[t + 1]);
+          if (!ok_p || !ok_n) return fail(c, CFD_ERR_MESH, "line segment links are inconsistent");
+        }
+      }
+      // the sweeps read off-segment neighbours while the segment's own cells
+      // are rewritten: no face may join two non-consecutive cells of a segment
+      std::vector<int> sof(N), tpos(N);
+      for (size_t k = 0; k + 1 < m->seg_ptr.size(); ++k)
+        for (int t = m->seg_ptr[k]; t < m->seg_ptr[k + 1]; ++t) {
+          sof[m->seg_cells[t]] = (int)k;
+          tpos[m->seg_cells[t]] = t;
+        }
+      for (int i = 0; i < N; ++i)
+        for (int q = 0; q < 4; ++q) {
+          const int en = (&nq[i].x)[q];
+          if (en < 0) continue;
+          const int o = en >> 2;
+          if (sof[o] == sof[i] && std::abs(tpos[o] - tpos[i]) != 1)
+            return fail(c, CFD_ERR_MESH, "line segment touches itself");
+        }
+    }
+    c->seg_ptr = m->seg_ptr;
+    c->segcol_ptr = m->segcol_ptr;
+    std::vector<int> lpos(P_, 0);
+    for (int t = 0; t < N; ++t) lpos[m->seg_cells[t]] = t;
+    std::vector<char2> ls(P_, make_char2(-1, -1));
+    for (int i = 0; i < N; ++i) ls[i] = make_char2((signed char)m->lslot_prev[i], (signed char)m->lslot_next[i]);
+    HIP_TRY(c, dalloc(&c->d_seg_ptr, m->seg_ptr.size()));
+    HIP_TRY(c, dalloc(&c->d_seg_cells, std::max(N, 1)));
+    HIP_TRY(c, dalloc(&c->d_lpos, P_));
+    HIP_TRY(c, dalloc(&c->d_lslot, P_));
+    HIP_TRY(c, hipMemcpy(c->d_seg_ptr, m->seg_ptr.data(), m->seg_ptr.size() * 4, hipMemcpyHostToDevice));
+    HIP_TRY(c, hipMemcpy(c->d_seg_cells, m->seg_cells.data(), (size_t)N * 4, hipMemcpyHostToDevice));
+    HIP_TRY(c, hipMemcpy(c->d_lpos, lpos.data(), (size_t)P_ * 4, hipMemcpyHostToDevice));
+    HIP_TRY(c, hipMemcpy(c->d_lslot, ls.data(), (size_t)P_ * sizeof(char2), hipMemcpyHostToDevice));
     HIP_TRY(c, hipMemcpy(c->d_ccol, col.data(), P_, hipMemcpyHostToDevice));
     HIP_TRY(c, hipMemcpy(c->d_ccells, m->ccells.data(), (size_t)N * 4, hipMemcpyHostToDevice));
     HIP_TRY(c, hipMemcpy(c->d_nq, nq.data(), (size_t)P_ * sizeof(int4), hipMemcpyHostToDevice));
@@ -494,7 +564,7 @@ int cfd_ctx_destroy(cfd_ctx* c) {
   (void)hipStreamSynchronize(c->stream);
   free_krylov(c);
   void* bufs[] = {c->d_vol, c->d_xc,  c->d_yc,  c->d_slots, c->d_own, c->d_nei, c->d_tag,  c->d_nx,
-                  c->d_ny,  c->d_area, c->d_xm, c->d_ym,    c->d_iperm, c->d_facemap, c->d_alt, c->grad, c->d_wd, c->d_dnw, c->d_corder, c->d_ccol, c->d_ccells, c->d_nq, c->d_cpos,
+                  c->d_ny,  c->d_area, c->d_xm, c->d_ym,    c->d_iperm, c->d_facemap, c->d_alt, c->grad, c->d_wd, c->d_dnw, c->d_corder, c->d_ccol, c->d_ccells, c->d_nq, c->d_cpos, c->d_seg_ptr, c->d_seg_cells, c->d_lpos, c->d_lslot,
                   c->phi,   c->dt,    c->Fm,    c->red,     c->scal, c->gradT, c->adj, c->tq, c->io6};
   for (void* b : bufs) (void)hipFree(b);
   for (int i = 0; i < 6; ++i) (void)hipFree(c->vec[i]);
@@ -698,7 +768,7 @@ int cfd_block_sgs(cfd_ctx* c, const double* U6, int precision, int transpose, co
   if ((rc = ensure_krylov(c, c->kry.m > 0 ? c->kry.m : 1, precision))) return rc;
   if ((rc = ensure_sgs(c))) return rc;
   const int nv = c->P.nvar, P_ = c->pitch;
-  launch_block_precond(c, c->vec[0], c->kry.blk, precision, nullptr, c->kry.aoff);
+  launch_block_precond(c, c->vec[0], c->kry.blk, precision, nullptr, c->kry.aoff, c->d_cpos);
   if (x6) {
     if ((rc = upload_state(c, x6, c->vec[1]))) return rc;
     launch_sgs_apply(c, c->kry.blk, c->kry.aoff, c->vec[1], 64, c->vec[2], 64, precision, transpose != 0);
@@ -738,6 +808,21 @@ int cfd_block_sgs(cfd_ctx* c, const double* U6, int precision, int transpose, co
   if (color)
     for (int i = 0; i < c->N; ++i) color[c->iperm[i]] = c->mesh->cell_color[i];
   return sync(c);
+}
+
+int cfd_line_precond(cfd_ctx* c, const double* U6, int precision, const double* x6, double* z6) {
+  if (!c || !U6 || !x6 || !z6 || (precision != 64 && precision != 32)) return CFD_ERR_ARG;
+  if (!c->dt_valid) return fail(c, CFD_ERR_STATE, "no local time step computed");
+  int rc;
+  if ((rc = upload_state(c, U6, c->vec[0]))) return rc;
+  if ((rc = ensure_krylov(c, c->kry.m > 0 ? c->kry.m : 1, precision))) return rc;
+  if ((rc = ensure_line(c))) return rc;
+  Krylov& k = c->kry;
+  launch_block_precond(c, c->vec[0], k.blk, precision, k.bd, k.aoff, c->d_lpos);
+  launch_line_factor(c, k.bd, k.aoff, k.blk, k.lineP, k.lineG, precision);
+  if ((rc = upload_state(c, x6, c->vec[1]))) return rc;
+  launch_line_apply(c, k.blk, k.lineP, k.lineG, k.aoff, c->vec[1], 64, c->vec[2], 64, precision);
+  return download_state(c, c->vec[2], z6);
 }
 
 int cfd_norm_l2(cfd_ctx* c, const double* v6, double* out) {
@@ -801,7 +886,7 @@ static int gmres_cycle(cfd_ctx* c, const double* U, const double* Rb, const doub
   const size_t es = prec == 32 ? 4 : 8;
   const size_t vs = (size_t)c->P.nvar * c->pitch;
   char* V = (char*)k.V;
-  const bool bj = o->precond == 1, sgs = o->precond == 2;
+  const bool bj = o->precond == 1, sgs = o->precond == 2, line = o->precond == 3;
   const bool tr = o->jv == 2;
   double sc[6];
   row_scales(c, sc);
@@ -832,9 +917,10 @@ static int gmres_cycle(cfd_ctx* c, const double* U, const double* Rb, const doub
     void* Vj1 = V + (size_t)(j + 1) * vs * es;
     if (bj) launch_block_apply(c, k.blk, Vj, k.z, prec, tr);
     else if (sgs) launch_sgs_apply(c, k.blk, k.aoff, Vj, prec, k.z, prec, prec, tr);
+    else if (line) launch_line_apply(c, k.blk, k.lineP, k.lineG, k.aoff, Vj, prec, k.z, prec, prec);
     else launch_precond_apply(c, k.z, Vj, prec, o->jv == 0 ? S_VNORM : -1);
     if (o->jv == 0) {
-      if (bj || sgs) launch_reduce(c, RED_L2, (const double*)k.z, nullptr, S_VNORM);
+      if (bj || sgs || line) launch_reduce(c, RED_L2, (const double*)k.z, nullptr, S_VNORM);
       launch_jv_fd(c, U, Rb, (const double*)k.z, (double*)Vj1, S_UNORM, S_VNORM, tmpU, tmpR);
     } else if (o->jv == 1) {
       launch_jv_tangent(c, U, k.z, Vj1, prec);
@@ -880,6 +966,9 @@ static int gmres_cycle(cfd_ctx* c, const double* U, const double* Rb, const doub
   } else if (sgs) {
     launch_solution(c, k.xt, V, k.h + 128, kk, prec, false);
     launch_sgs_apply(c, k.blk, k.aoff, k.xt, 64, x, 64, prec, tr);
+  } else if (line) {
+    launch_solution(c, k.xt, V, k.h + 128, kk, prec, false);
+    launch_line_apply(c, k.blk, k.lineP, k.lineG, k.aoff, k.xt, 64, x, 64, prec);
   } else {
     launch_solution(c, x, V, k.h + 128, kk, prec);
   }
@@ -897,7 +986,9 @@ static int gmres_dev(cfd_ctx* c, const double* U, const double* Rb, const double
   if (prec != 64 && prec != 32) return fail(c, CFD_ERR_ARG, "precision must be 64 or 32");
   if (o->jv < 0 || o->jv > 2) return fail(c, CFD_ERR_ARG, "jv must be 0 (FD), 1 (tangent) or 2 (transpose)");
   if (o->jv != 1 && prec != 64) return fail(c, CFD_ERR_ARG, "FD Jv and the transpose run in FP64 only");
-  if (o->precond < 0 || o->precond > 2) return fail(c, CFD_ERR_ARG, "unknown preconditioner");
+  if (o->precond < 0 || o->precond > 3) return fail(c, CFD_ERR_ARG, "unknown preconditioner");
+  if (o->precond == 3 && o->jv == 2)
+    return fail(c, CFD_ERR_ARG, "the line preconditioner has no transposed form (use precond 2 for jv 2)");
   if (o->restarts < 0) return fail(c, CFD_ERR_ARG, "restarts must be >= 0");
   int rc = ensure_krylov(c, m, prec);
   if (rc) return rc;
@@ -907,7 +998,11 @@ static int gmres_dev(cfd_ctx* c, const double* U, const double* Rb, const double
     launch_block_precond(c, U, k.blk, prec);
   } else if (o->precond == 2) {
     if ((rc = ensure_sgs(c))) return rc;
-    launch_block_precond(c, U, k.blk, prec, nullptr, k.aoff);
+    launch_block_precond(c, U, k.blk, prec, nullptr, k.aoff, c->d_cpos);
+  } else if (o->precond == 3) {
+    if ((rc = ensure_line(c))) return rc;
+    launch_block_precond(c, U, k.blk, prec, k.bd, k.aoff, c->d_lpos);
+    launch_line_factor(c, k.bd, k.aoff, k.blk, k.lineP, k.lineG, prec);
   } else {
     launch_diag_precond(c, U, k.D);
   }

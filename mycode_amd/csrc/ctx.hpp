@@ -36,6 +36,9 @@ struct Krylov {
   void* aoff = nullptr;        // neighbor coupling blocks (4 x nv x nv per cell), block SGS only
   double* xt = nullptr;        // basis combination before the block SGS solve (FP64)
   double* fjac = nullptr;      // face Jacobians of the first-order flux (2 nv^2 x F), block preconditioners
+  double* bd = nullptr;        // FP64 diagonal blocks in segment order (nv^2 x pitch), line preconditioner
+  void* lineG = nullptr;       // line factors G_t = Dp_t^-1 A_{t,next} (nv^2 x pitch, basis precision)
+  void* lineP = nullptr;       // line factors P_t = Dp_t^-1 A_{t,prev} (nv^2 x pitch, basis precision)
   double* z = nullptr;         // preconditioned direction (FP64)
   double* w = nullptr;         // accumulated solution of restarted GMRES (FP64)
   double* rr = nullptr;        // true residual of a restart cycle (FP64)
@@ -72,6 +75,11 @@ struct cfd_ctx {
   unsigned char* d_ccol = nullptr;  // per cell: color of the distance-1 cell coloring
   int* d_ccells = nullptr;          // cells grouped by color
   int* d_cpos = nullptr;            // per cell: its position in d_ccells (column of its SGS blocks)
+  std::vector<int> seg_ptr, segcol_ptr;  // line segments (host copies of the ranges)
+  int* d_seg_ptr = nullptr;         // n_segments + 1
+  int* d_seg_cells = nullptr;       // cells in segment order
+  int* d_lpos = nullptr;            // per cell: position in d_seg_cells (column of its line blocks)
+  char2* d_lslot = nullptr;         // per cell: slots of the previous / next cell of its segment (-1: none)
   int4* d_nq = nullptr;             // per cell slot: (neighbor << 2) | back slot, or -1
   // solver state
   double* grad = nullptr;   // 14 x pitch (rho u v p T gradients; k, omega with SST)
@@ -147,9 +155,15 @@ void launch_store_basis(cfd_ctx* c, void* Vj1, const double* w, const double* hd
 // x = D^-1 sum_{i<=k} y_i V_i (diag) or the plain combination
 void launch_solution(cfd_ctx* c, double* x, const void* V, const double* y, int k, int prec, bool diag = true);
 // block-Jacobi: assemble + invert (Binv: nv*nv x pitch of the basis type), apply, solution
-// with Aoff (block SGS) both Binv and Aoff are stored color-major (column = d_cpos[cell])
+// Binv, Bout and Aoff are stored in column pos[cell] (pos = nullptr: the cell itself); the block
+// SGS passes d_cpos (color-major), the line preconditioner d_lpos (segment order)
 void launch_block_precond(cfd_ctx* c, const double* U, void* Binv, int prec, double* Bout = nullptr,
-                          void* Aoff = nullptr);
+                          void* Aoff = nullptr, const int* pos = nullptr);
+// line preconditioner: factorisation of the segments (Bd: diagonal blocks in FP64, segment order)
+// and one symmetric line Gauss-Seidel application z = M^-1 x (same precision rules as the block SGS)
+void launch_line_factor(cfd_ctx* c, const double* Bd, const void* Aoff, void* Dpinv, void* Pm, void* G, int prec);
+void launch_line_apply(cfd_ctx* c, const void* Dpinv, const void* Pm, const void* G, const void* Aoff, const void* x,
+                       int xprec, void* z, int zprec, int bprec);
 // multicolor block SGS: z = M^-1 x (M^-T with transpose); blocks of precision bprec, x/z of xprec/zprec
 // (bprec 64 needs FP64 x and z; bprec 32 takes FP32 x and z or FP64 x and z)
 void launch_sgs_apply(cfd_ctx* c, const void* Binv, const void* Aoff, const void* x, int xprec, void* z,

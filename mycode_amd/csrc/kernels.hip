@@ -974,7 +974,7 @@ __global__ __launch_bounds__(BS) void k_block(DMesh m, Phys P, const double* __r
 #pragma unroll
     for (int i = 0; i < NV; ++i)
 #pragma unroll
-      for (int j = 0; j < NV; ++j) Bout[(size_t)(i * NV + j) * p + c] = B[i][j];
+      for (int j = 0; j < NV; ++j) Bout[(size_t)(i * NV + j) * p + oc] = B[i][j];
   }
   // Gauss-Jordan inverse
   double Inv[NV][NV];
@@ -1154,6 +1154,212 @@ __global__ __launch_bounds__(BS) void k_sgs_color(int n, int b0, const int* __re
 #pragma unroll
     for (int j = 0; j < NV; ++j) a += (double)Binv[(size_t)(TR ? j * NV + i : i * NV + j) * p + b] * s[j];
     z[i * p + c] = (TZ)a;
+  }
+}
+
+// ---- line-implicit block Gauss-Seidel ----------------------------------------------
+// Cells are grouped into segments (chains of strongly coupled cells, at most
+// 64, or single cells); segments of one color share no face, and no face joins
+// two non-consecutive cells of a segment.  The blocks are stored by line
+// position t (the cell's index in the segment list, so a color's cells are a
+// contiguous range).  Per color, a sweep solves every segment's
+// block-tridiagonal system exactly with the couplings to other segments taken
+// from the current z:
+//   k_line_rhs    (parallel over the color's cells)
+//                 w_t = Dp_t^-1 (x_t - sum_{off-segment q} A_{t,q} z_nbr),
+//   k_line_solve  (one thread per segment, only its own values in flight)
+//                 y_t = w_t - P_t y_{t-1},   z_t = y_t - G_t z_{t+1},
+// with Dp_t = B_t - A_{t,prev} G_{t-1}, P_t = Dp_t^-1 A_{t,prev} and
+// G_t = Dp_t^-1 A_{t,next} from k_line_factor.  Colors are swept ascending,
+// then descending.
+template <int NV, class TV>
+__global__ __launch_bounds__(64) void k_line_factor(int nseg, const int* __restrict__ seg_ptr,
+                                                    const char2* __restrict__ lslot, int p,
+                                                    const int* __restrict__ seg_cells,
+                                                    const double* __restrict__ Bd, const TV* __restrict__ Aoff,
+                                                    TV* __restrict__ Dpinv, TV* __restrict__ Pm,
+                                                    TV* __restrict__ G) {
+  const int sg = blockIdx.x * 64 + threadIdx.x;
+  if (sg >= nseg) return;
+  const int b = seg_ptr[sg], e = seg_ptr[sg + 1];
+  for (int t = b; t < e; ++t) {
+    const char2 qs = lslot[seg_cells[t]];
+    double M[NV][NV];
+#pragma unroll
+    for (int i = 0; i < NV; ++i)
+#pragma unroll
+      for (int j = 0; j < NV; ++j) M[i][j] = Bd[(size_t)(i * NV + j) * p + t];
+    if (t > b) {  // M = B - A_prev G_{t-1}
+      const int qp = qs.x;
+#pragma unroll
+      for (int i = 0; i < NV; ++i) {
+        double Li[NV];
+#pragma unroll
+        for (int k = 0; k < NV; ++k) Li[k] = (double)Aoff[(size_t)((qp * NV + i) * NV + k) * p + t];
+#pragma unroll
+        for (int j = 0; j < NV; ++j) {
+          double a = 0.0;
+#pragma unroll
+          for (int k = 0; k < NV; ++k) a += Li[k] * (double)G[(size_t)(k * NV + j) * p + (t - 1)];
+          M[i][j] -= a;
+        }
+      }
+    }
+    double Inv[NV][NV];
+#pragma unroll
+    for (int i = 0; i < NV; ++i)
+#pragma unroll
+      for (int j = 0; j < NV; ++j) Inv[i][j] = i == j ? 1.0 : 0.0;
+#pragma unroll
+    for (int col = 0; col < NV; ++col) {  // Gauss-Jordan, partial pivoting by selects
+      int piv = col;
+      double best = fabs(M[col][col]);
+#pragma unroll
+      for (int r = col + 1; r < NV; ++r)
+        if (fabs(M[r][col]) > best) {
+          best = fabs(M[r][col]);
+          piv = r;
+        }
+#pragma unroll
+      for (int r = col + 1; r < NV; ++r) {
+        if (piv == r) {
+#pragma unroll
+          for (int j = 0; j < NV; ++j) {
+            double tt = M[col][j];
+            M[col][j] = M[r][j];
+            M[r][j] = tt;
+            tt = Inv[col][j];
+            Inv[col][j] = Inv[r][j];
+            Inv[r][j] = tt;
+          }
+        }
+      }
+      const double ip = 1.0 / M[col][col];
+#pragma unroll
+      for (int j = 0; j < NV; ++j) {
+        M[col][j] *= ip;
+        Inv[col][j] *= ip;
+      }
+#pragma unroll
+      for (int r = 0; r < NV; ++r) {
+        if (r == col) continue;
+        const double f = M[r][col];
+#pragma unroll
+        for (int j = 0; j < NV; ++j) {
+          M[r][j] -= f * M[col][j];
+          Inv[r][j] -= f * Inv[col][j];
+        }
+      }
+    }
+#pragma unroll
+    for (int i = 0; i < NV; ++i)
+#pragma unroll
+      for (int j = 0; j < NV; ++j) Dpinv[(size_t)(i * NV + j) * p + t] = (TV)Inv[i][j];
+    // P_t = Dp^-1 A_prev, G_t = Dp^-1 A_next
+#pragma unroll
+    for (int side = 0; side < 2; ++side) {
+      const int q = side == 0 ? qs.x : qs.y;
+      if (q < 0) continue;
+      TV* out = side == 0 ? Pm : G;
+#pragma unroll
+      for (int j = 0; j < NV; ++j) {
+        double Ac[NV];
+#pragma unroll
+        for (int k = 0; k < NV; ++k) Ac[k] = (double)Aoff[(size_t)((q * NV + k) * NV + j) * p + t];
+#pragma unroll
+        for (int i = 0; i < NV; ++i) {
+          double a = 0.0;
+#pragma unroll
+          for (int k = 0; k < NV; ++k) a += Inv[i][k] * Ac[k];
+          out[(size_t)(i * NV + j) * p + t] = (TV)a;
+        }
+      }
+    }
+  }
+}
+
+template <int NV, class TB, class TX, class TZ>
+__global__ __launch_bounds__(BS) void k_line_rhs(int t0, int t1, const int* __restrict__ seg_cells,
+                                                  const char2* __restrict__ lslot, const int4* __restrict__ nq,
+                                                  int p, const TB* __restrict__ Dpinv, const TB* __restrict__ Aoff,
+                                                  const TX* __restrict__ x, TZ* __restrict__ z) {
+  const int t = t0 + xcd_block() * BS + threadIdx.x;
+  if (t >= t1) return;
+  const int c = seg_cells[t];
+  const char2 qs = lslot[c];
+  double s[NV];
+#pragma unroll
+  for (int i = 0; i < NV; ++i) s[i] = (double)x[i * p + c];
+  const int4 e4 = nq[c];
+  const int es[4] = {e4.x, e4.y, e4.z, e4.w};
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    const int en = es[q];
+    if (en < 0 || q == qs.x || q == qs.y) continue;  // no neighbor, or a neighbor in this segment
+    const int o = en >> 2;
+    double zo[NV];
+#pragma unroll
+    for (int j = 0; j < NV; ++j) zo[j] = (double)z[j * p + o];
+#pragma unroll
+    for (int i = 0; i < NV; ++i)
+#pragma unroll
+      for (int j = 0; j < NV; ++j) s[i] -= (double)Aoff[(size_t)((q * NV + i) * NV + j) * p + t] * zo[j];
+  }
+#pragma unroll
+  for (int i = 0; i < NV; ++i) {
+    double a = 0.0;
+#pragma unroll
+    for (int j = 0; j < NV; ++j) a += (double)Dpinv[(size_t)(i * NV + j) * p + t] * s[j];
+    z[i * p + c] = (TZ)a;
+  }
+}
+
+template <int NV, class TB, class TZ>
+__global__ __launch_bounds__(64) void k_line_solve(int s0, int s1, const int* __restrict__ seg_ptr,
+                                                   const int* __restrict__ seg_cells, int p,
+                                                   const TB* __restrict__ Pm, const TB* __restrict__ G,
+                                                   TZ* __restrict__ z) {
+  const int sg = s0 + blockIdx.x * 64 + threadIdx.x;
+  if (sg >= s1) return;
+  const int b = seg_ptr[sg], e = seg_ptr[sg + 1];
+  if (e - b < 2) return;  // a single cell: w is already its solution
+  double y[NV];
+  {
+    const int c = seg_cells[b];
+#pragma unroll
+    for (int i = 0; i < NV; ++i) y[i] = (double)z[i * p + c];
+  }
+  for (int t = b + 1; t < e; ++t) {  // y_t = w_t - P_t y_{t-1}
+    const int c = seg_cells[t];
+    double yn[NV];
+#pragma unroll
+    for (int i = 0; i < NV; ++i) {
+      double a = (double)z[i * p + c];
+#pragma unroll
+      for (int j = 0; j < NV; ++j) a -= (double)Pm[(size_t)(i * NV + j) * p + t] * y[j];
+      yn[i] = a;
+    }
+#pragma unroll
+    for (int i = 0; i < NV; ++i) {
+      y[i] = yn[i];
+      z[i * p + c] = (TZ)yn[i];
+    }
+  }
+  for (int t = e - 2; t >= b; --t) {  // z_t = y_t - G_t z_{t+1}; y holds z_{t+1}
+    const int c = seg_cells[t];
+    double zn[NV];
+#pragma unroll
+    for (int i = 0; i < NV; ++i) {
+      double a = (double)z[i * p + c];
+#pragma unroll
+      for (int j = 0; j < NV; ++j) a -= (double)G[(size_t)(i * NV + j) * p + t] * y[j];
+      zn[i] = a;
+    }
+#pragma unroll
+    for (int i = 0; i < NV; ++i) {
+      y[i] = zn[i];
+      z[i * p + c] = (TZ)zn[i];
+    }
   }
 }
 
@@ -1934,7 +2140,8 @@ void launch_store_basis(cfd_ctx* c, void* Vj1, const double* w, const double* hd
   else store_t<double>(c, (double*)Vj1, hdev + j + 1);
 }
 
-template <class TV> static void block_t(cfd_ctx* c, const double* U, TV* Binv, double* Bout, TV* Aoff) {
+template <class TV>
+static void block_t(cfd_ctx* c, const double* U, TV* Binv, double* Bout, TV* Aoff, const int* pos) {
   NV_DISPATCH(c->P.nvar, {
     const bool turb = NVt == 6 && c->P.turb;
     hipLaunchKernelGGL((k_gradlim<NVt, double>), dim3(nblocks(c->N)), dim3(BS), 0, c->stream, c->dm, c->phys, U,
@@ -1944,12 +2151,56 @@ template <class TV> static void block_t(cfd_ctx* c, const double* U, TV* Binv, d
                        c->kry.fjac);
     hipLaunchKernelGGL((k_block<NVt, TV>), dim3(nblocks(c->N)), dim3(BS), 0, c->stream, c->dm, c->phys, U,
                        (const double*)c->grad, c->dt, (const double*)c->kry.fjac, Binv, Bout, Aoff,
-                       turb ? (const double*)c->tq : nullptr, Aoff ? (const int*)c->d_cpos : nullptr);
+                       turb ? (const double*)c->tq : nullptr, pos);
   });
 }
-void launch_block_precond(cfd_ctx* c, const double* U, void* Binv, int prec, double* Bout, void* Aoff) {
-  if (prec == 32) block_t<float>(c, U, (float*)Binv, Bout, (float*)Aoff);
-  else block_t<double>(c, U, (double*)Binv, Bout, (double*)Aoff);
+void launch_block_precond(cfd_ctx* c, const double* U, void* Binv, int prec, double* Bout, void* Aoff,
+                          const int* pos) {
+  if (prec == 32) block_t<float>(c, U, (float*)Binv, Bout, (float*)Aoff, pos);
+  else block_t<double>(c, U, (double*)Binv, Bout, (double*)Aoff, pos);
+}
+template <class TV>
+static void line_factor_t(cfd_ctx* c, const double* Bd, const TV* Aoff, TV* Dpinv, TV* Pm, TV* G) {
+  const int ns = (int)c->seg_ptr.size() - 1;
+  NV_DISPATCH(c->P.nvar, hipLaunchKernelGGL((k_line_factor<NVt, TV>), dim3((ns + 63) / 64), dim3(64), 0, c->stream,
+                                            ns, c->d_seg_ptr, c->d_lslot, c->pitch, c->d_seg_cells, Bd, Aoff,
+                                            Dpinv, Pm, G));
+}
+void launch_line_factor(cfd_ctx* c, const double* Bd, const void* Aoff, void* Dpinv, void* Pm, void* G, int prec) {
+  if (prec == 32) line_factor_t<float>(c, Bd, (const float*)Aoff, (float*)Dpinv, (float*)Pm, (float*)G);
+  else line_factor_t<double>(c, Bd, (const double*)Aoff, (double*)Dpinv, (double*)Pm, (double*)G);
+}
+template <class TB, class TX, class TZ>
+static void line_t(cfd_ctx* c, const TB* Dpinv, const TB* Pm, const TB* G, const TB* Aoff, const TX* x, TZ* z) {
+  const int nc = (int)c->segcol_ptr.size() - 1;
+  NV_DISPATCH(c->P.nvar, {
+    hipLaunchKernelGGL(k_fill<TZ>, dim3(nblocks((size_t)NVt * c->pitch)), dim3(BS), 0, c->stream, z, TZ(0),
+                       (size_t)NVt * c->pitch);
+    for (int s = 0; s < 2 * nc - 1; ++s) {
+      const int col = s < nc ? s : 2 * nc - 2 - s;
+      const int s0 = c->segcol_ptr[col], s1 = c->segcol_ptr[col + 1];
+      if (s1 <= s0) continue;
+      const int t0 = c->seg_ptr[s0], t1 = c->seg_ptr[s1];
+      hipLaunchKernelGGL((k_line_rhs<NVt, TB, TX, TZ>), dim3(nblocks(t1 - t0)), dim3(BS), 0, c->stream, t0, t1,
+                         c->d_seg_cells, c->d_lslot, c->d_nq, c->pitch, Dpinv, Aoff, x, z);
+      hipLaunchKernelGGL((k_line_solve<NVt, TB, TZ>), dim3((s1 - s0 + 63) / 64), dim3(64), 0, c->stream, s0, s1,
+                         c->d_seg_ptr, c->d_seg_cells, c->pitch, Pm, G, z);
+    }
+  });
+}
+void launch_line_apply(cfd_ctx* c, const void* Dpinv, const void* Pm, const void* G, const void* Aoff, const void* x,
+                       int xprec, void* z, int zprec, int bprec) {
+  if (bprec == 32) {
+    if (xprec == 32 && zprec == 32)
+      line_t<float, float, float>(c, (const float*)Dpinv, (const float*)Pm, (const float*)G, (const float*)Aoff,
+                                  (const float*)x, (float*)z);
+    else
+      line_t<float, double, double>(c, (const float*)Dpinv, (const float*)Pm, (const float*)G, (const float*)Aoff,
+                                    (const double*)x, (double*)z);
+  } else {
+    line_t<double, double, double>(c, (const double*)Dpinv, (const double*)Pm, (const double*)G,
+                                   (const double*)Aoff, (const double*)x, (double*)z);
+  }
 }
 template <class TB, class TX, class TZ, bool TR>
 static void sgs_t(cfd_ctx* c, const TB* Binv, const TB* Aoff, const TX* x, TZ* z) {

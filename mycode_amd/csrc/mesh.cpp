@@ -253,6 +253,170 @@ void color_cells(cfd_mesh& m) {
   for (int i = 0; i < N; ++i) m.ccells[pos[m.cell_color[i]]++] = i;
 }
 
+// Line segments for the line-implicit preconditioner.  The coupling of two
+// cells through face f is w = A_f / |(x_n - x_o) . n_f| (the two-point
+// diffusion coefficient); a cell is anisotropic when its largest coupling is
+// at least 4 times its smallest.  From the most anisotropic unassigned cells,
+// chains grow in both directions through each cell's strongest coupling
+// (other than the one it came from) while the next cell is unassigned and
+// anisotropic and that face is at least half of both cells' strongest
+// couplings.  Chains are cut into segments of at most 64 cells; every other
+// cell is a segment of its own.  Segments are colored greedily so that no two
+// adjacent segments share a color.
+void build_lines(cfd_mesh& m) {
+  const int N = m.ncells(), F = m.nfaces();
+  constexpr int LMAX = 64;
+  // per RCM position: up to 4 slots in ascending reference face index
+  std::vector<int> sl_nb((size_t)4 * N, -1), cnt(N, 0);
+  std::vector<double> sl_w((size_t)4 * N, 0.0);
+  for (int f = 0; f < F; ++f) {
+    const int o = m.perm[m.own[f]];
+    const int n = m.nei[f] >= 0 ? m.perm[m.nei[f]] : -1;
+    double w = 0.0;
+    if (n >= 0) {
+      const int ro = m.own[f], rn = m.nei[f];
+      const double d = std::abs((m.xc[rn] - m.xc[ro]) * m.nx[f] + (m.yc[rn] - m.yc[ro]) * m.ny[f]);
+      w = m.area[f] / std::max(d, 1e-300);
+    }
+    sl_nb[(size_t)4 * o + cnt[o]] = n;
+    sl_w[(size_t)4 * o + cnt[o]++] = w;
+    if (n >= 0) {
+      sl_nb[(size_t)4 * n + cnt[n]] = o;
+      sl_w[(size_t)4 * n + cnt[n]++] = w;
+    }
+  }
+  std::vector<double> wmax(N, 0.0), ratio(N, 1.0);
+  for (int i = 0; i < N; ++i) {
+    double lo = 1e300, hi = 0.0;
+    int ni = 0;
+    for (int q = 0; q < 4; ++q)
+      if (sl_nb[(size_t)4 * i + q] >= 0) {
+        lo = std::min(lo, sl_w[(size_t)4 * i + q]);
+        hi = std::max(hi, sl_w[(size_t)4 * i + q]);
+        ++ni;
+      }
+    wmax[i] = hi;
+    ratio[i] = ni >= 2 ? hi / lo : 1.0;
+  }
+  auto aniso = [&](int i) { return ratio[i] >= 4.0; };
+  auto link = [&](int a, int b) {  // coupling of a to its neighbor b
+    for (int q = 0; q < 4; ++q)
+      if (sl_nb[(size_t)4 * a + q] == b) return sl_w[(size_t)4 * a + q];
+    return 0.0;
+  };
+  // the strongest neighbor of `cur` other than `avoid`, if the chain may continue there
+  std::vector<char> used(N, 0);
+  std::vector<int> chain_id(N, -1);
+  int cur_chain = 0;
+  auto step = [&](int cur, int avoid) {
+    int best = -1;
+    double bw = 0.0;
+    for (int q = 0; q < 4; ++q) {
+      const int b = sl_nb[(size_t)4 * cur + q];
+      if (b < 0 || b == avoid) continue;
+      if (sl_w[(size_t)4 * cur + q] > bw) {
+        bw = sl_w[(size_t)4 * cur + q];
+        best = b;
+      }
+    }
+    if (best < 0 || used[best] || !aniso(best)) return -1;
+    if (bw < 0.5 * wmax[cur] || link(best, cur) < 0.5 * wmax[best]) return -1;
+    for (int q = 0; q < 4; ++q) {  // a chain touches itself only between consecutive cells
+      const int b = sl_nb[(size_t)4 * best + q];
+      if (b >= 0 && b != cur && chain_id[b] == cur_chain) return -1;
+    }
+    return best;
+  };
+  std::vector<int> order(N);
+  for (int i = 0; i < N; ++i) order[i] = i;
+  std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return ratio[a] > ratio[b]; });
+  std::vector<std::vector<int>> segs;
+  for (int s : order) {
+    if (used[s]) continue;
+    std::vector<int> chain{s};
+    used[s] = 1;
+    chain_id[s] = ++cur_chain;
+    if (aniso(s)) {
+      for (int cur = s, prev = -1;;) {  // forward
+        const int nx = step(cur, prev);
+        if (nx < 0) break;
+        chain.push_back(nx);
+        used[nx] = 1;
+        chain_id[nx] = cur_chain;
+        prev = cur;
+        cur = nx;
+      }
+      std::vector<int> back;
+      for (int cur = s, prev = chain.size() > 1 ? chain[1] : -1;;) {  // backward
+        const int nx = step(cur, prev);
+        if (nx < 0) break;
+        back.push_back(nx);
+        used[nx] = 1;
+        chain_id[nx] = cur_chain;
+        prev = cur;
+        cur = nx;
+      }
+      chain.insert(chain.begin(), back.rbegin(), back.rend());
+    }
+    for (size_t b = 0; b < chain.size(); b += LMAX)
+      segs.emplace_back(chain.begin() + b, chain.begin() + std::min(chain.size(), b + LMAX));
+  }
+  // segment coloring (segments in order of their smallest cell position)
+  const int ns = (int)segs.size();
+  std::vector<int> seg_of(N, -1);
+  for (int k = 0; k < ns; ++k)
+    for (int c : segs[k]) seg_of[c] = k;
+  std::vector<int> sorder(ns);
+  for (int k = 0; k < ns; ++k) sorder[k] = k;
+  std::vector<int> smin(ns);
+  for (int k = 0; k < ns; ++k) smin[k] = *std::min_element(segs[k].begin(), segs[k].end());
+  std::stable_sort(sorder.begin(), sorder.end(), [&](int a, int b) { return smin[a] < smin[b]; });
+  std::vector<int> scol(ns, -1);
+  int nc = 0;
+  std::vector<char> taken;
+  for (int k : sorder) {
+    taken.assign(nc + 1, 0);
+    for (int c : segs[k])
+      for (int q = 0; q < 4; ++q) {
+        const int b = sl_nb[(size_t)4 * c + q];
+        if (b >= 0 && seg_of[b] != k && scol[seg_of[b]] >= 0) taken[scol[seg_of[b]]] = 1;
+      }
+    int col = 0;
+    while (taken[col]) ++col;
+    scol[k] = col;
+    nc = std::max(nc, col + 1);
+  }
+  // color-major segment list
+  std::vector<int> lst;
+  lst.reserve(ns);
+  m.segcol_ptr.assign(nc + 1, 0);
+  for (int col = 0; col < nc; ++col) {
+    m.segcol_ptr[col] = (int)lst.size();
+    for (int k : sorder)
+      if (scol[k] == col) lst.push_back(k);
+  }
+  m.segcol_ptr[nc] = (int)lst.size();
+  m.seg_ptr.assign(1, 0);
+  m.seg_cells.clear();
+  m.seg_cells.reserve(N);
+  m.lslot_prev.assign(N, -1);
+  m.lslot_next.assign(N, -1);
+  auto slot_of = [&](int a, int b) {
+    for (int q = 0; q < 4; ++q)
+      if (sl_nb[(size_t)4 * a + q] == b) return q;
+    return -1;
+  };
+  for (int k : lst) {
+    const std::vector<int>& sg = segs[k];
+    for (size_t i = 0; i < sg.size(); ++i) {
+      m.seg_cells.push_back(sg[i]);
+      if (i > 0) m.lslot_prev[sg[i]] = slot_of(sg[i], sg[i - 1]);
+      if (i + 1 < sg.size()) m.lslot_next[sg[i]] = slot_of(sg[i], sg[i + 1]);
+    }
+    m.seg_ptr.push_back((int)m.seg_cells.size());
+  }
+}
+
 // ---- synthetic meshes -------------------------------------------------------
 struct SplitMix64 {  // Vigna's SplitMix64
   uint64_t s;
@@ -308,6 +472,7 @@ int finish(cfd_mesh& m, std::string& err) {
   rcm(m);
   color_faces(m);
   color_cells(m);
+  build_lines(m);
   return CFD_OK;
 }
 }  // namespace cfdmesh
@@ -475,6 +640,19 @@ uint64_t cfd_mesh_hash(const cfd_mesh* m) {
   feed(m->cell_nodes.data(), m->cell_nodes.size() * 4);
   feed(m->bedges.data(), m->bedges.size() * 4);
   return h;
+}
+
+int cfd_mesh_get_lines(const cfd_mesh* m, int* n_segments, int* n_colors, int* seg_ptr, int* seg_cells,
+                       int* segcol_ptr) {
+  if (!m || !n_segments || !n_colors) return CFD_ERR_ARG;
+  const int ns = (int)m->seg_ptr.size() - 1, nc = (int)m->segcol_ptr.size() - 1;
+  *n_segments = ns;
+  *n_colors = nc;
+  if (seg_ptr) std::copy(m->seg_ptr.begin(), m->seg_ptr.end(), seg_ptr);
+  if (seg_cells)
+    for (size_t i = 0; i < m->seg_cells.size(); ++i) seg_cells[i] = m->iperm[m->seg_cells[i]];
+  if (segcol_ptr) std::copy(m->segcol_ptr.begin(), m->segcol_ptr.end(), segcol_ptr);
+  return CFD_OK;
 }
 
 int cfd_mesh_write_gmsh(const cfd_mesh* m, const char* path) {

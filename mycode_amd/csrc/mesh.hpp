@@ -31,6 +31,15 @@ struct cfd_mesh {
   std::vector<int> cell_color;  // per RCM position
   std::vector<int> ccells;      // RCM positions grouped by cell color, ascending within a color
   std::vector<int> ccell_ptr;   // n_cell_colors + 1
+  // line segments of the line-implicit preconditioner (RCM positions): cells
+  // grouped into chains along their strongest couplings, cut to at most 64
+  // cells; segments are listed color by color (no two segments of one color
+  // share a face), each segment's cells in chain order
+  std::vector<int> seg_ptr;     // n_segments + 1 offsets into seg_cells
+  std::vector<int> seg_cells;   // N cells
+  std::vector<int> segcol_ptr;  // n_segment_colors + 1 offsets into the segment list
+  std::vector<int> lslot_prev;  // per RCM position: slot of the previous cell of its segment, or -1
+  std::vector<int> lslot_next;  // per RCM position: slot of the next cell of its segment, or -1
   // synthetic-mesh extras (valid after cfd_mesh_generate)
   int config = 0;
   std::vector<double> init_U6;  // seeded initial state, reference order (may be empty)

@@ -100,3 +100,36 @@ def test_mesh_errors():
     cn = np.array([1, 2, 3, 1, 2, 4, 1, 2, 5])
     with pytest.raises(cfd.CfdError):
         cfd.Mesh.from_arrays(x, y, cp, cn, np.zeros((0, 3)))
+
+
+@pytest.mark.parametrize("config,scale", [(2, 0.25), (3, 0.1), (4, 0.05)])
+def test_line_segments(config, scale):
+    """Line segments of the line-implicit preconditioner: a permutation of the
+    cells in runs of at most 64, consecutive cells of a segment share a face,
+    no two segments of one color share a face, and on the stretched
+    boundary-layer meshes the wall-adjacent cells start long chains."""
+    m = cfd.Mesh.generate(config, scale)
+    sp, cells, cp = m.lines()
+    N = m.n_cells
+    assert sp[0] == 0 and sp[-1] == N and cp[0] == 0 and cp[-1] == len(sp) - 1
+    assert np.array_equal(np.sort(cells), np.arange(N))
+    lens = np.diff(sp)
+    assert lens.min() >= 1 and lens.max() <= 64
+    f = m.faces()
+    inter = f["neighbor"] >= 0
+    pairs = set(zip(f["owner"][inter].tolist(), f["neighbor"][inter].tolist()))
+    adjacent = lambda a, b: (a, b) in pairs or (b, a) in pairs
+    seg_of = np.empty(N, int)
+    for k in range(len(sp) - 1):
+        seg_of[cells[sp[k]:sp[k + 1]]] = k
+        for t in range(sp[k] + 1, sp[k + 1]):
+            assert adjacent(cells[t - 1], cells[t])
+    color = np.empty(len(sp) - 1, int)
+    for col in range(len(cp) - 1):
+        color[cp[col]:cp[col + 1]] = col
+    o, n = f["owner"][inter], f["neighbor"][inter]
+    so, sn = seg_of[o], seg_of[n]
+    cross = so != sn
+    assert np.all(color[so[cross]] != color[sn[cross]])
+    if config in (2, 3):
+        assert lens.max() >= 10
