@@ -271,7 +271,7 @@ typedef struct cfd_newton_opts {
                               clamped to [cfl_min, cfl_max] (no reference counterpart);
                            2: adaptive: after a step that lowered VM by 1 % or more,
                               x2 (GMRES at its tolerance) or x1.25 (GMRES missed it);
-                              after a smaller change up to a 20 % rise, held, or x0.7
+                              after a smaller change up to a 20 % rise, x1.1, or x0.7
                               when GMRES missed its tolerance; x0.5 after a larger rise;
                               a rise above 10x (or NaN) is rejected: the state is
                               restored and CFL x0.1; CFL stays in [0.01 cfl0, cfl_max]

@@ -1117,7 +1117,9 @@ int cfd_solve_implicit(cfd_ctx* c, double* U6, const cfd_newton_opts* opts, cfd_
         cfl *= gmres_ok ? 2.0 : 1.25;
       } else if (!gmres_ok) {  // stalled with an unconverged linear solve: an easier system
         cfl *= 0.7;
-      }  // a stall or small rise with GMRES at its tolerance holds the CFL
+      } else {  // a stall or small rise with GMRES at its tolerance: mild growth, so that a
+        cfl *= 1.1;  // transient that must pass through higher residuals is not frozen
+      }
       cfl = std::min(std::max(cfl, cmin), cmax);
       launch_timestep(c, U, cfl);
     }
