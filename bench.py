@@ -13,12 +13,12 @@ The JSON line carries:
                      inputs and outputs per residual over its HIP-event time;
   roofline_residual  the whole residual against B_res = N(16 n_v + 24) + 48 F;
   jfnk               FD Jv + FP64 Krylov against the tangent Jv with FP32
-                     matvec, block SGS and basis: milliseconds per GMRES
-                     iteration on this workload (and the FP32 / FD solution
+                     matvec, preconditioner and basis (line-implicit block
+                     Gauss-Seidel in both): milliseconds per GMRES iteration
+                     on this workload (and the FP32 / FD solution
                      differences), and the wall time to a 1e-10 drop of the
-                     volume-weighted residual norm on the inviscid bump case
-                     that converges within the benchmark's time (the SST plate
-                     does not reach 1e-10 with this preconditioner);
+                     volume-weighted residual norm on the laminar flat plate
+                     (the SST plate does not reach 1e-10 with this solver);
   cpu_baseline       the CPU restatement of the reference residual (g++ -O2,
                      one core) on the same mesh and state.
 """
@@ -84,11 +84,12 @@ def cpu_baseline(mesh, p, U, budget_s=25.0):
                       f"(median {t:.3f} s), oracle/cfd_oracle.cpp built g++ -O2"}
 
 
-def gmres_iteration_cost(solver, Ud, Rd, nv, pitch, cfl=50.0, m=40):
+def gmres_iteration_cost(solver, Ud, Rd, nv, pitch, cfl=50.0, m=40, precond=3):
     """Per-iteration cost of GMRES(m) at the bench state on the bench mesh:
-    FD Jv + FP64 Krylov vs the tangent with FP32 matvec, block SGS and basis
-    (and the FP64 tangent, to separate the FP32 error from the FD error).
-    The tolerance is unreachable, so every variant runs all m iterations."""
+    FD Jv + FP64 Krylov vs the tangent with FP32 matvec, preconditioner and
+    basis (and the FP64 tangent, to separate the FP32 error from the FD
+    error), both with the line-implicit preconditioner.  The tolerance is
+    unreachable, so every variant runs all m iterations."""
     import torch
     solver.dev_local_timestep(Ud.data_ptr(), cfl)
     solver.dev_residual(Ud.data_ptr(), Rd.data_ptr(), False)
@@ -97,7 +98,7 @@ def gmres_iteration_cost(solver, Ud, Rd, nv, pitch, cfl=50.0, m=40):
     out, xs = {}, {}
     for name, jv, prec in (("fd_fp64", 0, 64), ("tangent_fp64", 1, 64), ("tangent_fp32", 1, 32)):
         x = torch.zeros_like(Rd)
-        o = solver.gmres_opts(m=m, tol=1e-30, jv=jv, precision=prec, precond=2, restarts=0)
+        o = solver.gmres_opts(m=m, tol=1e-30, jv=jv, precision=prec, precond=precond, restarts=0)
         solver.dev_gmres(Ud.data_ptr(), Rd.data_ptr(), rhs.data_ptr(), x.data_ptr(), o)  # warm-up, allocations
         torch.cuda.synchronize()
         t0 = time.perf_counter()
@@ -111,7 +112,7 @@ def gmres_iteration_cost(solver, Ud, Rd, nv, pitch, cfl=50.0, m=40):
     out["speedup_tangent_fp32_vs_fd_fp64"] = out["fd_fp64"]["ms_per_iteration"] / out["tangent_fp32"]["ms_per_iteration"]
     out["rel_diff_fp32_vs_fp64_solution"] = rel(xs["tangent_fp32"])
     out["rel_diff_fd_vs_tangent_solution"] = rel(xs["fd_fp64"])
-    out["setup"] = f"GMRES({m}), block SGS, CFL {cfl}, at the benchmark state"
+    out["setup"] = f"GMRES({m}), line-implicit block Gauss-Seidel, CFL {cfl}, at the benchmark state"
     return out
 
 
@@ -119,16 +120,17 @@ def mesh_cells(solver):
     return solver.N
 
 
-def jfnk_converge(config=4, scale=0.25):
+def jfnk_converge(config=2, scale=1.0):
     """Time to a 1e-10 drop of the volume-weighted residual norm, FD Jv + FP64
-    Krylov against tangent Jv + FP32 Krylov, same preconditioner (block SGS),
-    GMRES(40) at tolerance 1e-2 and the adaptive CFL controller."""
+    Krylov against tangent Jv + FP32 Krylov, same preconditioner (line-implicit
+    block Gauss-Seidel), GMRES(40) at tolerance 1e-2 and the adaptive CFL
+    controller, on the laminar flat plate (76,790 cells)."""
     from mycode_amd import cfd, configs
     mesh, p, U0 = configs.setup(config, scale)
     out, sol = {}, {}
     for name, jv, prec in (("fd_fp64", 0, 64), ("tangent_fp32", 1, 32)):
         s = cfd.Solver(mesh, p)
-        g = s.gmres_opts(m=40, tol=1e-2, jv=jv, precision=prec, precond=2, restarts=0)
+        g = s.gmres_opts(m=40, tol=1e-2, jv=jv, precision=prec, precond=3, restarts=0)
         o = s.newton_opts(gmres=g, tolerance=0.0, rel_drop=1e-10, max_steps=600, controller=2, cfl0=1.0,
                           cfl_max=1e12)
         U, res, hist = s.solve_implicit(U0, o)
@@ -138,8 +140,7 @@ def jfnk_converge(config=4, scale=0.25):
     out["speedup_tangent_fp32_vs_fd_fp64"] = out["fd_fp64"]["seconds"] / out["tangent_fp32"]["seconds"]
     out["max_abs_Cp_diff"] = float(np.max(np.abs(sol["fd_fp64"][0] - sol["tangent_fp32"][0])))
     out["max_abs_Cf_diff"] = float(np.max(np.abs(sol["fd_fp64"][1] - sol["tangent_fp32"][1])))
-    out["workload"] = (f"inviscid bump channel M0.7, HLLC 2nd-order central, {mesh.n_cells} cells "
-                       f"(configuration {config} mesh scaled by {scale})")
+    out["workload"] = WORKLOADS[config] + f" ({mesh.n_cells} cells)"
     return out
 
 
