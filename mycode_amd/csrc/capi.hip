@@ -100,6 +100,7 @@ void free_krylov(cfd_ctx* c) {
   (void)hipFree(k.bd);
   (void)hipFree(k.lineG);
   (void)hipFree(k.lineP);
+  (void)hipFree(k.lineW);
   (void)hipFree(k.xt);
   (void)hipFree(k.z);
   (void)hipFree(k.w);
@@ -148,10 +149,11 @@ int ensure_line(cfd_ctx* c) {
   Krylov& k = c->kry;
   if (k.bd) return CFD_OK;
   const size_t es = k.prec == 32 ? 4 : 8;
-  const size_t nb = (size_t)c->P.nvar * c->P.nvar * c->pitch;
+  const size_t nv = c->P.nvar, nb = nv * nv * c->pitch, slots = std::max((size_t)c->nlb * 64, (size_t)1);
   HIP_TRY(c, hipMalloc((void**)&k.bd, nb * 8));
-  HIP_TRY(c, hipMalloc(&k.lineG, nb * es));
-  HIP_TRY(c, hipMalloc(&k.lineP, nb * es));
+  HIP_TRY(c, hipMalloc(&k.lineG, nv * nv * slots * es));
+  HIP_TRY(c, hipMalloc(&k.lineP, nv * nv * slots * es));
+  HIP_TRY(c, hipMalloc((void**)&k.lineW, nv * slots * 8));
   return CFD_OK;
 }
 
@@ -465,6 +467,7 @@ int cfd_ctx_create(const cfd_mesh* m, const cfd_params* p, cfd_ctx** out) {
         while (fl[(size_t)4 * o + r] != f) ++r;
         (&nq[i].x)[q] = (o << 2) | r;
       }
+    if (m->ccell_ptr.size() > 256) return fail(c, CFD_ERR_MESH, "more than 255 cell colors");
     std::vector<unsigned char> col(P_, 0);
     for (int i = 0; i < N; ++i) col[i] = (unsigned char)m->cell_color[i];
     c->ccell_ptr = m->ccell_ptr;
@@ -528,6 +531,55 @@ int cfd_ctx_create(const cfd_mesh* m, const cfd_params* p, cfd_ctx** out) {
     HIP_TRY(c, hipMemcpy(c->d_seg_cells, m->seg_cells.data(), (size_t)N * 4, hipMemcpyHostToDevice));
     HIP_TRY(c, hipMemcpy(c->d_lpos, lpos.data(), (size_t)P_ * 4, hipMemcpyHostToDevice));
     HIP_TRY(c, hipMemcpy(c->d_lslot, ls.data(), (size_t)P_ * sizeof(char2), hipMemcpyHostToDevice));
+    // bundles: per color, the multi-cell segments longest first, each bundle
+    // filled while the next segment fits in its 64 slots
+    {
+      const int ncol = (int)m->segcol_ptr.size() - 1;
+      std::vector<int> lb_t, lb_kl, lcomp(P_, -1), segs;
+      c->lbcol_ptr.assign(1, 0);
+      for (int col = 0; col < ncol; ++col) {
+        segs.clear();
+        for (int k = m->segcol_ptr[col]; k < m->segcol_ptr[col + 1]; ++k)
+          if (m->seg_ptr[k + 1] - m->seg_ptr[k] > 1) segs.push_back(k);
+        std::stable_sort(segs.begin(), segs.end(), [&](int a, int b) {
+          return m->seg_ptr[a + 1] - m->seg_ptr[a] > m->seg_ptr[b + 1] - m->seg_ptr[b];
+        });
+        int fill = 64;
+        for (int k : segs) {
+          const int b = m->seg_ptr[k], L = m->seg_ptr[k + 1] - b;
+          if (fill + L > 64) {
+            lb_t.insert(lb_t.end(), 64, -1);
+            lb_kl.insert(lb_kl.end(), 64, 0);
+            fill = 0;
+          }
+          const size_t base = lb_t.size() - 64 + fill;
+          for (int i = 0; i < L; ++i) {
+            lb_t[base + i] = b + i;
+            lb_kl[base + i] = i | L << 8;
+            lcomp[b + i] = (int)(base + i);
+          }
+          fill += L;
+        }
+        c->lbcol_ptr.push_back((int)(lb_t.size() / 64));
+      }
+      c->nlb = (int)(lb_t.size() / 64);
+      if (ncol > 255) return fail(c, CFD_ERR_MESH, "more than 255 segment colors");
+      std::vector<unsigned char> lcol(P_, 0);
+      for (int col = 0; col < ncol; ++col)
+        for (int k = m->segcol_ptr[col]; k < m->segcol_ptr[col + 1]; ++k)
+          for (int t = m->seg_ptr[k]; t < m->seg_ptr[k + 1]; ++t) lcol[m->seg_cells[t]] = (unsigned char)col;
+      HIP_TRY(c, dalloc(&c->d_lcol, P_));
+      HIP_TRY(c, hipMemcpy(c->d_lcol, lcol.data(), P_, hipMemcpyHostToDevice));
+      const size_t ns = std::max(lb_t.size(), (size_t)1);
+      HIP_TRY(c, dalloc(&c->d_lb_t, ns));
+      HIP_TRY(c, dalloc(&c->d_lb_kl, ns));
+      HIP_TRY(c, dalloc(&c->d_lcomp, P_));
+      if (!lb_t.empty()) {
+        HIP_TRY(c, hipMemcpy(c->d_lb_t, lb_t.data(), lb_t.size() * 4, hipMemcpyHostToDevice));
+        HIP_TRY(c, hipMemcpy(c->d_lb_kl, lb_kl.data(), lb_kl.size() * 4, hipMemcpyHostToDevice));
+      }
+      HIP_TRY(c, hipMemcpy(c->d_lcomp, lcomp.data(), (size_t)P_ * 4, hipMemcpyHostToDevice));
+    }
     HIP_TRY(c, hipMemcpy(c->d_ccol, col.data(), P_, hipMemcpyHostToDevice));
     HIP_TRY(c, hipMemcpy(c->d_ccells, m->ccells.data(), (size_t)N * 4, hipMemcpyHostToDevice));
     HIP_TRY(c, hipMemcpy(c->d_nq, nq.data(), (size_t)P_ * sizeof(int4), hipMemcpyHostToDevice));
@@ -564,7 +616,7 @@ int cfd_ctx_destroy(cfd_ctx* c) {
   (void)hipStreamSynchronize(c->stream);
   free_krylov(c);
   void* bufs[] = {c->d_vol, c->d_xc,  c->d_yc,  c->d_slots, c->d_own, c->d_nei, c->d_tag,  c->d_nx,
-                  c->d_ny,  c->d_area, c->d_xm, c->d_ym,    c->d_iperm, c->d_facemap, c->d_alt, c->grad, c->d_wd, c->d_dnw, c->d_corder, c->d_ccol, c->d_ccells, c->d_nq, c->d_cpos, c->d_seg_ptr, c->d_seg_cells, c->d_lpos, c->d_lslot,
+                  c->d_ny,  c->d_area, c->d_xm, c->d_ym,    c->d_iperm, c->d_facemap, c->d_alt, c->grad, c->d_wd, c->d_dnw, c->d_corder, c->d_ccol, c->d_ccells, c->d_nq, c->d_cpos, c->d_seg_ptr, c->d_seg_cells, c->d_lpos, c->d_lslot, c->d_lb_t, c->d_lb_kl, c->d_lcomp, c->d_lcol,
                   c->phi,   c->dt,    c->Fm,    c->red,     c->scal, c->gradT, c->adj, c->tq, c->io6};
   for (void* b : bufs) (void)hipFree(b);
   for (int i = 0; i < 6; ++i) (void)hipFree(c->vec[i]);

@@ -37,8 +37,9 @@ struct Krylov {
   double* xt = nullptr;        // basis combination before the block SGS solve (FP64)
   double* fjac = nullptr;      // face Jacobians of the first-order flux (2 nv^2 x F), block preconditioners
   double* bd = nullptr;        // FP64 diagonal blocks in segment order (nv^2 x pitch), line preconditioner
-  void* lineG = nullptr;       // line factors G_t = Dp_t^-1 A_{t,next} (nv^2 x pitch, basis precision)
-  void* lineP = nullptr;       // line factors P_t = Dp_t^-1 A_{t,prev} (nv^2 x pitch, basis precision)
+  void* lineG = nullptr;       // line factors G_t = Dp_t^-1 A_{t,next} (nv^2 per bundle slot, basis precision)
+  void* lineP = nullptr;       // line factors P_t = Dp_t^-1 A_{t,prev} (nv^2 per bundle slot, basis precision)
+  double* lineW = nullptr;     // w of the cells of multi-cell segments (nv per bundle slot, FP64)
   double* z = nullptr;         // preconditioned direction (FP64)
   double* w = nullptr;         // accumulated solution of restarted GMRES (FP64)
   double* rr = nullptr;        // true residual of a restart cycle (FP64)
@@ -80,6 +81,13 @@ struct cfd_ctx {
   int* d_seg_cells = nullptr;       // cells in segment order
   int* d_lpos = nullptr;            // per cell: position in d_seg_cells (column of its line blocks)
   char2* d_lslot = nullptr;         // per cell: slots of the previous / next cell of its segment (-1: none)
+  // multi-cell segments packed into 64-slot bundles (one wavefront each), per color longest first
+  int nlb = 0;                      // number of bundles
+  std::vector<int> lbcol_ptr;       // per color: range of its bundles
+  int* d_lb_t = nullptr;            // per slot: line position of its cell (-1: padding)
+  int* d_lb_kl = nullptr;           // per slot: place in the segment | segment length << 8
+  int* d_lcomp = nullptr;           // per line position: bundle slot (-1: single-cell segment)
+  unsigned char* d_lcol = nullptr;  // per cell: color of its segment
   int4* d_nq = nullptr;             // per cell slot: (neighbor << 2) | back slot, or -1
   // solver state
   double* grad = nullptr;   // 14 x pitch (rho u v p T gradients; k, omega with SST)

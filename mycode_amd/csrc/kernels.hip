@@ -1112,10 +1112,12 @@ __global__ __launch_bounds__(BS) void k_sgs_first(int N, int p, const unsigned c
   }
 }
 
-// cells [b0, b0 + n) of the color-grouped list
+// cells [b0, b0 + n) of the color-grouped list; couplings to cells of a color
+// above `above` are skipped (their z is zero during the forward sweep)
 template <int NV, class TB, class TX, class TZ, bool TR>
 __global__ __launch_bounds__(BS) void k_sgs_color(int n, int b0, const int* __restrict__ cells, int p,
                                                    const int4* __restrict__ nq, const int* __restrict__ pos,
+                                                   const unsigned char* __restrict__ ccol, int above,
                                                    const TB* __restrict__ Binv, const TB* __restrict__ Aoff,
                                                    const TX* __restrict__ x, TZ* __restrict__ z) {
   const int t = xcd_block() * BS + threadIdx.x;
@@ -1132,6 +1134,7 @@ __global__ __launch_bounds__(BS) void k_sgs_color(int n, int b0, const int* __re
     const int e = es[q];
     if (e < 0) continue;
     const int o = e >> 2;
+    if (ccol[o] > above) continue;  // forward sweep: z of a later color is still zero
     double zo[NV];
 #pragma unroll
     for (int j = 0; j < NV; ++j) zo[j] = (double)z[j * p + o];
@@ -1167,122 +1170,187 @@ __global__ __launch_bounds__(BS) void k_sgs_color(int n, int b0, const int* __re
 // from the current z:
 //   k_line_rhs    (parallel over the color's cells)
 //                 w_t = Dp_t^-1 (x_t - sum_{off-segment q} A_{t,q} z_nbr),
-//   k_line_solve  (one thread per segment, only its own values in flight)
-//                 y_t = w_t - P_t y_{t-1},   z_t = y_t - G_t z_{t+1},
+//   k_line_solve  y_t = w_t - P_t y_{t-1},   z_t = y_t - G_t z_{t+1},
 // with Dp_t = B_t - A_{t,prev} G_{t-1}, P_t = Dp_t^-1 A_{t,prev} and
 // G_t = Dp_t^-1 A_{t,next} from k_line_factor.  Colors are swept ascending,
-// then descending.
-template <int NV, class TV>
-__global__ __launch_bounds__(64) void k_line_factor(int nseg, const int* __restrict__ seg_ptr,
-                                                    const char2* __restrict__ lslot, int p,
-                                                    const int* __restrict__ seg_cells,
-                                                    const double* __restrict__ Bd, const TV* __restrict__ Aoff,
-                                                    TV* __restrict__ Dpinv, TV* __restrict__ Pm,
-                                                    TV* __restrict__ G) {
-  const int sg = blockIdx.x * 64 + threadIdx.x;
-  if (sg >= nseg) return;
-  const int b = seg_ptr[sg], e = seg_ptr[sg + 1];
-  for (int t = b; t < e; ++t) {
-    const char2 qs = lslot[seg_cells[t]];
-    double M[NV][NV];
+// then descending.  A single-cell segment has Dp = B, whose inverse the block
+// kernel has already written at its position, and z = w.
+//
+// Multi-cell segments are packed into bundles of 64 slots, one wavefront
+// each: a color's segments, longest first, fill a bundle while they fit, one
+// slot per cell in chain order (lb_t[m]: line position of slot m, -1 for
+// padding; lb_kl[m] = k | L << 8, the cell's place k in its segment of length
+// L).  The recurrences run across the lanes: at step s the lanes holding the
+// s-th cell of their segment update, taking the previous cell's value from the
+// lane below (the next cell's from the lane above) by a DPP wave shift.  Every load is
+// issued before the first step, so a step costs one block product and a few
+// lane moves rather than a chain of dependent memory loads.  P, G and w are kept
+// by slot (nv^2 or nv consecutive values per slot).
+
+// Neighbouring lanes of a wavefront by DPP wave shifts (a VALU move, no LDS
+// round trip): lane_prev gives lane l the value of lane l-1, lane_next that of
+// lane l+1 (lane 0, resp. 63, receives 0).  Every lane of the wave must be
+// active.
+constexpr int DPP_WAVE_SHL1 = 0x130, DPP_WAVE_SHR1 = 0x138;
+template <int CTRL>
+__device__ __forceinline__ float dpp_move(float v) {
+  return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), CTRL, 0xF, 0xF, false));
+}
+template <int CTRL>
+__device__ __forceinline__ double dpp_move(double v) {
+  const long long b = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_update_dpp(0, (int)b, CTRL, 0xF, 0xF, false);
+  const int hi = __builtin_amdgcn_update_dpp(0, (int)(b >> 32), CTRL, 0xF, 0xF, false);
+  return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+template <class T>
+__device__ __forceinline__ T lane_prev(T v) { return dpp_move<DPP_WAVE_SHR1>(v); }
+template <class T>
+__device__ __forceinline__ T lane_next(T v) { return dpp_move<DPP_WAVE_SHL1>(v); }
+
+// Gauss-Jordan inverse with partial pivoting by selects (all indices
+// compile-time, the matrix stays in registers); M is destroyed.
+template <int NV>
+__device__ __forceinline__ void line_inverse(double (&M)[NV][NV], double (&Inv)[NV][NV]) {
 #pragma unroll
-    for (int i = 0; i < NV; ++i)
+  for (int i = 0; i < NV; ++i)
 #pragma unroll
-      for (int j = 0; j < NV; ++j) M[i][j] = Bd[(size_t)(i * NV + j) * p + t];
-    if (t > b) {  // M = B - A_prev G_{t-1}
-      const int qp = qs.x;
+    for (int j = 0; j < NV; ++j) Inv[i][j] = i == j ? 1.0 : 0.0;
 #pragma unroll
-      for (int i = 0; i < NV; ++i) {
-        double Li[NV];
+  for (int col = 0; col < NV; ++col) {
+    int piv = col;
+    double best = fabs(M[col][col]);
 #pragma unroll
-        for (int k = 0; k < NV; ++k) Li[k] = (double)Aoff[(size_t)((qp * NV + i) * NV + k) * p + t];
+    for (int r = col + 1; r < NV; ++r)
+      if (fabs(M[r][col]) > best) {
+        best = fabs(M[r][col]);
+        piv = r;
+      }
+#pragma unroll
+    for (int r = col + 1; r < NV; ++r) {
+      if (piv == r) {
 #pragma unroll
         for (int j = 0; j < NV; ++j) {
-          double a = 0.0;
-#pragma unroll
-          for (int k = 0; k < NV; ++k) a += Li[k] * (double)G[(size_t)(k * NV + j) * p + (t - 1)];
-          M[i][j] -= a;
+          double tt = M[col][j];
+          M[col][j] = M[r][j];
+          M[r][j] = tt;
+          tt = Inv[col][j];
+          Inv[col][j] = Inv[r][j];
+          Inv[r][j] = tt;
         }
       }
     }
-    double Inv[NV][NV];
+    const double ip = 1.0 / M[col][col];
 #pragma unroll
-    for (int i = 0; i < NV; ++i)
-#pragma unroll
-      for (int j = 0; j < NV; ++j) Inv[i][j] = i == j ? 1.0 : 0.0;
-#pragma unroll
-    for (int col = 0; col < NV; ++col) {  // Gauss-Jordan, partial pivoting by selects
-      int piv = col;
-      double best = fabs(M[col][col]);
-#pragma unroll
-      for (int r = col + 1; r < NV; ++r)
-        if (fabs(M[r][col]) > best) {
-          best = fabs(M[r][col]);
-          piv = r;
-        }
-#pragma unroll
-      for (int r = col + 1; r < NV; ++r) {
-        if (piv == r) {
-#pragma unroll
-          for (int j = 0; j < NV; ++j) {
-            double tt = M[col][j];
-            M[col][j] = M[r][j];
-            M[r][j] = tt;
-            tt = Inv[col][j];
-            Inv[col][j] = Inv[r][j];
-            Inv[r][j] = tt;
-          }
-        }
-      }
-      const double ip = 1.0 / M[col][col];
-#pragma unroll
-      for (int j = 0; j < NV; ++j) {
-        M[col][j] *= ip;
-        Inv[col][j] *= ip;
-      }
-#pragma unroll
-      for (int r = 0; r < NV; ++r) {
-        if (r == col) continue;
-        const double f = M[r][col];
-#pragma unroll
-        for (int j = 0; j < NV; ++j) {
-          M[r][j] -= f * M[col][j];
-          Inv[r][j] -= f * Inv[col][j];
-        }
-      }
+    for (int j = 0; j < NV; ++j) {
+      M[col][j] *= ip;
+      Inv[col][j] *= ip;
     }
 #pragma unroll
-    for (int i = 0; i < NV; ++i)
-#pragma unroll
-      for (int j = 0; j < NV; ++j) Dpinv[(size_t)(i * NV + j) * p + t] = (TV)Inv[i][j];
-    // P_t = Dp^-1 A_prev, G_t = Dp^-1 A_next
-#pragma unroll
-    for (int side = 0; side < 2; ++side) {
-      const int q = side == 0 ? qs.x : qs.y;
-      if (q < 0) continue;
-      TV* out = side == 0 ? Pm : G;
+    for (int r = 0; r < NV; ++r) {
+      if (r == col) continue;
+      const double f = M[r][col];
 #pragma unroll
       for (int j = 0; j < NV; ++j) {
-        double Ac[NV];
-#pragma unroll
-        for (int k = 0; k < NV; ++k) Ac[k] = (double)Aoff[(size_t)((q * NV + k) * NV + j) * p + t];
-#pragma unroll
-        for (int i = 0; i < NV; ++i) {
-          double a = 0.0;
-#pragma unroll
-          for (int k = 0; k < NV; ++k) a += Inv[i][k] * Ac[k];
-          out[(size_t)(i * NV + j) * p + t] = (TV)a;
-        }
+        M[r][j] -= f * M[col][j];
+        Inv[r][j] -= f * Inv[col][j];
       }
     }
   }
 }
 
+// One wavefront per bundle.  The inputs of every lane (B in FP64, A_prev and
+// A_next) are staged in LDS at the start, lane-major so that each lane reads
+// only its own column; G of the previous cell arrives by a DPP wave shift.
+template <int NV, class TV>
+__global__ __launch_bounds__(64) void k_line_factor(const int* __restrict__ lb_t, const int* __restrict__ lb_kl,
+                                                    const char2* __restrict__ lslot, int p,
+                                                    const int* __restrict__ seg_cells,
+                                                    const double* __restrict__ Bd, const TV* __restrict__ Aoff,
+                                                    TV* __restrict__ Dpinv, TV* __restrict__ Pm,
+                                                    TV* __restrict__ G) {
+  constexpr int NN = NV * NV;
+  __shared__ double sB[NN][64];
+  __shared__ TV sA[2][NN][64];
+  const int l = threadIdx.x;
+  const size_t bb = (size_t)blockIdx.x * 64, m = bb + l;
+  const int t = lb_t[m];
+  const int kl = lb_kl[m];
+  const int k = kl & 255, L = kl >> 8;
+  const int steps = __builtin_amdgcn_readfirstlane(lb_kl[bb] >> 8);  // slot 0 holds a longest segment
+  {  // loads from clamped, in-bounds positions (a padding slot reads position 0,
+     // a missing neighbour slot 0) so that none is guarded; unused values are
+     // never read back
+    const int tc = t >= 0 ? t : 0;
+    const char2 qs = lslot[seg_cells[tc]];
+    const int qp = qs.x >= 0 ? qs.x : 0, qn = qs.y >= 0 ? qs.y : 0;
+#pragma unroll
+    for (int e = 0; e < NN; ++e) {
+      sB[e][l] = Bd[(size_t)e * p + tc];
+      sA[0][e][l] = Aoff[(size_t)(qp * NN + e) * p + tc];
+      sA[1][e][l] = Aoff[(size_t)(qn * NN + e) * p + tc];
+    }
+  }
+  TV g[NN];
+#pragma unroll
+  for (int e = 0; e < NN; ++e) g[e] = TV(0);
+  for (int s = 0; s < steps; ++s) {
+    TV gp[NN];
+#pragma unroll
+    for (int e = 0; e < NN; ++e) gp[e] = lane_prev(g[e]);
+    if (t >= 0 && k == s) {
+      double M[NV][NV];
+#pragma unroll
+      for (int i = 0; i < NV; ++i)
+#pragma unroll
+        for (int j = 0; j < NV; ++j) M[i][j] = sB[i * NV + j][l];
+      if (k > 0) {  // Dp = B - A_prev G_{t-1}
+#pragma unroll
+        for (int i = 0; i < NV; ++i)
+#pragma unroll
+          for (int j = 0; j < NV; ++j) {
+            double a = 0.0;
+#pragma unroll
+            for (int q = 0; q < NV; ++q) a = fma((double)sA[0][i * NV + q][l], (double)gp[q * NV + j], a);
+            M[i][j] -= a;
+          }
+      }
+      double Inv[NV][NV];
+      line_inverse<NV>(M, Inv);
+#pragma unroll
+      for (int i = 0; i < NV; ++i)
+#pragma unroll
+        for (int j = 0; j < NV; ++j) Dpinv[(size_t)(i * NV + j) * p + t] = (TV)Inv[i][j];
+#pragma unroll
+      for (int side = 0; side < 2; ++side) {  // P = Dp^-1 A_prev, G = Dp^-1 A_next
+        if (side == 0 ? k == 0 : k == L - 1) continue;
+        TV* out = side == 0 ? Pm : G;
+#pragma unroll
+        for (int i = 0; i < NV; ++i)
+#pragma unroll
+          for (int j = 0; j < NV; ++j) {
+            double a = 0.0;
+#pragma unroll
+            for (int q = 0; q < NV; ++q) a = fma(Inv[i][q], (double)sA[side][q * NV + j][l], a);
+            out[m * NN + i * NV + j] = (TV)a;
+            if (side == 1) g[i * NV + j] = (TV)a;
+          }
+      }
+    }
+  }
+}
+
+// w for the color's cells: straight into z for a single-cell segment, into the
+// bundle slot lcomp[t] (FP64) for a cell of a longer one; couplings to cells of
+// a segment color above `above` are skipped (zero z during the forward sweep)
 template <int NV, class TB, class TX, class TZ>
 __global__ __launch_bounds__(BS) void k_line_rhs(int t0, int t1, const int* __restrict__ seg_cells,
                                                   const char2* __restrict__ lslot, const int4* __restrict__ nq,
-                                                  int p, const TB* __restrict__ Dpinv, const TB* __restrict__ Aoff,
-                                                  const TX* __restrict__ x, TZ* __restrict__ z) {
+                                                  const int* __restrict__ lcomp,
+                                                  const unsigned char* __restrict__ lcol, int above, int p,
+                                                  const TB* __restrict__ Dpinv, const TB* __restrict__ Aoff,
+                                                  const TX* __restrict__ x, TZ* __restrict__ z,
+                                                  double* __restrict__ W) {
   const int t = t0 + xcd_block() * BS + threadIdx.x;
   if (t >= t1) return;
   const int c = seg_cells[t];
@@ -1297,6 +1365,7 @@ __global__ __launch_bounds__(BS) void k_line_rhs(int t0, int t1, const int* __re
     const int en = es[q];
     if (en < 0 || q == qs.x || q == qs.y) continue;  // no neighbor, or a neighbor in this segment
     const int o = en >> 2;
+    if (lcol[o] > above) continue;  // forward sweep: z of a later color is still zero
     double zo[NV];
 #pragma unroll
     for (int j = 0; j < NV; ++j) zo[j] = (double)z[j * p + o];
@@ -1305,61 +1374,81 @@ __global__ __launch_bounds__(BS) void k_line_rhs(int t0, int t1, const int* __re
 #pragma unroll
       for (int j = 0; j < NV; ++j) s[i] -= (double)Aoff[(size_t)((q * NV + i) * NV + j) * p + t] * zo[j];
   }
+  const int m = lcomp[t];
 #pragma unroll
   for (int i = 0; i < NV; ++i) {
     double a = 0.0;
 #pragma unroll
     for (int j = 0; j < NV; ++j) a += (double)Dpinv[(size_t)(i * NV + j) * p + t] * s[j];
-    z[i * p + c] = (TZ)a;
+    if (m >= 0) W[(size_t)m * NV + i] = a;
+    else z[i * p + c] = (TZ)a;
   }
 }
 
+// One wavefront per bundle b0 + blockIdx.x: forward and backward recurrences
+// across the lanes, then every cell's z.
 template <int NV, class TB, class TZ>
-__global__ __launch_bounds__(64) void k_line_solve(int s0, int s1, const int* __restrict__ seg_ptr,
-                                                   const int* __restrict__ seg_cells, int p,
-                                                   const TB* __restrict__ Pm, const TB* __restrict__ G,
-                                                   TZ* __restrict__ z) {
-  const int sg = s0 + blockIdx.x * 64 + threadIdx.x;
-  if (sg >= s1) return;
-  const int b = seg_ptr[sg], e = seg_ptr[sg + 1];
-  if (e - b < 2) return;  // a single cell: w is already its solution
+__global__ __launch_bounds__(64) void k_line_solve(int b0, const int* __restrict__ lb_t,
+                                                   const int* __restrict__ lb_kl, const int* __restrict__ seg_cells,
+                                                   int p, const TB* __restrict__ Pm, const TB* __restrict__ G,
+                                                   const double* __restrict__ W, TZ* __restrict__ z) {
+  constexpr int NN = NV * NV;
+  const size_t bb = (size_t)(b0 + blockIdx.x) * 64, m = bb + threadIdx.x;
+  const int t = lb_t[m];
+  const int kl = lb_kl[m];
+  const int k = kl & 255, L = kl >> 8;
+  const int steps = __builtin_amdgcn_readfirstlane(lb_kl[bb] >> 8);
+  const bool hp = t >= 0 && k > 0, hg = t >= 0 && k < L - 1;
+  // every slot is allocated: the loads are unconditional (a guarded load
+  // becomes a branch that waits for it), and values a lane does not own are
+  // never used
   double y[NV];
-  {
-    const int c = seg_cells[b];
+  TB pr[NN], gr[NN];
 #pragma unroll
-    for (int i = 0; i < NV; ++i) y[i] = (double)z[i * p + c];
+  for (int i = 0; i < NV; ++i) y[i] = W[m * NV + i];
+#pragma unroll
+  for (int e = 0; e < NN; ++e) {
+    pr[e] = Pm[m * NN + e];
+    gr[e] = G[m * NN + e];
   }
-  for (int t = b + 1; t < e; ++t) {  // y_t = w_t - P_t y_{t-1}
-    const int c = seg_cells[t];
-    double yn[NV];
+  for (int s = 1; s < steps; ++s) {  // y_t = w_t - P_t y_{t-1}
+    double yp[NV];
 #pragma unroll
-    for (int i = 0; i < NV; ++i) {
-      double a = (double)z[i * p + c];
+    for (int i = 0; i < NV; ++i) yp[i] = lane_prev(y[i]);
+    if (hp && k == s) {
+      double yn[NV];
 #pragma unroll
-      for (int j = 0; j < NV; ++j) a -= (double)Pm[(size_t)(i * NV + j) * p + t] * y[j];
-      yn[i] = a;
-    }
+      for (int i = 0; i < NV; ++i) {
+        double a = y[i];
 #pragma unroll
-    for (int i = 0; i < NV; ++i) {
-      y[i] = yn[i];
-      z[i * p + c] = (TZ)yn[i];
+        for (int j = 0; j < NV; ++j) a = fma(-(double)pr[i * NV + j], yp[j], a);
+        yn[i] = a;
+      }
+#pragma unroll
+      for (int i = 0; i < NV; ++i) y[i] = yn[i];
     }
   }
-  for (int t = e - 2; t >= b; --t) {  // z_t = y_t - G_t z_{t+1}; y holds z_{t+1}
-    const int c = seg_cells[t];
+  for (int s = steps - 2; s >= 0; --s) {  // z_t = y_t - G_t z_{t+1}
     double zn[NV];
 #pragma unroll
-    for (int i = 0; i < NV; ++i) {
-      double a = (double)z[i * p + c];
+    for (int i = 0; i < NV; ++i) zn[i] = lane_next(y[i]);
+    if (hg && k == s) {
+      double yn[NV];
 #pragma unroll
-      for (int j = 0; j < NV; ++j) a -= (double)G[(size_t)(i * NV + j) * p + t] * y[j];
-      zn[i] = a;
-    }
+      for (int i = 0; i < NV; ++i) {
+        double a = y[i];
 #pragma unroll
-    for (int i = 0; i < NV; ++i) {
-      y[i] = zn[i];
-      z[i * p + c] = (TZ)zn[i];
+        for (int j = 0; j < NV; ++j) a = fma(-(double)gr[i * NV + j], zn[j], a);
+        yn[i] = a;
+      }
+#pragma unroll
+      for (int i = 0; i < NV; ++i) y[i] = yn[i];
     }
+  }
+  if (t >= 0) {
+    const int c = seg_cells[t];
+#pragma unroll
+    for (int i = 0; i < NV; ++i) z[i * p + c] = (TZ)y[i];
   }
 }
 
@@ -2161,9 +2250,9 @@ void launch_block_precond(cfd_ctx* c, const double* U, void* Binv, int prec, dou
 }
 template <class TV>
 static void line_factor_t(cfd_ctx* c, const double* Bd, const TV* Aoff, TV* Dpinv, TV* Pm, TV* G) {
-  const int ns = (int)c->seg_ptr.size() - 1;
-  NV_DISPATCH(c->P.nvar, hipLaunchKernelGGL((k_line_factor<NVt, TV>), dim3((ns + 63) / 64), dim3(64), 0, c->stream,
-                                            ns, c->d_seg_ptr, c->d_lslot, c->pitch, c->d_seg_cells, Bd, Aoff,
+  if (c->nlb == 0) return;  // single-cell segments only: Dp^-1 = B^-1 from the block kernel
+  NV_DISPATCH(c->P.nvar, hipLaunchKernelGGL((k_line_factor<NVt, TV>), dim3(c->nlb), dim3(64), 0, c->stream,
+                                            c->d_lb_t, c->d_lb_kl, c->d_lslot, c->pitch, c->d_seg_cells, Bd, Aoff,
                                             Dpinv, Pm, G));
 }
 void launch_line_factor(cfd_ctx* c, const double* Bd, const void* Aoff, void* Dpinv, void* Pm, void* G, int prec) {
@@ -2173,6 +2262,7 @@ void launch_line_factor(cfd_ctx* c, const double* Bd, const void* Aoff, void* Dp
 template <class TB, class TX, class TZ>
 static void line_t(cfd_ctx* c, const TB* Dpinv, const TB* Pm, const TB* G, const TB* Aoff, const TX* x, TZ* z) {
   const int nc = (int)c->segcol_ptr.size() - 1;
+  double* W = c->kry.lineW;
   NV_DISPATCH(c->P.nvar, {
     hipLaunchKernelGGL(k_fill<TZ>, dim3(nblocks((size_t)NVt * c->pitch)), dim3(BS), 0, c->stream, z, TZ(0),
                        (size_t)NVt * c->pitch);
@@ -2182,9 +2272,12 @@ static void line_t(cfd_ctx* c, const TB* Dpinv, const TB* Pm, const TB* G, const
       if (s1 <= s0) continue;
       const int t0 = c->seg_ptr[s0], t1 = c->seg_ptr[s1];
       hipLaunchKernelGGL((k_line_rhs<NVt, TB, TX, TZ>), dim3(nblocks(t1 - t0)), dim3(BS), 0, c->stream, t0, t1,
-                         c->d_seg_cells, c->d_lslot, c->d_nq, c->pitch, Dpinv, Aoff, x, z);
-      hipLaunchKernelGGL((k_line_solve<NVt, TB, TZ>), dim3((s1 - s0 + 63) / 64), dim3(64), 0, c->stream, s0, s1,
-                         c->d_seg_ptr, c->d_seg_cells, c->pitch, Pm, G, z);
+                         c->d_seg_cells, c->d_lslot, c->d_nq, c->d_lcomp, c->d_lcol, s < nc ? col : 255,
+                         c->pitch, Dpinv, Aoff, x, z, W);
+      const int b0 = c->lbcol_ptr[col], b1 = c->lbcol_ptr[col + 1];
+      if (b1 > b0)
+        hipLaunchKernelGGL((k_line_solve<NVt, TB, TZ>), dim3(b1 - b0), dim3(64), 0, c->stream, b0, c->d_lb_t,
+                           c->d_lb_kl, c->d_seg_cells, c->pitch, Pm, G, (const double*)W, z);
     }
   });
 }
@@ -2213,7 +2306,8 @@ static void sgs_t(cfd_ctx* c, const TB* Binv, const TB* Aoff, const TX* x, TZ* z
       const int b = c->ccell_ptr[col], n = c->ccell_ptr[col + 1] - b;
       if (n == 0) continue;
       hipLaunchKernelGGL((k_sgs_color<NVt, TB, TX, TZ, TR>), dim3(nblocks(n)), dim3(BS), 0, c->stream, n, b,
-                         c->d_ccells, c->pitch, c->d_nq, c->d_cpos, Binv, Aoff, x, z);
+                         c->d_ccells, c->pitch, c->d_nq, c->d_cpos, c->d_ccol, s < nc ? col : 255, Binv, Aoff,
+                         x, z);
     }
   });
 }
