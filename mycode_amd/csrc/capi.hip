@@ -6,6 +6,7 @@
 #include <chrono>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <string>
 #include <vector>
@@ -100,6 +101,7 @@ void free_krylov(cfd_ctx* c) {
   (void)hipFree(k.bd);
   (void)hipFree(k.lineG);
   (void)hipFree(k.lineP);
+  (void)hipFree(k.lineS);
   (void)hipFree(k.lineW);
   (void)hipFree(k.xt);
   (void)hipFree(k.z);
@@ -153,6 +155,7 @@ int ensure_line(cfd_ctx* c) {
   HIP_TRY(c, hipMalloc((void**)&k.bd, nb * 8));
   HIP_TRY(c, hipMalloc(&k.lineG, nv * nv * slots * es));
   HIP_TRY(c, hipMalloc(&k.lineP, nv * nv * slots * es));
+  HIP_TRY(c, hipMalloc(&k.lineS, nv * c->pitch * 8));
   HIP_TRY(c, hipMalloc((void**)&k.lineW, nv * slots * 8));
   return CFD_OK;
 }
@@ -537,6 +540,7 @@ int cfd_ctx_create(const cfd_mesh* m, const cfd_params* p, cfd_ctx** out) {
       const int ncol = (int)m->segcol_ptr.size() - 1;
       std::vector<int> lb_t, lb_kl, lcomp(P_, -1), segs;
       c->lbcol_ptr.assign(1, 0);
+      c->lmcol_cells.assign(ncol, 0);
       for (int col = 0; col < ncol; ++col) {
         segs.clear();
         for (int k = m->segcol_ptr[col]; k < m->segcol_ptr[col + 1]; ++k)
@@ -552,6 +556,9 @@ int cfd_ctx_create(const cfd_mesh* m, const cfd_params* p, cfd_ctx** out) {
             lb_kl.insert(lb_kl.end(), 64, 0);
             fill = 0;
           }
+          if (b != m->seg_ptr[m->segcol_ptr[col]] + c->lmcol_cells[col])
+            return fail(c, CFD_ERR_MESH, "a color's multi-cell segments do not lead its line positions");
+          c->lmcol_cells[col] += L;
           const size_t base = lb_t.size() - 64 + fill;
           for (int i = 0; i < L; ++i) {
             lb_t[base + i] = b + i;
@@ -563,13 +570,25 @@ int cfd_ctx_create(const cfd_mesh* m, const cfd_params* p, cfd_ctx** out) {
         c->lbcol_ptr.push_back((int)(lb_t.size() / 64));
       }
       c->nlb = (int)(lb_t.size() / 64);
+      if (const char* e = std::getenv("CFD_LINE_FUSED_MAX_CELLS")) c->line_fused_max = std::atoi(e);
       if (ncol > 255) return fail(c, CFD_ERR_MESH, "more than 255 segment colors");
-      std::vector<unsigned char> lcol(P_, 0);
+      // per cell: off-segment neighbour slots of a lower (bits 0-3) and of a
+      // higher (bits 4-7) segment color
+      std::vector<int> lcol(N, 0);
       for (int col = 0; col < ncol; ++col)
         for (int k = m->segcol_ptr[col]; k < m->segcol_ptr[col + 1]; ++k)
-          for (int t = m->seg_ptr[k]; t < m->seg_ptr[k + 1]; ++t) lcol[m->seg_cells[t]] = (unsigned char)col;
-      HIP_TRY(c, dalloc(&c->d_lcol, P_));
-      HIP_TRY(c, hipMemcpy(c->d_lcol, lcol.data(), P_, hipMemcpyHostToDevice));
+          for (int t = m->seg_ptr[k]; t < m->seg_ptr[k + 1]; ++t) lcol[m->seg_cells[t]] = col;
+      std::vector<unsigned char> lmask(P_, 0);
+      for (int i = 0; i < N; ++i)
+        for (int q = 0; q < 4; ++q) {
+          const int en = (&nq[i].x)[q];
+          if (en < 0 || q == m->lslot_prev[i] || q == m->lslot_next[i]) continue;
+          const int o = en >> 2;
+          if (lcol[o] == lcol[i]) return fail(c, CFD_ERR_MESH, "neighbouring segments share a color");
+          lmask[i] |= (unsigned char)(lcol[o] < lcol[i] ? 1u << q : 16u << q);
+        }
+      HIP_TRY(c, dalloc(&c->d_lmask, P_));
+      HIP_TRY(c, hipMemcpy(c->d_lmask, lmask.data(), P_, hipMemcpyHostToDevice));
       const size_t ns = std::max(lb_t.size(), (size_t)1);
       HIP_TRY(c, dalloc(&c->d_lb_t, ns));
       HIP_TRY(c, dalloc(&c->d_lb_kl, ns));
@@ -616,7 +635,7 @@ int cfd_ctx_destroy(cfd_ctx* c) {
   (void)hipStreamSynchronize(c->stream);
   free_krylov(c);
   void* bufs[] = {c->d_vol, c->d_xc,  c->d_yc,  c->d_slots, c->d_own, c->d_nei, c->d_tag,  c->d_nx,
-                  c->d_ny,  c->d_area, c->d_xm, c->d_ym,    c->d_iperm, c->d_facemap, c->d_alt, c->grad, c->d_wd, c->d_dnw, c->d_corder, c->d_ccol, c->d_ccells, c->d_nq, c->d_cpos, c->d_seg_ptr, c->d_seg_cells, c->d_lpos, c->d_lslot, c->d_lb_t, c->d_lb_kl, c->d_lcomp, c->d_lcol,
+                  c->d_ny,  c->d_area, c->d_xm, c->d_ym,    c->d_iperm, c->d_facemap, c->d_alt, c->grad, c->d_wd, c->d_dnw, c->d_corder, c->d_ccol, c->d_ccells, c->d_nq, c->d_cpos, c->d_seg_ptr, c->d_seg_cells, c->d_lpos, c->d_lslot, c->d_lb_t, c->d_lb_kl, c->d_lcomp, c->d_lmask,
                   c->phi,   c->dt,    c->Fm,    c->red,     c->scal, c->gradT, c->adj, c->tq, c->io6};
   for (void* b : bufs) (void)hipFree(b);
   for (int i = 0; i < 6; ++i) (void)hipFree(c->vec[i]);

@@ -39,6 +39,7 @@ struct Krylov {
   double* bd = nullptr;        // FP64 diagonal blocks in segment order (nv^2 x pitch), line preconditioner
   void* lineG = nullptr;       // line factors G_t = Dp_t^-1 A_{t,next} (nv^2 per bundle slot, basis precision)
   void* lineP = nullptr;       // line factors P_t = Dp_t^-1 A_{t,prev} (nv^2 per bundle slot, basis precision)
+  void* lineS = nullptr;       // forward-pass partial sums of the line sweep (nv x pitch, up to FP64)
   double* lineW = nullptr;     // w of the cells of multi-cell segments (nv per bundle slot, FP64)
   double* z = nullptr;         // preconditioned direction (FP64)
   double* w = nullptr;         // accumulated solution of restarted GMRES (FP64)
@@ -84,10 +85,12 @@ struct cfd_ctx {
   // multi-cell segments packed into 64-slot bundles (one wavefront each), per color longest first
   int nlb = 0;                      // number of bundles
   std::vector<int> lbcol_ptr;       // per color: range of its bundles
+  std::vector<int> lmcol_cells;     // per color: cells in its multi-cell segments (they lead its positions)
   int* d_lb_t = nullptr;            // per slot: line position of its cell (-1: padding)
   int* d_lb_kl = nullptr;           // per slot: place in the segment | segment length << 8
   int* d_lcomp = nullptr;           // per line position: bundle slot (-1: single-cell segment)
-  unsigned char* d_lcol = nullptr;  // per cell: color of its segment
+  unsigned char* d_lmask = nullptr; // per cell: off-segment neighbour slots of lower (bits 0-3) / higher (4-7) color
+  int line_fused_max = 262144;      // below this many cells a line sweep color is one launch
   int4* d_nq = nullptr;             // per cell slot: (neighbor << 2) | back slot, or -1
   // solver state
   double* grad = nullptr;   // 14 x pitch (rho u v p T gradients; k, omega with SST)

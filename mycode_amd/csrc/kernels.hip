@@ -1168,9 +1168,8 @@ __global__ __launch_bounds__(BS) void k_sgs_color(int n, int b0, const int* __re
 // contiguous range).  Per color, a sweep solves every segment's
 // block-tridiagonal system exactly with the couplings to other segments taken
 // from the current z:
-//   k_line_rhs    (parallel over the color's cells)
-//                 w_t = Dp_t^-1 (x_t - sum_{off-segment q} A_{t,q} z_nbr),
-//   k_line_solve  y_t = w_t - P_t y_{t-1},   z_t = y_t - G_t z_{t+1},
+//   w_t = Dp_t^-1 (x_t - sum_{off-segment q} A_{t,q} z_nbr),
+//   y_t = w_t - P_t y_{t-1},   z_t = y_t - G_t z_{t+1}   (k_line_sweep, one launch per color)
 // with Dp_t = B_t - A_{t,prev} G_{t-1}, P_t = Dp_t^-1 A_{t,prev} and
 // G_t = Dp_t^-1 A_{t,next} from k_line_factor.  Colors are swept ascending,
 // then descending.  A single-cell segment has Dp = B, whose inverse the block
@@ -1182,10 +1181,10 @@ __global__ __launch_bounds__(BS) void k_sgs_color(int n, int b0, const int* __re
 // padding; lb_kl[m] = k | L << 8, the cell's place k in its segment of length
 // L).  The recurrences run across the lanes: at step s the lanes holding the
 // s-th cell of their segment update, taking the previous cell's value from the
-// lane below (the next cell's from the lane above) by a DPP wave shift.  Every load is
-// issued before the first step, so a step costs one block product and a few
-// lane moves rather than a chain of dependent memory loads.  P, G and w are kept
-// by slot (nv^2 or nv consecutive values per slot).
+// lane below (the next cell's from the lane above) by a DPP wave shift.  Every
+// load is issued before the first step, so a step costs one block product and
+// a few lane moves rather than a chain of dependent memory loads.  P and G are kept
+// by slot (nv^2 consecutive values per slot).
 
 // Neighbouring lanes of a wavefront by DPP wave shifts (a VALU move, no LDS
 // round trip): lane_prev gives lane l the value of lane l-1, lane_next that of
@@ -1340,32 +1339,83 @@ __global__ __launch_bounds__(64) void k_line_factor(const int* __restrict__ lb_t
   }
 }
 
-// w for the color's cells: straight into z for a single-cell segment, into the
-// bundle slot lcomp[t] (FP64) for a cell of a longer one; couplings to cells of
-// a segment color above `above` are skipped (zero z during the forward sweep)
-template <int NV, class TB, class TX, class TZ>
-__global__ __launch_bounds__(BS) void k_line_rhs(int t0, int t1, const int* __restrict__ seg_cells,
-                                                  const char2* __restrict__ lslot, const int4* __restrict__ nq,
-                                                  const int* __restrict__ lcomp,
-                                                  const unsigned char* __restrict__ lcol, int above, int p,
-                                                  const TB* __restrict__ Dpinv, const TB* __restrict__ Aoff,
-                                                  const TX* __restrict__ x, TZ* __restrict__ z,
-                                                  double* __restrict__ W) {
-  const int t = t0 + xcd_block() * BS + threadIdx.x;
-  if (t >= t1) return;
-  const int c = seg_cells[t];
-  const char2 qs = lslot[c];
-  double s[NV];
+// The two recurrences of a bundle, at step s by the lanes holding the s-th
+// cell of their segment: y_t = w_t - P_t y_{t-1} ascending, then
+// z_t = y_t - G_t z_{t+1} descending.  y holds w on entry and z on exit.
+template <int NV, class TB, class TC>
+__device__ __forceinline__ void line_recur(int steps, int k, bool hp, bool hg, const TB (&pr)[NV * NV],
+                                           const TB (&gr)[NV * NV], TC (&y)[NV]) {
+  for (int s = 1; s < steps; ++s) {  // y_t = w_t - P_t y_{t-1}
+    TC yp[NV];
 #pragma unroll
-  for (int i = 0; i < NV; ++i) s[i] = (double)x[i * p + c];
+    for (int i = 0; i < NV; ++i) yp[i] = lane_prev(y[i]);
+    if (hp && k == s) {
+      TC yn[NV];
+#pragma unroll
+      for (int i = 0; i < NV; ++i) {
+        TC a = y[i];
+#pragma unroll
+        for (int j = 0; j < NV; ++j) a = fma(-(TC)pr[i * NV + j], yp[j], a);
+        yn[i] = a;
+      }
+#pragma unroll
+      for (int i = 0; i < NV; ++i) y[i] = yn[i];
+    }
+  }
+  for (int s = steps - 2; s >= 0; --s) {  // z_t = y_t - G_t z_{t+1}
+    TC zn[NV];
+#pragma unroll
+    for (int i = 0; i < NV; ++i) zn[i] = lane_next(y[i]);
+    if (hg && k == s) {
+      TC yn[NV];
+#pragma unroll
+      for (int i = 0; i < NV; ++i) {
+        TC a = y[i];
+#pragma unroll
+        for (int j = 0; j < NV; ++j) a = fma(-(TC)gr[i * NV + j], zn[j], a);
+        yn[i] = a;
+      }
+#pragma unroll
+      for (int i = 0; i < NV; ++i) y[i] = yn[i];
+    }
+  }
+}
+
+// Sweep passes.  A color is visited once ascending (forward) and, below the
+// top color, once descending (backward).  During the forward pass only
+// neighbours of lower colors carry a nonzero z, and during the backward pass
+// the lower ones still hold exactly the values the forward pass used, so
+//   forward:  s = x - sum_{lower q} A_{t,q} z_nbr   (kept in sL below the top color)
+//   backward: s = sL - sum_{higher q} A_{t,q} z_nbr
+// reads every coupling block once per application instead of one and a half
+// times.  lmask[c] marks the off-segment neighbour slots of lower (bits 0-3)
+// and higher (bits 4-7) segment color.
+enum LinePass { LINE_FWD = 0, LINE_TOP = 1, LINE_BWD = 2 };
+
+// w_t = Dp_t^-1 s for the cell c at line position t (pass as above); keep
+// false suppresses the store of s (a padding lane standing in for a cell)
+template <int NV, class TB, class TX, class TZ>
+__device__ __forceinline__ void line_w(int t, int c, int pass, bool keep, const int4* __restrict__ nq,
+                                       const unsigned char* __restrict__ lmask, int p,
+                                       const TB* __restrict__ Dpinv, const TB* __restrict__ Aoff,
+                                       const TX* __restrict__ x, TZ* __restrict__ sL, const TZ* __restrict__ z,
+                                       double (&w)[NV]) {
+  const unsigned mk = lmask[c];
+  const unsigned use = pass == LINE_BWD ? mk >> 4 : mk & 15u;
+  double s[NV];
+  if (pass == LINE_BWD) {
+#pragma unroll
+    for (int i = 0; i < NV; ++i) s[i] = (double)sL[i * p + c];
+  } else {
+#pragma unroll
+    for (int i = 0; i < NV; ++i) s[i] = (double)x[i * p + c];
+  }
   const int4 e4 = nq[c];
   const int es[4] = {e4.x, e4.y, e4.z, e4.w};
 #pragma unroll
   for (int q = 0; q < 4; ++q) {
-    const int en = es[q];
-    if (en < 0 || q == qs.x || q == qs.y) continue;  // no neighbor, or a neighbor in this segment
-    const int o = en >> 2;
-    if (lcol[o] > above) continue;  // forward sweep: z of a later color is still zero
+    if (!((use >> q) & 1u)) continue;
+    const int o = es[q] >> 2;
     double zo[NV];
 #pragma unroll
     for (int j = 0; j < NV; ++j) zo[j] = (double)z[j * p + o];
@@ -1374,79 +1424,134 @@ __global__ __launch_bounds__(BS) void k_line_rhs(int t0, int t1, const int* __re
 #pragma unroll
       for (int j = 0; j < NV; ++j) s[i] -= (double)Aoff[(size_t)((q * NV + i) * NV + j) * p + t] * zo[j];
   }
-  const int m = lcomp[t];
+  if (pass == LINE_FWD && keep) {
+#pragma unroll
+    for (int i = 0; i < NV; ++i) sL[i * p + c] = (TZ)s[i];
+  }
 #pragma unroll
   for (int i = 0; i < NV; ++i) {
     double a = 0.0;
 #pragma unroll
     for (int j = 0; j < NV; ++j) a += (double)Dpinv[(size_t)(i * NV + j) * p + t] * s[j];
-    if (m >= 0) W[(size_t)m * NV + i] = a;
-    else z[i * p + c] = (TZ)a;
+    w[i] = a;
   }
 }
 
-// One wavefront per bundle b0 + blockIdx.x: forward and backward recurrences
-// across the lanes, then every cell's z.
+// All cells of one color (positions [t0, t1)) for large meshes: z = w for a
+// single-cell segment, w into its bundle slot (FP64) for a cell of a longer
+// one, whose recurrence k_line_solve then runs.  This bandwidth-bound part runs
+// at the occupancy its own registers allow.
+template <int NV, class TB, class TX, class TZ>
+__global__ __launch_bounds__(BS) void k_line_cells(int t0, int t1, int pass, const int* __restrict__ seg_cells,
+                                                   const int4* __restrict__ nq, const int* __restrict__ lcomp,
+                                                   const unsigned char* __restrict__ lmask, int p,
+                                                   const TB* __restrict__ Dpinv, const TB* __restrict__ Aoff,
+                                                   const TX* __restrict__ x, TZ* __restrict__ sL, TZ* __restrict__ z,
+                                                   double* __restrict__ W) {
+  const int t = t0 + xcd_block() * BS + threadIdx.x;
+  if (t >= t1) return;
+  const int c = seg_cells[t];
+  const int m = lcomp[t];
+  double w[NV];
+  line_w<NV, TB, TX, TZ>(t, c, pass, true, nq, lmask, p, Dpinv, Aoff, x, sL, z, w);
+  if (m >= 0) {
+#pragma unroll
+    for (int i = 0; i < NV; ++i) W[(size_t)m * NV + i] = w[i];
+  } else {
+#pragma unroll
+    for (int i = 0; i < NV; ++i) z[i * p + c] = (TZ)w[i];
+  }
+}
+
+// One wavefront per bundle b0 + blockIdx.x after k_line_cells: forward and
+// backward recurrences across the lanes (in the precision of z), then every
+// cell's z.  Every slot is allocated, so the loads are unconditional (a guarded
+// load becomes a branch that waits for it); values a lane does not own are
+// never used.
 template <int NV, class TB, class TZ>
 __global__ __launch_bounds__(64) void k_line_solve(int b0, const int* __restrict__ lb_t,
                                                    const int* __restrict__ lb_kl, const int* __restrict__ seg_cells,
                                                    int p, const TB* __restrict__ Pm, const TB* __restrict__ G,
                                                    const double* __restrict__ W, TZ* __restrict__ z) {
   constexpr int NN = NV * NV;
+  typedef TZ TC;
   const size_t bb = (size_t)(b0 + blockIdx.x) * 64, m = bb + threadIdx.x;
   const int t = lb_t[m];
   const int kl = lb_kl[m];
   const int k = kl & 255, L = kl >> 8;
-  const int steps = __builtin_amdgcn_readfirstlane(lb_kl[bb] >> 8);
+  const int steps = __builtin_amdgcn_readfirstlane(lb_kl[bb] >> 8);  // slot 0 holds a longest segment
   const bool hp = t >= 0 && k > 0, hg = t >= 0 && k < L - 1;
-  // every slot is allocated: the loads are unconditional (a guarded load
-  // becomes a branch that waits for it), and values a lane does not own are
-  // never used
-  double y[NV];
+  TC y[NV];
   TB pr[NN], gr[NN];
 #pragma unroll
-  for (int i = 0; i < NV; ++i) y[i] = W[m * NV + i];
+  for (int i = 0; i < NV; ++i) y[i] = (TC)W[m * NV + i];
 #pragma unroll
   for (int e = 0; e < NN; ++e) {
     pr[e] = Pm[m * NN + e];
     gr[e] = G[m * NN + e];
   }
-  for (int s = 1; s < steps; ++s) {  // y_t = w_t - P_t y_{t-1}
-    double yp[NV];
-#pragma unroll
-    for (int i = 0; i < NV; ++i) yp[i] = lane_prev(y[i]);
-    if (hp && k == s) {
-      double yn[NV];
-#pragma unroll
-      for (int i = 0; i < NV; ++i) {
-        double a = y[i];
-#pragma unroll
-        for (int j = 0; j < NV; ++j) a = fma(-(double)pr[i * NV + j], yp[j], a);
-        yn[i] = a;
-      }
-#pragma unroll
-      for (int i = 0; i < NV; ++i) y[i] = yn[i];
-    }
-  }
-  for (int s = steps - 2; s >= 0; --s) {  // z_t = y_t - G_t z_{t+1}
-    double zn[NV];
-#pragma unroll
-    for (int i = 0; i < NV; ++i) zn[i] = lane_next(y[i]);
-    if (hg && k == s) {
-      double yn[NV];
-#pragma unroll
-      for (int i = 0; i < NV; ++i) {
-        double a = y[i];
-#pragma unroll
-        for (int j = 0; j < NV; ++j) a = fma(-(double)gr[i * NV + j], zn[j], a);
-        yn[i] = a;
-      }
-#pragma unroll
-      for (int i = 0; i < NV; ++i) y[i] = yn[i];
-    }
-  }
+  line_recur<NV, TB, TC>(steps, k, hp, hg, pr, gr, y);
   if (t >= 0) {
     const int c = seg_cells[t];
+#pragma unroll
+    for (int i = 0; i < NV; ++i) z[i * p + c] = (TZ)y[i];
+  }
+}
+
+// One color of a sweep in one launch (64-thread blocks).  Blocks [0, nb) are
+// the color's bundles b0 .. b0 + nb - 1, first so that their recurrences start
+// early: each lane forms w of its cell, keeps it in registers, and the
+// recurrences run across the lanes.  Blocks from nb on take the line positions
+// [t0, t1) of the color's single-cell segments (which follow its longer ones)
+// 64 at a time and finish them (z = w).  Segments of
+// one color share no face, so no block reads a z that another block of the
+// launch writes.  The recurrences run in the precision of z (TZ).
+template <int NV, class TB, class TX, class TZ>
+__global__ __launch_bounds__(64) void k_line_sweep(int b0, int nb, int t0, int t1, int pass,
+                                                   const int* __restrict__ lb_t, const int* __restrict__ lb_kl,
+                                                   const int* __restrict__ seg_cells, const int4* __restrict__ nq,
+                                                   const unsigned char* __restrict__ lmask, int p,
+                                                   const TB* __restrict__ Dpinv, const TB* __restrict__ Aoff,
+                                                   const TB* __restrict__ Pm, const TB* __restrict__ G,
+                                                   const TX* __restrict__ x, TZ* __restrict__ sL,
+                                                   TZ* __restrict__ z) {
+  constexpr int NN = NV * NV;
+  typedef TZ TC;  // recurrence arithmetic
+  const int blk = blockIdx.x;
+  if (blk >= nb) {  // single-cell segments (uniform per block)
+    const int t = t0 + (blk - nb) * 64 + threadIdx.x;
+    if (t >= t1) return;
+    const int c = seg_cells[t];
+    double w[NV];
+    line_w<NV, TB, TX, TZ>(t, c, pass, true, nq, lmask, p, Dpinv, Aoff, x, sL, z, w);
+#pragma unroll
+    for (int i = 0; i < NV; ++i) z[i * p + c] = (TZ)w[i];
+    return;
+  }
+  const size_t bb = (size_t)(b0 + blk) * 64, m = bb + threadIdx.x;
+  const int t = lb_t[m];
+  const int kl = lb_kl[m];
+  const int k = kl & 255, L = kl >> 8;
+  const int steps = __builtin_amdgcn_readfirstlane(lb_kl[bb] >> 8);  // slot 0 holds a longest segment
+  const bool hp = t >= 0 && k > 0, hg = t >= 0 && k < L - 1;
+  // every lane stays active for the lane moves: a padding slot works on slot
+  // 0's cell and writes nothing; every slot of P and G is allocated, so their
+  // loads are unconditional (a guarded load becomes a branch that waits for it)
+  const int tc = t >= 0 ? t : lb_t[bb];
+  const int c = seg_cells[tc];
+  TB pr[NN], gr[NN];
+#pragma unroll
+  for (int e = 0; e < NN; ++e) {
+    pr[e] = Pm[m * NN + e];
+    gr[e] = G[m * NN + e];
+  }
+  double w[NV];
+  line_w<NV, TB, TX, TZ>(tc, c, pass, t >= 0, nq, lmask, p, Dpinv, Aoff, x, sL, z, w);
+  TC y[NV];
+#pragma unroll
+  for (int i = 0; i < NV; ++i) y[i] = (TC)w[i];
+  line_recur<NV, TB, TC>(steps, k, hp, hg, pr, gr, y);
+  if (t >= 0) {
 #pragma unroll
     for (int i = 0; i < NV; ++i) z[i * p + c] = (TZ)y[i];
   }
@@ -2262,7 +2367,7 @@ void launch_line_factor(cfd_ctx* c, const double* Bd, const void* Aoff, void* Dp
 template <class TB, class TX, class TZ>
 static void line_t(cfd_ctx* c, const TB* Dpinv, const TB* Pm, const TB* G, const TB* Aoff, const TX* x, TZ* z) {
   const int nc = (int)c->segcol_ptr.size() - 1;
-  double* W = c->kry.lineW;
+  TZ* sL = (TZ*)c->kry.lineS;
   NV_DISPATCH(c->P.nvar, {
     hipLaunchKernelGGL(k_fill<TZ>, dim3(nblocks((size_t)NVt * c->pitch)), dim3(BS), 0, c->stream, z, TZ(0),
                        (size_t)NVt * c->pitch);
@@ -2271,13 +2376,26 @@ static void line_t(cfd_ctx* c, const TB* Dpinv, const TB* Pm, const TB* G, const
       const int s0 = c->segcol_ptr[col], s1 = c->segcol_ptr[col + 1];
       if (s1 <= s0) continue;
       const int t0 = c->seg_ptr[s0], t1 = c->seg_ptr[s1];
-      hipLaunchKernelGGL((k_line_rhs<NVt, TB, TX, TZ>), dim3(nblocks(t1 - t0)), dim3(BS), 0, c->stream, t0, t1,
-                         c->d_seg_cells, c->d_lslot, c->d_nq, c->d_lcomp, c->d_lcol, s < nc ? col : 255,
-                         c->pitch, Dpinv, Aoff, x, z, W);
-      const int b0 = c->lbcol_ptr[col], b1 = c->lbcol_ptr[col + 1];
-      if (b1 > b0)
-        hipLaunchKernelGGL((k_line_solve<NVt, TB, TZ>), dim3(b1 - b0), dim3(64), 0, c->stream, b0, c->d_lb_t,
-                           c->d_lb_kl, c->d_seg_cells, c->pitch, Pm, G, (const double*)W, z);
+      const int b0 = c->lbcol_ptr[col], nb = c->lbcol_ptr[col + 1] - b0;
+      const int pass = s >= nc ? LINE_BWD : col == nc - 1 ? LINE_TOP : LINE_FWD;
+      // small meshes are bound by launches: one launch per color; large ones
+      // form w over all the color's cells in a higher-occupancy kernel first
+      const bool fused = c->N < c->line_fused_max;
+      if (fused) {
+        const int ts = t0 + c->lmcol_cells[col];  // first single-cell position of the color
+        const int nsb = (t1 - ts + 63) / 64;
+        if (nb + nsb > 0)
+          hipLaunchKernelGGL((k_line_sweep<NVt, TB, TX, TZ>), dim3(nb + nsb), dim3(64), 0, c->stream, b0, nb, ts, t1,
+                             pass, c->d_lb_t, c->d_lb_kl, c->d_seg_cells, c->d_nq, c->d_lmask, c->pitch, Dpinv,
+                             Aoff, Pm, G, x, sL, z);
+      } else {
+        hipLaunchKernelGGL((k_line_cells<NVt, TB, TX, TZ>), dim3(nblocks(t1 - t0)), dim3(BS), 0, c->stream, t0, t1,
+                           pass, c->d_seg_cells, c->d_nq, c->d_lcomp, c->d_lmask, c->pitch, Dpinv, Aoff, x, sL, z,
+                           c->kry.lineW);
+        if (nb > 0)
+          hipLaunchKernelGGL((k_line_solve<NVt, TB, TZ>), dim3(nb), dim3(64), 0, c->stream, b0, c->d_lb_t,
+                             c->d_lb_kl, c->d_seg_cells, c->pitch, Pm, G, (const double*)c->kry.lineW, z);
+      }
     }
   });
 }

@@ -386,14 +386,20 @@ void build_lines(cfd_mesh& m) {
     scol[k] = col;
     nc = std::max(nc, col + 1);
   }
-  // color-major segment list
+  // color-major segment list; within a color the multi-cell segments come
+  // first, longest first, so that the 64-cell groups the device sweeps pack
+  // them into occupy consecutive line positions (the order inside a color does
+  // not change the sweep)
   std::vector<int> lst;
   lst.reserve(ns);
   m.segcol_ptr.assign(nc + 1, 0);
   for (int col = 0; col < nc; ++col) {
     m.segcol_ptr[col] = (int)lst.size();
+    const size_t first = lst.size();
     for (int k : sorder)
       if (scol[k] == col) lst.push_back(k);
+    std::stable_sort(lst.begin() + first, lst.end(),
+                     [&](int a, int b) { return segs[a].size() > segs[b].size(); });
   }
   m.segcol_ptr[nc] = (int)lst.size();
   m.seg_ptr.assign(1, 0);

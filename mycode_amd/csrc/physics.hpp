@@ -52,6 +52,38 @@ CFD_ARITH HD Dual<F> operator/(A a, Dual<F> b) {
   return Dual<F>(q, -q * b.d / b.v);
 }
 
+// FP32 duals serve only the FP32 tangent, which keeps the derivative parts;
+// their value parts (which steer the branches) are FP32 approximations of the
+// FP64 state anyway.  Quotients and square roots therefore use the hardware
+// reciprocal and square root (about 1 ulp) instead of the correctly rounded
+// sequences, with one reciprocal per dual quotient instead of two divisions.
+HD float fast_rcp(float x) {
+#ifdef __HIP_DEVICE_COMPILE__
+  return __builtin_amdgcn_rcpf(x);
+#else
+  return 1.0f / x;
+#endif
+}
+HD float fast_sqrt(float x) {
+#ifdef __HIP_DEVICE_COMPILE__
+  return __builtin_amdgcn_sqrtf(x);
+#else
+  return std::sqrt(x);
+#endif
+}
+HD Dual<float> operator/(Dual<float> a, Dual<float> b) {
+  const float r = fast_rcp(b.v), q = a.v * r;
+  return Dual<float>(q, (a.d - q * b.d) * r);
+}
+HD Dual<float> operator/(Dual<float> a, double b) {
+  const float r = fast_rcp(float(b));
+  return Dual<float>(a.v * r, a.d * r);
+}
+HD Dual<float> operator/(double a, Dual<float> b) {
+  const float r = fast_rcp(b.v), q = float(a) * r;
+  return Dual<float>(q, -q * b.d * r);
+}
+
 // K-direction forward dual: value plus K directional derivatives, used for
 // face-local Jacobian-transpose products (one pass covers K inputs).
 template <class F, int K> struct DualV {
@@ -154,6 +186,10 @@ HD float sq_rt(float x) { return sqrtf(x); }
 template <class F> HD Dual<F> sq_rt(Dual<F> x) {
   F s = sq_rt(x.v);
   return Dual<F>(s, x.d / (F(2) * s));
+}
+HD Dual<float> sq_rt(Dual<float> x) {
+  const float s = fast_sqrt(x.v);
+  return Dual<float>(s, x.d * (0.5f * fast_rcp(s)));
 }
 HD double pw(double a, double b) { return pow(a, b); }
 HD float pw(float a, float b) { return powf(a, b); }

@@ -62,7 +62,12 @@ def _line_sweeps(B, A, nbr, sp, cells, cp, x):
 
 @pytest.mark.parametrize("config,scale", [(3, 0.02), (2, 0.05)])
 @pytest.mark.parametrize("prec", [64, 32])
-def test_line_apply_matches_dense_sweeps(config, scale, prec):
+@pytest.mark.parametrize("split", [False, True])
+def test_line_apply_matches_dense_sweeps(config, scale, prec, split, monkeypatch):
+    # small meshes sweep each color in one launch; large ones run the
+    # single-cell segments in a kernel of their own (forced here by the
+    # CFD_LINE_FUSED_MAX_CELLS threshold, read when the context is created)
+    monkeypatch.setenv("CFD_LINE_FUSED_MAX_CELLS", "0" if split else "262144")
     mesh, p, U = _state(config, scale)
     s = cfd.Solver(mesh, p)
     nv = p.nvar
@@ -77,7 +82,7 @@ def test_line_apply_matches_dense_sweeps(config, scale, prec):
     sp, cells, cp = mesh.lines()
     zr = _line_sweeps(B, A, nbr, sp, cells, cp, x[:, :nv])
     err = np.linalg.norm(z[:, :nv] - zr) / np.linalg.norm(zr)
-    print(f"config={config} FP{prec} N={s.N} segments={len(sp) - 1} longest={np.diff(sp).max()} "
+    print(f"config={config} FP{prec} split={split} N={s.N} segments={len(sp) - 1} longest={np.diff(sp).max()} "
           f"colors={len(cp) - 1} error {err:.2e}", flush=True)
     assert err < (1e-9 if prec == 64 else 1e-4)
 
