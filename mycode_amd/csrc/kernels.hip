@@ -1339,46 +1339,78 @@ __global__ __launch_bounds__(64) void k_line_factor(const int* __restrict__ lb_t
   }
 }
 
-// The two recurrences of a bundle, at step s by the lanes holding the s-th
-// cell of their segment: y_t = w_t - P_t y_{t-1} ascending, then
-// z_t = y_t - G_t z_{t+1} descending.  y holds w on entry and z on exit.
+// The two recurrences of a bundle, y_t = w_t - P_t y_{t-1} ascending and then
+// z_t = y_t - G_t z_{t+1} descending, as scans of affine maps across the
+// lanes.  Each lane holds the map y -> b + M y of its cell (b = w, M = -P; the
+// first cell of a segment has no map to compose with).  At distance
+// d = 1, 2, 4, ... every lane at least d cells into its segment composes its
+// map with that of the lane d below: (b, M) <- (b + M b', M M').  After
+// ceil(log2 L) levels b is y_t: that many dependent steps instead of L - 1.
+// The descending pass mirrors this with M = -G and the lane d above.  reach is
+// the number of cells of the segment before (ASC) or after the lane's cell;
+// steps, the longest segment of the bundle, bounds the distances.  y holds w
+// on entry and z on exit.
+template <int NV, bool ASC, class TB, class TC>
+__device__ __forceinline__ void line_scan(int steps, int reach, const TB (&A)[NV * NV], TC (&y)[NV]) {
+  constexpr int NN = NV * NV;
+  TC M[NN];
+#pragma unroll
+  for (int e = 0; e < NN; ++e) M[e] = -(TC)A[e];
+  for (int d = 1; d < steps; d <<= 1) {
+    const bool last = 2 * d >= steps;  // the maps are not needed after the last level
+    TC bo[NV];
+#pragma unroll
+    for (int i = 0; i < NV; ++i) bo[i] = ASC ? __shfl_up(y[i], d) : __shfl_down(y[i], d);
+    if (last) {
+      if (reach >= d) {
+        TC bn[NV];
+#pragma unroll
+        for (int i = 0; i < NV; ++i) {
+          TC a = y[i];
+#pragma unroll
+          for (int j = 0; j < NV; ++j) a = fma(M[i * NV + j], bo[j], a);
+          bn[i] = a;
+        }
+#pragma unroll
+        for (int i = 0; i < NV; ++i) y[i] = bn[i];
+      }
+      break;
+    }
+    TC Mo[NN];
+#pragma unroll
+    for (int e = 0; e < NN; ++e) Mo[e] = ASC ? __shfl_up(M[e], d) : __shfl_down(M[e], d);
+    if (reach >= d) {
+      TC bn[NV], Mn[NN];
+#pragma unroll
+      for (int i = 0; i < NV; ++i) {
+        TC a = y[i];
+#pragma unroll
+        for (int j = 0; j < NV; ++j) a = fma(M[i * NV + j], bo[j], a);
+        bn[i] = a;
+      }
+#pragma unroll
+      for (int i = 0; i < NV; ++i)
+#pragma unroll
+        for (int j = 0; j < NV; ++j) {
+          TC a = TC(0);
+#pragma unroll
+          for (int q = 0; q < NV; ++q) a = fma(M[i * NV + q], Mo[q * NV + j], a);
+          Mn[i * NV + j] = a;
+        }
+#pragma unroll
+      for (int i = 0; i < NV; ++i) y[i] = bn[i];
+#pragma unroll
+      for (int e = 0; e < NN; ++e) M[e] = Mn[e];
+    }
+  }
+}
+// Both passes for a lane at place k of a segment of length L (live: not a
+// padding slot)
 template <int NV, class TB, class TC>
-__device__ __forceinline__ void line_recur(int steps, int k, bool hp, bool hg, const TB (&pr)[NV * NV],
+__device__ __forceinline__ void line_recur(int steps, int k, int L, bool live, const TB (&pr)[NV * NV],
                                            const TB (&gr)[NV * NV], TC (&y)[NV]) {
-  for (int s = 1; s < steps; ++s) {  // y_t = w_t - P_t y_{t-1}
-    TC yp[NV];
-#pragma unroll
-    for (int i = 0; i < NV; ++i) yp[i] = lane_prev(y[i]);
-    if (hp && k == s) {
-      TC yn[NV];
-#pragma unroll
-      for (int i = 0; i < NV; ++i) {
-        TC a = y[i];
-#pragma unroll
-        for (int j = 0; j < NV; ++j) a = fma(-(TC)pr[i * NV + j], yp[j], a);
-        yn[i] = a;
-      }
-#pragma unroll
-      for (int i = 0; i < NV; ++i) y[i] = yn[i];
-    }
-  }
-  for (int s = steps - 2; s >= 0; --s) {  // z_t = y_t - G_t z_{t+1}
-    TC zn[NV];
-#pragma unroll
-    for (int i = 0; i < NV; ++i) zn[i] = lane_next(y[i]);
-    if (hg && k == s) {
-      TC yn[NV];
-#pragma unroll
-      for (int i = 0; i < NV; ++i) {
-        TC a = y[i];
-#pragma unroll
-        for (int j = 0; j < NV; ++j) a = fma(-(TC)gr[i * NV + j], zn[j], a);
-        yn[i] = a;
-      }
-#pragma unroll
-      for (int i = 0; i < NV; ++i) y[i] = yn[i];
-    }
-  }
+  line_scan<NV, true, TB, TC>(steps, live ? k : 0, pr, y);
+  line_scan<NV, false, TB, TC>(steps, live ? L - 1 - k : 0, gr, y);
 }
 
 // Sweep passes.  A color is visited once ascending (forward) and, below the
@@ -1480,7 +1512,6 @@ __global__ __launch_bounds__(64) void k_line_solve(int b0, const int* __restrict
   const int kl = lb_kl[m];
   const int k = kl & 255, L = kl >> 8;
   const int steps = __builtin_amdgcn_readfirstlane(lb_kl[bb] >> 8);  // slot 0 holds a longest segment
-  const bool hp = t >= 0 && k > 0, hg = t >= 0 && k < L - 1;
   TC y[NV];
   TB pr[NN], gr[NN];
 #pragma unroll
@@ -1490,7 +1521,7 @@ __global__ __launch_bounds__(64) void k_line_solve(int b0, const int* __restrict
     pr[e] = Pm[m * NN + e];
     gr[e] = G[m * NN + e];
   }
-  line_recur<NV, TB, TC>(steps, k, hp, hg, pr, gr, y);
+  line_recur<NV, TB, TC>(steps, k, L, t >= 0, pr, gr, y);
   if (t >= 0) {
     const int c = seg_cells[t];
 #pragma unroll
@@ -1533,7 +1564,6 @@ __global__ __launch_bounds__(64) void k_line_sweep(int b0, int nb, int t0, int t
   const int kl = lb_kl[m];
   const int k = kl & 255, L = kl >> 8;
   const int steps = __builtin_amdgcn_readfirstlane(lb_kl[bb] >> 8);  // slot 0 holds a longest segment
-  const bool hp = t >= 0 && k > 0, hg = t >= 0 && k < L - 1;
   // every lane stays active for the lane moves: a padding slot works on slot
   // 0's cell and writes nothing; every slot of P and G is allocated, so their
   // loads are unconditional (a guarded load becomes a branch that waits for it)
@@ -1550,7 +1580,7 @@ __global__ __launch_bounds__(64) void k_line_sweep(int b0, int nb, int t0, int t
   TC y[NV];
 #pragma unroll
   for (int i = 0; i < NV; ++i) y[i] = (TC)w[i];
-  line_recur<NV, TB, TC>(steps, k, hp, hg, pr, gr, y);
+  line_recur<NV, TB, TC>(steps, k, L, t >= 0, pr, gr, y);
   if (t >= 0) {
 #pragma unroll
     for (int i = 0; i < NV; ++i) z[i * p + c] = (TZ)y[i];
