@@ -35,7 +35,7 @@ struct Krylov {
   void* blk = nullptr;         // block-Jacobi inverses (nv x nv per cell)
   void* aoff = nullptr;        // neighbor coupling blocks (4 x nv x nv per cell), block SGS only
   double* xt = nullptr;        // basis combination before the block SGS solve (FP64)
-  double* fjac = nullptr;      // face Jacobians of the first-order flux (2 nv^2 x F), block preconditioners
+  double* fjac = nullptr;      // face Jacobians of the first-order flux (2 nv^2 x F, block precision)
   double* bd = nullptr;        // FP64 diagonal blocks in segment order (nv^2 x pitch), line preconditioner
   void* lineG = nullptr;       // line factors G_t = Dp_t^-1 A_{t,next} (nv^2 per bundle slot, basis precision)
   void* lineP = nullptr;       // line factors P_t = Dp_t^-1 A_{t,prev} (nv^2 per bundle slot, basis precision)

@@ -843,10 +843,12 @@ __global__ __launch_bounds__(BS) void k_diagprec(DMesh m, Phys P, const double* 
 template <int NV> __device__ __forceinline__ size_t jf_index(int f, int e) {
   return ((size_t)(f >> 6) * (2 * NV * NV) + e) * 64 + (f & 63);
 }
-template <int NV>
+// TV is the precision of the blocks: FP32 blocks take FP32 duals (the
+// preconditioner of the FP32 Krylov path needs no more), FP64 blocks FP64 ones.
+template <int NV, class TV>
 __global__ __launch_bounds__(BS) void k_face_jac(DMesh m, Phys P, const double* __restrict__ U,
-                                                  double* __restrict__ Jf) {
-  typedef Dual<double> D;
+                                                  TV* __restrict__ Jf) {
+  typedef Dual<TV> D;
   const int f = xcd_block() * BS + threadIdx.x;
   if (f >= m.F) return;
   const int p = m.pitch;
@@ -863,8 +865,8 @@ __global__ __launch_bounds__(BS) void k_face_jac(DMesh m, Phys P, const double* 
     S6<D> Ld, Rd;
 #pragma unroll
     for (int i = 0; i < 6; ++i) {
-      Ld[i] = D(UL[i], side == 0 && i == k ? 1.0 : 0.0);
-      Rd[i] = D(UR[i], side == 1 && i == k ? 1.0 : 0.0);
+      Ld[i] = D(TV(UL[i]), TV(side == 0 && i == k ? 1.0 : 0.0));
+      Rd[i] = D(TV(UR[i]), TV(side == 1 && i == k ? 1.0 : 0.0));
     }
     if (n < 0) Rd = ghost(P1, Ld, g.tg, g.nx, g.ny, g.dnw);
     const S6<D> Fl = flux_core<0, D>(P1, g, Ld, Rd, nullptr, nullptr);
@@ -876,10 +878,10 @@ __global__ __launch_bounds__(BS) void k_face_jac(DMesh m, Phys P, const double* 
 template <int NV, class TV>
 __global__ __launch_bounds__(BS) void k_block(DMesh m, Phys P, const double* __restrict__ U,
                                                const double* __restrict__ G, const double* __restrict__ dt,
-                                               const double* __restrict__ Jf, TV* __restrict__ Binv,
+                                               const TV* __restrict__ Jf, TV* __restrict__ Binv,
                                                double* __restrict__ Bout, TV* __restrict__ Aoff,
                                                const double* __restrict__ Q, const int* __restrict__ pos) {
-  typedef Dual<double> D;
+  typedef Dual<TV> D;  // SST source Jacobian in the precision of the blocks
   const int c = xcd_block() * BS + threadIdx.x;
   if (c >= m.N) return;
   const int p = m.pitch;
@@ -909,7 +911,7 @@ __global__ __launch_bounds__(BS) void k_block(DMesh m, Phys P, const double* __r
 #pragma unroll
     for (int i = 0; i < NV; ++i)
 #pragma unroll
-      for (int k = 0; k < NV; ++k) B[i][k] += sg * Jf[jf_index<NV>(f, es + i * NV + k)];
+      for (int k = 0; k < NV; ++k) B[i][k] += sg * (double)Jf[jf_index<NV>(f, es + i * NV + k)];
     // viscous diagonal: the two-point diffusion coefficient of each row,
     // (nu + nu_t) for momentum, gamma (nu / Pr + nu_t / Pr_t) for energy,
     // (nu + sigma nu_t) for k and omega, times A over the normal distance
@@ -947,23 +949,23 @@ __global__ __launch_bounds__(BS) void k_block(DMesh m, Phys P, const double* __r
 #pragma unroll
         for (int k = 0; k < NV; ++k)
           Aoff[(size_t)((q * NV + i) * NV + k) * p + oc] =
-              (TV)(sg * Jf[jf_index<NV>(f, eo + i * NV + k)] - (i == k ? vf[i] : 0.0));
+              (TV)(sg * (double)Jf[jf_index<NV>(f, eo + i * NV + k)] - (i == k ? vf[i] : 0.0));
     }
   }
   if constexpr (NV == 6) {
     if (P.turb) {  // source Jacobian with the gradients frozen
       D gd[14];
 #pragma unroll
-      for (int q = 0; q < 14; ++q) gd[q] = D(G[(size_t)q * p + c], 0.0);
+      for (int q = 0; q < 14; ++q) gd[q] = D(TV(G[(size_t)q * p + c]), TV(0));
       const double V = m.vol[c];
 #pragma unroll
       for (int k = 0; k < NV; ++k) {
         S6<D> Ud;
 #pragma unroll
-        for (int i = 0; i < 6; ++i) Ud[i] = D(Uc[i], i == k ? 1.0 : 0.0);
+        for (int i = 0; i < 6; ++i) Ud[i] = D(TV(Uc[i]), TV(i == k ? 1.0 : 0.0));
         const TurbQ<D> tq = turb_cell(P, Ud, gd, m.wd[c]);
-        B[4][k] -= V * tq.Sk.d;
-        B[5][k] -= V * tq.Sw.d;
+        B[4][k] -= V * (double)tq.Sk.d;
+        B[5][k] -= V * (double)tq.Sw.d;
       }
     }
   }
@@ -2371,10 +2373,10 @@ static void block_t(cfd_ctx* c, const double* U, TV* Binv, double* Bout, TV* Aof
     hipLaunchKernelGGL((k_gradlim<NVt, double>), dim3(nblocks(c->N)), dim3(BS), 0, c->stream, c->dm, c->phys, U,
                        (const double*)nullptr, c->grad, (double*)nullptr, c->phi, 0,
                        turb ? c->tq : (double*)nullptr, (double*)nullptr);
-    hipLaunchKernelGGL((k_face_jac<NVt>), dim3(nblocks(c->F)), dim3(BS), 0, c->stream, c->dm, c->phys, U,
-                       c->kry.fjac);
+    hipLaunchKernelGGL((k_face_jac<NVt, TV>), dim3(nblocks(c->F)), dim3(BS), 0, c->stream, c->dm, c->phys, U,
+                       (TV*)c->kry.fjac);
     hipLaunchKernelGGL((k_block<NVt, TV>), dim3(nblocks(c->N)), dim3(BS), 0, c->stream, c->dm, c->phys, U,
-                       (const double*)c->grad, c->dt, (const double*)c->kry.fjac, Binv, Bout, Aoff,
+                       (const double*)c->grad, c->dt, (const TV*)c->kry.fjac, Binv, Bout, Aoff,
                        turb ? (const double*)c->tq : nullptr, pos);
   });
 }
