@@ -578,17 +578,24 @@ int cfd_ctx_create(const cfd_mesh* m, const cfd_params* p, cfd_ctx** out) {
       for (int col = 0; col < ncol; ++col)
         for (int k = m->segcol_ptr[col]; k < m->segcol_ptr[col + 1]; ++k)
           for (int t = m->seg_ptr[k]; t < m->seg_ptr[k + 1]; ++t) lcol[m->seg_cells[t]] = col;
+      // both indexed by line position t, with the cells' neighbour table
       std::vector<unsigned char> lmask(P_, 0);
-      for (int i = 0; i < N; ++i)
+      std::vector<int4> lnq(P_, make_int4(-1, -1, -1, -1));
+      for (int t = 0; t < N; ++t) {
+        const int i = m->seg_cells[t];
+        lnq[t] = nq[i];
         for (int q = 0; q < 4; ++q) {
           const int en = (&nq[i].x)[q];
           if (en < 0 || q == m->lslot_prev[i] || q == m->lslot_next[i]) continue;
           const int o = en >> 2;
           if (lcol[o] == lcol[i]) return fail(c, CFD_ERR_MESH, "neighbouring segments share a color");
-          lmask[i] |= (unsigned char)(lcol[o] < lcol[i] ? 1u << q : 16u << q);
+          lmask[t] |= (unsigned char)(lcol[o] < lcol[i] ? 1u << q : 16u << q);
         }
+      }
       HIP_TRY(c, dalloc(&c->d_lmask, P_));
+      HIP_TRY(c, dalloc(&c->d_lnq, P_));
       HIP_TRY(c, hipMemcpy(c->d_lmask, lmask.data(), P_, hipMemcpyHostToDevice));
+      HIP_TRY(c, hipMemcpy(c->d_lnq, lnq.data(), (size_t)P_ * sizeof(int4), hipMemcpyHostToDevice));
       const size_t ns = std::max(lb_t.size(), (size_t)1);
       HIP_TRY(c, dalloc(&c->d_lb_t, ns));
       HIP_TRY(c, dalloc(&c->d_lb_kl, ns));
@@ -635,7 +642,7 @@ int cfd_ctx_destroy(cfd_ctx* c) {
   (void)hipStreamSynchronize(c->stream);
   free_krylov(c);
   void* bufs[] = {c->d_vol, c->d_xc,  c->d_yc,  c->d_slots, c->d_own, c->d_nei, c->d_tag,  c->d_nx,
-                  c->d_ny,  c->d_area, c->d_xm, c->d_ym,    c->d_iperm, c->d_facemap, c->d_alt, c->grad, c->d_wd, c->d_dnw, c->d_corder, c->d_ccol, c->d_ccells, c->d_nq, c->d_cpos, c->d_seg_ptr, c->d_seg_cells, c->d_lpos, c->d_lslot, c->d_lb_t, c->d_lb_kl, c->d_lcomp, c->d_lmask,
+                  c->d_ny,  c->d_area, c->d_xm, c->d_ym,    c->d_iperm, c->d_facemap, c->d_alt, c->grad, c->d_wd, c->d_dnw, c->d_corder, c->d_ccol, c->d_ccells, c->d_nq, c->d_cpos, c->d_seg_ptr, c->d_seg_cells, c->d_lpos, c->d_lslot, c->d_lb_t, c->d_lb_kl, c->d_lcomp, c->d_lmask, c->d_lnq,
                   c->phi,   c->dt,    c->Fm,    c->red,     c->scal, c->gradT, c->adj, c->tq, c->io6};
   for (void* b : bufs) (void)hipFree(b);
   for (int i = 0; i < 6; ++i) (void)hipFree(c->vec[i]);

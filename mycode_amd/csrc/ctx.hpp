@@ -89,7 +89,9 @@ struct cfd_ctx {
   int* d_lb_t = nullptr;            // per slot: line position of its cell (-1: padding)
   int* d_lb_kl = nullptr;           // per slot: place in the segment | segment length << 8
   int* d_lcomp = nullptr;           // per line position: bundle slot (-1: single-cell segment)
-  unsigned char* d_lmask = nullptr; // per cell: off-segment neighbour slots of lower (bits 0-3) / higher (4-7) color
+  unsigned char* d_lmask = nullptr; // per line position: off-segment neighbour slots of lower (bits 0-3) / higher
+                                    // (4-7) color
+  int4* d_lnq = nullptr;            // per line position: the cell's d_nq entry
   int line_fused_max = 262144;      // below this many cells a line sweep color is one launch
   int4* d_nq = nullptr;             // per cell slot: (neighbor << 2) | back slot, or -1
   // solver state

@@ -1427,14 +1427,24 @@ __device__ __forceinline__ void line_recur(int steps, int k, int L, bool live, c
 enum LinePass { LINE_FWD = 0, LINE_TOP = 1, LINE_BWD = 2 };
 
 // w_t = Dp_t^-1 s for the cell c at line position t (pass as above); keep
-// false suppresses the store of s (a padding lane standing in for a cell)
+// false suppresses the store of s (a padding lane standing in for a cell).
+// The neighbour table lnq and the masks lmask are stored by line position, and
+// FP32 blocks load Dp^-1 first, so that everything but x, s and the
+// neighbours' z is in flight after one dependent load (FP64 blocks load it
+// last: held from the start they cost more occupancy than the early load gains).
 template <int NV, class TB, class TX, class TZ>
-__device__ __forceinline__ void line_w(int t, int c, int pass, bool keep, const int4* __restrict__ nq,
+__device__ __forceinline__ void line_w(int t, int c, int pass, bool keep, const int4* __restrict__ lnq,
                                        const unsigned char* __restrict__ lmask, int p,
                                        const TB* __restrict__ Dpinv, const TB* __restrict__ Aoff,
                                        const TX* __restrict__ x, TZ* __restrict__ sL, const TZ* __restrict__ z,
                                        double (&w)[NV]) {
-  const unsigned mk = lmask[c];
+  constexpr bool early = sizeof(TB) == 4;
+  TB dp[NV * NV];
+  if (early) {
+#pragma unroll
+    for (int e = 0; e < NV * NV; ++e) dp[e] = Dpinv[(size_t)e * p + t];
+  }
+  const unsigned mk = lmask[t];
   const unsigned use = pass == LINE_BWD ? mk >> 4 : mk & 15u;
   double s[NV];
   if (pass == LINE_BWD) {
@@ -1444,7 +1454,7 @@ __device__ __forceinline__ void line_w(int t, int c, int pass, bool keep, const 
 #pragma unroll
     for (int i = 0; i < NV; ++i) s[i] = (double)x[i * p + c];
   }
-  const int4 e4 = nq[c];
+  const int4 e4 = lnq[t];
   const int es[4] = {e4.x, e4.y, e4.z, e4.w};
 #pragma unroll
   for (int q = 0; q < 4; ++q) {
@@ -1466,7 +1476,7 @@ __device__ __forceinline__ void line_w(int t, int c, int pass, bool keep, const 
   for (int i = 0; i < NV; ++i) {
     double a = 0.0;
 #pragma unroll
-    for (int j = 0; j < NV; ++j) a += (double)Dpinv[(size_t)(i * NV + j) * p + t] * s[j];
+    for (int j = 0; j < NV; ++j) a += (double)(early ? dp[i * NV + j] : Dpinv[(size_t)(i * NV + j) * p + t]) * s[j];
     w[i] = a;
   }
 }
@@ -2418,11 +2428,11 @@ static void line_t(cfd_ctx* c, const TB* Dpinv, const TB* Pm, const TB* G, const
         const int nsb = (t1 - ts + 63) / 64;
         if (nb + nsb > 0)
           hipLaunchKernelGGL((k_line_sweep<NVt, TB, TX, TZ>), dim3(nb + nsb), dim3(64), 0, c->stream, b0, nb, ts, t1,
-                             pass, c->d_lb_t, c->d_lb_kl, c->d_seg_cells, c->d_nq, c->d_lmask, c->pitch, Dpinv,
+                             pass, c->d_lb_t, c->d_lb_kl, c->d_seg_cells, c->d_lnq, c->d_lmask, c->pitch, Dpinv,
                              Aoff, Pm, G, x, sL, z);
       } else {
         hipLaunchKernelGGL((k_line_cells<NVt, TB, TX, TZ>), dim3(nblocks(t1 - t0)), dim3(BS), 0, c->stream, t0, t1,
-                           pass, c->d_seg_cells, c->d_nq, c->d_lcomp, c->d_lmask, c->pitch, Dpinv, Aoff, x, sL, z,
+                           pass, c->d_seg_cells, c->d_lnq, c->d_lcomp, c->d_lmask, c->pitch, Dpinv, Aoff, x, sL, z,
                            c->kry.lineW);
         if (nb > 0)
           hipLaunchKernelGGL((k_line_solve<NVt, TB, TZ>), dim3(nb), dim3(64), 0, c->stream, b0, c->d_lb_t,
