@@ -94,7 +94,7 @@ __global__ __launch_bounds__(BS) void k_gradlim(DMesh m, Phys P, const double* _
   const bool turb = NV == 6 && P.turb;
   const int nq = turb ? 7 : 5;
   const S6<T> Uc = ldT<NV, T>(U, v, p, c);
-  const T uc = Uc[1] / Uc[0], vc = Uc[2] / Uc[0], pc = pressure(P, Uc), Tc = pc / (Uc[0] * Rg);
+  const T uc = dv(Uc[1], Uc[0]), vc = dv(Uc[2], Uc[0]), pc = pressure(P, Uc), Tc = dv(pc, Uc[0] * Rg);
   T g[2 * NQM];
 #pragma unroll
   for (int q = 0; q < 2 * NQM; ++q) g[q] = T(0.0);
@@ -111,7 +111,7 @@ __global__ __launch_bounds__(BS) void k_gradlim(DMesh m, Phys P, const double* _
     const double nx = m.nx[f], ny = m.ny[f], A = m.area[f];
     const S6<T> Uo = other >= 0 ? ldT<NV, T>(U, v, p, other)
                                 : ghost(P, Uc, m.tag[f], nx, ny, turb ? m.dnw[f] : 0.0);
-    const T uo = Uo[1] / Uo[0], vo = Uo[2] / Uo[0], po = pressure(P, Uo), To = po / (Uo[0] * Rg);
+    const T uo = dv(Uo[1], Uo[0]), vo = dv(Uo[2], Uo[0]), po = pressure(P, Uo), To = dv(po, Uo[0] * Rg);
     T fv[NQM];
     fv[0] = 0.5 * (Uc[0] + Uo[0]);
     fv[1] = 0.5 * (uc + uo);
@@ -120,8 +120,8 @@ __global__ __launch_bounds__(BS) void k_gradlim(DMesh m, Phys P, const double* _
     fv[4] = 0.5 * (Tc + To);
     if constexpr (NQM == 7) {
       if (turb) {
-        fv[5] = 0.5 * (Uc[4] / Uc[0] + Uo[4] / Uo[0]);
-        fv[6] = 0.5 * (Uc[5] / Uc[0] + Uo[5] / Uo[0]);
+        fv[5] = 0.5 * (dv(Uc[4], Uc[0]) + dv(Uo[4], Uo[0]));
+        fv[6] = 0.5 * (dv(Uc[5], Uc[0]) + dv(Uo[5], Uo[0]));
       } else {
         fv[5] = fv[6] = T(0.0);
       }
@@ -146,7 +146,7 @@ __global__ __launch_bounds__(BS) void k_gradlim(DMesh m, Phys P, const double* _
 #pragma unroll
   for (int q = 0; q < 2 * NQM; ++q) {
     if (q < 2 * nq) {
-      g[q] = g[q] / vol;
+      g[q] = dv(g[q], vol);
       Sc::put(Gv, Gd, (size_t)q * p + c, g[q]);
     }
   }
@@ -246,7 +246,7 @@ __device__ __forceinline__ S6<T> flux_core(const Phys& P, const FaceGeo& g, cons
   if (P.spatial != CFD_FIRST_ORDER && !g.boundary) {
     const double dxL = g.xm - g.xco, dyL = g.ym - g.yco, dxR = g.xm - g.xcn, dyR = g.ym - g.ycn;
     const double phL = g.phL, phR = g.phR;
-    const T uL = L[1] / L[0], vL = L[2] / L[0], pL = pressure(P, L);
+    const T uL = dv(L[1], L[0]), vL = dv(L[2], L[0]), pL = pressure(P, L);
     T rLf = L[0] + phL * (gL[0] * dxL + gL[1] * dyL);
     T pLf = pL + phL * (gL[6] * dxL + gL[7] * dyL);
     const T uLf = uL + phL * (gL[2] * dxL + gL[3] * dyL);
@@ -254,7 +254,7 @@ __device__ __forceinline__ S6<T> flux_core(const Phys& P, const FaceGeo& g, cons
     if (val(rLf) < 1e-4) rLf = L[0];
     if (val(pLf) < 1e-4) pLf = pL;
     Lr = prim2cons(P, rLf, uLf, vLf, pLf);
-    const T uR = R[1] / R[0], vR = R[2] / R[0], pR = pressure(P, R);
+    const T uR = dv(R[1], R[0]), vR = dv(R[2], R[0]), pR = pressure(P, R);
     T rRf = R[0] + phR * (gR[0] * dxR + gR[1] * dyR);
     T pRf = pR + phR * (gR[6] * dxR + gR[7] * dyR);
     const T uRf = uR + phR * (gR[2] * dxR + gR[3] * dyR);
@@ -263,7 +263,7 @@ __device__ __forceinline__ S6<T> flux_core(const Phys& P, const FaceGeo& g, cons
     if (val(pRf) < 1e-4) pRf = pR;
     Rr = prim2cons(P, rRf, uRf, vRf, pRf);
     if (P.turb) {  // MUSCL on specific k and omega, clipped at zero
-      const T kL = L[4] / L[0], wL = L[5] / L[0], kR = R[4] / R[0], wR = R[5] / R[0];
+      const T kL = dv(L[4], L[0]), wL = dv(L[5], L[0]), kR = dv(R[4], R[0]), wR = dv(R[5], R[0]);
       T kLf = kL + phL * (gL[10] * dxL + gL[11] * dyL), wLf = wL + phL * (gL[12] * dxL + gL[13] * dyL);
       T kRf = kR + phR * (gR[10] * dxR + gR[11] * dyR), wRf = wR + phR * (gR[12] * dxR + gR[13] * dyR);
       if (val(kLf) < 0.0) kLf = kL;
@@ -289,22 +289,22 @@ __device__ __forceinline__ S6<T> flux_core(const Phys& P, const FaceGeo& g, cons
   if (P.turb) {  // k and omega carried by the mass flux, upwinded
     const T mf = Fl[0];
     const T mp = val(mf) > 0.0 ? mf : T(0.0), mm = val(mf) > 0.0 ? T(0.0) : mf;
-    Fl[4] = mp * (Lr[4] / Lr[0]) + mm * (Rr[4] / Rr[0]);
-    Fl[5] = mp * (Lr[5] / Lr[0]) + mm * (Rr[5] / Rr[0]);
+    Fl[4] = mp * dv(Lr[4], Lr[0]) + mm * dv(Rr[4], Rr[0]);
+    Fl[5] = mp * dv(Lr[5], Lr[0]) + mm * dv(Rr[5], Rr[0]);
   }
   if (VISC) {
     const T rf = 0.5 * (L[0] + R[0]);
-    const T uf = 0.5 * (L[1] / L[0] + R[1] / R[0]);
-    const T vf = 0.5 * (L[2] / L[0] + R[2] / R[0]);
+    const T uf = 0.5 * (dv(L[1], L[0]) + dv(R[1], R[0]));
+    const T vf = 0.5 * (dv(L[2], L[0]) + dv(R[2], R[0]));
     T ux, uy, vx, vy, Tx, Ty;
     if (g.boundary && g.tg == 4) {
       const double dx = g.xm - g.xco, dy = g.ym - g.yco;
       const double d2 = dx * dx + dy * dy;
-      const T uc = L[1] / L[0], vc = L[2] / L[0];
-      ux = -uc * dx / d2;
-      uy = -uc * dy / d2;
-      vx = -vc * dx / d2;
-      vy = -vc * dy / d2;
+      const T uc = dv(L[1], L[0]), vc = dv(L[2], L[0]);
+      ux = dv(-uc * dx, d2);
+      uy = dv(-uc * dy, d2);
+      vx = dv(-vc * dx, d2);
+      vy = dv(-vc * dy, d2);
       Tx = T(0.0);
       Ty = T(0.0);
     } else {
@@ -318,18 +318,18 @@ __device__ __forceinline__ S6<T> flux_core(const Phys& P, const FaceGeo& g, cons
         const double dxLR = g.xcn - g.xco, dyLR = g.ycn - g.yco;
         const double d2 = dxLR * dxLR + dyLR * dyLR;
         if (d2 > 1e-14) {
-          const T du = (R[1] / R[0]) - (L[1] / L[0]);
-          const T dv = (R[2] / R[0]) - (L[2] / L[0]);
+          const T du = dv(R[1], R[0]) - dv(L[1], L[0]);
+          const T dvv = dv(R[2], R[0]) - dv(L[2], L[0]);
           const T ux0 = ux, uy0 = uy, vx0 = vx, vy0 = vy;
-          ux = ux0 - (ux0 * dxLR + uy0 * dyLR - du) * dxLR / d2;
-          uy = uy0 - (ux0 * dxLR + uy0 * dyLR - du) * dyLR / d2;
-          vx = vx0 - (vx0 * dxLR + vy0 * dyLR - dv) * dxLR / d2;
-          vy = vy0 - (vx0 * dxLR + vy0 * dyLR - dv) * dyLR / d2;
+          ux = ux0 - dv((ux0 * dxLR + uy0 * dyLR - du) * dxLR, d2);
+          uy = uy0 - dv((ux0 * dxLR + uy0 * dyLR - du) * dyLR, d2);
+          vx = vx0 - dv((vx0 * dxLR + vy0 * dyLR - dvv) * dxLR, d2);
+          vy = vy0 - dv((vx0 * dxLR + vy0 * dyLR - dvv) * dyLR, d2);
         }
       }
     }
     T mu = rf * P.nu;
-    T kc = mu * P.Cp / P.Pr;
+    T kc = dv(mu * P.Cp, P.Pr);
     T mut = T(0.0), F1f = T(0.0);
     if (P.turb) {  // eddy viscosity in stress and heat flux
       const TurbQ<T> qo = tqo ? *tqo : turb_cell(P, L, gL, g.wdo);
@@ -341,7 +341,7 @@ __device__ __forceinline__ S6<T> flux_core(const Phys& P, const FaceGeo& g, cons
         mut = g.tg == 4 ? T(0.0) : qo.mut;
         F1f = qo.F1;
       }
-      kc = P.Cp * (mu / P.Pr + mut / sst::Prt);
+      kc = P.Cp * (dv(mu, P.Pr) + dv(mut, sst::Prt));
       mu = mu + mut;
     }
     const T div = ux + vy;
@@ -362,17 +362,17 @@ __device__ __forceinline__ S6<T> flux_core(const Phys& P, const FaceGeo& g, cons
       // gradients decouple the two triangles of a quad); averaged gradients on
       // the other boundary faces
       T kn, wn;
-      const T kcl = L[4] / L[0], wcl = L[5] / L[0];
+      const T kcl = dv(L[4], L[0]), wcl = dv(L[5], L[0]);
       if (g.boundary && g.tg == 4) {
         const double dn = g.dnw;
         const double ww = omega_wall(P, g.dnw);
-        kn = (0.0 - kcl) / dn;
-        wn = (ww - wcl) / dn;
+        kn = dv(0.0 - kcl, dn);
+        wn = dv(ww - wcl, dn);
       } else if (!g.boundary) {
         double dn = fabs((g.xcn - g.xco) * nx + (g.ycn - g.yco) * ny);
         if (dn < 1e-300) dn = 1e-300;
-        kn = ((R[4] / R[0]) - kcl) / dn;
-        wn = ((R[5] / R[0]) - wcl) / dn;
+        kn = dv(dv(R[4], R[0]) - kcl, dn);
+        wn = dv(dv(R[5], R[0]) - wcl, dn);
       } else {
         kn = 0.5 * (gL[10] + gR[10]) * nx + 0.5 * (gL[11] + gR[11]) * ny;
         wn = 0.5 * (gL[12] + gR[12]) * nx + 0.5 * (gL[13] + gR[13]) * ny;
@@ -434,6 +434,12 @@ __device__ __forceinline__ S6<T> face_flux(const DMesh& m, const Phys& P, const 
   return flux_core<VISC, T>(P, g, L, R, gL, gR);
 }
 
+// The face buffer holds F * A of face f in the row Fm[f * NV + k] (k the
+// variable): a face's values are adjacent, so they move as 16-byte (FP64, or
+// FP32 with NV = 4) or 8-byte (FP32, NV = 6) vectors.  Rows start on that
+// alignment because the buffer does (device allocations are 256-byte aligned).
+template <int NV, class F> constexpr size_t row_align() { return (sizeof(F) * NV) % 16 == 0 ? 16 : 8; }
+
 // MODE 0: scatter into `out` (the residual) for faces [f0, f1) of one color.
 // MODE 1: write the face buffer.  The tangent writes derivative parts only.
 template <int NV, int VISC, int MODE, class T>
@@ -458,9 +464,11 @@ __global__ __launch_bounds__(BS) void k_flux(DMesh m, Phys P, const double* __re
 #pragma unroll
       for (int k = 0; k < NV; ++k) out[k * p + n] = out[k * p + n] - Sc::dpart(TA[k]);
     }
-  } else {
+  } else {  // face-major row of NV values, written with wide stores
+    typedef typename Scal<T>::F F;
+    F* row = (F*)__builtin_assume_aligned(out + (size_t)f * NV, row_align<NV, F>());
 #pragma unroll
-    for (int k = 0; k < NV; ++k) out[(size_t)k * m.F + f] = Sc::dpart(TA[k]);
+    for (int k = 0; k < NV; ++k) row[k] = Sc::dpart(TA[k]);
   }
 }
 
@@ -479,13 +487,16 @@ __global__ __launch_bounds__(BS) void k_gather(DMesh m, const F* __restrict__ Fm
   for (int j = 0; j < 4; ++j) {
     const int e = sl[j];
     if (e < 0) continue;
-    const int f = e >> 1;
+    const F* row = (const F*)__builtin_assume_aligned(Fm + (size_t)(e >> 1) * NV, row_align<NV, F>());
+    F fr[NV];
+#pragma unroll
+    for (int k = 0; k < NV; ++k) fr[k] = row[k];
     if (e & 1) {
 #pragma unroll
-      for (int k = 0; k < NV; ++k) r[k] = r[k] - Fm[(size_t)k * m.F + f];
+      for (int k = 0; k < NV; ++k) r[k] = r[k] - fr[k];
     } else {
 #pragma unroll
-      for (int k = 0; k < NV; ++k) r[k] = r[k] + Fm[(size_t)k * m.F + f];
+      for (int k = 0; k < NV; ++k) r[k] = r[k] + fr[k];
     }
   }
   if constexpr (NV == 6) {

@@ -22,6 +22,32 @@
 
 namespace cfd {
 
+// ---- FP64 quotients that share their denominator's reciprocal -----------------
+// gfx950 forms an IEEE FP64 quotient a / b as: r = v_rcp_f64(b) refined by two
+// Newton steps r += r (1 - b r); q = a r; q += (a - b q) r; plus operand
+// scaling (v_div_scale / v_div_fmas) that only acts when an exponent nears the
+// ends of the double range, and a fix-up (v_div_fixup) for zero, infinite and
+// NaN operands.  r depends on b alone, so `dv` writes the same steps without
+// the scaling and fix-up: each quotient is bit-identical to a / b for finite,
+// nonzero b of ordinary magnitude (densities, pressures, lengths, coefficients
+// here), and the compiler merges the r of quotients with one denominator
+// (u, v, k, omega by rho; the 14 gradients by the volume), which saves seven
+// of the ten instructions of every further quotient.  Host code divides.
+HD double dv(double a, double b) {
+#ifdef __HIP_DEVICE_COMPILE__
+  double r = __builtin_amdgcn_rcp(b);
+  double e = __builtin_fma(-b, r, 1.0);
+  r = __builtin_fma(r, e, r);
+  e = __builtin_fma(-b, r, 1.0);
+  r = __builtin_fma(r, e, r);
+  const double q = a * r;
+  return __builtin_fma(__builtin_fma(-b, q, a), r, q);
+#else
+  return a / b;
+#endif
+}
+template <class A, class B> HD auto dv(A a, B b) -> decltype(a / b) { return a / b; }
+
 // ---- dual numbers for forward-mode differentiation -------------------------
 template <class F> struct Dual {
   F v, d;
@@ -50,6 +76,18 @@ CFD_ARITH HD Dual<F> operator/(Dual<F> a, A b) { return Dual<F>(a.v / F(b), a.d 
 CFD_ARITH HD Dual<F> operator/(A a, Dual<F> b) {
   F q = F(a) / b.v;
   return Dual<F>(q, -q * b.d / b.v);
+}
+
+// FP64 dual quotients: both parts divide by b.v, so they share its reciprocal
+// (same values as the generic quotients above).
+HD Dual<double> operator/(Dual<double> a, Dual<double> b) {
+  const double q = dv(a.v, b.v);
+  return Dual<double>(q, dv(a.d - q * b.d, b.v));
+}
+HD Dual<double> operator/(Dual<double> a, double b) { return Dual<double>(dv(a.v, b), dv(a.d, b)); }
+HD Dual<double> operator/(double a, Dual<double> b) {
+  const double q = dv(a, b.v);
+  return Dual<double>(q, dv(-q * b.d, b.v));
 }
 
 // FP32 duals serve only the FP32 tangent, which keeps the derivative parts;
@@ -245,15 +283,15 @@ struct Phys {
 };
 
 template <class T> HD T pressure(const Phys& P, const S6<T>& U) {
-  T V2 = (U[1] * U[1] + U[2] * U[2]) / (U[0] * U[0]);
+  T V2 = dv(U[1] * U[1] + U[2] * U[2], U[0] * U[0]);
   return (P.gamma - 1.0) * (U[3] - 0.5 * U[0] * V2);
 }
 template <class T> HD S6<T> prim2cons(const Phys& P, T rho, T u, T v, T p) {
-  T rE = p / (P.gamma - 1.0) + 0.5 * rho * (u * u + v * v);
+  T rE = dv(p, P.gamma - 1.0) + 0.5 * rho * (u * u + v * v);
   return S6<T>(rho, rho * u, rho * v, rE, rho * T(0.0), rho * T(0.0));
 }
 template <class T> HD S6<T> prim2cons(const Phys& P, T rho, T u, T v, T p, T k, T w) {
-  T rE = p / (P.gamma - 1.0) + 0.5 * rho * (u * u + v * v);
+  T rE = dv(p, P.gamma - 1.0) + 0.5 * rho * (u * u + v * v);
   return S6<T>(rho, rho * u, rho * v, rE, rho * k, rho * w);
 }
 
@@ -276,9 +314,9 @@ template <class F> HD Dual<F> th(Dual<F> x) {
 template <class F, int K> HD DualV<F, K> th(const DualV<F, K>& x) {
   DualV<F, K> r;
   r.v = tanh(x.v);
-  const F dv = F(1) - r.v * r.v;
+  const F dt = F(1) - r.v * r.v;
 #pragma unroll
-  for (int i = 0; i < K; ++i) r.d[i] = dv * x.d[i];
+  for (int i = 0; i < K; ++i) r.d[i] = dt * x.d[i];
   return r;
 }
 
@@ -292,18 +330,18 @@ template <class T> HD TurbQ<T> turb_cell(const Phys& P, const S6<T>& U, const T*
   using namespace sst;
   const double nu = P.nu;
   const T rho = U[0];
-  const T k = mx(U[4] / rho, T(1e-20)), w = mx(U[5] / rho, T(1e-20));
+  const T k = mx(dv(U[4], rho), T(1e-20)), w = mx(dv(U[5], rho), T(1e-20));
   const T ux = g[2], uy = g[3], vx = g[4], vy = g[5];
   const T S2 = 2.0 * ux * ux + 2.0 * vy * vy + (uy + vx) * (uy + vx);
   const T S = sq_rt(S2 + 1e-20);
   const T gkw = g[10] * g[12] + g[11] * g[13];
-  const T CD = mx(2.0 * rho * sw2 * gkw / w, T(1e-10));
+  const T CD = mx(dv(2.0 * rho * sw2 * gkw, w), T(1e-10));
   const T sk = sq_rt(k);
-  const T arg1 = mn(mx(sk / (bs * w * d), 500.0 * nu / (d * d * w)), 4.0 * rho * sw2 * k / (CD * d * d));
+  const T arg1 = mn(mx(dv(sk, bs * w * d), dv(500.0 * nu, d * d * w)), dv(4.0 * rho * sw2 * k, CD * d * d));
   const T F1 = th(arg1 * arg1 * arg1 * arg1);
-  const T arg2 = mx(2.0 * sk / (bs * w * d), 500.0 * nu / (d * d * w));
+  const T arg2 = mx(dv(2.0 * sk, bs * w * d), dv(500.0 * nu, d * d * w));
   const T F2 = th(arg2 * arg2);
-  const T mut = rho * a1 * k / mx(a1 * w, S * F2);
+  const T mut = dv(rho * a1 * k, mx(a1 * w, S * F2));
   const T Pk = mn(mut * S2, 10.0 * bs * rho * w * k);
   const T beta = F1 * b1 + (1.0 - F1) * b2;
   const T gam = F1 * g1() + (1.0 - F1) * g2();
@@ -311,7 +349,7 @@ template <class T> HD TurbQ<T> turb_cell(const Phys& P, const S6<T>& U, const T*
   q.mut = mut;
   q.F1 = F1;
   q.Sk = Pk - bs * rho * w * k;
-  q.Sw = gam * rho * S2 - beta * rho * w * w + 2.0 * (1.0 - F1) * rho * sw2 * gkw / w;
+  q.Sw = gam * rho * S2 - beta * rho * w * w + dv(2.0 * (1.0 - F1) * rho * sw2 * gkw, w);
   return q;
 }
 // omega at a no-slip wall: 10 * 6 nu / (beta1 dn^2), dn the owner centroid's normal distance
@@ -320,7 +358,7 @@ template <class T> HD S6<T> frozen(const double* s) {
   return S6<T>(T(s[0]), T(s[1]), T(s[2]), T(s[3]), T(s[4]), T(s[5]));
 }
 template <class T> HD S6<T> slip_ghost(const S6<T>& U, double nx, double ny) {
-  T u = U[1] / U[0], v = U[2] / U[0];
+  T u = dv(U[1], U[0]), v = dv(U[2], U[0]);
   T vn = u * nx + v * ny;
   T ug = u - 2.0 * vn * nx, vg = v - 2.0 * vn * ny;
   S6<T> G = U;
@@ -335,14 +373,14 @@ template <class T> HD S6<T> ghost(const Phys& P, const S6<T>& U, int tag, double
     if (tag == 1) {
       T pin = pressure(P, U);
       T puse = mn(pin, T(P.P0 - 1e-4));
-      T Tg = P.T0 * pw(puse / P.P0, (P.gamma - 1.0) / P.gamma);
-      T rg = puse / (P.R * Tg);
+      T Tg = P.T0 * pw(dv(puse, P.P0), dv(P.gamma - 1.0, P.gamma));
+      T rg = dv(puse, P.R * Tg);
       T V2 = 2.0 * P.Cp * (P.T0 - Tg);
       T V = val(V2) > 0 ? sq_rt(V2) : T(0.0);
       return prim2cons(P, rg, V, T(0.0), puse);
     }
     if (tag == 2) {
-      T rho = U[0], u = U[1] / U[0], v = U[2] / U[0];
+      T rho = U[0], u = dv(U[1], U[0]), v = dv(U[2], U[0]);
       return prim2cons(P, rho, u, v, T(P.pexit));
     }
     if (tag == 3) return slip_ghost(U, nx, ny);
@@ -357,14 +395,14 @@ template <class T> HD S6<T> ghost(const Phys& P, const S6<T>& U, int tag, double
   if (tag == 1 || tag == 5) return frozen<T>(P.fs);
   if (tag == 3) return slip_ghost(U, nx, ny);
   if (tag == 4) {
-    T ug = -(U[1] / U[0]), vg = -(U[2] / U[0]);
-    T ek = 0.5 * (U[1] * U[1] + U[2] * U[2]) / U[0];
+    T ug = -dv(U[1], U[0]), vg = -dv(U[2], U[0]);
+    T ek = dv(0.5 * (U[1] * U[1] + U[2] * U[2]), U[0]);
     T pin = (P.gamma - 1.0) * (U[3] - ek);
-    T Tin = pin / (U[0] * P.R);
-    T rg = pin / (P.R * Tin);
+    T Tin = dv(pin, U[0] * P.R);
+    T rg = dv(pin, P.R * Tin);
     if (P.turb) {  // k = 0 and omega = omega_wall on the wall face
-      T kg = -(U[4] / U[0]);
-      T wg = 2.0 * omega_wall(P, dn) - U[5] / U[0];
+      T kg = -dv(U[4], U[0]);
+      T wg = 2.0 * omega_wall(P, dn) - dv(U[5], U[0]);
       return prim2cons(P, rg, ug, vg, pin, kg, wg);
     }
     return prim2cons(P, rg, ug, vg, pin);
@@ -375,7 +413,7 @@ template <class T> HD S6<T> ghost(const Phys& P, const S6<T>& U, int tag, double
 
 template <class T> HD S6<T> normal_flux(const Phys& P, const S6<T>& U, double nx, double ny) {
   T p = pressure(P, U);
-  T u = U[1] / U[0], v = U[2] / U[0];
+  T u = dv(U[1], U[0]), v = dv(U[2], U[0]);
   T un = u * nx + v * ny;
   return S6<T>(U[0] * un, U[1] * un + p * nx, U[2] * un + p * ny, (U[3] + p) * un, T(0.0), T(0.0));
 }
@@ -383,20 +421,20 @@ template <class T> HD S6<T> normal_flux(const Phys& P, const S6<T>& U, double nx
 template <class T> HD S6<T> flux_hllc(const Phys& P, const S6<T>& L, const S6<T>& R, double nx, double ny) {
   const double g = P.gamma;
   T pL = pressure(P, L), pR = pressure(P, R);
-  T uL = L[1] / L[0], uR = R[1] / R[0];
-  T vL = L[2] / L[0], vR = R[2] / R[0];
+  T uL = dv(L[1], L[0]), uR = dv(R[1], R[0]);
+  T vL = dv(L[2], L[0]), vR = dv(R[2], R[0]);
   T unL = uL * nx + vL * ny, unR = uR * nx + vR * ny;
-  T cL = sq_rt(g * pL / L[0]), cR = sq_rt(g * pR / R[0]);
-  T hL = (L[3] + pL) / L[0], hR = (R[3] + pR) / R[0];
+  T cL = sq_rt(dv(g * pL, L[0])), cR = sq_rt(dv(g * pR, R[0]));
+  T hL = dv(L[3] + pL, L[0]), hR = dv(R[3] + pR, R[0]);
   T sL = sq_rt(L[0]), sR = sq_rt(R[0]);
-  T ut = (sL * uL + sR * uR) / (sL + sR);
-  T vt = (sL * vL + sR * vR) / (sL + sR);
-  T ht = (sL * hL + sR * hR) / (sL + sR);
+  T ut = dv(sL * uL + sR * uR, sL + sR);
+  T vt = dv(sL * vL + sR * vR, sL + sR);
+  T ht = dv(sL * hL + sR * hR, sL + sR);
   T unt = ut * nx + vt * ny;
   T ct = sq_rt((g - 1.0) * (ht - 0.5 * (ut * ut + vt * vt)));
   T SL = mn(unL - cL, unt - ct);
   T SR = mx(unR + cR, unt + ct);
-  T Ss = (pR - pL + L[0] * unL * (SL - unL) - R[0] * unR * (SR - unR)) / (L[0] * (SL - unL) - R[0] * (SR - unR));
+  T Ss = dv(pR - pL + L[0] * unL * (SL - unL) - R[0] * unR * (SR - unR), L[0] * (SL - unL) - R[0] * (SR - unR));
   if (val(SL) >= 0.0) return normal_flux(P, L, nx, ny);
   if (val(SR) <= 0.0) return normal_flux(P, R, nx, ny);
   // star region: upwind side K with wave speed S
@@ -404,9 +442,9 @@ template <class T> HD S6<T> flux_hllc(const Phys& P, const S6<T>& L, const S6<T>
   const S6<T>& K = left ? L : R;
   T S = left ? SL : SR;
   T pK = left ? pL : pR, uK = left ? uL : uR, vK = left ? vL : vR, unK = left ? unL : unR;
-  T coef = K[0] * (S - unK) / (S - Ss);
+  T coef = dv(K[0] * (S - unK), S - Ss);
   S6<T> Us(coef, coef * (uK + (Ss - unK) * nx), coef * (vK + (Ss - unK) * ny),
-           coef * (K[3] / K[0] + (Ss - unK) * (Ss + pK / (K[0] * (S - unK)))), T(0.0), T(0.0));
+           coef * (dv(K[3], K[0]) + (Ss - unK) * (Ss + dv(pK, K[0] * (S - unK)))), T(0.0), T(0.0));
   return add(normal_flux(P, K, nx, ny), scale(sub(Us, K), S));
 }
 
@@ -414,13 +452,13 @@ template <class T> HD S6<T> flux_llf(const Phys& P, const S6<T>& L, const S6<T>&
   const double g = P.gamma;
   S6<T> FL = normal_flux(P, L, nx, ny), FR = normal_flux(P, R, nx, ny);
   T pL = pressure(P, L);
-  T uL = L[1] / L[0], vL = L[2] / L[0];
+  T uL = dv(L[1], L[0]), vL = dv(L[2], L[0]);
   T unL = uL * nx + vL * ny;
-  T cL = sq_rt(g * pL / L[0]);
+  T cL = sq_rt(dv(g * pL, L[0]));
   T pR = pressure(P, R);
-  T uR = R[1] / R[0], vR = R[2] / R[0];
+  T uR = dv(R[1], R[0]), vR = dv(R[2], R[0]);
   T unR = uR * nx + vR * ny;
-  T cR = sq_rt(g * pR / R[0]);
+  T cR = sq_rt(dv(g * pR, R[0]));
   T a = mx(ab(unL) + cL, ab(unR) + cR);
   S6<T> D = scale(sub(R, L), T(0.5 * a));
   S6<T> A = scale(add(FL, FR), T(0.5));
@@ -430,13 +468,13 @@ template <class T> HD S6<T> flux_llf(const Phys& P, const S6<T>& L, const S6<T>&
 template <class T> HD S6<T> flux_roe(const Phys& P, const S6<T>& L, const S6<T>& R, double nx, double ny) {
   const double g = P.gamma;
   T pL = pressure(P, L);
-  T uL = L[1] / L[0], vL = L[2] / L[0];
-  T hL = (L[3] + pL) / L[0];
+  T uL = dv(L[1], L[0]), vL = dv(L[2], L[0]);
+  T hL = dv(L[3] + pL, L[0]);
   T pR = pressure(P, R);
-  T uR = R[1] / R[0], vR = R[2] / R[0];
-  T hR = (R[3] + pR) / R[0];
+  T uR = dv(R[1], R[0]), vR = dv(R[2], R[0]);
+  T hR = dv(R[3] + pR, R[0]);
   T sL = sq_rt(L[0]), sR = sq_rt(R[0]);
-  T inv = 1.0 / (sL + sR);
+  T inv = dv(1.0, sL + sR);
   T ut = (sL * uL + sR * uR) * inv;
   T vt = (sL * vL + sR * vR) * inv;
   T ht = (sL * hL + sR * hR) * inv;
@@ -449,25 +487,25 @@ template <class T> HD S6<T> flux_roe(const Phys& P, const S6<T>& L, const S6<T>&
   T eps = 0.1 * ct;
 #pragma unroll
   for (int i = 0; i < 4; ++i) {
-    if (val(ab(lam[i])) < val(eps)) lam[i] = 0.5 * (lam[i] * lam[i] / eps + eps);
+    if (val(ab(lam[i])) < val(eps)) lam[i] = 0.5 * (dv(lam[i] * lam[i], eps) + eps);
     lam[i] = ab(lam[i]);
   }
-  T drho = R[0] - L[0], du = uR - uL, dv = vR - vL, dp = pR - pL;
-  T dun = du * nx + dv * ny;
-  T dut = -du * ny + dv * nx;
+  T drho = R[0] - L[0], du = uR - uL, dvl = vR - vL, dp = pR - pL;
+  T dun = du * nx + dvl * ny;
+  T dut = -du * ny + dvl * nx;
   T rt = sL * sR;
-  T a0 = 0.5 / c2 * (dp - rt * ct * dun);
-  T a1 = drho - dp / c2;
+  T a0 = dv(0.5, c2) * (dp - rt * ct * dun);
+  T a1 = drho - dv(dp, c2);
   T a2 = rt * dut;
-  T a3 = 0.5 / c2 * (dp + rt * ct * dun);
+  T a3 = dv(0.5, c2) * (dp + rt * ct * dun);
   T w1 = lam[0] * a0, w2 = lam[1] * a1, w3 = lam[2] * a2, w4 = lam[3] * a3;
   S6<T> D;
   D[0] = w1 * 1.0 + w2 * 1.0 + w3 * 0.0 + w4 * 1.0;
   D[1] = w1 * (ut - ct * nx) + w2 * ut + w3 * (-ny) + w4 * (ut + ct * nx);
   D[2] = w1 * (vt - ct * ny) + w2 * vt + w3 * nx + w4 * (vt + ct * ny);
   D[3] = w1 * (ht - unt * ct) + w2 * (0.5 * V2) + w3 * utt + w4 * (ht + unt * ct);
-  D[4] = ab(unt) * (R[4] / R[0] - L[4] / L[0]) * rt;
-  D[5] = ab(unt) * (R[5] / R[0] - L[5] / L[0]) * rt;
+  D[4] = ab(unt) * (dv(R[4], R[0]) - dv(L[4], L[0])) * rt;
+  D[5] = ab(unt) * (dv(R[5], R[0]) - dv(L[5], L[0])) * rt;
   S6<T> FL = normal_flux(P, L, nx, ny), FR = normal_flux(P, R, nx, ny);
   return sub(scale(add(FL, FR), T(0.5)), scale(D, T(0.5)));
 }
