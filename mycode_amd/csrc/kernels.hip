@@ -199,9 +199,13 @@ struct FaceGeo {
   bool boundary;
 };
 
+// Every load is unconditional apart from kernel-uniform switches, so the
+// loads of one face go out together: a boundary face (n < 0) reads its owner
+// in place of the missing neighbour and the value is not used.
 __device__ __forceinline__ FaceGeo face_geo(const DMesh& m, const Phys& P, const double* __restrict__ phi, int f,
                                             int o, int n, bool need_cells) {
   FaceGeo g;
+  const int nn = n >= 0 ? n : o;
   g.nx = m.nx[f];
   g.ny = m.ny[f];
   g.A = m.area[f];
@@ -213,7 +217,8 @@ __device__ __forceinline__ FaceGeo face_geo(const DMesh& m, const Phys& P, const
   g.wdo = g.wdn = g.dnw = 1.0;
   if (P.turb) {
     g.wdo = m.wd[o];
-    g.wdn = n >= 0 ? m.wd[n] : g.wdo;
+    const double wn = m.wd[nn];
+    g.wdn = n >= 0 ? wn : g.wdo;
     g.dnw = m.dnw[f];
   }
   if (need_cells) {
@@ -221,13 +226,15 @@ __device__ __forceinline__ FaceGeo face_geo(const DMesh& m, const Phys& P, const
     g.ym = m.ym[f];
     g.xco = m.xc[o];
     g.yco = m.yc[o];
+    const double xn = m.xc[nn], yn = m.yc[nn];
     if (n >= 0) {
-      g.xcn = m.xc[n];
-      g.ycn = m.yc[n];
+      g.xcn = xn;
+      g.ycn = yn;
     }
     if (P.spatial == CFD_SECOND_ORDER_LIMITED) {
       g.phL = phi[o];
-      g.phR = n >= 0 ? phi[n] : 0.5;
+      const double pn = phi[nn];
+      g.phR = n >= 0 ? pn : 0.5;
     }
   }
   return g;
@@ -401,34 +408,39 @@ __device__ __forceinline__ S6<T> face_flux(const DMesh& m, const Phys& P, const 
   const int p = m.pitch;
   o = m.own[f];
   n = m.nei[f];
-  const bool second = P.spatial != CFD_FIRST_ORDER && n >= 0;
-  const FaceGeo g = face_geo(m, P, phi, f, o, n, second || VISC);
+  // All of the face's cell data is loaded before any of it is used (a
+  // boundary face reads its owner twice), so the loads overlap: the face's
+  // cells, then everything indexed by them, in two dependent rounds.
+  const int nn = n >= 0 ? n : o;
+  const bool useg = VISC || P.spatial != CFD_FIRST_ORDER;
+  const FaceGeo g = face_geo(m, P, phi, f, o, n, useg);
   constexpr int NG = NV == 6 ? 14 : 10;
   const int ng = (NV == 6 && P.turb) ? 14 : 10;
   const S6<T> L = ldT<NV, T>(U, v, p, o);
-  const S6<T> R = n >= 0 ? ldT<NV, T>(U, v, p, n) : ghost(P, L, g.tg, g.nx, g.ny, g.dnw);
+  const S6<T> Rn = ldT<NV, T>(U, v, p, nn);
   T gL[NG], gR[NG];
-  if (second || VISC) {
+  if (useg) {
 #pragma unroll
-    for (int q = 0; q < NG; ++q) gL[q] = q < ng ? Sc::get(Gv, Gd, (size_t)q * p + o) : T(0.0);
-    if (n >= 0) {
-#pragma unroll
-      for (int q = 0; q < NG; ++q) gR[q] = q < ng ? Sc::get(Gv, Gd, (size_t)q * p + n) : T(0.0);
-    } else {
-#pragma unroll
-      for (int q = 0; q < NG; ++q) gR[q] = gL[q];
+    for (int q = 0; q < NG; ++q) {
+      gL[q] = q < ng ? Sc::get(Gv, Gd, (size_t)q * p + o) : T(0.0);
+      gR[q] = q < ng ? Sc::get(Gv, Gd, (size_t)q * p + nn) : T(0.0);
     }
   }
-  if (NV == 6 && VISC && P.turb) {
-    TurbQ<T> qo, qn;
+  TurbQ<T> qo, qn;
+  const bool tq = NV == 6 && VISC && P.turb;
+  if (tq) {
     qo.mut = Sc::get(Qv, Qd, (size_t)o);
     qo.F1 = Sc::get(Qv, Qd, (size_t)p + o);
-    if (n >= 0) {
-      qn.mut = Sc::get(Qv, Qd, (size_t)n);
-      qn.F1 = Sc::get(Qv, Qd, (size_t)p + n);
-    } else {
-      qn = qo;
-    }
+    qn.mut = Sc::get(Qv, Qd, (size_t)nn);
+    qn.F1 = Sc::get(Qv, Qd, (size_t)p + nn);
+  }
+  const S6<T> R = n >= 0 ? Rn : ghost(P, L, g.tg, g.nx, g.ny, g.dnw);
+  if (useg && n < 0) {  // boundary faces use the owner's gradients on both sides
+#pragma unroll
+    for (int q = 0; q < NG; ++q) gR[q] = gL[q];
+  }
+  if (tq) {
+    if (n < 0) qn = qo;
     return flux_core<VISC, T>(P, g, L, R, gL, gR, &qo, &qn);
   }
   return flux_core<VISC, T>(P, g, L, R, gL, gR);
