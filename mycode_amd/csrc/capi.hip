@@ -110,6 +110,9 @@ void free_krylov(cfd_ctx* c) {
   (void)hipFree(k.w2);
   (void)hipFree(k.h);
   (void)hipHostFree(k.hpin);
+  (void)hipHostFree(k.hcol);
+  for (hipEvent_t& e : k.col_ev)
+    if (e) (void)hipEventDestroy(e);
   k = Krylov();
 }
 
@@ -129,6 +132,8 @@ int ensure_krylov(cfd_ctx* c, int m, int prec) {
   HIP_TRY(c, hipMalloc((void**)&k.w2, vs * 8));
   HIP_TRY(c, hipMalloc((void**)&k.h, 256 * 8));
   HIP_TRY(c, hipHostMalloc((void**)&k.hpin, 256 * 8, hipHostMallocDefault));
+  HIP_TRY(c, hipHostMalloc((void**)&k.hcol, (size_t)64 * 64 * 8, hipHostMallocDefault));
+  for (hipEvent_t& e : k.col_ev) HIP_TRY(c, hipEventCreateWithFlags(&e, hipEventDisableTiming));
   k.m = m;
   k.prec = prec;
   return CFD_OK;
@@ -988,9 +993,9 @@ static int gmres_cycle(cfd_ctx* c, const double* U, const double* Rb, const doub
   std::vector<std::vector<double>> H(m + 1, std::vector<double>(m, 0.0));
   std::vector<double> sn(m, 0.0), cs(m, 0.0), g(m + 1, 0.0);
   g[0] = beta;
-  int kk = 0;
-  for (int j = 0; j < m; ++j) {
-    kk = j;
+  // Arnoldi step j: preconditioner, product, orthogonalisation; its
+  // Hessenberg column goes to pinned slot j and event col_ev[j & 1] marks it.
+  auto step = [&](int j) -> int {
     void* Vj = V + (size_t)j * vs * es;
     void* Vj1 = V + (size_t)(j + 1) * vs * es;
     if (bj) launch_block_apply(c, k.blk, Vj, k.z, prec, tr);
@@ -1008,9 +1013,23 @@ static int gmres_cycle(cfd_ctx* c, const double* U, const double* Rb, const doub
     if (o->scaling) launch_rowscale(c, Vj1, sc, prec);
     launch_orth(c, V, j, nullptr, prec, k.h);
     launch_store_basis(c, Vj1, nullptr, k.h, j, prec);
-    HIP_TRY(c, hipMemcpyAsync(k.hpin, k.h, (size_t)(j + 2) * 8, hipMemcpyDeviceToHost, c->stream));
-    HIP_TRY(c, hipStreamSynchronize(c->stream));
-    for (int i = 0; i <= j + 1; ++i) H[i][j] = k.hpin[i];
+    HIP_TRY(c, hipMemcpyAsync(k.hcol + (size_t)64 * j, k.h, (size_t)(j + 2) * 8, hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(c, hipEventRecord(k.col_ev[j & 1], c->stream));
+    return CFD_OK;
+  };
+  // The steps run one ahead of the host: step j + 1 is enqueued before the
+  // host waits for column j, so the device keeps working while the host
+  // rotates column j.  If column j meets the tolerance, step j + 1 was extra
+  // work: it wrote only V_{j+2} and its own slot, which nothing reads.  Step
+  // j's event is waited for before step j + 2 records the same event again.
+  int kk = 0;
+  if ((rc = step(0))) return rc;
+  for (int j = 0; j < m; ++j) {
+    kk = j;
+    if (j + 1 < m && (rc = step(j + 1))) return rc;
+    HIP_TRY(c, hipEventSynchronize(k.col_ev[j & 1]));
+    const double* col = k.hcol + (size_t)64 * j;
+    for (int i = 0; i <= j + 1; ++i) H[i][j] = col[i];
     for (int i = 0; i < j; ++i) {
       const double t = cs[i] * H[i][j] + sn[i] * H[i + 1][j];
       H[i + 1][j] = -sn[i] * H[i][j] + cs[i] * H[i + 1][j];

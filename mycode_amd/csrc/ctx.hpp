@@ -48,6 +48,8 @@ struct Krylov {
   double last_rel = 0.0;       // relative residual reached by the last solve
   double* h = nullptr;         // device copy of a Hessenberg column (+ norm)
   double* hpin = nullptr;      // pinned host mirror of h
+  double* hcol = nullptr;      // pinned host slots of the Hessenberg columns, 64 doubles per Arnoldi step
+  hipEvent_t col_ev[2] = {nullptr, nullptr};  // column j copied to its slot (ping-pong by j parity)
   size_t bytes = 0;
 };
 

@@ -245,6 +245,25 @@ void color_cells(cfd_mesh& m) {
     m.cell_color[i] = c;
     nc = std::max(nc, c + 1);
   }
+  // iterated greedy, as for the line segments below: whole classes in reverse
+  // order, which never adds a color and usually folds the sparse last ones
+  for (int round = 0; round < 16 && nc > 2; ++round) {
+    std::vector<int> ncol(N, -1);
+    int nn = 0;
+    for (int col = nc - 1; col >= 0; --col)
+      for (int i = 0; i < N; ++i) {
+        if (m.cell_color[i] != col) continue;
+        uint32_t used = 0;
+        for (int k = deg[i]; k < deg[i + 1]; ++k)
+          if (ncol[adj[k]] >= 0) used |= 1u << ncol[adj[k]];
+        int c = 0;
+        while (used & (1u << c)) ++c;
+        ncol[i] = c;
+        nn = std::max(nn, c + 1);
+      }
+    m.cell_color.swap(ncol);
+    nc = nn;
+  }
   m.ccell_ptr.assign(nc + 1, 0);
   for (int i = 0; i < N; ++i) m.ccell_ptr[m.cell_color[i] + 1]++;
   for (int c = 0; c < nc; ++c) m.ccell_ptr[c + 1] += m.ccell_ptr[c];
@@ -385,6 +404,35 @@ void build_lines(cfd_mesh& m) {
     while (taken[col]) ++col;
     scol[k] = col;
     nc = std::max(nc, col + 1);
+  }
+  // Iterated greedy: recolor the segments class by class, the classes in
+  // reverse order (the late, sparse ones first), each segment taking the
+  // lowest color free among its neighbours.  Visiting whole classes in turn
+  // never needs more colors than the classes it starts from, and the last few
+  // classes, which hold a handful of segments on these meshes yet cost two
+  // sweep launches each, usually fold into earlier ones.
+  for (int round = 0; round < 16 && nc > 2; ++round) {
+    std::vector<int> rorder;
+    rorder.reserve(ns);
+    for (int col = nc - 1; col >= 0; --col)
+      for (int k : sorder)
+        if (scol[k] == col) rorder.push_back(k);
+    std::vector<int> ncol(ns, -1);
+    int nn = 0;
+    for (int k : rorder) {
+      taken.assign(nn + 1, 0);
+      for (int c : segs[k])
+        for (int q = 0; q < 4; ++q) {
+          const int b = sl_nb[(size_t)4 * c + q];
+          if (b >= 0 && seg_of[b] != k && ncol[seg_of[b]] >= 0) taken[ncol[seg_of[b]]] = 1;
+        }
+      int col = 0;
+      while (taken[col]) ++col;
+      ncol[k] = col;
+      nn = std::max(nn, col + 1);
+    }
+    scol.swap(ncol);
+    nc = nn;
   }
   // color-major segment list; within a color the multi-cell segments come
   // first, longest first, so that the 64-cell groups the device sweeps pack
