@@ -127,6 +127,7 @@ int ensure_krylov(cfd_ctx* c, int m, int prec) {
   HIP_TRY(c, hipMalloc(&k.blk, (size_t)c->P.nvar * c->P.nvar * c->pitch * es));
   HIP_TRY(c, hipMalloc((void**)&k.fjac, (size_t)2 * c->P.nvar * c->P.nvar * ((c->F + 63) / 64 * 64) * 8));
   HIP_TRY(c, hipMalloc((void**)&k.z, vs * 8));
+  HIP_TRY(c, hipMemset(k.z, 0, vs * 8));  // padding entries stay zero (the line sweep writes cells only)
   HIP_TRY(c, hipMalloc((void**)&k.w, vs * 8));
   HIP_TRY(c, hipMalloc((void**)&k.rr, vs * 8));
   HIP_TRY(c, hipMalloc((void**)&k.w2, vs * 8));
@@ -993,8 +994,9 @@ static int gmres_cycle(cfd_ctx* c, const double* U, const double* Rb, const doub
   std::vector<std::vector<double>> H(m + 1, std::vector<double>(m, 0.0));
   std::vector<double> sn(m, 0.0), cs(m, 0.0), g(m + 1, 0.0);
   g[0] = beta;
-  // Arnoldi step j: preconditioner, product, orthogonalisation; its
-  // Hessenberg column goes to pinned slot j and event col_ev[j & 1] marks it.
+  // Arnoldi step j: preconditioner, product, orthogonalisation; the
+  // orthogonalisation writes its Hessenberg column to pinned slot j, and event
+  // col_ev[j & 1] marks the step done.
   auto step = [&](int j) -> int {
     void* Vj = V + (size_t)j * vs * es;
     void* Vj1 = V + (size_t)(j + 1) * vs * es;
@@ -1011,9 +1013,8 @@ static int gmres_cycle(cfd_ctx* c, const double* U, const double* Rb, const doub
       launch_vjp(c, U, (const double*)k.z, (double*)Vj1);
     }
     if (o->scaling) launch_rowscale(c, Vj1, sc, prec);
-    launch_orth(c, V, j, nullptr, prec, k.h);
+    launch_orth(c, V, j, k.hcol + (size_t)64 * j, prec, k.h);
     launch_store_basis(c, Vj1, nullptr, k.h, j, prec);
-    HIP_TRY(c, hipMemcpyAsync(k.hcol + (size_t)64 * j, k.h, (size_t)(j + 2) * 8, hipMemcpyDeviceToHost, c->stream));
     HIP_TRY(c, hipEventRecord(k.col_ev[j & 1], c->stream));
     return CFD_OK;
   };

@@ -165,7 +165,8 @@ void launch_krylov_init(cfd_ctx* c, void* V0, const double* rhs, double sign, in
 void launch_precond_apply(cfd_ctx* c, void* z, const void* Vj, int prec, int slot_norm);
 void launch_d2f(cfd_ctx* c, const double* a, float* b, size_t n);
 void launch_f2d(cfd_ctx* c, const float* a, double* b, size_t n);
-void launch_orth(cfd_ctx* c, void* V, int j, double* w, int prec, double* hdev);
+// hout: pinned host slot that receives the finished column h[0..j+1] (or null)
+void launch_orth(cfd_ctx* c, void* V, int j, double* hout, int prec, double* hdev);
 void launch_store_basis(cfd_ctx* c, void* Vj1, const double* w, const double* hdev, int j, int prec);
 // x = D^-1 sum_{i<=k} y_i V_i (diag) or the plain combination
 void launch_solution(cfd_ctx* c, double* x, const void* V, const double* y, int k, int prec, bool diag = true);

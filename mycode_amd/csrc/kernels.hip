@@ -2016,11 +2016,21 @@ __global__ void k_reorth_test(const double* __restrict__ h, int n, double* __res
     for (int i = 0; i < n; ++i) h2[i] = 0.0;
 }
 
+// h += h2 (the second pass's projections) and h[n] = |w|; with hout (pinned
+// host memory) the finished column is also written there, so the host reads
+// it after the stream's event without a separate copy
 __global__ void k_hacc(double* __restrict__ h, const double* __restrict__ h2, int n, double* __restrict__ nrm2,
-                       double* __restrict__ nrm_out) {
+                       double* __restrict__ nrm_out, double* __restrict__ hout) {
   const int i = threadIdx.x;
-  if (i < n) h[i] += h2[i];
-  if (i == 0) *nrm_out = sqrt(*nrm2);
+  if (i < n) {
+    h[i] += h2[i];
+    if (hout) hout[i] = h[i];
+  }
+  if (i == 0) {
+    const double r = sqrt(*nrm2);
+    *nrm_out = r;
+    if (hout) hout[n] = r;
+  }
 }
 
 template <int NV, class TV>
@@ -2356,7 +2366,7 @@ void launch_precond_apply(cfd_ctx* c, void* z, const void* Vj, int prec, int slo
 // Classical Gram-Schmidt with one reorthogonalisation pass (CGS2), FP64
 // accumulation.  V_{j+1} is orthogonalised in place; hdev[0..j] receives the
 // Hessenberg column and hdev[j+1] the norm of the result.
-template <class TV> static void orth_t(cfd_ctx* c, TV* V, int j, double* hdev) {
+template <class TV> static void orth_t(cfd_ctx* c, TV* V, int j, double* hdev, double* hout) {
   const int N = c->N, p = c->pitch, nv = c->P.nvar;
   const size_t vs = (size_t)nv * p;
   TV* w = V + (size_t)(j + 1) * vs;
@@ -2381,12 +2391,11 @@ template <class TV> static void orth_t(cfd_ctx* c, TV* V, int j, double* hdev) {
   NV_DISPATCH(nv, hipLaunchKernelGGL((k_maxpy<NVt, TV>), dim3(nblocks(N)), dim3(BS), 0, c->stream, N, p, vs, V, nvec,
                                      w, (const double*)h2, (const double*)flag));
   reduce_t<TV, TV>(c, RED_L2, w, w, 7);
-  hipLaunchKernelGGL(k_hacc, dim3(1), dim3(64), 0, c->stream, hdev, h2, nvec, c->scal + 7, hdev + j + 1);
+  hipLaunchKernelGGL(k_hacc, dim3(1), dim3(64), 0, c->stream, hdev, h2, nvec, c->scal + 7, hdev + j + 1, hout);
 }
-void launch_orth(cfd_ctx* c, void* V, int j, double* w, int prec, double* hdev) {
-  (void)w;
-  if (prec == 32) orth_t<float>(c, (float*)V, j, hdev);
-  else orth_t<double>(c, (double*)V, j, hdev);
+void launch_orth(cfd_ctx* c, void* V, int j, double* hout, int prec, double* hdev) {
+  if (prec == 32) orth_t<float>(c, (float*)V, j, hdev, hout);
+  else orth_t<double>(c, (double*)V, j, hdev, hout);
 }
 
 template <class TV> static void store_t(cfd_ctx* c, TV* Vj1, const double* nrm) {
@@ -2433,9 +2442,10 @@ template <class TB, class TX, class TZ>
 static void line_t(cfd_ctx* c, const TB* Dpinv, const TB* Pm, const TB* G, const TB* Aoff, const TX* x, TZ* z) {
   const int nc = (int)c->segcol_ptr.size() - 1;
   TZ* sL = (TZ*)c->kry.lineS;
+  // z needs no clearing: with a proper segment coloring every z a cell reads
+  // was written earlier in the same application (lower colors going up,
+  // higher colors coming down), and only cells' entries are ever read
   NV_DISPATCH(c->P.nvar, {
-    hipLaunchKernelGGL(k_fill<TZ>, dim3(nblocks((size_t)NVt * c->pitch)), dim3(BS), 0, c->stream, z, TZ(0),
-                       (size_t)NVt * c->pitch);
     for (int s = 0; s < 2 * nc - 1; ++s) {
       const int col = s < nc ? s : 2 * nc - 2 - s;
       const int s0 = c->segcol_ptr[col], s1 = c->segcol_ptr[col + 1];
