@@ -102,6 +102,7 @@ void free_krylov(cfd_ctx* c) {
   (void)hipFree(k.lineG);
   (void)hipFree(k.lineP);
   (void)hipFree(k.lineS);
+  (void)hipFree(k.lineZ);
   (void)hipFree(k.lineW);
   (void)hipFree(k.xt);
   (void)hipFree(k.z);
@@ -162,6 +163,7 @@ int ensure_line(cfd_ctx* c) {
   HIP_TRY(c, hipMalloc(&k.lineG, nv * nv * slots * es));
   HIP_TRY(c, hipMalloc(&k.lineP, nv * nv * slots * es));
   HIP_TRY(c, hipMalloc(&k.lineS, nv * c->pitch * 8));
+  HIP_TRY(c, hipMalloc(&k.lineZ, nv * c->pitch * 8));
   HIP_TRY(c, hipMalloc((void**)&k.lineW, nv * slots * 8));
   return CFD_OK;
 }
@@ -585,11 +587,15 @@ int cfd_ctx_create(const cfd_mesh* m, const cfd_params* p, cfd_ctx** out) {
         for (int k = m->segcol_ptr[col]; k < m->segcol_ptr[col + 1]; ++k)
           for (int t = m->seg_ptr[k]; t < m->seg_ptr[k + 1]; ++t) lcol[m->seg_cells[t]] = col;
       // both indexed by line position t, with the cells' neighbour table
+      // (neighbours given by their line positions)
       std::vector<unsigned char> lmask(P_, 0);
       std::vector<int4> lnq(P_, make_int4(-1, -1, -1, -1));
       for (int t = 0; t < N; ++t) {
         const int i = m->seg_cells[t];
-        lnq[t] = nq[i];
+        for (int q = 0; q < 4; ++q) {  // neighbour entries by line position: (t_o << 2) | back slot
+          const int en = (&nq[i].x)[q];
+          (&lnq[t].x)[q] = en < 0 ? en : (lpos[en >> 2] << 2) | (en & 3);
+        }
         for (int q = 0; q < 4; ++q) {
           const int en = (&nq[i].x)[q];
           if (en < 0 || q == m->lslot_prev[i] || q == m->lslot_next[i]) continue;

@@ -39,7 +39,8 @@ struct Krylov {
   double* bd = nullptr;        // FP64 diagonal blocks in segment order (nv^2 x pitch), line preconditioner
   void* lineG = nullptr;       // line factors G_t = Dp_t^-1 A_{t,next} (nv^2 per bundle slot, basis precision)
   void* lineP = nullptr;       // line factors P_t = Dp_t^-1 A_{t,prev} (nv^2 per bundle slot, basis precision)
-  void* lineS = nullptr;       // forward-pass partial sums of the line sweep (nv x pitch, up to FP64)
+  void* lineS = nullptr;       // forward-pass partial sums of the line sweep by line position (nv x pitch, up to FP64)
+  void* lineZ = nullptr;       // the line sweep's z by line position (nv x pitch, up to FP64)
   double* lineW = nullptr;     // w of the cells of multi-cell segments (nv per bundle slot, FP64)
   double* z = nullptr;         // preconditioned direction (FP64)
   double* w = nullptr;         // accumulated solution of restarted GMRES (FP64)

@@ -1451,6 +1451,9 @@ enum LinePass { LINE_FWD = 0, LINE_TOP = 1, LINE_BWD = 2 };
 
 // w_t = Dp_t^-1 s for the cell c at line position t (pass as above); keep
 // false suppresses the store of s (a padding lane standing in for a cell).
+// x is indexed by cell; the sweep's own vectors, the kept sums sL and the
+// result z, are indexed by line position (so are the neighbour entries of lnq),
+// which keeps their loads and stores coalesced along the segments.
 // The neighbour table lnq and the masks lmask are stored by line position, and
 // FP32 blocks load Dp^-1 first, so that everything but x, s and the
 // neighbours' z is in flight after one dependent load (FP64 blocks load it
@@ -1472,7 +1475,7 @@ __device__ __forceinline__ void line_w(int t, int c, int pass, bool keep, const 
   double s[NV];
   if (pass == LINE_BWD) {
 #pragma unroll
-    for (int i = 0; i < NV; ++i) s[i] = (double)sL[i * p + c];
+    for (int i = 0; i < NV; ++i) s[i] = (double)sL[i * p + t];
   } else {
 #pragma unroll
     for (int i = 0; i < NV; ++i) s[i] = (double)x[i * p + c];
@@ -1493,7 +1496,7 @@ __device__ __forceinline__ void line_w(int t, int c, int pass, bool keep, const 
   }
   if (pass == LINE_FWD && keep) {
 #pragma unroll
-    for (int i = 0; i < NV; ++i) sL[i * p + c] = (TZ)s[i];
+    for (int i = 0; i < NV; ++i) sL[i * p + t] = (TZ)s[i];
   }
 #pragma unroll
   for (int i = 0; i < NV; ++i) {
@@ -1526,7 +1529,7 @@ __global__ __launch_bounds__(BS) void k_line_cells(int t0, int t1, int pass, con
     for (int i = 0; i < NV; ++i) W[(size_t)m * NV + i] = w[i];
   } else {
 #pragma unroll
-    for (int i = 0; i < NV; ++i) z[i * p + c] = (TZ)w[i];
+    for (int i = 0; i < NV; ++i) z[i * p + t] = (TZ)w[i];
   }
 }
 
@@ -1558,9 +1561,8 @@ __global__ __launch_bounds__(64) void k_line_solve(int b0, const int* __restrict
   }
   line_recur<NV, TB, TC>(steps, k, L, t >= 0, pr, gr, y);
   if (t >= 0) {
-    const int c = seg_cells[t];
 #pragma unroll
-    for (int i = 0; i < NV; ++i) z[i * p + c] = (TZ)y[i];
+    for (int i = 0; i < NV; ++i) z[i * p + t] = (TZ)y[i];
   }
 }
 
@@ -1591,7 +1593,7 @@ __global__ __launch_bounds__(64) void k_line_sweep(int b0, int nb, int t0, int t
     double w[NV];
     line_w<NV, TB, TX, TZ>(t, c, pass, true, nq, lmask, p, Dpinv, Aoff, x, sL, z, w);
 #pragma unroll
-    for (int i = 0; i < NV; ++i) z[i * p + c] = (TZ)w[i];
+    for (int i = 0; i < NV; ++i) z[i * p + t] = (TZ)w[i];
     return;
   }
   const size_t bb = (size_t)(b0 + blk) * 64, m = bb + threadIdx.x;
@@ -1618,8 +1620,20 @@ __global__ __launch_bounds__(64) void k_line_sweep(int b0, int nb, int t0, int t
   line_recur<NV, TB, TC>(steps, k, L, t >= 0, pr, gr, y);
   if (t >= 0) {
 #pragma unroll
-    for (int i = 0; i < NV; ++i) z[i * p + c] = (TZ)y[i];
+    for (int i = 0; i < NV; ++i) z[i * p + t] = (TZ)y[i];
   }
+}
+
+// z in cell order from the sweep's result by line position (reads gathered,
+// writes coalesced)
+template <int NV, class TZ>
+__global__ __launch_bounds__(BS) void k_line_gather(int N, int p, const int* __restrict__ lpos,
+                                                    const TZ* __restrict__ zt, TZ* __restrict__ z) {
+  const int c = xcd_block() * BS + threadIdx.x;
+  if (c >= N) return;
+  const int t = lpos[c];
+#pragma unroll
+  for (int i = 0; i < NV; ++i) z[i * p + c] = zt[i * p + t];
 }
 
 // ---- reductions (deterministic: fixed grid, fixed tree) ------------------------------
@@ -2439,9 +2453,10 @@ void launch_line_factor(cfd_ctx* c, const double* Bd, const void* Aoff, void* Dp
   else line_factor_t<double>(c, Bd, (const double*)Aoff, (double*)Dpinv, (double*)Pm, (double*)G);
 }
 template <class TB, class TX, class TZ>
-static void line_t(cfd_ctx* c, const TB* Dpinv, const TB* Pm, const TB* G, const TB* Aoff, const TX* x, TZ* z) {
+static void line_t(cfd_ctx* c, const TB* Dpinv, const TB* Pm, const TB* G, const TB* Aoff, const TX* x, TZ* zc) {
   const int nc = (int)c->segcol_ptr.size() - 1;
   TZ* sL = (TZ*)c->kry.lineS;
+  TZ* z = (TZ*)c->kry.lineZ;  // the sweeps' z by line position; zc (by cell) is formed at the end
   // z needs no clearing: with a proper segment coloring every z a cell reads
   // was written earlier in the same application (lower colors going up,
   // higher colors coming down), and only cells' entries are ever read
@@ -2472,6 +2487,8 @@ static void line_t(cfd_ctx* c, const TB* Dpinv, const TB* Pm, const TB* G, const
                              c->d_lb_kl, c->d_seg_cells, c->pitch, Pm, G, (const double*)c->kry.lineW, z);
       }
     }
+    hipLaunchKernelGGL((k_line_gather<NVt, TZ>), dim3(nblocks(c->N)), dim3(BS), 0, c->stream, c->N, c->pitch,
+                       c->d_lpos, (const TZ*)z, zc);
   });
 }
 void launch_line_apply(cfd_ctx* c, const void* Dpinv, const void* Pm, const void* G, const void* Aoff, const void* x,
