@@ -150,6 +150,10 @@ int cfd_mesh_write_gmsh(const cfd_mesh* m, const char* path);
  * (n_colors + 1) may be NULL to query the sizes. */
 int cfd_mesh_get_lines(const cfd_mesh* m, int* n_segments, int* n_colors, int* seg_ptr, int* seg_cells,
                        int* segcol_ptr);
+/* Cell colors of the multicolor block symmetric Gauss-Seidel preconditioner
+ * (no reference counterpart): a distance-1 coloring, adjacent cells differ.
+ * color (N, reference cell order) may be NULL to query n_colors. */
+int cfd_mesh_get_cell_colors(const cfd_mesh* m, int* n_colors, int* color);
 /* main.cpp saveResults (main.cpp:8-51): legacy ASCII VTK of the mesh with the
  * cell fields Mach, Pressure and Density of U6 (reference cell order),
  * pressure with the given gamma (the reference uses Flux.hpp's 1.4), numbers

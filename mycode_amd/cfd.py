@@ -120,6 +120,7 @@ SIGNATURES = [
     ("cfd_bl_profile", C.c_int, [_VP, _DP, C.c_double, C.c_int, _DP, _IP]),
     ("cfd_mesh_write_vtk", C.c_int, [_VP, _DP, C.c_double, C.c_char_p]),
     ("cfd_mesh_get_lines", C.c_int, [_VP, _IP, _IP, _IP, _IP, _IP]),
+    ("cfd_mesh_get_cell_colors", C.c_int, [_VP, _IP, _IP]),
     ("cfd_line_precond", C.c_int, [_VP, _DP, C.c_int, _DP, _DP]),
     ("cfd_solve_unsteady", C.c_int, [_VP, _DP, C.c_double, C.c_int, C.POINTER(GmresOpts), C.c_int, C.c_double,
                                      C.c_double, _DP, _IP]),
@@ -315,6 +316,13 @@ class Mesh:
         cp = np.zeros(nc.value + 1, np.int32)
         _check(self.lib.cfd_mesh_get_lines(self._h, C.byref(ns), C.byref(nc), _ip(sp), _ip(sc), _ip(cp)))
         return sp, sc, cp
+
+    def cell_colors(self):
+        """(n_colors, color per cell) of the block SGS preconditioner's cell coloring."""
+        nc = C.c_int()
+        color = np.zeros(self.n_cells, np.int32)
+        _check(self.lib.cfd_mesh_get_cell_colors(self._h, C.byref(nc), _ip(color)))
+        return nc.value, color
 
     def write_vtk(self, U, path: str, gamma: float = 1.4):
         """main.cpp saveResults: legacy VTK with Mach, Pressure, Density cell fields."""

@@ -709,6 +709,14 @@ int cfd_mesh_get_lines(const cfd_mesh* m, int* n_segments, int* n_colors, int* s
   return CFD_OK;
 }
 
+int cfd_mesh_get_cell_colors(const cfd_mesh* m, int* n_colors, int* color) {
+  if (!m || !n_colors) return CFD_ERR_ARG;
+  *n_colors = (int)m->ccell_ptr.size() - 1;
+  if (color)
+    for (size_t i = 0; i < m->cell_color.size(); ++i) color[m->iperm[i]] = m->cell_color[i];
+  return CFD_OK;
+}
+
 int cfd_mesh_write_gmsh(const cfd_mesh* m, const char* path) {
   if (!m || !path) return CFD_ERR_ARG;
   FILE* f = std::fopen(path, "w");

@@ -133,3 +133,21 @@ def test_line_segments(config, scale):
     assert np.all(color[so[cross]] != color[sn[cross]])
     if config in (2, 3):
         assert lens.max() >= 10
+
+
+@pytest.mark.parametrize("config,scale", [(1, 0.25), (2, 0.25), (3, 0.1), (4, 0.05)])
+def test_cell_coloring(config, scale):
+    """Cell colors of the block SGS sweeps: no two cells that share a face
+    have one color, every color is used, and the iterated greedy recoloring
+    keeps these meshes at four colors or fewer."""
+    m = cfd.Mesh.generate(config, scale)
+    nc, color = m.cell_colors()
+    f = m.faces()
+    inter = f["neighbor"] >= 0
+    o, n = f["owner"][inter], f["neighbor"][inter]
+    assert np.all(color[o] != color[n])
+    assert color.min() == 0 and color.max() == nc - 1
+    assert np.array_equal(np.unique(color), np.arange(nc))
+    assert nc <= 4
+    sp, cells, cp = m.lines()
+    assert len(cp) - 1 <= 4
