@@ -110,6 +110,8 @@ void free_krylov(cfd_ctx* c) {
   (void)hipFree(k.rr);
   (void)hipFree(k.w2);
   (void)hipFree(k.h);
+  (void)hipFree(k.part);
+  (void)hipFree(k.dc);
   (void)hipHostFree(k.hpin);
   (void)hipHostFree(k.hcol);
   for (hipEvent_t& e : k.col_ev)
@@ -133,6 +135,8 @@ int ensure_krylov(cfd_ctx* c, int m, int prec) {
   HIP_TRY(c, hipMalloc((void**)&k.rr, vs * 8));
   HIP_TRY(c, hipMalloc((void**)&k.w2, vs * 8));
   HIP_TRY(c, hipMalloc((void**)&k.h, 256 * 8));
+  HIP_TRY(c, hipMalloc((void**)&k.part, (size_t)2 * (m + 2) * 1024 * 8));
+  HIP_TRY(c, hipMalloc((void**)&k.dc, (size_t)(64 * 64 + 64 + 192) * 8));
   HIP_TRY(c, hipHostMalloc((void**)&k.hpin, 256 * 8, hipHostMallocDefault));
   HIP_TRY(c, hipHostMalloc((void**)&k.hcol, (size_t)64 * 64 * 8, hipHostMallocDefault));
   for (hipEvent_t& e : k.col_ev) HIP_TRY(c, hipEventCreateWithFlags(&e, hipEventDisableTiming));
@@ -1000,40 +1004,46 @@ static int gmres_cycle(cfd_ctx* c, const double* U, const double* Rb, const doub
   std::vector<std::vector<double>> H(m + 1, std::vector<double>(m, 0.0));
   std::vector<double> sn(m, 0.0), cs(m, 0.0), g(m + 1, 0.0);
   g[0] = beta;
-  // Arnoldi step j: preconditioner, product, orthogonalisation; the
-  // orthogonalisation writes its Hessenberg column to pinned slot j, and event
-  // col_ev[j & 1] marks the step done.
+  // Arnoldi step j (Gram-Schmidt with delayed reorthogonalisation): the
+  // preconditioner and the product act on V[j], the candidate orthogonalised
+  // once; the step then finishes V[j] (the basis vector q_j) and column j-1 of
+  // H, which goes to pinned slot j-1 (event col_ev[(j-1) & 1]), and leaves the
+  // next candidate in V[j+1].  Step m only finishes column m-1.
   auto step = [&](int j) -> int {
-    void* Vj = V + (size_t)j * vs * es;
-    void* Vj1 = V + (size_t)(j + 1) * vs * es;
-    if (bj) launch_block_apply(c, k.blk, Vj, k.z, prec, tr);
-    else if (sgs) launch_sgs_apply(c, k.blk, k.aoff, Vj, prec, k.z, prec, prec, tr);
-    else if (line) launch_line_apply(c, k.blk, k.lineP, k.lineG, k.aoff, Vj, prec, k.z, prec, prec);
-    else launch_precond_apply(c, k.z, Vj, prec, o->jv == 0 ? S_VNORM : -1);
-    if (o->jv == 0) {
-      if (bj || sgs || line) launch_reduce(c, RED_L2, (const double*)k.z, nullptr, S_VNORM);
-      launch_jv_fd(c, U, Rb, (const double*)k.z, (double*)Vj1, S_UNORM, S_VNORM, tmpU, tmpR);
-    } else if (o->jv == 1) {
-      launch_jv_tangent(c, U, k.z, Vj1, prec);
+    double* hslot = j > 0 ? k.hcol + (size_t)64 * (j - 1) : nullptr;
+    if (j < m) {
+      void* Vj = V + (size_t)j * vs * es;
+      void* Vj1 = V + (size_t)(j + 1) * vs * es;
+      if (bj) launch_block_apply(c, k.blk, Vj, k.z, prec, tr);
+      else if (sgs) launch_sgs_apply(c, k.blk, k.aoff, Vj, prec, k.z, prec, prec, tr);
+      else if (line) launch_line_apply(c, k.blk, k.lineP, k.lineG, k.aoff, Vj, prec, k.z, prec, prec);
+      else launch_precond_apply(c, k.z, Vj, prec, o->jv == 0 ? S_VNORM : -1);
+      if (o->jv == 0) {
+        if (bj || sgs || line) launch_reduce(c, RED_L2, (const double*)k.z, nullptr, S_VNORM);
+        launch_jv_fd(c, U, Rb, (const double*)k.z, (double*)Vj1, S_UNORM, S_VNORM, tmpU, tmpR);
+      } else if (o->jv == 1) {
+        launch_jv_tangent(c, U, k.z, Vj1, prec);
+      } else {
+        launch_vjp(c, U, (const double*)k.z, (double*)Vj1);
+      }
+      if (o->scaling) launch_rowscale(c, Vj1, sc, prec);
+      launch_dcgs(c, V, j, false, hslot, prec);
     } else {
-      launch_vjp(c, U, (const double*)k.z, (double*)Vj1);
+      launch_dcgs(c, V, j, true, hslot, prec);
     }
-    if (o->scaling) launch_rowscale(c, Vj1, sc, prec);
-    launch_orth(c, V, j, k.hcol + (size_t)64 * j, prec, k.h);
-    launch_store_basis(c, Vj1, nullptr, k.h, j, prec);
-    HIP_TRY(c, hipEventRecord(k.col_ev[j & 1], c->stream));
+    if (j > 0) HIP_TRY(c, hipEventRecord(k.col_ev[(j - 1) & 1], c->stream));
     return CFD_OK;
   };
-  // The steps run one ahead of the host: step j + 1 is enqueued before the
-  // host waits for column j, so the device keeps working while the host
-  // rotates column j.  If column j meets the tolerance, step j + 1 was extra
-  // work: it wrote only V_{j+2} and its own slot, which nothing reads.  Step
-  // j's event is waited for before step j + 2 records the same event again.
+  // Column col is final after step col + 1.  The device runs up to two steps
+  // ahead of the host's rotations; the steps past the column that meets the
+  // tolerance were extra work that nothing reads.  Column col's event is
+  // waited for before step col + 3 records the same event again.
   int kk = 0;
   if ((rc = step(0))) return rc;
+  if ((rc = step(1))) return rc;
   for (int j = 0; j < m; ++j) {
     kk = j;
-    if (j + 1 < m && (rc = step(j + 1))) return rc;
+    if (j + 2 <= m && (rc = step(j + 2))) return rc;
     HIP_TRY(c, hipEventSynchronize(k.col_ev[j & 1]));
     const double* col = k.hcol + (size_t)64 * j;
     for (int i = 0; i <= j + 1; ++i) H[i][j] = col[i];

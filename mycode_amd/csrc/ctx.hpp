@@ -47,8 +47,10 @@ struct Krylov {
   double* rr = nullptr;        // true residual of a restart cycle (FP64)
   double* w2 = nullptr;        // row-scaled right-hand side (FP64)
   double last_rel = 0.0;       // relative residual reached by the last solve
-  double* h = nullptr;         // device copy of a Hessenberg column (+ norm)
-  double* hpin = nullptr;      // pinned host mirror of h
+  double* h = nullptr;         // device scalars: an Arnoldi step's reduction sums [0, 124), solution weights y [128, 192)
+  double* part = nullptr;      // per-block partial sums of the Arnoldi reductions (2 (m + 2) x 1024)
+  double* dc = nullptr;        // device Hessenberg matrix (64 x 64, column-major), partial column, step coefficients
+  double* hpin = nullptr;      // pinned host staging of the solution weights y
   double* hcol = nullptr;      // pinned host slots of the Hessenberg columns, 64 doubles per Arnoldi step
   hipEvent_t col_ev[2] = {nullptr, nullptr};  // column j copied to its slot (ping-pong by j parity)
   size_t bytes = 0;
@@ -166,9 +168,9 @@ void launch_krylov_init(cfd_ctx* c, void* V0, const double* rhs, double sign, in
 void launch_precond_apply(cfd_ctx* c, void* z, const void* Vj, int prec, int slot_norm);
 void launch_d2f(cfd_ctx* c, const double* a, float* b, size_t n);
 void launch_f2d(cfd_ctx* c, const float* a, double* b, size_t n);
-// hout: pinned host slot that receives the finished column h[0..j+1] (or null)
-void launch_orth(cfd_ctx* c, void* V, int j, double* hout, int prec, double* hdev);
-void launch_store_basis(cfd_ctx* c, void* Vj1, const double* w, const double* hdev, int j, int prec);
+// Arnoldi step j with delayed reorthogonalisation (or, with last, only the
+// reduction that finishes column j-1); column j-1 of H goes to hout
+void launch_dcgs(cfd_ctx* c, void* V, int j, bool last, double* hout, int prec);
 // x = D^-1 sum_{i<=k} y_i V_i (diag) or the plain combination
 void launch_solution(cfd_ctx* c, double* x, const void* V, const double* y, int k, int prec, bool diag = true);
 // block-Jacobi: assemble + invert (Binv: nv*nv x pitch of the basis type), apply, solution

@@ -1955,107 +1955,147 @@ __global__ __launch_bounds__(BS) void k_rowscale(int N, int p, TV* __restrict__ 
   for (int k = 0; k < NV; ++k) w[k * p + c] = (TV)((double)w[k * p + c] * s.s[k]);
 }
 
-// partial dots of w against up to 8 basis vectors (chunk = blockIdx.y);
-// indices nvec .. ntot-1 (at most one) dot w with itself.  With `gate`, the
-// launch does nothing unless *gate is nonzero.
+// ---- Arnoldi with delayed reorthogonalisation (DCGS2) ---------------------------------
+// Step j multiplies the candidate u_j = V[j], orthogonalised once against the
+// final basis Q = V[0 .. j-1] but not yet a second time, and then finishes it
+// together with the next candidate, reading Q twice per step (one reduction,
+// one update) where two-pass Gram-Schmidt read it four times:
+//   reduction   s = Q^T u, alpha = u.u, t = Q^T w, beta = u.w  (w = V[j+1] = A M^-1 u)
+//   r = sqrt(alpha - s.s);  column j-1 of H: h_prev + s, H[j][j-1] = r
+//   q_j = (u - Q s) / r,    delta = q_j . w = (beta - s.t) / r
+//   u_{j+1} = (w - Q t - delta q_j) / r   (w projected off [Q, q_j] once, scaled)
+//   next partial column: h_prev[i < j] = (t_i - (H s)_i) / r,
+//                        h_prev[j] = (delta - r s_{j-1}) / r
+// because A M^-1 Q s = Q (H s) + q_j r s_{j-1} from the finished columns.  A
+// last reduction of u_m (no product) finishes column m-1.
+
+// Projections of u and of w on V_0 .. V_{nq-1} and on u itself, 8 vectors per
+// grid row: R[i] = V_i . u (R[nq] = u . u) and, with w, R[nq + 1 + i] =
+// V_i . w (R[2 nq + 1] = u . w); per-block partials into part.
 template <int NV, class TV>
-__global__ __launch_bounds__(BS) void k_mdot(int N, int p, size_t vstride, const TV* __restrict__ V, int nvec,
-                                              int ntot, const TV* __restrict__ w, double* __restrict__ part,
-                                              const double* __restrict__ gate) {
-  if (gate && *gate == 0.0) return;
+__global__ __launch_bounds__(BS) void k_dots2(int N, int p, size_t vstride, const TV* __restrict__ V, int nq,
+                                               const TV* __restrict__ u, const TV* __restrict__ w,
+                                               double* __restrict__ part) {
   __shared__ double sh[BS / 64];
   const int i0 = blockIdx.y * 8;
-  const int cnt = min(8, ntot - i0);
-  double acc[8];
+  const int cnt = min(8, nq + 1 - i0);
+  const bool two = w != nullptr;
+  double au[8], aw[8];
 #pragma unroll
-  for (int i = 0; i < 8; ++i) acc[i] = 0.0;
+  for (int i = 0; i < 8; ++i) au[i] = aw[i] = 0.0;
   for (int c = blockIdx.x * BS + threadIdx.x; c < N; c += gridDim.x * BS) {
-    double wv[NV];
+    double uv[NV], wv[NV];
 #pragma unroll
-    for (int k = 0; k < NV; ++k) wv[k] = (double)w[k * p + c];
+    for (int k = 0; k < NV; ++k) {
+      uv[k] = (double)u[k * p + c];
+      wv[k] = two ? (double)w[k * p + c] : 0.0;
+    }
 #pragma unroll
     for (int i = 0; i < 8; ++i) {
       if (i < cnt) {
-        const TV* Vi = i0 + i < nvec ? V + (size_t)(i0 + i) * vstride : w;
-        double t = 0.0;
+        const TV* Vi = i0 + i < nq ? V + (size_t)(i0 + i) * vstride : u;
+        double tu = 0.0, tw = 0.0;
 #pragma unroll
-        for (int k = 0; k < NV; ++k) t += wv[k] * (double)Vi[k * p + c];
-        acc[i] += t;
+        for (int k = 0; k < NV; ++k) {
+          const double vi = (double)Vi[k * p + c];
+          tu += uv[k] * vi;
+          tw += wv[k] * vi;
+        }
+        au[i] += tu;
+        aw[i] += tw;
       }
     }
   }
   for (int i = 0; i < cnt; ++i) {
-    const double s = block_sum(acc[i], sh);
-    if (threadIdx.x == 0) part[(size_t)(i0 + i) * gridDim.x + blockIdx.x] = s;
+    double sv = block_sum(au[i], sh);
+    if (threadIdx.x == 0) part[(size_t)(i0 + i) * gridDim.x + blockIdx.x] = sv;
+    if (two) {
+      sv = block_sum(aw[i], sh);
+      if (threadIdx.x == 0) part[(size_t)(nq + 1 + i0 + i) * gridDim.x + blockIdx.x] = sv;
+    }
   }
 }
 
-// w -= sum_i h[i] V_i ; hacc[i] += h[i] (second pass of CGS2 accumulates)
+// The step's scalars from the reduction R (one wave; j <= 60): finishes column
+// j-1 of H (device copy Hd, column-major with leading dimension ldh, and hout,
+// pinned host memory), forms the next partial column hp and the update's
+// coefficients coef: s at [0, j), t at [64, 64 + j), delta at 128, 1/r at 129.
+// Step 0 has Q empty and r = 1.  A norm r at or below 1e-12 (breakdown) keeps
+// the vectors unscaled, as the reference normalises only above it
+// (Solver.cpp:1550-1556).
+__global__ __launch_bounds__(64) void k_dcgs_coef(const double* __restrict__ R, int j, int last,
+                                                   double* __restrict__ Hd, int ldh, double* __restrict__ hp,
+                                                   double* __restrict__ coef, double* __restrict__ hout) {
+  const int i = threadIdx.x;
+  const double si = i < j ? R[i] : 0.0;
+  const double ss = __shfl(wave_sum(si * si), 0, 64);
+  const double r = j > 0 ? sqrt(fmax(R[j] - ss, 0.0)) : 1.0;
+  if (j > 0) {
+    if (i < j) {
+      const double hv = hp[i] + si;
+      Hd[(size_t)(j - 1) * ldh + i] = hv;
+      if (hout) hout[i] = hv;
+    }
+    if (i == j) {
+      Hd[(size_t)(j - 1) * ldh + j] = r;
+      if (hout) hout[j] = r;
+    }
+  }
+  if (last) return;
+  __syncthreads();
+  const double ir = r > 1e-12 ? 1.0 / r : 1.0;
+  const double ti = i < j ? R[j + 1 + i] : 0.0;
+  const double st = __shfl(wave_sum(si * ti), 0, 64);
+  const double delta = (R[2 * j + 1] - st) * ir;
+  double gi = 0.0;
+  if (i < j)
+    for (int k = i > 0 ? i - 1 : 0; k < j; ++k) gi += Hd[(size_t)k * ldh + i] * R[k];
+  const double gamma = j > 0 ? r * R[j - 1] : 0.0;
+  if (i < j) {
+    hp[i] = (ti - gi) * ir;
+    coef[i] = si;
+    coef[64 + i] = ti;
+  }
+  if (i == j) hp[j] = (delta - gamma) * ir;
+  if (i == 0) {
+    coef[128] = delta;
+    coef[129] = ir;
+  }
+}
+
+// q_j = (u - Q s) / r into V[j] and u_{j+1} = (w - Q t - delta q_j) / r into
+// V[j+1], reading Q once; u_{j+1} uses q_j as stored.
 template <int NV, class TV>
-__global__ __launch_bounds__(BS) void k_maxpy(int N, int p, size_t vstride, const TV* __restrict__ V, int nvec,
-                                               TV* __restrict__ w, const double* __restrict__ h,
-                                               const double* __restrict__ gate) {
-  if (gate && *gate == 0.0) return;
+__global__ __launch_bounds__(BS) void k_dcgs_update(int N, int p, size_t vstride, TV* __restrict__ V, int j,
+                                                     const double* __restrict__ coef) {
   const int c = xcd_block() * BS + threadIdx.x;
   if (c >= N) return;
-  // all NV rows per basis vector, and the basis loop unrolled so that several
-  // vectors' loads are in flight at once
-  double s[NV];
+  TV* u = V + (size_t)j * vstride;
+  TV* w = u + vstride;
+  double as[NV], aw[NV];
 #pragma unroll
-  for (int k = 0; k < NV; ++k) s[k] = (double)w[k * p + c];
+  for (int k = 0; k < NV; ++k) {
+    as[k] = (double)u[k * p + c];
+    aw[k] = (double)w[k * p + c];
+  }
 #pragma unroll 4
-  for (int i = 0; i < nvec; ++i) {
-    const double hi = h[i];
+  for (int i = 0; i < j; ++i) {
+    const double si = coef[i], ti = coef[64 + i];
     const TV* Vi = V + (size_t)i * vstride + c;
 #pragma unroll
-    for (int k = 0; k < NV; ++k) s[k] -= hi * (double)Vi[k * p];
+    for (int k = 0; k < NV; ++k) {
+      const double vi = (double)Vi[k * p];
+      as[k] -= si * vi;
+      aw[k] -= ti * vi;
+    }
   }
+  const double delta = coef[128], ir = coef[129];
 #pragma unroll
-  for (int k = 0; k < NV; ++k) w[k * p + c] = (TV)s[k];
-}
-
-// Reorthogonalisation test after the first Gram-Schmidt pass ("twice is
-// enough"): with h the projections and h[n] = |w|^2 before the pass,
-// |w'|^2 = |w|^2 - |h|^2; a second pass runs only if |w'| < 0.7 |w|.
-// Otherwise its projections h2 are zero and *flag = 0 gates its launches off.
-__global__ void k_reorth_test(const double* __restrict__ h, int n, double* __restrict__ h2,
-                              double* __restrict__ flag) {
-  if (threadIdx.x != 0) return;
-  double hn = 0.0;
-  for (int i = 0; i < n; ++i) hn += h[i] * h[i];
-  const double w0 = h[n], w1 = w0 - hn;
-  const bool again = !(w1 >= 0.49 * w0);
-  *flag = again ? 1.0 : 0.0;
-  if (!again)
-    for (int i = 0; i < n; ++i) h2[i] = 0.0;
-}
-
-// h += h2 (the second pass's projections) and h[n] = |w|; with hout (pinned
-// host memory) the finished column is also written there, so the host reads
-// it after the stream's event without a separate copy
-__global__ void k_hacc(double* __restrict__ h, const double* __restrict__ h2, int n, double* __restrict__ nrm2,
-                       double* __restrict__ nrm_out, double* __restrict__ hout) {
-  const int i = threadIdx.x;
-  if (i < n) {
-    h[i] += h2[i];
-    if (hout) hout[i] = h[i];
+  for (int k = 0; k < NV; ++k) {
+    const TV q = (TV)(as[k] * ir);
+    u[k * p + c] = q;
+    w[k * p + c] = (TV)((aw[k] - delta * (double)q) * ir);
   }
-  if (i == 0) {
-    const double r = sqrt(*nrm2);
-    *nrm_out = r;
-    if (hout) hout[n] = r;
-  }
-}
-
-template <int NV, class TV>
-__global__ __launch_bounds__(BS) void k_normalize(int N, int p, TV* __restrict__ w, const double* __restrict__ nrm) {
-  const int c = blockIdx.x * BS + threadIdx.x;
-  if (c >= N) return;
-  const double h = *nrm;
-  if (!(h > 1e-12)) return;
-  const double ih = 1.0 / h;
-#pragma unroll
-  for (int k = 0; k < NV; ++k) w[k * p + c] = (TV)((double)w[k * p + c] * ih);
 }
 
 // x = sum_{i<=k} y_i V_i / D, accumulated in i order (Solver.cpp:1610-1615)
@@ -2377,49 +2417,34 @@ void launch_precond_apply(cfd_ctx* c, void* z, const void* Vj, int prec, int slo
   else precond_t<double, double>(c, (double*)z, (const double*)Vj, slot_norm);
 }
 
-// Classical Gram-Schmidt with one reorthogonalisation pass (CGS2), FP64
-// accumulation.  V_{j+1} is orthogonalised in place; hdev[0..j] receives the
-// Hessenberg column and hdev[j+1] the norm of the result.
-template <class TV> static void orth_t(cfd_ctx* c, TV* V, int j, double* hdev, double* hout) {
+// One DCGS2 step j (or, with last, the reduction that finishes column j-1
+// only); column j-1 of H goes to hout.  Device scalars: R at kry.h[0, 124),
+// H, the partial column and the coefficients in kry.dc.
+template <class TV> static void dcgs_t(cfd_ctx* c, TV* V, int j, bool last, double* hout) {
   const int N = c->N, p = c->pitch, nv = c->P.nvar;
   const size_t vs = (size_t)nv * p;
-  TV* w = V + (size_t)(j + 1) * vs;
-  const int nvec = j + 1;
-  const int nb = std::min(RED_BLOCKS, std::max(1, nblocks(N)));  // nvec * nb partials fit c->red
-  const int nchunk = (nvec + 7) / 8;
-  double* h2 = hdev + 64;
-  double* flag = c->scal + 8;
-  // pass 1 also forms |w|^2 (into hdev[nvec], overwritten by the final norm)
-  const int nchunk1 = (nvec + 1 + 7) / 8;
-  NV_DISPATCH(nv, hipLaunchKernelGGL((k_mdot<NVt, TV>), dim3(nb, nchunk1), dim3(BS), 0, c->stream, N, p, vs, V, nvec,
-                                     nvec + 1, w, c->red, (const double*)nullptr));
-  hipLaunchKernelGGL(k_red_final, dim3(nvec + 1), dim3(BS), 0, c->stream, c->red, nb, nvec + 1, hdev,
-                     (const double*)nullptr);
-  NV_DISPATCH(nv, hipLaunchKernelGGL((k_maxpy<NVt, TV>), dim3(nblocks(N)), dim3(BS), 0, c->stream, N, p, vs, V, nvec,
-                                     w, (const double*)hdev, (const double*)nullptr));
-  hipLaunchKernelGGL(k_reorth_test, dim3(1), dim3(64), 0, c->stream, (const double*)hdev, nvec, h2, flag);
-  // pass 2 only where the first lost too much of w
-  NV_DISPATCH(nv, hipLaunchKernelGGL((k_mdot<NVt, TV>), dim3(nb, nchunk), dim3(BS), 0, c->stream, N, p, vs, V, nvec,
-                                     nvec, w, c->red, (const double*)flag));
-  hipLaunchKernelGGL(k_red_final, dim3(nvec), dim3(BS), 0, c->stream, c->red, nb, nvec, h2, (const double*)flag);
-  NV_DISPATCH(nv, hipLaunchKernelGGL((k_maxpy<NVt, TV>), dim3(nblocks(N)), dim3(BS), 0, c->stream, N, p, vs, V, nvec,
-                                     w, (const double*)h2, (const double*)flag));
-  reduce_t<TV, TV>(c, RED_L2, w, w, 7);
-  hipLaunchKernelGGL(k_hacc, dim3(1), dim3(64), 0, c->stream, hdev, h2, nvec, c->scal + 7, hdev + j + 1, hout);
+  Krylov& k = c->kry;
+  const int nb = std::min(RED_BLOCKS, std::max(1, nblocks(N)));
+  const int nsum = last ? j + 1 : 2 * (j + 1);
+  TV* u = V + (size_t)j * vs;
+  double* Hd = k.dc;
+  double* hp = k.dc + 64 * 64;
+  double* coef = hp + 64;
+  NV_DISPATCH(nv, {
+    hipLaunchKernelGGL((k_dots2<NVt, TV>), dim3(nb, (j + 1 + 7) / 8), dim3(BS), 0, c->stream, N, p, vs, (const TV*)V,
+                       j, (const TV*)u, last ? (const TV*)nullptr : (const TV*)(u + vs), k.part);
+    hipLaunchKernelGGL(k_red_final, dim3(nsum), dim3(BS), 0, c->stream, (const double*)k.part, nb, nsum, k.h,
+                       (const double*)nullptr);
+    hipLaunchKernelGGL(k_dcgs_coef, dim3(1), dim3(64), 0, c->stream, (const double*)k.h, j, last ? 1 : 0, Hd, 64,
+                       hp, coef, hout);
+    if (!last)
+      hipLaunchKernelGGL((k_dcgs_update<NVt, TV>), dim3(nblocks(N)), dim3(BS), 0, c->stream, N, p, vs, V, j,
+                         (const double*)coef);
+  });
 }
-void launch_orth(cfd_ctx* c, void* V, int j, double* hout, int prec, double* hdev) {
-  if (prec == 32) orth_t<float>(c, (float*)V, j, hdev, hout);
-  else orth_t<double>(c, (double*)V, j, hdev, hout);
-}
-
-template <class TV> static void store_t(cfd_ctx* c, TV* Vj1, const double* nrm) {
-  NV_DISPATCH(c->P.nvar, hipLaunchKernelGGL((k_normalize<NVt, TV>), dim3(nblocks(c->N)), dim3(BS), 0, c->stream,
-                                            c->N, c->pitch, Vj1, nrm));
-}
-void launch_store_basis(cfd_ctx* c, void* Vj1, const double* w, const double* hdev, int j, int prec) {
-  (void)w;
-  if (prec == 32) store_t<float>(c, (float*)Vj1, hdev + j + 1);
-  else store_t<double>(c, (double*)Vj1, hdev + j + 1);
+void launch_dcgs(cfd_ctx* c, void* V, int j, bool last, double* hout, int prec) {
+  if (prec == 32) dcgs_t<float>(c, (float*)V, j, last, hout);
+  else dcgs_t<double>(c, (double*)V, j, last, hout);
 }
 
 template <class TV>
