@@ -12,6 +12,8 @@ The JSON line carries:
   roofline           the dominant residual kernel: algorithmic bytes of its own
                      inputs and outputs per residual over its HIP-event time;
   roofline_residual  the whole residual against B_res = N(16 n_v + 24) + 48 F;
+  tangent_products   forward tangent products J v per second on the same
+                     state (FP32 and FP64 directions), beside the residual;
   jfnk               FD Jv + FP64 Krylov against the tangent Jv with FP32
                      matvec, preconditioner and basis (line-implicit block
                      Gauss-Seidel in both): milliseconds per GMRES iteration
@@ -103,6 +105,7 @@ def gmres_iteration_cost(solver, Ud, Rd, nv, pitch, cfl=50.0, m=40, precond=3, l
     for name, jv, prec in legs:
         x = torch.zeros_like(Rd)
         o = solver.gmres_opts(m=m, tol=1e-30, jv=jv, precision=prec, precond=precond, restarts=0)
+        torch.cuda.synchronize()  # torch's kernels and the library's run on different streams
         solver.dev_gmres(Ud.data_ptr(), Rd.data_ptr(), rhs.data_ptr(), x.data_ptr(), o)  # warm-up, allocations
         torch.cuda.synchronize()
         t0 = time.perf_counter()
@@ -118,6 +121,30 @@ def gmres_iteration_cost(solver, Ud, Rd, nv, pitch, cfl=50.0, m=40, precond=3, l
         out["rel_diff_fp32_vs_fp64_solution"] = rel(xs["tangent_fp32"])
         out["rel_diff_fd_vs_tangent_solution"] = rel(xs["fd_fp64"])
     out["setup"] = f"GMRES({m}), line-implicit block Gauss-Seidel, CFL {cfl}, at the benchmark state"
+    return out
+
+
+def tangent_throughput(solver, Ud, nv, pitch, N, reps=20, cfl=50.0):
+    """Forward tangent products J v (the full second-order residual's
+    tangent, as GMRES uses it) per second on the workload's state, FP32 and
+    FP64 directions, against the FP64 residual: wall time of `reps` device
+    products after a warm-up, on the context stream."""
+    import torch
+    solver.dev_local_timestep(Ud.data_ptr(), cfl)
+    out = {}
+    for prec, dt in ((32, torch.float32), (64, torch.float64)):
+        v = torch.randn(nv * pitch, dtype=torch.float64, device=Ud.device, generator=torch.Generator(
+            device=Ud.device).manual_seed(7)).to(dt)
+        Jv = torch.zeros_like(v)
+        torch.cuda.synchronize()  # torch's kernels and the library's run on different streams
+        solver.dev_jv_tangent(Ud.data_ptr(), v.data_ptr(), Jv.data_ptr(), prec)
+        solver.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            solver.dev_jv_tangent(Ud.data_ptr(), v.data_ptr(), Jv.data_ptr(), prec)
+        solver.synchronize()
+        dt_s = (time.perf_counter() - t0) / reps
+        out[f"fp{prec}"] = {"ms_per_product": 1e3 * dt_s, "Mcell_products_per_s": N / dt_s / 1e6}
     return out
 
 
@@ -230,6 +257,7 @@ def main():
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
     R6 = torch.zeros(N * 6, dtype=torch.float64, device=dev)
+    torch.cuda.synchronize()  # the zero fill (torch's stream) must finish before the unpack (context stream)
     solver.dev_unpack(Rd.data_ptr(), R6.data_ptr())
     solver.synchronize()
     R_dev = R6.cpu().numpy().reshape(N, 6)
@@ -271,6 +299,7 @@ def main():
                               "frac": ach_res / HBM_PEAK_GBS, "B_res": B_res, "ms_event": ms[3]},
         "stage_ms": {"gradients": ms[0], "face_flux": ms[1], "assembly": ms[2], "residual": ms[3]},
     }
+    out["tangent_products"] = tangent_throughput(solver, Ud, nv, pitch, N)
     if not args.no_jfnk:
         out["jfnk"] = {"per_iteration_on_this_workload": gmres_iteration_cost(solver, Ud, Rd, nv, pitch),
                        "time_to_1e-10": jfnk_converge()}

@@ -578,6 +578,11 @@ class Solver:
         self._c(self.lib.cfd_dev_jv_fd(self._h, C.c_void_p(U_ptr), C.c_void_p(Rb_ptr), C.c_void_p(v_ptr),
                                        C.c_void_p(Jv_ptr)), "cfd_dev_jv_fd")
 
+    def dev_jv_tangent(self, U_ptr, v_ptr, Jv_ptr, precision: int = 64):
+        """Forward tangent J v on device SoA vectors (v, Jv FP64 or FP32 by precision)."""
+        self._c(self.lib.cfd_dev_jv_tangent(self._h, C.c_void_p(U_ptr), C.c_void_p(v_ptr), C.c_void_p(Jv_ptr),
+                                            int(precision)), "cfd_dev_jv_tangent")
+
     def dev_time_residual(self, U_ptr: int, R_ptr: int, reps: int, update_limiters: bool = False):
         """Average ms per residual of (gradients, face flux, assembly, total),
         timed with HIP events on the context stream, and flux launches per residual."""

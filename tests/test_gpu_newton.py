@@ -66,6 +66,7 @@ def test_dev_gmres_matches_host_gmres():
     torch.cuda.synchronize()
     it = s.dev_gmres(Ud.data_ptr(), Rd.data_ptr(), rhs.data_ptr(), x.data_ptr(), o)
     x6 = torch.zeros(s.N * 6, dtype=torch.float64, device=dev)
+    torch.cuda.synchronize()  # torch's fill runs on its own stream, the library on the context stream
     s.dev_unpack(x.data_ptr(), x6.data_ptr())
     s.synchronize()
     xd = x6.cpu().numpy().reshape(-1, 6)
