@@ -560,7 +560,9 @@ class Solver:
         self._c(self.lib.cfd_wall_outputs(self._h, _dp(U), _dp(Cp), _dp(Cf), C.byref(J)), "cfd_wall_outputs")
         return Cp, Cf, J.value
 
-    # device-handle variants (torch tensors or raw pointers)
+    # device-handle variants (raw device pointers, e.g. torch tensors' data_ptr()).
+    # They run on the context's own stream: torch work that produces or clears
+    # a buffer passed here must have finished (torch.cuda.synchronize()) first.
     def dev_residual(self, U_ptr: int, R_ptr: int, update_limiters: bool = True):
         self._c(self.lib.cfd_dev_residual(self._h, C.c_void_p(U_ptr), C.c_void_p(R_ptr), int(update_limiters)),
                 "cfd_dev_residual")
