@@ -13,7 +13,8 @@ The JSON line carries:
                      inputs and outputs per residual over its HIP-event time;
   roofline_residual  the whole residual against B_res = N(16 n_v + 24) + 48 F;
   tangent_products   forward tangent products J v per second on the same
-                     state (FP32 and FP64 directions), beside the residual;
+                     state (FP32 and FP64 directions) and reverse products
+                     J^T w (discrete adjoint, FP64), beside the residual;
   jfnk               FD Jv + FP64 Krylov against the tangent Jv with FP32
                      matvec, preconditioner and basis (line-implicit block
                      Gauss-Seidel in both): milliseconds per GMRES iteration
@@ -127,11 +128,24 @@ def gmres_iteration_cost(solver, Ud, Rd, nv, pitch, cfl=50.0, m=40, precond=3, l
 def tangent_throughput(solver, Ud, nv, pitch, N, reps=20, cfl=50.0):
     """Forward tangent products J v (the full second-order residual's
     tangent, as GMRES uses it) per second on the workload's state, FP32 and
-    FP64 directions, against the FP64 residual: wall time of `reps` device
-    products after a warm-up, on the context stream."""
+    FP64 directions, and reverse products J^T w (the discrete adjoint, FP64),
+    against the FP64 residual: wall time of `reps` device products after a
+    warm-up, on the context stream."""
     import torch
     solver.dev_local_timestep(Ud.data_ptr(), cfl)
     out = {}
+    w = torch.randn(nv * pitch, dtype=torch.float64, device=Ud.device,
+                    generator=torch.Generator(device=Ud.device).manual_seed(8))
+    JTw = torch.zeros_like(w)
+    torch.cuda.synchronize()
+    solver.dev_vjp_adjoint(Ud.data_ptr(), w.data_ptr(), JTw.data_ptr())
+    solver.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        solver.dev_vjp_adjoint(Ud.data_ptr(), w.data_ptr(), JTw.data_ptr())
+    solver.synchronize()
+    dt_s = (time.perf_counter() - t0) / reps
+    out["adjoint_fp64"] = {"ms_per_product": 1e3 * dt_s, "Mcell_products_per_s": N / dt_s / 1e6}
     for prec, dt in ((32, torch.float32), (64, torch.float64)):
         v = torch.randn(nv * pitch, dtype=torch.float64, device=Ud.device, generator=torch.Generator(
             device=Ud.device).manual_seed(7)).to(dt)

@@ -585,6 +585,11 @@ class Solver:
         self._c(self.lib.cfd_dev_jv_tangent(self._h, C.c_void_p(U_ptr), C.c_void_p(v_ptr), C.c_void_p(Jv_ptr),
                                             int(precision)), "cfd_dev_jv_tangent")
 
+    def dev_vjp_adjoint(self, U_ptr, w_ptr, JTw_ptr):
+        """Reverse product J^T w (with the time term) on device SoA FP64 vectors."""
+        self._c(self.lib.cfd_dev_vjp_adjoint(self._h, C.c_void_p(U_ptr), C.c_void_p(w_ptr), C.c_void_p(JTw_ptr)),
+                "cfd_dev_vjp_adjoint")
+
     def dev_time_residual(self, U_ptr: int, R_ptr: int, reps: int, update_limiters: bool = False):
         """Average ms per residual of (gradients, face flux, assembly, total),
         timed with HIP events on the context stream, and flux launches per residual."""

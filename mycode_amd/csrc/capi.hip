@@ -271,8 +271,19 @@ int layout_faces(cfd_ctx* c) {
   }
   std::vector<int> fpos(F);  // reference face -> device face
   for (int d = 0; d < F; ++d) fpos[c->face_order[d]] = d;
+  // within each color the interior faces come first (the order inside a
+  // color does not change any result), so that kernels for interior faces
+  // only take a color's leading run [color_ptr, color_bptr)
   std::vector<int> corder(F);
-  for (int i = 0; i < F; ++i) corder[i] = fpos[m->face_order[i]];
+  c->color_bptr.assign(c->color_ptr.size() > 0 ? c->color_ptr.size() - 1 : 0, 0);
+  for (size_t col = 0; col + 1 < c->color_ptr.size(); ++col) {
+    int k = c->color_ptr[col];
+    for (int pass = 0; pass < 2; ++pass) {
+      if (pass == 1) c->color_bptr[col] = k;
+      for (int i = c->color_ptr[col]; i < c->color_ptr[col + 1]; ++i)
+        if ((m->nei[m->face_order[i]] >= 0) == (pass == 0)) corder[k++] = fpos[m->face_order[i]];
+    }
+  }
   std::vector<int> own(F), nei(F), tag(F);
   for (int d = 0; d < F; ++d) {
     const int f = c->face_order[d];

@@ -65,6 +65,7 @@ struct cfd_ctx {
   cfd::Phys phys{};
   int N = 0, F = 0, pitch = 0, ncolors = 0;
   std::vector<int> color_ptr;
+  std::vector<int> color_bptr;  // per color: first boundary face in corder (interior faces lead each color)
   std::vector<int> face_order, perm, iperm;  // host copies
   std::vector<int> ccell_ptr;                 // cell color ranges into d_ccells
   cfd::DMesh dm{};

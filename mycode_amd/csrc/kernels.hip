@@ -548,6 +548,7 @@ __global__ __launch_bounds__(BS) void k_tcombine(int N, int p, F* __restrict__ J
 //   k_vjp_gg    the Green-Gauss transpose: aG_c / V_c back onto the face means
 //               and through the primitive (and ghost) maps onto U.
 constexpr int VK = 4;
+constexpr int VJP_KD = 8;  // directions per dual pass of k_vjp_face_inv
 
 template <int NV, int VISC>
 __global__ __launch_bounds__(BS) void k_vjp_face(DMesh m, Phys P, const double* __restrict__ U,
@@ -627,6 +628,113 @@ __global__ __launch_bounds__(BS) void k_vjp_face(DMesh m, Phys P, const double* 
       else dst = aG + (size_t)(idx - nU - ng) * p + n;
       *dst = *dst + s;
     }
+  }
+}
+
+// Interior faces of an inviscid second-order residual with four variables:
+// k_vjp_face's product, using the face's structure.  Such a face's flux F A
+// depends on the two cells' gradients only through the reconstructed face
+// primitives (rho, u, v, p) of each side, linearly (weights phi dx, phi dy,
+// none where a reconstruction clip took the cell value), and on the cell
+// states through their primitives.  So lambda^T d(F A) is formed by forward
+// duals over the 8 face primitives (two passes of VK directions through
+// prim2cons and the Riemann flux) and carried back by hand onto the gradient
+// adjoints and, through the primitive map's Jacobian, onto the states.
+template <int NV, int KD>
+__global__ __launch_bounds__(BS) void k_vjp_face_inv(DMesh m, Phys P, const double* __restrict__ U,
+                                                      const double* __restrict__ G, const double* __restrict__ phi,
+                                                      const double* __restrict__ w, double* __restrict__ aU,
+                                                      double* __restrict__ aG, int f0, int f1) {
+  static_assert(NV == 4 && 8 % KD == 0, "four variables, KD directions per pass");
+  typedef DualV<double, KD> T;
+  const int i = f0 + xcd_block() * BS + threadIdx.x;
+  if (i >= f1) return;
+  const int f = m.corder[i];
+  const int p = m.pitch;
+  const int o = m.own[f], n = m.nei[f];
+  const FaceGeo g = face_geo(m, P, phi, f, o, n, true);
+  const int cell[2] = {o, n};
+  const double dx[2] = {g.xm - g.xco, g.xm - g.xcn}, dy[2] = {g.ym - g.yco, g.ym - g.ycn};
+  const double ph[2] = {g.phL, g.phR};
+  double lam[NV];
+#pragma unroll
+  for (int k = 0; k < NV; ++k) lam[k] = w[k * p + o] - w[k * p + n];
+  // cell primitives and reconstructed face primitives, as flux_core forms them
+  double rho[2], uc[2], vc[2], Wf[2][4];
+  bool clip_r[2], clip_p[2];
+#pragma unroll
+  for (int sd = 0; sd < 2; ++sd) {
+    const int c = cell[sd];
+    const S6<double> Us(U[c], U[p + c], U[2 * p + c], U[3 * p + c]);
+    double gq[8];
+#pragma unroll
+    for (int q = 0; q < 8; ++q) gq[q] = G[(size_t)q * p + c];
+    rho[sd] = Us[0];
+    uc[sd] = dv(Us[1], Us[0]);
+    vc[sd] = dv(Us[2], Us[0]);
+    const double pc = pressure(P, Us);
+    double rf = Us[0] + ph[sd] * (gq[0] * dx[sd] + gq[1] * dy[sd]);
+    double pf = pc + ph[sd] * (gq[6] * dx[sd] + gq[7] * dy[sd]);
+    const double uf = uc[sd] + ph[sd] * (gq[2] * dx[sd] + gq[3] * dy[sd]);
+    const double vf = vc[sd] + ph[sd] * (gq[4] * dx[sd] + gq[5] * dy[sd]);
+    clip_r[sd] = rf < 1e-4;
+    clip_p[sd] = pf < 1e-4;
+    if (clip_r[sd]) rf = Us[0];
+    if (clip_p[sd]) pf = pc;
+    Wf[sd][0] = rf;
+    Wf[sd][1] = uf;
+    Wf[sd][2] = vf;
+    Wf[sd][3] = pf;
+  }
+  // lambda^T d(F A) / d(face primitives), KD directions per pass
+  double aW[2][4];
+#pragma unroll
+  for (int base = 0; base < 8; base += KD) {
+    T q[2][4];
+#pragma unroll
+    for (int s2 = 0; s2 < 2; ++s2)
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        q[s2][j] = T(Wf[s2][j]);
+#pragma unroll
+        for (int d = 0; d < KD; ++d) q[s2][j].d[d] = (4 * s2 + j == base + d) ? 1.0 : 0.0;
+      }
+    const S6<T> Lr = prim2cons(P, q[0][0], q[0][1], q[0][2], q[0][3]);
+    const S6<T> Rr = prim2cons(P, q[1][0], q[1][1], q[1][2], q[1][3]);
+    const S6<T> Fl = riemann(P, Lr, Rr, g.nx, g.ny, g.scheme);
+#pragma unroll
+    for (int d = 0; d < KD; ++d) {
+      double sacc = 0.0;
+#pragma unroll
+      for (int k = 0; k < NV; ++k) sacc += lam[k] * (Fl[k].d[d] * g.A);
+      aW[(base + d) >> 2][(base + d) & 3] = sacc;
+    }
+  }
+  // back through the reconstruction and the primitive map
+  const double gm1 = P.gamma - 1.0;
+#pragma unroll
+  for (int sd = 0; sd < 2; ++sd) {
+    const int c = cell[sd];
+    const double wx = ph[sd] * dx[sd], wy = ph[sd] * dy[sd];
+    const double ar = aW[sd][0], au = aW[sd][1], av = aW[sd][2], ap = aW[sd][3];
+    if (!clip_r[sd]) {
+      aG[c] = aG[c] + ar * wx;
+      aG[(size_t)p + c] = aG[(size_t)p + c] + ar * wy;
+    }
+    aG[(size_t)2 * p + c] = aG[(size_t)2 * p + c] + au * wx;
+    aG[(size_t)3 * p + c] = aG[(size_t)3 * p + c] + au * wy;
+    aG[(size_t)4 * p + c] = aG[(size_t)4 * p + c] + av * wx;
+    aG[(size_t)5 * p + c] = aG[(size_t)5 * p + c] + av * wy;
+    if (!clip_p[sd]) {
+      aG[(size_t)6 * p + c] = aG[(size_t)6 * p + c] + ap * wx;
+      aG[(size_t)7 * p + c] = aG[(size_t)7 * p + c] + ap * wy;
+    }
+    // d(rho, u, v, p)/dU^T: u = m1/rho, v = m2/rho, p = (gamma-1)(E - (m1^2 + m2^2)/(2 rho))
+    const double u = uc[sd], v = vc[sd], ir = 1.0 / rho[sd];
+    aU[c] = aU[c] + (ar - au * u * ir - av * v * ir + ap * gm1 * 0.5 * (u * u + v * v));
+    aU[p + c] = aU[p + c] + (au * ir - ap * gm1 * u);
+    aU[2 * p + c] = aU[2 * p + c] + (av * ir - ap * gm1 * v);
+    aU[3 * p + c] = aU[3 * p + c] + ap * gm1;
   }
 }
 
@@ -2213,11 +2321,19 @@ static void vjp_nv(cfd_ctx* c, const double* U, const double* w, double* JTw) {
                      (const double*)nullptr, c->grad, (double*)nullptr, c->phi, 0);
   launch_fill(c, JTw, 0.0, (size_t)NV * m.pitch);
   launch_fill(c, aG, 0.0, (size_t)14 * m.pitch);
+  // an inviscid second-order residual with four variables takes its interior
+  // faces (each color's leading run) through the structured product; the
+  // boundary faces, and every face of the other residuals, through the
+  // general one
+  const bool inv = NV == 4 && !VISC && P.spatial != CFD_FIRST_ORDER;
   for (int col = 0; col < c->ncolors; ++col) {
-    const int f0 = c->color_ptr[col], f1 = c->color_ptr[col + 1];
-    if (f1 > f0)
-      hipLaunchKernelGGL((k_vjp_face<NV, VISC>), dim3(nblocks(f1 - f0)), dim3(BS), 0, c->stream, m, P, U,
-                         (const double*)c->grad, (const double*)c->phi, w, JTw, aG, f0, f1);
+    const int f0 = c->color_ptr[col], f1 = c->color_ptr[col + 1], fb = inv ? c->color_bptr[col] : f0;
+    if (fb > f0)
+      hipLaunchKernelGGL((k_vjp_face_inv<4, VJP_KD>), dim3(nblocks(fb - f0)), dim3(BS), 0, c->stream, m, P, U,
+                         (const double*)c->grad, (const double*)c->phi, w, JTw, aG, f0, fb);
+    if (f1 > fb)
+      hipLaunchKernelGGL((k_vjp_face<NV, VISC>), dim3(nblocks(f1 - fb)), dim3(BS), 0, c->stream, m, P, U,
+                         (const double*)c->grad, (const double*)c->phi, w, JTw, aG, fb, f1);
   }
   if (NV == 6 && P.turb)
     hipLaunchKernelGGL((k_vjp_src<NV>), dim3(nblocks(m.N)), dim3(BS), 0, c->stream, m, P, U,
