@@ -78,7 +78,9 @@ typedef struct cfd_params {
   int nvar;              /* 4 (Euler / laminar) or 6 (SST k-omega)              */
   int turb;              /* 0 none, 1 SST k-omega (no reference counterpart)    */
   int flux_alt;          /* flux used on faces flagged by cfd_set_face_flux    */
-  int pad0;
+  int turb_first_order;  /* SST only: 0 = k and omega reconstructed (MUSCL) like
+                            the mean flow; 1 = k and omega convected first-order
+                            upwind (the cell values), the mean flow unchanged  */
   /* post-shock state for the SWBLI tag-6 boundary (primitive) */
   double shock_rho, shock_u, shock_v, shock_p;
   /* SST k-omega: freestream specific k and omega of the frozen inflow */

@@ -32,7 +32,7 @@ class Params(C.Structure):
         ("P_total_inlet", C.c_double), ("T_total_inlet", C.c_double),
         ("P_static_exit", C.c_double), ("nu", C.c_double), ("Pr", C.c_double),
         ("flux", C.c_int), ("spatial", C.c_int), ("bcset", C.c_int), ("viscous", C.c_int),
-        ("nvar", C.c_int), ("turb", C.c_int), ("flux_alt", C.c_int), ("pad0", C.c_int),
+        ("nvar", C.c_int), ("turb", C.c_int), ("flux_alt", C.c_int), ("turb_first_order", C.c_int),
         ("shock_rho", C.c_double), ("shock_u", C.c_double), ("shock_v", C.c_double),
         ("shock_p", C.c_double), ("turb_k_inf", C.c_double), ("turb_w_inf", C.c_double),
         ("reserved", C.c_double * 6),

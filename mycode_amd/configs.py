@@ -97,6 +97,11 @@ def params_for(config: int) -> cfd.Params:
         p.bcset, p.flux, p.spatial, p.viscous, p.nvar = cfd.BC_PLATE, cfd.ROE, cfd.SECOND_ORDER_CENTRAL, 1, 6
         a2 = p.gamma * p.R_gas * p.T_total_inlet / (1.0 + 0.5 * (p.gamma - 1.0) * p.M_inf ** 2)
         p.turb, p.turb_k_inf, p.turb_w_inf = 1, 9e-9 * a2, 1e-6 * a2 / p.nu
+        # k and omega convected first-order upwind: the second-order
+        # reconstruction carries the wall omega (~1e8 1/s) into the upwind
+        # faces of the cells just ahead of the leading edge, pins their omega
+        # at its floor and stalls the Newton iteration near a 3e-6 drop
+        p.turb_first_order = 1
     elif config == 4:
         cfd.flow_conditions(p, 0.7, 101325.0, 300.0)
         p.bcset, p.flux, p.spatial, p.viscous, p.nvar = cfd.BC_BUMP, cfd.HLLC, cfd.SECOND_ORDER_CENTRAL, 0, 4

@@ -80,6 +80,7 @@ void fill_phys(cfd_ctx* c) {
   q.viscous = p.viscous;
   q.nvar = p.nvar;
   q.turb = p.turb;
+  q.turb1 = p.turb && p.turb_first_order;
 }
 
 int check_params(const cfd_params* p) {
@@ -88,6 +89,7 @@ int check_params(const cfd_params* p) {
   if (p->flux < 0 || p->flux > 3 || p->flux_alt < 0 || p->flux_alt > 3) return CFD_ERR_ARG;
   if (p->spatial < 0 || p->spatial > 2 || p->bcset < 0 || p->bcset > 3) return CFD_ERR_ARG;
   if (p->turb && (p->nvar != 6 || !p->viscous || !(p->turb_k_inf > 0) || !(p->turb_w_inf > 0))) return CFD_ERR_ARG;
+  if (p->turb_first_order != 0 && p->turb_first_order != 1) return CFD_ERR_ARG;
   return CFD_OK;
 }
 

@@ -269,14 +269,19 @@ __device__ __forceinline__ S6<T> flux_core(const Phys& P, const FaceGeo& g, cons
     if (val(rRf) < 1e-4) rRf = R[0];
     if (val(pRf) < 1e-4) pRf = pR;
     Rr = prim2cons(P, rRf, uRf, vRf, pRf);
-    if (P.turb) {  // MUSCL on specific k and omega, clipped at zero
+    if (P.turb) {  // MUSCL on specific k and omega, clipped at zero; or the cell values
       const T kL = dv(L[4], L[0]), wL = dv(L[5], L[0]), kR = dv(R[4], R[0]), wR = dv(R[5], R[0]);
-      T kLf = kL + phL * (gL[10] * dxL + gL[11] * dyL), wLf = wL + phL * (gL[12] * dxL + gL[13] * dyL);
-      T kRf = kR + phR * (gR[10] * dxR + gR[11] * dyR), wRf = wR + phR * (gR[12] * dxR + gR[13] * dyR);
-      if (val(kLf) < 0.0) kLf = kL;
-      if (val(wLf) < 0.0) wLf = wL;
-      if (val(kRf) < 0.0) kRf = kR;
-      if (val(wRf) < 0.0) wRf = wR;
+      T kLf = kL, wLf = wL, kRf = kR, wRf = wR;
+      if (!P.turb1) {
+        kLf = kL + phL * (gL[10] * dxL + gL[11] * dyL);
+        wLf = wL + phL * (gL[12] * dxL + gL[13] * dyL);
+        kRf = kR + phR * (gR[10] * dxR + gR[11] * dyR);
+        wRf = wR + phR * (gR[12] * dxR + gR[13] * dyR);
+        if (val(kLf) < 0.0) kLf = kL;
+        if (val(wLf) < 0.0) wLf = wL;
+        if (val(kRf) < 0.0) kRf = kR;
+        if (val(wRf) < 0.0) wRf = wR;
+      }
       Lr[4] = rLf * kLf;
       Lr[5] = rLf * wLf;
       Rr[4] = rRf * kRf;

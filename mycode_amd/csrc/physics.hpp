@@ -280,6 +280,7 @@ struct Phys {
   double sh[6];  // frozen post-shock state (SWBLI tag 6)
   double kmin, wmin;  // SST positivity floors of specific k and omega
   int flux, flux_alt, spatial, bcset, viscous, nvar, turb;
+  int turb1;  // SST: first-order upwind convection of k and omega
 };
 
 template <class T> HD T pressure(const Phys& P, const S6<T>& U) {

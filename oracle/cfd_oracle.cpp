@@ -597,12 +597,17 @@ template <class T> struct Disc {
         Rr = p2c(rRf, uRf, vRf, pRf);
         if (P.turb) {  // extension: MUSCL on specific k and omega, clipped like rho and p
           T kL = L[4] / L[0], wL = L[5] / L[0], kR = R[4] / R[0], wR = R[5] / R[0];
-          T kLf = kL + phL * (gL[10] * dxL + gL[11] * dyL), wLf = wL + phL * (gL[12] * dxL + gL[13] * dyL);
-          T kRf = kR + phR * (gR[10] * dxR + gR[11] * dyR), wRf = wR + phR * (gR[12] * dxR + gR[13] * dyR);
-          if (re(kLf) < 0.0) kLf = kL;
-          if (re(wLf) < 0.0) wLf = wL;
-          if (re(kRf) < 0.0) kRf = kR;
-          if (re(wRf) < 0.0) wRf = wR;
+          T kLf = kL, wLf = wL, kRf = kR, wRf = wR;  // turb_first_order: the cell values
+          if (!P.turb_first_order) {
+            kLf = kL + phL * (gL[10] * dxL + gL[11] * dyL);
+            wLf = wL + phL * (gL[12] * dxL + gL[13] * dyL);
+            kRf = kR + phR * (gR[10] * dxR + gR[11] * dyR);
+            wRf = wR + phR * (gR[12] * dxR + gR[13] * dyR);
+            if (re(kLf) < 0.0) kLf = kL;
+            if (re(wLf) < 0.0) wLf = wL;
+            if (re(kRf) < 0.0) kRf = kR;
+            if (re(wRf) < 0.0) wRf = wR;
+          }
           Lr[4] = rLf * kLf;
           Lr[5] = rLf * wLf;
           Rr[4] = rRf * kRf;

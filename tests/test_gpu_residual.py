@@ -63,14 +63,17 @@ def test_gradients_limiter_vs_reference(bump_golden):
     assert np.max(np.abs(s.diagonal_preconditioner(g["U"]) / g["D_cfl5"] - 1)) < 1e-14
 
 
-CASES = [  # (config, scale, viscous override)
-    (1, 0.25, None), (2, 0.1, None), (2, 0.1, 0), (3, 0.03, None), (4, 0.05, None), (5, 0.03, None),
+CASES = [  # (config, scale, viscous override; for SST: k and omega reconstructed, "muscl")
+    (1, 0.25, None), (2, 0.1, None), (2, 0.1, 0), (3, 0.03, None), (3, 0.03, "muscl"), (4, 0.05, None),
+    (5, 0.03, None),
 ]
 
 
 def _case(config, scale, visc, flux=None, spatial=None):
     mesh, p, U = configs.setup(config, scale)
-    if visc is not None:
+    if visc == "muscl":
+        p.turb_first_order = 0
+    elif visc is not None:
         p.viscous = visc
     if flux is not None:
         p.flux = flux
@@ -98,7 +101,7 @@ def _case(config, scale, visc, flux=None, spatial=None):
 @pytest.mark.parametrize("flux", [0, 1, 2])
 def test_residual_configs_vs_oracle(config, scale, visc, flux):
     mesh, p, U, o, alt = _case(config, scale, visc, flux=flux)
-    print(f"config={config} scale={scale} visc={p.viscous} flux={flux} N={mesh.n_cells}", flush=True)
+    print(f"config={config} scale={scale} visc={p.viscous} turb_first_order={p.turb_first_order} flux={flux} N={mesh.n_cells}", flush=True)
     Ro = o.compute_residuals(U)
     S = o.residual_scale(U)
     for assembly in (1, 0):

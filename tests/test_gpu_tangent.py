@@ -19,9 +19,11 @@ pytestmark = pytest.mark.gpu
 CASES = [(1, 0.1), (2, 0.05), (3, 0.02), (4, 0.03), (5, 0.02)]
 
 
-def _setup(config, scale, flux, assembly=0):
+def _setup(config, scale, flux, assembly=0, tfo=None):
     mesh, p, U = configs.setup(config, scale)
     p.flux = flux
+    if tfo is not None:
+        p.turb_first_order = tfo
     o = oracle.OracleSolver(oracle.OracleMesh.from_mesh(mesh), p)
     s = cfd.Solver(mesh, p, assembly=assembly)
     if config == 5:
@@ -54,6 +56,23 @@ def test_tangent_fp64_vs_complex_step(config, scale, flux, assembly):
     s, o, U, v, p = _setup(config, scale, flux, assembly)
     print(f"config={config} flux={flux} assembly={assembly} N={s.N}", flush=True)
     R = s.compute_residuals(U, True)
+    o.compute_residuals(U, True)
+    s.compute_local_time_step(U, 3.0)
+    dt = s.dt_local()
+    vol = s.mesh.cells()["volume"]
+    J = s.jv_tangent(U, v, 64)
+    _, dR = o.residual_complex_step(U, v, 1e-30, update_limiters=False)
+    Jref = dR + v * (vol / (dt + 1e-30))[:, None]
+    assert rel(J, Jref) < 1e-11
+
+
+@pytest.mark.parametrize("flux", [0, 1, 2])
+def test_tangent_sst_reconstructed_kw_vs_complex_step(flux):
+    """The SST plate with k and omega reconstructed like the mean flow
+    (turb_first_order = 0; configuration 3 itself convects them first-order)."""
+    s, o, U, v, p = _setup(3, 0.02, flux, 1, tfo=0)
+    print(f"config=3 turb_first_order=0 flux={flux} N={s.N}", flush=True)
+    s.compute_residuals(U, True)
     o.compute_residuals(U, True)
     s.compute_local_time_step(U, 3.0)
     dt = s.dt_local()
