@@ -28,6 +28,15 @@ __device__ __forceinline__ int xcd_block() {
 }
 
 inline int nblocks(size_t n) { return (int)((n + BS - 1) / BS); }
+inline int nblocks(size_t n, int bs) { return (int)((n + bs - 1) / bs); }
+
+// Block sizes of the residual's register-heavy kernels (2 and 3 waves per
+// SIMD).  Smaller blocks let a CU refill its wave slots one wave at a time;
+// on the 1.32M-cell plate the face flux takes 148 us in 64-thread blocks
+// against 156 us in 256-thread ones, and the gradients 120.6 against 122.7
+// in 128-thread blocks (the gather, at 8 waves per SIMD, gains nothing).
+constexpr int FLUX_BS = 64;
+constexpr int GRAD_BS = 128;
 
 template <int NV, class T>
 __device__ __forceinline__ S6<double> ld(const T* __restrict__ U, int p, int c) {
@@ -79,7 +88,7 @@ __device__ __forceinline__ S6<T> ldT(const double* __restrict__ U, const typenam
 // With SST the cell's eddy viscosity, blending F1 and V * (k, omega sources)
 // are formed here once per cell (rows 0..3 of Qv / Qd) for the face fluxes.
 template <int NV, class T>
-__global__ __launch_bounds__(BS) void k_gradlim(DMesh m, Phys P, const double* __restrict__ U,
+__global__ __launch_bounds__(GRAD_BS) void k_gradlim(DMesh m, Phys P, const double* __restrict__ U,
                                                  const typename Scal<T>::F* __restrict__ v,
                                                  typename Scal<T>::F* __restrict__ Gv,
                                                  typename Scal<T>::F* __restrict__ Gd, double* __restrict__ phi,
@@ -87,7 +96,7 @@ __global__ __launch_bounds__(BS) void k_gradlim(DMesh m, Phys P, const double* _
                                                  typename Scal<T>::F* __restrict__ Qd = nullptr) {
   typedef Scal<T> Sc;
   constexpr int NQM = NV == 6 ? 7 : 5;  // gradient variables: rho u v p T (+ k omega with SST)
-  const int c = xcd_block() * BS + threadIdx.x;
+  const int c = xcd_block() * GRAD_BS + threadIdx.x;
   if (c >= m.N) return;
   const int p = m.pitch;
   const double Rg = P.R;
@@ -460,7 +469,7 @@ template <int NV, class F> constexpr size_t row_align() { return (sizeof(F) * NV
 // MODE 0: scatter into `out` (the residual) for faces [f0, f1) of one color.
 // MODE 1: write the face buffer.  The tangent writes derivative parts only.
 template <int NV, int VISC, int MODE, class T>
-__global__ __launch_bounds__(BS) void k_flux(DMesh m, Phys P, const double* __restrict__ U,
+__global__ __launch_bounds__(FLUX_BS) void k_flux(DMesh m, Phys P, const double* __restrict__ U,
                                               const typename Scal<T>::F* __restrict__ v,
                                               const typename Scal<T>::F* __restrict__ Gv,
                                               const typename Scal<T>::F* __restrict__ Gd,
@@ -468,7 +477,7 @@ __global__ __launch_bounds__(BS) void k_flux(DMesh m, Phys P, const double* __re
                                               int f0, int f1, const typename Scal<T>::F* __restrict__ Qv,
                                               const typename Scal<T>::F* __restrict__ Qd) {
   typedef Scal<T> Sc;
-  const int i = f0 + xcd_block() * BS + threadIdx.x;
+  const int i = f0 + xcd_block() * FLUX_BS + threadIdx.x;
   if (i >= f1) return;
   const int f = MODE == 0 ? m.corder[i] : i;
   int o, n;
@@ -2251,7 +2260,7 @@ static void residual_nv(cfd_ctx* c, const double* U, double* R, bool upd, hipEve
   const bool lim = P.spatial == CFD_SECOND_ORDER_LIMITED;
   const int do_lim = lim && (upd || !c->phi_valid);
   if (ev) (void)hipEventRecord(ev[0], c->stream);
-  hipLaunchKernelGGL((k_gradlim<NV, double>), dim3(nblocks(m.N)), dim3(BS), 0, c->stream, m, P, U,
+  hipLaunchKernelGGL((k_gradlim<NV, double>), dim3(nblocks(m.N, GRAD_BS)), dim3(GRAD_BS), 0, c->stream, m, P, U,
                      (const double*)nullptr, c->grad, (double*)nullptr, c->phi, do_lim, c->tq, (double*)nullptr);
   if (do_lim) c->phi_valid = true;
   if (ev) (void)hipEventRecord(ev[1], c->stream);
@@ -2261,14 +2270,14 @@ static void residual_nv(cfd_ctx* c, const double* U, double* R, bool upd, hipEve
     for (int col = 0; col < c->ncolors; ++col) {
       const int f0 = c->color_ptr[col], f1 = c->color_ptr[col + 1];
       if (f1 > f0)
-        hipLaunchKernelGGL((k_flux<NV, VISC, 0, double>), dim3(nblocks(f1 - f0)), dim3(BS), 0, c->stream, m, P, U,
+        hipLaunchKernelGGL((k_flux<NV, VISC, 0, double>), dim3(nblocks(f1 - f0, FLUX_BS)), dim3(FLUX_BS), 0, c->stream, m, P, U,
                            (const double*)nullptr, c->grad, (const double*)nullptr, c->phi, R, f0, f1,
                            (const double*)c->tq, (const double*)nullptr);
     }
     if (ev) (void)hipEventRecord(ev[3], c->stream);
   } else {
     if (ev) (void)hipEventRecord(ev[2], c->stream);
-    hipLaunchKernelGGL((k_flux<NV, VISC, 1, double>), dim3(nblocks(m.F)), dim3(BS), 0, c->stream, m, P, U,
+    hipLaunchKernelGGL((k_flux<NV, VISC, 1, double>), dim3(nblocks(m.F, FLUX_BS)), dim3(FLUX_BS), 0, c->stream, m, P, U,
                        (const double*)nullptr, c->grad, (const double*)nullptr, c->phi, c->Fm, 0, m.F,
                        (const double*)c->tq, (const double*)nullptr);
     if (ev) (void)hipEventRecord(ev[3], c->stream);
@@ -2292,7 +2301,7 @@ static void tangent_nv(cfd_ctx* c, const double* U, const F* v, F* Jv) {
   F* Gd = Gv + (size_t)14 * m.pitch;
   F* Qv = (F*)c->tq;
   F* Qd = Qv + (size_t)4 * m.pitch;
-  hipLaunchKernelGGL((k_gradlim<NV, T>), dim3(nblocks(m.N)), dim3(BS), 0, c->stream, m, P, U, v, Gv, Gd, c->phi, 0,
+  hipLaunchKernelGGL((k_gradlim<NV, T>), dim3(nblocks(m.N, GRAD_BS)), dim3(GRAD_BS), 0, c->stream, m, P, U, v, Gv, Gd, c->phi, 0,
                      Qv, Qd);
   if (c->assembly == 0) {
     hipLaunchKernelGGL(k_fill<F>, dim3(nblocks((size_t)NV * m.pitch)), dim3(BS), 0, c->stream, Jv, F(0),
@@ -2300,12 +2309,12 @@ static void tangent_nv(cfd_ctx* c, const double* U, const F* v, F* Jv) {
     for (int col = 0; col < c->ncolors; ++col) {
       const int f0 = c->color_ptr[col], f1 = c->color_ptr[col + 1];
       if (f1 > f0)
-        hipLaunchKernelGGL((k_flux<NV, VISC, 0, T>), dim3(nblocks(f1 - f0)), dim3(BS), 0, c->stream, m, P, U, v,
+        hipLaunchKernelGGL((k_flux<NV, VISC, 0, T>), dim3(nblocks(f1 - f0, FLUX_BS)), dim3(FLUX_BS), 0, c->stream, m, P, U, v,
                            (const F*)Gv, (const F*)Gd, c->phi, Jv, f0, f1, (const F*)Qv, (const F*)Qd);
     }
   } else {
     F* Fm = (F*)c->Fm;
-    hipLaunchKernelGGL((k_flux<NV, VISC, 1, T>), dim3(nblocks(m.F)), dim3(BS), 0, c->stream, m, P, U, v,
+    hipLaunchKernelGGL((k_flux<NV, VISC, 1, T>), dim3(nblocks(m.F, FLUX_BS)), dim3(FLUX_BS), 0, c->stream, m, P, U, v,
                        (const F*)Gv, (const F*)Gd, c->phi, Fm, 0, m.F, (const F*)Qv, (const F*)Qd);
     hipLaunchKernelGGL((k_gather<NV, F>), dim3(nblocks(m.N)), dim3(BS), 0, c->stream, m, (const F*)Fm, Jv,
                        NV == 6 && P.turb ? (const F*)Qd : nullptr, v, (const double*)c->dt);
@@ -2322,7 +2331,7 @@ static void vjp_nv(cfd_ctx* c, const double* U, const double* w, double* JTw) {
   const DMesh& m = c->dm;
   const Phys& P = c->phys;
   double* aG = c->adj;
-  hipLaunchKernelGGL((k_gradlim<NV, double>), dim3(nblocks(m.N)), dim3(BS), 0, c->stream, m, P, U,
+  hipLaunchKernelGGL((k_gradlim<NV, double>), dim3(nblocks(m.N, GRAD_BS)), dim3(GRAD_BS), 0, c->stream, m, P, U,
                      (const double*)nullptr, c->grad, (double*)nullptr, c->phi, 0);
   launch_fill(c, JTw, 0.0, (size_t)NV * m.pitch);
   launch_fill(c, aG, 0.0, (size_t)14 * m.pitch);
@@ -2572,7 +2581,7 @@ template <class TV>
 static void block_t(cfd_ctx* c, const double* U, TV* Binv, double* Bout, TV* Aoff, const int* pos) {
   NV_DISPATCH(c->P.nvar, {
     const bool turb = NVt == 6 && c->P.turb;
-    hipLaunchKernelGGL((k_gradlim<NVt, double>), dim3(nblocks(c->N)), dim3(BS), 0, c->stream, c->dm, c->phys, U,
+    hipLaunchKernelGGL((k_gradlim<NVt, double>), dim3(nblocks(c->N, GRAD_BS)), dim3(GRAD_BS), 0, c->stream, c->dm, c->phys, U,
                        (const double*)nullptr, c->grad, (double*)nullptr, c->phi, 0,
                        turb ? c->tq : (double*)nullptr, (double*)nullptr);
     hipLaunchKernelGGL((k_face_jac<NVt, TV>), dim3(nblocks(c->F)), dim3(BS), 0, c->stream, c->dm, c->phys, U,
