@@ -37,6 +37,10 @@ inline int nblocks(size_t n, int bs) { return (int)((n + bs - 1) / bs); }
 // in 128-thread blocks (the gather, at 8 waves per SIMD, gains nothing).
 constexpr int FLUX_BS = 64;
 constexpr int GRAD_BS = 128;
+// The line preconditioner's per-cell pass: a GMRES(40) iteration on that
+// plate, 2.052 / 1.270 ms (FD + FP64 / tangent + FP32) in 64-thread blocks
+// against 2.080 / 1.274 ms in 256-thread ones.
+constexpr int LINE_BS = 64;
 
 template <int NV, class T>
 __device__ __forceinline__ S6<double> ld(const T* __restrict__ U, int p, int c) {
@@ -565,13 +569,13 @@ constexpr int VK = 4;
 constexpr int VJP_KD = 8;  // directions per dual pass of k_vjp_face_inv
 
 template <int NV, int VISC>
-__global__ __launch_bounds__(BS) void k_vjp_face(DMesh m, Phys P, const double* __restrict__ U,
+__global__ __launch_bounds__(FLUX_BS) void k_vjp_face(DMesh m, Phys P, const double* __restrict__ U,
                                                   const double* __restrict__ G, const double* __restrict__ phi,
                                                   const double* __restrict__ w, double* __restrict__ aU,
                                                   double* __restrict__ aG, int f0, int f1) {
   typedef DualV<double, VK> T;
   constexpr int NGM = NV == 6 ? 14 : 10;
-  const int i = f0 + xcd_block() * BS + threadIdx.x;
+  const int i = f0 + xcd_block() * FLUX_BS + threadIdx.x;
   if (i >= f1) return;
   const int f = m.corder[i];
   const int p = m.pitch;
@@ -655,13 +659,13 @@ __global__ __launch_bounds__(BS) void k_vjp_face(DMesh m, Phys P, const double* 
 // prim2cons and the Riemann flux) and carried back by hand onto the gradient
 // adjoints and, through the primitive map's Jacobian, onto the states.
 template <int NV, int KD>
-__global__ __launch_bounds__(BS) void k_vjp_face_inv(DMesh m, Phys P, const double* __restrict__ U,
+__global__ __launch_bounds__(FLUX_BS) void k_vjp_face_inv(DMesh m, Phys P, const double* __restrict__ U,
                                                       const double* __restrict__ G, const double* __restrict__ phi,
                                                       const double* __restrict__ w, double* __restrict__ aU,
                                                       double* __restrict__ aG, int f0, int f1) {
   static_assert(NV == 4 && 8 % KD == 0, "four variables, KD directions per pass");
   typedef DualV<double, KD> T;
-  const int i = f0 + xcd_block() * BS + threadIdx.x;
+  const int i = f0 + xcd_block() * FLUX_BS + threadIdx.x;
   if (i >= f1) return;
   const int f = m.corder[i];
   const int p = m.pitch;
@@ -1634,13 +1638,13 @@ __device__ __forceinline__ void line_w(int t, int c, int pass, bool keep, const 
 // one, whose recurrence k_line_solve then runs.  This bandwidth-bound part runs
 // at the occupancy its own registers allow.
 template <int NV, class TB, class TX, class TZ>
-__global__ __launch_bounds__(BS) void k_line_cells(int t0, int t1, int pass, const int* __restrict__ seg_cells,
+__global__ __launch_bounds__(LINE_BS) void k_line_cells(int t0, int t1, int pass, const int* __restrict__ seg_cells,
                                                    const int4* __restrict__ nq, const int* __restrict__ lcomp,
                                                    const unsigned char* __restrict__ lmask, int p,
                                                    const TB* __restrict__ Dpinv, const TB* __restrict__ Aoff,
                                                    const TX* __restrict__ x, TZ* __restrict__ sL, TZ* __restrict__ z,
                                                    double* __restrict__ W) {
-  const int t = t0 + xcd_block() * BS + threadIdx.x;
+  const int t = t0 + xcd_block() * LINE_BS + threadIdx.x;
   if (t >= t1) return;
   const int c = seg_cells[t];
   const int m = lcomp[t];
@@ -2343,10 +2347,10 @@ static void vjp_nv(cfd_ctx* c, const double* U, const double* w, double* JTw) {
   for (int col = 0; col < c->ncolors; ++col) {
     const int f0 = c->color_ptr[col], f1 = c->color_ptr[col + 1], fb = inv ? c->color_bptr[col] : f0;
     if (fb > f0)
-      hipLaunchKernelGGL((k_vjp_face_inv<4, VJP_KD>), dim3(nblocks(fb - f0)), dim3(BS), 0, c->stream, m, P, U,
+      hipLaunchKernelGGL((k_vjp_face_inv<4, VJP_KD>), dim3(nblocks(fb - f0, FLUX_BS)), dim3(FLUX_BS), 0, c->stream, m, P, U,
                          (const double*)c->grad, (const double*)c->phi, w, JTw, aG, f0, fb);
     if (f1 > fb)
-      hipLaunchKernelGGL((k_vjp_face<NV, VISC>), dim3(nblocks(f1 - fb)), dim3(BS), 0, c->stream, m, P, U,
+      hipLaunchKernelGGL((k_vjp_face<NV, VISC>), dim3(nblocks(f1 - fb, FLUX_BS)), dim3(FLUX_BS), 0, c->stream, m, P, U,
                          (const double*)c->grad, (const double*)c->phi, w, JTw, aG, fb, f1);
   }
   if (NV == 6 && P.turb)
@@ -2634,9 +2638,9 @@ static void line_t(cfd_ctx* c, const TB* Dpinv, const TB* Pm, const TB* G, const
                              pass, c->d_lb_t, c->d_lb_kl, c->d_seg_cells, c->d_lnq, c->d_lmask, c->pitch, Dpinv,
                              Aoff, Pm, G, x, sL, z);
       } else {
-        hipLaunchKernelGGL((k_line_cells<NVt, TB, TX, TZ>), dim3(nblocks(t1 - t0)), dim3(BS), 0, c->stream, t0, t1,
-                           pass, c->d_seg_cells, c->d_lnq, c->d_lcomp, c->d_lmask, c->pitch, Dpinv, Aoff, x, sL, z,
-                           c->kry.lineW);
+        hipLaunchKernelGGL((k_line_cells<NVt, TB, TX, TZ>), dim3(nblocks(t1 - t0, LINE_BS)), dim3(LINE_BS), 0,
+                           c->stream, t0, t1, pass, c->d_seg_cells, c->d_lnq, c->d_lcomp, c->d_lmask, c->pitch, Dpinv,
+                           Aoff, x, sL, z, c->kry.lineW);
         if (nb > 0)
           hipLaunchKernelGGL((k_line_solve<NVt, TB, TZ>), dim3(nb), dim3(64), 0, c->stream, b0, c->d_lb_t,
                              c->d_lb_kl, c->d_seg_cells, c->pitch, Pm, G, (const double*)c->kry.lineW, z);
