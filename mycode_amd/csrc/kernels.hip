@@ -649,16 +649,56 @@ __global__ __launch_bounds__(FLUX_BS) void k_vjp_face(DMesh m, Phys P, const dou
   }
 }
 
-// Interior faces of an inviscid second-order residual with four variables:
-// k_vjp_face's product, using the face's structure.  Such a face's flux F A
+// The viscous terms flux_core subtracts on an interior face without SST,
+// written as a function of the two cell states and the six averaged gradient
+// components (u_x, u_y, v_x, v_y, T_x, T_y), which is how they depend on the
+// gradients there: the same operations as flux_core's viscous block.  The
+// structured adjoint differentiates it in these 14 inputs.
+template <class T>
+__device__ __forceinline__ S6<T> visc_interior(const Phys& P, const FaceGeo& g, const S6<T>& L, const S6<T>& R,
+                                               const T (&gb)[6]) {
+  const double nx = g.nx, ny = g.ny;
+  const T rf = 0.5 * (L[0] + R[0]);
+  const T uf = 0.5 * (dv(L[1], L[0]) + dv(R[1], R[0]));
+  const T vf = 0.5 * (dv(L[2], L[0]) + dv(R[2], R[0]));
+  T ux = gb[0], uy = gb[1], vx = gb[2], vy = gb[3];
+  const T Tx = gb[4], Ty = gb[5];
+  const double dxLR = g.xcn - g.xco, dyLR = g.ycn - g.yco;
+  const double d2 = dxLR * dxLR + dyLR * dyLR;
+  if (d2 > 1e-14) {
+    const T du = dv(R[1], R[0]) - dv(L[1], L[0]);
+    const T dvv = dv(R[2], R[0]) - dv(L[2], L[0]);
+    const T ux0 = ux, uy0 = uy, vx0 = vx, vy0 = vy;
+    ux = ux0 - dv((ux0 * dxLR + uy0 * dyLR - du) * dxLR, d2);
+    uy = uy0 - dv((ux0 * dxLR + uy0 * dyLR - du) * dyLR, d2);
+    vx = vx0 - dv((vx0 * dxLR + vy0 * dyLR - dvv) * dxLR, d2);
+    vy = vy0 - dv((vx0 * dxLR + vy0 * dyLR - dvv) * dyLR, d2);
+  }
+  const T mu = rf * P.nu;
+  const T kc = dv(mu * P.Cp, P.Pr);
+  const T div = ux + vy;
+  const T txx = mu * (2.0 * ux - (2.0 / 3.0) * div);
+  const T tyy = mu * (2.0 * vy - (2.0 / 3.0) * div);
+  const T txy = mu * (uy + vx);
+  const T qx = -kc * Tx, qy = -kc * Ty;
+  const T wx = uf * txx + vf * txy - qx;
+  const T wy = uf * txy + vf * tyy - qy;
+  return S6<T>(T(0.0), txx * nx + txy * ny, txy * nx + tyy * ny, wx * nx + wy * ny);
+}
+
+// Interior faces of a second-order residual with four variables (Euler or
+// laminar): k_vjp_face's product, using the face's structure.  Such a face's flux F A
 // depends on the two cells' gradients only through the reconstructed face
 // primitives (rho, u, v, p) of each side, linearly (weights phi dx, phi dy,
 // none where a reconstruction clip took the cell value), and on the cell
 // states through their primitives.  So lambda^T d(F A) is formed by forward
 // duals over the 8 face primitives (two passes of VK directions through
 // prim2cons and the Riemann flux) and carried back by hand onto the gradient
-// adjoints and, through the primitive map's Jacobian, onto the states.
-template <int NV, int KD>
+// adjoints and, through the primitive map's Jacobian, onto the states.  The
+// viscous terms (VISC) depend on the two states and on the averages of six
+// gradient components: duals over those 14 inputs, the averages' adjoints
+// shared half and half by the two cells.
+template <int NV, int KD, int VISC>
 __global__ __launch_bounds__(FLUX_BS) void k_vjp_face_inv(DMesh m, Phys P, const double* __restrict__ U,
                                                       const double* __restrict__ G, const double* __restrict__ phi,
                                                       const double* __restrict__ w, double* __restrict__ aU,
@@ -678,12 +718,14 @@ __global__ __launch_bounds__(FLUX_BS) void k_vjp_face_inv(DMesh m, Phys P, const
 #pragma unroll
   for (int k = 0; k < NV; ++k) lam[k] = w[k * p + o] - w[k * p + n];
   // cell primitives and reconstructed face primitives, as flux_core forms them
-  double rho[2], uc[2], vc[2], Wf[2][4];
+  double rho[2], uc[2], vc[2], Wf[2][4], Uc[2][4];
   bool clip_r[2], clip_p[2];
 #pragma unroll
   for (int sd = 0; sd < 2; ++sd) {
     const int c = cell[sd];
     const S6<double> Us(U[c], U[p + c], U[2 * p + c], U[3 * p + c]);
+#pragma unroll
+    for (int k = 0; k < 4; ++k) Uc[sd][k] = Us[k];
     double gq[8];
 #pragma unroll
     for (int q = 0; q < 8; ++q) gq[q] = G[(size_t)q * p + c];
@@ -753,6 +795,53 @@ __global__ __launch_bounds__(FLUX_BS) void k_vjp_face_inv(DMesh m, Phys P, const
     aU[p + c] = aU[p + c] + (au * ir - ap * gm1 * u);
     aU[2 * p + c] = aU[2 * p + c] + (av * ir - ap * gm1 * v);
     aU[3 * p + c] = aU[3 * p + c] + ap * gm1;
+  }
+  if constexpr (VISC != 0) {
+    constexpr int KV = 7;  // two passes over (L, R, averaged gradients)
+    typedef DualV<double, KV> TV;
+    const int gq[6] = {2, 3, 4, 5, 8, 9};
+    double gbv[6];
+#pragma unroll
+    for (int q = 0; q < 6; ++q) gbv[q] = 0.5 * (G[(size_t)gq[q] * p + o] + G[(size_t)gq[q] * p + n]);
+    double ain[14];
+#pragma unroll
+    for (int base = 0; base < 14; base += KV) {
+      S6<TV> Ld, Rd;
+      TV gbd[6];
+#pragma unroll
+      for (int k = 0; k < 4; ++k) {
+        Ld[k] = TV(Uc[0][k]);
+        Rd[k] = TV(Uc[1][k]);
+#pragma unroll
+        for (int d = 0; d < KV; ++d) {
+          Ld[k].d[d] = (k == base + d) ? 1.0 : 0.0;
+          Rd[k].d[d] = (4 + k == base + d) ? 1.0 : 0.0;
+        }
+      }
+#pragma unroll
+      for (int q = 0; q < 6; ++q) {
+        gbd[q] = TV(gbv[q]);
+#pragma unroll
+        for (int d = 0; d < KV; ++d) gbd[q].d[d] = (8 + q == base + d) ? 1.0 : 0.0;
+      }
+      const S6<TV> Vv = visc_interior(P, g, Ld, Rd, gbd);
+#pragma unroll
+      for (int d = 0; d < KV; ++d) {
+        double sacc = 0.0;
+#pragma unroll
+        for (int k = 1; k < NV; ++k) sacc += lam[k] * (-(Vv[k].d[d]) * g.A);
+        ain[base + d] = sacc;
+      }
+    }
+#pragma unroll
+    for (int sd = 0; sd < 2; ++sd) {
+      const int c = cell[sd];
+#pragma unroll
+      for (int k = 0; k < 4; ++k) aU[(size_t)k * p + c] = aU[(size_t)k * p + c] + ain[4 * sd + k];
+#pragma unroll
+      for (int q = 0; q < 6; ++q)
+        aG[(size_t)gq[q] * p + c] = aG[(size_t)gq[q] * p + c] + 0.5 * ain[8 + q];
+    }
   }
 }
 
@@ -2339,15 +2428,16 @@ static void vjp_nv(cfd_ctx* c, const double* U, const double* w, double* JTw) {
                      (const double*)nullptr, c->grad, (double*)nullptr, c->phi, 0);
   launch_fill(c, JTw, 0.0, (size_t)NV * m.pitch);
   launch_fill(c, aG, 0.0, (size_t)14 * m.pitch);
-  // an inviscid second-order residual with four variables takes its interior
-  // faces (each color's leading run) through the structured product; the
-  // boundary faces, and every face of the other residuals, through the
-  // general one
-  const bool inv = NV == 4 && !VISC && P.spatial != CFD_FIRST_ORDER;
+  // a second-order residual with four variables (Euler or laminar) takes its
+  // interior faces (each color's leading run) through the structured
+  // product; the boundary faces, and every face of the other residuals,
+  // through the general one
+  const bool inv = NV == 4 && P.spatial != CFD_FIRST_ORDER;
   for (int col = 0; col < c->ncolors; ++col) {
     const int f0 = c->color_ptr[col], f1 = c->color_ptr[col + 1], fb = inv ? c->color_bptr[col] : f0;
     if (fb > f0)
-      hipLaunchKernelGGL((k_vjp_face_inv<4, VJP_KD>), dim3(nblocks(fb - f0, FLUX_BS)), dim3(FLUX_BS), 0, c->stream, m, P, U,
+      hipLaunchKernelGGL((k_vjp_face_inv<4, VJP_KD, VISC>), dim3(nblocks(fb - f0, FLUX_BS)), dim3(FLUX_BS), 0,
+                         c->stream, m, P, U,
                          (const double*)c->grad, (const double*)c->phi, w, JTw, aG, f0, fb);
     if (f1 > fb)
       hipLaunchKernelGGL((k_vjp_face<NV, VISC>), dim3(nblocks(f1 - fb, FLUX_BS)), dim3(FLUX_BS), 0, c->stream, m, P, U,
