@@ -647,7 +647,7 @@ int cfd_ctx_create(const cfd_mesh* m, const cfd_params* p, cfd_ctx** out) {
   HIP_TRY(c, dalloc(&c->red, 65536));
   HIP_TRY(c, dalloc(&c->scal, 64));
   HIP_TRY(c, dalloc(&c->gradT, (size_t)28 * P_));
-  HIP_TRY(c, dalloc(&c->adj, (size_t)14 * P_));
+  HIP_TRY(c, dalloc(&c->adj, (size_t)16 * P_));
   HIP_TRY(c, dalloc(&c->tq, (size_t)8 * P_));
   HIP_TRY(c, hipHostMalloc((void**)&c->scal_host, 64 * 8, hipHostMallocDefault));
   HIP_TRY(c, dalloc(&c->io6, (size_t)6 * N));

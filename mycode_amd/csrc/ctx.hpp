@@ -109,7 +109,7 @@ struct cfd_ctx {
   double* scal = nullptr;   // device scalars (norms, eps, ...)
   double* scal_host = nullptr;  // pinned
   double* gradT = nullptr;  // tangent gradients (2 x 14 x pitch: values, derivatives)
-  double* adj = nullptr;    // adjoint of the gradients (14 x pitch)
+  double* adj = nullptr;    // adjoints of the gradients (14 x pitch), of SST mu_t and F1 (2 x pitch)
   double* tq = nullptr;     // SST per cell: mu_t, F1, V S_k, V S_w (values; + derivatives in the tangent)
   bool phi_valid = false, dt_valid = false;
   // host-pointer staging: AoS device copy + SoA vectors

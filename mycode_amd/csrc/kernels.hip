@@ -649,14 +649,15 @@ __global__ __launch_bounds__(FLUX_BS) void k_vjp_face(DMesh m, Phys P, const dou
   }
 }
 
-// The viscous terms flux_core subtracts on an interior face without SST,
-// written as a function of the two cell states and the six averaged gradient
-// components (u_x, u_y, v_x, v_y, T_x, T_y), which is how they depend on the
-// gradients there: the same operations as flux_core's viscous block.  The
-// structured adjoint differentiates it in these 14 inputs.
-template <class T>
+// The viscous terms flux_core subtracts on an interior face, written as a
+// function of the two cell states, the six averaged gradient components
+// (u_x, u_y, v_x, v_y, T_x, T_y), which is how they depend on the gradients
+// there, and with SST (TURB) of the two cells' eddy viscosity and F1
+// (tq = mu_t,o, mu_t,n, F1_o, F1_n): the same operations as flux_core's
+// viscous block.  The structured adjoint differentiates it in these inputs.
+template <bool TURB, class T>
 __device__ __forceinline__ S6<T> visc_interior(const Phys& P, const FaceGeo& g, const S6<T>& L, const S6<T>& R,
-                                               const T (&gb)[6]) {
+                                               const T (&gb)[6], const T* tq) {
   const double nx = g.nx, ny = g.ny;
   const T rf = 0.5 * (L[0] + R[0]);
   const T uf = 0.5 * (dv(L[1], L[0]) + dv(R[1], R[0]));
@@ -674,8 +675,15 @@ __device__ __forceinline__ S6<T> visc_interior(const Phys& P, const FaceGeo& g, 
     vx = vx0 - dv((vx0 * dxLR + vy0 * dyLR - dvv) * dxLR, d2);
     vy = vy0 - dv((vx0 * dxLR + vy0 * dyLR - dvv) * dyLR, d2);
   }
-  const T mu = rf * P.nu;
-  const T kc = dv(mu * P.Cp, P.Pr);
+  T mu = rf * P.nu;
+  T kc = dv(mu * P.Cp, P.Pr);
+  T mut = T(0.0), F1f = T(0.0);
+  if constexpr (TURB) {
+    mut = 0.5 * (tq[0] + tq[1]);
+    F1f = 0.5 * (tq[2] + tq[3]);
+    kc = P.Cp * (dv(mu, P.Pr) + dv(mut, sst::Prt));
+    mu = mu + mut;
+  }
   const T div = ux + vy;
   const T txx = mu * (2.0 * ux - (2.0 / 3.0) * div);
   const T tyy = mu * (2.0 * vy - (2.0 / 3.0) * div);
@@ -683,11 +691,24 @@ __device__ __forceinline__ S6<T> visc_interior(const Phys& P, const FaceGeo& g, 
   const T qx = -kc * Tx, qy = -kc * Ty;
   const T wx = uf * txx + vf * txy - qx;
   const T wy = uf * txy + vf * tyy - qy;
-  return S6<T>(T(0.0), txx * nx + txy * ny, txy * nx + tyy * ny, wx * nx + wy * ny);
+  S6<T> V(T(0.0), txx * nx + txy * ny, txy * nx + tyy * ny, wx * nx + wy * ny);
+  if constexpr (TURB) {  // two-point normal differences of k and omega
+    const T kcl = dv(L[4], L[0]), wcl = dv(L[5], L[0]);
+    double dn = fabs((g.xcn - g.xco) * nx + (g.ycn - g.yco) * ny);
+    if (dn < 1e-300) dn = 1e-300;
+    const T kn = dv(dv(R[4], R[0]) - kcl, dn);
+    const T wn = dv(dv(R[5], R[0]) - wcl, dn);
+    const T mul = rf * P.nu;
+    const T sgk = F1f * sst::sk1 + (1.0 - F1f) * sst::sk2, sgw = F1f * sst::sw1 + (1.0 - F1f) * sst::sw2;
+    V[4] = (mul + sgk * mut) * kn;
+    V[5] = (mul + sgw * mut) * wn;
+  }
+  return V;
 }
 
 // Interior faces of a second-order residual with four variables (Euler or
-// laminar): k_vjp_face's product, using the face's structure.  Such a face's flux F A
+// laminar), or with SST and k, omega convected first-order upwind (NV = 6):
+// k_vjp_face's product, using the face's structure.  Such a face's flux F A
 // depends on the two cells' gradients only through the reconstructed face
 // primitives (rho, u, v, p) of each side, linearly (weights phi dx, phi dy,
 // none where a reconstruction clip took the cell value), and on the cell
@@ -697,13 +718,19 @@ __device__ __forceinline__ S6<T> visc_interior(const Phys& P, const FaceGeo& g, 
 // adjoints and, through the primitive map's Jacobian, onto the states.  The
 // viscous terms (VISC) depend on the two states and on the averages of six
 // gradient components: duals over those 14 inputs, the averages' adjoints
-// shared half and half by the two cells.
+// shared half and half by the two cells.  With SST the k and omega fluxes are
+// the mass flux times the upwind cell's specific k and omega, carried back by
+// hand, and the viscous terms also depend on the cells' mu_t and F1 (read
+// from Q, rows 0 and 1): their adjoints go to aQ (rows 0 and 1), which
+// k_vjp_src carries back through the cell's SST terms.
 template <int NV, int KD, int VISC>
 __global__ __launch_bounds__(FLUX_BS) void k_vjp_face_inv(DMesh m, Phys P, const double* __restrict__ U,
                                                       const double* __restrict__ G, const double* __restrict__ phi,
-                                                      const double* __restrict__ w, double* __restrict__ aU,
-                                                      double* __restrict__ aG, int f0, int f1) {
-  static_assert(NV == 4 && 8 % KD == 0, "four variables, KD directions per pass");
+                                                      const double* __restrict__ Q, const double* __restrict__ w,
+                                                      double* __restrict__ aU, double* __restrict__ aG,
+                                                      double* __restrict__ aQ, int f0, int f1) {
+  static_assert((NV == 4 || (NV == 6 && VISC)) && 8 % KD == 0, "KD directions per pass");
+  constexpr bool SST = NV == 6;
   typedef DualV<double, KD> T;
   const int i = f0 + xcd_block() * FLUX_BS + threadIdx.x;
   if (i >= f1) return;
@@ -718,14 +745,17 @@ __global__ __launch_bounds__(FLUX_BS) void k_vjp_face_inv(DMesh m, Phys P, const
 #pragma unroll
   for (int k = 0; k < NV; ++k) lam[k] = w[k * p + o] - w[k * p + n];
   // cell primitives and reconstructed face primitives, as flux_core forms them
-  double rho[2], uc[2], vc[2], Wf[2][4], Uc[2][4];
+  double rho[2], uc[2], vc[2], Wf[2][4], Uc[2][NV], kw[2][2];
   bool clip_r[2], clip_p[2];
 #pragma unroll
   for (int sd = 0; sd < 2; ++sd) {
     const int c = cell[sd];
-    const S6<double> Us(U[c], U[p + c], U[2 * p + c], U[3 * p + c]);
+    const S6<double> Us(U[c], U[p + c], U[2 * p + c], U[3 * p + c], SST ? U[4 * p + c] : 0.0,
+                        SST ? U[5 * p + c] : 0.0);
 #pragma unroll
-    for (int k = 0; k < 4; ++k) Uc[sd][k] = Us[k];
+    for (int k = 0; k < NV; ++k) Uc[sd][k] = Us[k];
+    kw[sd][0] = SST ? dv(Us[4], Us[0]) : 0.0;  // the upwind values of the k and omega fluxes
+    kw[sd][1] = SST ? dv(Us[5], Us[0]) : 0.0;
     double gq[8];
 #pragma unroll
     for (int q = 0; q < 8; ++q) gq[q] = G[(size_t)q * p + c];
@@ -746,8 +776,9 @@ __global__ __launch_bounds__(FLUX_BS) void k_vjp_face_inv(DMesh m, Phys P, const
     Wf[sd][2] = vf;
     Wf[sd][3] = pf;
   }
-  // lambda^T d(F A) / d(face primitives), KD directions per pass
-  double aW[2][4];
+  // lambda^T d(F A) / d(face primitives), KD directions per pass; with SST
+  // also the mass flux, which carries k and omega
+  double aW[2][4], mflux = 0.0;
 #pragma unroll
   for (int base = 0; base < 8; base += KD) {
     T q[2][4];
@@ -762,11 +793,16 @@ __global__ __launch_bounds__(FLUX_BS) void k_vjp_face_inv(DMesh m, Phys P, const
     const S6<T> Lr = prim2cons(P, q[0][0], q[0][1], q[0][2], q[0][3]);
     const S6<T> Rr = prim2cons(P, q[1][0], q[1][1], q[1][2], q[1][3]);
     const S6<T> Fl = riemann(P, Lr, Rr, g.nx, g.ny, g.scheme);
+    mflux = Fl[0].v;
+    // SST: F_4 = m k_up, F_5 = m omega_up with the upwind cell's values
+    const int up = Fl[0].v > 0.0 ? 0 : 1;
+    const double lkw = SST ? lam[4] * kw[up][0] + lam[5] * kw[up][1] : 0.0;
 #pragma unroll
     for (int d = 0; d < KD; ++d) {
       double sacc = 0.0;
 #pragma unroll
-      for (int k = 0; k < NV; ++k) sacc += lam[k] * (Fl[k].d[d] * g.A);
+      for (int k = 0; k < 4; ++k) sacc += lam[k] * (Fl[k].d[d] * g.A);
+      if (SST) sacc += lkw * (Fl[0].d[d] * g.A);
       aW[(base + d) >> 2][(base + d) & 3] = sacc;
     }
   }
@@ -796,35 +832,61 @@ __global__ __launch_bounds__(FLUX_BS) void k_vjp_face_inv(DMesh m, Phys P, const
     aU[2 * p + c] = aU[2 * p + c] + (av * ir - ap * gm1 * v);
     aU[3 * p + c] = aU[3 * p + c] + ap * gm1;
   }
+  if constexpr (SST) {  // the upwind specific k and omega: d(U4 / U0), d(U5 / U0)
+    const double mp = mflux > 0.0 ? mflux : 0.0, mm = mflux > 0.0 ? 0.0 : mflux;
+#pragma unroll
+    for (int sd = 0; sd < 2; ++sd) {
+      const int c = cell[sd];
+      const double mf = (sd == 0 ? mp : mm) * g.A, ir = 1.0 / rho[sd];
+      const double ak = lam[4] * mf, aw = lam[5] * mf;
+      aU[c] = aU[c] - (ak * kw[sd][0] + aw * kw[sd][1]) * ir;
+      aU[(size_t)4 * p + c] = aU[(size_t)4 * p + c] + ak * ir;
+      aU[(size_t)5 * p + c] = aU[(size_t)5 * p + c] + aw * ir;
+    }
+  }
   if constexpr (VISC != 0) {
-    constexpr int KV = 7;  // two passes over (L, R, averaged gradients)
+    // duals over (L, R, averaged gradients[, mu_t and F1 of both cells])
+    constexpr int NIN = 2 * NV + 6 + (SST ? 4 : 0);
+    constexpr int KV = SST ? 6 : 7;
     typedef DualV<double, KV> TV;
     const int gq[6] = {2, 3, 4, 5, 8, 9};
-    double gbv[6];
+    double gbv[6], tqv[4] = {0.0, 0.0, 0.0, 0.0};
 #pragma unroll
     for (int q = 0; q < 6; ++q) gbv[q] = 0.5 * (G[(size_t)gq[q] * p + o] + G[(size_t)gq[q] * p + n]);
-    double ain[14];
+    if (SST) {
+      tqv[0] = Q[o];
+      tqv[1] = Q[n];
+      tqv[2] = Q[p + o];
+      tqv[3] = Q[p + n];
+    }
+    double ain[NIN + KV];
 #pragma unroll
-    for (int base = 0; base < 14; base += KV) {
+    for (int base = 0; base < NIN; base += KV) {
       S6<TV> Ld, Rd;
-      TV gbd[6];
+      TV gbd[6], tqd[4];
 #pragma unroll
-      for (int k = 0; k < 4; ++k) {
+      for (int k = 0; k < NV; ++k) {
         Ld[k] = TV(Uc[0][k]);
         Rd[k] = TV(Uc[1][k]);
 #pragma unroll
         for (int d = 0; d < KV; ++d) {
           Ld[k].d[d] = (k == base + d) ? 1.0 : 0.0;
-          Rd[k].d[d] = (4 + k == base + d) ? 1.0 : 0.0;
+          Rd[k].d[d] = (NV + k == base + d) ? 1.0 : 0.0;
         }
       }
 #pragma unroll
       for (int q = 0; q < 6; ++q) {
         gbd[q] = TV(gbv[q]);
 #pragma unroll
-        for (int d = 0; d < KV; ++d) gbd[q].d[d] = (8 + q == base + d) ? 1.0 : 0.0;
+        for (int d = 0; d < KV; ++d) gbd[q].d[d] = (2 * NV + q == base + d) ? 1.0 : 0.0;
       }
-      const S6<TV> Vv = visc_interior(P, g, Ld, Rd, gbd);
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        tqd[q] = TV(tqv[q]);
+#pragma unroll
+        for (int d = 0; d < KV; ++d) tqd[q].d[d] = (2 * NV + 6 + q == base + d) ? 1.0 : 0.0;
+      }
+      const S6<TV> Vv = visc_interior<SST>(P, g, Ld, Rd, gbd, tqd);
 #pragma unroll
       for (int d = 0; d < KV; ++d) {
         double sacc = 0.0;
@@ -837,25 +899,33 @@ __global__ __launch_bounds__(FLUX_BS) void k_vjp_face_inv(DMesh m, Phys P, const
     for (int sd = 0; sd < 2; ++sd) {
       const int c = cell[sd];
 #pragma unroll
-      for (int k = 0; k < 4; ++k) aU[(size_t)k * p + c] = aU[(size_t)k * p + c] + ain[4 * sd + k];
+      for (int k = 0; k < NV; ++k) aU[(size_t)k * p + c] = aU[(size_t)k * p + c] + ain[NV * sd + k];
 #pragma unroll
       for (int q = 0; q < 6; ++q)
-        aG[(size_t)gq[q] * p + c] = aG[(size_t)gq[q] * p + c] + 0.5 * ain[8 + q];
+        aG[(size_t)gq[q] * p + c] = aG[(size_t)gq[q] * p + c] + 0.5 * ain[2 * NV + q];
+      if (SST) {
+        aQ[c] = aQ[c] + ain[2 * NV + 6 + sd];
+        aQ[(size_t)p + c] = aQ[(size_t)p + c] + ain[2 * NV + 8 + sd];
+      }
     }
   }
 }
 
-// SST sources: w_c^T d(-V S)/d(U_c, G_c) onto aU_c and aG_c (one thread per cell)
+// SST cell terms: w_c^T d(-V S)/d(U_c, G_c), plus the face sweep's adjoints
+// of the cell's mu_t and F1 (aQ rows 0 and 1) times d(mu_t, F1)/d(U_c, G_c),
+// onto aU_c and aG_c (one thread per cell)
 template <int NV>
 __global__ __launch_bounds__(BS) void k_vjp_src(DMesh m, Phys P, const double* __restrict__ U,
                                                  const double* __restrict__ G, const double* __restrict__ w,
-                                                 double* __restrict__ aU, double* __restrict__ aG) {
+                                                 const double* __restrict__ aQ, double* __restrict__ aU,
+                                                 double* __restrict__ aG) {
   typedef DualV<double, VK> T;
   const int c = xcd_block() * BS + threadIdx.x;
   if (c >= m.N) return;
   const int p = m.pitch;
   const double V = m.vol[c];
   const double l4 = -V * w[4 * p + c], l5 = -V * w[5 * p + c];
+  const double amut = aQ[c], aF1 = aQ[p + c];
   for (int base = 0; base < NV + 14; base += VK) {
     S6<T> Uc;
     T g[14];
@@ -876,7 +946,7 @@ __global__ __launch_bounds__(BS) void k_vjp_src(DMesh m, Phys P, const double* _
     for (int d = 0; d < VK; ++d) {
       const int idx = base + d;
       if (idx >= NV + 14) break;
-      const double s = l4 * tq.Sk.d[d] + l5 * tq.Sw.d[d];
+      const double s = l4 * tq.Sk.d[d] + l5 * tq.Sw.d[d] + amut * tq.mut.d[d] + aF1 * tq.F1.d[d];
       double* dst = idx < NV ? aU + (size_t)idx * p + c : aG + (size_t)(idx - NV) * p + c;
       *dst = *dst + s;
     }
@@ -2423,29 +2493,33 @@ template <int NV, int VISC>
 static void vjp_nv(cfd_ctx* c, const double* U, const double* w, double* JTw) {
   const DMesh& m = c->dm;
   const Phys& P = c->phys;
-  double* aG = c->adj;
+  double* aG = c->adj;                         // 14 rows: adjoints of the gradients
+  double* aQ = c->adj + (size_t)14 * m.pitch;  // 2 rows: adjoints of mu_t and F1 (SST)
   hipLaunchKernelGGL((k_gradlim<NV, double>), dim3(nblocks(m.N, GRAD_BS)), dim3(GRAD_BS), 0, c->stream, m, P, U,
-                     (const double*)nullptr, c->grad, (double*)nullptr, c->phi, 0);
+                     (const double*)nullptr, c->grad, (double*)nullptr, c->phi, 0, c->tq, (double*)nullptr);
   launch_fill(c, JTw, 0.0, (size_t)NV * m.pitch);
-  launch_fill(c, aG, 0.0, (size_t)14 * m.pitch);
-  // a second-order residual with four variables (Euler or laminar) takes its
-  // interior faces (each color's leading run) through the structured
-  // product; the boundary faces, and every face of the other residuals,
-  // through the general one
-  const bool inv = NV == 4 && P.spatial != CFD_FIRST_ORDER;
+  launch_fill(c, aG, 0.0, (size_t)16 * m.pitch);
+  // a second-order residual with four variables (Euler or laminar), or with
+  // SST and first-order k and omega convection, takes its interior faces
+  // (each color's leading run) through the structured product; the boundary
+  // faces, and every face of the other residuals, through the general one
+  constexpr bool has_inv = NV == 4 || VISC != 0;
+  const bool inv = has_inv && P.spatial != CFD_FIRST_ORDER && (NV == 4 || (P.turb && P.turb1));
   for (int col = 0; col < c->ncolors; ++col) {
     const int f0 = c->color_ptr[col], f1 = c->color_ptr[col + 1], fb = inv ? c->color_bptr[col] : f0;
-    if (fb > f0)
-      hipLaunchKernelGGL((k_vjp_face_inv<4, VJP_KD, VISC>), dim3(nblocks(fb - f0, FLUX_BS)), dim3(FLUX_BS), 0,
-                         c->stream, m, P, U,
-                         (const double*)c->grad, (const double*)c->phi, w, JTw, aG, f0, fb);
+    if constexpr (has_inv) {
+      if (fb > f0)
+        hipLaunchKernelGGL((k_vjp_face_inv<NV, VJP_KD, VISC>), dim3(nblocks(fb - f0, FLUX_BS)), dim3(FLUX_BS), 0,
+                           c->stream, m, P, U, (const double*)c->grad, (const double*)c->phi,
+                           (const double*)c->tq, w, JTw, aG, aQ, f0, fb);
+    }
     if (f1 > fb)
       hipLaunchKernelGGL((k_vjp_face<NV, VISC>), dim3(nblocks(f1 - fb, FLUX_BS)), dim3(FLUX_BS), 0, c->stream, m, P, U,
                          (const double*)c->grad, (const double*)c->phi, w, JTw, aG, fb, f1);
   }
   if (NV == 6 && P.turb)
     hipLaunchKernelGGL((k_vjp_src<NV>), dim3(nblocks(m.N)), dim3(BS), 0, c->stream, m, P, U,
-                       (const double*)c->grad, w, JTw, aG);
+                       (const double*)c->grad, w, (const double*)aQ, JTw, aG);
   for (int col = 0; col < c->ncolors; ++col) {
     const int f0 = c->color_ptr[col], f1 = c->color_ptr[col + 1];
     if (f1 > f0)

@@ -17,11 +17,13 @@ pytestmark = pytest.mark.gpu
 CASES = [(1, 0.1), (2, 0.05), (3, 0.02), (4, 0.03), (5, 0.02)]
 
 
-def _setup(config, scale, flux, spatial=None):
+def _setup(config, scale, flux, spatial=None, tfo=None):
     mesh, p, U = configs.setup(config, scale)
     p.flux = flux
     if spatial is not None:
         p.spatial = spatial
+    if tfo is not None:
+        p.turb_first_order = tfo
     s = cfd.Solver(mesh, p)
     o = oracle.OracleSolver(oracle.OracleMesh.from_mesh(mesh), p)
     if config == 5:
@@ -40,11 +42,14 @@ def _setup(config, scale, flux, spatial=None):
     return s, o, U, p, rng
 
 
-@pytest.mark.parametrize("config,scale", CASES)
+@pytest.mark.parametrize("config,scale,tfo", [c + (None,) for c in CASES] + [(3, 0.02, 0), (3, 0.02, 1)])
 @pytest.mark.parametrize("flux", [0, 1, 2])
-def test_dot_product_identity(config, scale, flux):
-    s, o, U, p, rng = _setup(config, scale, flux)
-    print(f"config={config} flux={flux} N={s.N}", flush=True)
+def test_dot_product_identity(config, scale, tfo, flux):
+    """Structured interior-face products (Euler, laminar, SST with first-order
+    k and omega convection) and the general one (SST with k and omega
+    reconstructed: turb_first_order = 0) against the forward tangent."""
+    s, o, U, p, rng = _setup(config, scale, flux, tfo=tfo)
+    print(f"config={config} flux={flux} turb_first_order={p.turb_first_order} N={s.N}", flush=True)
     s.compute_residuals(U, True)
     s.compute_local_time_step(U, 3.0)
     v = rng.standard_normal(U.shape) * np.abs(U).mean(0)
