@@ -41,6 +41,9 @@ constexpr int GRAD_BS = 128;
 // plate, 2.052 / 1.270 ms (FD + FP64 / tangent + FP32) in 64-thread blocks
 // against 2.080 / 1.274 ms in 256-thread ones.
 constexpr int LINE_BS = 64;
+// Basis vectors per block of the Arnoldi reduction: every block also reads
+// the candidate u and the product w, so larger groups read them fewer times.
+constexpr int DOTS_GROUP = 16;
 
 template <int NV, class T>
 __device__ __forceinline__ S6<double> ld(const T* __restrict__ U, int p, int c) {
@@ -2257,17 +2260,17 @@ __global__ __launch_bounds__(BS) void k_rowscale(int N, int p, TV* __restrict__ 
 // Projections of u and of w on V_0 .. V_{nq-1} and on u itself, 8 vectors per
 // grid row: R[i] = V_i . u (R[nq] = u . u) and, with w, R[nq + 1 + i] =
 // V_i . w (R[2 nq + 1] = u . w); per-block partials into part.
-template <int NV, class TV>
+template <int NV, class TV, int DG>
 __global__ __launch_bounds__(BS) void k_dots2(int N, int p, size_t vstride, const TV* __restrict__ V, int nq,
                                                const TV* __restrict__ u, const TV* __restrict__ w,
                                                double* __restrict__ part) {
   __shared__ double sh[BS / 64];
-  const int i0 = blockIdx.y * 8;
-  const int cnt = min(8, nq + 1 - i0);
+  const int i0 = blockIdx.y * DG;
+  const int cnt = min(DG, nq + 1 - i0);
   const bool two = w != nullptr;
-  double au[8], aw[8];
+  double au[DG], aw[DG];
 #pragma unroll
-  for (int i = 0; i < 8; ++i) au[i] = aw[i] = 0.0;
+  for (int i = 0; i < DG; ++i) au[i] = aw[i] = 0.0;
   for (int c = blockIdx.x * BS + threadIdx.x; c < N; c += gridDim.x * BS) {
     double uv[NV], wv[NV];
 #pragma unroll
@@ -2276,7 +2279,7 @@ __global__ __launch_bounds__(BS) void k_dots2(int N, int p, size_t vstride, cons
       wv[k] = two ? (double)w[k * p + c] : 0.0;
     }
 #pragma unroll
-    for (int i = 0; i < 8; ++i) {
+    for (int i = 0; i < DG; ++i) {
       if (i < cnt) {
         const TV* Vi = i0 + i < nq ? V + (size_t)(i0 + i) * vstride : u;
         double tu = 0.0, tw = 0.0;
@@ -2729,8 +2732,9 @@ template <class TV> static void dcgs_t(cfd_ctx* c, TV* V, int j, bool last, doub
   double* hp = k.dc + 64 * 64;
   double* coef = hp + 64;
   NV_DISPATCH(nv, {
-    hipLaunchKernelGGL((k_dots2<NVt, TV>), dim3(nb, (j + 1 + 7) / 8), dim3(BS), 0, c->stream, N, p, vs, (const TV*)V,
-                       j, (const TV*)u, last ? (const TV*)nullptr : (const TV*)(u + vs), k.part);
+    hipLaunchKernelGGL((k_dots2<NVt, TV, DOTS_GROUP>), dim3(nb, (j + DOTS_GROUP) / DOTS_GROUP), dim3(BS), 0,
+                       c->stream, N, p, vs, (const TV*)V, j, (const TV*)u,
+                       last ? (const TV*)nullptr : (const TV*)(u + vs), k.part);
     hipLaunchKernelGGL(k_red_final, dim3(nsum), dim3(BS), 0, c->stream, (const double*)k.part, nb, nsum, k.h,
                        (const double*)nullptr);
     hipLaunchKernelGGL(k_dcgs_coef, dim3(1), dim3(64), 0, c->stream, (const double*)k.h, j, last ? 1 : 0, Hd, 64,
