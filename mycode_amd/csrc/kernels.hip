@@ -968,68 +968,75 @@ template <class T> __device__ __forceinline__ void prims(const Phys& P, const S6
   }
 }
 
+// Green-Gauss transpose, one thread per cell: the gradients are face means
+// of the primitives q over the cell's faces, so with the face mean's adjoint
+// aq_f = A (aG_o . n / V_o - aG_n . n / V_n) (aG holds the 2 slots of each
+// primitive), aU_c += 1/2 (sum_f aq_f)^T dq(U_c)/dU_c, plus on a boundary face
+// 1/2 aq_f^T dq(ghost(U_c))/dU_c.  One Jacobian of q per cell, and no write
+// to another cell, so a single launch.
 template <int NV>
 __global__ __launch_bounds__(BS) void k_vjp_gg(DMesh m, Phys P, const double* __restrict__ U,
-                                                const double* __restrict__ aG, double* __restrict__ aU, int f0,
-                                                int f1) {
+                                                const double* __restrict__ aG, double* __restrict__ aU) {
   typedef DualV<double, NV> T;
   constexpr int NQM = NV == 6 ? 7 : 5;
-  const int i = f0 + xcd_block() * BS + threadIdx.x;
-  if (i >= f1) return;
-  const int f = m.corder[i];
+  const int c = xcd_block() * BS + threadIdx.x;
+  if (c >= m.N) return;
   const int p = m.pitch;
-  const int o = m.own[f], n = m.nei[f];
   const bool turb = NV == 6 && P.turb;
   const int nq = turb ? 7 : 5;
-  const double nx = m.nx[f], ny = m.ny[f], A = m.area[f];
-  const double vo = m.vol[o];
-  double aq[NQM];
-#pragma unroll
-  for (int k = 0; k < NQM; ++k) {
-    if (k >= nq) {
-      aq[k] = 0.0;
-      continue;
-    }
-    aq[k] = A * ((aG[(size_t)(2 * k) * p + o] * nx + aG[(size_t)(2 * k + 1) * p + o] * ny) / vo);
-    if (n >= 0)
-      aq[k] -= A * ((aG[(size_t)(2 * k) * p + n] * nx + aG[(size_t)(2 * k + 1) * p + n] * ny) / m.vol[n]);
-  }
-  S6<T> L;
+  S6<T> Uc;
 #pragma unroll
   for (int j = 0; j < NV; ++j) {
-    L[j] = T(U[j * p + o]);
+    Uc[j] = T(U[j * p + c]);
 #pragma unroll
-    for (int d = 0; d < NV; ++d) L[j].d[d] = j == d ? 1.0 : 0.0;
+    for (int d = 0; d < NV; ++d) Uc[j].d[d] = j == d ? 1.0 : 0.0;
   }
-  T qL[NQM], qR[NQM];
-  prims(P, L, qL, turb);
-  if (n >= 0) {
-    S6<T> R;
+  double S[NQM], ag[NV];
 #pragma unroll
-    for (int j = 0; j < NV; ++j) {
-      R[j] = T(U[j * p + n]);
+  for (int k = 0; k < NQM; ++k) S[k] = 0.0;
 #pragma unroll
-      for (int d = 0; d < NV; ++d) R[j].d[d] = j == d ? 1.0 : 0.0;
-    }
-    prims(P, R, qR, turb);
-  } else {
-    prims(P, ghost(P, L, m.tag[f], nx, ny, turb ? m.dnw[f] : 0.0), qR, turb);
-  }
+  for (int j = 0; j < NV; ++j) ag[j] = 0.0;
+  const int4 s4 = m.slots[c];
+  const int sl[4] = {s4.x, s4.y, s4.z, s4.w};
 #pragma unroll
-  for (int j = 0; j < NV; ++j) {
-    double so = 0.0, sn = 0.0;
+  for (int q = 0; q < 4; ++q) {
+    const int e = sl[q];
+    if (e < 0) continue;
+    const int f = e >> 1;
+    const int o = m.own[f], n = m.nei[f];
+    const double nx = m.nx[f], ny = m.ny[f], A = m.area[f];
+    const double vo = m.vol[o];
+    double aq[NQM];
 #pragma unroll
     for (int k = 0; k < NQM; ++k) {
-      if (k >= nq) break;
-      so += aq[k] * 0.5 * qL[k].d[j];
-      sn += aq[k] * 0.5 * qR[k].d[j];
+      if (k >= nq) {
+        aq[k] = 0.0;
+        continue;
+      }
+      aq[k] = A * ((aG[(size_t)(2 * k) * p + o] * nx + aG[(size_t)(2 * k + 1) * p + o] * ny) / vo);
+      if (n >= 0)
+        aq[k] -= A * ((aG[(size_t)(2 * k) * p + n] * nx + aG[(size_t)(2 * k + 1) * p + n] * ny) / m.vol[n]);
+      S[k] += aq[k];
     }
-    if (n >= 0) {
-      aU[(size_t)j * p + o] += so;
-      aU[(size_t)j * p + n] += sn;
-    } else {
-      aU[(size_t)j * p + o] += so + sn;
+    if (n < 0) {  // the ghost state's primitives enter the face mean too
+      T qg[NQM];
+      prims(P, ghost(P, Uc, m.tag[f], nx, ny, turb ? m.dnw[f] : 0.0), qg, turb);
+#pragma unroll
+      for (int j = 0; j < NV; ++j)
+#pragma unroll
+        for (int k = 0; k < NQM; ++k)
+          if (k < nq) ag[j] += aq[k] * 0.5 * qg[k].d[j];
     }
+  }
+  T qc[NQM];
+  prims(P, Uc, qc, turb);
+#pragma unroll
+  for (int j = 0; j < NV; ++j) {
+    double s = ag[j];
+#pragma unroll
+    for (int k = 0; k < NQM; ++k)
+      if (k < nq) s += S[k] * 0.5 * qc[k].d[j];
+    aU[(size_t)j * p + c] += s;
   }
 }
 
@@ -2523,12 +2530,7 @@ static void vjp_nv(cfd_ctx* c, const double* U, const double* w, double* JTw) {
   if (NV == 6 && P.turb)
     hipLaunchKernelGGL((k_vjp_src<NV>), dim3(nblocks(m.N)), dim3(BS), 0, c->stream, m, P, U,
                        (const double*)c->grad, w, (const double*)aQ, JTw, aG);
-  for (int col = 0; col < c->ncolors; ++col) {
-    const int f0 = c->color_ptr[col], f1 = c->color_ptr[col + 1];
-    if (f1 > f0)
-      hipLaunchKernelGGL((k_vjp_gg<NV>), dim3(nblocks(f1 - f0)), dim3(BS), 0, c->stream, m, P, U,
-                         (const double*)aG, JTw, f0, f1);
-  }
+  hipLaunchKernelGGL((k_vjp_gg<NV>), dim3(nblocks(m.N)), dim3(BS), 0, c->stream, m, P, U, (const double*)aG, JTw);
   hipLaunchKernelGGL((k_tcombine<NV, double>), dim3(nblocks(m.N)), dim3(BS), 0, c->stream, m.N, m.pitch, JTw, w,
                      c->d_vol, c->dt);
 }
