@@ -96,6 +96,7 @@ __device__ __forceinline__ S6<T> ldT(const double* __restrict__ U, const typenam
 // are formed here once per cell (rows 0..3 of Qv / Qd) for the face fluxes.
 // The FP64 residual's instance is held to 128 VGPRs (4 waves per SIMD instead
 // of 3 at 130, two registers spilled): 121 -> 119 us on the 1.32M-cell plate.
+// The FP32 tangent's (133 VGPRs, five spilled at 128) got slower that way.
 template <int NV, class T>
 __global__ __launch_bounds__(GRAD_BS) __attribute__((amdgpu_waves_per_eu(std::is_same<T, double>::value ? 4 : 1, 8)))
 void k_gradlim(DMesh m, Phys P, const double* __restrict__ U,
