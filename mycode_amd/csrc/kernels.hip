@@ -573,6 +573,8 @@ __global__ __launch_bounds__(BS) void k_tcombine(int N, int p, F* __restrict__ J
 //   k_vjp_gg    the Green-Gauss transpose: aG_c / V_c back onto the face means
 //               and through the primitive (and ghost) maps onto U.
 constexpr int VK = 4;
+// passes of VK directions that cover a face's inputs (both states, both gradients)
+template <int NV> __host__ __device__ constexpr int vjp_passes() { return (2 * NV + 2 * (NV == 6 ? 14 : 10) + VK - 1) / VK; }
 constexpr int VJP_KD = 8;  // directions per dual pass of k_vjp_face_inv
 
 template <int NV, int VISC>
@@ -582,7 +584,10 @@ __global__ __launch_bounds__(FLUX_BS) void k_vjp_face(DMesh m, Phys P, const dou
                                                   double* __restrict__ aG, int f0, int f1) {
   typedef DualV<double, VK> T;
   constexpr int NGM = NV == 6 ? 14 : 10;
-  const int i = f0 + xcd_block() * FLUX_BS + threadIdx.x;
+  // one thread per (face, pass of VK directions): the passes of a face run on
+  // neighbouring lanes and write disjoint inputs' adjoints
+  const int t = xcd_block() * FLUX_BS + threadIdx.x;
+  const int i = f0 + t / vjp_passes<NV>(), pass = t % vjp_passes<NV>();
   if (i >= f1) return;
   const int f = m.corder[i];
   const int p = m.pitch;
@@ -600,7 +605,7 @@ __global__ __launch_bounds__(FLUX_BS) void k_vjp_face(DMesh m, Phys P, const dou
   }
   const int nU = interior ? 2 * NV : NV;
   const int nin = nU + (need ? (interior ? 2 * ng : ng) : 0);
-  for (int base = 0; base < nin; base += VK) {
+  for (int base = pass * VK; base < nin && base < (pass + 1) * VK; base += VK) {
     S6<T> L, R;
     T gL[NGM], gR[NGM];
 #pragma unroll
@@ -2528,7 +2533,8 @@ static void vjp_nv(cfd_ctx* c, const double* U, const double* w, double* JTw) {
                            (const double*)c->tq, w, JTw, aG, aQ, f0, fb);
     }
     if (f1 > fb)
-      hipLaunchKernelGGL((k_vjp_face<NV, VISC>), dim3(nblocks(f1 - fb, FLUX_BS)), dim3(FLUX_BS), 0, c->stream, m, P, U,
+      hipLaunchKernelGGL((k_vjp_face<NV, VISC>), dim3(nblocks((size_t)(f1 - fb) * vjp_passes<NV>(), FLUX_BS)),
+                         dim3(FLUX_BS), 0, c->stream, m, P, U,
                          (const double*)c->grad, (const double*)c->phi, w, JTw, aG, fb, f1);
   }
   if (NV == 6 && P.turb)
