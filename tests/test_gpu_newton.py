@@ -4,7 +4,7 @@ device-pointer GMRES entry point.
 Bars: the laminar plate (4,814 cells) reaches a 1e-10 drop of the
 volume-weighted residual norm within 200 Newton steps with FD + FP64 Krylov
 and with tangent + FP32 Krylov; their wall Cp and Cf agree within 1e-8 (the
-north-star Cf bar is 1e-6); the device GMRES gives the host GMRES's
+target Cf bar is 1e-6); the device GMRES gives the host GMRES's
 iteration count and solution to 1e-12 relative.
 """
 import numpy as np
