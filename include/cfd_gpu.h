@@ -180,8 +180,19 @@ int cfd_ctx_get_params(const cfd_ctx* ctx, cfd_params* p);
 /* Elements per SoA row of device vectors. */
 int cfd_pitch(const cfd_ctx* ctx);
 int cfd_ncells(const cfd_ctx* ctx);
-/* hipStream_t of the context, as void*. */
+/* hipStream_t the context launches and copies on, as void*. */
 void* cfd_stream(const cfd_ctx* ctx);
+/* The stream the context created for itself (its default), as void*. */
+void* cfd_ctx_own_stream(const cfd_ctx* ctx);
+/* Launch and copy on the caller's hipStream_t `stream` (any stream of the
+ * context's device, including the null stream 0) from now on, so that work
+ * the caller queues on that stream before and after a cfd_dev_* call is
+ * ordered with it without device-wide synchronisation.  Work already queued
+ * on the previous stream is waited for first.  The caller keeps ownership;
+ * cfd_ctx_set_stream(ctx, cfd_ctx_own_stream(ctx)) returns to the context's
+ * own stream, which it does before the caller destroys theirs.  No reference
+ * counterpart (the reference is single-threaded CPU code). */
+int cfd_ctx_set_stream(cfd_ctx* ctx, void* stream);
 /* Per-face flux override (0 = params.flux, 1 = params.flux_alt), reference face order. */
 int cfd_set_face_flux(cfd_ctx* ctx, const unsigned char* use_alt);
 /* Residual assembly strategy: 0 = edge-colored scatter (faces laid out color
@@ -310,9 +321,11 @@ int cfd_solve_implicit(cfd_ctx* ctx, double* U6, const cfd_newton_opts* opts,
  * (gopts, NULL = the reference's FD Jv + scalar diagonal, m = 40, tol 1e-2),
  * stopping when VM||F|| over rho, rho u, rho v, rho E < tolerance after at
  * least one update, U += relax dU without clips.  max_newton <= 0, tolerance
- * <= 0 and relax <= 0 select the reference's 15, 1e-6 and 0.7.  hist_vm
- * (n_steps x max_newton, may be NULL) receives VM||F|| per Newton step,
- * hist_newton (n_steps, may be NULL) the updates taken per physical step. */
+ * <= 0 and relax <= 0 select the reference's 15, 1e-6 and 0.7; max_newton <= 0
+ * with a non-NULL hist_vm is an error (CFD_ERR_ARG), since the rows of hist_vm
+ * are max_newton long.  hist_vm (n_steps x max_newton, may be NULL) receives
+ * VM||F|| per Newton step, hist_newton (n_steps, may be NULL) the updates
+ * taken per physical step. */
 int cfd_solve_unsteady(cfd_ctx* ctx, double* U6, double dt_phys, int n_steps, const cfd_gmres_opts* gopts,
                        int max_newton, double tolerance, double relax, double* hist_vm, int* hist_newton);
 /* Solver::timeStepExplicit(dt) — Solver.cpp:1219-1243 (forward Euler). */
@@ -322,9 +335,23 @@ int cfd_rk3_step(cfd_ctx* ctx, double* U6, double dt);
 
 /* Wall outputs: Cp per wall face (tags 3 and 4), Cf per tag-4 face
  * (Solver::extractSkinFriction, Solver.cpp:1899-1979), both in reference face
- * order (0 elsewhere), and J = sum over tag-3 faces of p_owner * area
- * (cf. calculateDrag, Solver.cpp:1307-1325). */
+ * order (0 elsewhere), and J = sum over tag-3 faces of p_owner * area, the
+ * wall pressure integral used as the design functional (no reference
+ * counterpart; calculateDrag below is the reference's projected form). */
 int cfd_wall_outputs(cfd_ctx* ctx, const double* U6, double* Cp, double* Cf, double* J);
+/* Solver::calculateDrag (Solver.cpp:1307-1325) at U6: the sum over the
+ * tag-3 faces, in face order, of -p_owner * area * n_x.  The reference
+ * prints it; here it is returned. */
+int cfd_drag(cfd_ctx* ctx, const double* U6, double* drag);
+/* Solver::extractWallPressure (Solver.cpp:2027-2070) at U6: one row
+ * (x_mid * x_scale, p_owner / p_divisor) per tag-3 face whose owner centroid
+ * has y_c < y_max, ascending in x (rows of equal x by pressure).  The
+ * reference writes the rows to a CSV file with y_max = 0.2, x_scale = 12
+ * (ft -> in) and p_divisor = 144 (psf -> psi); here they fill rows2 (up to
+ * `capacity` rows) and *n_rows receives their number (an error if it
+ * exceeds the capacity). */
+int cfd_wall_pressure(cfd_ctx* ctx, const double* U6, double y_max, double x_scale, double p_divisor, int capacity,
+                      double* rows2, int* n_rows);
 
 /* Solver::calculateEntropyError (Solver.cpp:1620-1646): the volume-weighted L2
  * norm of (p / rho^gamma - S_ref) / S_ref over the cells with p, rho > 0, with

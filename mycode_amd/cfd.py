@@ -101,6 +101,10 @@ SIGNATURES = [
     ("cfd_pitch", C.c_int, [_VP]),
     ("cfd_ncells", C.c_int, [_VP]),
     ("cfd_stream", C.c_void_p, [_VP]),
+    ("cfd_ctx_own_stream", C.c_void_p, [_VP]),
+    ("cfd_ctx_set_stream", C.c_int, [_VP, C.c_void_p]),
+    ("cfd_drag", C.c_int, [_VP, _DP, _DP]),
+    ("cfd_wall_pressure", C.c_int, [_VP, _DP, C.c_double, C.c_double, C.c_double, C.c_int, _DP, _IP]),
     ("cfd_set_face_flux", C.c_int, [_VP, C.POINTER(C.c_ubyte)]),
     ("cfd_set_assembly", C.c_int, [_VP, C.c_int]),
     ("cfd_residual", C.c_int, [_VP, _DP, _DP, C.c_int]),
@@ -560,9 +564,29 @@ class Solver:
         self._c(self.lib.cfd_wall_outputs(self._h, _dp(U), _dp(Cp), _dp(Cf), C.byref(J)), "cfd_wall_outputs")
         return Cp, Cf, J.value
 
+    def calculate_drag(self, U) -> float:
+        """Solver::calculateDrag: sum over tag-3 faces of -p_owner A n_x."""
+        out = C.c_double()
+        self._c(self.lib.cfd_drag(self._h, _dp(_state(U, self.N)), C.byref(out)), "cfd_drag")
+        return out.value
+
+    def extract_wall_pressure(self, U, y_max: float = 0.2, x_scale: float = 12.0, p_divisor: float = 144.0):
+        """Solver::extractWallPressure rows (x_mid * x_scale, p_owner / p_divisor)
+        of the tag-3 faces with owner y_c < y_max, ascending in x."""
+        U = _state(U, self.N)
+        n = C.c_int()
+        cap = self.mesh.n_faces
+        rows = np.zeros((max(cap, 1), 2))
+        self._c(self.lib.cfd_wall_pressure(self._h, _dp(U), y_max, x_scale, p_divisor, cap, _dp(rows), C.byref(n)),
+                "cfd_wall_pressure")
+        return rows[:n.value]
+
     # device-handle variants (raw device pointers, e.g. torch tensors' data_ptr()).
-    # They run on the context's own stream: torch work that produces or clears
-    # a buffer passed here must have finished (torch.cuda.synchronize()) first.
+    # They run on the context's stream (its own unless set_stream chose
+    # another): torch work that produces or clears a buffer passed here must
+    # be ordered before the call, either by running the context on torch's
+    # current stream (set_stream(torch.cuda.current_stream().cuda_stream)) or
+    # by torch.cuda.synchronize().
     def dev_residual(self, U_ptr: int, R_ptr: int, update_limiters: bool = True):
         self._c(self.lib.cfd_dev_residual(self._h, C.c_void_p(U_ptr), C.c_void_p(R_ptr), int(update_limiters)),
                 "cfd_dev_residual")
@@ -618,6 +642,8 @@ class Solver:
         """Solver::solveUnsteady (BDF1 dual time stepping): (U, VM||F|| per Newton
         step (n_steps x max_newton, zero after the stop), updates per step)."""
         U = _state(U, self.N).copy()
+        if max_newton <= 0:
+            max_newton = 15  # the reference's default (Solver.cpp:1819-1820)
         hvm = np.zeros((max(1, n_steps), max_newton))
         hn = np.zeros(max(1, n_steps), dtype=np.int32)
         g = C.byref(gmres) if gmres is not None else None
@@ -642,3 +668,9 @@ class Solver:
     @property
     def stream(self) -> int:
         return int(self.lib.cfd_stream(self._h) or 0)
+
+    def set_stream(self, stream: int | None):
+        """Launch on the given hipStream_t handle (e.g. torch's
+        current_stream().cuda_stream); None returns to the context's own."""
+        h = self.lib.cfd_ctx_own_stream(self._h) if stream is None else stream
+        self._c(self.lib.cfd_ctx_set_stream(self._h, C.c_void_p(h or 0)), "cfd_ctx_set_stream")

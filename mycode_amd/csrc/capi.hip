@@ -34,6 +34,14 @@ int fail(cfd_ctx* c, int code, const std::string& msg) {
   return code;
 }
 
+// The products hold the limiter frozen (Solver.cpp:1366 evaluates with
+// update_limiters = false), so with SECOND_ORDER_LIMITED one must exist.
+int need_limiter(cfd_ctx* c) {
+  if (c->P.spatial == CFD_SECOND_ORDER_LIMITED && !c->phi_valid)
+    return fail(c, CFD_ERR_STATE, "limiter not formed: evaluate the residual first");
+  return CFD_OK;
+}
+
 template <class T> hipError_t dalloc(T** p, size_t n) { return hipMalloc((void**)p, std::max<size_t>(n, 1) * sizeof(T)); }
 
 void fill_phys(cfd_ctx* c) {
@@ -419,7 +427,8 @@ int cfd_ctx_create(const cfd_mesh* m, const cfd_params* p, cfd_ctx** out) {
   c->P = *p;
   fill_phys(c);
   HIP_TRY(c, hipGetDevice(&c->device));
-  HIP_TRY(c, hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+  HIP_TRY(c, hipStreamCreateWithFlags(&c->own_stream, hipStreamNonBlocking));
+  c->stream = c->own_stream;
   const int N = m->ncells(), F = m->nfaces();
   c->N = N;
   c->F = F;
@@ -663,6 +672,7 @@ int cfd_ctx_create(const cfd_mesh* m, const cfd_params* p, cfd_ctx** out) {
 int cfd_ctx_update_geometry(cfd_ctx* c, const cfd_mesh* m) {
   if (!c || !m || m->ncells() != c->N || m->nfaces() != c->F) return CFD_ERR_ARG;
   if (m != c->mesh) return fail(c, CFD_ERR_ARG, "geometry must come from the context's own mesh");
+  c->phi_valid = false;  // the limiter depends on the geometry
   return upload_geometry(c, m);
 }
 
@@ -677,7 +687,7 @@ int cfd_ctx_destroy(cfd_ctx* c) {
   for (int i = 0; i < 6; ++i) (void)hipFree(c->vec[i]);
   (void)hipFree(c->aux);
   (void)hipHostFree(c->scal_host);
-  if (c->stream) (void)hipStreamDestroy(c->stream);
+  if (c->own_stream) (void)hipStreamDestroy(c->own_stream);
   delete c;
   return CFD_OK;
 }
@@ -687,7 +697,8 @@ const char* cfd_last_error(const cfd_ctx* c) { return c ? c->err.c_str() : "null
 int cfd_ctx_set_params(cfd_ctx* c, const cfd_params* p) {
   if (!c || check_params(p)) return CFD_ERR_ARG;
   if (p->nvar != c->P.nvar) return fail(c, CFD_ERR_ARG, "nvar cannot change on a live context");
-  if (p->turb != c->P.turb) c->phi_valid = false;
+  // the stored limiter belongs to the scheme and physics it was formed with
+  if (p->turb != c->P.turb || p->spatial != c->P.spatial || p->bcset != c->P.bcset) c->phi_valid = false;
   c->P = *p;
   fill_phys(c);
   return CFD_OK;
@@ -700,6 +711,22 @@ int cfd_ctx_get_params(const cfd_ctx* c, cfd_params* p) {
 int cfd_pitch(const cfd_ctx* c) { return c ? c->pitch : -1; }
 int cfd_ncells(const cfd_ctx* c) { return c ? c->N : -1; }
 void* cfd_stream(const cfd_ctx* c) { return c ? (void*)c->stream : nullptr; }
+void* cfd_ctx_own_stream(const cfd_ctx* c) { return c ? (void*)c->own_stream : nullptr; }
+
+int cfd_ctx_set_stream(cfd_ctx* c, void* stream) {
+  if (!c) return CFD_ERR_ARG;
+  hipStream_t s = (hipStream_t)stream;
+  if (s == c->stream) return CFD_OK;
+  // work already queued on the old stream finishes before the new one is used
+  HIP_TRY(c, hipStreamSynchronize(c->stream));
+  if (s) {
+    int dev = -1;
+    HIP_TRY(c, hipStreamGetDevice(s, &dev));
+    if (dev != c->device) return fail(c, CFD_ERR_ARG, "stream belongs to another device than the context");
+  }
+  c->stream = s;
+  return CFD_OK;
+}
 
 int cfd_set_face_flux(cfd_ctx* c, const unsigned char* use_alt) {
   if (!c) return CFD_ERR_ARG;
@@ -799,8 +826,7 @@ int cfd_jv_fd(cfd_ctx* c, const double* U6, const double* R6, const double* v6, 
 int cfd_jv_tangent(cfd_ctx* c, const double* U6, const double* v6, double* Jv6, int precision) {
   if (!c || !U6 || !v6 || !Jv6 || (precision != 64 && precision != 32)) return CFD_ERR_ARG;
   if (!c->dt_valid) return fail(c, CFD_ERR_STATE, "no local time step computed");
-  if (c->P.spatial == CFD_SECOND_ORDER_LIMITED && !c->phi_valid)
-    return fail(c, CFD_ERR_STATE, "limiter not formed: evaluate the residual first");
+  if (int rc = need_limiter(c)) return rc;
   int rc;
   if ((rc = upload_state(c, U6, c->vec[0]))) return rc;
   if ((rc = upload_state(c, v6, c->vec[1]))) return rc;
@@ -820,8 +846,7 @@ int cfd_jv_tangent(cfd_ctx* c, const double* U6, const double* v6, double* Jv6, 
 int cfd_vjp_adjoint(cfd_ctx* c, const double* U6, const double* w6, double* JTw6) {
   if (!c || !U6 || !w6 || !JTw6) return CFD_ERR_ARG;
   if (!c->dt_valid) return fail(c, CFD_ERR_STATE, "no local time step computed");
-  if (c->P.spatial == CFD_SECOND_ORDER_LIMITED && !c->phi_valid)
-    return fail(c, CFD_ERR_STATE, "limiter not formed: evaluate the residual first");
+  if (int rc = need_limiter(c)) return rc;
   int rc;
   if ((rc = upload_state(c, U6, c->vec[0]))) return rc;
   if ((rc = upload_state(c, w6, c->vec[1]))) return rc;
@@ -1311,6 +1336,10 @@ int cfd_solve_implicit(cfd_ctx* c, double* U6, const cfd_newton_opts* opts, cfd_
 int cfd_solve_unsteady(cfd_ctx* c, double* U6, double dt_phys, int n_steps, const cfd_gmres_opts* gopts,
                        int max_newton, double tolerance, double relax, double* hist_vm, int* hist_newton) {
   if (!c || !U6 || !(dt_phys > 0) || n_steps < 0) return CFD_ERR_ARG;
+  // hist_vm rows are max_newton long as the caller sized them: a default
+  // chosen here would overrun a buffer sized for the value passed
+  if (max_newton <= 0 && hist_vm)
+    return fail(c, CFD_ERR_ARG, "max_newton must be given (> 0) when hist_vm is requested");
   cfd_gmres_opts g;
   if (gopts) g = *gopts;
   else cfd_gmres_opts_default(&g);
@@ -1416,8 +1445,10 @@ int cfd_rk3_step(cfd_ctx* c, double* U6, double dt) {
   return download_state(c, U0, U6);
 }
 
-int cfd_wall_outputs(cfd_ctx* c, const double* U6, double* Cp, double* Cf, double* J) {
-  if (!c || !U6) return CFD_ERR_ARG;
+// Per reference face: Cp and Cf (k_wall), and the owner pressure of the
+// wall faces (tags 3 and 4, 0 elsewhere); host vectors in reference order.
+static int wall_faces(cfd_ctx* c, const double* U6, std::vector<double>& cp, std::vector<double>& cf,
+                      std::vector<double>& pw) {
   int rc = upload_state(c, U6, c->vec[0]);
   if (rc) return rc;
   double* out3 = c->Fm;  // 3 x F scratch (face buffer is free between residuals)
@@ -1425,16 +1456,67 @@ int cfd_wall_outputs(cfd_ctx* c, const double* U6, double* Cp, double* Cf, doubl
   std::vector<double> h((size_t)3 * c->F);
   HIP_TRY(c, hipMemcpyAsync(h.data(), out3, h.size() * 8, hipMemcpyDeviceToHost, c->stream));
   HIP_TRY(c, hipStreamSynchronize(c->stream));
-  std::vector<int> fpos(c->F);
-  for (int d = 0; d < c->F; ++d) fpos[c->face_order[d]] = d;
+  cp.assign(c->F, 0.0);
+  cf.assign(c->F, 0.0);
+  pw.assign(c->F, 0.0);
+  for (int d = 0; d < c->F; ++d) {
+    const int f = c->face_order[d];
+    cp[f] = h[d];
+    cf[f] = h[c->F + d];
+    pw[f] = h[2 * c->F + d];
+  }
+  return CFD_OK;
+}
+
+int cfd_wall_outputs(cfd_ctx* c, const double* U6, double* Cp, double* Cf, double* J) {
+  if (!c || !U6) return CFD_ERR_ARG;
+  std::vector<double> cp, cf, pw;
+  if (int rc = wall_faces(c, U6, cp, cf, pw)) return rc;
+  const cfd_mesh& m = *c->mesh;
   double j = 0.0;
   for (int f = 0; f < c->F; ++f) {
-    const int d = fpos[f];
-    if (Cp) Cp[f] = h[d];
-    if (Cf) Cf[f] = h[c->F + d];
-    j += h[2 * c->F + d];
+    if (Cp) Cp[f] = cp[f];
+    if (Cf) Cf[f] = cf[f];
+    if (m.nei[f] < 0 && m.tag[f] == 3) j += pw[f] * m.area[f];
   }
   if (J) *J = j;
+  return CFD_OK;
+}
+
+int cfd_drag(cfd_ctx* c, const double* U6, double* drag) {
+  if (!c || !U6 || !drag) return CFD_ERR_ARG;
+  std::vector<double> cp, cf, pw;
+  if (int rc = wall_faces(c, U6, cp, cf, pw)) return rc;
+  const cfd_mesh& m = *c->mesh;
+  // Solver.cpp:1310-1321: over the faces in order, Fx = -p A n_x on tag 3
+  double d = 0.0;
+  for (int f = 0; f < c->F; ++f)
+    if (m.tag[f] == 3) {
+      const double fx = -pw[f] * m.area[f] * m.nx[f];
+      d += fx;
+    }
+  *drag = d;
+  return CFD_OK;
+}
+
+int cfd_wall_pressure(cfd_ctx* c, const double* U6, double y_max, double x_scale, double p_divisor, int capacity,
+                      double* rows2, int* n_rows) {
+  if (!c || !U6 || !n_rows || capacity < 0 || (capacity > 0 && !rows2) || p_divisor == 0.0) return CFD_ERR_ARG;
+  std::vector<double> cp, cf, pw;
+  if (int rc = wall_faces(c, U6, cp, cf, pw)) return rc;
+  const cfd_mesh& m = *c->mesh;
+  // Solver.cpp:2043-2062: tag-3 faces whose owner centroid lies below y_max
+  std::vector<std::pair<double, double>> s;
+  for (int f = 0; f < c->F; ++f)
+    if (m.tag[f] == 3 && m.yc[m.own[f]] < y_max) s.emplace_back(m.xm[f] * x_scale, pw[f] / p_divisor);
+  // ascending x; equal x (the reference's order is unspecified there) by pressure
+  std::sort(s.begin(), s.end());
+  *n_rows = (int)s.size();
+  if ((int)s.size() > capacity) return fail(c, CFD_ERR_ARG, "wall pressure rows exceed the capacity");
+  for (size_t i = 0; i < s.size(); ++i) {
+    rows2[2 * i] = s[i].first;
+    rows2[2 * i + 1] = s[i].second;
+  }
   return CFD_OK;
 }
 
@@ -1475,6 +1557,7 @@ int cfd_dev_jv_fd(cfd_ctx* c, const double* U, const double* Rb, const double* v
 int cfd_dev_jv_tangent(cfd_ctx* c, const double* U, const void* v, void* Jv, int precision) {
   if (!c || !U || !v || !Jv || (precision != 64 && precision != 32)) return CFD_ERR_ARG;
   if (!c->dt_valid) return fail(c, CFD_ERR_STATE, "no local time step computed");
+  if (int rc = need_limiter(c)) return rc;
   launch_jv_tangent(c, U, v, Jv, precision);
   HIP_TRY(c, hipGetLastError());
   return CFD_OK;
@@ -1483,6 +1566,7 @@ int cfd_dev_jv_tangent(cfd_ctx* c, const double* U, const void* v, void* Jv, int
 int cfd_dev_vjp_adjoint(cfd_ctx* c, const double* U, const double* w, double* JTw) {
   if (!c || !U || !w || !JTw) return CFD_ERR_ARG;
   if (!c->dt_valid) return fail(c, CFD_ERR_STATE, "no local time step computed");
+  if (int rc = need_limiter(c)) return rc;
   launch_vjp(c, U, w, JTw);
   HIP_TRY(c, hipGetLastError());
   return CFD_OK;
@@ -1492,6 +1576,7 @@ int cfd_dev_gmres(cfd_ctx* c, const double* U, const double* R, const double* rh
                   const cfd_gmres_opts* opts, int* iters_out) {
   if (!c || !U || !R || !rhs || !x) return CFD_ERR_ARG;
   if (!c->dt_valid) return fail(c, CFD_ERR_STATE, "no local time step computed");
+  if (int rc = need_limiter(c)) return rc;
   cfd_gmres_opts o;
   if (opts) o = *opts;
   else cfd_gmres_opts_default(&o);

@@ -60,7 +60,8 @@ struct Krylov {
 
 struct cfd_ctx {
   int device = 0;
-  hipStream_t stream = nullptr;
+  hipStream_t stream = nullptr;      // the stream every launch and copy of the context goes to
+  hipStream_t own_stream = nullptr;  // created with the context; `stream` unless cfd_ctx_set_stream chose another
   cfd_params P{};
   cfd::Phys phys{};
   int N = 0, F = 0, pitch = 0, ncolors = 0;

@@ -2198,14 +2198,14 @@ __global__ __launch_bounds__(BS) void k_wall(DMesh m, Phys P, const double* __re
                                               double* __restrict__ out3) {
   const int f = blockIdx.x * BS + threadIdx.x;
   if (f >= m.F) return;
-  double cp = 0.0, cf = 0.0, jp = 0.0;
+  double cp = 0.0, cf = 0.0, pw = 0.0;
   const int tg = m.tag[f];
   if (m.nei[f] < 0 && (tg == 3 || tg == 4)) {
     const int o = m.own[f];
     const S6<double> L = ld<NV>(U, m.pitch, o);
     const double pr = pressure(P, L);
     cp = (pr - P.pexit) / qinf;
-    if (tg == 3) jp = pr * m.area[f];
+    pw = pr;
     if (tg == 4) {
       const double uc = L[1] / L[0], vc = L[2] / L[0];
       double tx = -m.ny[f], ty = m.nx[f];
@@ -2223,7 +2223,7 @@ __global__ __launch_bounds__(BS) void k_wall(DMesh m, Phys P, const double* __re
   }
   out3[f] = cp;
   out3[m.F + f] = cf;
-  out3[2 * m.F + f] = jp;
+  out3[2 * m.F + f] = pw;
 }
 
 // ---- Krylov vector kernels (basis element type TV: double or float) -------------------

@@ -1235,6 +1235,27 @@ void orc_face_flux(orc_solver* s, const double* U6, double* F6, int update) {
   s->s.D.residual(Solver::S(U6), R.data(), update != 0, Solver::S(F6));
 }
 
+// SST per-cell quantities at U6 (turbulent parameter sets): mu_t, F1 and
+// the source densities S_k, S_omega (per unit volume), from the Green-Gauss
+// gradients of U6 and the wall distance
+void orc_turb_cells(orc_solver* s, const double* U6, double* out4) {
+  Solver& S = s->s;
+  const St<double>* U = Solver::S(U6);
+  S.D.gradients(U);
+  for (int c = 0; c < S.N(); ++c) {
+    const auto t = S.D.turb_cell(U[c], S.D.grads[c], S.wdist[c]);
+    out4[4 * c] = t.mut;
+    out4[4 * c + 1] = t.F1;
+    out4[4 * c + 2] = t.Sk;
+    out4[4 * c + 3] = t.Sw;
+  }
+}
+// 14 gradient slots per cell after the last residual (with SST: k, omega last)
+void orc_get_gradients14(orc_solver* s, double* g14) {
+  for (int c = 0; c < s->s.N(); ++c)
+    for (int k = 0; k < 14; ++k) g14[14 * c + k] = s->s.D.grads[c][k];
+}
+
 void orc_get_gradients(orc_solver* s, double* g10) {
   for (int c = 0; c < s->s.N(); ++c)
     for (int k = 0; k < 10; ++k) g10[10 * c + k] = s->s.D.grads[c][k];

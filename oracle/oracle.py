@@ -16,6 +16,10 @@ import numpy as np
 HERE = os.path.dirname(os.path.abspath(__file__))
 ORACLE_LIB = os.path.join(HERE, "liboracle.so")
 REF_LIB = os.path.join(HERE, "_ref", "libref.so")
+# variants of the reference built by `make ref-variants` (oracle/Makefile):
+# "visc" viscous flux + bump BCs, "plate" viscous + flat-plate BCs,
+# "super" inviscid + supersonic BCs
+REF_VARIANTS = ("visc", "plate", "super")
 REFERENCE = "/root/reference"
 
 _DP = C.POINTER(C.c_double)
@@ -32,14 +36,15 @@ def _ip(a):
 
 
 def build(ref: bool = True):
-    """make liboracle.so (always) and _ref/libref.so (when the reference is present)."""
+    """make liboracle.so (always) and, when the reference is present, its
+    compiled builds _ref/libref.so and _ref/libref_<variant>.so."""
     subprocess.run(["make", "-s", "-C", HERE, "liboracle.so"], check=True)
     if ref and os.path.isdir(os.path.join(REFERENCE, "src")):
-        subprocess.run(["make", "-s", "-C", HERE, "ref"], check=True)
+        subprocess.run(["make", "-s", "-C", HERE, "ref", "ref-variants"], check=True)
 
 
 _orc = None
-_ref = None
+_ref = {}
 
 
 def oracle_lib():
@@ -66,6 +71,8 @@ def oracle_lib():
             "orc_residual_cs": (None, [_VP, _DP, _DP, C.c_double, _DP, _DP, C.c_int]),
             "orc_face_flux": (None, [_VP, _DP, _DP, C.c_int]),
             "orc_get_gradients": (None, [_VP, _DP]),
+            "orc_get_gradients14": (None, [_VP, _DP]),
+            "orc_turb_cells": (None, [_VP, _DP, _DP]),
             "orc_get_limiters": (None, [_VP, _DP]),
             "orc_set_limiters": (None, [_VP, _DP]),
             "orc_local_timestep": (None, [_VP, _DP, C.c_double]),
@@ -93,16 +100,20 @@ def oracle_lib():
     return _orc
 
 
-def ref_available() -> bool:
-    return os.path.exists(REF_LIB)
+def ref_lib_path(variant: str | None = None) -> str:
+    return REF_LIB if variant is None else os.path.join(HERE, "_ref", f"libref_{variant}.so")
 
 
-def ref_lib():
-    global _ref
-    if _ref is None:
-        if not ref_available():
-            raise FileNotFoundError(REF_LIB)
-        lib = C.CDLL(REF_LIB)
+def ref_available(variant: str | None = None) -> bool:
+    return os.path.exists(ref_lib_path(variant))
+
+
+def ref_lib(variant: str | None = None):
+    if variant not in _ref:
+        path = ref_lib_path(variant)
+        if not os.path.exists(path):
+            raise FileNotFoundError(path)
+        lib = C.CDLL(path)
         sig = {
             "ref_mesh_load": (_VP, [C.c_char_p]),
             "ref_mesh_destroy": (None, [_VP]),
@@ -134,13 +145,16 @@ def ref_lib():
             "ref_set_nu": (None, [_VP, C.c_double]),
             "ref_kinematic_viscosity": (None, [_VP, C.c_double, C.c_double]),
             "ref_get_nu": (C.c_double, [_VP]),
+            "ref_skin_friction": (None, [_VP, _DP, C.c_char_p]),
+            "ref_wall_pressure": (None, [_VP, _DP, C.c_char_p]),
+            "ref_drag": (C.c_double, [_VP, _DP]),
         }
         for name, (res, args) in sig.items():
             fn = getattr(lib, name)
             fn.restype = res
             fn.argtypes = args
-        _ref = lib
-    return _ref
+        _ref[variant] = lib
+    return _ref[variant]
 
 
 def _faces_from(lib_fn, h, F):
@@ -260,6 +274,17 @@ class OracleSolver:
         self.lib.orc_get_limiters(self.h, _dp(p))
         return p
 
+    def gradients14(self):
+        g = np.zeros((self.N, 14))
+        self.lib.orc_get_gradients14(self.h, _dp(g))
+        return g
+
+    def turb_cells(self, U):
+        """SST per cell at U: (mu_t, F1, S_k, S_omega) with S per unit volume."""
+        out = np.zeros((self.N, 4))
+        self.lib.orc_turb_cells(self.h, _dp(np.ascontiguousarray(U, np.float64)), _dp(out))
+        return out
+
     def set_limiters(self, phi):
         phi = np.ascontiguousarray(phi, np.float64)
         self.lib.orc_set_limiters(self.h, _dp(phi))
@@ -353,8 +378,9 @@ class OracleSolver:
 class RefMesh:
     """Mesh::load of the compiled reference (this container only)."""
 
-    def __init__(self, path: str):
-        self.lib = ref_lib()
+    def __init__(self, path: str, variant: str | None = None):
+        self.variant = variant
+        self.lib = ref_lib(variant)
         self.h = self.lib.ref_mesh_load(path.encode())
         nn, nc, nf = C.c_int(), C.c_int(), C.c_int()
         self.lib.ref_mesh_sizes(self.h, C.byref(nn), C.byref(nc), C.byref(nf))
@@ -374,10 +400,11 @@ class RefMesh:
 
 
 class RefSolver:
-    """The reference Solver (inviscid, bump boundary conditions, as shipped)."""
+    """The reference Solver: as shipped (inviscid, bump boundary conditions)
+    or, with the mesh's variant, one of the commented-in variants."""
 
     def __init__(self, mesh: RefMesh, M, P, T, flux, spatial, nu=-1.0, mode=0):
-        self.lib = ref_lib()
+        self.lib = ref_lib(mesh.variant)
         self.N = mesh.n_cells
         self.mesh = mesh
         self.h = self.lib.ref_solver_create(mesh.h, mode, M, P, T, nu, flux, spatial)
@@ -463,3 +490,15 @@ class RefSolver:
 
     def set_nu(self, nu):
         self.lib.ref_set_nu(self.h, nu)
+
+    def skin_friction_csv(self, U, path):
+        """Solver::extractSkinFriction: writes x,Cf,Cf_theory rows to path."""
+        self.lib.ref_skin_friction(self.h, _dp(np.ascontiguousarray(U, np.float64)), path.encode())
+
+    def wall_pressure_csv(self, U, path):
+        """Solver::extractWallPressure: writes x_in, p_psi rows to path."""
+        self.lib.ref_wall_pressure(self.h, _dp(np.ascontiguousarray(U, np.float64)), path.encode())
+
+    def drag(self, U):
+        """The value Solver::calculateDrag prints (read at 17 digits)."""
+        return self.lib.ref_drag(self.h, _dp(np.ascontiguousarray(U, np.float64)))

@@ -191,6 +191,43 @@ void ref_save_vtk(void* mp, const double* U6, int step) {
   const Mesh* m = static_cast<Mesh*>(mp);
   saveResults(*m, to_states(U6, m->cells.size()), step);
 }
+// Solver::extractSkinFriction(path) on U6: the reference writes x, Cf and
+// the Blasius value per no-slip face to a CSV file (6 significant digits).
+void ref_skin_friction(void* sp, const double* U6, const char* path) {
+  Solver* s = static_cast<Solver*>(sp);
+  s->U_current = to_states(U6, s->mesh.cells.size());
+  std::ostringstream cap;
+  std::streambuf* old = std::cout.rdbuf(cap.rdbuf());
+  s->extractSkinFriction(path);
+  std::cout.rdbuf(old);
+}
+// Solver::extractWallPressure(path) on U6 (CSV rows x_in, p_psi).
+void ref_wall_pressure(void* sp, const double* U6, const char* path) {
+  Solver* s = static_cast<Solver*>(sp);
+  s->U_current = to_states(U6, s->mesh.cells.size());
+  std::ostringstream cap;
+  std::streambuf* old = std::cout.rdbuf(cap.rdbuf());
+  s->extractWallPressure(path);
+  std::cout.rdbuf(old);
+}
+// Solver::calculateDrag on U6 prints " Drag :<value>"; the value is read
+// back from what it prints.  The stream precision is raised to 17 digits
+// around the call, so the printed value is the double the reference summed.
+double ref_drag(void* sp, const double* U6) {
+  Solver* s = static_cast<Solver*>(sp);
+  s->U_current = to_states(U6, s->mesh.cells.size());
+  std::ostringstream cap;
+  cap.precision(17);
+  std::streambuf* old = std::cout.rdbuf(cap.rdbuf());
+  const std::streamsize prec = std::cout.precision(17);
+  s->calculateDrag();
+  std::cout.precision(prec);
+  std::cout.rdbuf(old);
+  const std::string t = cap.str();
+  const std::string key = "Drag :";
+  const size_t k = t.find(key);
+  return k == std::string::npos ? -1.0 : std::strtod(t.c_str() + k + key.size(), nullptr);
+}
 void ref_set_nu(void* sp, double nu) { static_cast<Solver*>(sp)->nu = nu; }
 void ref_kinematic_viscosity(void* sp, double Re, double L) { static_cast<Solver*>(sp)->setKinematicViscosity(Re, L); }
 double ref_get_nu(void* sp) { return static_cast<Solver*>(sp)->nu; }

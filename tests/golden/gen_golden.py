@@ -17,11 +17,31 @@ reference's own output (Solver.cpp compiled with its makefile flags) on:
                          two BDF1 dual-time steps (Solver::solveUnsteady, dt
                          1e-5) from a seeded perturbed state, the entropy
                          error it prints, the boundary-layer profile rows it
-                         writes, and the SHA-256 of main.cpp's VTK output.
+                         writes, and the SHA-256 of main.cpp's VTK output;
+  ref_variants.npz       the reference's commented-in variants, compiled by
+                         `make -C oracle ref-variants` from Solver.cpp with
+                         the edits oracle/ref_variant.sh lists:
+                           plate_*  viscous + flat-plate BCs (Solver.cpp:787,
+                                    1088-1139) on the shipped mesh
+                                    laminar_flat_plate: residuals of 2
+                                    schemes, limiter, local time step,
+                                    diagonal preconditioner, FD Jv, a 10-step
+                                    Newton trajectory, the extractSkinFriction
+                                    and extractWallPressure CSV rows;
+                           super_*  supersonic BCs (Solver.cpp:973-1011) on
+                                    the synthetic wedge above: residuals of 2
+                                    schemes, two explicit steps, calculateDrag,
+                                    extractWallPressure rows;
+                           visc_*   viscous + bump BCs on bump_1230_th:
+                                    residuals of 2 schemes, gradients.
+                         Each state is smooth (shear layer, waves) with a
+                         seeded 2 % perturbation, so every viscous term is
+                         active.
 
 The shipped meshes have CRLF line ends, which the reference's readGmsh only
 accepts in Windows text mode; a CR-stripped temporary copy is fed to it.
-Usage:  python tests/golden/gen_golden.py
+Usage:  python tests/golden/gen_golden.py [bump] [wedge] [next] [variants]
+        (no argument: all of them)
 """
 from __future__ import annotations
 
@@ -34,7 +54,7 @@ import numpy as np
 ROOT = os.path.dirname(os.path.dirname(os.path.dirname(os.path.abspath(__file__))))
 sys.path.insert(0, ROOT)
 
-from mycode_amd import cfd  # noqa: E402
+from mycode_amd import cfd, configs  # noqa: E402
 from oracle import oracle  # noqa: E402
 
 OUT = os.path.dirname(os.path.abspath(__file__))
@@ -140,16 +160,143 @@ def gen_next(tmp: str):
     np.savez_compressed(os.path.join(OUT, "ref_bump_1230_next.npz"), **out)
 
 
+def smooth_state(p, xc, yc, seed, noise=0.02):
+    """A state with a wall shear layer, pressure and density waves and a
+    seeded multiplicative perturbation of each primitive."""
+    rng = np.random.default_rng(seed)
+    g = p.gamma
+    fT = 1.0 + 0.5 * (g - 1.0) * p.M_inf ** 2
+    Ts = p.T_total_inlet / fT
+    rho0 = p.P_static_exit / (p.R_gas * Ts)
+    u0 = p.M_inf * np.sqrt(g * p.R_gas * Ts)
+    N = len(xc)
+    ly = max(float(yc.max() - yc.min()), 1e-9)
+    rho = rho0 * (1 + 0.05 * np.sin(3 * xc) * np.cos(2 * yc / ly)) * (1 + noise * rng.standard_normal(N))
+    u = u0 * np.tanh((yc - yc.min()) / (0.05 * ly) + 0.05) * (1 + noise * rng.standard_normal(N))
+    v = 0.02 * u0 * np.sin(4 * xc) * (1 + noise * rng.standard_normal(N))
+    pr = p.P_static_exit * (1 + 0.03 * np.cos(2 * xc + yc)) * (1 + noise * rng.standard_normal(N))
+    U = np.zeros((N, 6))
+    U[:, 0], U[:, 1], U[:, 2] = rho, rho * u, rho * v
+    U[:, 3] = pr / (g - 1) + 0.5 * rho * (u * u + v * v)
+    return U
+
+
+def variant_params(bcset, viscous, M, nu):
+    p = cfd.default_params()
+    cfd.flow_conditions(p, M, 101325.0, 300.0)
+    p.nu = nu
+    p.bcset, p.viscous, p.nvar = bcset, viscous, 4
+    return p
+
+
+def _csv_rows(path, ncol):
+    return np.loadtxt(path, delimiter=",", skiprows=1, ndmin=2).reshape(-1, ncol)
+
+
+def direction(U):
+    """The Jv direction of the variant fixtures, formed from the state
+    (exact products, so a test rebuilds it bit for bit)."""
+    return U[:, :4] * np.array([1e-2, -3e-2, 5e-2, -1e-2])
+
+
+def gen_variants(tmp: str):
+    out = {}
+
+    def load(var, path, prefix):
+        rm = oracle.RefMesh(path, var)
+        a = mesh_arrays(path, rm)
+        for k in ("x", "y", "cell_ptr", "cell_nodes", "bedges"):
+            out[f"{prefix}_{k}"] = a[k]
+        return rm
+
+    def shipped(name):
+        return lf_copy(os.path.join(oracle.REFERENCE, "mesh", name), tmp)
+
+    # viscous + flat-plate BCs, laminar_flat_plate (Mach 0.2, Re_L 1e5 on L = 1)
+    rm = load("plate", shipped("laminar_flat_plate"), "plate")
+    c = rm.cells()
+    nu = 0.2 * np.sqrt(1.4 * 287.0 * 300.0) / 1e5  # u_inf L / Re at T = 300 K
+    U = smooth_state(variant_params(cfd.BC_PLATE, 1, 0.2, nu), c["x_c"], c["y_c"], 11)
+    out.update(plate_M=0.2, plate_nu=nu, plate_U=U[:, :4])
+    for fl, sp in ((1, 1), (2, 2)):
+        rs = oracle.RefSolver(rm, 0.2, 101325.0, 300.0, fl, sp, nu=nu)
+        R = rs.compute_residuals(U)
+        out[f"plate_R_{fl}{sp}"] = R[:, :4]
+        if (fl, sp) == (2, 2):
+            out["plate_phi_22"] = rs.limiters()
+        if (fl, sp) == (1, 1):
+            rs.compute_local_time_step(U, 5.0)
+            out["plate_dt_cfl5"] = rs.dt_local()
+            out["plate_D_cfl5"] = rs.diagonal_preconditioner(U)
+            v = np.zeros_like(U)
+            v[:, :4] = direction(U)
+            out["plate_Jv_11"] = rs.apply_jacobian(U, R, v)[:, :4]
+            cf = os.path.join(tmp, "cf.csv")
+            rs.skin_friction_csv(U, cf)
+            out["plate_cf_rows"] = _csv_rows(cf, 3)
+            wp = os.path.join(tmp, "wp.csv")
+            rs.wall_pressure_csv(U, wp)
+            out["plate_wp_rows"] = _csv_rows(wp, 2)
+    rs = oracle.RefSolver(rm, 0.2, 101325.0, 300.0, 1, 0, nu=nu)
+    U0 = np.tile(configs.freestream_state(variant_params(cfd.BC_PLATE, 1, 0.2, nu)), (rm.n_cells, 1))
+    out["plate_newton_steps"] = 10
+    out["plate_newton_U"] = rs.solve_implicit(U0, 1e-12, 10)[:, :4]
+
+    # supersonic BCs on the synthetic wedge of configuration 1 at quarter
+    # scale, written by cfd_mesh_write_gmsh and read by Mesh::load
+    m = cfd.Mesh.generate(1, 0.25)
+    path = os.path.join(tmp, "wedge_q.msh")
+    m.write_gmsh(path)
+    rm = load("super", path, "super")
+    c = rm.cells()
+    U = smooth_state(variant_params(cfd.BC_SUPERSONIC, 0, 2.5, 0.01), c["x_c"], c["y_c"], 17)
+    out.update(super_M=2.5, super_U=U[:, :4])
+    for fl, sp in ((2, 2), (0, 0)):
+        rs = oracle.RefSolver(rm, 2.5, 101325.0, 300.0, fl, sp)
+        out[f"super_R_{fl}{sp}"] = rs.compute_residuals(U)[:, :4]
+    rs = oracle.RefSolver(rm, 2.5, 101325.0, 300.0, 2, 2)
+    dt = 2e-7
+    U1 = rs.time_step_explicit(U, dt)
+    out["super_dt"] = dt
+    out["super_explicit_U1"] = U1[:, :4]
+    out["super_explicit_U2"] = rs.time_step_explicit(0.75 * U + 0.25 * U1, dt)[:, :4]
+    out["super_drag"] = rs.drag(U)
+    wp = os.path.join(tmp, "wp.csv")
+    rs.wall_pressure_csv(U, wp)
+    out["super_wp_rows"] = _csv_rows(wp, 2)
+
+    # viscous + bump BCs, bump_1230_th (Mach 0.5, the default nu = 0.01)
+    rm = load("visc", shipped("bump_1230_th"), "visc")
+    c = rm.cells()
+    U = smooth_state(variant_params(cfd.BC_BUMP, 1, 0.5, 0.01), c["x_c"], c["y_c"], 19)
+    out.update(visc_M=0.5, visc_U=U[:, :4])
+    for fl, sp in ((1, 1), (2, 2)):
+        rs = oracle.RefSolver(rm, 0.5, 101325.0, 300.0, fl, sp)
+        out[f"visc_R_{fl}{sp}"] = rs.compute_residuals(U)[:, :4]
+        if (fl, sp) == (1, 1):
+            out["visc_grad_11"] = rs.gradients()
+    # calculateDrag of the reference as shipped, on the same state
+    rs = oracle.RefSolver(oracle.RefMesh(shipped("bump_1230_th")), 0.5, 101325.0, 300.0, 2, 2)
+    out["bump_drag"] = rs.drag(U)
+    np.savez_compressed(os.path.join(OUT, "ref_variants.npz"), **out)
+
+
 def main():
-    if not oracle.ref_available():
+    if not (oracle.ref_available() and all(oracle.ref_available(v) for v in oracle.REF_VARIANTS)):
         oracle.build(ref=True)
     with tempfile.TemporaryDirectory() as tmp:
         cwd = os.getcwd()
         os.chdir(tmp)  # the reference writes results/residual_history.csv only if results/ exists
         try:
-            gen_bump(tmp)
-            gen_wedge(tmp)
-            gen_next(tmp)
+            which = sys.argv[1:] or ["bump", "wedge", "next", "variants"]
+            if "bump" in which:
+                gen_bump(tmp)
+            if "wedge" in which:
+                gen_wedge(tmp)
+            if "next" in which:
+                gen_next(tmp)
+            if "variants" in which:
+                gen_variants(tmp)
         finally:
             os.chdir(cwd)
     for f in sorted(os.listdir(OUT)):
