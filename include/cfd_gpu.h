@@ -381,6 +381,17 @@ int cfd_dev_vjp_adjoint(cfd_ctx* ctx, const double* U, const double* w, double* 
 int cfd_dev_gmres(cfd_ctx* ctx, const double* U, const double* R, const double* rhs, double* x,
                   const cfd_gmres_opts* opts, int* iters_out);
 int cfd_dev_synchronize(cfd_ctx* ctx);
+/* cfd_norm_vw (Solver::norm_L2_VolumeWeighted, Solver.cpp:1280-1303) of the
+ * device SoA vector R, written to out_dev[0] (device memory) on the context
+ * stream, allocating nothing.  No counterpart in the reference: it is for
+ * callers that cannot allocate (code under HIP graph capture, tight serving
+ * loops).  Its scratch (the reduction's first-stage partial sums and totals,
+ * 16.1 KiB) is a static device array the library holds on each GPU until the
+ * process ends; all calls on one GPU share it, so calls that overlap on one
+ * GPU (several streams or threads) race.  The sums are the same as
+ * cfd_norm_vw's (same grid, same order).  Vectors of nvar x pitch beyond
+ * 2^31 - 1 elements are rejected (CFD_ERR_ARG). */
+int cfd_dev_norm_vw_noalloc(cfd_ctx* ctx, const double* R, double* out_dev);
 
 /* ---- measurement -------------------------------------------------------- */
 /* Runs `reps` residual evaluations on device vectors U -> R and records HIP

@@ -146,6 +146,7 @@ SIGNATURES = [
     ("cfd_dev_local_timestep", C.c_int, [_VP, _VP, C.c_double]),
     ("cfd_dev_vjp_adjoint", C.c_int, [_VP, _VP, _VP, _VP]),
     ("cfd_dev_synchronize", C.c_int, [_VP]),
+    ("cfd_dev_norm_vw_noalloc", C.c_int, [_VP, _VP, _VP]),
     ("cfd_dev_time_residual", C.c_int, [_VP, _VP, _VP, C.c_int, C.c_int, _DP, _IP]),
 ]
 
@@ -657,6 +658,12 @@ class Solver:
         self._c(self.lib.cfd_dev_gmres(self._h, C.c_void_p(U_ptr), C.c_void_p(R_ptr), C.c_void_p(rhs_ptr),
                                        C.c_void_p(x_ptr), C.byref(opts), C.byref(it)), "cfd_dev_gmres")
         return it.value
+
+    def dev_norm_vw_noalloc(self, R_ptr: int, out_ptr: int):
+        """Volume-weighted norm of a device SoA vector into out_ptr[0] (device),
+        with no allocation (static per-GPU scratch shared by all calls)."""
+        self._c(self.lib.cfd_dev_norm_vw_noalloc(self._h, C.c_void_p(R_ptr), C.c_void_p(out_ptr)),
+                "cfd_dev_norm_vw_noalloc")
 
     def synchronize(self):
         self._c(self.lib.cfd_dev_synchronize(self._h), "cfd_dev_synchronize")

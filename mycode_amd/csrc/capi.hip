@@ -427,6 +427,7 @@ int cfd_ctx_create(const cfd_mesh* m, const cfd_params* p, cfd_ctx** out) {
   c->P = *p;
   fill_phys(c);
   HIP_TRY(c, hipGetDevice(&c->device));
+  if (int rc = noalloc_init(c)) return rc;
   HIP_TRY(c, hipStreamCreateWithFlags(&c->own_stream, hipStreamNonBlocking));
   c->stream = c->own_stream;
   const int N = m->ncells(), F = m->nfaces();

@@ -124,6 +124,11 @@ struct cfd_ctx {
 };
 
 namespace cfd {
+// Most blocks of a reduction's first stage (one partial sum per block and sum).
+constexpr int RED_BLOCKS = 1024;
+// Per-device address of the library's static scratch array (noalloc.hip);
+// called by cfd_ctx_create with the context's device current.
+int noalloc_init(cfd_ctx* c);
 // ---- launchers (kernels.hip) -------------------------------------------------
 // All launch on ctx->stream.  Vectors are SoA (nv x pitch, RCM order).
 void launch_residual(cfd_ctx* c, const double* U, double* R, bool update_limiters);
@@ -138,6 +143,11 @@ void launch_diag_precond(cfd_ctx* c, const double* U, double* D);
 // RED_VW4: the same over rows 0-3 (rho, rho u, rho v, rho E)
 enum RedKind { RED_L2 = 0, RED_DOT = 1, RED_VW = 2, RED_VW4 = 3 };
 void launch_reduce(cfd_ctx* c, int kind, const double* a, const double* b, int slot);
+// RED_VW of the FP64 SoA vector a with its first-stage partials in part
+// (2 RED_BLOCKS doubles) and the two sums in sums[0..1] (the same grid and
+// order as launch_reduce), then out[0] = sqrt(sums[0] / sums[1]) (0 if the
+// volume sum is 0); nothing is allocated
+void launch_norm_vw_into(cfd_ctx* c, const double* a, double* part, double* sums, double* out);
 void launch_fill(cfd_ctx* c, double* x, double v, size_t n);
 // FD Jv: Jv = (R(U + eps v) - Rb) / eps + (vol/dt) v, eps from scal[su] (|U|^2) and scal[sv] (|v|^2)
 void launch_jv_fd(cfd_ctx* c, const double* U, const double* Rb, const double* v, double* Jv, int su, int sv,

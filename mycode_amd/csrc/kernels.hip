@@ -1940,8 +1940,7 @@ __global__ __launch_bounds__(BS) void k_line_gather(int N, int p, const int* __r
   for (int i = 0; i < NV; ++i) z[i * p + c] = zt[i * p + t];
 }
 
-// ---- reductions (deterministic: fixed grid, fixed tree) ------------------------------
-constexpr int RED_BLOCKS = 1024;
+// ---- reductions (deterministic: fixed grid, fixed tree; RED_BLOCKS in ctx.hpp) -------
 
 __device__ __forceinline__ double wave_sum(double v) {
 #pragma unroll
@@ -2624,6 +2623,18 @@ static void reduce_t(cfd_ctx* c, int kind, const TA* a, const TB* b, int slot) {
   });
   const int nsum = kind == RED_VW || kind == RED_VW4 ? 2 : 1;
   hipLaunchKernelGGL(k_red_final, dim3(nsum), dim3(BS), 0, c->stream, c->red, nb, nsum, c->scal + slot);
+}
+
+__global__ void k_vw_finish(const double* __restrict__ sums, double* __restrict__ out) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) out[0] = sums[1] == 0.0 ? 0.0 : sqrt(sums[0] / sums[1]);
+}
+
+void launch_norm_vw_into(cfd_ctx* c, const double* a, double* part, double* sums, double* out) {
+  const int nb = red_grid(c->N);
+  NV_DISPATCH(c->P.nvar, hipLaunchKernelGGL((k_red<NVt, RED_VW, double, double>), dim3(nb), dim3(BS), 0,
+                                            c->stream, c->N, c->pitch, a, a, c->d_vol, part));
+  hipLaunchKernelGGL(k_red_final, dim3(2), dim3(BS), 0, c->stream, part, nb, 2, sums);
+  hipLaunchKernelGGL(k_vw_finish, dim3(1), dim3(64), 0, c->stream, sums, out);
 }
 
 void launch_reduce(cfd_ctx* c, int kind, const double* a, const double* b, int slot) {
