@@ -3,35 +3,50 @@ turbulent flat plate (the configuration the headline metric is quoted on).
 
 Workload: turbulent flat plate, Mach 0.2, Re = 5e6, 1200 x 1000 stretched
 quads with a seeded 10 % split into triangles (1,320,146 cells, 2,522,346
-faces), Roe + viscous + SST k-omega, second-order central, six conserved
-variables, plate boundary conditions.  One "step" is one full FP64 residual
-evaluation on device-resident state.
+faces), Roe + viscous + SST k-omega with k and omega convected first-order
+upwind, mean flow second-order central, six conserved variables, plate
+boundary conditions.  One "step" is one full FP64 residual evaluation on
+device-resident state.
 
 The JSON line carries:
   value              Mcell-residuals/s (cells x residuals / wall second) over
                      the timed region: `steps` residuals launched back to back;
-  roofline           the dominant residual kernel: algorithmic bytes of its own
-                     inputs and outputs per residual over its HIP-event time,
-                     from a second pass of `steps` residuals right after the
-                     timed one with HIP events between the stages on the
-                     context stream (the events add about 25 us per residual
-                     of wall time, which is why the timed region has none);
-  roofline_residual  the whole residual against B_res = N(16 n_v + 24) + 48 F
-                     over the timed region's wall time per residual;
-  tangent_products   forward tangent products J v per second on the same
-                     state (FP32 and FP64 directions) and reverse products
-                     J^T w (discrete adjoint, FP64), beside the residual;
+  roofline           the dominant residual kernel against the compulsory
+                     bytes of one residual, B_res = N(16 n_v + 24) + 48 F
+                     (U read and R written once, cell and face geometry),
+                     over that kernel's HIP-event time per launch, from a
+                     second pass of `steps` residuals with HIP events between
+                     the stages on the context stream (the events add about
+                     25 us per residual of wall time, which is why the timed
+                     region has none); `traffic` is that kernel's HBM bytes
+                     per launch from the committed counter summary;
+  roofline_residual  the whole residual against B_res over the timed
+                     region's wall time per residual;
+  stage_traffic      each residual kernel against the compulsory bytes of its
+                     own inputs and outputs (gradients and face buffer
+                     included), the view that locates wasted re-reads;
+  tangent_products   forward tangent products J v (FP32 and FP64) and reverse
+                     products J^T w (FP64) on the same state, beside the
+                     residual;
+  other_configs      the same residual, roofline and product measurements on
+                     configuration 4 (bump, 824,378 cells) and configuration 5
+                     (shock / boundary-layer interaction, 4,399,865 cells);
   jfnk               FD Jv + FP64 Krylov against the tangent Jv with FP32
-                     matvec, preconditioner and basis (line-implicit block
-                     Gauss-Seidel in both): milliseconds per GMRES iteration
-                     on this workload (and the FP32 / FD solution
-                     differences), the wall time to a 1e-10 drop of the
-                     volume-weighted residual norm on the laminar flat plate
-                     (GMRES(40)) and on this workload (GMRES(60), one
-                     restart), with the largest Cp and Cf differences
-                     between the two converged solutions;
-  cpu_baseline       the CPU restatement of the reference residual (g++ -O2,
-                     one core) on the same mesh and state.
+                     matvec, preconditioner and basis: per GMRES iteration on
+                     this workload; the wall time to a 1e-10 drop of the
+                     volume-weighted residual norm on this workload and on the
+                     laminar flat plate, with the per-equation drops and the
+                     Cp / Cf differences of the two solutions; and the
+                     laminar plate as configuration 2 states it (GMRES(30),
+                     block-Jacobi, FD against the tangent, all FP64);
+  design_gradient    configuration 4 at full size: steady state, discrete
+                     adjoint and the gradient of the bump wall-pressure
+                     integral for 64 bump-node displacements, checked against
+                     central differences of re-converged flows (step 1e-6);
+  cpu_baseline       the CPU restatement of the reference (g++ -O2, one core)
+                     on the same mesh and state: residual and FD Jv times, and
+                     the reference algorithm's Newton-step time extrapolated
+                     from them; host core count and CPU model.
 """
 from __future__ import annotations
 
@@ -51,7 +66,10 @@ CONFIG = 3
 WORKLOADS = {
     2: "laminar flat plate M0.2 Re_L=1e5, 400x160 stretched quads 20% split, Roe + viscous, 2nd-order central",
     3: "turbulent flat plate M0.2 Re=5e6, 1200x1000 stretched quads 10% split, Roe + viscous + SST k-omega, "
-       "2nd-order central",
+       "mean flow 2nd-order central, k and omega convected 1st-order upwind",
+    4: "inviscid bump M0.7, 3x1 channel, 5% circular arc, 1280x560 quads 15% split, HLLC, 2nd-order central",
+    5: "shock / boundary-layer interaction M2.0 Re_x=3e5, 8-degree incident shock, 2000x2000 stretched quads "
+       "10% split, LLF in the shock band and HLLC elsewhere, viscous, Barth-Jespersen limited",
 }
 
 
@@ -80,19 +98,59 @@ def stage_bytes(N, F, nv, assembly, limiter, turb):
     return grad, flux, asm
 
 
+def host_info():
+    """Host cores (those this process may run on) and CPU model."""
+    try:
+        ncores = len(os.sched_getaffinity(0))
+    except AttributeError:
+        ncores = os.cpu_count()
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return ncores, model
+
+
 def cpu_baseline(mesh, p, U, budget_s=25.0):
+    """The oracle (the CPU restatement of the reference, g++ -O2, one core) on
+    the workload's mesh and state: the residual (median of up to 3) and the FD
+    Jv (median of up to 2, Solver::applyJacobian), and from them the time of
+    one Newton step of the reference algorithm (Solver.cpp:1696-1799: a local
+    time step, a residual and GMRES(40) with one FD Jv per iteration, taken
+    at its 40-iteration cap) and the extrapolated time of `steps` such steps."""
     from oracle import oracle
     o = oracle.OracleSolver(oracle.OracleMesh.from_mesh(mesh), p)
-    times = []
     t_start = time.perf_counter()
-    while len(times) < 3 and time.perf_counter() - t_start < budget_s:
+    res_t = []
+    while len(res_t) < 3 and time.perf_counter() - t_start < budget_s:
         t0 = time.perf_counter()
-        o.compute_residuals(U)
-        times.append(time.perf_counter() - t0)
-    t = float(np.median(times))
-    return {"value": mesh.n_cells / t / 1e6, "unit": "Mcell-residuals/s", "cores": 1, "kind": "port",
-            "sample": f"{len(times)} full residual evaluations of the same {mesh.n_cells}-cell mesh and state "
-                      f"(median {t:.3f} s), oracle/cfd_oracle.cpp built g++ -O2"}
+        R = o.compute_residuals(U)
+        res_t.append(time.perf_counter() - t0)
+    o.compute_local_time_step(U, 10.0)
+    v = np.random.default_rng(3).standard_normal(U.shape) * np.abs(U).mean(0)
+    v[:, p.nvar:] = 0.0
+    jv_t = []
+    while len(jv_t) < 2 and time.perf_counter() - t_start < 2 * budget_s:
+        t0 = time.perf_counter()
+        o.apply_jacobian(U, R, v)
+        jv_t.append(time.perf_counter() - t0)
+    t_res, t_jv = float(np.median(res_t)), float(np.median(jv_t))
+    ncores, model = host_info()
+    step = 2 * t_res + 40 * t_jv
+    return {"value": mesh.n_cells / t_res / 1e6, "unit": "Mcell-residuals/s", "cores": 1, "kind": "port",
+            "sample": f"{len(res_t)} full residual evaluations and {len(jv_t)} FD Jv products of the same "
+                      f"{mesh.n_cells}-cell mesh and state (medians {t_res:.3f} s and {t_jv:.3f} s), "
+                      "oracle/cfd_oracle.cpp built g++ -O2, one thread",
+            "residual_s": t_res, "fd_jv_s": t_jv,
+            "newton_step_s_estimate": step,
+            "newton_step_note": "reference algorithm: local time step + residual + GMRES(40) at its cap, one FD Jv "
+                                "per iteration; estimated from the two measured times",
+            "host_cores": ncores, "host_cpu": model}
 
 
 LEGS = (("fd_fp64", 0, 64), ("tangent_fp64", 1, 64), ("tangent_fp32", 1, 32))
@@ -173,32 +231,141 @@ def mesh_cells(solver):
     return solver.N
 
 
-def jfnk_converge(config=2, scale=1.0, rel_drop=1e-10, max_steps=600, m=40, restarts=0):
+def vm_per_equation(mesh, R, nv):
+    """Volume-weighted norm of each equation's residual (their squares sum to
+    the square of the reference's VM norm)."""
+    vol = mesh.cells()["volume"]
+    return np.sqrt(np.sum(R[:, :nv] ** 2 / vol[:, None], 0) / np.sum(vol))
+
+
+EQ_NAMES = ("rho", "rho_u", "rho_v", "rho_E", "rho_k", "rho_omega")
+
+
+def jfnk_converge(config=2, scale=1.0, rel_drop=1e-10, max_steps=600, m=40, restarts=0, precond=3,
+                  legs=(("fd_fp64", 0, 64), ("tangent_fp32", 1, 32)), tol=1e-2, label=None):
     """Time to a rel_drop drop of the volume-weighted residual norm, FD Jv +
-    FP64 Krylov against tangent Jv + FP32 Krylov, same preconditioner
-    (line-implicit block Gauss-Seidel), GMRES(m) with `restarts` restarts at
-    tolerance 1e-2 and the adaptive CFL controller, from the configuration's
-    initial state: the laminar flat plate (76,790 cells) with GMRES(40), the
-    turbulent flat plate (1,320,146 cells) with GMRES(60) and one restart."""
+    FP64 Krylov against tangent Jv + FP32 Krylov, same preconditioner, GMRES(m)
+    with `restarts` restarts at tolerance `tol` and the adaptive CFL
+    controller, from the configuration's initial state.  Reports each leg's
+    per-equation drops and the largest Cp and Cf differences between the
+    legs' solutions."""
     from mycode_amd import cfd, configs
     mesh, p, U0 = configs.setup(config, scale)
     out, sol = {}, {}
-    for name, jv, prec in (("fd_fp64", 0, 64), ("tangent_fp32", 1, 32)):
+    for name, jv, prec in legs:
         s = cfd.Solver(mesh, p)
-        g = s.gmres_opts(m=m, tol=1e-2, jv=jv, precision=prec, precond=3, restarts=restarts)
+        g = s.gmres_opts(m=m, tol=tol, jv=jv, precision=prec, precond=precond, restarts=restarts)
         o = s.newton_opts(gmres=g, tolerance=0.0, rel_drop=rel_drop, max_steps=max_steps, controller=2, cfl0=1.0,
-                          cfl_max=1e12)
+                          cfl_max=1e12, log=50)  # progress to stderr every 50 Newton steps
         U, res, hist = s.solve_implicit(U0, o)
         sol[name] = s.wall_outputs(U)
+        d0 = vm_per_equation(mesh, s.compute_residuals(U0), p.nvar)
+        d1 = vm_per_equation(mesh, s.compute_residuals(U), p.nvar)
         out[name] = {"seconds": res.seconds, "newton_steps": res.steps, "converged": bool(res.converged),
-                     "drop": float(hist["vm"][-1] / hist["vm"][0]), "gmres_iterations": res.total_gmres_iters}
-        print(f"bench: {WORKLOADS[config]}, {name}: {res.steps} Newton steps, {res.seconds:.1f} s, "
-              f"drop {out[name]['drop']:.2e}", file=sys.stderr, flush=True)
-    out["speedup_tangent_fp32_vs_fd_fp64"] = out["fd_fp64"]["seconds"] / out["tangent_fp32"]["seconds"]
-    out["max_abs_Cp_diff"] = float(np.max(np.abs(sol["fd_fp64"][0] - sol["tangent_fp32"][0])))
-    out["max_abs_Cf_diff"] = float(np.max(np.abs(sol["fd_fp64"][1] - sol["tangent_fp32"][1])))
-    out["workload"] = WORKLOADS[config] + f" ({mesh.n_cells} cells)"
-    out["gmres"] = f"GMRES({m}), {restarts} restart(s), tolerance 1e-2"
+                     "drop": float(hist["vm"][-1] / hist["vm"][0]), "gmres_iterations": res.total_gmres_iters,
+                     "drop_per_equation": {EQ_NAMES[k]: float(d1[k] / d0[k]) for k in range(p.nvar)}}
+        print(f"bench: {label or WORKLOADS[config]}, {name}: {res.steps} Newton steps, {res.seconds:.1f} s, "
+              f"drop {out[name]['drop']:.2e}, converged {bool(res.converged)}", file=sys.stderr, flush=True)
+    a, b = legs[0][0], legs[-1][0]
+    out[f"speedup_{b}_vs_{a}"] = out[a]["seconds"] / out[b]["seconds"]
+    out["max_abs_Cp_diff"] = float(np.max(np.abs(sol[a][0] - sol[b][0])))
+    out["max_abs_Cf_diff"] = float(np.max(np.abs(sol[a][1] - sol[b][1])))
+    out["workload"] = (label or WORKLOADS[config]) + f" ({mesh.n_cells} cells)"
+    out["gmres"] = f"GMRES({m}), {restarts} restart(s), tolerance {tol:g}, preconditioner " + \
+        {1: "block-Jacobi", 2: "block SGS", 3: "line-implicit block Gauss-Seidel"}[precond]
+    return out
+
+
+def other_config(config, steps, warmup):
+    """Residual throughput, its roofline (B_res over the dominant kernel's
+    HIP-event time and over the whole residual's wall time), and the tangent
+    and adjoint products of another configuration at full size."""
+    import torch
+    from mycode_amd import cfd, configs
+    t0 = time.perf_counter()
+    mesh, p, U0 = configs.setup(config, 1.0)
+    U = U0.copy()
+    U[:, :4] *= 1.0 + 1e-3 * np.random.default_rng(12345).standard_normal((U.shape[0], 4))
+    N, F, nv = mesh.n_cells, mesh.n_faces, p.nvar
+    s = cfd.Solver(mesh, p)
+    if config == 5:
+        s.set_face_flux(configs.shock_band(mesh))
+    s.set_stream(torch.cuda.current_stream().cuda_stream)
+    dev = torch.device("cuda:0")
+    Ud = torch.zeros(nv * s.pitch, dtype=torch.float64, device=dev)
+    Rd = torch.zeros_like(Ud)
+    s.dev_pack(torch.from_numpy(U).to(dev).data_ptr(), Ud.data_ptr())
+    s.dev_residual(Ud.data_ptr(), Rd.data_ptr(), True)  # forms the limiter (configuration 5)
+    for _ in range(warmup):
+        s.dev_residual(Ud.data_ptr(), Rd.data_ptr(), False)
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    for _ in range(steps):
+        s.dev_residual(Ud.data_ptr(), Rd.data_ptr(), False)
+    torch.cuda.synchronize()
+    wall = (time.perf_counter() - t1) / steps
+    ms, _ = s.dev_time_residual(Ud.data_ptr(), Rd.data_ptr(), steps, False)
+    B_res = N * (16 * nv + 24) + 48 * F
+    stages = {"gradients": ms[0], "face_flux": ms[1], "assembly": ms[2]}
+    dom = max(stages, key=stages.get)
+    prods = tangent_throughput(s, Ud, nv, s.pitch, N)
+    s.set_stream(None)
+    return {"workload": WORKLOADS[config], "cells": N, "faces": F, "nvar": nv, "mesh_hash": hex(mesh.hash()),
+            "Mcell_residuals_per_s": N / wall / 1e6, "ms_per_residual": wall * 1e3,
+            "stage_ms": {k: v for k, v in stages.items()},
+            "roofline": {"bound": "hbm", "kernel": dom, "achieved": B_res / (stages[dom] * 1e-3) / 1e9,
+                         "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                         "frac": B_res / (stages[dom] * 1e-3) / 1e9 / HBM_PEAK_GBS, "B_res": B_res},
+            "roofline_residual": {"achieved": B_res / wall / 1e9, "frac": B_res / wall / 1e9 / HBM_PEAK_GBS},
+            "tangent_products": prods, "seconds": time.perf_counter() - t0}
+
+
+def design_gradient_leg(h=1e-6, fd_modes=(5, 20, 31, 47), budget_s=240.0):
+    """Configuration 4 at full size: the steady state (tangent + FP32 Krylov,
+    block SGS, adaptive CFL), the discrete adjoint of the bump wall-pressure
+    integral and its gradient for 64 bump-node displacements, and central
+    differences of re-converged flows (step h) for `fd_modes` while the
+    time budget lasts."""
+    from mycode_amd import adjoint, cfd, configs
+    t0 = time.perf_counter()
+    mesh, p, U0 = configs.setup(4, 1.0)
+    s = cfd.Solver(mesh, p)
+    U, res, hist = adjoint.converge(s, U0, rel_drop=1e-12, log_every=50)
+    t_conv = time.perf_counter() - t0
+    print(f"bench: design: steady state in {res.steps} Newton steps, {t_conv:.1f} s", file=sys.stderr, flush=True)
+    modes = adjoint.design_modes(mesh, 64)
+    t1 = time.perf_counter()
+    psi, its, outer, defect = adjoint.adjoint_solve(
+        s, U, adjoint.functional_dU(mesh, p, U),
+        log=lambda msg: print("bench: design: " + msg, file=sys.stderr, flush=True))
+    t_adj = time.perf_counter() - t1
+    t2 = time.perf_counter()
+    g = adjoint.gradient_from_adjoint(mesh, s, p, U, modes, psi)
+    t_grad = time.perf_counter() - t2
+    print(f"bench: design: adjoint in {outer} outer iterations, {t_adj:.1f} s; 64-mode gradient in {t_grad:.1f} s",
+          file=sys.stderr, flush=True)
+    out = {"workload": WORKLOADS[4] + f" ({mesh.n_cells} cells)", "converged": bool(res.converged),
+           "newton_steps": res.steps, "drop": float(hist["vm"][-1] / hist["vm"][0]), "converge_s": t_conv,
+           "adjoint_s": t_adj, "adjoint_outer_iterations": outer, "adjoint_gmres_iterations": its,
+           "adjoint_relative_defect": defect, "gradient_64_modes_s": t_grad,
+           "functional": adjoint.functional(mesh, p, U), "gradient": [float(x) for x in g], "fd_check": []}
+    x0, y0, _, _, _ = mesh.desc()
+    for k in fd_modes:
+        if time.perf_counter() - t0 > budget_s:
+            out["fd_check_note"] = f"time budget of {budget_s:.0f} s reached"
+            break
+        J = []
+        for sgn in (1.0, -1.0):
+            m2 = cfd.Mesh.generate(4, 1.0)
+            m2.set_nodes(x0, y0 + sgn * h * modes[k])
+            s2 = cfd.Solver(m2, p)
+            U2, r2, _ = adjoint.converge(s2, U, rel_drop=0.0, tolerance=2.0 * res.res_vm, cfl0=100.0, log_every=50)
+            J.append(adjoint.functional(m2, p, U2))
+        fd = (J[0] - J[1]) / (2 * h)
+        out["fd_check"].append({"mode": k, "adjoint": float(g[k]), "central_fd": fd, "step": h,
+                                "rel_diff": abs(g[k] - fd) / abs(fd)})
+        print(f"bench: design mode {k}: adjoint {g[k]:.8e} FD {fd:.8e}", file=sys.stderr, flush=True)
+    out["seconds"] = time.perf_counter() - t0
     return out
 
 
@@ -238,6 +405,8 @@ def main():
     ap.add_argument("--scale", type=float, default=1.0, help="mesh coarsening (1.0 = the named configuration)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-jfnk", action="store_true", help="skip the JFNK legs")
+    ap.add_argument("--no-others", action="store_true", help="skip configurations 4 and 5")
+    ap.add_argument("--no-design", action="store_true", help="skip the configuration-4 design gradient")
     ap.add_argument("--gmres-only", action="store_true",
                     help="only the GMRES(40) legs FD + FP64 and tangent + FP32 on the workload (for a kernel trace)")
     ap.add_argument("--copy-calibration", action="store_true",
@@ -296,8 +465,8 @@ def main():
     stages = {"gradients": (bg, ms[0]), "face_flux": (bf, ms[1]), "assembly": (ba, ms[2])}
     dom = max(stages, key=lambda k: stages[k][1])
     b_dom, t_dom = stages[dom]
-    achieved = b_dom / (t_dom * 1e-3) / 1e9
     B_res = N * (16 * nv + 24) + 48 * F
+    achieved = B_res / (t_dom * 1e-3) / 1e9  # compulsory bytes of one residual over the dominant kernel's time
     ach_res = B_res / (wall / args.steps) / 1e9
     wl = WORKLOADS.get(args.config, f"configuration {args.config}") + ", FP64 residual"
     if args.scale != 1.0:
@@ -320,20 +489,38 @@ def main():
                    "flux_launches_per_residual": launches, "mesh_hash": hex(mesh.hash())},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(dom, hex(mesh.hash())),
-                     "kernel": dom,
-                     "algorithmic_bytes_per_residual": b_dom, "ms_per_residual": t_dom},
+                     "kernel": dom, "algorithmic_bytes_per_launch": B_res, "ms_per_launch": t_dom,
+                     "bytes_rule": "B_res = N (16 n_v + 24) + 48 F: U read and R written once, cell volume and "
+                                   "centroid, face indices and geometry"},
         "roofline_residual": {"bound": "hbm", "achieved": ach_res, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                               "frac": ach_res / HBM_PEAK_GBS, "B_res": B_res,
                               "ms_wall": wall / args.steps * 1e3, "ms_event": ms[3]},
+        "stage_traffic": {k: {"bytes": b, "ms": t, "GB_per_s": b / (t * 1e-3) / 1e9,
+                              "frac": b / (t * 1e-3) / 1e9 / HBM_PEAK_GBS}
+                          for k, (b, t) in stages.items()},
         "stage_ms": {"gradients": ms[0], "face_flux": ms[1], "assembly": ms[2], "residual": ms[3]},
     }
     out["tangent_products"] = tangent_throughput(solver, Ud, nv, pitch, N)
+    t_bench = time.perf_counter()
+    if not args.no_others and args.config == CONFIG and args.scale == 1.0:
+        out["other_configs"] = {}
+        for cfg in (4, 5):
+            out["other_configs"][str(cfg)] = other_config(cfg, args.steps, args.warmup)
+            print(f"bench: configuration {cfg}: {out['other_configs'][str(cfg)]['ms_per_residual']:.3f} ms per "
+                  "residual", file=sys.stderr, flush=True)
     if not args.no_jfnk:
         out["jfnk"] = {"per_iteration_on_this_workload": gmres_iteration_cost(solver, Ud, Rd, nv, pitch),
                        "time_to_1e-10": jfnk_converge()}
+        # configuration 2 as it is stated: GMRES(30), block-Jacobi, FD against the tangent, all FP64
+        out["jfnk"]["configuration_2_as_stated"] = jfnk_converge(
+            2, 1.0, rel_drop=1e-10, max_steps=1500, m=30, restarts=0, precond=1,
+            legs=(("fd_fp64", 0, 64), ("tangent_fp64", 1, 64)),
+            label=WORKLOADS[2] + ", GMRES(30), block-Jacobi, FD vs tangent, all FP64")
         if args.config == CONFIG and args.scale == 1.0:
             out["jfnk"]["time_to_1e-10_this_workload"] = jfnk_converge(CONFIG, 1.0, rel_drop=1e-10, max_steps=800,
                                                                        m=60, restarts=1)
+    if not args.no_design and args.config == CONFIG and args.scale == 1.0:
+        out["design_gradient"] = design_gradient_leg(budget_s=max(60.0, 420.0 - (time.perf_counter() - t_bench)))
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(mesh, p, U)
     print(json.dumps(out), flush=True)

@@ -280,7 +280,8 @@ typedef struct cfd_newton_opts {
   double tolerance;     /* absolute VM-norm tolerance (reference)                 */
   double rel_drop;      /* stop when VM_k / VM_0 <= rel_drop (0 = off)             */
   int max_steps;
-  int log;              /* print one JSON line per step to stdout                  */
+  int log;              /* 1: one JSON line per step to stdout; k > 1: one line
+                           every k steps to stderr (progress of long solves)     */
   double cfl0;          /* 0.1                                                     */
   cfd_gmres_opts gmres;
   int controller;       /* 0: the reference CFL controller (Solver.cpp:1747-1796);

@@ -70,19 +70,85 @@ def design_modes(mesh: cfd.Mesh, n_design: int = 64) -> np.ndarray:
     return modes
 
 
-def adjoint_solve(solver: cfd.Solver, U, rhs, tol=1e-10, m=60, restarts=40, precond=2):
-    """psi with (dR/dU)^T psi = rhs at the steady state (pseudo-time term removed)."""
-    solver.compute_residuals(U, True)
-    solver.compute_local_time_step(U, 1e30)
-    R = np.zeros_like(U)
-    psi, it = solver.solve_gmres(U, R, rhs, solver.gmres_opts(m=m, tol=tol, jv=2, precision=64,
-                                                             precond=precond, restarts=restarts))
-    return psi, it
+def adjoint_solve(solver: cfd.Solver, U, rhs, tol=1e-10, cfl0=10.0, cfl_max=1e6, m=60, restarts=2,
+                  inner_tol=1e-2, max_outer=600, precond=2, log=None, log_every=25):
+    """psi with (dR/dU)^T psi = rhs at the steady state U, by defect correction
+    in pseudo-time, the adjoint counterpart of the Newton iteration that found
+    U:  psi <- psi + (dR/dU^T + V/dtau)^-1 (rhs - dR/dU^T psi), the shifted
+    transposed system solved by GMRES (the library's reverse product,
+    ``jv = 2``) with the block SGS preconditioner.  Every vector stays on the
+    device (torch tensors in the library's layout, on torch's stream).
+
+    The CFL follows the rule of the library's adaptive controller: x2 after
+    an iteration that lowered the defect by 1 % or more with GMRES at its
+    tolerance (x1.25 when GMRES missed it), x1.1 (x0.7) after a smaller change
+    up to a 20 % rise, x0.5 after a larger rise; a rise above 10x is undone
+    and the CFL cut tenfold; the CFL stays in [1e-2 cfl0, cfl_max].  Stops at
+    |defect| <= tol |rhs|.  Returns (psi (N x 6, reference order), total
+    GMRES iterations, outer iterations, relative defect)."""
+    import torch
+
+    dev = torch.device("cuda")
+    nv, P, N = solver.params().nvar, solver.pitch, solver.N
+    solver.set_stream(torch.cuda.current_stream().cuda_stream)
+    try:
+        def to_dev(a6):
+            t6 = torch.from_numpy(np.ascontiguousarray(a6, np.float64)).to(dev)
+            out = torch.zeros(nv * P, dtype=torch.float64, device=dev)
+            solver.dev_pack(t6.data_ptr(), out.data_ptr())
+            return out, t6
+
+        Ud, _k1 = to_dev(U)
+        rhs_d, _k2 = to_dev(rhs)
+        zero = torch.zeros_like(Ud)
+        solver.dev_residual(Ud.data_ptr(), zero.data_ptr(), True)  # forms the limiter the products hold frozen
+        zero.zero_()
+        psi = torch.zeros_like(Ud)
+        r = rhs_d.clone()
+        x = torch.zeros_like(Ud)
+        Ax = torch.zeros_like(Ud)
+        bn = float(torch.linalg.norm(rhs_d))
+        rn = bn
+        cfl, its, k = cfl0, 0, 0
+        cap = m * (restarts + 1)
+        opts = solver.gmres_opts(m=m, tol=inner_tol, jv=2, precision=64, precond=precond, restarts=restarts)
+        for k in range(max_outer):
+            if rn <= tol * bn:
+                break
+            solver.dev_local_timestep(Ud.data_ptr(), cfl)
+            it = solver.dev_gmres(Ud.data_ptr(), zero.data_ptr(), r.data_ptr(), x.data_ptr(), opts)
+            its += it
+            gmres_ok = it < cap
+            trial = psi + x
+            solver.dev_local_timestep(Ud.data_ptr(), 1e30)  # steady operator: the time term vanishes
+            solver.dev_vjp_adjoint(Ud.data_ptr(), trial.data_ptr(), Ax.data_ptr())
+            r_new = rhs_d - Ax
+            rn_new = float(torch.linalg.norm(r_new))
+            if not np.isfinite(rn_new) or rn_new > 10.0 * rn:
+                cfl = max(0.1 * cfl, 1e-2 * cfl0)
+                continue
+            if rn_new <= 0.99 * rn:
+                cfl *= 2.0 if gmres_ok else 1.25
+            elif rn_new <= 1.2 * rn:
+                cfl *= 1.1 if gmres_ok else 0.7
+            else:
+                cfl *= 0.5
+            cfl = min(max(cfl, 1e-2 * cfl0), cfl_max)
+            psi, r, rn = trial, r_new, rn_new
+            if log and k % log_every == 0:
+                log(f"adjoint outer {k}: defect {rn / bn:.3e}, cfl {cfl:.3g}, gmres {it}")
+        psi6 = torch.zeros((N, 6), dtype=torch.float64, device=dev)
+        solver.dev_unpack(psi.data_ptr(), psi6.data_ptr())
+        out = psi6.cpu().numpy()
+    finally:
+        solver.set_stream(None)
+    return out, its, k, rn / bn
 
 
-def design_gradient(mesh: cfd.Mesh, solver: cfd.Solver, p: cfd.Params, U, modes, step=1e-7, tol=1e-10):
-    """Adjoint gradient dJ/dalpha for every row of `modes` at the converged state U."""
-    psi, it = adjoint_solve(solver, U, functional_dU(mesh, p, U), tol=tol)
+def gradient_from_adjoint(mesh: cfd.Mesh, solver: cfd.Solver, p: cfd.Params, U, modes, psi, step=1e-7):
+    """dJ/dalpha_k = dJ/dX . dX/dalpha_k - psi^T (dR/dX . dX/dalpha_k) for every
+    row of `modes`, the two partials at fixed U by central differences of the
+    node coordinates with `step`."""
     x0, y0, _, _, _ = mesh.desc()
     grad = np.zeros(len(modes))
     try:
@@ -100,17 +166,27 @@ def design_gradient(mesh: cfd.Mesh, solver: cfd.Solver, p: cfd.Params, U, modes,
     finally:
         mesh.set_nodes(x0, y0)
         solver.update_geometry(mesh)
-    return grad, psi, it
+    return grad
 
 
-def converge(solver: cfd.Solver, U0, rel_drop=1e-12, max_steps=400, jv=1, precision=64, tolerance=0.0,
-             cfl0=1.0):
-    """Steady state by JFNK with the SER controller and the block SGS preconditioner.
+def design_gradient(mesh: cfd.Mesh, solver: cfd.Solver, p: cfd.Params, U, modes, step=1e-7, tol=1e-10, log=None):
+    """Adjoint gradient dJ/dalpha for every row of `modes` at the converged
+    state U; returns (gradient, psi, adjoint GMRES iterations)."""
+    psi, it, _, _ = adjoint_solve(solver, U, functional_dU(mesh, p, U), tol=tol, log=log)
+    return gradient_from_adjoint(mesh, solver, p, U, modes, psi, step), psi, it
+
+
+def converge(solver: cfd.Solver, U0, rel_drop=1e-10, max_steps=600, jv=1, precision=32, tolerance=0.0,
+             cfl0=1.0, log_every=0):
+    """Steady state by JFNK: the tangent product (FP32 matvec, preconditioner
+    and basis by default), block SGS, GMRES(60) with two restarts at a 1e-2
+    tolerance and the adaptive CFL controller.
 
     Stops at a drop of `rel_drop` of the volume-weighted residual norm or below
     the absolute `tolerance` (a re-solve from a nearby converged state needs
-    the absolute test: its own initial residual is already small)."""
-    g = solver.gmres_opts(m=60, tol=1e-4, jv=jv, precision=precision, precond=2, restarts=2)
-    o = solver.newton_opts(gmres=g, tolerance=tolerance, rel_drop=rel_drop, max_steps=max_steps, controller=1,
-                           cfl0=cfl0, cfl_max=1e12)
+    the absolute test: its own initial residual is already small).
+    log_every > 1 prints the Newton progress to stderr every that many steps."""
+    g = solver.gmres_opts(m=60, tol=1e-2, jv=jv, precision=precision, precond=2, restarts=2)
+    o = solver.newton_opts(gmres=g, tolerance=tolerance, rel_drop=rel_drop, max_steps=max_steps, controller=2,
+                           cfl0=cfl0, cfl_max=1e12, log=log_every)
     return solver.solve_implicit(U0, o)

@@ -35,8 +35,27 @@ def _run(cmd):
     return r
 
 
+ROOT = os.path.dirname(HERE)
+HOST_TEST_SRC = os.path.join(ROOT, "tests", "host_abi_test.cpp")
+HOST_TEST_BIN = os.path.join(ROOT, "tests", "host_abi_test")
+
+
+def build_host_test(verbose: bool = False) -> str:
+    """Compile tests/host_abi_test.cpp, a C++ host that uses only
+    include/cfd_gpu.h, against the library (run by the GPU tests)."""
+    cmd = ["g++", "-std=c++17", "-O2", "-Wall", "-Wextra", "-Werror", f"-I{os.path.join(ROOT, 'include')}",
+           HOST_TEST_SRC, f"-L{HERE}", "-lcfdgpu", "-Wl,-rpath,$ORIGIN/../mycode_amd",
+           "-Wl,-rpath-link,/opt/rocm/lib", "-o", HOST_TEST_BIN + ".tmp"]
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    _run(cmd)
+    os.replace(HOST_TEST_BIN + ".tmp", HOST_TEST_BIN)
+    return HOST_TEST_BIN
+
+
 def build(verbose: bool = False) -> str:
-    """Compile every source and link the shared library; returns its path."""
+    """Compile every source and link the shared library (and the C++ host
+    test program); returns the library's path."""
     os.makedirs(BUILD, exist_ok=True)
     objs = []
     for src in sorted(os.listdir(CSRC)):
@@ -56,6 +75,8 @@ def build(verbose: bool = False) -> str:
     tmp = LIB + ".tmp"
     _run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", *objs, "-o", tmp])
     os.replace(tmp, LIB)
+    if os.path.exists(HOST_TEST_SRC):
+        build_host_test(verbose)
     return LIB
 
 

@@ -1289,7 +1289,11 @@ int cfd_solve_implicit(cfd_ctx* c, double* U6, const cfd_newton_opts* opts, cfd_
     if ((rc = gmres_dev(c, U, R, R, -1.0, dU, &o.gmres, &it, c->vec[4], c->vec[5]))) return rc;
     total_it += it;
     if (hist_iters) hist_iters[step] = it;
-    if (o.log) std::printf("{\"step\": %d, \"vm\": %.6e, \"cfl\": %.4g, \"gmres\": %d}\n", step, vm, cfl, it);
+    if (o.log == 1) std::printf("{\"step\": %d, \"vm\": %.6e, \"cfl\": %.4g, \"gmres\": %d}\n", step, vm, cfl, it);
+    if (o.log > 1 && step % o.log == 0) {
+      std::fprintf(stderr, "{\"step\": %d, \"vm\": %.6e, \"cfl\": %.4g, \"gmres\": %d}\n", step, vm, cfl, it);
+      std::fflush(stderr);
+    }
     if (o.controller == 0) {  // Solver.cpp:1747-1796
       if (it >= 29) {
         cfl *= 0.5;

@@ -52,3 +52,28 @@ def test_params_helpers_match_reference_formulas():
     assert p.P_total_inlet == 101325.0 * fT ** (1.4 / 0.4)
     U = cfd.initial_state(p)
     assert abs(U[1] / U[0] / (1.4 * 287 * 300) ** 0.5 - 0.5) < 1e-12
+
+
+def test_host_program_compiles_and_links_against_the_header(tmp_path):
+    """A C++ host using only include/cfd_gpu.h and a 48-byte State vector
+    (tests/host_abi_test.cpp) compiles warning-free and links: every entry
+    point it calls resolves in the library."""
+    import subprocess
+
+    exe = tmp_path / "host_abi_test"
+    cmd = ["g++", "-std=c++17", "-O2", "-Wall", "-Wextra", "-Werror", f"-I{os.path.join(ROOT, 'include')}",
+           os.path.join(ROOT, "tests", "host_abi_test.cpp"), f"-L{os.path.join(ROOT, 'mycode_amd')}", "-lcfdgpu",
+           "-Wl,-rpath-link,/opt/rocm/lib", "-o", str(exe)]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr
+    assert exe.exists()
+
+
+def test_header_compiles_as_c(tmp_path):
+    import subprocess
+
+    src = tmp_path / "h.c"
+    src.write_text('#include "cfd_gpu.h"\nint main(void) { cfd_params p; return cfd_params_default(&p); }\n')
+    r = subprocess.run(["gcc", "-std=c99", "-Wall", "-Werror", f"-I{os.path.join(ROOT, 'include')}", "-c", str(src),
+                        "-o", str(tmp_path / "h.o")], capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr
