@@ -260,7 +260,9 @@ typedef struct cfd_gmres_opts {
   double tol;       /* relative tolerance on |g_{j+1}| (reference: 1e-2, 1587)     */
   int jv;           /* 0 finite difference (reference), 1 tangent, 2 transpose:
                        solve J^T x = RHS with the reverse product (adjoint)      */
-  int precision;    /* 64: FP64 basis/matvec; 32: FP32 basis, matvec, precond      */
+  int precision;    /* 64: FP64 basis/matvec; 32: FP32 basis, matvec, precond
+                       (jv 1 and 2; with jv 2 the reverse product itself runs in
+                       FP64 between FP32 basis vectors)                         */
   int precond;      /* 0 scalar diagonal (reference, 1424-1465), 1 block-Jacobi,   *
                      * 2 multicolor block symmetric Gauss-Seidel, 3 line-implicit *
                      * symmetric Gauss-Seidel (jv 0 and 1 only)                    */

@@ -71,13 +71,15 @@ def design_modes(mesh: cfd.Mesh, n_design: int = 64) -> np.ndarray:
 
 
 def adjoint_solve(solver: cfd.Solver, U, rhs, tol=1e-10, cfl0=10.0, cfl_max=1e6, m=60, restarts=2,
-                  inner_tol=1e-2, max_outer=600, precond=2, log=None, log_every=25):
+                  inner_tol=1e-2, max_outer=600, precond=2, precision=32, log=None, log_every=25):
     """psi with (dR/dU)^T psi = rhs at the steady state U, by defect correction
     in pseudo-time, the adjoint counterpart of the Newton iteration that found
     U:  psi <- psi + (dR/dU^T + V/dtau)^-1 (rhs - dR/dU^T psi), the shifted
     transposed system solved by GMRES (the library's reverse product,
-    ``jv = 2``) with the block SGS preconditioner.  Every vector stays on the
-    device (torch tensors in the library's layout, on torch's stream).
+    ``jv = 2``) with the block SGS preconditioner; the inner solve's basis and
+    preconditioner run in `precision` (the reverse product itself in FP64),
+    the defect and psi in FP64.  Every vector stays on the device (torch
+    tensors in the library's layout, on torch's stream).
 
     The CFL follows the rule of the library's adaptive controller: x2 after
     an iteration that lowered the defect by 1 % or more with GMRES at its
@@ -111,7 +113,7 @@ def adjoint_solve(solver: cfd.Solver, U, rhs, tol=1e-10, cfl0=10.0, cfl_max=1e6,
         rn = bn
         cfl, its, k = cfl0, 0, 0
         cap = m * (restarts + 1)
-        opts = solver.gmres_opts(m=m, tol=inner_tol, jv=2, precision=64, precond=precond, restarts=restarts)
+        opts = solver.gmres_opts(m=m, tol=inner_tol, jv=2, precision=precision, precond=precond, restarts=restarts)
         for k in range(max_outer):
             if rn <= tol * bn:
                 break

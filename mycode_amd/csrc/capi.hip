@@ -1062,6 +1062,10 @@ static int gmres_cycle(cfd_ctx* c, const double* U, const double* Rb, const doub
         launch_jv_fd(c, U, Rb, (const double*)k.z, (double*)Vj1, S_UNORM, S_VNORM, tmpU, tmpR);
       } else if (o->jv == 1) {
         launch_jv_tangent(c, U, k.z, Vj1, prec);
+      } else if (prec == 32) {  // the reverse product runs in FP64 on FP32 basis vectors
+        launch_f2d(c, (const float*)k.z, tmpU, vs);
+        launch_vjp(c, U, tmpU, tmpR);
+        launch_d2f(c, tmpR, (float*)Vj1, vs);
       } else {
         launch_vjp(c, U, (const double*)k.z, (double*)Vj1);
       }
@@ -1138,7 +1142,7 @@ static int gmres_dev(cfd_ctx* c, const double* U, const double* Rb, const double
   if (m < 1 || m > 60) return fail(c, CFD_ERR_ARG, "GMRES m must be in [1, 60]");
   if (prec != 64 && prec != 32) return fail(c, CFD_ERR_ARG, "precision must be 64 or 32");
   if (o->jv < 0 || o->jv > 2) return fail(c, CFD_ERR_ARG, "jv must be 0 (FD), 1 (tangent) or 2 (transpose)");
-  if (o->jv != 1 && prec != 64) return fail(c, CFD_ERR_ARG, "FD Jv and the transpose run in FP64 only");
+  if (o->jv == 0 && prec != 64) return fail(c, CFD_ERR_ARG, "the FD Jv runs in FP64 only");
   if (o->precond < 0 || o->precond > 3) return fail(c, CFD_ERR_ARG, "unknown preconditioner");
   if (o->precond == 3 && o->jv == 2)
     return fail(c, CFD_ERR_ARG, "the line preconditioner has no transposed form (use precond 2 for jv 2)");

@@ -96,3 +96,31 @@ def test_adjoint_deterministic():
     w = rng.standard_normal(U.shape)
     w[:, p.nvar:] = 0
     assert np.array_equal(s.vjp_adjoint(U, w), s.vjp_adjoint(U, w))
+
+
+@pytest.mark.parametrize("config,scale", [(4, 0.05), (1, 0.1)])
+def test_transposed_gmres_fp32_and_fp64(config, scale):
+    """GMRES on the transposed system (J^T + V/dtau) x = b with the block SGS
+    preconditioner, with an FP64 and with an FP32 basis and preconditioner
+    (the reverse product runs in FP64 in both): each solution's true
+    residual, formed with the FP64 reverse product, meets the tolerance, and
+    the two solutions agree to 1e-4 (the FP32 inner solve's error).  The
+    design adjoint (configuration 4) uses the FP32 form inside its FP64
+    defect correction.  (With SST the omega rows span eleven decades more
+    than the mean flow's, beyond what an FP32 basis resolves to this bar.)"""
+    s, o, U, p, rng = _setup(config, scale, 2)
+    print(f"config={config} N={s.N}", flush=True)
+    s.compute_residuals(U, True)
+    s.compute_local_time_step(U, 20.0)
+    b = rng.standard_normal(U.shape)
+    b[:, p.nvar:] = 0
+    zero = np.zeros_like(U)
+    xs = {}
+    for prec in (64, 32):
+        opts = s.gmres_opts(m=60, tol=1e-6, jv=2, precision=prec, precond=2, restarts=8)
+        x, it = s.solve_gmres(U, zero, b, opts)
+        res = np.linalg.norm(s.vjp_adjoint(U, x) - b) / np.linalg.norm(b)
+        print(f"precision {prec}: {it} iterations, true relative residual {res:.2e}", flush=True)
+        assert res < 1e-5
+        xs[prec] = x
+    assert np.linalg.norm(xs[32] - xs[64]) <= 1e-4 * np.linalg.norm(xs[64])
