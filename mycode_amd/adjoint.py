@@ -79,7 +79,10 @@ def adjoint_solve(solver: cfd.Solver, U, rhs, tol=1e-10, cfl0=10.0, cfl_max=1e6,
     ``jv = 2``) with the block SGS preconditioner; the inner solve's basis and
     preconditioner run in `precision` (the reverse product itself in FP64),
     the defect and psi in FP64.  Every vector stays on the device (torch
-    tensors in the library's layout, on torch's stream).
+    tensors in the library's layout, on torch's stream).  The FP32 basis
+    suits systems like the subsonic bump's; where the shifted Jacobian's
+    condition number is far above 1/eps32 (the supersonic wedge's is near
+    6.5e12 at CFL 20) the inner solve needs ``precision=64``.
 
     The CFL follows the rule of the library's adaptive controller: x2 after
     an iteration that lowered the defect by 1 % or more with GMRES at its
