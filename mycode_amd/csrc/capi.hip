@@ -44,6 +44,14 @@ int need_limiter(cfd_ctx* c) {
 
 template <class T> hipError_t dalloc(T** p, size_t n) { return hipMalloc((void**)p, std::max<size_t>(n, 1) * sizeof(T)); }
 
+// The structured reverse product's face buffer, allocated on first use (a
+// context that never forms a reverse product does not hold it).
+int ensure_adjoint(cfd_ctx* c) {
+  if (c->adjF) return CFD_OK;
+  HIP_TRY(c, dalloc(&c->adjF, (size_t)cfd::VJP_ROW_MAX * c->F));
+  return CFD_OK;
+}
+
 void fill_phys(cfd_ctx* c) {
   const cfd_params& p = c->P;
   Phys& q = c->phys;
@@ -683,7 +691,7 @@ int cfd_ctx_destroy(cfd_ctx* c) {
   free_krylov(c);
   void* bufs[] = {c->d_vol, c->d_xc,  c->d_yc,  c->d_slots, c->d_own, c->d_nei, c->d_tag,  c->d_nx,
                   c->d_ny,  c->d_area, c->d_xm, c->d_ym,    c->d_iperm, c->d_facemap, c->d_alt, c->grad, c->d_wd, c->d_dnw, c->d_corder, c->d_ccol, c->d_ccells, c->d_nq, c->d_cpos, c->d_seg_ptr, c->d_seg_cells, c->d_lpos, c->d_lslot, c->d_lb_t, c->d_lb_kl, c->d_lcomp, c->d_lmask, c->d_lnq,
-                  c->phi,   c->dt,    c->Fm,    c->red,     c->scal, c->gradT, c->adj, c->tq, c->io6};
+                  c->phi,   c->dt,    c->Fm,    c->red,     c->scal, c->gradT, c->adj, c->adjF, c->tq, c->io6};
   for (void* b : bufs) (void)hipFree(b);
   for (int i = 0; i < 6; ++i) (void)hipFree(c->vec[i]);
   (void)hipFree(c->aux);
@@ -848,6 +856,7 @@ int cfd_vjp_adjoint(cfd_ctx* c, const double* U6, const double* w6, double* JTw6
   if (!c || !U6 || !w6 || !JTw6) return CFD_ERR_ARG;
   if (!c->dt_valid) return fail(c, CFD_ERR_STATE, "no local time step computed");
   if (int rc = need_limiter(c)) return rc;
+  if (int rc = ensure_adjoint(c)) return rc;
   int rc;
   if ((rc = upload_state(c, U6, c->vec[0]))) return rc;
   if ((rc = upload_state(c, w6, c->vec[1]))) return rc;
@@ -1149,6 +1158,7 @@ static int gmres_dev(cfd_ctx* c, const double* U, const double* Rb, const double
   if (o->restarts < 0) return fail(c, CFD_ERR_ARG, "restarts must be >= 0");
   int rc = ensure_krylov(c, m, prec);
   if (rc) return rc;
+  if (o->jv == 2 && (rc = ensure_adjoint(c))) return rc;
   Krylov& k = c->kry;
   const size_t vs = (size_t)c->P.nvar * c->pitch;
   if (o->precond == 1) {
@@ -1576,6 +1586,7 @@ int cfd_dev_vjp_adjoint(cfd_ctx* c, const double* U, const double* w, double* JT
   if (!c || !U || !w || !JTw) return CFD_ERR_ARG;
   if (!c->dt_valid) return fail(c, CFD_ERR_STATE, "no local time step computed");
   if (int rc = need_limiter(c)) return rc;
+  if (int rc = ensure_adjoint(c)) return rc;
   launch_vjp(c, U, w, JTw);
   HIP_TRY(c, hipGetLastError());
   return CFD_OK;

@@ -111,6 +111,7 @@ struct cfd_ctx {
   double* scal_host = nullptr;  // pinned
   double* gradT = nullptr;  // tangent gradients (2 x 14 x pitch: values, derivatives)
   double* adj = nullptr;    // adjoints of the gradients (14 x pitch), of SST mu_t and F1 (2 x pitch)
+  double* adjF = nullptr;   // adjoint face buffer (cfd::VJP_ROW_MAX x F), allocated by ensure_adjoint before the first reverse product
   double* tq = nullptr;     // SST per cell: mu_t, F1, V S_k, V S_w (values; + derivatives in the tangent)
   bool phi_valid = false, dt_valid = false;
   // host-pointer staging: AoS device copy + SoA vectors
@@ -126,6 +127,9 @@ struct cfd_ctx {
 namespace cfd {
 // Most blocks of a reduction's first stage (one partial sum per block and sum).
 constexpr int RED_BLOCKS = 1024;
+// Doubles per face of the adjoint face buffer (SST: both sides' 6 state and 8
+// gradient adjoints, 6 averaged-gradient and 4 mu_t / F1 adjoints).
+constexpr int VJP_ROW_MAX = 38;
 // Per-device address of the library's static scratch array (noalloc.hip);
 // called by cfd_ctx_create with the context's device current.
 int noalloc_init(cfd_ctx* c);

@@ -27,6 +27,27 @@ __device__ __forceinline__ int xcd_block() {
   return x * q + min(x, r) + (b >> 3);
 }
 
+// Neighbouring lanes of a wavefront by DPP wave shifts (a VALU move, no LDS
+// round trip): lane_prev gives lane l the value of lane l-1, lane_next that of
+// lane l+1 (lane 0, resp. 63, receives 0).  Every lane of the wave must be
+// active.
+constexpr int DPP_WAVE_SHL1 = 0x130, DPP_WAVE_SHR1 = 0x138;
+template <int CTRL>
+__device__ __forceinline__ float dpp_move(float v) {
+  return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), CTRL, 0xF, 0xF, false));
+}
+template <int CTRL>
+__device__ __forceinline__ double dpp_move(double v) {
+  const long long b = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_update_dpp(0, (int)b, CTRL, 0xF, 0xF, false);
+  const int hi = __builtin_amdgcn_update_dpp(0, (int)(b >> 32), CTRL, 0xF, 0xF, false);
+  return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+template <class T>
+__device__ __forceinline__ T lane_prev(T v) { return dpp_move<DPP_WAVE_SHR1>(v); }
+template <class T>
+__device__ __forceinline__ T lane_next(T v) { return dpp_move<DPP_WAVE_SHL1>(v); }
+
 inline int nblocks(size_t n) { return (int)((n + BS - 1) / BS); }
 inline int nblocks(size_t n, int bs) { return (int)((n + bs - 1) / bs); }
 
@@ -720,37 +741,60 @@ __device__ __forceinline__ S6<T> visc_interior(const Phys& P, const FaceGeo& g, 
 
 // Interior faces of a second-order residual with four variables (Euler or
 // laminar), or with SST and k, omega convected first-order upwind (NV = 6):
-// k_vjp_face's product, using the face's structure.  Such a face's flux F A
-// depends on the two cells' gradients only through the reconstructed face
-// primitives (rho, u, v, p) of each side, linearly (weights phi dx, phi dy,
-// none where a reconstruction clip took the cell value), and on the cell
-// states through their primitives.  So lambda^T d(F A) is formed by forward
-// duals over the 8 face primitives (two passes of VK directions through
-// prim2cons and the Riemann flux) and carried back by hand onto the gradient
-// adjoints and, through the primitive map's Jacobian, onto the states.  The
-// viscous terms (VISC) depend on the two states and on the averages of six
-// gradient components: duals over those 14 inputs, the averages' adjoints
-// shared half and half by the two cells.  With SST the k and omega fluxes are
-// the mass flux times the upwind cell's specific k and omega, carried back by
-// hand, and the viscous terms also depend on the cells' mu_t and F1 (read
-// from Q, rows 0 and 1): their adjoints go to aQ (rows 0 and 1), which
-// k_vjp_src carries back through the cell's SST terms.
+// k_vjp_face's product, using the face's structure, one thread per face over
+// all faces in one launch.  Such a face's flux F A depends on the two cells'
+// gradients only through the reconstructed face primitives (rho, u, v, p) of
+// each side, linearly (weights phi dx, phi dy, none where a reconstruction
+// clip took the cell value), and on the cell states through their
+// primitives.  So lambda^T d(F A) is formed by forward duals over the 8 face
+// primitives (passes of KD directions through prim2cons and the Riemann flux)
+// and carried back by hand onto the gradient adjoints and, through the
+// primitive map's Jacobian, onto the states.  The viscous terms (VISC) depend
+// on the two states and on the averages of six gradient components: duals
+// over those 14 inputs.  With SST the k and omega fluxes are the mass flux
+// times the upwind cell's specific k and omega, carried back by hand, and the
+// viscous terms also depend on the cells' mu_t and F1 (read from Q, rows 0
+// and 1).
+//
+// The results go to the face's row of the adjoint face buffer (layout below),
+// not to the cells: k_vjp_gather sums each cell's rows.  Against one launch
+// per edge color with a read-modify-write of both cells' adjoint rows, this
+// halves the face pass on the 1.32M-cell SST plate (1.75 -> 0.84 ms per
+// product): the faces are read in owner order and written as contiguous
+// rows, and there is one launch instead of six tails.  Spreading a face over
+// four lanes of 2-direction duals (a quad of wavefronts exchanging partial
+// results through LDS) was measured too: it recomputes the primal flux in
+// every lane and ran at 1.70 ms.
+//
+// Face buffer row (doubles): side 0 then side 1, each the NV state adjoints
+// and gradient slots 0-7; then the six averaged-gradient adjoints (viscous;
+// each goes half to each cell) and the adjoints of mu_t,o, mu_t,n, F1_o, F1_n
+// (SST).  A boundary face's row is zero: the general kernel adds its product
+// straight into the cells' rows.
+template <int NV, int VISC> __host__ __device__ constexpr int vjp_row() {
+  return 2 * (NV + 8) + (VISC ? 6 : 0) + (NV == 6 ? 4 : 0);
+}
+static_assert(vjp_row<6, 1>() <= VJP_ROW_MAX && vjp_row<4, 1>() <= VJP_ROW_MAX, "adjoint face buffer row");
 template <int NV, int KD, int VISC>
 __global__ __launch_bounds__(FLUX_BS) void k_vjp_face_inv(DMesh m, Phys P, const double* __restrict__ U,
                                                       const double* __restrict__ G, const double* __restrict__ phi,
                                                       const double* __restrict__ Q, const double* __restrict__ w,
-                                                      double* __restrict__ aU, double* __restrict__ aG,
-                                                      double* __restrict__ aQ, int f0, int f1) {
+                                                      double* __restrict__ buf) {
   static_assert((NV == 4 || (NV == 6 && VISC)) && 8 % KD == 0, "KD directions per pass");
   constexpr bool SST = NV == 6;
   typedef DualV<double, KD> T;
-  const int i = f0 + xcd_block() * FLUX_BS + threadIdx.x;
-  if (i >= f1) return;
-  const int f = m.corder[i];
+  constexpr int ROW = vjp_row<NV, VISC>();
+  const int f = xcd_block() * FLUX_BS + threadIdx.x;
+  if (f >= m.F) return;
   const int p = m.pitch;
   const int o = m.own[f], n = m.nei[f];
+  double* row = buf + (size_t)f * ROW;
+  if (n < 0) {
+#pragma unroll
+    for (int q = 0; q < ROW; ++q) row[q] = 0.0;
+    return;
+  }
   const FaceGeo g = face_geo(m, P, phi, f, o, n, true);
-  const int cell[2] = {o, n};
   const double dx[2] = {g.xm - g.xco, g.xm - g.xcn}, dy[2] = {g.ym - g.yco, g.ym - g.ycn};
   const double ph[2] = {g.phL, g.phR};
   double lam[NV];
@@ -761,7 +805,7 @@ __global__ __launch_bounds__(FLUX_BS) void k_vjp_face_inv(DMesh m, Phys P, const
   bool clip_r[2], clip_p[2];
 #pragma unroll
   for (int sd = 0; sd < 2; ++sd) {
-    const int c = cell[sd];
+    const int c = sd ? n : o;
     const S6<double> Us(U[c], U[p + c], U[2 * p + c], U[3 * p + c], SST ? U[4 * p + c] : 0.0,
                         SST ? U[5 * p + c] : 0.0);
 #pragma unroll
@@ -820,40 +864,35 @@ __global__ __launch_bounds__(FLUX_BS) void k_vjp_face_inv(DMesh m, Phys P, const
   }
   // back through the reconstruction and the primitive map
   const double gm1 = P.gamma - 1.0;
+  double ru[2][NV];
 #pragma unroll
   for (int sd = 0; sd < 2; ++sd) {
-    const int c = cell[sd];
+    double* rs = row + sd * (NV + 8);
     const double wx = ph[sd] * dx[sd], wy = ph[sd] * dy[sd];
     const double ar = aW[sd][0], au = aW[sd][1], av = aW[sd][2], ap = aW[sd][3];
-    if (!clip_r[sd]) {
-      aG[c] = aG[c] + ar * wx;
-      aG[(size_t)p + c] = aG[(size_t)p + c] + ar * wy;
-    }
-    aG[(size_t)2 * p + c] = aG[(size_t)2 * p + c] + au * wx;
-    aG[(size_t)3 * p + c] = aG[(size_t)3 * p + c] + au * wy;
-    aG[(size_t)4 * p + c] = aG[(size_t)4 * p + c] + av * wx;
-    aG[(size_t)5 * p + c] = aG[(size_t)5 * p + c] + av * wy;
-    if (!clip_p[sd]) {
-      aG[(size_t)6 * p + c] = aG[(size_t)6 * p + c] + ap * wx;
-      aG[(size_t)7 * p + c] = aG[(size_t)7 * p + c] + ap * wy;
-    }
-    // d(rho, u, v, p)/dU^T: u = m1/rho, v = m2/rho, p = (gamma-1)(E - (m1^2 + m2^2)/(2 rho))
+    rs[NV + 0] = clip_r[sd] ? 0.0 : ar * wx;
+    rs[NV + 1] = clip_r[sd] ? 0.0 : ar * wy;
+    rs[NV + 2] = au * wx;
+    rs[NV + 3] = au * wy;
+    rs[NV + 4] = av * wx;
+    rs[NV + 5] = av * wy;
+    rs[NV + 6] = clip_p[sd] ? 0.0 : ap * wx;
+    rs[NV + 7] = clip_p[sd] ? 0.0 : ap * wy;
     const double u = uc[sd], v = vc[sd], ir = 1.0 / rho[sd];
-    aU[c] = aU[c] + (ar - au * u * ir - av * v * ir + ap * gm1 * 0.5 * (u * u + v * v));
-    aU[p + c] = aU[p + c] + (au * ir - ap * gm1 * u);
-    aU[2 * p + c] = aU[2 * p + c] + (av * ir - ap * gm1 * v);
-    aU[3 * p + c] = aU[3 * p + c] + ap * gm1;
+    ru[sd][0] = ar - au * u * ir - av * v * ir + ap * gm1 * 0.5 * (u * u + v * v);
+    ru[sd][1] = au * ir - ap * gm1 * u;
+    ru[sd][2] = av * ir - ap * gm1 * v;
+    ru[sd][3] = ap * gm1;
   }
   if constexpr (SST) {  // the upwind specific k and omega: d(U4 / U0), d(U5 / U0)
     const double mp = mflux > 0.0 ? mflux : 0.0, mm = mflux > 0.0 ? 0.0 : mflux;
 #pragma unroll
     for (int sd = 0; sd < 2; ++sd) {
-      const int c = cell[sd];
       const double mf = (sd == 0 ? mp : mm) * g.A, ir = 1.0 / rho[sd];
       const double ak = lam[4] * mf, aw = lam[5] * mf;
-      aU[c] = aU[c] - (ak * kw[sd][0] + aw * kw[sd][1]) * ir;
-      aU[(size_t)4 * p + c] = aU[(size_t)4 * p + c] + ak * ir;
-      aU[(size_t)5 * p + c] = aU[(size_t)5 * p + c] + aw * ir;
+      ru[sd][0] = ru[sd][0] - (ak * kw[sd][0] + aw * kw[sd][1]) * ir;
+      ru[sd][4] = ak * ir;
+      ru[sd][5] = aw * ir;
     }
   }
   if constexpr (VISC != 0) {
@@ -908,19 +947,76 @@ __global__ __launch_bounds__(FLUX_BS) void k_vjp_face_inv(DMesh m, Phys P, const
       }
     }
 #pragma unroll
-    for (int sd = 0; sd < 2; ++sd) {
-      const int c = cell[sd];
+    for (int sd = 0; sd < 2; ++sd)
 #pragma unroll
-      for (int k = 0; k < NV; ++k) aU[(size_t)k * p + c] = aU[(size_t)k * p + c] + ain[NV * sd + k];
+      for (int k = 0; k < NV; ++k) ru[sd][k] += ain[NV * sd + k];
 #pragma unroll
-      for (int q = 0; q < 6; ++q)
-        aG[(size_t)gq[q] * p + c] = aG[(size_t)gq[q] * p + c] + 0.5 * ain[2 * NV + q];
-      if (SST) {
-        aQ[c] = aQ[c] + ain[2 * NV + 6 + sd];
-        aQ[(size_t)p + c] = aQ[(size_t)p + c] + ain[2 * NV + 8 + sd];
+    for (int q = 0; q < 6; ++q) row[2 * (NV + 8) + q] = ain[2 * NV + q];
+    if (SST) {
+#pragma unroll
+      for (int q = 0; q < 4; ++q) row[2 * (NV + 8) + 6 + q] = ain[2 * NV + 6 + q];
+    }
+  }
+#pragma unroll
+  for (int sd = 0; sd < 2; ++sd)
+#pragma unroll
+    for (int k = 0; k < NV; ++k) row[sd * (NV + 8) + k] = ru[sd][k];
+}
+
+
+// Per cell: the sums of its faces' rows of the adjoint face buffer, in the
+// cell's slot order (zeros past the last cell, up to the pitch).  They are the
+// first values of the cell's state adjoint
+// (aU), its 14 gradient adjoints (aG; the averaged components' adjoints half to
+// each cell, the k and omega slots zero) and, with SST, its mu_t and F1
+// adjoints (aQ): the kernels that follow add to them.
+template <int NV, int VISC>
+__global__ __launch_bounds__(BS) void k_vjp_gather(DMesh m, const double* __restrict__ buf, double* __restrict__ aU,
+                                                   double* __restrict__ aG, double* __restrict__ aQ) {
+  constexpr bool SST = NV == 6;
+  constexpr int ROW = vjp_row<NV, VISC>();
+  constexpr int NGS = VISC ? 10 : 8;
+  const int c = xcd_block() * BS + threadIdx.x;
+  const int p = m.pitch;
+  if (c >= p) return;
+  const int4 s4 = c < m.N ? m.slots[c] : make_int4(-1, -1, -1, -1);  // the padding cells get zeros
+  const int sl[4] = {s4.x, s4.y, s4.z, s4.w};
+  double su[NV], sg[NGS], sq[2] = {0.0, 0.0};
+#pragma unroll
+  for (int k = 0; k < NV; ++k) su[k] = 0.0;
+#pragma unroll
+  for (int q = 0; q < NGS; ++q) sg[q] = 0.0;
+#pragma unroll
+  for (int j = 0; j < 4; ++j) {
+    const int e = sl[j];
+    if (e < 0) continue;
+    const int sd = e & 1;
+    const double* r = buf + (size_t)(e >> 1) * ROW;
+    const double* rs = r + sd * (NV + 8);
+#pragma unroll
+    for (int k = 0; k < NV; ++k) su[k] += rs[k];
+#pragma unroll
+    for (int q = 0; q < 8; ++q) sg[q] += rs[NV + q];
+    if constexpr (VISC != 0) {
+      const double* ga = r + 2 * (NV + 8);
+      sg[2] += 0.5 * ga[0];
+      sg[3] += 0.5 * ga[1];
+      sg[4] += 0.5 * ga[2];
+      sg[5] += 0.5 * ga[3];
+      sg[8] += 0.5 * ga[4];
+      sg[9] += 0.5 * ga[5];
+      if constexpr (SST) {
+        sq[0] += ga[6 + sd];
+        sq[1] += ga[8 + sd];
       }
     }
   }
+#pragma unroll
+  for (int k = 0; k < NV; ++k) aU[k * p + c] = su[k];
+#pragma unroll
+  for (int q = 0; q < 14; ++q) aG[(size_t)q * p + c] = q < NGS ? sg[q] : 0.0;
+  aQ[c] = sq[0];
+  aQ[p + c] = sq[1];
 }
 
 // SST cell terms: w_c^T d(-V S)/d(U_c, G_c), plus the face sweep's adjoints
@@ -1014,7 +1110,7 @@ __global__ __launch_bounds__(BS) void k_vjp_gg(DMesh m, Phys P, const double* __
     const int f = e >> 1;
     const int o = m.own[f], n = m.nei[f];
     const double nx = m.nx[f], ny = m.ny[f], A = m.area[f];
-    const double vo = m.vol[o];
+    const double vo = m.vol[o], vn = n >= 0 ? m.vol[n] : 1.0;
     double aq[NQM];
 #pragma unroll
     for (int k = 0; k < NQM; ++k) {
@@ -1022,9 +1118,9 @@ __global__ __launch_bounds__(BS) void k_vjp_gg(DMesh m, Phys P, const double* __
         aq[k] = 0.0;
         continue;
       }
-      aq[k] = A * ((aG[(size_t)(2 * k) * p + o] * nx + aG[(size_t)(2 * k + 1) * p + o] * ny) / vo);
+      aq[k] = A * dv(aG[(size_t)(2 * k) * p + o] * nx + aG[(size_t)(2 * k + 1) * p + o] * ny, vo);
       if (n >= 0)
-        aq[k] -= A * ((aG[(size_t)(2 * k) * p + n] * nx + aG[(size_t)(2 * k + 1) * p + n] * ny) / m.vol[n]);
+        aq[k] -= A * dv(aG[(size_t)(2 * k) * p + n] * nx + aG[(size_t)(2 * k + 1) * p + n] * ny, vn);
       S[k] += aq[k];
     }
     if (n < 0) {  // the ghost state's primitives enter the face mean too
@@ -1515,26 +1611,6 @@ __global__ __launch_bounds__(BS) void k_sgs_color(int n, int b0, const int* __re
 // a few lane moves rather than a chain of dependent memory loads.  P and G are kept
 // by slot (nv^2 consecutive values per slot).
 
-// Neighbouring lanes of a wavefront by DPP wave shifts (a VALU move, no LDS
-// round trip): lane_prev gives lane l the value of lane l-1, lane_next that of
-// lane l+1 (lane 0, resp. 63, receives 0).  Every lane of the wave must be
-// active.
-constexpr int DPP_WAVE_SHL1 = 0x130, DPP_WAVE_SHR1 = 0x138;
-template <int CTRL>
-__device__ __forceinline__ float dpp_move(float v) {
-  return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), CTRL, 0xF, 0xF, false));
-}
-template <int CTRL>
-__device__ __forceinline__ double dpp_move(double v) {
-  const long long b = __double_as_longlong(v);
-  const int lo = __builtin_amdgcn_update_dpp(0, (int)b, CTRL, 0xF, 0xF, false);
-  const int hi = __builtin_amdgcn_update_dpp(0, (int)(b >> 32), CTRL, 0xF, 0xF, false);
-  return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
-}
-template <class T>
-__device__ __forceinline__ T lane_prev(T v) { return dpp_move<DPP_WAVE_SHR1>(v); }
-template <class T>
-__device__ __forceinline__ T lane_next(T v) { return dpp_move<DPP_WAVE_SHL1>(v); }
 
 // Gauss-Jordan inverse with partial pivoting by selects (all indices
 // compile-time, the matrix stays in registers); M is destroyed.
@@ -2515,22 +2591,28 @@ static void vjp_nv(cfd_ctx* c, const double* U, const double* w, double* JTw) {
   double* aQ = c->adj + (size_t)14 * m.pitch;  // 2 rows: adjoints of mu_t and F1 (SST)
   hipLaunchKernelGGL((k_gradlim<NV, double>), dim3(nblocks(m.N, GRAD_BS)), dim3(GRAD_BS), 0, c->stream, m, P, U,
                      (const double*)nullptr, c->grad, (double*)nullptr, c->phi, 0, c->tq, (double*)nullptr);
-  launch_fill(c, JTw, 0.0, (size_t)NV * m.pitch);
-  launch_fill(c, aG, 0.0, (size_t)16 * m.pitch);
   // a second-order residual with four variables (Euler or laminar), or with
   // SST and first-order k and omega convection, takes its interior faces
-  // (each color's leading run) through the structured product; the boundary
-  // faces, and every face of the other residuals, through the general one
+  // through the structured product (the adjoint face buffer, then per-cell
+  // sums, which initialise the adjoint rows); the boundary faces (each
+  // color's trailing run), and every face of the other residuals, through
+  // the general one
   constexpr bool has_inv = NV == 4 || VISC != 0;
   const bool inv = has_inv && P.spatial != CFD_FIRST_ORDER && (NV == 4 || (P.turb && P.turb1));
+  if constexpr (has_inv) {
+    if (inv) {
+      hipLaunchKernelGGL((k_vjp_face_inv<NV, VJP_KD, VISC>), dim3(nblocks(m.F, FLUX_BS)), dim3(FLUX_BS), 0, c->stream,
+                         m, P, U, (const double*)c->grad, (const double*)c->phi, (const double*)c->tq, w, c->adjF);
+      hipLaunchKernelGGL((k_vjp_gather<NV, VISC>), dim3(nblocks(m.pitch)), dim3(BS), 0, c->stream, m,
+                         (const double*)c->adjF, JTw, aG, aQ);
+    }
+  }
+  if (!inv) {
+    launch_fill(c, JTw, 0.0, (size_t)NV * m.pitch);
+    launch_fill(c, aG, 0.0, (size_t)16 * m.pitch);
+  }
   for (int col = 0; col < c->ncolors; ++col) {
     const int f0 = c->color_ptr[col], f1 = c->color_ptr[col + 1], fb = inv ? c->color_bptr[col] : f0;
-    if constexpr (has_inv) {
-      if (fb > f0)
-        hipLaunchKernelGGL((k_vjp_face_inv<NV, VJP_KD, VISC>), dim3(nblocks(fb - f0, FLUX_BS)), dim3(FLUX_BS), 0,
-                           c->stream, m, P, U, (const double*)c->grad, (const double*)c->phi,
-                           (const double*)c->tq, w, JTw, aG, aQ, f0, fb);
-    }
     if (f1 > fb)
       hipLaunchKernelGGL((k_vjp_face<NV, VISC>), dim3(nblocks((size_t)(f1 - fb) * vjp_passes<NV>(), FLUX_BS)),
                          dim3(FLUX_BS), 0, c->stream, m, P, U,

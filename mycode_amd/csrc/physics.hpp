@@ -147,11 +147,14 @@ DV_BIN(+, a.d[i] + b.d[i])
 DV_BIN(-, a.d[i] - b.d[i])
 DV_BIN(*, a.d[i] * b.v + a.v * b.d[i])
 #undef DV_BIN
+// The quotients of a K-direction dual all divide by b.v: written with dv
+// (FP64), the K + 1 quotients share one reciprocal instead of running K + 1
+// divisions, with the same values.
 template <class F, int K> HD DualV<F, K> operator/(const DualV<F, K>& a, const DualV<F, K>& b) {
   DualV<F, K> r;
-  r.v = a.v / b.v;
+  r.v = dv(a.v, b.v);
 #pragma unroll
-  for (int i = 0; i < K; ++i) r.d[i] = (a.d[i] - r.v * b.d[i]) / b.v;
+  for (int i = 0; i < K; ++i) r.d[i] = dv(a.d[i] - r.v * b.d[i], b.v);
   return r;
 }
 template <class F, int K> HD DualV<F, K> operator-(const DualV<F, K>& a) {
@@ -182,24 +185,25 @@ CFD_ARITHV HD DualV<F, K> operator*(const DualV<F, K>& a, A b) {
 CFD_ARITHV HD DualV<F, K> operator*(A b, const DualV<F, K>& a) { return a * b; }
 CFD_ARITHV HD DualV<F, K> operator/(const DualV<F, K>& a, A b) {
   DualV<F, K> r;
-  r.v = a.v / F(b);
+  r.v = dv(a.v, F(b));
 #pragma unroll
-  for (int i = 0; i < K; ++i) r.d[i] = a.d[i] / F(b);
+  for (int i = 0; i < K; ++i) r.d[i] = dv(a.d[i], F(b));
   return r;
 }
 CFD_ARITHV HD DualV<F, K> operator/(A b, const DualV<F, K>& a) {
   DualV<F, K> r;
-  r.v = F(b) / a.v;
+  r.v = dv(F(b), a.v);
 #pragma unroll
-  for (int i = 0; i < K; ++i) r.d[i] = -r.v * a.d[i] / a.v;
+  for (int i = 0; i < K; ++i) r.d[i] = dv(-r.v * a.d[i], a.v);
   return r;
 }
 template <class F, int K> HD F val(const DualV<F, K>& x) { return x.v; }
 template <class F, int K> HD DualV<F, K> sq_rt(const DualV<F, K>& x) {
   DualV<F, K> r;
   r.v = sqrt(x.v);
+  const F h = F(2) * r.v;
 #pragma unroll
-  for (int i = 0; i < K; ++i) r.d[i] = x.d[i] / (F(2) * r.v);
+  for (int i = 0; i < K; ++i) r.d[i] = dv(x.d[i], h);
   return r;
 }
 template <class F, int K, class A> HD DualV<F, K> pw(const DualV<F, K>& a, A bb) {
