@@ -127,9 +127,9 @@ struct cfd_ctx {
 namespace cfd {
 // Most blocks of a reduction's first stage (one partial sum per block and sum).
 constexpr int RED_BLOCKS = 1024;
-// Doubles per face of the adjoint face buffer (SST: both sides' 6 state and 8
-// gradient adjoints, 6 averaged-gradient and 4 mu_t / F1 adjoints).
-constexpr int VJP_ROW_MAX = 38;
+// Doubles per face of the adjoint face buffer: two sides of a 16-double row
+// (state and gradient adjoints) and of 2 mu_t / F1 adjoints (SST).
+constexpr int VJP_ROW_MAX = 36;
 // Per-device address of the library's static scratch array (noalloc.hip);
 // called by cfd_ctx_create with the context's device current.
 int noalloc_init(cfd_ctx* c);

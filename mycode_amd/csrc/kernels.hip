@@ -14,6 +14,8 @@
 // Every block index goes through xcd_block(), which gives each of the eight
 // XCDs a contiguous range of cells / faces so neighbouring gathers hit its L2.
 #include "ctx.hpp"
+#include "riemann_vjp.hpp"
+#include "visc.hpp"
 
 namespace cfd {
 
@@ -232,13 +234,6 @@ __global__ __launch_bounds__(BS) void k_turb_src(int N, int p, const F* __restri
 }
 
 // ---- face flux ------------------------------------------------------------------------
-// Geometry and frozen quantities of one face, as the flux core sees them.
-struct FaceGeo {
-  double nx, ny, A, xm, ym, xco, yco, xcn, ycn, phL, phR;
-  double wdo, wdn, dnw;  // SST: wall distances of the two cells, owner normal distance
-  int tg, scheme;
-  bool boundary;
-};
 
 // Every load is unconditional apart from kernel-uniform switches, so the
 // loads of one face go out together: a boundary face (n < 0) reads its owner
@@ -596,7 +591,6 @@ __global__ __launch_bounds__(BS) void k_tcombine(int N, int p, F* __restrict__ J
 constexpr int VK = 4;
 // passes of VK directions that cover a face's inputs (both states, both gradients)
 template <int NV> __host__ __device__ constexpr int vjp_passes() { return (2 * NV + 2 * (NV == 6 ? 14 : 10) + VK - 1) / VK; }
-constexpr int VJP_KD = 8;  // directions per dual pass of k_vjp_face_inv
 
 template <int NV, int VISC>
 __global__ __launch_bounds__(FLUX_BS) void k_vjp_face(DMesh m, Phys P, const double* __restrict__ U,
@@ -682,62 +676,6 @@ __global__ __launch_bounds__(FLUX_BS) void k_vjp_face(DMesh m, Phys P, const dou
   }
 }
 
-// The viscous terms flux_core subtracts on an interior face, written as a
-// function of the two cell states, the six averaged gradient components
-// (u_x, u_y, v_x, v_y, T_x, T_y), which is how they depend on the gradients
-// there, and with SST (TURB) of the two cells' eddy viscosity and F1
-// (tq = mu_t,o, mu_t,n, F1_o, F1_n): the same operations as flux_core's
-// viscous block.  The structured adjoint differentiates it in these inputs.
-template <bool TURB, class T>
-__device__ __forceinline__ S6<T> visc_interior(const Phys& P, const FaceGeo& g, const S6<T>& L, const S6<T>& R,
-                                               const T (&gb)[6], const T* tq) {
-  const double nx = g.nx, ny = g.ny;
-  const T rf = 0.5 * (L[0] + R[0]);
-  const T uf = 0.5 * (dv(L[1], L[0]) + dv(R[1], R[0]));
-  const T vf = 0.5 * (dv(L[2], L[0]) + dv(R[2], R[0]));
-  T ux = gb[0], uy = gb[1], vx = gb[2], vy = gb[3];
-  const T Tx = gb[4], Ty = gb[5];
-  const double dxLR = g.xcn - g.xco, dyLR = g.ycn - g.yco;
-  const double d2 = dxLR * dxLR + dyLR * dyLR;
-  if (d2 > 1e-14) {
-    const T du = dv(R[1], R[0]) - dv(L[1], L[0]);
-    const T dvv = dv(R[2], R[0]) - dv(L[2], L[0]);
-    const T ux0 = ux, uy0 = uy, vx0 = vx, vy0 = vy;
-    ux = ux0 - dv((ux0 * dxLR + uy0 * dyLR - du) * dxLR, d2);
-    uy = uy0 - dv((ux0 * dxLR + uy0 * dyLR - du) * dyLR, d2);
-    vx = vx0 - dv((vx0 * dxLR + vy0 * dyLR - dvv) * dxLR, d2);
-    vy = vy0 - dv((vx0 * dxLR + vy0 * dyLR - dvv) * dyLR, d2);
-  }
-  T mu = rf * P.nu;
-  T kc = dv(mu * P.Cp, P.Pr);
-  T mut = T(0.0), F1f = T(0.0);
-  if constexpr (TURB) {
-    mut = 0.5 * (tq[0] + tq[1]);
-    F1f = 0.5 * (tq[2] + tq[3]);
-    kc = P.Cp * (dv(mu, P.Pr) + dv(mut, sst::Prt));
-    mu = mu + mut;
-  }
-  const T div = ux + vy;
-  const T txx = mu * (2.0 * ux - (2.0 / 3.0) * div);
-  const T tyy = mu * (2.0 * vy - (2.0 / 3.0) * div);
-  const T txy = mu * (uy + vx);
-  const T qx = -kc * Tx, qy = -kc * Ty;
-  const T wx = uf * txx + vf * txy - qx;
-  const T wy = uf * txy + vf * tyy - qy;
-  S6<T> V(T(0.0), txx * nx + txy * ny, txy * nx + tyy * ny, wx * nx + wy * ny);
-  if constexpr (TURB) {  // two-point normal differences of k and omega
-    const T kcl = dv(L[4], L[0]), wcl = dv(L[5], L[0]);
-    double dn = fabs((g.xcn - g.xco) * nx + (g.ycn - g.yco) * ny);
-    if (dn < 1e-300) dn = 1e-300;
-    const T kn = dv(dv(R[4], R[0]) - kcl, dn);
-    const T wn = dv(dv(R[5], R[0]) - wcl, dn);
-    const T mul = rf * P.nu;
-    const T sgk = F1f * sst::sk1 + (1.0 - F1f) * sst::sk2, sgw = F1f * sst::sw1 + (1.0 - F1f) * sst::sw2;
-    V[4] = (mul + sgk * mut) * kn;
-    V[5] = (mul + sgw * mut) * wn;
-  }
-  return V;
-}
 
 // Interior faces of a second-order residual with four variables (Euler or
 // laminar), or with SST and k, omega convected first-order upwind (NV = 6):
@@ -746,15 +684,18 @@ __device__ __forceinline__ S6<T> visc_interior(const Phys& P, const FaceGeo& g, 
 // gradients only through the reconstructed face primitives (rho, u, v, p) of
 // each side, linearly (weights phi dx, phi dy, none where a reconstruction
 // clip took the cell value), and on the cell states through their
-// primitives.  So lambda^T d(F A) is formed by forward duals over the 8 face
-// primitives (passes of KD directions through prim2cons and the Riemann flux)
-// and carried back by hand onto the gradient adjoints and, through the
-// primitive map's Jacobian, onto the states.  The viscous terms (VISC) depend
-// on the two states and on the averages of six gradient components: duals
-// over those 14 inputs.  With SST the k and omega fluxes are the mass flux
-// times the upwind cell's specific k and omega, carried back by hand, and the
-// viscous terms also depend on the cells' mu_t and F1 (read from Q, rows 0
-// and 1).
+// primitives.  So lambda^T d(F A) is formed by the hand-written reverse sweeps
+// of the Riemann flux and of prim2cons (riemann_vjp.hpp) over the 8 face
+// primitives and carried back by hand onto the gradient adjoints and, through
+// the primitive map's Jacobian, onto the states.  The viscous terms (VISC)
+// depend on the two states and on the averages of six gradient components
+// (and with SST on the cells' mu_t and F1, read from Q rows 0 and 1): their
+// adjoints come from the reverse sweep visc_interior_vjp (visc.hpp).  Each
+// sweep costs a few forward evaluations; forward duals took one pass of 8
+// directions through the Riemann flux and two (laminar) or four (SST) passes
+// of 6-7 directions through the viscous terms, and held the whole register
+// file.  With SST the k and omega fluxes are the mass flux times the upwind
+// cell's specific k and omega, carried back by hand.
 //
 // The results go to the face's row of the adjoint face buffer (layout below),
 // not to the cells: k_vjp_gather sums each cell's rows.  Against one launch
@@ -766,32 +707,42 @@ __device__ __forceinline__ S6<T> visc_interior(const Phys& P, const FaceGeo& g, 
 // results through LDS) was measured too: it recomputes the primal flux in
 // every lane and ran at 1.70 ms.
 //
-// Face buffer row (doubles): side 0 then side 1, each the NV state adjoints
-// and gradient slots 0-7; then the six averaged-gradient adjoints (viscous;
-// each goes half to each cell) and the adjoints of mu_t,o, mu_t,n, F1_o, F1_n
-// (SST).  A boundary face's row is zero: the general kernel adds its product
-// straight into the cells' rows.
-template <int NV, int VISC> __host__ __device__ constexpr int vjp_row() {
-  return 2 * (NV + 8) + (VISC ? 6 : 0) + (NV == 6 ? 4 : 0);
-}
-static_assert(vjp_row<6, 1>() <= VJP_ROW_MAX && vjp_row<4, 1>() <= VJP_ROW_MAX, "adjoint face buffer row");
-template <int NV, int KD, int VISC>
-__global__ __launch_bounds__(FLUX_BS) void k_vjp_face_inv(DMesh m, Phys P, const double* __restrict__ U,
+// Face buffer: two arrays of rows, side 0 (the owner's) then side 1 (the
+// neighbour's), each F rows of vjp_row doubles: the side's NV state adjoints
+// and its gradient slots 0-7 (0-9 with viscous terms: the averaged
+// components' adjoints go half to each side); then, with SST, the sides'
+// mu_t and F1 adjoints, (side, face) pairs of doubles.  A cell reads only its
+// own side's rows, and the owner side's rows of a cell's faces are adjacent
+// (faces in owner order).  A boundary face's rows are zero: the general
+// kernel adds its product straight into the cells' rows.
+// One side's row: NV state and 8 (Euler) or 10 (viscous) gradient
+// adjoints.  With SST that is 16 doubles, one 128-byte cache line (the buffer
+// is 256-byte aligned), and the mu_t and F1 adjoints go to a third array of
+// two doubles per side and face: a 144-byte row straddled two lines and the
+// per-cell sums took 271 instead of 233 us on the 1.32M-cell plate.  Padding
+// the Euler rows (12 doubles) to 16 made the bump's sums slower (137 against
+// 105 us): the padding is read too.
+template <int NV, int VISC> __host__ __device__ constexpr int vjp_row() { return NV + (VISC ? 10 : 8); }
+static_assert(vjp_row<6, 1>() == 16 && 2 * 16 + 2 * 2 <= VJP_ROW_MAX, "adjoint face buffer rows");
+template <int NV, int VISC>
+__global__ __launch_bounds__(FLUX_BS) __attribute__((amdgpu_waves_per_eu(2, 8))) void k_vjp_face_inv(DMesh m, Phys P, const double* __restrict__ U,
                                                       const double* __restrict__ G, const double* __restrict__ phi,
                                                       const double* __restrict__ Q, const double* __restrict__ w,
                                                       double* __restrict__ buf) {
-  static_assert((NV == 4 || (NV == 6 && VISC)) && 8 % KD == 0, "KD directions per pass");
+  static_assert(NV == 4 || (NV == 6 && VISC), "four variables, or SST with viscous terms");
   constexpr bool SST = NV == 6;
-  typedef DualV<double, KD> T;
+  constexpr int NGS = VISC ? 10 : 8;
   constexpr int ROW = vjp_row<NV, VISC>();
   const int f = xcd_block() * FLUX_BS + threadIdx.x;
   if (f >= m.F) return;
   const int p = m.pitch;
   const int o = m.own[f], n = m.nei[f];
-  double* row = buf + (size_t)f * ROW;
+  double* rw[2] = {buf + (size_t)f * ROW, buf + ((size_t)m.F + f) * ROW};
+  double* rq[2] = {buf + (size_t)2 * m.F * ROW + (size_t)2 * f, buf + (size_t)2 * m.F * ROW + (size_t)2 * (m.F + f)};
   if (n < 0) {
 #pragma unroll
-    for (int q = 0; q < ROW; ++q) row[q] = 0.0;
+    for (int q = 0; q < NV + NGS; ++q) rw[0][q] = rw[1][q] = 0.0;
+    if (SST) rq[0][0] = rq[0][1] = rq[1][0] = rq[1][1] = 0.0;
     return;
   }
   const FaceGeo g = face_geo(m, P, phi, f, o, n, true);
@@ -832,50 +783,40 @@ __global__ __launch_bounds__(FLUX_BS) void k_vjp_face_inv(DMesh m, Phys P, const
     Wf[sd][2] = vf;
     Wf[sd][3] = pf;
   }
-  // lambda^T d(F A) / d(face primitives), KD directions per pass; with SST
-  // also the mass flux, which carries k and omega
-  double aW[2][4], mflux = 0.0;
-#pragma unroll
-  for (int base = 0; base < 8; base += KD) {
-    T q[2][4];
-#pragma unroll
-    for (int s2 = 0; s2 < 2; ++s2)
-#pragma unroll
-      for (int j = 0; j < 4; ++j) {
-        q[s2][j] = T(Wf[s2][j]);
-#pragma unroll
-        for (int d = 0; d < KD; ++d) q[s2][j].d[d] = (4 * s2 + j == base + d) ? 1.0 : 0.0;
-      }
-    const S6<T> Lr = prim2cons(P, q[0][0], q[0][1], q[0][2], q[0][3]);
-    const S6<T> Rr = prim2cons(P, q[1][0], q[1][1], q[1][2], q[1][3]);
-    const S6<T> Fl = riemann(P, Lr, Rr, g.nx, g.ny, g.scheme);
-    mflux = Fl[0].v;
-    // SST: F_4 = m k_up, F_5 = m omega_up with the upwind cell's values
-    const int up = Fl[0].v > 0.0 ? 0 : 1;
-    const double lkw = SST ? lam[4] * kw[up][0] + lam[5] * kw[up][1] : 0.0;
-#pragma unroll
-    for (int d = 0; d < KD; ++d) {
-      double sacc = 0.0;
-#pragma unroll
-      for (int k = 0; k < 4; ++k) sacc += lam[k] * (Fl[k].d[d] * g.A);
-      if (SST) sacc += lkw * (Fl[0].d[d] * g.A);
-      aW[(base + d) >> 2][(base + d) & 3] = sacc;
+  // lambda^T d(F A) / d(face primitives) by the reverse sweep of the Riemann
+  // flux and of prim2cons; with SST the mass flux also carries k and omega
+  // (its adjoint gains lambda_k k_up + lambda_w w_up, and its sign picks the
+  // upwind cell)
+  double aW[2][4] = {{0.0, 0.0, 0.0, 0.0}, {0.0, 0.0, 0.0, 0.0}}, mflux = 0.0;
+  {
+    const S6<double> Lr = prim2cons(P, Wf[0][0], Wf[0][1], Wf[0][2], Wf[0][3]);
+    const S6<double> Rr = prim2cons(P, Wf[1][0], Wf[1][1], Wf[1][2], Wf[1][3]);
+    double lkw = 0.0;
+    if (SST) {
+      mflux = riemann(P, Lr, Rr, g.nx, g.ny, g.scheme)[0];
+      const int up = mflux > 0.0 ? 0 : 1;
+      lkw = lam[4] * kw[up][0] + lam[5] * kw[up][1];
     }
+    const double Fb[4] = {(lam[0] + lkw) * g.A, lam[1] * g.A, lam[2] * g.A, lam[3] * g.A};
+    double Lb[4] = {0.0, 0.0, 0.0, 0.0}, Rb[4] = {0.0, 0.0, 0.0, 0.0};
+    riemann_vjp(P, Lr, Rr, g.nx, g.ny, g.scheme, Fb, Lb, Rb);
+    prim2cons_vjp(P, Wf[0][0], Wf[0][1], Wf[0][2], Lb, aW[0]);
+    prim2cons_vjp(P, Wf[1][0], Wf[1][1], Wf[1][2], Rb, aW[1]);
   }
   // back through the reconstruction and the primitive map
   const double gm1 = P.gamma - 1.0;
-  double ru[2][NV];
+  double ru[2][NV], gsl[2][4];  // state rows; gradient slots 2-5 (the viscous part adds to them)
 #pragma unroll
   for (int sd = 0; sd < 2; ++sd) {
-    double* rs = row + sd * (NV + 8);
+    double* rs = rw[sd];
     const double wx = ph[sd] * dx[sd], wy = ph[sd] * dy[sd];
     const double ar = aW[sd][0], au = aW[sd][1], av = aW[sd][2], ap = aW[sd][3];
     rs[NV + 0] = clip_r[sd] ? 0.0 : ar * wx;
     rs[NV + 1] = clip_r[sd] ? 0.0 : ar * wy;
-    rs[NV + 2] = au * wx;
-    rs[NV + 3] = au * wy;
-    rs[NV + 4] = av * wx;
-    rs[NV + 5] = av * wy;
+    gsl[sd][0] = au * wx;
+    gsl[sd][1] = au * wy;
+    gsl[sd][2] = av * wx;
+    gsl[sd][3] = av * wy;
     rs[NV + 6] = clip_p[sd] ? 0.0 : ap * wx;
     rs[NV + 7] = clip_p[sd] ? 0.0 : ap * wy;
     const double u = uc[sd], v = vc[sd], ir = 1.0 / rho[sd];
@@ -896,10 +837,9 @@ __global__ __launch_bounds__(FLUX_BS) void k_vjp_face_inv(DMesh m, Phys P, const
     }
   }
   if constexpr (VISC != 0) {
-    // duals over (L, R, averaged gradients[, mu_t and F1 of both cells])
+    // the viscous terms' adjoints of (L, R, averaged gradients[, mu_t and F1
+    // of both cells]) by their reverse sweep, seeded with -lambda A
     constexpr int NIN = 2 * NV + 6 + (SST ? 4 : 0);
-    constexpr int KV = SST ? 6 : 7;
-    typedef DualV<double, KV> TV;
     const int gq[6] = {2, 3, 4, 5, 8, 9};
     double gbv[6], tqv[4] = {0.0, 0.0, 0.0, 0.0};
 #pragma unroll
@@ -910,72 +850,69 @@ __global__ __launch_bounds__(FLUX_BS) void k_vjp_face_inv(DMesh m, Phys P, const
       tqv[2] = Q[p + o];
       tqv[3] = Q[p + n];
     }
-    double ain[NIN + KV];
+    S6<double> Ls, Rs;
+    double Vb[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
 #pragma unroll
-    for (int base = 0; base < NIN; base += KV) {
-      S6<TV> Ld, Rd;
-      TV gbd[6], tqd[4];
+    for (int k = 0; k < NV; ++k) {
+      Ls[k] = Uc[0][k];
+      Rs[k] = Uc[1][k];
+      if (k > 0) Vb[k] = -lam[k] * g.A;
+    }
+    double Lb[6], Rb[6], gbb[6], tqb[4] = {0.0, 0.0, 0.0, 0.0};
+    visc_interior_vjp<SST>(P, g, Ls, Rs, gbv, tqv, Vb, Lb, Rb, gbb, tqb);
+    double ain[NIN];
 #pragma unroll
-      for (int k = 0; k < NV; ++k) {
-        Ld[k] = TV(Uc[0][k]);
-        Rd[k] = TV(Uc[1][k]);
-#pragma unroll
-        for (int d = 0; d < KV; ++d) {
-          Ld[k].d[d] = (k == base + d) ? 1.0 : 0.0;
-          Rd[k].d[d] = (NV + k == base + d) ? 1.0 : 0.0;
-        }
-      }
-#pragma unroll
-      for (int q = 0; q < 6; ++q) {
-        gbd[q] = TV(gbv[q]);
-#pragma unroll
-        for (int d = 0; d < KV; ++d) gbd[q].d[d] = (2 * NV + q == base + d) ? 1.0 : 0.0;
-      }
-#pragma unroll
-      for (int q = 0; q < 4; ++q) {
-        tqd[q] = TV(tqv[q]);
-#pragma unroll
-        for (int d = 0; d < KV; ++d) tqd[q].d[d] = (2 * NV + 6 + q == base + d) ? 1.0 : 0.0;
-      }
-      const S6<TV> Vv = visc_interior<SST>(P, g, Ld, Rd, gbd, tqd);
-#pragma unroll
-      for (int d = 0; d < KV; ++d) {
-        double sacc = 0.0;
-#pragma unroll
-        for (int k = 1; k < NV; ++k) sacc += lam[k] * (-(Vv[k].d[d]) * g.A);
-        ain[base + d] = sacc;
-      }
+    for (int k = 0; k < NV; ++k) {
+      ain[k] = Lb[k];
+      ain[NV + k] = Rb[k];
     }
 #pragma unroll
-    for (int sd = 0; sd < 2; ++sd)
+    for (int q = 0; q < 6; ++q) ain[2 * NV + q] = gbb[q];
+    if (SST) {
+#pragma unroll
+      for (int q = 0; q < 4; ++q) ain[2 * NV + 6 + q] = tqb[q];
+    }
+    // each side: its state rows, half of each averaged component (slots 2-5
+    // add to the reconstruction's; 8 and 9 are the viscous terms' alone) and,
+    // with SST, its own cell's mu_t and F1
+#pragma unroll
+    for (int sd = 0; sd < 2; ++sd) {
+      double* rs = rw[sd];
 #pragma unroll
       for (int k = 0; k < NV; ++k) ru[sd][k] += ain[NV * sd + k];
 #pragma unroll
-    for (int q = 0; q < 6; ++q) row[2 * (NV + 8) + q] = ain[2 * NV + q];
-    if (SST) {
-#pragma unroll
-      for (int q = 0; q < 4; ++q) row[2 * (NV + 8) + 6 + q] = ain[2 * NV + 6 + q];
+      for (int q = 0; q < 4; ++q) rs[NV + 2 + q] = gsl[sd][q] + 0.5 * ain[2 * NV + q];
+      rs[NV + 8] = 0.5 * ain[2 * NV + 4];
+      rs[NV + 9] = 0.5 * ain[2 * NV + 5];
+      if (SST) {
+        rq[sd][0] = ain[2 * NV + 6 + sd];
+        rq[sd][1] = ain[2 * NV + 8 + sd];
+      }
     }
+  } else {
+#pragma unroll
+    for (int sd = 0; sd < 2; ++sd)
+#pragma unroll
+      for (int q = 0; q < 4; ++q) rw[sd][NV + 2 + q] = gsl[sd][q];
   }
 #pragma unroll
   for (int sd = 0; sd < 2; ++sd)
 #pragma unroll
-    for (int k = 0; k < NV; ++k) row[sd * (NV + 8) + k] = ru[sd][k];
+    for (int k = 0; k < NV; ++k) rw[sd][k] = ru[sd][k];
 }
 
 
-// Per cell: the sums of its faces' rows of the adjoint face buffer, in the
-// cell's slot order (zeros past the last cell, up to the pitch).  They are the
-// first values of the cell's state adjoint
-// (aU), its 14 gradient adjoints (aG; the averaged components' adjoints half to
-// each cell, the k and omega slots zero) and, with SST, its mu_t and F1
-// adjoints (aQ): the kernels that follow add to them.
+// Per cell: the sums of its own side's rows of the adjoint face buffer over
+// its faces, in the cell's slot order (zeros past the last cell, up to the
+// pitch).  They are the first values of the cell's state adjoint (aU), its 14
+// gradient adjoints (aG; the k and omega slots zero) and, with SST, its mu_t
+// and F1 adjoints (aQ): the kernels that follow add to them.
 template <int NV, int VISC>
 __global__ __launch_bounds__(BS) void k_vjp_gather(DMesh m, const double* __restrict__ buf, double* __restrict__ aU,
                                                    double* __restrict__ aG, double* __restrict__ aQ) {
   constexpr bool SST = NV == 6;
-  constexpr int ROW = vjp_row<NV, VISC>();
   constexpr int NGS = VISC ? 10 : 8;
+  constexpr int ROW = vjp_row<NV, VISC>();
   const int c = xcd_block() * BS + threadIdx.x;
   const int p = m.pitch;
   if (c >= p) return;
@@ -990,25 +927,16 @@ __global__ __launch_bounds__(BS) void k_vjp_gather(DMesh m, const double* __rest
   for (int j = 0; j < 4; ++j) {
     const int e = sl[j];
     if (e < 0) continue;
-    const int sd = e & 1;
-    const double* r = buf + (size_t)(e >> 1) * ROW;
-    const double* rs = r + sd * (NV + 8);
+    const size_t sf = (size_t)(e & 1) * m.F + (e >> 1);
+    const double* rs = buf + sf * ROW;
 #pragma unroll
     for (int k = 0; k < NV; ++k) su[k] += rs[k];
 #pragma unroll
-    for (int q = 0; q < 8; ++q) sg[q] += rs[NV + q];
-    if constexpr (VISC != 0) {
-      const double* ga = r + 2 * (NV + 8);
-      sg[2] += 0.5 * ga[0];
-      sg[3] += 0.5 * ga[1];
-      sg[4] += 0.5 * ga[2];
-      sg[5] += 0.5 * ga[3];
-      sg[8] += 0.5 * ga[4];
-      sg[9] += 0.5 * ga[5];
-      if constexpr (SST) {
-        sq[0] += ga[6 + sd];
-        sq[1] += ga[8 + sd];
-      }
+    for (int q = 0; q < NGS; ++q) sg[q] += rs[NV + q];
+    if constexpr (SST) {
+      const double* rq = buf + (size_t)2 * m.F * ROW + 2 * sf;
+      sq[0] += rq[0];
+      sq[1] += rq[1];
     }
   }
 #pragma unroll
@@ -1076,31 +1004,28 @@ template <class T> __device__ __forceinline__ void prims(const Phys& P, const S6
 // Green-Gauss transpose, one thread per cell: the gradients are face means
 // of the primitives q over the cell's faces, so with the face mean's adjoint
 // aq_f = A (aG_o . n / V_o - aG_n . n / V_n) (aG holds the 2 slots of each
-// primitive), aU_c += 1/2 (sum_f aq_f)^T dq(U_c)/dU_c, plus on a boundary face
-// 1/2 aq_f^T dq(ghost(U_c))/dU_c.  One Jacobian of q per cell, and no write
-// to another cell, so a single launch.
+// primitive), aU_c += 1/2 (sum_f aq_f)^T dq(U_c)/dU_c, written out for
+// q = (rho, u, v, p, T[, k, omega]); no write to another cell, so a single
+// launch.  The cell's own aG / V is read once, and the product's time term
+// (vol / dt) w (Solver.cpp:1381-1384) is added here.  A boundary face's mean
+// also holds the ghost state's primitives: k_vjp_gg_ghost adds that part.
 template <int NV>
 __global__ __launch_bounds__(BS) void k_vjp_gg(DMesh m, Phys P, const double* __restrict__ U,
-                                                const double* __restrict__ aG, double* __restrict__ aU) {
-  typedef DualV<double, NV> T;
+                                                const double* __restrict__ aG, double* __restrict__ aU,
+                                                const double* __restrict__ w, const double* __restrict__ dt) {
   constexpr int NQM = NV == 6 ? 7 : 5;
   const int c = xcd_block() * BS + threadIdx.x;
   if (c >= m.N) return;
   const int p = m.pitch;
   const bool turb = NV == 6 && P.turb;
   const int nq = turb ? 7 : 5;
-  S6<T> Uc;
+  const double vc = m.vol[c];
+  double gc[2 * NQM];  // the cell's own gradient adjoints
 #pragma unroll
-  for (int j = 0; j < NV; ++j) {
-    Uc[j] = T(U[j * p + c]);
-#pragma unroll
-    for (int d = 0; d < NV; ++d) Uc[j].d[d] = j == d ? 1.0 : 0.0;
-  }
-  double S[NQM], ag[NV];
+  for (int k = 0; k < 2 * NQM; ++k) gc[k] = k < 2 * nq ? aG[(size_t)k * p + c] : 0.0;
+  double S[NQM];
 #pragma unroll
   for (int k = 0; k < NQM; ++k) S[k] = 0.0;
-#pragma unroll
-  for (int j = 0; j < NV; ++j) ag[j] = 0.0;
   const int4 s4 = m.slots[c];
   const int sl[4] = {s4.x, s4.y, s4.z, s4.w};
 #pragma unroll
@@ -1108,41 +1033,83 @@ __global__ __launch_bounds__(BS) void k_vjp_gg(DMesh m, Phys P, const double* __
     const int e = sl[q];
     if (e < 0) continue;
     const int f = e >> 1;
-    const int o = m.own[f], n = m.nei[f];
+    const bool as_nei = e & 1;
+    const int other = as_nei ? m.own[f] : m.nei[f];
     const double nx = m.nx[f], ny = m.ny[f], A = m.area[f];
-    const double vo = m.vol[o], vn = n >= 0 ? m.vol[n] : 1.0;
-    double aq[NQM];
+    const double vb = other >= 0 ? m.vol[other] : 1.0;
 #pragma unroll
     for (int k = 0; k < NQM; ++k) {
-      if (k >= nq) {
-        aq[k] = 0.0;
-        continue;
-      }
-      aq[k] = A * dv(aG[(size_t)(2 * k) * p + o] * nx + aG[(size_t)(2 * k + 1) * p + o] * ny, vo);
-      if (n >= 0)
-        aq[k] -= A * dv(aG[(size_t)(2 * k) * p + n] * nx + aG[(size_t)(2 * k + 1) * p + n] * ny, vn);
-      S[k] += aq[k];
-    }
-    if (n < 0) {  // the ghost state's primitives enter the face mean too
-      T qg[NQM];
-      prims(P, ghost(P, Uc, m.tag[f], nx, ny, turb ? m.dnw[f] : 0.0), qg, turb);
-#pragma unroll
-      for (int j = 0; j < NV; ++j)
-#pragma unroll
-        for (int k = 0; k < NQM; ++k)
-          if (k < nq) ag[j] += aq[k] * 0.5 * qg[k].d[j];
+      if (k >= nq) continue;
+      const double own_part = A * dv(gc[2 * k] * nx + gc[2 * k + 1] * ny, vc);
+      double oth_part = 0.0;
+      if (other >= 0)
+        oth_part = A * dv(aG[(size_t)(2 * k) * p + other] * nx + aG[(size_t)(2 * k + 1) * p + other] * ny, vb);
+      // aq = A (aG_o . n / V_o - aG_n . n / V_n)
+      S[k] += as_nei ? oth_part - own_part : own_part - oth_part;
     }
   }
-  T qc[NQM];
-  prims(P, Uc, qc, turb);
-#pragma unroll
-  for (int j = 0; j < NV; ++j) {
-    double s = ag[j];
-#pragma unroll
-    for (int k = 0; k < NQM; ++k)
-      if (k < nq) s += S[k] * 0.5 * qc[k].d[j];
-    aU[(size_t)j * p + c] += s;
+  // a = S / 2 through dq/dU: u = U1/rho, v = U2/rho, p = (gamma-1)(U3 -
+  // (U1^2 + U2^2)/(2 rho)), T = p / (rho R), k = U4/rho, omega = U5/rho
+  const double r = U[c], u = dv(U[p + c], r), v = dv(U[2 * p + c], r);
+  const double pr = pressure(P, S6<double>(r, U[p + c], U[2 * p + c], U[3 * p + c]));
+  const double gm1 = P.gamma - 1.0, rR = r * P.R, Tc = dv(pr, rR);
+  const double ar = 0.5 * S[0], au = 0.5 * S[1], av = 0.5 * S[2], aT = 0.5 * S[4];
+  const double ap = 0.5 * S[3] + dv(aT, rR);  // p's adjoint, with T's share
+  double wk = 0.0, k_ = 0.0, w_ = 0.0;
+  if (NV == 6 && turb) {
+    k_ = dv(U[4 * p + c], r);
+    w_ = dv(U[5 * p + c], r);
+    wk = 0.5 * S[5] * k_ + 0.5 * S[6] * w_;
   }
+  double s[NV];
+  s[0] = ar - dv(au * u + av * v + wk + aT * Tc, r) + ap * gm1 * 0.5 * (u * u + v * v);
+  s[1] = dv(au, r) - ap * gm1 * u;
+  s[2] = dv(av, r) - ap * gm1 * v;
+  s[3] = ap * gm1;
+  if (NV == 6) {
+    s[4] = turb ? dv(0.5 * S[5], r) : 0.0;
+    s[5] = turb ? dv(0.5 * S[6], r) : 0.0;
+  }
+  const double tc = vc / (dt[c] + 1e-30);
+#pragma unroll
+  for (int j = 0; j < NV; ++j) aU[(size_t)j * p + c] = (aU[(size_t)j * p + c] + s[j]) + w[(size_t)j * p + c] * tc;
+}
+
+// The ghost state's part of k_vjp_gg for the boundary faces [f0, f1) of one
+// color (so no two share an owner): aU_o += 1/2 aq_f^T dq(ghost(U_o))/dU_o,
+// aq_f = A aG_o . n / V_o.  One thread per (face, state row j): a 1-direction
+// dual along U_o[j] gives row j, so each thread writes its own row.
+template <int NV>
+__global__ __launch_bounds__(BS) void k_vjp_gg_ghost(DMesh m, Phys P, const double* __restrict__ U,
+                                                      const double* __restrict__ aG, double* __restrict__ aU,
+                                                      int f0, int f1) {
+  typedef DualV<double, 1> T;
+  constexpr int NQM = NV == 6 ? 7 : 5;
+  const int t = blockIdx.x * BS + threadIdx.x;
+  const int i = f0 + t / NV, j = t % NV;
+  if (i >= f1) return;
+  const int f = m.corder[i];
+  const int c = m.own[f];
+  const int p = m.pitch;
+  const bool turb = NV == 6 && P.turb;
+  const int nq = turb ? 7 : 5;
+  const double nx = m.nx[f], ny = m.ny[f], A = m.area[f], vc = m.vol[c];
+  S6<T> Uc;
+#pragma unroll
+  for (int k = 0; k < NV; ++k) {
+    Uc[k] = T(U[k * p + c]);
+    Uc[k].d[0] = k == j ? 1.0 : 0.0;
+  }
+  T qg[NQM];
+  prims(P, ghost(P, Uc, m.tag[f], nx, ny, turb ? m.dnw[f] : 0.0), qg, turb);
+  double ag = 0.0;
+#pragma unroll
+  for (int k = 0; k < NQM; ++k) {
+    if (k >= nq) continue;
+    const double aq = A * dv(aG[(size_t)(2 * k) * p + c] * nx + aG[(size_t)(2 * k + 1) * p + c] * ny, vc);
+    ag += aq * 0.5 * qg[k].d[0];
+  }
+  aU[(size_t)j * p + c] += ag;
 }
 
 template <class F>
@@ -2601,7 +2568,7 @@ static void vjp_nv(cfd_ctx* c, const double* U, const double* w, double* JTw) {
   const bool inv = has_inv && P.spatial != CFD_FIRST_ORDER && (NV == 4 || (P.turb && P.turb1));
   if constexpr (has_inv) {
     if (inv) {
-      hipLaunchKernelGGL((k_vjp_face_inv<NV, VJP_KD, VISC>), dim3(nblocks(m.F, FLUX_BS)), dim3(FLUX_BS), 0, c->stream,
+      hipLaunchKernelGGL((k_vjp_face_inv<NV, VISC>), dim3(nblocks(m.F, FLUX_BS)), dim3(FLUX_BS), 0, c->stream,
                          m, P, U, (const double*)c->grad, (const double*)c->phi, (const double*)c->tq, w, c->adjF);
       hipLaunchKernelGGL((k_vjp_gather<NV, VISC>), dim3(nblocks(m.pitch)), dim3(BS), 0, c->stream, m,
                          (const double*)c->adjF, JTw, aG, aQ);
@@ -2621,9 +2588,14 @@ static void vjp_nv(cfd_ctx* c, const double* U, const double* w, double* JTw) {
   if (NV == 6 && P.turb)
     hipLaunchKernelGGL((k_vjp_src<NV>), dim3(nblocks(m.N)), dim3(BS), 0, c->stream, m, P, U,
                        (const double*)c->grad, w, (const double*)aQ, JTw, aG);
-  hipLaunchKernelGGL((k_vjp_gg<NV>), dim3(nblocks(m.N)), dim3(BS), 0, c->stream, m, P, U, (const double*)aG, JTw);
-  hipLaunchKernelGGL((k_tcombine<NV, double>), dim3(nblocks(m.N)), dim3(BS), 0, c->stream, m.N, m.pitch, JTw, w,
-                     c->d_vol, c->dt);
+  hipLaunchKernelGGL((k_vjp_gg<NV>), dim3(nblocks(m.N)), dim3(BS), 0, c->stream, m, P, U, (const double*)aG, JTw, w,
+                     (const double*)c->dt);
+  for (int col = 0; col < c->ncolors; ++col) {  // the ghost states' part, boundary faces color by color
+    const int fb = c->color_bptr[col], f1 = c->color_ptr[col + 1];
+    if (f1 > fb)
+      hipLaunchKernelGGL((k_vjp_gg_ghost<NV>), dim3(nblocks((size_t)(f1 - fb) * NV)), dim3(BS), 0, c->stream, m, P, U,
+                         (const double*)aG, JTw, fb, f1);
+  }
 }
 
 void launch_vjp(cfd_ctx* c, const double* U, const double* w, double* JTw) {
