@@ -949,44 +949,35 @@ __global__ __launch_bounds__(BS) void k_vjp_gather(DMesh m, const double* __rest
 
 // SST cell terms: w_c^T d(-V S)/d(U_c, G_c), plus the face sweep's adjoints
 // of the cell's mu_t and F1 (aQ rows 0 and 1) times d(mu_t, F1)/d(U_c, G_c),
-// onto aU_c and aG_c (one thread per cell)
+// onto aU_c and aG_c (one thread per cell), by the reverse sweep of
+// turb_cell (five passes of 4-direction duals took 199 us per product on the
+// 1.32M-cell plate).
 template <int NV>
 __global__ __launch_bounds__(BS) void k_vjp_src(DMesh m, Phys P, const double* __restrict__ U,
                                                  const double* __restrict__ G, const double* __restrict__ w,
                                                  const double* __restrict__ aQ, double* __restrict__ aU,
                                                  double* __restrict__ aG) {
-  typedef DualV<double, VK> T;
+  static_assert(NV == 6, "SST only");
   const int c = xcd_block() * BS + threadIdx.x;
   if (c >= m.N) return;
   const int p = m.pitch;
   const double V = m.vol[c];
-  const double l4 = -V * w[4 * p + c], l5 = -V * w[5 * p + c];
-  const double amut = aQ[c], aF1 = aQ[p + c];
-  for (int base = 0; base < NV + 14; base += VK) {
-    S6<T> Uc;
-    T g[14];
+  S6<double> Uc;
 #pragma unroll
-    for (int j = 0; j < NV; ++j) {
-      Uc[j] = T(U[j * p + c]);
+  for (int j = 0; j < NV; ++j) Uc[j] = U[j * p + c];
+  double g[14];
 #pragma unroll
-      for (int d = 0; d < VK; ++d) Uc[j].d[d] = (j == base + d) ? 1.0 : 0.0;
-    }
+  for (int q = 0; q < 14; ++q) g[q] = G[(size_t)q * p + c];
+  double Ub[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0}, gb[14];
 #pragma unroll
-    for (int q = 0; q < 14; ++q) {
-      g[q] = T(G[(size_t)q * p + c]);
+  for (int q = 0; q < 14; ++q) gb[q] = 0.0;
+  turb_cell_vjp(P, Uc, g, m.wd[c], -V * w[4 * p + c], -V * w[5 * p + c], aQ[c], aQ[p + c], Ub, gb);
+  aU[c] += Ub[0];
+  aU[(size_t)4 * p + c] += Ub[4];
+  aU[(size_t)5 * p + c] += Ub[5];
+  const int rows[8] = {2, 3, 4, 5, 10, 11, 12, 13};
 #pragma unroll
-      for (int d = 0; d < VK; ++d) g[q].d[d] = (NV + q == base + d) ? 1.0 : 0.0;
-    }
-    const TurbQ<T> tq = turb_cell(P, Uc, g, m.wd[c]);
-#pragma unroll
-    for (int d = 0; d < VK; ++d) {
-      const int idx = base + d;
-      if (idx >= NV + 14) break;
-      const double s = l4 * tq.Sk.d[d] + l5 * tq.Sw.d[d] + amut * tq.mut.d[d] + aF1 * tq.F1.d[d];
-      double* dst = idx < NV ? aU + (size_t)idx * p + c : aG + (size_t)(idx - NV) * p + c;
-      *dst = *dst + s;
-    }
-  }
+  for (int q = 0; q < 8; ++q) aG[(size_t)rows[q] * p + c] += gb[rows[q]];
 }
 
 template <class T> __device__ __forceinline__ void prims(const Phys& P, const S6<T>& U, T* q, bool turb) {
@@ -2585,9 +2576,11 @@ static void vjp_nv(cfd_ctx* c, const double* U, const double* w, double* JTw) {
                          dim3(FLUX_BS), 0, c->stream, m, P, U,
                          (const double*)c->grad, (const double*)c->phi, w, JTw, aG, fb, f1);
   }
-  if (NV == 6 && P.turb)
-    hipLaunchKernelGGL((k_vjp_src<NV>), dim3(nblocks(m.N)), dim3(BS), 0, c->stream, m, P, U,
-                       (const double*)c->grad, w, (const double*)aQ, JTw, aG);
+  if constexpr (NV == 6) {
+    if (P.turb)
+      hipLaunchKernelGGL((k_vjp_src<NV>), dim3(nblocks(m.N)), dim3(BS), 0, c->stream, m, P, U,
+                         (const double*)c->grad, w, (const double*)aQ, JTw, aG);
+  }
   hipLaunchKernelGGL((k_vjp_gg<NV>), dim3(nblocks(m.N)), dim3(BS), 0, c->stream, m, P, U, (const double*)aG, JTw, w,
                      (const double*)c->dt);
   for (int col = 0; col < c->ncolors; ++col) {  // the ghost states' part, boundary faces color by color

@@ -357,6 +357,127 @@ template <class T> HD TurbQ<T> turb_cell(const Phys& P, const S6<T>& U, const T*
   q.Sw = gam * rho * S2 - beta * rho * w * w + dv(2.0 * (1.0 - F1) * rho * sw2 * gkw, w);
   return q;
 }
+
+// Reverse sweep of turb_cell for double inputs: adds to Ub (rows 0, 4, 5) and
+// gb (slots 2-5, 10-13) the gradient of lk Sk + lw Sw + amut mu_t + aF1 F1
+// with respect to the cell state and gradients.  The forward values are
+// recomputed in turb_cell's order and every min / max takes the branch
+// turb_cell takes.
+HD void turb_cell_vjp(const Phys& P, const S6<double>& U, const double* g, double d, double lk, double lw,
+                      double amut, double aF1, double (&Ub)[6], double (&gb)[14]) {
+  using namespace sst;
+  const double nu = P.nu;
+  // ---- forward
+  const double rho = U[0];
+  const double k0 = dv(U[4], rho), w0 = dv(U[5], rho);
+  const bool kcl = k0 < 1e-20, wcl = w0 < 1e-20;
+  const double k = kcl ? 1e-20 : k0, w = wcl ? 1e-20 : w0;
+  const double ux = g[2], uy = g[3], vx = g[4], vy = g[5];
+  const double S2 = 2.0 * ux * ux + 2.0 * vy * vy + (uy + vx) * (uy + vx);
+  const double S = sqrt(S2 + 1e-20);
+  const double gkw = g[10] * g[12] + g[11] * g[13];
+  const double CD0 = dv(2.0 * rho * sw2 * gkw, w);
+  const bool cdcl = CD0 < 1e-10;
+  const double CD = cdcl ? 1e-10 : CD0;
+  const double sk = sqrt(k);
+  const double A1 = dv(sk, bs * w * d), A2 = dv(500.0 * nu, d * d * w), A3 = dv(4.0 * rho * sw2 * k, CD * d * d);
+  const bool a2big = A1 < A2;
+  const double mA = a2big ? A2 : A1;
+  const bool a3min = A3 < mA;
+  const double arg1 = a3min ? A3 : mA;
+  const double F1 = tanh(arg1 * arg1 * arg1 * arg1);
+  const double B1 = dv(2.0 * sk, bs * w * d);
+  const bool b2big = B1 < A2;
+  const double arg2 = b2big ? A2 : B1;
+  const double F2 = tanh(arg2 * arg2);
+  const bool dsf = a1 * w < S * F2;
+  const double den = dsf ? S * F2 : a1 * w;
+  const double mut = dv(rho * a1 * k, den);
+  const double P1 = mut * S2, P2 = 10.0 * bs * rho * w * k;
+  const bool p2min = P2 < P1;
+  const double beta = F1 * b1 + (1.0 - F1) * b2;
+  const double gam = F1 * g1() + (1.0 - F1) * g2();
+  // ---- reverse
+  double rhob = 0.0, kb = 0.0, wb = 0.0, S2b = 0.0, gkwb = 0.0, F1b = aF1, mutb = amut;
+  // Sw = gam rho S2 - beta rho w^2 + 2 (1 - F1) rho sw2 gkw / w
+  const double cw = dv(2.0 * rho * sw2 * gkw, w);
+  const double gamb = lw * rho * S2, betab = -lw * rho * w * w;
+  rhob += lw * (gam * S2 - beta * w * w + dv(2.0 * (1.0 - F1) * sw2 * gkw, w));
+  S2b += lw * gam * rho;
+  wb += lw * (-2.0 * beta * rho * w - (1.0 - F1) * dv(cw, w));
+  F1b += -lw * cw;
+  gkwb += lw * dv(2.0 * (1.0 - F1) * rho * sw2, w);
+  // Sk = Pk - bs rho w k
+  const double Pkb = lk;
+  rhob -= lk * bs * w * k;
+  wb -= lk * bs * rho * k;
+  kb -= lk * bs * rho * w;
+  F1b += betab * (b1 - b2) + gamb * (g1() - g2());
+  // Pk = min(mut S2, 10 bs rho w k)
+  if (p2min) {
+    rhob += Pkb * 10.0 * bs * w * k;
+    wb += Pkb * 10.0 * bs * rho * k;
+    kb += Pkb * 10.0 * bs * rho * w;
+  } else {
+    mutb += Pkb * S2;
+    S2b += Pkb * mut;
+  }
+  // mut = rho a1 k / den, den = max(a1 w, S F2)
+  rhob += dv(mutb * a1 * k, den);
+  kb += dv(mutb * rho * a1, den);
+  const double denb = -dv(mutb * mut, den);
+  double Sb = 0.0, F2b = 0.0;
+  if (dsf) {
+    Sb += denb * F2;
+    F2b += denb * S;
+  } else {
+    wb += denb * a1;
+  }
+  // F2 = tanh(arg2^2), arg2 = max(B1, A2)
+  const double arg2b = F2b * (1.0 - F2 * F2) * 2.0 * arg2;
+  double A2b = 0.0, B1b = 0.0, A1b = 0.0, A3b = 0.0;
+  if (b2big) A2b += arg2b;
+  else B1b += arg2b;
+  // F1 = tanh(arg1^4), arg1 = min(max(A1, A2), A3)
+  const double arg1b = F1b * (1.0 - F1 * F1) * 4.0 * arg1 * arg1 * arg1;
+  if (a3min) {
+    A3b += arg1b;
+  } else if (a2big) {
+    A2b += arg1b;
+  } else {
+    A1b += arg1b;
+  }
+  // A1 = sk / (bs w d), B1 = 2 sk / (bs w d), A2 = 500 nu / (d^2 w), A3 = 4 rho sw2 k / (CD d^2)
+  const double skb = dv(A1b + 2.0 * B1b, bs * w * d);
+  wb -= dv(A1b * A1 + B1b * B1 + A2b * A2, w);
+  const double a3c = dv(4.0 * sw2, CD * d * d);
+  rhob += A3b * a3c * k;
+  kb += A3b * a3c * rho;
+  const double CDb = -dv(A3b * A3, CD);
+  kb += dv(skb, 2.0 * sk);
+  // CD = max(2 rho sw2 gkw / w, 1e-10)
+  if (!cdcl) {
+    rhob += CDb * dv(2.0 * sw2 * gkw, w);
+    gkwb += CDb * dv(2.0 * rho * sw2, w);
+    wb -= dv(CDb * CD0, w);
+  }
+  // S = sqrt(S2 + 1e-20)
+  S2b += dv(Sb, 2.0 * S);
+  // S2 = 2 ux^2 + 2 vy^2 + (uy + vx)^2, gkw = kx wx + ky wy
+  gb[2] += S2b * 4.0 * ux;
+  gb[5] += S2b * 4.0 * vy;
+  gb[3] += S2b * 2.0 * (uy + vx);
+  gb[4] += S2b * 2.0 * (uy + vx);
+  gb[10] += gkwb * g[12];
+  gb[12] += gkwb * g[10];
+  gb[11] += gkwb * g[13];
+  gb[13] += gkwb * g[11];
+  // k = max(U4 / rho, 1e-20), w = max(U5 / rho, 1e-20), rho = U0
+  const double k0b = kcl ? 0.0 : kb, w0b = wcl ? 0.0 : wb;
+  Ub[4] += dv(k0b, rho);
+  Ub[5] += dv(w0b, rho);
+  Ub[0] += rhob - dv(k0b * k0 + w0b * w0, rho);
+}
 // omega at a no-slip wall: 10 * 6 nu / (beta1 dn^2), dn the owner centroid's normal distance
 HD double omega_wall(const Phys& P, double dn) { return 60.0 * P.nu / (sst::b1 * dn * dn); }
 template <class T> HD S6<T> frozen(const double* s) {
