@@ -1073,10 +1073,10 @@ static int gmres_cycle(cfd_ctx* c, const double* U, const double* Rb, const doub
         launch_jv_tangent(c, U, k.z, Vj1, prec);
       } else if (prec == 32) {  // the reverse product runs in FP64 on FP32 basis vectors
         launch_f2d(c, (const float*)k.z, tmpU, vs);
-        launch_vjp(c, U, tmpU, tmpR);
+        launch_vjp(c, U, tmpU, tmpR, true);
         launch_d2f(c, tmpR, (float*)Vj1, vs);
       } else {
-        launch_vjp(c, U, (const double*)k.z, (double*)Vj1);
+        launch_vjp(c, U, (const double*)k.z, (double*)Vj1, true);
       }
       if (o->scaling) launch_rowscale(c, Vj1, sc, prec);
       launch_dcgs(c, V, j, false, hslot, prec);
@@ -1173,6 +1173,9 @@ static int gmres_dev(cfd_ctx* c, const double* U, const double* Rb, const double
   } else {
     launch_diag_precond(c, U, k.D);
   }
+  // U is fixed for the whole solve: the reverse products share its gradients,
+  // formed once here (nothing in the solve writes them with another state)
+  if (o->jv == 2) launch_vjp_gradients(c, U);
   launch_reduce(c, RED_L2, U, nullptr, S_UNORM);
   double b2;
   if (o->scaling) {
@@ -1210,7 +1213,7 @@ static int gmres_dev(cfd_ctx* c, const double* U, const double* Rb, const double
           launch_jv_tangent(c, U, xs, Ax, 64);
         }
       } else {
-        launch_vjp(c, U, xs, Ax);
+        launch_vjp(c, U, xs, Ax, true);
       }
       launch_axpby(c, Ax, -1.0, r, sign, vs);  // Ax <- sign r - A xs
       if ((rc = gmres_cycle(c, U, Rb, Ax, 1.0, x, o, tol_abs, &it, &res, tmpU, tmpR))) return rc;

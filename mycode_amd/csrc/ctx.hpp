@@ -159,7 +159,11 @@ void launch_jv_fd(cfd_ctx* c, const double* U, const double* Rb, const double* v
 // forward tangent (limiter frozen): Jv = dR/dU v + (vol/dt) v; prec 64 or 32; v, Jv double or float
 void launch_jv_tangent(cfd_ctx* c, const double* U, const void* v, void* Jv, int prec);
 // reverse product (limiter frozen): JTw = (dR/dU)^T w + (vol/dt) w
-void launch_vjp(cfd_ctx* c, const double* U, const double* w, double* JTw);
+// J^T w; grads_ready: the context's gradient rows already hold U's gradients
+// (launch_vjp_gradients since the last kernel that wrote them), so the product
+// does not form them again
+void launch_vjp(cfd_ctx* c, const double* U, const double* w, double* JTw, bool grads_ready = false);
+void launch_vjp_gradients(cfd_ctx* c, const double* U);
 // vector kernels
 void launch_axpby(cfd_ctx* c, double* y, double a, const double* x, double b, size_t n);
 // w_k *= s_k for the nv rows of an SoA vector of the basis type

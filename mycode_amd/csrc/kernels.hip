@@ -2542,13 +2542,14 @@ static void tangent_nv(cfd_ctx* c, const double* U, const F* v, F* Jv) {
 }
 
 template <int NV, int VISC>
-static void vjp_nv(cfd_ctx* c, const double* U, const double* w, double* JTw) {
+static void vjp_nv(cfd_ctx* c, const double* U, const double* w, double* JTw, bool grads_ready) {
   const DMesh& m = c->dm;
   const Phys& P = c->phys;
   double* aG = c->adj;                         // 14 rows: adjoints of the gradients
   double* aQ = c->adj + (size_t)14 * m.pitch;  // 2 rows: adjoints of mu_t and F1 (SST)
-  hipLaunchKernelGGL((k_gradlim<NV, double>), dim3(nblocks(m.N, GRAD_BS)), dim3(GRAD_BS), 0, c->stream, m, P, U,
-                     (const double*)nullptr, c->grad, (double*)nullptr, c->phi, 0, c->tq, (double*)nullptr);
+  if (!grads_ready)  // the gradients (and SST cell quantities) of U, limiter frozen
+    hipLaunchKernelGGL((k_gradlim<NV, double>), dim3(nblocks(m.N, GRAD_BS)), dim3(GRAD_BS), 0, c->stream, m, P, U,
+                       (const double*)nullptr, c->grad, (double*)nullptr, c->phi, 0, c->tq, (double*)nullptr);
   // a second-order residual with four variables (Euler or laminar), or with
   // SST and first-order k and omega convection, takes its interior faces
   // through the structured product (the adjoint face buffer, then per-cell
@@ -2591,15 +2592,22 @@ static void vjp_nv(cfd_ctx* c, const double* U, const double* w, double* JTw) {
   }
 }
 
-void launch_vjp(cfd_ctx* c, const double* U, const double* w, double* JTw) {
+void launch_vjp(cfd_ctx* c, const double* U, const double* w, double* JTw, bool grads_ready) {
   const bool visc = c->phys.viscous != 0;
   if (c->P.nvar == 6) {
-    if (visc) vjp_nv<6, 1>(c, U, w, JTw);
-    else vjp_nv<6, 0>(c, U, w, JTw);
+    if (visc) vjp_nv<6, 1>(c, U, w, JTw, grads_ready);
+    else vjp_nv<6, 0>(c, U, w, JTw, grads_ready);
   } else {
-    if (visc) vjp_nv<4, 1>(c, U, w, JTw);
-    else vjp_nv<4, 0>(c, U, w, JTw);
+    if (visc) vjp_nv<4, 1>(c, U, w, JTw, grads_ready);
+    else vjp_nv<4, 0>(c, U, w, JTw, grads_ready);
   }
+}
+
+void launch_vjp_gradients(cfd_ctx* c, const double* U) {
+  const DMesh& m = c->dm;
+  NV_DISPATCH(c->P.nvar, hipLaunchKernelGGL((k_gradlim<NVt, double>), dim3(nblocks(m.N, GRAD_BS)), dim3(GRAD_BS), 0,
+                                            c->stream, m, c->phys, U, (const double*)nullptr, c->grad,
+                                            (double*)nullptr, c->phi, 0, c->tq, (double*)nullptr));
 }
 
 template <class F> static void tangent_t(cfd_ctx* c, const double* U, const F* v, F* Jv) {
