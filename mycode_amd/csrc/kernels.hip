@@ -724,6 +724,9 @@ __global__ __launch_bounds__(FLUX_BS) void k_vjp_face(DMesh m, Phys P, const dou
 // 105 us): the padding is read too.
 template <int NV, int VISC> __host__ __device__ constexpr int vjp_row() { return NV + (VISC ? 10 : 8); }
 static_assert(vjp_row<6, 1>() == 16 && 2 * 16 + 2 * 2 <= VJP_ROW_MAX, "adjoint face buffer rows");
+// Held to two waves per SIMD: the SST instance needs 270 registers for one
+// wave and spills 18 at two; both run the plate's face pass in the same time,
+// and the two-wave form leaves room for the four-variable instances (214-238).
 template <int NV, int VISC>
 __global__ __launch_bounds__(FLUX_BS) __attribute__((amdgpu_waves_per_eu(2, 8))) void k_vjp_face_inv(DMesh m, Phys P, const double* __restrict__ U,
                                                       const double* __restrict__ G, const double* __restrict__ phi,
