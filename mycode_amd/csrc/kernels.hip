@@ -144,6 +144,7 @@ void k_gradlim(DMesh m, Phys P, const double* __restrict__ U,
   double pmax = val(pc), pmin = val(pc);
   const int4 s4 = m.slots[c];
   const int sl[4] = {s4.x, s4.y, s4.z, s4.w};
+  bool bnd = false;  // the cell has a boundary face
 #pragma unroll
   for (int k = 0; k < 4; ++k) {
     const int e = sl[k];
@@ -151,6 +152,7 @@ void k_gradlim(DMesh m, Phys P, const double* __restrict__ U,
     const int f = e >> 1;
     const bool as_nei = e & 1;
     const int other = as_nei ? m.own[f] : m.nei[f];
+    bnd = bnd || other < 0;
     const double nx = m.nx[f], ny = m.ny[f], A = m.area[f];
     const S6<T> Uo = other >= 0 ? ldT<NV, T>(U, v, p, other)
                                 : ghost(P, Uc, m.tag[f], nx, ny, turb ? m.dnw[f] : 0.0);
@@ -186,11 +188,15 @@ void k_gradlim(DMesh m, Phys P, const double* __restrict__ U,
     pmin = mn(pmin, (double)val(po));
   }
   const double vol = m.vol[c];
+  // flags bit 1: the k and omega rows (10-13) are read only on boundary faces
+  // (k and omega convected with the cell values), so only cells with one
+  // write them
+  const bool wkw = !(do_lim & 2) || bnd;
 #pragma unroll
   for (int q = 0; q < 2 * NQM; ++q) {
     if (q < 2 * nq) {
       g[q] = dv(g[q], vol);
-      Sc::put(Gv, Gd, (size_t)q * p + c, g[q]);
+      if (q < 10 || wkw) Sc::put(Gv, Gd, (size_t)q * p + c, g[q]);
     }
   }
   if constexpr (NQM == 7) {
@@ -203,7 +209,7 @@ void k_gradlim(DMesh m, Phys P, const double* __restrict__ U,
     }
   }
   if constexpr (std::is_same<T, double>::value) {
-    if (!do_lim) return;
+    if (!(do_lim & 1)) return;
     const double xc = m.xc[c], yc = m.yc[c];
     double ph = 1.0;
 #pragma unroll
@@ -463,8 +469,11 @@ __device__ __forceinline__ S6<T> face_flux(const DMesh& m, const Phys& P, const 
   if (useg) {
 #pragma unroll
     for (int q = 0; q < NG; ++q) {
-      gL[q] = q < ng ? Sc::get(Gv, Gd, (size_t)q * p + o) : T(0.0);
-      gR[q] = q < ng ? Sc::get(Gv, Gd, (size_t)q * p + nn) : T(0.0);
+      // the k and omega rows (10-13) only where they are read: reconstructed
+      // k and omega, or a boundary face's averaged gradients
+      const bool rd = q < ng && (q < 10 || !P.turb1 || n < 0);
+      gL[q] = rd ? Sc::get(Gv, Gd, (size_t)q * p + o) : T(0.0);
+      gR[q] = rd ? Sc::get(Gv, Gd, (size_t)q * p + nn) : T(0.0);
     }
   }
   TurbQ<T> qo, qn;
@@ -2477,9 +2486,11 @@ static void residual_nv(cfd_ctx* c, const double* U, double* R, bool upd, hipEve
   const Phys& P = c->phys;
   const bool lim = P.spatial == CFD_SECOND_ORDER_LIMITED;
   const int do_lim = lim && (upd || !c->phi_valid);
+  const int kw_bnd = NV == 6 && P.turb && P.turb1 ? 2 : 0;  // k, omega rows only where a boundary face reads them
   if (ev) (void)hipEventRecord(ev[0], c->stream);
   hipLaunchKernelGGL((k_gradlim<NV, double>), dim3(nblocks(m.N, GRAD_BS)), dim3(GRAD_BS), 0, c->stream, m, P, U,
-                     (const double*)nullptr, c->grad, (double*)nullptr, c->phi, do_lim, c->tq, (double*)nullptr);
+                     (const double*)nullptr, c->grad, (double*)nullptr, c->phi, do_lim | kw_bnd, c->tq,
+                     (double*)nullptr);
   if (do_lim) c->phi_valid = true;
   if (ev) (void)hipEventRecord(ev[1], c->stream);
   if (c->assembly == 0) {
