@@ -261,7 +261,7 @@ __device__ __forceinline__ FaceGeo face_geo(const DMesh& m, const Phys& P, const
     g.wdo = m.wd[o];
     const double wn = m.wd[nn];
     g.wdn = n >= 0 ? wn : g.wdo;
-    g.dnw = m.dnw[f];
+    g.dnw = n < 0 ? m.dnw[f] : 1.0;  // read only by the ghost state and the wall terms
   }
   if (need_cells) {
     g.xm = m.xm[f];
