@@ -76,7 +76,7 @@ WORKLOADS = {
 }
 
 
-def stage_bytes(N, F, nv, assembly, limiter, turb):
+def stage_bytes(N, F, nv, assembly, limiter, turb, kw_recon=False):
     """Compulsory bytes of each residual stage's own inputs and outputs.
 
     gradients: U, V and slots per cell, face geometry and indices, the
@@ -84,8 +84,10 @@ def stage_bytes(N, F, nv, assembly, limiter, turb):
     mu_t, F1, V S_k, V S_w written; flux: both cells' U, gradients (limiter,
     mu_t and F1) and the centroids, face geometry, and the face buffer (or the
     scatter's read-modify-write of R); assembly: the face buffer, slots and R
-    (and the two SST source rows)."""
-    ng = 14 if turb else 10
+    (and the two SST source rows).  The k and omega gradient rows count only
+    when k and omega are reconstructed (kw_recon); convected with the cell
+    values they are written and read on the boundary alone."""
+    ng = 14 if turb and kw_recon else 10
     grad = N * (8 * nv + 16 + 8 * ng + (8 + 16 if limiter else 0)) + F * (8 + 24 + (16 if limiter else 0))
     if turb:
         grad += N * (8 + 32)
@@ -514,7 +516,7 @@ def main():
     assert np.array_equal(R_dev, R_host), "device and host residual paths disagree"
 
     limiter = p.spatial == cfd.SECOND_ORDER_LIMITED
-    bg, bf, ba = stage_bytes(N, F, nv, args.assembly, limiter, bool(p.turb))
+    bg, bf, ba = stage_bytes(N, F, nv, args.assembly, limiter, bool(p.turb), bool(p.turb) and not p.turb_first_order)
     stages = {"gradients": (bg, ms[0]), "face_flux": (bf, ms[1]), "assembly": (ba, ms[2])}
     dom = max(stages, key=lambda k: stages[k][1])
     b_dom, t_dom = stages[dom]
