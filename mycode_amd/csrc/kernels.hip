@@ -755,6 +755,43 @@ __global__ __launch_bounds__(FLUX_BS) __attribute__((amdgpu_waves_per_eu(2, 8)))
 #pragma unroll
     for (int q = 0; q < NV + NGS; ++q) rw[0][q] = rw[1][q] = 0.0;
     if (SST) rq[0][0] = rq[0][1] = rq[1][0] = rq[1][1] = 0.0;
+    if constexpr (VISC == 0) {
+      // Euler boundary face (no reconstruction there): F A of the owner's
+      // state alone, the wall pressure flux or the Riemann flux against the
+      // ghost state, whose Jacobian comes from 4-direction duals; lambda = w_o
+      const double nx = m.nx[f], ny = m.ny[f], A = m.area[f];
+      const int tg = m.tag[f];
+      S6<double> L;
+#pragma unroll
+      for (int k = 0; k < NV; ++k) L[k] = U[k * p + o];
+      double Fb[4], Lb[4] = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+      for (int k = 0; k < 4; ++k) Fb[k] = w[k * p + o] * A;
+      if (tg == 3 || tg == 4) {  // (0, p nx, p ny, 0)
+        prim_uvp_vjp(P, L, dv(L[1], L[0]), dv(L[2], L[0]), Fb[1] * nx + Fb[2] * ny, 0.0, 0.0, Lb);
+      } else {
+        typedef DualV<double, 4> T4;
+        S6<T4> Ld;
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+          Ld[k] = T4(L[k]);
+#pragma unroll
+          for (int d = 0; d < 4; ++d) Ld[k].d[d] = k == d ? 1.0 : 0.0;
+        }
+        const S6<T4> Rg = ghost(P, Ld, tg, nx, ny, 0.0);
+        S6<double> R;
+#pragma unroll
+        for (int k = 0; k < 4; ++k) R[k] = Rg[k].v;
+        double Rb[4] = {0.0, 0.0, 0.0, 0.0};
+        riemann_vjp(P, L, R, nx, ny, (m.alt && m.alt[f]) ? P.flux_alt : P.flux, Fb, Lb, Rb);
+#pragma unroll
+        for (int j = 0; j < 4; ++j)
+#pragma unroll
+          for (int k = 0; k < 4; ++k) Lb[j] += Rb[k] * Rg[k].d[j];
+      }
+#pragma unroll
+      for (int k = 0; k < 4; ++k) rw[0][k] = Lb[k];
+    }
     return;
   }
   const FaceGeo g = face_geo(m, P, phi, f, o, n, true);
@@ -2584,7 +2621,8 @@ static void vjp_nv(cfd_ctx* c, const double* U, const double* w, double* JTw, bo
     launch_fill(c, JTw, 0.0, (size_t)NV * m.pitch);
     launch_fill(c, aG, 0.0, (size_t)16 * m.pitch);
   }
-  for (int col = 0; col < c->ncolors; ++col) {
+  // (with Euler the structured pass also covers the boundary faces)
+  for (int col = 0; col < c->ncolors && !(inv && VISC == 0); ++col) {
     const int f0 = c->color_ptr[col], f1 = c->color_ptr[col + 1], fb = inv ? c->color_bptr[col] : f0;
     if (f1 > fb)
       hipLaunchKernelGGL((k_vjp_face<NV, VISC>), dim3(nblocks((size_t)(f1 - fb) * vjp_passes<NV>(), FLUX_BS)),
