@@ -953,12 +953,15 @@ __global__ __launch_bounds__(FLUX_BS) __attribute__((amdgpu_waves_per_eu(2, 8)))
 
 // Per cell: the sums of its own side's rows of the adjoint face buffer over
 // its faces, in the cell's slot order (zeros past the last cell, up to the
-// pitch).  They are the first values of the cell's state adjoint (aU), its 14
-// gradient adjoints (aG; the k and omega slots zero) and, with SST, its mu_t
-// and F1 adjoints (aQ): the kernels that follow add to them.
+// pitch), and with SST the cell terms' part (the reverse sweep of turb_cell:
+// the cell's source adjoint -V w and the summed mu_t and F1 adjoints, which
+// are not stored).  They are the first values of the cell's state adjoint
+// (aU) and its 14 gradient adjoints (aG): the kernels that follow add to them.
 template <int NV, int VISC>
-__global__ __launch_bounds__(BS) void k_vjp_gather(DMesh m, const double* __restrict__ buf, double* __restrict__ aU,
-                                                   double* __restrict__ aG, double* __restrict__ aQ) {
+__global__ __launch_bounds__(BS) void k_vjp_gather(DMesh m, Phys P, const double* __restrict__ buf,
+                                                   const double* __restrict__ U, const double* __restrict__ G,
+                                                   const double* __restrict__ w, double* __restrict__ aU,
+                                                   double* __restrict__ aG) {
   constexpr bool SST = NV == 6;
   constexpr int NGS = VISC ? 10 : 8;
   constexpr int ROW = vjp_row<NV, VISC>();
@@ -967,11 +970,11 @@ __global__ __launch_bounds__(BS) void k_vjp_gather(DMesh m, const double* __rest
   if (c >= p) return;
   const int4 s4 = c < m.N ? m.slots[c] : make_int4(-1, -1, -1, -1);  // the padding cells get zeros
   const int sl[4] = {s4.x, s4.y, s4.z, s4.w};
-  double su[NV], sg[NGS], sq[2] = {0.0, 0.0};
+  double su[NV], sg[14], sq[2] = {0.0, 0.0};
 #pragma unroll
   for (int k = 0; k < NV; ++k) su[k] = 0.0;
 #pragma unroll
-  for (int q = 0; q < NGS; ++q) sg[q] = 0.0;
+  for (int q = 0; q < 14; ++q) sg[q] = 0.0;
 #pragma unroll
   for (int j = 0; j < 4; ++j) {
     const int e = sl[j];
@@ -988,12 +991,26 @@ __global__ __launch_bounds__(BS) void k_vjp_gather(DMesh m, const double* __rest
       sq[1] += rq[1];
     }
   }
+  if constexpr (SST) {
+    if (P.turb && c < m.N) {
+      S6<double> Uc;
+#pragma unroll
+      for (int j = 0; j < NV; ++j) Uc[j] = U[j * p + c];
+      double g[14];
+#pragma unroll
+      for (int q = 0; q < 14; ++q) g[q] = G[(size_t)q * p + c];
+      const double V = m.vol[c];
+      double Ub[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+      turb_cell_vjp(P, Uc, g, m.wd[c], -V * w[4 * p + c], -V * w[5 * p + c], sq[0], sq[1], Ub, sg);
+      su[0] += Ub[0];
+      su[4] += Ub[4];
+      su[5] += Ub[5];
+    }
+  }
 #pragma unroll
   for (int k = 0; k < NV; ++k) aU[k * p + c] = su[k];
 #pragma unroll
-  for (int q = 0; q < 14; ++q) aG[(size_t)q * p + c] = q < NGS ? sg[q] : 0.0;
-  aQ[c] = sq[0];
-  aQ[p + c] = sq[1];
+  for (int q = 0; q < 14; ++q) aG[(size_t)q * p + c] = sg[q];
 }
 
 // SST cell terms: w_c^T d(-V S)/d(U_c, G_c), plus the face sweep's adjoints
@@ -2613,8 +2630,8 @@ static void vjp_nv(cfd_ctx* c, const double* U, const double* w, double* JTw, bo
     if (inv) {
       hipLaunchKernelGGL((k_vjp_face_inv<NV, VISC>), dim3(nblocks(m.F, FLUX_BS)), dim3(FLUX_BS), 0, c->stream,
                          m, P, U, (const double*)c->grad, (const double*)c->phi, (const double*)c->tq, w, c->adjF);
-      hipLaunchKernelGGL((k_vjp_gather<NV, VISC>), dim3(nblocks(m.pitch)), dim3(BS), 0, c->stream, m,
-                         (const double*)c->adjF, JTw, aG, aQ);
+      hipLaunchKernelGGL((k_vjp_gather<NV, VISC>), dim3(nblocks(m.pitch)), dim3(BS), 0, c->stream, m, P,
+                         (const double*)c->adjF, U, (const double*)c->grad, w, JTw, aG);
     }
   }
   if (!inv) {
@@ -2629,8 +2646,8 @@ static void vjp_nv(cfd_ctx* c, const double* U, const double* w, double* JTw, bo
                          dim3(FLUX_BS), 0, c->stream, m, P, U,
                          (const double*)c->grad, (const double*)c->phi, w, JTw, aG, fb, f1);
   }
-  if constexpr (NV == 6) {
-    if (P.turb)
+  if constexpr (NV == 6) {  // (the structured path forms the SST cell terms in k_vjp_gather)
+    if (P.turb && !inv)
       hipLaunchKernelGGL((k_vjp_src<NV>), dim3(nblocks(m.N)), dim3(BS), 0, c->stream, m, P, U,
                          (const double*)c->grad, w, (const double*)aQ, JTw, aG);
   }
