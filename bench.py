@@ -387,7 +387,7 @@ def design_gradient_leg(h=1e-6, fd_modes=(5, 20, 31, 47), deadline=None):
     print(f"bench: design: steady state in {res.steps} Newton steps, {t_conv:.1f} s", file=sys.stderr, flush=True)
     modes = adjoint.design_modes(mesh, 64)
     t1 = time.perf_counter()
-    psi, its, outer, defect = adjoint.adjoint_solve(
+    psi, its, outer, defect = adjoint.adjoint_solve_fgmres(
         s, U, adjoint.functional_dU(mesh, p, U),
         log=lambda msg: print("bench: design: " + msg, file=sys.stderr, flush=True))
     t_adj = time.perf_counter() - t1
@@ -398,8 +398,9 @@ def design_gradient_leg(h=1e-6, fd_modes=(5, 20, 31, 47), deadline=None):
           file=sys.stderr, flush=True)
     out = {"workload": WORKLOADS[4] + f" ({mesh.n_cells} cells)", "converged": bool(res.converged),
            "newton_steps": res.steps, "drop": float(hist["vm"][-1] / hist["vm"][0]), "converge_s": t_conv,
-           "adjoint_s": t_adj, "adjoint_outer_iterations": outer, "adjoint_gmres_iterations": its,
-           "adjoint_relative_defect": defect, "gradient_64_modes_s": t_grad,
+           "adjoint_s": t_adj, "adjoint_solver": "FGMRES(200), inner transposed GMRES(60) at CFL 1e6",
+           "adjoint_outer_iterations": outer, "adjoint_gmres_iterations": its,
+           "adjoint_relative_residual": defect, "gradient_64_modes_s": t_grad,
            "functional": adjoint.functional(mesh, p, U), "gradient": [float(x) for x in g], "fd_check": []}
     x0, y0, _, _, _ = mesh.desc()
     last = 0.0

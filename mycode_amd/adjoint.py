@@ -6,8 +6,8 @@ J(U, X), the total derivative is
     dJ/dalpha_k = dJ/dX . dX/dalpha_k  -  psi^T (dR/dX . dX/dalpha_k),
     (dR/dU)^T psi = (dJ/dU)^T,
 
-psi from GMRES on the transposed system (the library's reverse product,
-``jv = 2``), and the two partials at fixed U by central differences of the
+psi from flexible GMRES on the transposed system, preconditioned by GMRES
+on the shifted transposed system (the library's reverse product, ``jv = 2``), and the two partials at fixed U by central differences of the
 geometry (the residual and J are evaluated on the GPU with the moved mesh).
 
 Configuration 4: J = sum over the bump faces (bottom wall, 1 <= x <= 2) of
@@ -150,6 +150,109 @@ def adjoint_solve(solver: cfd.Solver, U, rhs, tol=1e-10, cfl0=10.0, cfl_max=1e6,
     return out, its, k, rn / bn
 
 
+def adjoint_solve_fgmres(solver: cfd.Solver, U, rhs, tol=1e-10, cfl=1e6, m_outer=200, max_outer=1000, m=60,
+                         restarts=0, inner_tol=1e-2, precond=2, precision=32, log=None, log_every=25):
+    """psi with (dR/dU)^T psi = rhs at the steady state U by flexible GMRES
+    (FGMRES(m_outer), FP64 basis, classical Gram-Schmidt with one
+    reorthogonalisation, Hessenberg least squares on the host): each outer
+    step's preconditioner is one approximate solve of the shifted transposed
+    system (dR/dU^T + V/dtau(cfl)) z = v by the library's GMRES (``jv = 2``,
+    GMRES(m) with `restarts` at `inner_tol`, block SGS, `precision` basis), so
+    the outer Krylov space accelerates the pseudo-time iteration that
+    adjoint_solve runs as plain defect correction.  The outer products are
+    the steady reverse product.  Every vector stays on the device; the outer
+    space holds 2 m_outer + 1 FP64 vectors (10.6 GB for the 824,378-cell bump
+    at m_outer = 200), sized so that the outer iteration rarely restarts.
+
+    On the configuration-4 bump at full size (tol 1e-10) the defaults took
+    170 outer steps and 10,200 inner iterations, where the defect correction
+    needed 376 outer steps and 67,041 inner iterations; restarting the outer
+    iteration every 60 or 100 steps cost 2.2x or 1.6x the inner iterations.
+    Returns (psi (N x 6, reference order), inner GMRES iterations, outer
+    iterations, relative residual |rhs - dR/dU^T psi| / |rhs|)."""
+    import torch
+
+    dev = torch.device("cuda")
+    nv, P, N = solver.params().nvar, solver.pitch, solver.N
+    n = nv * P
+    solver.set_stream(torch.cuda.current_stream().cuda_stream)
+    try:
+        def to_dev(a6):
+            t6 = torch.from_numpy(np.ascontiguousarray(a6, np.float64)).to(dev)
+            out = torch.zeros(n, dtype=torch.float64, device=dev)
+            solver.dev_pack(t6.data_ptr(), out.data_ptr())
+            return out, t6
+
+        Ud, _k1 = to_dev(U)
+        rhs_d, _k2 = to_dev(rhs)
+        zero = torch.zeros_like(Ud)
+        solver.dev_residual(Ud.data_ptr(), zero.data_ptr(), True)  # forms the limiter the products hold frozen
+        zero.zero_()
+        opts = solver.gmres_opts(m=m, tol=inner_tol, jv=2, precision=precision, precond=precond, restarts=restarts)
+        V = torch.zeros((m_outer + 1, n), dtype=torch.float64, device=dev)
+        Z = torch.zeros((m_outer, n), dtype=torch.float64, device=dev)
+        w = torch.zeros(n, dtype=torch.float64, device=dev)
+        psi = torch.zeros_like(Ud)
+        bn = float(torch.linalg.norm(rhs_d))
+        r = rhs_d.clone()
+        rn = bn
+        inner, outer = 0, 0
+
+        def apply_steady(x, out):
+            solver.dev_local_timestep(Ud.data_ptr(), 1e30)  # the time term vanishes
+            solver.dev_vjp_adjoint(Ud.data_ptr(), x.data_ptr(), out.data_ptr())
+
+        while rn > tol * bn and outer < max_outer:
+            V[0] = r / rn
+            H = np.zeros((m_outer + 1, m_outer))
+            g = np.zeros(m_outer + 1)
+            g[0] = rn
+            cs, sn = np.zeros(m_outer), np.zeros(m_outer)
+            j_end = 0
+            for j in range(m_outer):
+                solver.dev_local_timestep(Ud.data_ptr(), cfl)
+                Z[j].zero_()
+                inner += solver.dev_gmres(Ud.data_ptr(), zero.data_ptr(), V[j].data_ptr(), Z[j].data_ptr(), opts)
+                apply_steady(Z[j], w)
+                h = torch.mv(V[: j + 1], w)  # classical Gram-Schmidt, twice
+                w -= torch.mv(V[: j + 1].t(), h)
+                h2 = torch.mv(V[: j + 1], w)
+                w -= torch.mv(V[: j + 1].t(), h2)
+                hc = (h + h2).cpu().numpy()
+                hn = float(torch.linalg.norm(w))
+                H[: j + 1, j] = hc
+                H[j + 1, j] = hn
+                if hn > 0.0:
+                    V[j + 1] = w / hn
+                for i in range(j):  # the earlier rotations on the new column
+                    t = cs[i] * H[i, j] + sn[i] * H[i + 1, j]
+                    H[i + 1, j] = -sn[i] * H[i, j] + cs[i] * H[i + 1, j]
+                    H[i, j] = t
+                den = np.hypot(H[j, j], H[j + 1, j])
+                cs[j], sn[j] = (1.0, 0.0) if den == 0.0 else (H[j, j] / den, H[j + 1, j] / den)
+                H[j, j] = den
+                H[j + 1, j] = 0.0
+                g[j + 1] = -sn[j] * g[j]
+                g[j] = cs[j] * g[j]
+                outer += 1
+                j_end = j + 1
+                if log and outer % log_every == 0:
+                    log(f"adjoint fgmres {outer}: estimate {abs(g[j + 1]) / bn:.3e}, inner {inner}")
+                if abs(g[j + 1]) <= tol * bn or hn == 0.0 or outer >= max_outer:
+                    break
+            y = np.linalg.solve(np.triu(H[:j_end, :j_end]), g[:j_end]) if j_end else np.zeros(0)
+            psi += torch.mv(Z[:j_end].t(), torch.from_numpy(y).to(dev))
+            apply_steady(psi, w)
+            r = rhs_d - w
+            rn = float(torch.linalg.norm(r))
+        psi6 = torch.zeros((N, 6), dtype=torch.float64, device=dev)
+        solver.dev_unpack(psi.data_ptr(), psi6.data_ptr())
+        out = psi6.cpu().numpy()
+    finally:
+        solver.set_stream(None)
+    return out, inner, outer, rn / bn
+
+
 def gradient_from_adjoint(mesh: cfd.Mesh, solver: cfd.Solver, p: cfd.Params, U, modes, psi, step=1e-7):
     """dJ/dalpha_k = dJ/dX . dX/dalpha_k - psi^T (dR/dX . dX/dalpha_k) for every
     row of `modes`, the two partials at fixed U by central differences of the
@@ -177,7 +280,7 @@ def gradient_from_adjoint(mesh: cfd.Mesh, solver: cfd.Solver, p: cfd.Params, U, 
 def design_gradient(mesh: cfd.Mesh, solver: cfd.Solver, p: cfd.Params, U, modes, step=1e-7, tol=1e-10, log=None):
     """Adjoint gradient dJ/dalpha for every row of `modes` at the converged
     state U; returns (gradient, psi, adjoint GMRES iterations)."""
-    psi, it, _, _ = adjoint_solve(solver, U, functional_dU(mesh, p, U), tol=tol, log=log)
+    psi, it, _, _ = adjoint_solve_fgmres(solver, U, functional_dU(mesh, p, U), tol=tol, log=log)
     return gradient_from_adjoint(mesh, solver, p, U, modes, psi, step), psi, it
 
 
