@@ -398,6 +398,8 @@ def design_gradient_leg(h=1e-6, fd_modes=(5, 20, 31, 47), deadline=None):
           file=sys.stderr, flush=True)
     out = {"workload": WORKLOADS[4] + f" ({mesh.n_cells} cells)", "converged": bool(res.converged),
            "newton_steps": res.steps, "drop": float(hist["vm"][-1] / hist["vm"][0]), "converge_s": t_conv,
+           "steady_solver": "pseudo-time JFNK (tangent, FP32 Krylov, block SGS) to a 1e-4 drop, then Newton "
+                            "steps solved by FGMRES(100) over GMRES(60) at CFL 1e6",
            "adjoint_s": t_adj, "adjoint_solver": "FGMRES(200), inner transposed GMRES(60) at CFL 1e6",
            "adjoint_outer_iterations": outer, "adjoint_gmres_iterations": its,
            "adjoint_relative_residual": defect, "gradient_64_modes_s": t_grad,
@@ -414,7 +416,8 @@ def design_gradient_leg(h=1e-6, fd_modes=(5, 20, 31, 47), deadline=None):
             m2 = cfd.Mesh.generate(4, 1.0)
             m2.set_nodes(x0, y0 + sgn * h * modes[k])
             s2 = cfd.Solver(m2, p)
-            U2, r2, _ = adjoint.converge(s2, U, rel_drop=0.0, tolerance=2.0 * res.res_vm, cfl0=1e4, log_every=50)
+            U2, r2, _ = adjoint.converge(s2, U, rel_drop=0.0, tolerance=2.0 * res.res_vm, cfl0=1e4, log_every=50,
+                                         newton_below=1.0)
             J.append(adjoint.functional(m2, p, U2))
         fd = (J[0] - J[1]) / (2 * h)
         out["fd_check"].append({"mode": k, "adjoint": float(g[k]), "central_fd": fd, "step": h,

@@ -150,13 +150,96 @@ def adjoint_solve(solver: cfd.Solver, U, rhs, tol=1e-10, cfl0=10.0, cfl_max=1e6,
     return out, its, k, rn / bn
 
 
+def _fgmres(apply_A, apply_M, b, x, tol, V, Z, w, max_outer, log=None, log_every=25, what="fgmres"):
+    """Flexible GMRES on device vectors (torch FP64, the library's layout):
+    x += the correction that brings |b - A x| to tol |b| or below, restarted
+    every Z.shape[0] steps, at most max_outer steps in all.  Step j
+    preconditions the basis vector v_j by z_j = apply_M(v_j) (any
+    approximate solve; apply_M(v, z) writes z and returns its own inner
+    iteration count) and orthogonalises A z_j by classical Gram-Schmidt with
+    one reorthogonalisation; the Hessenberg least squares (Givens rotations)
+    runs on the host in FP64.  V ((m + 1) x n), Z (m x n) and w (n) are
+    the caller's work space.  Returns (inner iterations, outer steps,
+    relative residual |b - A x| / |b| recomputed at the end)."""
+    import torch
+
+    m = Z.shape[0]
+    bn = float(torch.linalg.norm(b))
+    if bn == 0.0:
+        return 0, 0, 0.0
+    apply_A(x, w)
+    r = b - w
+    rn = float(torch.linalg.norm(r))
+    inner, outer = 0, 0
+    while rn > tol * bn and outer < max_outer:
+        V[0] = r / rn
+        H = np.zeros((m + 1, m))
+        g = np.zeros(m + 1)
+        g[0] = rn
+        cs, sn = np.zeros(m), np.zeros(m)
+        j_end = 0
+        for j in range(m):
+            Z[j].zero_()
+            inner += apply_M(V[j], Z[j])
+            apply_A(Z[j], w)
+            h = torch.mv(V[: j + 1], w)  # classical Gram-Schmidt, twice
+            w -= torch.mv(V[: j + 1].t(), h)
+            h2 = torch.mv(V[: j + 1], w)
+            w -= torch.mv(V[: j + 1].t(), h2)
+            hn = float(torch.linalg.norm(w))
+            H[: j + 1, j] = (h + h2).cpu().numpy()
+            H[j + 1, j] = hn
+            if hn > 0.0:
+                V[j + 1] = w / hn
+            for i in range(j):  # the earlier rotations on the new column
+                t = cs[i] * H[i, j] + sn[i] * H[i + 1, j]
+                H[i + 1, j] = -sn[i] * H[i, j] + cs[i] * H[i + 1, j]
+                H[i, j] = t
+            den = np.hypot(H[j, j], H[j + 1, j])
+            cs[j], sn[j] = (1.0, 0.0) if den == 0.0 else (H[j, j] / den, H[j + 1, j] / den)
+            H[j, j] = den
+            H[j + 1, j] = 0.0
+            g[j + 1] = -sn[j] * g[j]
+            g[j] = cs[j] * g[j]
+            outer += 1
+            j_end = j + 1
+            if log and outer % log_every == 0:
+                log(f"{what} {outer}: estimate {abs(g[j + 1]) / bn:.3e}, inner {inner}")
+            if abs(g[j + 1]) <= tol * bn or hn == 0.0 or outer >= max_outer:
+                break
+        y = np.linalg.solve(np.triu(H[:j_end, :j_end]), g[:j_end])
+        x += torch.mv(Z[:j_end].t(), torch.from_numpy(y).to(x.device))
+        apply_A(x, w)
+        r = b - w
+        rn = float(torch.linalg.norm(r))
+    return inner, outer, rn / bn
+
+
+def _to_device(solver: cfd.Solver, a6):
+    """(N x 6) host array -> device vector in the library's layout."""
+    import torch
+
+    t6 = torch.from_numpy(np.ascontiguousarray(a6, np.float64)).to("cuda")
+    out = torch.zeros(solver.params().nvar * solver.pitch, dtype=torch.float64, device="cuda")
+    solver.dev_pack(t6.data_ptr(), out.data_ptr())
+    torch.cuda.current_stream().synchronize()  # t6 may be freed on return
+    return out
+
+
+def _to_host(solver: cfd.Solver, x):
+    import torch
+
+    x6 = torch.zeros((solver.N, 6), dtype=torch.float64, device="cuda")
+    solver.dev_unpack(x.data_ptr(), x6.data_ptr())
+    return x6.cpu().numpy()
+
+
 def adjoint_solve_fgmres(solver: cfd.Solver, U, rhs, tol=1e-10, cfl=1e6, m_outer=200, max_outer=1000, m=60,
                          restarts=0, inner_tol=1e-2, precond=2, precision=32, log=None, log_every=25):
     """psi with (dR/dU)^T psi = rhs at the steady state U by flexible GMRES
-    (FGMRES(m_outer), FP64 basis, classical Gram-Schmidt with one
-    reorthogonalisation, Hessenberg least squares on the host): each outer
-    step's preconditioner is one approximate solve of the shifted transposed
-    system (dR/dU^T + V/dtau(cfl)) z = v by the library's GMRES (``jv = 2``,
+    (FGMRES(m_outer), FP64 basis): each outer step's preconditioner is one
+    approximate solve of the shifted transposed system
+    (dR/dU^T + V/dtau(cfl)) z = v by the library's GMRES (``jv = 2``,
     GMRES(m) with `restarts` at `inner_tol`, block SGS, `precision` basis), so
     the outer Krylov space accelerates the pseudo-time iteration that
     adjoint_solve runs as plain defect correction.  The outer products are
@@ -172,85 +255,34 @@ def adjoint_solve_fgmres(solver: cfd.Solver, U, rhs, tol=1e-10, cfl=1e6, m_outer
     iterations, relative residual |rhs - dR/dU^T psi| / |rhs|)."""
     import torch
 
-    dev = torch.device("cuda")
-    nv, P, N = solver.params().nvar, solver.pitch, solver.N
-    n = nv * P
+    n = solver.params().nvar * solver.pitch
     solver.set_stream(torch.cuda.current_stream().cuda_stream)
     try:
-        def to_dev(a6):
-            t6 = torch.from_numpy(np.ascontiguousarray(a6, np.float64)).to(dev)
-            out = torch.zeros(n, dtype=torch.float64, device=dev)
-            solver.dev_pack(t6.data_ptr(), out.data_ptr())
-            return out, t6
-
-        Ud, _k1 = to_dev(U)
-        rhs_d, _k2 = to_dev(rhs)
+        Ud = _to_device(solver, U)
+        rhs_d = _to_device(solver, rhs)
         zero = torch.zeros_like(Ud)
         solver.dev_residual(Ud.data_ptr(), zero.data_ptr(), True)  # forms the limiter the products hold frozen
         zero.zero_()
         opts = solver.gmres_opts(m=m, tol=inner_tol, jv=2, precision=precision, precond=precond, restarts=restarts)
-        V = torch.zeros((m_outer + 1, n), dtype=torch.float64, device=dev)
-        Z = torch.zeros((m_outer, n), dtype=torch.float64, device=dev)
-        w = torch.zeros(n, dtype=torch.float64, device=dev)
-        psi = torch.zeros_like(Ud)
-        bn = float(torch.linalg.norm(rhs_d))
-        r = rhs_d.clone()
-        rn = bn
-        inner, outer = 0, 0
 
-        def apply_steady(x, out):
-            solver.dev_local_timestep(Ud.data_ptr(), 1e30)  # the time term vanishes
+        def apply_A(x, out):  # the steady reverse product: the time term vanishes
+            solver.dev_local_timestep(Ud.data_ptr(), 1e30)
             solver.dev_vjp_adjoint(Ud.data_ptr(), x.data_ptr(), out.data_ptr())
 
-        while rn > tol * bn and outer < max_outer:
-            V[0] = r / rn
-            H = np.zeros((m_outer + 1, m_outer))
-            g = np.zeros(m_outer + 1)
-            g[0] = rn
-            cs, sn = np.zeros(m_outer), np.zeros(m_outer)
-            j_end = 0
-            for j in range(m_outer):
-                solver.dev_local_timestep(Ud.data_ptr(), cfl)
-                Z[j].zero_()
-                inner += solver.dev_gmres(Ud.data_ptr(), zero.data_ptr(), V[j].data_ptr(), Z[j].data_ptr(), opts)
-                apply_steady(Z[j], w)
-                h = torch.mv(V[: j + 1], w)  # classical Gram-Schmidt, twice
-                w -= torch.mv(V[: j + 1].t(), h)
-                h2 = torch.mv(V[: j + 1], w)
-                w -= torch.mv(V[: j + 1].t(), h2)
-                hc = (h + h2).cpu().numpy()
-                hn = float(torch.linalg.norm(w))
-                H[: j + 1, j] = hc
-                H[j + 1, j] = hn
-                if hn > 0.0:
-                    V[j + 1] = w / hn
-                for i in range(j):  # the earlier rotations on the new column
-                    t = cs[i] * H[i, j] + sn[i] * H[i + 1, j]
-                    H[i + 1, j] = -sn[i] * H[i, j] + cs[i] * H[i + 1, j]
-                    H[i, j] = t
-                den = np.hypot(H[j, j], H[j + 1, j])
-                cs[j], sn[j] = (1.0, 0.0) if den == 0.0 else (H[j, j] / den, H[j + 1, j] / den)
-                H[j, j] = den
-                H[j + 1, j] = 0.0
-                g[j + 1] = -sn[j] * g[j]
-                g[j] = cs[j] * g[j]
-                outer += 1
-                j_end = j + 1
-                if log and outer % log_every == 0:
-                    log(f"adjoint fgmres {outer}: estimate {abs(g[j + 1]) / bn:.3e}, inner {inner}")
-                if abs(g[j + 1]) <= tol * bn or hn == 0.0 or outer >= max_outer:
-                    break
-            y = np.linalg.solve(np.triu(H[:j_end, :j_end]), g[:j_end]) if j_end else np.zeros(0)
-            psi += torch.mv(Z[:j_end].t(), torch.from_numpy(y).to(dev))
-            apply_steady(psi, w)
-            r = rhs_d - w
-            rn = float(torch.linalg.norm(r))
-        psi6 = torch.zeros((N, 6), dtype=torch.float64, device=dev)
-        solver.dev_unpack(psi.data_ptr(), psi6.data_ptr())
-        out = psi6.cpu().numpy()
+        def apply_M(v, z):
+            solver.dev_local_timestep(Ud.data_ptr(), cfl)
+            return solver.dev_gmres(Ud.data_ptr(), zero.data_ptr(), v.data_ptr(), z.data_ptr(), opts)
+
+        V = torch.zeros((m_outer + 1, n), dtype=torch.float64, device="cuda")
+        Z = torch.zeros((m_outer, n), dtype=torch.float64, device="cuda")
+        w = torch.zeros(n, dtype=torch.float64, device="cuda")
+        psi = torch.zeros_like(Ud)
+        inner, outer, rel = _fgmres(apply_A, apply_M, rhs_d, psi, tol, V, Z, w, max_outer, log, log_every,
+                                    "adjoint fgmres")
+        out = _to_host(solver, psi)
     finally:
         solver.set_stream(None)
-    return out, inner, outer, rn / bn
+    return out, inner, outer, rel
 
 
 def gradient_from_adjoint(mesh: cfd.Mesh, solver: cfd.Solver, p: cfd.Params, U, modes, psi, step=1e-7):
@@ -284,17 +316,151 @@ def design_gradient(mesh: cfd.Mesh, solver: cfd.Solver, p: cfd.Params, U, modes,
     return gradient_from_adjoint(mesh, solver, p, U, modes, psi, step), psi, it
 
 
+def newton_fgmres(solver: cfd.Solver, U, vm_target, jv=1, precision=32, precond=2, cfl=1e6, eta=1e-3,
+                  m_outer=100, max_outer=600, max_newton=12, log=None):
+    """Newton steps on R(U) = 0 from U (N x 6) to a volume-weighted residual
+    norm of vm_target.  Each step solves dR/dU dU = -R by FGMRES(m_outer) in
+    FP64 (the outer product: the tangent (jv 1) or finite-difference (jv 0)
+    Jacobian-vector product with the pseudo-time term switched off), each
+    outer step preconditioned by one GMRES(60) solve of the shifted system at
+    CFL `cfl` (`precond`, `precision` basis), to a relative tolerance of
+    max(eta, 0.3 vm_target / vm), so that the last step is solved no tighter
+    than the target needs; then a backtracking search over the step lengths
+    1, 1/2, ..., 1/16 keeps the first that lowers the norm.
+
+    Near a converged state the Newton steps shed the pseudo-transient
+    iteration's many capped GMRES solves: on the configuration-4 bump at full
+    size, from a 1e-4 drop, three steps of about 5,000 inner iterations each
+    reached a 1e-12 drop in 9 s, where the pseudo-time iteration took 324
+    more steps of up to 180 GMRES iterations (33 s).  Returns (U, Newton steps, inner GMRES
+    iterations, the norms after each step, ok); ok is False when a step
+    found no length that lowers the norm (U is then the last accepted state)
+    or lowered it by less than half."""
+    import torch
+
+    nv, n = solver.params().nvar, solver.params().nvar * solver.pitch
+    solver.set_stream(torch.cuda.current_stream().cuda_stream)
+    try:
+        Ud = _to_device(solver, U)
+        Rd = torch.zeros_like(Ud)
+        out = torch.zeros(1, dtype=torch.float64, device="cuda")
+        opts = solver.gmres_opts(m=60, tol=1e-2, jv=jv, precision=precision, precond=precond, restarts=0)
+
+        def residual():
+            solver.dev_residual(Ud.data_ptr(), Rd.data_ptr(), True)
+            solver.dev_norm_vw_noalloc(Rd.data_ptr(), out.data_ptr())
+            return float(out.item())
+
+        def apply_A(x, y):
+            solver.dev_local_timestep(Ud.data_ptr(), 1e30)  # the time term vanishes
+            if jv == 0:
+                solver.dev_jv_fd(Ud.data_ptr(), Rd.data_ptr(), x.data_ptr(), y.data_ptr())
+            else:
+                solver.dev_jv_tangent(Ud.data_ptr(), x.data_ptr(), y.data_ptr(), 64)
+
+        def apply_M(v, z):
+            solver.dev_local_timestep(Ud.data_ptr(), cfl)
+            return solver.dev_gmres(Ud.data_ptr(), Rd.data_ptr(), v.data_ptr(), z.data_ptr(), opts)
+
+        V = torch.zeros((m_outer + 1, n), dtype=torch.float64, device="cuda")
+        Z = torch.zeros((m_outer, n), dtype=torch.float64, device="cuda")
+        w = torch.zeros(n, dtype=torch.float64, device="cuda")
+        vm = residual()
+        steps, inner, ok, vms = 0, 0, True, []
+        while vm > vm_target and steps < max_newton:
+            tol = min(0.1, max(eta, 0.3 * vm_target / vm))
+            dU = torch.zeros_like(Ud)
+            it, outer, rel = _fgmres(apply_A, apply_M, -Rd, dU, tol, V, Z, w, max_outer)
+            inner += it
+            U_old = Ud.clone()
+            lam, vm_new = 1.0, float("inf")
+            while lam >= 1.0 / 16:
+                torch.add(U_old, dU, alpha=lam, out=Ud)
+                vm_new = residual()
+                if vm_new < vm:
+                    break
+                lam *= 0.5
+            steps += 1
+            if log:
+                log(f"newton {steps}: vm {vm:.3e} -> {vm_new:.3e}, linear {rel:.1e} in {outer} outer / {it} inner, "
+                    f"step length {lam:g}")
+            if not vm_new < vm:
+                Ud.copy_(U_old)
+                residual()
+                ok = False
+                break
+            ok = vm_new <= 0.5 * vm
+            vm = vm_new
+            vms.append(vm)
+            if not ok:
+                break
+        U_out = _to_host(solver, Ud)
+    finally:
+        solver.set_stream(None)
+    out_U = np.array(U, dtype=np.float64, copy=True)
+    out_U[:, :nv] = U_out[:, :nv]
+    return out_U, steps, inner, vms, ok
+
+
 def converge(solver: cfd.Solver, U0, rel_drop=1e-10, max_steps=600, jv=1, precision=32, tolerance=0.0,
-             cfl0=1.0, log_every=0):
+             cfl0=1.0, log_every=0, newton_below=1e-4):
     """Steady state by JFNK: the tangent product (FP32 matvec, preconditioner
     and basis by default), block SGS, GMRES(60) with two restarts at a 1e-2
-    tolerance and the adaptive CFL controller.
+    tolerance and the adaptive CFL controller, then (newton_below > 0)
+    Newton steps solved by FGMRES (newton_fgmres) from a drop of
+    newton_below on; newton_below = 1 goes straight to them (a re-solve from
+    a nearby converged state).  Should a Newton step fail to halve the norm,
+    the pseudo-time iteration resumes from the last accepted state.
 
     Stops at a drop of `rel_drop` of the volume-weighted residual norm or below
     the absolute `tolerance` (a re-solve from a nearby converged state needs
     the absolute test: its own initial residual is already small).
-    log_every > 1 prints the Newton progress to stderr every that many steps."""
+    log_every > 1 prints the progress to stderr every that many steps.
+    Returns (U, result, history) as Solver.solve_implicit does, with the
+    Newton steps counted in result.steps, result.total_gmres_iters (their
+    inner iterations) and result.seconds, and appended to history['vm']."""
+    import sys
+    import time
+
     g = solver.gmres_opts(m=60, tol=1e-2, jv=jv, precision=precision, precond=2, restarts=2)
-    o = solver.newton_opts(gmres=g, tolerance=tolerance, rel_drop=rel_drop, max_steps=max_steps, controller=2,
-                           cfl0=cfl0, cfl_max=1e12, log=log_every)
-    return solver.solve_implicit(U0, o)
+
+    def pseudo_time(U, drop, steps, cfl, tol):
+        o = solver.newton_opts(gmres=g, tolerance=tol, rel_drop=drop, max_steps=steps, controller=2,
+                               cfl0=cfl, cfl_max=1e12, log=log_every)
+        return solver.solve_implicit(U, o)
+
+    if newton_below <= 0.0:
+        return pseudo_time(U0, rel_drop, max_steps, cfl0, tolerance)
+    U, res, hist = pseudo_time(U0, max(rel_drop, newton_below), max_steps, cfl0, tolerance)
+    vm0 = float(hist["vm"][0])
+    target = max(rel_drop * vm0, tolerance)
+    if res.res_vm <= target or res.steps >= max_steps:
+        return U, res, hist
+    log = (lambda m: print("newton-fgmres: " + m, file=sys.stderr, flush=True)) if log_every > 1 else None
+    t0 = time.perf_counter()
+    U, k, inner, vms, ok = newton_fgmres(solver, U, target, jv=jv, precision=precision, log=log)
+    res.seconds += time.perf_counter() - t0
+    res.steps += k
+    res.total_gmres_iters += inner
+    if vms:
+        res.res_vm = vms[-1]
+    vm = res.res_vm
+    hist = dict(hist)
+    hist["vm"] = np.concatenate([hist["vm"], vms])
+    if vm <= target:
+        res.converged = 1
+        return U, res, hist
+    if ok or res.steps >= max_steps:
+        res.converged = 0
+        return U, res, hist
+    cfl = float(hist["cfl"][-1]) if len(hist["cfl"]) else cfl0
+    U, r2, h2 = pseudo_time(U, 0.0, max_steps - res.steps, cfl, target)  # from the last accepted Newton state
+    r2.steps += res.steps
+    r2.total_gmres_iters += res.total_gmres_iters
+    r2.seconds += res.seconds
+    r2.res_vm0 = res.res_vm0
+    conv = r2.res_vm <= target
+    r2.converged = int(conv)
+    h2 = dict(h2)
+    h2["vm"] = np.concatenate([hist["vm"], h2["vm"]])
+    return U, r2, h2
