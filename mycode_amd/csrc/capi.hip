@@ -119,6 +119,7 @@ void free_krylov(cfd_ctx* c) {
   (void)hipFree(k.bd);
   (void)hipFree(k.lineG);
   (void)hipFree(k.lineP);
+  (void)hipFree(k.lineA);
   (void)hipFree(k.lineS);
   (void)hipFree(k.lineZ);
   (void)hipFree(k.lineW);
@@ -184,6 +185,7 @@ int ensure_line(cfd_ctx* c) {
   HIP_TRY(c, hipMalloc((void**)&k.bd, nb * 8));
   HIP_TRY(c, hipMalloc(&k.lineG, nv * nv * slots * es));
   HIP_TRY(c, hipMalloc(&k.lineP, nv * nv * slots * es));
+  HIP_TRY(c, hipMalloc(&k.lineA, 4 * nb * es));
   HIP_TRY(c, hipMalloc(&k.lineS, nv * c->pitch * 8));
   HIP_TRY(c, hipMalloc(&k.lineZ, nv * c->pitch * 8));
   HIP_TRY(c, hipMalloc((void**)&k.lineW, nv * slots * 8));
@@ -621,28 +623,36 @@ int cfd_ctx_create(const cfd_mesh* m, const cfd_params* p, cfd_ctx** out) {
       for (int col = 0; col < ncol; ++col)
         for (int k = m->segcol_ptr[col]; k < m->segcol_ptr[col + 1]; ++k)
           for (int t = m->seg_ptr[k]; t < m->seg_ptr[k + 1]; ++t) lcol[m->seg_cells[t]] = col;
-      // both indexed by line position t, with the cells' neighbour table
-      // (neighbours given by their line positions)
+      // all indexed by line position t.  The off-segment neighbours go to
+      // packed slots: those of lower colors to slots 0, 1, ... and those of
+      // higher colors to slots 3, 2, ..., each group in ascending cell-slot
+      // order (the order the sweep subtracts them in).  A sweep pass then
+      // reads the same packed slot in every lane of a wavefront, where the
+      // cell slots of the couplings it needs differ from cell to cell.
       std::vector<unsigned char> lmask(P_, 0);
-      std::vector<int4> lnq(P_, make_int4(-1, -1, -1, -1));
+      std::vector<int4> lnc(P_, make_int4(-1, -1, -1, -1));
+      std::vector<char4> lqm(P_, make_char4(-1, -1, -1, -1));
       for (int t = 0; t < N; ++t) {
         const int i = m->seg_cells[t];
-        for (int q = 0; q < 4; ++q) {  // neighbour entries by line position: (t_o << 2) | back slot
-          const int en = (&nq[i].x)[q];
-          (&lnq[t].x)[q] = en < 0 ? en : (lpos[en >> 2] << 2) | (en & 3);
-        }
+        int lo = 0, hi = 3;
         for (int q = 0; q < 4; ++q) {
           const int en = (&nq[i].x)[q];
           if (en < 0 || q == m->lslot_prev[i] || q == m->lslot_next[i]) continue;
           const int o = en >> 2;
           if (lcol[o] == lcol[i]) return fail(c, CFD_ERR_MESH, "neighbouring segments share a color");
-          lmask[t] |= (unsigned char)(lcol[o] < lcol[i] ? 1u << q : 16u << q);
+          const bool lower = lcol[o] < lcol[i];
+          lmask[t] |= (unsigned char)(lower ? 1u << q : 16u << q);
+          const int r = lower ? lo++ : hi--;
+          (&lnc[t].x)[r] = lpos[o];
+          (&lqm[t].x)[r] = (signed char)q;
         }
       }
       HIP_TRY(c, dalloc(&c->d_lmask, P_));
-      HIP_TRY(c, dalloc(&c->d_lnq, P_));
+      HIP_TRY(c, dalloc(&c->d_lnc, P_));
+      HIP_TRY(c, dalloc(&c->d_lqm, P_));
       HIP_TRY(c, hipMemcpy(c->d_lmask, lmask.data(), P_, hipMemcpyHostToDevice));
-      HIP_TRY(c, hipMemcpy(c->d_lnq, lnq.data(), (size_t)P_ * sizeof(int4), hipMemcpyHostToDevice));
+      HIP_TRY(c, hipMemcpy(c->d_lnc, lnc.data(), (size_t)P_ * sizeof(int4), hipMemcpyHostToDevice));
+      HIP_TRY(c, hipMemcpy(c->d_lqm, lqm.data(), (size_t)P_ * sizeof(char4), hipMemcpyHostToDevice));
       const size_t ns = std::max(lb_t.size(), (size_t)1);
       HIP_TRY(c, dalloc(&c->d_lb_t, ns));
       HIP_TRY(c, dalloc(&c->d_lb_kl, ns));
@@ -690,7 +700,7 @@ int cfd_ctx_destroy(cfd_ctx* c) {
   (void)hipStreamSynchronize(c->stream);
   free_krylov(c);
   void* bufs[] = {c->d_vol, c->d_xc,  c->d_yc,  c->d_slots, c->d_own, c->d_nei, c->d_tag,  c->d_nx,
-                  c->d_ny,  c->d_area, c->d_xm, c->d_ym,    c->d_iperm, c->d_facemap, c->d_alt, c->grad, c->d_wd, c->d_dnw, c->d_corder, c->d_ccol, c->d_ccells, c->d_nq, c->d_cpos, c->d_seg_ptr, c->d_seg_cells, c->d_lpos, c->d_lslot, c->d_lb_t, c->d_lb_kl, c->d_lcomp, c->d_lmask, c->d_lnq,
+                  c->d_ny,  c->d_area, c->d_xm, c->d_ym,    c->d_iperm, c->d_facemap, c->d_alt, c->grad, c->d_wd, c->d_dnw, c->d_corder, c->d_ccol, c->d_ccells, c->d_nq, c->d_cpos, c->d_seg_ptr, c->d_seg_cells, c->d_lpos, c->d_lslot, c->d_lb_t, c->d_lb_kl, c->d_lcomp, c->d_lmask, c->d_lnc, c->d_lqm,
                   c->phi,   c->dt,    c->Fm,    c->red,     c->scal, c->gradT, c->adj, c->adjF, c->tq, c->io6};
   for (void* b : bufs) (void)hipFree(b);
   for (int i = 0; i < 6; ++i) (void)hipFree(c->vec[i]);
@@ -961,9 +971,9 @@ int cfd_line_precond(cfd_ctx* c, const double* U6, int precision, const double* 
   if ((rc = ensure_line(c))) return rc;
   Krylov& k = c->kry;
   launch_block_precond(c, c->vec[0], k.blk, precision, k.bd, k.aoff, c->d_lpos);
-  launch_line_factor(c, k.bd, k.aoff, k.blk, k.lineP, k.lineG, precision);
+  launch_line_factor(c, k.bd, k.aoff, k.blk, k.lineP, k.lineG, k.lineA, precision);
   if ((rc = upload_state(c, x6, c->vec[1]))) return rc;
-  launch_line_apply(c, k.blk, k.lineP, k.lineG, k.aoff, c->vec[1], 64, c->vec[2], 64, precision);
+  launch_line_apply(c, k.blk, k.lineP, k.lineG, k.lineA, c->vec[1], 64, c->vec[2], 64, precision);
   return download_state(c, c->vec[2], z6);
 }
 
@@ -1064,7 +1074,7 @@ static int gmres_cycle(cfd_ctx* c, const double* U, const double* Rb, const doub
       void* Vj1 = V + (size_t)(j + 1) * vs * es;
       if (bj) launch_block_apply(c, k.blk, Vj, k.z, prec, tr);
       else if (sgs) launch_sgs_apply(c, k.blk, k.aoff, Vj, prec, k.z, prec, prec, tr);
-      else if (line) launch_line_apply(c, k.blk, k.lineP, k.lineG, k.aoff, Vj, prec, k.z, prec, prec);
+      else if (line) launch_line_apply(c, k.blk, k.lineP, k.lineG, k.lineA, Vj, prec, k.z, prec, prec);
       else launch_precond_apply(c, k.z, Vj, prec, o->jv == 0 ? S_VNORM : -1);
       if (o->jv == 0) {
         if (bj || sgs || line) launch_reduce(c, RED_L2, (const double*)k.z, nullptr, S_VNORM);
@@ -1134,7 +1144,7 @@ static int gmres_cycle(cfd_ctx* c, const double* U, const double* Rb, const doub
     launch_sgs_apply(c, k.blk, k.aoff, k.xt, 64, x, 64, prec, tr);
   } else if (line) {
     launch_solution(c, k.xt, V, k.h + 128, kk, prec, false);
-    launch_line_apply(c, k.blk, k.lineP, k.lineG, k.aoff, k.xt, 64, x, 64, prec);
+    launch_line_apply(c, k.blk, k.lineP, k.lineG, k.lineA, k.xt, 64, x, 64, prec);
   } else {
     launch_solution(c, x, V, k.h + 128, kk, prec);
   }
@@ -1169,7 +1179,7 @@ static int gmres_dev(cfd_ctx* c, const double* U, const double* Rb, const double
   } else if (o->precond == 3) {
     if ((rc = ensure_line(c))) return rc;
     launch_block_precond(c, U, k.blk, prec, k.bd, k.aoff, c->d_lpos);
-    launch_line_factor(c, k.bd, k.aoff, k.blk, k.lineP, k.lineG, prec);
+    launch_line_factor(c, k.bd, k.aoff, k.blk, k.lineP, k.lineG, k.lineA, prec);
   } else {
     launch_diag_precond(c, U, k.D);
   }

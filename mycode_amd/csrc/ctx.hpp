@@ -39,6 +39,9 @@ struct Krylov {
   double* bd = nullptr;        // FP64 diagonal blocks in segment order (nv^2 x pitch), line preconditioner
   void* lineG = nullptr;       // line factors G_t = Dp_t^-1 A_{t,next} (nv^2 per bundle slot, basis precision)
   void* lineP = nullptr;       // line factors P_t = Dp_t^-1 A_{t,prev} (nv^2 per bundle slot, basis precision)
+  void* lineA = nullptr;       // off-segment coupling blocks by line position in packed slots: those of lower
+                               // segment colors from slot 0 up, of higher colors from slot 3 down (4 nv^2 x pitch,
+                               // block precision)
   void* lineS = nullptr;       // forward-pass partial sums of the line sweep by line position (nv x pitch, up to FP64)
   void* lineZ = nullptr;       // the line sweep's z by line position (nv x pitch, up to FP64)
   double* lineW = nullptr;     // w of the cells of multi-cell segments (nv per bundle slot, FP64)
@@ -98,7 +101,9 @@ struct cfd_ctx {
   int* d_lcomp = nullptr;           // per line position: bundle slot (-1: single-cell segment)
   unsigned char* d_lmask = nullptr; // per line position: off-segment neighbour slots of lower (bits 0-3) / higher
                                     // (4-7) color
-  int4* d_lnq = nullptr;            // per line position: the cell's d_nq entry
+  int4* d_lnc = nullptr;            // per line position: the line positions of the off-segment neighbours in the
+                                    // packed slot order of Krylov::lineA (-1: none)
+  char4* d_lqm = nullptr;           // per line position: the cell slot each packed slot takes its block from
   int line_fused_max = 262144;      // below this many cells a line sweep color is one launch
   int4* d_nq = nullptr;             // per cell slot: (neighbor << 2) | back slot, or -1
   // solver state
@@ -198,10 +203,12 @@ void launch_solution(cfd_ctx* c, double* x, const void* V, const double* y, int 
 // SGS passes d_cpos (color-major), the line preconditioner d_lpos (segment order)
 void launch_block_precond(cfd_ctx* c, const double* U, void* Binv, int prec, double* Bout = nullptr,
                           void* Aoff = nullptr, const int* pos = nullptr);
-// line preconditioner: factorisation of the segments (Bd: diagonal blocks in FP64, segment order)
-// and one symmetric line Gauss-Seidel application z = M^-1 x (same precision rules as the block SGS)
-void launch_line_factor(cfd_ctx* c, const double* Bd, const void* Aoff, void* Dpinv, void* Pm, void* G, int prec);
-void launch_line_apply(cfd_ctx* c, const void* Dpinv, const void* Pm, const void* G, const void* Aoff, const void* x,
+// line preconditioner: factorisation of the segments (Bd: diagonal blocks in FP64, segment order;
+// Aoff: the coupling blocks by cell slot; Ac receives the off-segment ones in packed slots) and one
+// symmetric line Gauss-Seidel application z = M^-1 x (same precision rules as the block SGS)
+void launch_line_factor(cfd_ctx* c, const double* Bd, const void* Aoff, void* Dpinv, void* Pm, void* G, void* Ac,
+                        int prec);
+void launch_line_apply(cfd_ctx* c, const void* Dpinv, const void* Pm, const void* G, const void* Ac, const void* x,
                        int xprec, void* z, int zprec, int bprec);
 // multicolor block SGS: z = M^-1 x (M^-T with transpose); blocks of precision bprec, x/z of xprec/zprec
 // (bprec 64 needs FP64 x and z; bprec 32 takes FP32 x and z or FP64 x and z)

@@ -1853,19 +1853,44 @@ __device__ __forceinline__ void line_recur(int steps, int k, int L, bool live, c
 // and higher (bits 4-7) segment color.
 enum LinePass { LINE_FWD = 0, LINE_TOP = 1, LINE_BWD = 2 };
 
-// w_t = Dp_t^-1 s for the cell c at line position t (pass as above); keep
+// w_t = Dp_t^-1 s for the cell at line position t (pass as above); keep
 // false suppresses the store of s (a padding lane standing in for a cell).
 // x is indexed by cell; the sweep's own vectors, the kept sums sL and the
-// result z, are indexed by line position (so are the neighbour entries of lnq),
-// which keeps their loads and stores coalesced along the segments.
-// The neighbour table lnq and the masks lmask are stored by line position, and
-// FP32 blocks load Dp^-1 first, so that everything but x, s and the
-// neighbours' z is in flight after one dependent load (FP64 blocks load it
-// last: held from the start they cost more occupancy than the early load gains).
+// result z, are indexed by line position (so are the neighbour positions of
+// lnc), which keeps their loads and stores coalesced along the segments.
+// (Gathering x into line order first, in a kernel of its own, cost more than
+// it saved: 42 / 62 us per application in FP32 / FP64 on the 1.32M-cell
+// plate, against 60 / 102 us per color launch.)  The off-segment
+// couplings come from the packed slots of Ac: the forward pass takes slots
+// 0, 1, ... (lower colors), the backward pass slots 3, 2, ... (higher), so
+// that every lane of a wavefront reads the same slot, in the cells'
+// ascending slot order as before.  FP32 blocks load Dp^-1 first, so that
+// everything but s and the neighbours' z is in flight after one dependent
+// load (FP64 blocks load it last: held from the start they cost more
+// occupancy than the early load gains).
+// s -= the n couplings of packed slots r0, r0 + dr, ... (dr = +-1, uniform)
+template <int NV, class TB, class TZ>
+__device__ __forceinline__ void line_couple(int t, int n, int r0, int dr, int4 e4, int p, const TB* __restrict__ Ac,
+                                            const TZ* __restrict__ z, double (&s)[NV]) {
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    if (k >= n) continue;
+    const int r = r0 + dr * k;
+    const int o = r == 0 ? e4.x : r == 1 ? e4.y : r == 2 ? e4.z : e4.w;
+    const TB* __restrict__ A = Ac + (size_t)r * NV * NV * p + t;
+    double zo[NV];
+#pragma unroll
+    for (int j = 0; j < NV; ++j) zo[j] = (double)z[j * p + o];
+#pragma unroll
+    for (int i = 0; i < NV; ++i)
+#pragma unroll
+      for (int j = 0; j < NV; ++j) s[i] -= (double)A[(size_t)(i * NV + j) * p] * zo[j];
+  }
+}
 template <int NV, class TB, class TX, class TZ>
-__device__ __forceinline__ void line_w(int t, int c, int pass, bool keep, const int4* __restrict__ lnq,
+__device__ __forceinline__ void line_w(int t, int c, int pass, bool keep, const int4* __restrict__ lnc,
                                        const unsigned char* __restrict__ lmask, int p,
-                                       const TB* __restrict__ Dpinv, const TB* __restrict__ Aoff,
+                                       const TB* __restrict__ Dpinv, const TB* __restrict__ Ac,
                                        const TX* __restrict__ x, TZ* __restrict__ sL, const TZ* __restrict__ z,
                                        double (&w)[NV]) {
   constexpr bool early = sizeof(TB) == 4;
@@ -1875,7 +1900,6 @@ __device__ __forceinline__ void line_w(int t, int c, int pass, bool keep, const 
     for (int e = 0; e < NV * NV; ++e) dp[e] = Dpinv[(size_t)e * p + t];
   }
   const unsigned mk = lmask[t];
-  const unsigned use = pass == LINE_BWD ? mk >> 4 : mk & 15u;
   double s[NV];
   if (pass == LINE_BWD) {
 #pragma unroll
@@ -1884,20 +1908,9 @@ __device__ __forceinline__ void line_w(int t, int c, int pass, bool keep, const 
 #pragma unroll
     for (int i = 0; i < NV; ++i) s[i] = (double)x[i * p + c];
   }
-  const int4 e4 = lnq[t];
-  const int es[4] = {e4.x, e4.y, e4.z, e4.w};
-#pragma unroll
-  for (int q = 0; q < 4; ++q) {
-    if (!((use >> q) & 1u)) continue;
-    const int o = es[q] >> 2;
-    double zo[NV];
-#pragma unroll
-    for (int j = 0; j < NV; ++j) zo[j] = (double)z[j * p + o];
-#pragma unroll
-    for (int i = 0; i < NV; ++i)
-#pragma unroll
-      for (int j = 0; j < NV; ++j) s[i] -= (double)Aoff[(size_t)((q * NV + i) * NV + j) * p + t] * zo[j];
-  }
+  const int4 e4 = lnc[t];
+  const bool bwd = pass == LINE_BWD;
+  line_couple<NV, TB, TZ>(t, __popc(bwd ? mk >> 4 : mk & 15u), bwd ? 3 : 0, bwd ? -1 : 1, e4, p, Ac, z, s);
   if (pass == LINE_FWD && keep) {
 #pragma unroll
     for (int i = 0; i < NV; ++i) sL[i * p + t] = (TZ)s[i];
@@ -1911,15 +1924,35 @@ __device__ __forceinline__ void line_w(int t, int c, int pass, bool keep, const 
   }
 }
 
+// The off-segment coupling blocks of every line position into its packed
+// slots (lqm: the cell slot each packed slot takes, -1 for none), once per
+// factorisation.
+template <int NV, class TB>
+__global__ __launch_bounds__(BS) void k_line_pack(int N, int p, const char4* __restrict__ lqm,
+                                                  const TB* __restrict__ Aoff, TB* __restrict__ Ac) {
+  constexpr int NN = NV * NV;
+  const int t = xcd_block() * BS + threadIdx.x;
+  if (t >= N) return;
+  const char4 q4 = lqm[t];
+  const int qs[4] = {q4.x, q4.y, q4.z, q4.w};
+#pragma unroll
+  for (int r = 0; r < 4; ++r) {
+    const int q = qs[r];
+    if (q < 0) continue;
+#pragma unroll
+    for (int e = 0; e < NN; ++e) Ac[(size_t)(r * NN + e) * p + t] = Aoff[(size_t)(q * NN + e) * p + t];
+  }
+}
+
 // All cells of one color (positions [t0, t1)) for large meshes: z = w for a
 // single-cell segment, w into its bundle slot (FP64) for a cell of a longer
 // one, whose recurrence k_line_solve then runs.  This bandwidth-bound part runs
 // at the occupancy its own registers allow.
 template <int NV, class TB, class TX, class TZ>
 __global__ __launch_bounds__(LINE_BS) void k_line_cells(int t0, int t1, int pass, const int* __restrict__ seg_cells,
-                                                   const int4* __restrict__ nq, const int* __restrict__ lcomp,
+                                                   const int4* __restrict__ lnc, const int* __restrict__ lcomp,
                                                    const unsigned char* __restrict__ lmask, int p,
-                                                   const TB* __restrict__ Dpinv, const TB* __restrict__ Aoff,
+                                                   const TB* __restrict__ Dpinv, const TB* __restrict__ Ac,
                                                    const TX* __restrict__ x, TZ* __restrict__ sL, TZ* __restrict__ z,
                                                    double* __restrict__ W) {
   const int t = t0 + xcd_block() * LINE_BS + threadIdx.x;
@@ -1927,7 +1960,7 @@ __global__ __launch_bounds__(LINE_BS) void k_line_cells(int t0, int t1, int pass
   const int c = seg_cells[t];
   const int m = lcomp[t];
   double w[NV];
-  line_w<NV, TB, TX, TZ>(t, c, pass, true, nq, lmask, p, Dpinv, Aoff, x, sL, z, w);
+  line_w<NV, TB, TX, TZ>(t, c, pass, true, lnc, lmask, p, Dpinv, Ac, x, sL, z, w);
   if (m >= 0) {
 #pragma unroll
     for (int i = 0; i < NV; ++i) W[(size_t)m * NV + i] = w[i];
@@ -1981,9 +2014,9 @@ __global__ __launch_bounds__(64) void k_line_solve(int b0, const int* __restrict
 template <int NV, class TB, class TX, class TZ>
 __global__ __launch_bounds__(64) void k_line_sweep(int b0, int nb, int t0, int t1, int pass,
                                                    const int* __restrict__ lb_t, const int* __restrict__ lb_kl,
-                                                   const int* __restrict__ seg_cells, const int4* __restrict__ nq,
+                                                   const int* __restrict__ seg_cells, const int4* __restrict__ lnc,
                                                    const unsigned char* __restrict__ lmask, int p,
-                                                   const TB* __restrict__ Dpinv, const TB* __restrict__ Aoff,
+                                                   const TB* __restrict__ Dpinv, const TB* __restrict__ Ac,
                                                    const TB* __restrict__ Pm, const TB* __restrict__ G,
                                                    const TX* __restrict__ x, TZ* __restrict__ sL,
                                                    TZ* __restrict__ z) {
@@ -1995,7 +2028,7 @@ __global__ __launch_bounds__(64) void k_line_sweep(int b0, int nb, int t0, int t
     if (t >= t1) return;
     const int c = seg_cells[t];
     double w[NV];
-    line_w<NV, TB, TX, TZ>(t, c, pass, true, nq, lmask, p, Dpinv, Aoff, x, sL, z, w);
+    line_w<NV, TB, TX, TZ>(t, c, pass, true, lnc, lmask, p, Dpinv, Ac, x, sL, z, w);
 #pragma unroll
     for (int i = 0; i < NV; ++i) z[i * p + t] = (TZ)w[i];
     return;
@@ -2017,7 +2050,7 @@ __global__ __launch_bounds__(64) void k_line_sweep(int b0, int nb, int t0, int t
     gr[e] = G[m * NN + e];
   }
   double w[NV];
-  line_w<NV, TB, TX, TZ>(tc, c, pass, t >= 0, nq, lmask, p, Dpinv, Aoff, x, sL, z, w);
+  line_w<NV, TB, TX, TZ>(tc, c, pass, t >= 0, lnc, lmask, p, Dpinv, Ac, x, sL, z, w);
   TC y[NV];
 #pragma unroll
   for (int i = 0; i < NV; ++i) y[i] = (TC)w[i];
@@ -2916,18 +2949,21 @@ void launch_block_precond(cfd_ctx* c, const double* U, void* Binv, int prec, dou
   else block_t<double>(c, U, (double*)Binv, Bout, (double*)Aoff, pos);
 }
 template <class TV>
-static void line_factor_t(cfd_ctx* c, const double* Bd, const TV* Aoff, TV* Dpinv, TV* Pm, TV* G) {
+static void line_factor_t(cfd_ctx* c, const double* Bd, const TV* Aoff, TV* Dpinv, TV* Pm, TV* G, TV* Ac) {
+  NV_DISPATCH(c->P.nvar, hipLaunchKernelGGL((k_line_pack<NVt, TV>), dim3(nblocks(c->N)), dim3(BS), 0, c->stream,
+                                            c->N, c->pitch, c->d_lqm, Aoff, Ac));
   if (c->nlb == 0) return;  // single-cell segments only: Dp^-1 = B^-1 from the block kernel
   NV_DISPATCH(c->P.nvar, hipLaunchKernelGGL((k_line_factor<NVt, TV>), dim3(c->nlb), dim3(64), 0, c->stream,
                                             c->d_lb_t, c->d_lb_kl, c->d_lslot, c->pitch, c->d_seg_cells, Bd, Aoff,
                                             Dpinv, Pm, G));
 }
-void launch_line_factor(cfd_ctx* c, const double* Bd, const void* Aoff, void* Dpinv, void* Pm, void* G, int prec) {
-  if (prec == 32) line_factor_t<float>(c, Bd, (const float*)Aoff, (float*)Dpinv, (float*)Pm, (float*)G);
-  else line_factor_t<double>(c, Bd, (const double*)Aoff, (double*)Dpinv, (double*)Pm, (double*)G);
+void launch_line_factor(cfd_ctx* c, const double* Bd, const void* Aoff, void* Dpinv, void* Pm, void* G, void* Ac,
+                        int prec) {
+  if (prec == 32) line_factor_t<float>(c, Bd, (const float*)Aoff, (float*)Dpinv, (float*)Pm, (float*)G, (float*)Ac);
+  else line_factor_t<double>(c, Bd, (const double*)Aoff, (double*)Dpinv, (double*)Pm, (double*)G, (double*)Ac);
 }
 template <class TB, class TX, class TZ>
-static void line_t(cfd_ctx* c, const TB* Dpinv, const TB* Pm, const TB* G, const TB* Aoff, const TX* x, TZ* zc) {
+static void line_t(cfd_ctx* c, const TB* Dpinv, const TB* Pm, const TB* G, const TB* Ac, const TX* x, TZ* zc) {
   const int nc = (int)c->segcol_ptr.size() - 1;
   TZ* sL = (TZ*)c->kry.lineS;
   TZ* z = (TZ*)c->kry.lineZ;  // the sweeps' z by line position; zc (by cell) is formed at the end
@@ -2950,12 +2986,12 @@ static void line_t(cfd_ctx* c, const TB* Dpinv, const TB* Pm, const TB* G, const
         const int nsb = (t1 - ts + 63) / 64;
         if (nb + nsb > 0)
           hipLaunchKernelGGL((k_line_sweep<NVt, TB, TX, TZ>), dim3(nb + nsb), dim3(64), 0, c->stream, b0, nb, ts, t1,
-                             pass, c->d_lb_t, c->d_lb_kl, c->d_seg_cells, c->d_lnq, c->d_lmask, c->pitch, Dpinv,
-                             Aoff, Pm, G, x, sL, z);
+                             pass, c->d_lb_t, c->d_lb_kl, c->d_seg_cells, c->d_lnc, c->d_lmask, c->pitch, Dpinv,
+                             Ac, Pm, G, x, sL, z);
       } else {
         hipLaunchKernelGGL((k_line_cells<NVt, TB, TX, TZ>), dim3(nblocks(t1 - t0, LINE_BS)), dim3(LINE_BS), 0,
-                           c->stream, t0, t1, pass, c->d_seg_cells, c->d_lnq, c->d_lcomp, c->d_lmask, c->pitch, Dpinv,
-                           Aoff, x, sL, z, c->kry.lineW);
+                           c->stream, t0, t1, pass, c->d_seg_cells, c->d_lnc, c->d_lcomp, c->d_lmask, c->pitch, Dpinv,
+                           Ac, x, sL, z, c->kry.lineW);
         if (nb > 0)
           hipLaunchKernelGGL((k_line_solve<NVt, TB, TZ>), dim3(nb), dim3(64), 0, c->stream, b0, c->d_lb_t,
                              c->d_lb_kl, c->d_seg_cells, c->pitch, Pm, G, (const double*)c->kry.lineW, z);
@@ -2965,18 +3001,18 @@ static void line_t(cfd_ctx* c, const TB* Dpinv, const TB* Pm, const TB* G, const
                        c->d_lpos, (const TZ*)z, zc);
   });
 }
-void launch_line_apply(cfd_ctx* c, const void* Dpinv, const void* Pm, const void* G, const void* Aoff, const void* x,
+void launch_line_apply(cfd_ctx* c, const void* Dpinv, const void* Pm, const void* G, const void* Ac, const void* x,
                        int xprec, void* z, int zprec, int bprec) {
   if (bprec == 32) {
     if (xprec == 32 && zprec == 32)
-      line_t<float, float, float>(c, (const float*)Dpinv, (const float*)Pm, (const float*)G, (const float*)Aoff,
+      line_t<float, float, float>(c, (const float*)Dpinv, (const float*)Pm, (const float*)G, (const float*)Ac,
                                   (const float*)x, (float*)z);
     else
-      line_t<float, double, double>(c, (const float*)Dpinv, (const float*)Pm, (const float*)G, (const float*)Aoff,
+      line_t<float, double, double>(c, (const float*)Dpinv, (const float*)Pm, (const float*)G, (const float*)Ac,
                                     (const double*)x, (double*)z);
   } else {
     line_t<double, double, double>(c, (const double*)Dpinv, (const double*)Pm, (const double*)G,
-                                   (const double*)Aoff, (const double*)x, (double*)z);
+                                   (const double*)Ac, (const double*)x, (double*)z);
   }
 }
 template <class TB, class TX, class TZ, bool TR>
