@@ -21,7 +21,8 @@ The JSON line carries:
                      region has none); `traffic` is that kernel's HBM bytes
                      per launch from the committed counter summary;
   roofline_residual  the whole residual against B_res over the timed
-                     region's wall time per residual;
+                     region's wall time per residual (both also against the
+                     achievable line, a stream triad measured in the run);
   stage_traffic      each residual kernel against the compulsory bytes of its
                      own inputs and outputs (gradients and face buffer
                      included), the view that locates wasted re-reads;
@@ -46,7 +47,11 @@ The JSON line carries:
   cpu_baseline       the CPU restatement of the reference (g++ -O2, one core)
                      on the same mesh and state: residual and FD Jv times, and
                      the reference algorithm's Newton-step time extrapolated
-                     from them; host core count and CPU model.
+                     from them; host core count and CPU model; the FP64
+                     operation count of one residual (op-counting scalar);
+  roofline_fp64      that count over the timed residual's wall time, and per
+                     stage over the HIP-event times of its kernels, against
+                     the 78.6 TFLOP/s FP64 vector peak (with cpu_baseline).
 """
 from __future__ import annotations
 
@@ -65,6 +70,7 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+FP64_PEAK_TFLOPS = 78.6  # MI355X dense FP64 vector
 CONFIG = 3
 WORKLOADS = {
     2: "laminar flat plate M0.2 Re_L=1e5, 400x160 stretched quads 20% split, Roe + viscous, 2nd-order central",
@@ -127,7 +133,9 @@ def cpu_baseline(mesh, p, U, budget_s=25.0):
     Jv (median of up to 2, Solver::applyJacobian), and from them the time of
     one Newton step of the reference algorithm (Solver.cpp:1696-1799: a local
     time step, a residual and GMRES(40) with one FD Jv per iteration, taken
-    at its 40-iteration cap) and the extrapolated time of `steps` such steps."""
+    at its 40-iteration cap) and the extrapolated time of `steps` such steps.
+    Also the FP64 operation count of one residual on the same state (the
+    residual run on an op-counting scalar), which the FP64 roofline uses."""
     from oracle import oracle
     o = oracle.OracleSolver(oracle.OracleMesh.from_mesh(mesh), p)
     t_start = time.perf_counter()
@@ -145,6 +153,7 @@ def cpu_baseline(mesh, p, U, budget_s=25.0):
         o.apply_jacobian(U, R, v)
         jv_t.append(time.perf_counter() - t0)
     t_res, t_jv = float(np.median(res_t)), float(np.median(jv_t))
+    ops, cell_ops, _ = o.residual_opcount(U)
     ncores, model = host_info()
     step = 2 * t_res + 40 * t_jv
     return {"value": mesh.n_cells / t_res / 1e6, "unit": "Mcell-residuals/s", "cores": 1, "kind": "port",
@@ -155,7 +164,11 @@ def cpu_baseline(mesh, p, U, budget_s=25.0):
             "newton_step_s_estimate": step,
             "newton_step_note": "reference algorithm: local time step + residual + GMRES(40) at its cap, one FD Jv "
                                 "per iteration; estimated from the two measured times",
-            "host_cores": ncores, "host_cpu": model}
+            "host_cores": ncores, "host_cpu": model,
+            "fp64_ops_per_residual": ops, "fp64_ops_cell_stage": cell_ops,
+            "fp64_ops_note": "the oracle's residual run once on an op-counting scalar: + - * / and square root one "
+                             "flop each as written, pow and tanh apart (trans); the cell stage is the gradients, "
+                             "limiter and SST cell terms, the rest the face fluxes and their sums"}
 
 
 LEGS = (("fd_fp64", 0, 64), ("tangent_fp64", 1, 64), ("tangent_fp32", 1, 32))
@@ -428,6 +441,28 @@ def design_gradient_leg(h=1e-6, fd_modes=(5, 20, 31, 47), deadline=None):
     return out
 
 
+def triad_bandwidth(n=64 * 1024 * 1024, reps=20):
+    """The achievable HBM line: a stream triad a = b + s c over three FP64
+    arrays of n values (512 MiB each by default, far beyond the caches; torch's
+    elementwise kernel), 3 x 8 n bytes per pass, timed by events on torch's
+    stream over `reps` passes after three untimed ones.  GB/s."""
+    import torch
+    b = torch.rand(n, dtype=torch.float64, device="cuda")
+    c = torch.rand_like(b)
+    a = torch.empty_like(b)
+    for _ in range(3):
+        torch.add(b, c, alpha=0.5, out=a)
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(reps):
+        torch.add(b, c, alpha=0.5, out=a)
+    e1.record()
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / reps
+    del a, b, c
+    return 3 * 8 * n / (ms * 1e-3) / 1e9
+
+
 def copy_calibration():
     import torch
     n = 32 * 1024 * 1024  # doubles: 256 MiB read and 256 MiB written per copy
@@ -527,6 +562,7 @@ def main():
     B_res = N * (16 * nv + 24) + 48 * F
     achieved = B_res / (t_dom * 1e-3) / 1e9  # compulsory bytes of one residual over the dominant kernel's time
     ach_res = B_res / (wall / args.steps) / 1e9
+    triad = triad_bandwidth()
     wl = WORKLOADS.get(args.config, f"configuration {args.config}") + ", FP64 residual"
     if args.scale != 1.0:
         wl += f" (mesh scaled by {args.scale})"
@@ -549,10 +585,13 @@ def main():
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(dom, hex(mesh.hash())),
                      "kernel": dom, "algorithmic_bytes_per_launch": B_res, "ms_per_launch": t_dom,
+                     "achievable": triad, "frac_of_achievable": achieved / triad,
+                     "achievable_rule": "stream triad a = b + s c, 3 x 512 MiB FP64, torch's elementwise kernel",
                      "bytes_rule": "B_res = N (16 n_v + 24) + 48 F: U read and R written once, cell volume and "
                                    "centroid, face indices and geometry"},
         "roofline_residual": {"bound": "hbm", "achieved": ach_res, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                               "frac": ach_res / HBM_PEAK_GBS, "B_res": B_res,
+                              "frac_of_achievable": ach_res / triad,
                               "ms_wall": wall / args.steps * 1e3, "ms_event": ms[3]},
         "stage_traffic": {k: {"bytes": b, "ms": t, "GB_per_s": b / (t * 1e-3) / 1e9,
                               "frac": b / (t * 1e-3) / 1e9 / HBM_PEAK_GBS}
@@ -586,7 +625,21 @@ def main():
     if not args.no_design and args.config == CONFIG and args.scale == 1.0:
         out["design_gradient"] = design_gradient_leg(deadline=T_START + RUN_BUDGET_S)
     if not args.no_cpu:
-        out["cpu_baseline"] = cpu_baseline(mesh, p, U)
+        out["cpu_baseline"] = cb = cpu_baseline(mesh, p, U)
+        f_all, f_cell = cb["fp64_ops_per_residual"]["flops"], cb["fp64_ops_cell_stage"]["flops"]
+
+        def rate(f, t_ms):
+            return {"flops": f, "ms": t_ms, "achieved": f / (t_ms * 1e-3) / 1e12,
+                    "frac": f / (t_ms * 1e-3) / 1e12 / FP64_PEAK_TFLOPS}
+        out["roofline_fp64"] = dict(
+            bound="fp64", unit="TFLOP/s", peak=FP64_PEAK_TFLOPS, flops_per_residual=f_all,
+            flops_per_cell=f_all / N, **{k: v for k, v in rate(f_all, wall / args.steps * 1e3).items()
+                                         if k in ("achieved", "frac")},
+            stages={"cells (gradients kernel)": rate(f_cell, ms[0]),
+                    "faces (flux and assembly kernels)": rate(f_all - f_cell, ms[1] + ms[2])},
+            count_rule="the reference algorithm's FP64 operations of one residual, counted by the CPU "
+                       "restatement on an op-counting scalar (+ - * / sqrt one flop each); the GPU's "
+                       "executed FP64 instructions are higher (division and square root expand)")
     print(json.dumps(out), flush=True)
 
 

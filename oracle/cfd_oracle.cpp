@@ -55,6 +55,34 @@ inline cplx spow(const cplx& a, double b) {
   return cplx(r, b * std::pow(a.real(), b - 1.0) * a.imag());
 }
 
+// ---- op-counting scalar (measurement; extension) ----------------------------------
+// The FP64 operation count of a residual evaluation: the discretisation run
+// with this type counts + and - (add), * (mul), / (div) and the square root
+// as one flop each, as written (no multiply-add contraction), and pow and
+// tanh apart (trans); negation, comparisons, min and max are not counted.
+// Test and measurement infrastructure like the rest of this file.
+struct OpC {
+  double v;
+  OpC() : v(0.0) {}
+  OpC(double x) : v(x) {}
+};
+struct OpCounts {
+  unsigned long long add = 0, mul = 0, div = 0, sqrt = 0, trans = 0;
+};
+static OpCounts g_opc;
+inline OpC operator+(OpC a, OpC b) { ++g_opc.add; return OpC(a.v + b.v); }
+inline OpC operator-(OpC a, OpC b) { ++g_opc.add; return OpC(a.v - b.v); }
+inline OpC operator*(OpC a, OpC b) { ++g_opc.mul; return OpC(a.v * b.v); }
+inline OpC operator/(OpC a, OpC b) { ++g_opc.div; return OpC(a.v / b.v); }
+inline OpC operator-(OpC a) { return OpC(-a.v); }
+inline OpC& operator+=(OpC& a, OpC b) { return a = a + b; }
+inline OpC& operator-=(OpC& a, OpC b) { return a = a - b; }
+inline OpC& operator*=(OpC& a, OpC b) { return a = a * b; }
+inline OpC& operator/=(OpC& a, OpC b) { return a = a / b; }
+inline double re(const OpC& x) { return x.v; }
+inline OpC ssqrt(const OpC& x) { ++g_opc.sqrt; return OpC(std::sqrt(x.v)); }
+inline OpC spow(const OpC& a, double b) { ++g_opc.trans; return OpC(std::pow(a.v, b)); }
+
 // ---- State (Types.hpp:55-82) -------------------------------------------------
 template <class T> struct St {
   T q[6];
@@ -212,6 +240,7 @@ static void compute_geometry(const Topo& t, const std::vector<T>& X, const std::
 // ---- the discretisation ---------------------------------------------------------
 template <class T> struct Disc;
 inline double ttanh_(double x) { return std::tanh(x); }
+inline OpC ttanh_(const OpC& x) { ++g_opc.trans; return OpC(std::tanh(x.v)); }
 inline cplx ttanh_(const cplx& x) {
   double t = std::tanh(x.real());
   return cplx(t, (1.0 - t * t) * x.imag());
@@ -1227,6 +1256,57 @@ void orc_residual_cs(orc_solver* s, const double* U6, const double* v6, double h
       Rre[6 * c + k] = R[c][k].real();
       dR[6 * c + k] = R[c][k].imag() / h;
     }
+}
+
+// extension (measurement): the FP64 operations of one residual at U6 by the
+// op-counting scalar (the geometry is given, not counted).  counts[0..4]:
+// add, mul, div, sqrt, trans of the whole residual (the limiter formed when
+// update is set); counts[5..9]: the same for its cell stage alone (the
+// Green-Gauss gradients, the limiter and, with SST, the per-cell turbulence
+// terms), the rest being the face fluxes and their sums.  R6 (may be null)
+// receives the residual, which equals orc_residual's bit for bit: the counter
+// changes no operation.
+void orc_residual_opcount(orc_solver* s, const double* U6, int update, unsigned long long* counts, double* R6) {
+  Solver& S = s->s;
+  const int n = S.N(), nf = S.topo.nfaces();
+  Geo<OpC> go;
+  const Geo<double>& gd = S.geo;
+  go.vol.assign(gd.vol.begin(), gd.vol.end());
+  go.xc.assign(gd.xc.begin(), gd.xc.end());
+  go.yc.assign(gd.yc.begin(), gd.yc.end());
+  go.nx.assign(gd.nx.begin(), gd.nx.begin() + nf);
+  go.ny.assign(gd.ny.begin(), gd.ny.begin() + nf);
+  go.area.assign(gd.area.begin(), gd.area.begin() + nf);
+  go.xm.assign(gd.xm.begin(), gd.xm.begin() + nf);
+  go.ym.assign(gd.ym.begin(), gd.ym.begin() + nf);
+  Disc<OpC> D;
+  D.topo = &S.topo;
+  D.geo = &go;
+  D.P = S.P;
+  D.face_alt = S.face_alt.empty() ? nullptr : S.face_alt.data();
+  D.wdist = S.wdist.data();
+  if (!update && !S.D.limiters.empty()) {
+    D.limiters.resize(n);
+    for (int c = 0; c < n; ++c) D.limiters[c] = S.D.limiters[c];
+  }
+  std::vector<St<OpC>> U(n), R(n);
+  for (int c = 0; c < n; ++c)
+    for (int k = 0; k < 6; ++k) U[c][k] = OpC(U6[6 * c + k]);
+  g_opc = OpCounts();
+  D.gradients(U.data());
+  if (S.P.spatial == CFD_SECOND_ORDER_LIMITED && (update || (int)D.limiters.size() != n)) D.limiter(U.data());
+  if (S.P.turb)
+    for (int c = 0; c < n; ++c) (void)D.turb_cell(U[c], D.grads[c], S.wdist[c]);
+  const OpCounts cell = g_opc;
+  g_opc = OpCounts();
+  D.residual(U.data(), R.data(), update != 0);
+  const OpCounts all = g_opc;
+  const unsigned long long out[10] = {all.add, all.mul, all.div, all.sqrt, all.trans,
+                                      cell.add, cell.mul, cell.div, cell.sqrt, cell.trans};
+  std::memcpy(counts, out, sizeof(out));
+  if (R6)
+    for (int c = 0; c < n; ++c)
+      for (int k = 0; k < 6; ++k) R6[6 * c + k] = R[c][k].v;
 }
 
 // per-face contribution F*A (reference face order) of the residual at U

@@ -24,6 +24,9 @@ void orc_set_params(orc_solver* s, const cfd_params* p);
 void orc_set_face_flux(orc_solver* s, const unsigned char* alt);
 void orc_wall_distance(orc_solver* s, double* d);
 void orc_residual(orc_solver* s, const double* U6, double* R6, int update);
+/* extension (measurement): FP64 operation counts of one residual (add, mul,
+ * div, sqrt, trans; then the same for its cell stage) */
+void orc_residual_opcount(orc_solver* s, const double* U6, int update, unsigned long long* counts, double* R6);
 void orc_residual_cs(orc_solver* s, const double* U6, const double* v6, double h, double* Rre, double* dR,
                      int update);
 void orc_face_flux(orc_solver* s, const double* U6, double* F6, int update);

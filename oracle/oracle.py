@@ -69,6 +69,7 @@ def oracle_lib():
             "orc_wall_distance": (None, [_VP, _DP]),
             "orc_residual": (None, [_VP, _DP, _DP, C.c_int]),
             "orc_residual_cs": (None, [_VP, _DP, _DP, C.c_double, _DP, _DP, C.c_int]),
+            "orc_residual_opcount": (None, [_VP, _DP, C.c_int, C.POINTER(C.c_ulonglong), _DP]),
             "orc_face_flux": (None, [_VP, _DP, _DP, C.c_int]),
             "orc_get_gradients": (None, [_VP, _DP]),
             "orc_get_gradients14": (None, [_VP, _DP]),
@@ -239,6 +240,25 @@ class OracleSolver:
         R = np.zeros((self.N, 6))
         self.lib.orc_residual(self.h, _dp(U), _dp(R), int(update_limiters))
         return R
+
+    OPS = ("add", "mul", "div", "sqrt", "trans")
+
+    def residual_opcount(self, U, update_limiters=True):
+        """FP64 operations of one residual at U, counted by running the
+        discretisation on an op-counting scalar: (whole residual, its cell
+        stage: gradients, limiter, SST cell terms), each a dict of add, mul,
+        div, sqrt and trans (pow, tanh) counts plus flops = add + mul + div +
+        sqrt; and the residual itself (bit-identical to compute_residuals)."""
+        U = np.ascontiguousarray(U, np.float64)
+        R = np.zeros((self.N, 6))
+        c = (C.c_ulonglong * 10)()
+        self.lib.orc_residual_opcount(self.h, _dp(U), int(update_limiters), c, _dp(R))
+        out = []
+        for base in (0, 5):
+            d = {k: int(c[base + i]) for i, k in enumerate(self.OPS)}
+            d["flops"] = d["add"] + d["mul"] + d["div"] + d["sqrt"]
+            out.append(d)
+        return out[0], out[1], R
 
     def residual_complex_step(self, U, v, h=1e-30, update_limiters=True):
         U = np.ascontiguousarray(U, np.float64)
