@@ -167,8 +167,11 @@ def _fgmres(apply_A, apply_M, b, x, tol, V, Z, w, max_outer, log=None, log_every
     bn = float(torch.linalg.norm(b))
     if bn == 0.0:
         return 0, 0, 0.0
-    apply_A(x, w)
-    r = b - w
+    if bool(torch.any(x != 0)):
+        apply_A(x, w)
+        r = b - w
+    else:  # no product of a zero vector: the finite-difference product scales its step by 1 / |v|
+        r = b.clone()
     rn = float(torch.linalg.norm(r))
     inner, outer = 0, 0
     while rn > tol * bn and outer < max_outer:
