@@ -311,6 +311,25 @@ int layout_faces(cfd_ctx* c) {
     nei[d] = m->nei[f] >= 0 ? m->perm[m->nei[f]] : -1;
     tag[d] = m->tag[f];
   }
+  // the boundary faces grouped by owner, each group in corder order (ascending color),
+  // so that one thread per group can add them in the order the per-color launches did
+  std::vector<int4> grp;
+  {
+    std::vector<int> gidx(N, -1);
+    for (size_t col = 0; col + 1 < c->color_ptr.size(); ++col)
+      for (int i = c->color_bptr[col]; i < c->color_ptr[col + 1]; ++i) {
+        const int d = corder[i], o = own[d];
+        if (gidx[o] < 0) {
+          gidx[o] = (int)grp.size();
+          grp.push_back(make_int4(-1, -1, -1, -1));
+        }
+        int* g = &grp[gidx[o]].x;
+        int k = 0;
+        while (k < 4 && g[k] >= 0) ++k;
+        if (k == 4) return fail(c, CFD_ERR_MESH, "cell with more than 4 boundary faces");
+        g[k] = d;
+      }
+  }
   // per-cell face slots in ascending reference face index
   std::vector<int4> slots(P_, make_int4(-1, -1, -1, -1));
   std::vector<int> cnt(N, 0);
@@ -326,6 +345,10 @@ int layout_faces(cfd_ctx* c) {
     }
   }
   HIP_TRY(c, hipStreamSynchronize(c->stream));
+  if (!c->d_bgrp) HIP_TRY(c, dalloc(&c->d_bgrp, std::max<size_t>(grp.size(), 1)));
+  c->nbgrp = (int)grp.size();
+  if (!grp.empty())
+    HIP_TRY(c, hipMemcpy(c->d_bgrp, grp.data(), grp.size() * sizeof(int4), hipMemcpyHostToDevice));
   HIP_TRY(c, hipMemcpy(c->d_slots, slots.data(), P_ * sizeof(int4), hipMemcpyHostToDevice));
   HIP_TRY(c, hipMemcpy(c->d_own, own.data(), F * 4, hipMemcpyHostToDevice));
   HIP_TRY(c, hipMemcpy(c->d_nei, nei.data(), F * 4, hipMemcpyHostToDevice));
@@ -700,7 +723,7 @@ int cfd_ctx_destroy(cfd_ctx* c) {
   (void)hipStreamSynchronize(c->stream);
   free_krylov(c);
   void* bufs[] = {c->d_vol, c->d_xc,  c->d_yc,  c->d_slots, c->d_own, c->d_nei, c->d_tag,  c->d_nx,
-                  c->d_ny,  c->d_area, c->d_xm, c->d_ym,    c->d_iperm, c->d_facemap, c->d_alt, c->grad, c->d_wd, c->d_dnw, c->d_corder, c->d_ccol, c->d_ccells, c->d_nq, c->d_cpos, c->d_seg_ptr, c->d_seg_cells, c->d_lpos, c->d_lslot, c->d_lb_t, c->d_lb_kl, c->d_lcomp, c->d_lmask, c->d_lnc, c->d_lqm,
+                  c->d_ny,  c->d_area, c->d_xm, c->d_ym,    c->d_iperm, c->d_facemap, c->d_alt, c->grad, c->d_wd, c->d_dnw, c->d_corder, c->d_bgrp, c->d_ccol, c->d_ccells, c->d_nq, c->d_cpos, c->d_seg_ptr, c->d_seg_cells, c->d_lpos, c->d_lslot, c->d_lb_t, c->d_lb_kl, c->d_lcomp, c->d_lmask, c->d_lnc, c->d_lqm,
                   c->phi,   c->dt,    c->Fm,    c->red,     c->scal, c->gradT, c->adj, c->adjF, c->tq, c->io6};
   for (void* b : bufs) (void)hipFree(b);
   for (int i = 0; i < 6; ++i) (void)hipFree(c->vec[i]);

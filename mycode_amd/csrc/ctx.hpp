@@ -70,6 +70,8 @@ struct cfd_ctx {
   int N = 0, F = 0, pitch = 0, ncolors = 0;
   std::vector<int> color_ptr;
   std::vector<int> color_bptr;  // per color: first boundary face in corder (interior faces lead each color)
+  int4* d_bgrp = nullptr;       // per cell with boundary faces: those faces (device index, corder order; -1 pads)
+  int nbgrp = 0;
   std::vector<int> face_order, perm, iperm;  // host copies
   std::vector<int> ccell_ptr;                 // cell color ranges into d_ccells
   cfd::DMesh dm{};

@@ -601,19 +601,15 @@ constexpr int VK = 4;
 // passes of VK directions that cover a face's inputs (both states, both gradients)
 template <int NV> __host__ __device__ constexpr int vjp_passes() { return (2 * NV + 2 * (NV == 6 ? 14 : 10) + VK - 1) / VK; }
 
+// One pass of VK directions of face f: w^T dF/d(inputs base .. base + VK - 1),
+// added to the inputs' adjoint rows.
 template <int NV, int VISC>
-__global__ __launch_bounds__(FLUX_BS) void k_vjp_face(DMesh m, Phys P, const double* __restrict__ U,
-                                                  const double* __restrict__ G, const double* __restrict__ phi,
-                                                  const double* __restrict__ w, double* __restrict__ aU,
-                                                  double* __restrict__ aG, int f0, int f1) {
+__device__ __forceinline__ void vjp_face_pass(const DMesh& m, const Phys& P, const double* __restrict__ U,
+                                              const double* __restrict__ G, const double* __restrict__ phi,
+                                              const double* __restrict__ w, double* __restrict__ aU,
+                                              double* __restrict__ aG, int f, int pass) {
   typedef DualV<double, VK> T;
   constexpr int NGM = NV == 6 ? 14 : 10;
-  // one thread per (face, pass of VK directions): the passes of a face run on
-  // neighbouring lanes and write disjoint inputs' adjoints
-  const int t = xcd_block() * FLUX_BS + threadIdx.x;
-  const int i = f0 + t / vjp_passes<NV>(), pass = t % vjp_passes<NV>();
-  if (i >= f1) return;
-  const int f = m.corder[i];
   const int p = m.pitch;
   const int o = m.own[f], n = m.nei[f];
   const bool interior = n >= 0;
@@ -682,6 +678,43 @@ __global__ __launch_bounds__(FLUX_BS) void k_vjp_face(DMesh m, Phys P, const dou
       else dst = aG + (size_t)(idx - nU - ng) * p + n;
       *dst = *dst + s;
     }
+  }
+}
+
+// One thread per (face, pass) over the faces corder[f0 .. f1) of one color
+// (no two share a cell): the passes of a face run on neighbouring lanes and
+// write disjoint inputs' adjoints.
+template <int NV, int VISC>
+__global__ __launch_bounds__(FLUX_BS) void k_vjp_face(DMesh m, Phys P, const double* __restrict__ U,
+                                                  const double* __restrict__ G, const double* __restrict__ phi,
+                                                  const double* __restrict__ w, double* __restrict__ aU,
+                                                  double* __restrict__ aG, int f0, int f1) {
+  const int t = xcd_block() * FLUX_BS + threadIdx.x;
+  const int i = f0 + t / vjp_passes<NV>(), pass = t % vjp_passes<NV>();
+  if (i >= f1) return;
+  vjp_face_pass<NV, VISC>(m, P, U, G, phi, w, aU, aG, m.corder[i], pass);
+}
+
+// The boundary faces in one launch: one thread per (cell with boundary faces,
+// pass), which adds its cell's boundary faces one after the other in the
+// order of their colors, as the per-color launches did (the sums are the
+// same to the bit).  Each launch of the general kernel lasts about as long
+// as one thread's dual passes whatever its size (28 us on the 1.32M-cell SST
+// plate), so three color launches cost three times one.
+template <int NV, int VISC>
+__global__ __launch_bounds__(FLUX_BS) void k_vjp_face_bnd(DMesh m, Phys P, const double* __restrict__ U,
+                                                      const double* __restrict__ G, const double* __restrict__ phi,
+                                                      const double* __restrict__ w, double* __restrict__ aU,
+                                                      double* __restrict__ aG, const int4* __restrict__ grp,
+                                                      int ngrp) {
+  const int t = xcd_block() * FLUX_BS + threadIdx.x;
+  const int gi = t / vjp_passes<NV>(), pass = t % vjp_passes<NV>();
+  if (gi >= ngrp) return;
+  const int4 g4 = grp[gi];
+  const int fs[4] = {g4.x, g4.y, g4.z, g4.w};
+  for (int k = 0; k < 4; ++k) {
+    if (fs[k] < 0) break;
+    vjp_face_pass<NV, VISC>(m, P, U, G, phi, w, aU, aG, fs[k], pass);
   }
 }
 
@@ -1132,20 +1165,16 @@ __global__ __launch_bounds__(BS) void k_vjp_gg(DMesh m, Phys P, const double* __
   for (int j = 0; j < NV; ++j) aU[(size_t)j * p + c] = (aU[(size_t)j * p + c] + s[j]) + w[(size_t)j * p + c] * tc;
 }
 
-// The ghost state's part of k_vjp_gg for the boundary faces [f0, f1) of one
-// color (so no two share an owner): aU_o += 1/2 aq_f^T dq(ghost(U_o))/dU_o,
-// aq_f = A aG_o . n / V_o.  One thread per (face, state row j): a 1-direction
-// dual along U_o[j] gives row j, so each thread writes its own row.
+// The ghost state's part of k_vjp_gg for the boundary faces:
+// aU_o += 1/2 aq_f^T dq(ghost(U_o))/dU_o, aq_f = A aG_o . n / V_o.  One thread
+// per (cell with boundary faces, state row j), over its boundary faces in the
+// order of their colors: a 1-direction dual along U_o[j] gives row j, so each
+// thread writes its own row, and one launch serves every boundary face.
 template <int NV>
-__global__ __launch_bounds__(BS) void k_vjp_gg_ghost(DMesh m, Phys P, const double* __restrict__ U,
-                                                      const double* __restrict__ aG, double* __restrict__ aU,
-                                                      int f0, int f1) {
+__device__ __forceinline__ void vjp_ghost_row(const DMesh& m, const Phys& P, const double* __restrict__ U,
+                                              const double* __restrict__ aG, double* __restrict__ aU, int f, int j) {
   typedef DualV<double, 1> T;
   constexpr int NQM = NV == 6 ? 7 : 5;
-  const int t = blockIdx.x * BS + threadIdx.x;
-  const int i = f0 + t / NV, j = t % NV;
-  if (i >= f1) return;
-  const int f = m.corder[i];
   const int c = m.own[f];
   const int p = m.pitch;
   const bool turb = NV == 6 && P.turb;
@@ -1167,6 +1196,20 @@ __global__ __launch_bounds__(BS) void k_vjp_gg_ghost(DMesh m, Phys P, const doub
     ag += aq * 0.5 * qg[k].d[0];
   }
   aU[(size_t)j * p + c] += ag;
+}
+template <int NV>
+__global__ __launch_bounds__(BS) void k_vjp_gg_ghost(DMesh m, Phys P, const double* __restrict__ U,
+                                                      const double* __restrict__ aG, double* __restrict__ aU,
+                                                      const int4* __restrict__ grp, int ngrp) {
+  const int t = blockIdx.x * BS + threadIdx.x;
+  const int gi = t / NV, j = t % NV;
+  if (gi >= ngrp) return;
+  const int4 g4 = grp[gi];
+  const int fs[4] = {g4.x, g4.y, g4.z, g4.w};
+  for (int k = 0; k < 4; ++k) {
+    if (fs[k] < 0) break;
+    vjp_ghost_row<NV>(m, P, U, aG, aU, fs[k], j);
+  }
 }
 
 template <class F>
@@ -2672,12 +2715,19 @@ static void vjp_nv(cfd_ctx* c, const double* U, const double* w, double* JTw, bo
     launch_fill(c, aG, 0.0, (size_t)16 * m.pitch);
   }
   // (with Euler the structured pass also covers the boundary faces)
-  for (int col = 0; col < c->ncolors && !(inv && VISC == 0); ++col) {
-    const int f0 = c->color_ptr[col], f1 = c->color_ptr[col + 1], fb = inv ? c->color_bptr[col] : f0;
-    if (f1 > fb)
-      hipLaunchKernelGGL((k_vjp_face<NV, VISC>), dim3(nblocks((size_t)(f1 - fb) * vjp_passes<NV>(), FLUX_BS)),
-                         dim3(FLUX_BS), 0, c->stream, m, P, U,
-                         (const double*)c->grad, (const double*)c->phi, w, JTw, aG, fb, f1);
+  if (inv) {
+    if (VISC != 0 && c->nbgrp > 0)
+      hipLaunchKernelGGL((k_vjp_face_bnd<NV, VISC>), dim3(nblocks((size_t)c->nbgrp * vjp_passes<NV>(), FLUX_BS)),
+                         dim3(FLUX_BS), 0, c->stream, m, P, U, (const double*)c->grad, (const double*)c->phi, w,
+                         JTw, aG, (const int4*)c->d_bgrp, c->nbgrp);
+  } else {
+    for (int col = 0; col < c->ncolors; ++col) {
+      const int f0 = c->color_ptr[col], f1 = c->color_ptr[col + 1];
+      if (f1 > f0)
+        hipLaunchKernelGGL((k_vjp_face<NV, VISC>), dim3(nblocks((size_t)(f1 - f0) * vjp_passes<NV>(), FLUX_BS)),
+                           dim3(FLUX_BS), 0, c->stream, m, P, U,
+                           (const double*)c->grad, (const double*)c->phi, w, JTw, aG, f0, f1);
+    }
   }
   if constexpr (NV == 6) {  // (the structured path forms the SST cell terms in k_vjp_gather)
     if (P.turb && !inv)
@@ -2686,12 +2736,9 @@ static void vjp_nv(cfd_ctx* c, const double* U, const double* w, double* JTw, bo
   }
   hipLaunchKernelGGL((k_vjp_gg<NV>), dim3(nblocks(m.N)), dim3(BS), 0, c->stream, m, P, U, (const double*)aG, JTw, w,
                      (const double*)c->dt);
-  for (int col = 0; col < c->ncolors; ++col) {  // the ghost states' part, boundary faces color by color
-    const int fb = c->color_bptr[col], f1 = c->color_ptr[col + 1];
-    if (f1 > fb)
-      hipLaunchKernelGGL((k_vjp_gg_ghost<NV>), dim3(nblocks((size_t)(f1 - fb) * NV)), dim3(BS), 0, c->stream, m, P, U,
-                         (const double*)aG, JTw, fb, f1);
-  }
+  if (c->nbgrp > 0)  // the ghost states' part
+    hipLaunchKernelGGL((k_vjp_gg_ghost<NV>), dim3(nblocks((size_t)c->nbgrp * NV)), dim3(BS), 0, c->stream, m, P, U,
+                       (const double*)aG, JTw, (const int4*)c->d_bgrp, c->nbgrp);
 }
 
 void launch_vjp(cfd_ctx* c, const double* U, const double* w, double* JTw, bool grads_ready) {
