@@ -83,3 +83,27 @@ def test_fgmres_adjoint_matches_defect_correction():
         assert r <= 1.5e-10
     assert np.linalg.norm(psi_f - psi_d) <= 1e-8 * np.linalg.norm(psi_d)
     assert inner < its
+
+
+def test_newton_fgmres_tangent_and_fd_products():
+    """The Newton stage of adjoint.converge on the configuration-4 mesh at 0.1
+    scale, from the pseudo-time iteration's state at a 1e-4 drop: with the
+    tangent product (FP32 inner Krylov) and with the finite-difference product
+    (FP64 inner Krylov) the norm falls by at least 10x per step (the first
+    step, from the pseudo-time state, about 60x; the later ones by the linear
+    tolerance, 1e-3), both reach a 1e-11 drop in at most four steps, and the
+    two converged states agree to 1e-9 relative."""
+    mesh, p, U0 = configs.setup(4, 0.1)
+    s = cfd.Solver(mesh, p)
+    U1, res, hist = adjoint.converge(s, U0, rel_drop=1e-4, newton_below=0.0)
+    assert res.converged
+    target = 1e-11 * float(hist["vm"][0])
+    out = {}
+    for jv, prec in ((1, 32), (0, 64)):
+        U, k, inner, vms, ok = adjoint.newton_fgmres(s, U1, target, jv=jv, precision=prec)
+        print(f"jv={jv}: {k} Newton steps, {inner} inner iterations, norms {['%.2e' % v for v in vms]}", flush=True)
+        assert ok and vms[-1] <= target and k <= 4
+        norms = [res.res_vm] + vms
+        assert all(b <= 0.1 * a for a, b in zip(norms[:-1], norms[1:]))
+        out[jv] = U
+    assert np.linalg.norm(out[0] - out[1]) <= 1e-9 * np.linalg.norm(out[1])
