@@ -597,13 +597,20 @@ __global__ __launch_bounds__(BS) void k_tcombine(int N, int p, F* __restrict__ J
 //               (gradients recomputed, never stored per face);
 //   k_vjp_gg    the Green-Gauss transpose: aG_c / V_c back onto the face means
 //               and through the primitive (and ghost) maps onto U.
-constexpr int VK = 4;
+// Directions per pass.  Two directions keep the SST instance nearly free of
+// spills (4: 208 of 512 registers spilled, 2: 6) and cost least overall:
+// 13.8 / 10.1 / 15.1 ms per product with 4 / 2 / 1 on the 1.32M-cell SST
+// plate with k and omega reconstructed (this kernel's whole product), and
+// 1.068 -> 1.040 ms with the boundary faces of the plate's product.
+constexpr int VK_GEN = 2;
 // passes of VK directions that cover a face's inputs (both states, both gradients)
-template <int NV> __host__ __device__ constexpr int vjp_passes() { return (2 * NV + 2 * (NV == 6 ? 14 : 10) + VK - 1) / VK; }
+template <int NV, int VK> __host__ __device__ constexpr int vjp_passes() {
+  return (2 * NV + 2 * (NV == 6 ? 14 : 10) + VK - 1) / VK;
+}
 
 // One pass of VK directions of face f: w^T dF/d(inputs base .. base + VK - 1),
 // added to the inputs' adjoint rows.
-template <int NV, int VISC>
+template <int NV, int VISC, int VK>
 __device__ __forceinline__ void vjp_face_pass(const DMesh& m, const Phys& P, const double* __restrict__ U,
                                               const double* __restrict__ G, const double* __restrict__ phi,
                                               const double* __restrict__ w, double* __restrict__ aU,
@@ -690,9 +697,9 @@ __global__ __launch_bounds__(FLUX_BS) void k_vjp_face(DMesh m, Phys P, const dou
                                                   const double* __restrict__ w, double* __restrict__ aU,
                                                   double* __restrict__ aG, int f0, int f1) {
   const int t = xcd_block() * FLUX_BS + threadIdx.x;
-  const int i = f0 + t / vjp_passes<NV>(), pass = t % vjp_passes<NV>();
+  const int i = f0 + t / vjp_passes<NV, VK_GEN>(), pass = t % vjp_passes<NV, VK_GEN>();
   if (i >= f1) return;
-  vjp_face_pass<NV, VISC>(m, P, U, G, phi, w, aU, aG, m.corder[i], pass);
+  vjp_face_pass<NV, VISC, VK_GEN>(m, P, U, G, phi, w, aU, aG, m.corder[i], pass);
 }
 
 // The boundary faces in one launch: one thread per (cell with boundary faces,
@@ -708,13 +715,13 @@ __global__ __launch_bounds__(FLUX_BS) void k_vjp_face_bnd(DMesh m, Phys P, const
                                                       double* __restrict__ aG, const int4* __restrict__ grp,
                                                       int ngrp) {
   const int t = xcd_block() * FLUX_BS + threadIdx.x;
-  const int gi = t / vjp_passes<NV>(), pass = t % vjp_passes<NV>();
+  const int gi = t / vjp_passes<NV, VK_GEN>(), pass = t % vjp_passes<NV, VK_GEN>();
   if (gi >= ngrp) return;
   const int4 g4 = grp[gi];
   const int fs[4] = {g4.x, g4.y, g4.z, g4.w};
   for (int k = 0; k < 4; ++k) {
     if (fs[k] < 0) break;
-    vjp_face_pass<NV, VISC>(m, P, U, G, phi, w, aU, aG, fs[k], pass);
+    vjp_face_pass<NV, VISC, VK_GEN>(m, P, U, G, phi, w, aU, aG, fs[k], pass);
   }
 }
 
@@ -2717,14 +2724,14 @@ static void vjp_nv(cfd_ctx* c, const double* U, const double* w, double* JTw, bo
   // (with Euler the structured pass also covers the boundary faces)
   if (inv) {
     if (VISC != 0 && c->nbgrp > 0)
-      hipLaunchKernelGGL((k_vjp_face_bnd<NV, VISC>), dim3(nblocks((size_t)c->nbgrp * vjp_passes<NV>(), FLUX_BS)),
+      hipLaunchKernelGGL((k_vjp_face_bnd<NV, VISC>), dim3(nblocks((size_t)c->nbgrp * vjp_passes<NV, VK_GEN>(), FLUX_BS)),
                          dim3(FLUX_BS), 0, c->stream, m, P, U, (const double*)c->grad, (const double*)c->phi, w,
                          JTw, aG, (const int4*)c->d_bgrp, c->nbgrp);
   } else {
     for (int col = 0; col < c->ncolors; ++col) {
       const int f0 = c->color_ptr[col], f1 = c->color_ptr[col + 1];
       if (f1 > f0)
-        hipLaunchKernelGGL((k_vjp_face<NV, VISC>), dim3(nblocks((size_t)(f1 - f0) * vjp_passes<NV>(), FLUX_BS)),
+        hipLaunchKernelGGL((k_vjp_face<NV, VISC>), dim3(nblocks((size_t)(f1 - f0) * vjp_passes<NV, VK_GEN>(), FLUX_BS)),
                            dim3(FLUX_BS), 0, c->stream, m, P, U,
                            (const double*)c->grad, (const double*)c->phi, w, JTw, aG, f0, f1);
     }
