@@ -413,7 +413,8 @@ def converge(solver: cfd.Solver, U0, rel_drop=1e-10, max_steps=600, jv=1, precis
     Newton steps solved by FGMRES (newton_fgmres) from a drop of
     newton_below on; newton_below = 1 goes straight to them (a re-solve from
     a nearby converged state).  Should a Newton step fail to halve the norm,
-    the pseudo-time iteration resumes from the last accepted state.
+    or the Newton steps end short of the target, the pseudo-time iteration
+    resumes from the last accepted state.
 
     Stops at a drop of `rel_drop` of the volume-weighted residual norm or below
     the absolute `tolerance` (a re-solve from a nearby converged state needs
@@ -453,7 +454,7 @@ def converge(solver: cfd.Solver, U0, rel_drop=1e-10, max_steps=600, jv=1, precis
     if vm <= target:
         res.converged = 1
         return U, res, hist
-    if ok or res.steps >= max_steps:
+    if res.steps >= max_steps:
         res.converged = 0
         return U, res, hist
     cfl = float(hist["cfl"][-1]) if len(hist["cfl"]) else cfl0

@@ -80,3 +80,14 @@ def test_straight_newton_and_off_switch(monkeypatch):
     s2 = FakeSolver()
     adjoint.converge(s2, np.zeros((4, 6)), rel_drop=1e-10, newton_below=0.0)
     assert len(s2.calls) == 1 and s2.calls[0]["rel_drop"] == 1e-10 and calls == [2.0]
+
+
+def test_newton_steps_short_of_the_target_resume_pseudo_time(monkeypatch):
+    def fake_newton(solver, U, target, **kw):
+        return U + 10.0, 12, 900, [2.0 ** -k for k in range(12)], True  # halving, but the step cap is reached
+
+    monkeypatch.setattr(adjoint, "newton_fgmres", fake_newton)
+    s = FakeSolver()
+    U, res, hist = adjoint.converge(s, np.zeros((4, 6)), rel_drop=1e-12, newton_below=1e-4, max_steps=600)
+    assert len(s.calls) == 2 and s.calls[1]["tolerance"] == pytest.approx(1e-12 * s.vm0)
+    assert res.converged == 1 and res.steps == 10 + 12 + 10
