@@ -1050,7 +1050,8 @@ __global__ __launch_bounds__(BS) void k_vjp_gather(DMesh m, Phys P, const double
 #pragma unroll
   for (int k = 0; k < NV; ++k) aU[k * p + c] = su[k];
 #pragma unroll
-  for (int q = 0; q < 14; ++q) aG[(size_t)q * p + c] = sg[q];
+  for (int q = 0; q < 14; ++q)  // the k and omega rows (10-13) exist with SST only
+    if (q < 10 || (SST && P.turb)) aG[(size_t)q * p + c] = sg[q];
 }
 
 // SST cell terms: w_c^T d(-V S)/d(U_c, G_c), plus the face sweep's adjoints
