@@ -758,9 +758,12 @@ __global__ __launch_bounds__(FLUX_BS) void k_vjp_face_bnd(DMesh m, Phys P, const
 //
 // Face buffer: two arrays of rows, side 0 (the owner's) then side 1 (the
 // neighbour's), each F rows of vjp_row doubles: the side's NV state adjoints
-// and its gradient slots 0-7 (0-9 with viscous terms: the averaged
-// components' adjoints go half to each side); then, with SST, the sides'
-// mu_t and F1 adjoints, (side, face) pairs of doubles.  A cell reads only its
+// and, with viscous terms, its gradient slots 0-9 (the averaged components'
+// adjoints go half to each side); without them, the adjoints of the side's
+// four reconstructed face primitives, whose gradient slots k_vjp_gather forms
+// as a (wx, wy) from the face midpoint (12 doubles a row had the slots
+// themselves: the bump's face pass and sums moved 148 MB more); then, with
+// SST, the sides' mu_t and F1 adjoints, (side, face) pairs of doubles.  A cell reads only its
 // own side's rows, and the owner side's rows of a cell's faces are adjacent
 // (faces in owner order).  A boundary face's rows are zero: the general
 // kernel adds its product straight into the cells' rows.
@@ -771,7 +774,7 @@ __global__ __launch_bounds__(FLUX_BS) void k_vjp_face_bnd(DMesh m, Phys P, const
 // per-cell sums took 271 instead of 233 us on the 1.32M-cell plate.  Padding
 // the Euler rows (12 doubles) to 16 made the bump's sums slower (137 against
 // 105 us): the padding is read too.
-template <int NV, int VISC> __host__ __device__ constexpr int vjp_row() { return NV + (VISC ? 10 : 8); }
+template <int NV, int VISC> __host__ __device__ constexpr int vjp_row() { return NV + (VISC ? 10 : 4); }
 static_assert(vjp_row<6, 1>() == 16 && 2 * 16 + 2 * 2 <= VJP_ROW_MAX, "adjoint face buffer rows");
 // Held to two waves per SIMD: the SST instance needs 270 registers for one
 // wave and spills 18 at two; both run the plate's face pass in the same time,
@@ -783,7 +786,7 @@ __global__ __launch_bounds__(FLUX_BS) __attribute__((amdgpu_waves_per_eu(2, 8)))
                                                       double* __restrict__ buf) {
   static_assert(NV == 4 || (NV == 6 && VISC), "four variables, or SST with viscous terms");
   constexpr bool SST = NV == 6;
-  constexpr int NGS = VISC ? 10 : 8;
+  constexpr int NGS = VISC ? 10 : 4;
   constexpr int ROW = vjp_row<NV, VISC>();
   const int f = xcd_block() * FLUX_BS + threadIdx.x;
   if (f >= m.F) return;
@@ -900,14 +903,21 @@ __global__ __launch_bounds__(FLUX_BS) __attribute__((amdgpu_waves_per_eu(2, 8)))
     double* rs = rw[sd];
     const double wx = ph[sd] * dx[sd], wy = ph[sd] * dy[sd];
     const double ar = aW[sd][0], au = aW[sd][1], av = aW[sd][2], ap = aW[sd][3];
-    rs[NV + 0] = clip_r[sd] ? 0.0 : ar * wx;
-    rs[NV + 1] = clip_r[sd] ? 0.0 : ar * wy;
-    gsl[sd][0] = au * wx;
-    gsl[sd][1] = au * wy;
-    gsl[sd][2] = av * wx;
-    gsl[sd][3] = av * wy;
-    rs[NV + 6] = clip_p[sd] ? 0.0 : ap * wx;
-    rs[NV + 7] = clip_p[sd] ? 0.0 : ap * wy;
+    if constexpr (VISC != 0) {
+      rs[NV + 0] = clip_r[sd] ? 0.0 : ar * wx;
+      rs[NV + 1] = clip_r[sd] ? 0.0 : ar * wy;
+      gsl[sd][0] = au * wx;
+      gsl[sd][1] = au * wy;
+      gsl[sd][2] = av * wx;
+      gsl[sd][3] = av * wy;
+      rs[NV + 6] = clip_p[sd] ? 0.0 : ap * wx;
+      rs[NV + 7] = clip_p[sd] ? 0.0 : ap * wy;
+    } else {  // the face primitives' adjoints: k_vjp_gather forms the slots a (wx, wy)
+      rs[NV + 0] = clip_r[sd] ? 0.0 : ar;
+      rs[NV + 1] = au;
+      rs[NV + 2] = av;
+      rs[NV + 3] = clip_p[sd] ? 0.0 : ap;
+    }
     const double u = uc[sd], v = vc[sd], ir = 1.0 / rho[sd];
     ru[sd][0] = ar - au * u * ir - av * v * ir + ap * gm1 * 0.5 * (u * u + v * v);
     ru[sd][1] = au * ir - ap * gm1 * u;
@@ -978,11 +988,6 @@ __global__ __launch_bounds__(FLUX_BS) __attribute__((amdgpu_waves_per_eu(2, 8)))
         rq[sd][1] = ain[2 * NV + 8 + sd];
       }
     }
-  } else {
-#pragma unroll
-    for (int sd = 0; sd < 2; ++sd)
-#pragma unroll
-      for (int q = 0; q < 4; ++q) rw[sd][NV + 2 + q] = gsl[sd][q];
   }
 #pragma unroll
   for (int sd = 0; sd < 2; ++sd)
@@ -1000,14 +1005,20 @@ __global__ __launch_bounds__(FLUX_BS) __attribute__((amdgpu_waves_per_eu(2, 8)))
 template <int NV, int VISC>
 __global__ __launch_bounds__(BS) void k_vjp_gather(DMesh m, Phys P, const double* __restrict__ buf,
                                                    const double* __restrict__ U, const double* __restrict__ G,
-                                                   const double* __restrict__ w, double* __restrict__ aU,
-                                                   double* __restrict__ aG) {
+                                                   const double* __restrict__ phi, const double* __restrict__ w,
+                                                   double* __restrict__ aU, double* __restrict__ aG) {
   constexpr bool SST = NV == 6;
-  constexpr int NGS = VISC ? 10 : 8;
+  constexpr int NGS = VISC ? 10 : 4;
   constexpr int ROW = vjp_row<NV, VISC>();
   const int c = xcd_block() * BS + threadIdx.x;
   const int p = m.pitch;
   if (c >= p) return;
+  double xc = 0.0, yc = 0.0, ph = 0.5;  // the reconstruction's vector: ph (x_m - x_c), as face_geo forms it
+  if (VISC == 0 && c < m.N) {
+    xc = m.xc[c];
+    yc = m.yc[c];
+    if (P.spatial == CFD_SECOND_ORDER_LIMITED) ph = phi[c];
+  }
   const int4 s4 = c < m.N ? m.slots[c] : make_int4(-1, -1, -1, -1);  // the padding cells get zeros
   const int sl[4] = {s4.x, s4.y, s4.z, s4.w};
   double su[NV], sg[14], sq[2] = {0.0, 0.0};
@@ -1023,8 +1034,18 @@ __global__ __launch_bounds__(BS) void k_vjp_gather(DMesh m, Phys P, const double
     const double* rs = buf + sf * ROW;
 #pragma unroll
     for (int k = 0; k < NV; ++k) su[k] += rs[k];
+    if constexpr (VISC != 0) {
 #pragma unroll
-    for (int q = 0; q < NGS; ++q) sg[q] += rs[NV + q];
+      for (int q = 0; q < NGS; ++q) sg[q] += rs[NV + q];
+    } else {
+      const int f = e >> 1;
+      const double wx = ph * (m.xm[f] - xc), wy = ph * (m.ym[f] - yc);
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        sg[2 * q] += rs[NV + q] * wx;
+        sg[2 * q + 1] += rs[NV + q] * wy;
+      }
+    }
     if constexpr (SST) {
       const double* rq = buf + (size_t)2 * m.F * ROW + 2 * sf;
       sq[0] += rq[0];
@@ -2717,7 +2738,7 @@ static void vjp_nv(cfd_ctx* c, const double* U, const double* w, double* JTw, bo
       hipLaunchKernelGGL((k_vjp_face_inv<NV, VISC>), dim3(nblocks(m.F, FLUX_BS)), dim3(FLUX_BS), 0, c->stream,
                          m, P, U, (const double*)c->grad, (const double*)c->phi, (const double*)c->tq, w, c->adjF);
       hipLaunchKernelGGL((k_vjp_gather<NV, VISC>), dim3(nblocks(m.pitch)), dim3(BS), 0, c->stream, m, P,
-                         (const double*)c->adjF, U, (const double*)c->grad, w, JTw, aG);
+                         (const double*)c->adjF, U, (const double*)c->grad, (const double*)c->phi, w, JTw, aG);
     }
   }
   if (!inv) {
