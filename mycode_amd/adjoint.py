@@ -327,16 +327,17 @@ def newton_fgmres(solver: cfd.Solver, U, vm_target, jv=1, precision=32, precond=
     Jacobian-vector product with the pseudo-time term switched off), each
     outer step preconditioned by one GMRES(60) solve of the shifted system at
     CFL `cfl` (`precond`, `precision` basis), to a relative tolerance of
-    max(eta, 0.3 vm_target / vm), so that the last step is solved no tighter
-    than the target needs; then a backtracking search over the step lengths
+    min(0.1, max(eta, 0.3 vm_target / vm)), so that the last step is solved
+    no tighter than the target needs; then a backtracking search over the step lengths
     1, 1/2, ..., 1/16 keeps the first that lowers the norm.
 
     Near a converged state the Newton steps shed the pseudo-transient
     iteration's many capped GMRES solves: on the configuration-4 bump at full
     size, from a 1e-4 drop, three steps of about 5,000 inner iterations each
     reached a 1e-12 drop in 9 s, where the pseudo-time iteration took 324
-    more steps of up to 180 GMRES iterations (33 s).  Returns (U, Newton steps, inner GMRES
-    iterations, the norms after each step, ok); ok is False when a step
+    more steps of up to 180 GMRES iterations (33 s).  Returns (U, Newton
+    steps, inner GMRES iterations, the norms after each step, ok); ok is
+    False when a step
     found no length that lowers the norm (U is then the last accepted state)
     or lowered it by less than half."""
     import torch
