@@ -255,7 +255,7 @@ int cfd_norm_vw(cfd_ctx* ctx, const double* R6, double* out);
 
 /* GMRES options. */
 typedef struct cfd_gmres_opts {
-  int m;            /* Krylov dimension (reference: 40, Solver.cpp:1479)          */
+  int m;            /* Krylov dimension, 1..250 (reference: 40, Solver.cpp:1479)  */
   int restarts;     /* extra restart cycles (reference: 0)                         */
   double tol;       /* relative tolerance on |g_{j+1}| (reference: 1e-2, 1587)     */
   int jv;           /* 0 finite difference (reference), 1 tangent, 2 transpose:

@@ -153,11 +153,16 @@ int ensure_krylov(cfd_ctx* c, int m, int prec) {
   HIP_TRY(c, hipMalloc((void**)&k.w, vs * 8));
   HIP_TRY(c, hipMalloc((void**)&k.rr, vs * 8));
   HIP_TRY(c, hipMalloc((void**)&k.w2, vs * 8));
-  HIP_TRY(c, hipMalloc((void**)&k.h, 256 * 8));
+  // the step's scalars: R (2 (j + 1) sums) at h[0..), y at h[yoff..); H
+  // columns and the coefficients with leading dimension ld (64 up to m = 62)
+  k.ld = std::max(64, m + 2);
+  k.yoff = std::max(128, 2 * (m + 2));
+  const size_t ld = (size_t)k.ld;
+  HIP_TRY(c, hipMalloc((void**)&k.h, ((size_t)k.yoff + m + 8) * 8));
   HIP_TRY(c, hipMalloc((void**)&k.part, (size_t)2 * (m + 2) * 1024 * 8));
-  HIP_TRY(c, hipMalloc((void**)&k.dc, (size_t)(64 * 64 + 64 + 192) * 8));
-  HIP_TRY(c, hipHostMalloc((void**)&k.hpin, 256 * 8, hipHostMallocDefault));
-  HIP_TRY(c, hipHostMalloc((void**)&k.hcol, (size_t)64 * 64 * 8, hipHostMallocDefault));
+  HIP_TRY(c, hipMalloc((void**)&k.dc, (ld * ld + 3 * ld + 8) * 8));
+  HIP_TRY(c, hipHostMalloc((void**)&k.hpin, std::max<size_t>(256, m + 8) * 8, hipHostMallocDefault));
+  HIP_TRY(c, hipHostMalloc((void**)&k.hcol, ld * std::max(64, m + 1) * 8, hipHostMallocDefault));
   for (hipEvent_t& e : k.col_ev) HIP_TRY(c, hipEventCreateWithFlags(&e, hipEventDisableTiming));
   k.m = m;
   k.prec = prec;
@@ -1091,7 +1096,7 @@ static int gmres_cycle(cfd_ctx* c, const double* U, const double* Rb, const doub
   // H, which goes to pinned slot j-1 (event col_ev[(j-1) & 1]), and leaves the
   // next candidate in V[j+1].  Step m only finishes column m-1.
   auto step = [&](int j) -> int {
-    double* hslot = j > 0 ? k.hcol + (size_t)64 * (j - 1) : nullptr;
+    double* hslot = j > 0 ? k.hcol + (size_t)k.ld * (j - 1) : nullptr;
     if (j < m) {
       void* Vj = V + (size_t)j * vs * es;
       void* Vj1 = V + (size_t)(j + 1) * vs * es;
@@ -1130,7 +1135,7 @@ static int gmres_cycle(cfd_ctx* c, const double* U, const double* Rb, const doub
     kk = j;
     if (j + 2 <= m && (rc = step(j + 2))) return rc;
     HIP_TRY(c, hipEventSynchronize(k.col_ev[j & 1]));
-    const double* col = k.hcol + (size_t)64 * j;
+    const double* col = k.hcol + (size_t)k.ld * j;
     for (int i = 0; i <= j + 1; ++i) H[i][j] = col[i];
     for (int i = 0; i < j; ++i) {
       const double t = cs[i] * H[i][j] + sn[i] * H[i + 1][j];
@@ -1159,17 +1164,18 @@ static int gmres_cycle(cfd_ctx* c, const double* U, const double* Rb, const doub
     y[i] /= H[i][i];
   }
   std::memcpy(k.hpin, y.data(), (kk + 1) * 8);
-  HIP_TRY(c, hipMemcpyAsync(k.h + 128, k.hpin, (kk + 1) * 8, hipMemcpyHostToDevice, c->stream));
+  double* yd = k.h + k.yoff;
+  HIP_TRY(c, hipMemcpyAsync(yd, k.hpin, (kk + 1) * 8, hipMemcpyHostToDevice, c->stream));
   if (bj) {
-    launch_solution_bj(c, x, V, k.h + 128, kk, k.blk, prec, tr);
+    launch_solution_bj(c, x, V, yd, kk, k.blk, prec, tr);
   } else if (sgs) {
-    launch_solution(c, k.xt, V, k.h + 128, kk, prec, false);
+    launch_solution(c, k.xt, V, yd, kk, prec, false);
     launch_sgs_apply(c, k.blk, k.aoff, k.xt, 64, x, 64, prec, tr);
   } else if (line) {
-    launch_solution(c, k.xt, V, k.h + 128, kk, prec, false);
+    launch_solution(c, k.xt, V, yd, kk, prec, false);
     launch_line_apply(c, k.blk, k.lineP, k.lineG, k.lineA, k.xt, 64, x, 64, prec);
   } else {
-    launch_solution(c, x, V, k.h + 128, kk, prec);
+    launch_solution(c, x, V, yd, kk, prec);
   }
   *iters = kk + 1;
   *res_out = std::abs(g[kk + 1]);
@@ -1181,7 +1187,7 @@ static int gmres_cycle(cfd_ctx* c, const double* U, const double* Rb, const doub
 static int gmres_dev(cfd_ctx* c, const double* U, const double* Rb, const double* r, double sign, double* x,
                      const cfd_gmres_opts* o, int* iters, double* tmpU, double* tmpR) {
   const int m = o->m, prec = o->precision;
-  if (m < 1 || m > 60) return fail(c, CFD_ERR_ARG, "GMRES m must be in [1, 60]");
+  if (m < 1 || m > cfd::GMRES_M_MAX) return fail(c, CFD_ERR_ARG, "GMRES m must be in [1, 250]");
   if (prec != 64 && prec != 32) return fail(c, CFD_ERR_ARG, "precision must be 64 or 32");
   if (o->jv < 0 || o->jv > 2) return fail(c, CFD_ERR_ARG, "jv must be 0 (FD), 1 (tangent) or 2 (transpose)");
   if (o->jv == 0 && prec != 64) return fail(c, CFD_ERR_ARG, "the FD Jv runs in FP64 only");

@@ -30,6 +30,8 @@ struct DMesh {
 // Workspace of one GMRES solve.
 struct Krylov {
   int m = 0, prec = 64;        // basis size, element precision (64 or 32)
+  int ld = 64;                 // leading dimension of the Arnoldi step's H columns and coefficients: 64, or m + 2
+  int yoff = 128;              // offset of the least-squares solution y in h
   void* V = nullptr;           // (m + 1) vectors, nv x pitch each
   double* D = nullptr;         // scalar preconditioner
   void* blk = nullptr;         // block-Jacobi inverses (nv x nv per cell)
@@ -134,6 +136,7 @@ struct cfd_ctx {
 namespace cfd {
 // Most blocks of a reduction's first stage (one partial sum per block and sum).
 constexpr int RED_BLOCKS = 1024;
+constexpr int GMRES_M_MAX = 250;  // largest Krylov dimension a GMRES cycle takes
 // Doubles per face of the adjoint face buffer: two sides of a 16-double row
 // (state and gradient adjoints) and of 2 mu_t / F1 adjoints (SST).
 constexpr int VJP_ROW_MAX = 36;

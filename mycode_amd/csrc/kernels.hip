@@ -2526,27 +2526,30 @@ __global__ __launch_bounds__(BS) void k_dots2(int N, int p, size_t vstride, cons
   }
 }
 
-// The step's scalars from the reduction R (one wave; j <= 60): finishes column
-// j-1 of H (device copy Hd, column-major with leading dimension ldh, and hout,
-// pinned host memory), forms the next partial column hp and the update's
-// coefficients coef: s at [0, j), t at [64, 64 + j), delta at 128, 1/r at 129.
-// Step 0 has Q empty and r = 1.  A norm r at or below 1e-12 (breakdown) keeps
-// the vectors unscaled, as the reference normalises only above it
-// (Solver.cpp:1550-1556).
+// The step's scalars from the reduction R (one wave; each lane takes the
+// basis indices i = lane, lane + 64, ...: up to 64 vectors a lane's partial
+// sums hold one term each, as when the kernel took one index per lane):
+// finishes column j-1 of H (device copy Hd, column-major with leading
+// dimension ldh, and hout, pinned host memory), forms the next partial column
+// hp and the update's coefficients coef: s at [0, j), t at [ldh, ldh + j),
+// delta at 2 ldh, 1/r at 2 ldh + 1.  Step 0 has Q empty and r = 1.  A norm r
+// at or below 1e-12 (breakdown) keeps the vectors unscaled, as the reference
+// normalises only above it (Solver.cpp:1550-1556).
 __global__ __launch_bounds__(64) void k_dcgs_coef(const double* __restrict__ R, int j, int last,
                                                    double* __restrict__ Hd, int ldh, double* __restrict__ hp,
                                                    double* __restrict__ coef, double* __restrict__ hout) {
-  const int i = threadIdx.x;
-  const double si = i < j ? R[i] : 0.0;
-  const double ss = __shfl(wave_sum(si * si), 0, 64);
+  const int l = threadIdx.x;
+  double ssq = 0.0;
+  for (int i = l; i < j; i += 64) ssq += R[i] * R[i];
+  const double ss = __shfl(wave_sum(ssq), 0, 64);
   const double r = j > 0 ? sqrt(fmax(R[j] - ss, 0.0)) : 1.0;
   if (j > 0) {
-    if (i < j) {
-      const double hv = hp[i] + si;
+    for (int i = l; i < j; i += 64) {
+      const double hv = hp[i] + R[i];
       Hd[(size_t)(j - 1) * ldh + i] = hv;
       if (hout) hout[i] = hv;
     }
-    if (i == j) {
+    if (l == (j & 63)) {
       Hd[(size_t)(j - 1) * ldh + j] = r;
       if (hout) hout[j] = r;
     }
@@ -2554,30 +2557,41 @@ __global__ __launch_bounds__(64) void k_dcgs_coef(const double* __restrict__ R, 
   if (last) return;
   __syncthreads();
   const double ir = r > 1e-12 ? 1.0 / r : 1.0;
-  const double ti = i < j ? R[j + 1 + i] : 0.0;
-  const double st = __shfl(wave_sum(si * ti), 0, 64);
+  double sst = 0.0;
+  for (int i = l; i < j; i += 64) sst += R[i] * R[j + 1 + i];
+  const double st = __shfl(wave_sum(sst), 0, 64);
   const double delta = (R[2 * j + 1] - st) * ir;
-  double gi = 0.0;
-  if (i < j)
-    for (int k = i > 0 ? i - 1 : 0; k < j; ++k) gi += Hd[(size_t)k * ldh + i] * R[k];
   const double gamma = j > 0 ? r * R[j - 1] : 0.0;
-  if (i < j) {
+  for (int i = l; i < j; i += 64) {
+    double gi = 0.0;
+    for (int k = i > 0 ? i - 1 : 0; k < j; ++k) gi += Hd[(size_t)k * ldh + i] * R[k];
+    const double ti = R[j + 1 + i];
     hp[i] = (ti - gi) * ir;
-    coef[i] = si;
-    coef[64 + i] = ti;
+    coef[i] = R[i];
+    coef[ldh + i] = ti;
   }
-  if (i == j) hp[j] = (delta - gamma) * ir;
-  if (i == 0) {
-    coef[128] = delta;
-    coef[129] = ir;
+  if (l == (j & 63)) hp[j] = (delta - gamma) * ir;
+  if (l == 0) {
+    coef[2 * ldh] = delta;
+    coef[2 * ldh + 1] = ir;
   }
 }
 
 // q_j = (u - Q s) / r into V[j] and u_{j+1} = (w - Q t - delta q_j) / r into
 // V[j+1], reading Q once; u_{j+1} uses q_j as stored.
-template <int NV, class TV>
+// LDC > 0: the coefficients' stride as a constant (t at coef + LDC, delta and
+// 1/r at coef + 2 LDC); 0: the runtime stride ldc.  The two forms compile to
+// different loops: on the 1.32M-cell SST plate the FP32 update took 139 us
+// with the constant stride of 64 and 191 us with a runtime one, the FP64
+// update 293.5 and 277.8 us, so the launcher takes the constant form for FP32
+// bases of up to 62 vectors and the runtime form otherwise.
+template <int NV, class TV, int LDC>
 __global__ __launch_bounds__(BS) void k_dcgs_update(int N, int p, size_t vstride, TV* __restrict__ V, int j,
-                                                     const double* __restrict__ coef) {
+                                                     const double* __restrict__ coef, int ldc) {
+  const int ld = LDC > 0 ? LDC : ldc;
+  const double* __restrict__ cs = coef;
+  const double* __restrict__ ct = coef + ld;
+  const double* __restrict__ cd = coef + 2 * ld;
   const int c = xcd_block() * BS + threadIdx.x;
   if (c >= N) return;
   TV* u = V + (size_t)j * vstride;
@@ -2590,7 +2604,7 @@ __global__ __launch_bounds__(BS) void k_dcgs_update(int N, int p, size_t vstride
   }
 #pragma unroll 4
   for (int i = 0; i < j; ++i) {
-    const double si = coef[i], ti = coef[64 + i];
+    const double si = cs[i], ti = ct[i];
     const TV* Vi = V + (size_t)i * vstride + c;
 #pragma unroll
     for (int k = 0; k < NV; ++k) {
@@ -2599,7 +2613,7 @@ __global__ __launch_bounds__(BS) void k_dcgs_update(int N, int p, size_t vstride
       aw[k] -= ti * vi;
     }
   }
-  const double delta = coef[128], ir = coef[129];
+  const double delta = cd[0], ir = cd[1];
 #pragma unroll
   for (int k = 0; k < NV; ++k) {
     const TV q = (TV)(as[k] * ir);
@@ -2977,8 +2991,9 @@ void launch_precond_apply(cfd_ctx* c, void* z, const void* Vj, int prec, int slo
 }
 
 // One DCGS2 step j (or, with last, the reduction that finishes column j-1
-// only); column j-1 of H goes to hout.  Device scalars: R at kry.h[0, 124),
-// H, the partial column and the coefficients in kry.dc.
+// only); column j-1 of H goes to hout.  Device scalars: R at kry.h[0, 2 j + 2),
+// H, the partial column and the coefficients in kry.dc (leading dimension
+// kry.ld).
 template <class TV> static void dcgs_t(cfd_ctx* c, TV* V, int j, bool last, double* hout) {
   const int N = c->N, p = c->pitch, nv = c->P.nvar;
   const size_t vs = (size_t)nv * p;
@@ -2986,20 +3001,26 @@ template <class TV> static void dcgs_t(cfd_ctx* c, TV* V, int j, bool last, doub
   const int nb = std::min(RED_BLOCKS, std::max(1, nblocks(N)));
   const int nsum = last ? j + 1 : 2 * (j + 1);
   TV* u = V + (size_t)j * vs;
+  const int ld = k.ld;
   double* Hd = k.dc;
-  double* hp = k.dc + 64 * 64;
-  double* coef = hp + 64;
+  double* hp = k.dc + (size_t)ld * ld;
+  double* coef = hp + ld;
   NV_DISPATCH(nv, {
     hipLaunchKernelGGL((k_dots2<NVt, TV, DOTS_GROUP>), dim3(nb, (j + DOTS_GROUP) / DOTS_GROUP), dim3(BS), 0,
                        c->stream, N, p, vs, (const TV*)V, j, (const TV*)u,
                        last ? (const TV*)nullptr : (const TV*)(u + vs), k.part);
     hipLaunchKernelGGL(k_red_final, dim3(nsum), dim3(BS), 0, c->stream, (const double*)k.part, nb, nsum, k.h,
                        (const double*)nullptr);
-    hipLaunchKernelGGL(k_dcgs_coef, dim3(1), dim3(64), 0, c->stream, (const double*)k.h, j, last ? 1 : 0, Hd, 64,
+    hipLaunchKernelGGL(k_dcgs_coef, dim3(1), dim3(64), 0, c->stream, (const double*)k.h, j, last ? 1 : 0, Hd, ld,
                        hp, coef, hout);
-    if (!last)
-      hipLaunchKernelGGL((k_dcgs_update<NVt, TV>), dim3(nblocks(N)), dim3(BS), 0, c->stream, N, p, vs, V, j,
-                         (const double*)coef);
+    if (!last) {
+      if (std::is_same<TV, float>::value && ld == 64)
+        hipLaunchKernelGGL((k_dcgs_update<NVt, TV, 64>), dim3(nblocks(N)), dim3(BS), 0, c->stream, N, p, vs, V, j,
+                           (const double*)coef, ld);
+      else
+        hipLaunchKernelGGL((k_dcgs_update<NVt, TV, 0>), dim3(nblocks(N)), dim3(BS), 0, c->stream, N, p, vs, V, j,
+                           (const double*)coef, ld);
+    }
   });
 }
 void launch_dcgs(cfd_ctx* c, void* V, int j, bool last, double* hout, int prec) {

@@ -63,7 +63,12 @@ def test_gmres_vs_reference(bump_golden, fs):
     assert rel(x[:, :4], g[f"gmres_x_{fs}"]) < 1e-5
 
 
-def test_gmres_plate_vs_oracle():
+@pytest.mark.parametrize("m,tol,cfl", [(30, 1e-3, 2.0), (150, 1e-6, 30.0)])
+def test_gmres_plate_vs_oracle(m, tol, cfl):
+    """Device GMRES against the oracle's restatement of the reference's
+    (Solver.cpp:1472-1618); with m = 150 at CFL 30 the cycle converges
+    after more than 64 Arnoldi steps (108 in the oracle; the coefficient
+    kernel's lanes then take several basis vectors each)."""
     mesh, p, U = configs.setup(2, 0.1)
     U = U.copy()
     U[:, :4] *= 1 + 0.01 * np.random.default_rng(3).standard_normal((U.shape[0], 4))
@@ -71,12 +76,15 @@ def test_gmres_plate_vs_oracle():
     o = oracle.OracleSolver(oracle.OracleMesh.from_mesh(mesh), p)
     R = s.compute_residuals(U)
     Ro = o.compute_residuals(U)
-    s.compute_local_time_step(U, 2.0)
-    o.compute_local_time_step(U, 2.0)
-    x, it = s.solve_gmres(U, R, -R, s.gmres_opts(m=30, tol=1e-3))
-    xo, ito = o.solve_gmres(U, Ro, -Ro, m=30, tol=1e-3)
+    s.compute_local_time_step(U, cfl)
+    o.compute_local_time_step(U, cfl)
+    x, it = s.solve_gmres(U, R, -R, s.gmres_opts(m=m, tol=tol))
+    xo, ito = o.solve_gmres(U, Ro, -Ro, m=m, tol=tol)
+    print(f"m={m}: {it} iterations (oracle {ito})", flush=True)
     assert abs(it - ito) <= 1
     assert rel(x, xo) < 1e-5
+    if m > 64:
+        assert 64 < it < m
 
 
 @pytest.mark.parametrize("fs", ["00", "22"])
