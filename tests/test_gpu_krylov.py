@@ -174,3 +174,11 @@ def test_error_paths():
     bad.nvar = 5
     with pytest.raises(cfd.CfdError):
         cfd.Solver(mesh, bad)
+    # the Krylov dimension is 1..250; the largest allocates its basis and runs
+    R = s.compute_residuals(U)
+    s.compute_local_time_step(U, 5.0)
+    for m in (0, 251):
+        with pytest.raises(cfd.CfdError, match=r"GMRES m must be in \[1, 250\]"):
+            s.solve_gmres(U, R, -R, s.gmres_opts(m=m))
+    x, it = s.solve_gmres(U, R, -R, s.gmres_opts(m=250, tol=1e-2))
+    assert 0 < it <= 250 and np.all(np.isfinite(x))
