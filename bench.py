@@ -608,11 +608,12 @@ def main():
     if not args.no_jfnk:
         out["jfnk"] = {"per_iteration_on_this_workload": gmres_iteration_cost(solver, Ud, Rd, nv, pitch),
                        "time_to_1e-10": jfnk_converge()}
-        # configuration 2 as it is stated: GMRES(30), block-Jacobi, FD against the tangent, all FP64
+        # configuration 2 as it is stated: GMRES(30), block-Jacobi, FD against the tangent, all FP64; with
+        # two restarts per solve (without them both legs stall near a 1.5e-4 drop)
         out["jfnk"]["configuration_2_as_stated"] = jfnk_converge(
-            2, 1.0, rel_drop=1e-10, max_steps=1500, m=30, restarts=0, precond=1,
+            2, 1.0, rel_drop=1e-10, max_steps=1500, m=30, restarts=2, precond=1,
             legs=(("fd_fp64", 0, 64), ("tangent_fp64", 1, 64)),
-            label=WORKLOADS[2] + ", GMRES(30), block-Jacobi, FD vs tangent, all FP64")
+            label=WORKLOADS[2] + ", GMRES(30) with two restarts, block-Jacobi, FD vs tangent, all FP64")
         if args.config == CONFIG and args.scale == 1.0:
             ref = load_cf_reference(mesh)
             out["jfnk"]["time_to_1e-10_this_workload"] = jfnk_converge(
