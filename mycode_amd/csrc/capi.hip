@@ -336,16 +336,20 @@ int layout_faces(cfd_ctx* c) {
       }
   }
   // per-cell face slots in ascending reference face index
-  std::vector<int4> slots(P_, make_int4(-1, -1, -1, -1));
+  // and, slot by slot, the cell across each face (so a kernel reaches the
+  // neighbour's data without first loading the face's owner and neighbour)
+  std::vector<int4> slots(P_, make_int4(-1, -1, -1, -1)), cnb(P_, make_int4(-1, -1, -1, -1));
   std::vector<int> cnt(N, 0);
   for (int f = 0; f < F; ++f) {
     const int d = fpos[f];
     const int co = m->perm[m->own[f]];
+    const int cn = m->nei[f] >= 0 ? m->perm[m->nei[f]] : -1;
     if (cnt[co] >= 4) return fail(c, CFD_ERR_MESH, "cell with more than 4 faces");
+    (&cnb[co].x)[cnt[co]] = cn;
     (&slots[co].x)[cnt[co]++] = d << 1;
-    if (m->nei[f] >= 0) {
-      const int cn = m->perm[m->nei[f]];
+    if (cn >= 0) {
       if (cnt[cn] >= 4) return fail(c, CFD_ERR_MESH, "cell with more than 4 faces");
+      (&cnb[cn].x)[cnt[cn]] = co;
       (&slots[cn].x)[cnt[cn]++] = (d << 1) | 1;
     }
   }
@@ -355,6 +359,7 @@ int layout_faces(cfd_ctx* c) {
   if (!grp.empty())
     HIP_TRY(c, hipMemcpy(c->d_bgrp, grp.data(), grp.size() * sizeof(int4), hipMemcpyHostToDevice));
   HIP_TRY(c, hipMemcpy(c->d_slots, slots.data(), P_ * sizeof(int4), hipMemcpyHostToDevice));
+  HIP_TRY(c, hipMemcpy(c->d_cnb, cnb.data(), P_ * sizeof(int4), hipMemcpyHostToDevice));
   HIP_TRY(c, hipMemcpy(c->d_own, own.data(), F * 4, hipMemcpyHostToDevice));
   HIP_TRY(c, hipMemcpy(c->d_nei, nei.data(), F * 4, hipMemcpyHostToDevice));
   HIP_TRY(c, hipMemcpy(c->d_tag, tag.data(), F * 4, hipMemcpyHostToDevice));
@@ -483,6 +488,7 @@ int cfd_ctx_create(const cfd_mesh* m, const cfd_params* p, cfd_ctx** out) {
   HIP_TRY(c, dalloc(&c->d_xc, P_));
   HIP_TRY(c, dalloc(&c->d_yc, P_));
   HIP_TRY(c, dalloc(&c->d_slots, P_));
+  HIP_TRY(c, dalloc(&c->d_cnb, P_));
   HIP_TRY(c, dalloc(&c->d_own, F));
   HIP_TRY(c, dalloc(&c->d_nei, F));
   HIP_TRY(c, dalloc(&c->d_tag, F));
@@ -505,6 +511,7 @@ int cfd_ctx_create(const cfd_mesh* m, const cfd_params* p, cfd_ctx** out) {
   d.xc = c->d_xc;
   d.yc = c->d_yc;
   d.slots = c->d_slots;
+  d.cnb = c->d_cnb;
   d.own = c->d_own;
   d.nei = c->d_nei;
   d.tag = c->d_tag;
@@ -727,7 +734,7 @@ int cfd_ctx_destroy(cfd_ctx* c) {
   if (!c) return CFD_OK;
   (void)hipStreamSynchronize(c->stream);
   free_krylov(c);
-  void* bufs[] = {c->d_vol, c->d_xc,  c->d_yc,  c->d_slots, c->d_own, c->d_nei, c->d_tag,  c->d_nx,
+  void* bufs[] = {c->d_vol, c->d_xc,  c->d_yc,  c->d_slots, c->d_cnb, c->d_own, c->d_nei, c->d_tag,  c->d_nx,
                   c->d_ny,  c->d_area, c->d_xm, c->d_ym,    c->d_iperm, c->d_facemap, c->d_alt, c->grad, c->d_wd, c->d_dnw, c->d_corder, c->d_bgrp, c->d_ccol, c->d_ccells, c->d_nq, c->d_cpos, c->d_seg_ptr, c->d_seg_cells, c->d_lpos, c->d_lslot, c->d_lb_t, c->d_lb_kl, c->d_lcomp, c->d_lmask, c->d_lnc, c->d_lqm,
                   c->phi,   c->dt,    c->Fm,    c->red,     c->scal, c->gradT, c->adj, c->adjF, c->tq, c->io6};
   for (void* b : bufs) (void)hipFree(b);

@@ -19,6 +19,7 @@ struct DMesh {
   int N, F, pitch;
   const double *vol, *xc, *yc;
   const int4* slots;
+  const int4* cnb;  // per cell: the other cell of each slot's face, -1 for a boundary face or an empty slot
   const int *own, *nei, *tag;
   const double *nx, *ny, *area, *xm, *ym;
   const unsigned char* alt;  // per device face: 1 = use Phys::flux_alt (may be null)
@@ -83,6 +84,7 @@ struct cfd_ctx {
   // device mesh storage
   double *d_vol = nullptr, *d_xc = nullptr, *d_yc = nullptr;
   int4* d_slots = nullptr;
+  int4* d_cnb = nullptr;
   int *d_own = nullptr, *d_nei = nullptr, *d_tag = nullptr, *d_iperm = nullptr, *d_facemap = nullptr;
   int* d_corder = nullptr;
   double *d_nx = nullptr, *d_ny = nullptr, *d_area = nullptr, *d_xm = nullptr, *d_ym = nullptr;

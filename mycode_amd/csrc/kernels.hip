@@ -142,8 +142,8 @@ void k_gradlim(DMesh m, Phys P, const double* __restrict__ U,
 #pragma unroll
   for (int q = 0; q < 2 * NQM; ++q) g[q] = T(0.0);
   double pmax = val(pc), pmin = val(pc);
-  const int4 s4 = m.slots[c];
-  const int sl[4] = {s4.x, s4.y, s4.z, s4.w};
+  const int4 s4 = m.slots[c], n4 = m.cnb[c];
+  const int sl[4] = {s4.x, s4.y, s4.z, s4.w}, on[4] = {n4.x, n4.y, n4.z, n4.w};
   bool bnd = false;  // the cell has a boundary face
 #pragma unroll
   for (int k = 0; k < 4; ++k) {
@@ -151,7 +151,7 @@ void k_gradlim(DMesh m, Phys P, const double* __restrict__ U,
     if (e < 0) continue;
     const int f = e >> 1;
     const bool as_nei = e & 1;
-    const int other = as_nei ? m.own[f] : m.nei[f];
+    const int other = on[k];
     bnd = bnd || other < 0;
     const double nx = m.nx[f], ny = m.ny[f], A = m.area[f];
     const S6<T> Uo = other >= 0 ? ldT<NV, T>(U, v, p, other)
@@ -1149,15 +1149,15 @@ __global__ __launch_bounds__(BS) void k_vjp_gg(DMesh m, Phys P, const double* __
   double S[NQM];
 #pragma unroll
   for (int k = 0; k < NQM; ++k) S[k] = 0.0;
-  const int4 s4 = m.slots[c];
-  const int sl[4] = {s4.x, s4.y, s4.z, s4.w};
+  const int4 s4 = m.slots[c], n4 = m.cnb[c];
+  const int sl[4] = {s4.x, s4.y, s4.z, s4.w}, on[4] = {n4.x, n4.y, n4.z, n4.w};
 #pragma unroll
   for (int q = 0; q < 4; ++q) {
     const int e = sl[q];
     if (e < 0) continue;
     const int f = e >> 1;
     const bool as_nei = e & 1;
-    const int other = as_nei ? m.own[f] : m.nei[f];
+    const int other = on[q];
     const double nx = m.nx[f], ny = m.ny[f], A = m.area[f];
     const double vb = other >= 0 ? m.vol[other] : 1.0;
 #pragma unroll
