@@ -154,8 +154,9 @@ int ensure_krylov(cfd_ctx* c, int m, int prec) {
   HIP_TRY(c, hipMalloc((void**)&k.rr, vs * 8));
   HIP_TRY(c, hipMalloc((void**)&k.w2, vs * 8));
   // the step's scalars: R (2 (j + 1) sums) at h[0..), y at h[yoff..); H
-  // columns and the coefficients with leading dimension ld (64 up to m = 62)
-  k.ld = std::max(64, m + 2);
+  // columns and the coefficients with leading dimension ld: 64, 128 or 256,
+  // the first that holds m + 2 (the FP32 update kernel is compiled for each)
+  k.ld = m + 2 <= 64 ? 64 : m + 2 <= 128 ? 128 : 256;
   k.yoff = std::max(128, 2 * (m + 2));
   const size_t ld = (size_t)k.ld;
   HIP_TRY(c, hipMalloc((void**)&k.h, ((size_t)k.yoff + m + 8) * 8));

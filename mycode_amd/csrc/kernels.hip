@@ -2584,7 +2584,7 @@ __global__ __launch_bounds__(64) void k_dcgs_coef(const double* __restrict__ R, 
 // different loops: on the 1.32M-cell SST plate the FP32 update took 139 us
 // with the constant stride of 64 and 191 us with a runtime one, the FP64
 // update 293.5 and 277.8 us, so the launcher takes the constant form for FP32
-// bases of up to 62 vectors and the runtime form otherwise.
+// bases (leading dimension 64, 128 or 256) and the runtime form for FP64.
 template <int NV, class TV, int LDC>
 __global__ __launch_bounds__(BS) void k_dcgs_update(int N, int p, size_t vstride, TV* __restrict__ V, int j,
                                                      const double* __restrict__ coef, int ldc) {
@@ -3014,12 +3014,20 @@ template <class TV> static void dcgs_t(cfd_ctx* c, TV* V, int j, bool last, doub
     hipLaunchKernelGGL(k_dcgs_coef, dim3(1), dim3(64), 0, c->stream, (const double*)k.h, j, last ? 1 : 0, Hd, ld,
                        hp, coef, hout);
     if (!last) {
-      if (std::is_same<TV, float>::value && ld == 64)
-        hipLaunchKernelGGL((k_dcgs_update<NVt, TV, 64>), dim3(nblocks(N)), dim3(BS), 0, c->stream, N, p, vs, V, j,
-                           (const double*)coef, ld);
-      else
+      if constexpr (std::is_same<TV, float>::value) {
+        if (ld == 64)
+          hipLaunchKernelGGL((k_dcgs_update<NVt, TV, 64>), dim3(nblocks(N)), dim3(BS), 0, c->stream, N, p, vs, V,
+                             j, (const double*)coef, ld);
+        else if (ld == 128)
+          hipLaunchKernelGGL((k_dcgs_update<NVt, TV, 128>), dim3(nblocks(N)), dim3(BS), 0, c->stream, N, p, vs, V,
+                             j, (const double*)coef, ld);
+        else
+          hipLaunchKernelGGL((k_dcgs_update<NVt, TV, 256>), dim3(nblocks(N)), dim3(BS), 0, c->stream, N, p, vs, V,
+                             j, (const double*)coef, ld);
+      } else {
         hipLaunchKernelGGL((k_dcgs_update<NVt, TV, 0>), dim3(nblocks(N)), dim3(BS), 0, c->stream, N, p, vs, V, j,
                            (const double*)coef, ld);
+      }
     }
   });
 }
