@@ -117,11 +117,12 @@ __device__ __forceinline__ S6<T> ldT(const double* __restrict__ U, const typenam
 // the FP64 residual; the tangent keeps it frozen.
 // With SST the cell's eddy viscosity, blending F1 and V * (k, omega sources)
 // are formed here once per cell (rows 0..3 of Qv / Qd) for the face fluxes.
-// The FP64 residual's instance is held to 128 VGPRs (4 waves per SIMD instead
-// of 3 at 130, two registers spilled): 121 -> 119 us on the 1.32M-cell plate.
-// The FP32 tangent's (133 VGPRs, five spilled at 128) got slower that way.
+// The FP64 residual's four-variable instance runs at 4 waves per SIMD (100
+// VGPRs); the SST one at 3 (134 VGPRs): held to 128 it spills six registers
+// and took 115.9 against 110.5 us on the 1.32M-cell plate.  The FP32
+// tangent's (135 VGPRs, five spilled at 128) also runs at 3.
 template <int NV, class T>
-__global__ __launch_bounds__(GRAD_BS) __attribute__((amdgpu_waves_per_eu(std::is_same<T, double>::value ? 4 : 1, 8)))
+__global__ __launch_bounds__(GRAD_BS) __attribute__((amdgpu_waves_per_eu(std::is_same<T, double>::value && NV == 4 ? 4 : 1, 8)))
 void k_gradlim(DMesh m, Phys P, const double* __restrict__ U,
                                                  const typename Scal<T>::F* __restrict__ v,
                                                  typename Scal<T>::F* __restrict__ Gv,
